@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark: 3-D heat diffusion 512^3 FP64 (T_eff, fraction of MI355X HBM peak), with the
+triad A = B + C over three 512^3 arrays as the bandwidth calibration and the plain
+(non-memopt) kernel beside the memopt (z-streaming) kernel.
+
+One JSON line on stdout. T_eff follows the reference's definition
+(A_eff = 2*D_u + D_k; T, T2 written/read once, Cp read once: 3 * 8 B * n^3 per step).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "parallelstencil.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E peak
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_kernel(ps, launch, steps, warmup):
+    """Runs `launch(i)` warmup + steps times asynchronously on the library stream; returns
+    (wall ms per step, kernel ms per launch from HIP events on that stream)."""
+    for i in range(warmup):
+        launch(i)
+    ps.synchronize()
+    e0, e1 = ps.Event(), ps.Event()
+    t0 = time.perf_counter()
+    e0.record()
+    for i in range(steps):
+        launch(i)
+    e1.record()
+    ps.synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1e3 / steps, e0.elapsed_ms(e1) / steps
+
+
+def cpu_baseline(n, budget_s=12.0):
+    """The oracle's diffusion step (Threads-backend loop nest, OpenMP over z) on the host cores."""
+    from oracle import ps_oracle as O
+    import numpy as np
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    O.set_threads(cores)
+    T0, Cp, p = O.diffusion_inputs(n, seed=1)
+    T2 = np.asfortranarray(T0.copy())
+    args = (p["lam"], p["dt"], p["dx"], p["dy"], p["dz"])
+    O.diffusion3d_flux(T2, T0, Cp, *args, fast=True)  # warm-up
+    steps, t0 = 0, time.perf_counter()
+    a, b = T0, T2
+    while steps < 3 or time.perf_counter() - t0 < budget_s:
+        O.diffusion3d_flux(b, a, Cp, *args, fast=True)
+        a, b = b, a
+        steps += 1
+    dt = (time.perf_counter() - t0) / steps
+    return {"value": round(3 * 8 * n ** 3 / dt / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} steps of the diffusion3D {n}^3 FP64 step (oracle, contraction allowed, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built", file=sys.stderr)
+        sys.exit(2)
+
+    import parallelstencil as ps
+    from parallelstencil import workloads as W
+
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3)
+    n, K, Wm = args.n, args.steps, args.warmup
+    dev = ps.device_info()
+    log(f"device {dev['name']} {dev['arch']} CUs={dev['cu_count']} reported-HBM={dev['hbm_peak_gbs']:.0f} GB/s")
+    bytes_step = 3 * 8 * n ** 3
+    res = {}
+
+    if args.only in ("", "triad"):
+        k = W.define_triad3d(__name__)
+        A, B, C = ps.zeros(n, n, n), ps.fill(1.7, n, n, n), ps.fill(0.5, n, n, n)
+        state = [A, B]
+
+        def triad(i):
+            k(state[0], state[1], C, async_=True)
+            state[0], state[1] = state[1], state[0]
+        ms, kms = time_kernel(ps, triad, K, Wm)
+        res["triad"] = (ms, kms)
+        log(f"triad {n}^3: {ms:.4f} ms/rep wall, {kms:.4f} ms kernel, T_eff {bytes_step / kms / 1e6:.1f} GB/s")
+        del A, B, C, state
+
+    T0, Cp, p = W.diffusion_inputs(n, seed=1)
+    pa = (p["lam"], p["dt"], p["dx"], p["dy"], p["dz"])
+    T, T2, Cpd = ps.zeros(n, n, n), ps.zeros(n, n, n), ps.zeros(n, n, n)
+    Cpd.copy_from(Cp)
+    for name, memopt in (("plain", False), ("memopt", True)):
+        if args.only and args.only != name:
+            continue
+        k = W.define_diffusion3d(memopt, __name__)
+        T.copy_from(T0)
+        T2.copy_from(T0)
+        st = [T, T2]
+
+        def step(i):
+            k(st[1], st[0], Cpd, *pa, async_=True)
+            st[0], st[1] = st[1], st[0]
+        ms, kms = time_kernel(ps, step, K, Wm)
+        res[name] = (ms, kms)
+        log(f"diffusion {name} {n}^3: {ms:.4f} ms/step wall, {kms:.4f} ms kernel, T_eff {bytes_step / kms / 1e6:.1f} GB/s")
+    del T0, Cp
+
+    head = "memopt" if "memopt" in res else ("plain" if "plain" in res else "triad")
+    ms, kms = res[head]
+    teff_wall = bytes_step / (ms * 1e-3) / 1e9
+    achieved = bytes_step / (kms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"diffusion3d_{n}_{head}_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    out = {
+        "metric": "T_eff GB/s, %HBM peak (diffusion3D 512³); Stokes2D time-to-tol: warm vs plain",
+        "value": round(teff_wall, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": Wm,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"diffusion3D {n}^3 FP64, flux form with on-the-fly fluxes, {head} kernel",
+                   "n": n, "fields": "T, T2, Cp (3 x 8 B per cell per step)", "hbm_peak_gbs": HBM_PEAK_GBS},
+        "hbm_frac": round(teff_wall / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "kernels": {k_: {"ms_wall": round(v[0], 5), "ms_kernel": round(v[1], 5),
+                         "T_eff_gbs": round(bytes_step / (v[1] * 1e-3) / 1e9, 2)} for k_, v in res.items()},
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,370 @@
+"""CPU oracle for the ParallelStencil test-bed kernels -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker (and, for bench.py, as the timed CPU baseline). The
+product path (parallelstencil.jl_amd) never imports it.
+
+What it contains
+  * ctypes wrappers of ps_oracle.c: the Threads-backend loop nest of the
+    reference (src/ParallelKernel/parallel.jl:715-768) for the test-bed kernels,
+    compiled with -ffp-contract=off (Julia semantics);
+  * numpy restatements of the reference's finite-difference known-answer
+    formulas (test/test_FiniteDifferences{1,2,3}D.jl) and of the
+    closed-form-input generator of test/test_parallel.jl;
+  * seeded input generators (SplitMix64) for the benchmark configurations.
+
+How each function is pinned (a check that a plausible mistake fails), so
+that parity is not anchored on a formula merely retyped:
+  fd1/fd2/fd3       reference KAT formulas themselves (test_FiniteDifferences*D.jl:
+                    the broadcast expressions are the golden values), plus
+                    closed forms in tests/test_oracle.py: d(linear)=slope,
+                    d2(quadratic)=2a, av/harm of constants, maxloc of a known peak.
+  diffusion3d_otf_test   test/test_parallel.jl:411-434 KAT (integer-cubed input,
+                    unit constants: exact), and against diffusion3d_flux with
+                    _d = 1/d where both are exact, and the d2 form.
+  diffusion3d_flux  linear-in-x field is stationary (d2 = 0 exactly); agreement with
+                    the d2 form to rounding for lam=1 constant Cp (mathematically
+                    identical); conservation of sum(T*Cp) over the interior
+                    except boundary fluxes; single-cell brute-force evaluation.
+  diffusion3d_d2    mathematically equal to the flux form (tolerance), closed form on
+                    a quadratic field (d2 = 2 exactly on integer grids).
+  acoustic2d/3d     discrete energy/mass: sum(P) is conserved by compute_P! when the
+                    velocity boundaries are zero (telescoping sums); brute-force
+                    single-point evaluation; FP32/BF16 variants against FP64.
+  stokes2d_*        brute-force single-point evaluation of each statement from the
+                    kernel text; a uniform velocity field gives zero strain rates and
+                    zero divergence; residual definition of miniapps/Stokes2D.jl:117-123.
+  bc_x/bc_y         KAT on an index-valued array.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_libs = {}
+
+
+def _lib(fast=False):
+    key = "fast" if fast else "exact"
+    if key not in _libs:
+        path = os.path.join(_HERE, "libpsoracle_fast.so" if fast else "libpsoracle.so")
+        if not os.path.exists(path):
+            raise ImportError(f"{path} missing: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        _libs[key] = L
+    return _libs[key]
+
+
+def _p(a):
+    assert a.flags.f_contiguous or a.ndim == 1 or a.flags.c_contiguous and a.ndim <= 1
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f(a, dtype=np.float64):
+    if not (isinstance(a, np.ndarray) and a.dtype == dtype and a.flags.f_contiguous):
+        raise TypeError("oracle arrays must be Fortran-ordered numpy arrays of the right dtype")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+D = ctypes.c_double
+F = ctypes.c_float
+I = ctypes.c_int
+
+
+def set_threads(n):
+    """Threads used by the OpenMP loops (the Threads backend's nthreads)."""
+    os.environ["OMP_NUM_THREADS"] = str(n)
+    try:
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(int(n))
+    except OSError:
+        pass
+
+
+# ---------------------------------------------------------------- kernels
+def triad(T2, T, Ci, fast=False):
+    L = _lib(fast)
+    L.pso_triad(ctypes.c_int64(T.size), _f(T2), _f(T), _f(Ci))
+
+
+def diffusion3d_flux(T2, T, Cp, lam, dt, dx, dy, dz, fast=False):
+    nx, ny, nz = T.shape
+    _lib(fast).pso_diffusion3d_flux(I(nx), I(ny), I(nz), _f(T2), _f(T), _f(Cp), D(lam), D(dt), D(dx), D(dy), D(dz))
+
+
+def diffusion3d_otf_test(T2, T, Ci, lam, dt, _dx, _dy, _dz):
+    nx, ny, nz = T.shape
+    _lib().pso_diffusion3d_otf_test(I(nx), I(ny), I(nz), _f(T2), _f(T), _f(Ci), D(lam), D(dt), D(_dx), D(_dy), D(_dz))
+
+
+def diffusion3d_d2(T2, T, Ci, lam, dt, dx, dy, dz):
+    nx, ny, nz = T.shape
+    _lib().pso_diffusion3d_d2(I(nx), I(ny), I(nz), _f(T2), _f(T), _f(Ci), D(lam), D(dt), D(dx), D(dy), D(dz))
+
+
+def acoustic3d_v(Vx, Vy, Vz, P, dt, rho, dx, dy, dz, prec="f64"):
+    nx, ny, nz = P.shape
+    if prec == "f64":
+        _lib().pso_acoustic3d_v_f64(I(nx), I(ny), I(nz), _f(Vx), _f(Vy), _f(Vz), _f(P), D(dt), D(rho), D(dx), D(dy), D(dz))
+    elif prec == "f32":
+        t = np.float32
+        _lib().pso_acoustic3d_v_f32(I(nx), I(ny), I(nz), _f(Vx, t), _f(Vy, t), _f(Vz, t), _f(P, t), F(dt), F(rho), F(dx), F(dy), F(dz))
+    else:
+        t = np.uint16
+        _lib().pso_acoustic3d_v_bf16(I(nx), I(ny), I(nz), _f(Vx, t), _f(Vy, t), _f(Vz, t), _f(P, t), F(dt), F(rho), F(dx), F(dy), F(dz))
+
+
+def acoustic3d_p(P, Vx, Vy, Vz, dt, k, dx, dy, dz, prec="f64"):
+    nx, ny, nz = P.shape
+    if prec == "f64":
+        _lib().pso_acoustic3d_p_f64(I(nx), I(ny), I(nz), _f(P), _f(Vx), _f(Vy), _f(Vz), D(dt), D(k), D(dx), D(dy), D(dz))
+    elif prec == "f32":
+        t = np.float32
+        _lib().pso_acoustic3d_p_f32(I(nx), I(ny), I(nz), _f(P, t), _f(Vx, t), _f(Vy, t), _f(Vz, t), F(dt), F(k), F(dx), F(dy), F(dz))
+    else:
+        t = np.uint16
+        _lib().pso_acoustic3d_p_bf16(I(nx), I(ny), I(nz), _f(P, t), _f(Vx, t), _f(Vy, t), _f(Vz, t), F(dt), F(k), F(dx), F(dy), F(dz))
+
+
+def acoustic2d_v(Vx, Vy, P, dt, rho, dx, dy):
+    nx, ny = P.shape
+    _lib().pso_acoustic2d_v_f64(I(nx), I(ny), _f(Vx), _f(Vy), _f(P), D(dt), D(rho), D(dx), D(dy))
+
+
+def acoustic2d_p(P, Vx, Vy, dt, k, dx, dy):
+    nx, ny = P.shape
+    _lib().pso_acoustic2d_p_f64(I(nx), I(ny), _f(P), _f(Vx), _f(Vy), D(dt), D(k), D(dx), D(dy))
+
+
+def stokes2d_timesteps(dtVx, dtVy, dtPt, Mus, Vsc, Ptsc, min_dxy2, max_nxy):
+    nx, ny = Mus.shape
+    _lib().pso_stokes2d_timesteps(I(nx), I(ny), _f(dtVx), _f(dtVy), _f(dtPt), _f(Mus), D(Vsc), D(Ptsc), D(min_dxy2), ctypes.c_int64(max_nxy))
+
+
+def stokes2d_p(divV, Pt, Vx, Vy, dtPt, dx, dy):
+    nx, ny = Pt.shape
+    _lib().pso_stokes2d_p(I(nx), I(ny), _f(divV), _f(Pt), _f(Vx), _f(Vy), _f(dtPt), D(dx), D(dy))
+
+
+def stokes2d_tau(divV, txx, tyy, txy, Vx, Vy, Mus, dx, dy):
+    nx, ny = Mus.shape
+    _lib().pso_stokes2d_tau(I(nx), I(ny), _f(divV), _f(txx), _f(tyy), _f(txy), _f(Vx), _f(Vy), _f(Mus), D(dx), D(dy))
+
+
+def stokes2d_dv(Rx, Ry, dVxdt, dVydt, Pt, Rog, txx, tyy, txy, dampX, dampY, dx, dy):
+    nx, ny = Pt.shape
+    _lib().pso_stokes2d_dv(I(nx), I(ny), _f(Rx), _f(Ry), _f(dVxdt), _f(dVydt), _f(Pt), _f(Rog), _f(txx), _f(tyy), _f(txy),
+                           D(dampX), D(dampY), D(dx), D(dy))
+
+
+def stokes2d_v(Vx, Vy, dVxdt, dVydt, dtVx, dtVy):
+    nx, ny = Vy.shape[0], Vx.shape[1]
+    _lib().pso_stokes2d_v(I(nx), I(ny), _f(Vx), _f(Vy), _f(dVxdt), _f(dVydt), _f(dtVx), _f(dtVy))
+
+
+def bc_x(A):
+    _lib().pso_bc_x(I(A.shape[0]), I(A.shape[1]), _f(A))
+
+
+def bc_y(A):
+    _lib().pso_bc_y(I(A.shape[0]), I(A.shape[1]), _f(A))
+
+
+def mean_abs(A):
+    L = _lib()
+    L.pso_mean_abs.restype = D
+    a = np.asfortranarray(A, dtype=np.float64)
+    return L.pso_mean_abs(ctypes.c_int64(a.size), _f(a))
+
+
+# ------------------------------------------------- finite-difference KATs
+def _s(a, b):
+    """Julia range a:end-b (1-based, inclusive) as a python slice on one axis."""
+    return slice(a - 1, None if b == 0 else -b)
+
+
+def fd3(name, A):
+    """Value of @name(A) over its natural 3-D result range, as the broadcast
+    formulas of test/test_FiniteDifferences3D.jl:65-163 compute it."""
+    a, i, o = _s(1, 0), _s(2, 1), None  # 1:end, 2:end-1
+    lo, hi = _s(1, 1), _s(2, 0)  # 1:end-1, 2:end
+    l2, h2 = _s(1, 2), _s(3, 0)  # 1:end-2, 3:end
+    A = np.asarray(A)
+    if name == "d_xa": return A[hi, a, a] - A[lo, a, a]
+    if name == "d_ya": return A[a, hi, a] - A[a, lo, a]
+    if name == "d_za": return A[a, a, hi] - A[a, a, lo]
+    if name == "d_xi": return A[hi, i, i] - A[lo, i, i]
+    if name == "d_yi": return A[i, hi, i] - A[i, lo, i]
+    if name == "d_zi": return A[i, i, hi] - A[i, i, lo]
+    if name == "d2_xi": return (A[h2, i, i] - A[i, i, i]) - (A[i, i, i] - A[l2, i, i])
+    if name == "d2_yi": return (A[i, h2, i] - A[i, i, i]) - (A[i, i, i] - A[i, l2, i])
+    if name == "d2_zi": return (A[i, i, h2] - A[i, i, i]) - (A[i, i, i] - A[i, i, l2])
+    if name == "all": return A.copy()
+    if name == "inn": return A[i, i, i]
+    if name == "inn_x": return A[i, a, a]
+    if name == "inn_y": return A[a, i, a]
+    if name == "inn_z": return A[a, a, i]
+    if name == "inn_xy": return A[i, i, a]
+    if name == "inn_xz": return A[i, a, i]
+    if name == "inn_yz": return A[a, i, i]
+    if name == "av":
+        return (((((((A[lo, lo, lo] + A[hi, lo, lo]) + A[lo, hi, lo]) + A[hi, hi, lo]) + A[lo, lo, hi]) + A[hi, lo, hi])
+                 + A[lo, hi, hi]) + A[hi, hi, hi]) * 0.125
+    if name == "av_xa": return (A[hi, a, a] + A[lo, a, a]) * 0.5
+    if name == "av_ya": return (A[a, hi, a] + A[a, lo, a]) * 0.5
+    if name == "av_za": return (A[a, a, hi] + A[a, a, lo]) * 0.5
+    if name == "av_xi": return (A[hi, i, i] + A[lo, i, i]) * 0.5
+    if name == "av_yi": return (A[i, hi, i] + A[i, lo, i]) * 0.5
+    if name == "av_zi": return (A[i, i, hi] + A[i, i, lo]) * 0.5
+    if name == "av_xya": return (((A[lo, lo, a] + A[hi, lo, a]) + A[lo, hi, a]) + A[hi, hi, a]) * 0.25
+    if name == "av_xza": return (((A[lo, a, lo] + A[hi, a, lo]) + A[lo, a, hi]) + A[hi, a, hi]) * 0.25
+    if name == "av_yza": return (((A[a, lo, lo] + A[a, hi, lo]) + A[a, lo, hi]) + A[a, hi, hi]) * 0.25
+    if name == "av_xyi": return (((A[lo, lo, i] + A[hi, lo, i]) + A[lo, hi, i]) + A[hi, hi, i]) * 0.25
+    if name == "av_xzi": return (((A[lo, i, lo] + A[hi, i, lo]) + A[lo, i, hi]) + A[hi, i, hi]) * 0.25
+    if name == "av_yzi": return (((A[i, lo, lo] + A[i, hi, lo]) + A[i, lo, hi]) + A[i, hi, hi]) * 0.25
+    inv = lambda x: 1.0 / x  # noqa: E731
+    if name == "harm":
+        s = inv(A[lo, lo, lo]) + inv(A[hi, lo, lo]) + inv(A[lo, hi, lo]) + inv(A[hi, hi, lo]) + inv(A[lo, lo, hi]) \
+            + inv(A[hi, lo, hi]) + inv(A[lo, hi, hi]) + inv(A[hi, hi, hi])
+        return 8.0 / s
+    if name == "harm_xa": return 2.0 / (inv(A[hi, a, a]) + inv(A[lo, a, a]))
+    if name == "harm_ya": return 2.0 / (inv(A[a, hi, a]) + inv(A[a, lo, a]))
+    if name == "harm_za": return 2.0 / (inv(A[a, a, hi]) + inv(A[a, a, lo]))
+    if name == "harm_xi": return 2.0 / (inv(A[hi, i, i]) + inv(A[lo, i, i]))
+    if name == "harm_yi": return 2.0 / (inv(A[i, hi, i]) + inv(A[i, lo, i]))
+    if name == "harm_zi": return 2.0 / (inv(A[i, i, hi]) + inv(A[i, i, lo]))
+    if name == "harm_xya": return 4.0 / (inv(A[lo, lo, a]) + inv(A[hi, lo, a]) + inv(A[lo, hi, a]) + inv(A[hi, hi, a]))
+    if name == "harm_xza": return 4.0 / (inv(A[lo, a, lo]) + inv(A[hi, a, lo]) + inv(A[lo, a, hi]) + inv(A[hi, a, hi]))
+    if name == "harm_yza": return 4.0 / (inv(A[a, lo, lo]) + inv(A[a, hi, lo]) + inv(A[a, lo, hi]) + inv(A[a, hi, hi]))
+    if name == "harm_xyi": return 4.0 / (inv(A[lo, lo, i]) + inv(A[hi, lo, i]) + inv(A[lo, hi, i]) + inv(A[hi, hi, i]))
+    if name == "harm_xzi": return 4.0 / (inv(A[lo, i, lo]) + inv(A[hi, i, lo]) + inv(A[lo, i, hi]) + inv(A[hi, i, hi]))
+    if name == "harm_yzi": return 4.0 / (inv(A[i, lo, lo]) + inv(A[i, hi, lo]) + inv(A[i, lo, hi]) + inv(A[i, hi, hi]))
+    if name == "maxloc":
+        m = np.maximum
+        return m(m(m(m(m(m(A[l2, i, i], A[h2, i, i]), A[i, i, i]), A[i, l2, i]), A[i, h2, i]), A[i, i, l2]), A[i, i, h2])
+    if name == "minloc":
+        m = np.minimum
+        return m(m(m(m(m(m(A[l2, i, i], A[h2, i, i]), A[i, i, i]), A[i, l2, i]), A[i, h2, i]), A[i, i, l2]), A[i, i, h2])
+    raise KeyError(name)
+
+
+def fd2(name, A):
+    """2-D counterparts (test/test_FiniteDifferences2D.jl:53-138)."""
+    a, i = _s(1, 0), _s(2, 1)
+    lo, hi, l2, h2 = _s(1, 1), _s(2, 0), _s(1, 2), _s(3, 0)
+    A = np.asarray(A)
+    inv = lambda x: 1.0 / x  # noqa: E731
+    tab = {
+        "d_xa": lambda: A[hi, a] - A[lo, a],
+        "d_ya": lambda: A[a, hi] - A[a, lo],
+        "d_xi": lambda: A[hi, i] - A[lo, i],
+        "d_yi": lambda: A[i, hi] - A[i, lo],
+        "d2_xa": lambda: (A[h2, a] - A[i, a]) - (A[i, a] - A[l2, a]),
+        "d2_ya": lambda: (A[a, h2] - A[a, i]) - (A[a, i] - A[a, l2]),
+        "d2_xi": lambda: (A[h2, i] - A[i, i]) - (A[i, i] - A[l2, i]),
+        "d2_yi": lambda: (A[i, h2] - A[i, i]) - (A[i, i] - A[i, l2]),
+        "all": lambda: A.copy(),
+        "inn": lambda: A[i, i],
+        "inn_x": lambda: A[i, a],
+        "inn_y": lambda: A[a, i],
+        "av": lambda: (((A[lo, lo] + A[hi, lo]) + A[lo, hi]) + A[hi, hi]) * 0.25,
+        "av_xa": lambda: (A[lo, a] + A[hi, a]) * 0.5,
+        "av_ya": lambda: (A[a, lo] + A[a, hi]) * 0.5,
+        "av_xi": lambda: (A[lo, i] + A[hi, i]) * 0.5,
+        "av_yi": lambda: (A[i, lo] + A[i, hi]) * 0.5,
+        "harm": lambda: 4.0 / (inv(A[lo, lo]) + inv(A[hi, lo]) + inv(A[lo, hi]) + inv(A[hi, hi])),
+        "harm_xa": lambda: 2.0 / (inv(A[lo, a]) + inv(A[hi, a])),
+        "harm_ya": lambda: 2.0 / (inv(A[a, lo]) + inv(A[a, hi])),
+        "harm_xi": lambda: 2.0 / (inv(A[lo, i]) + inv(A[hi, i])),
+        "harm_yi": lambda: 2.0 / (inv(A[i, lo]) + inv(A[i, hi])),
+        "maxloc": lambda: np.maximum(np.maximum(np.maximum(np.maximum(A[l2, i], A[h2, i]), A[i, i]), A[i, l2]), A[i, h2]),
+        "minloc": lambda: np.minimum(np.minimum(np.minimum(np.minimum(A[l2, i], A[h2, i]), A[i, i]), A[i, l2]), A[i, h2]),
+    }
+    return tab[name]()
+
+
+def fd1(name, A):
+    """1-D counterparts (test/test_FiniteDifferences1D.jl:47-97)."""
+    A = np.asarray(A)
+    tab = {
+        "d": lambda: A[1:] - A[:-1],
+        "d2": lambda: (A[2:] - A[1:-1]) - (A[1:-1] - A[:-2]),
+        "all": lambda: A.copy(),
+        "inn": lambda: A[1:-1],
+        "av": lambda: (A[:-1] + A[1:]) * 0.5,
+        "harm": lambda: 2.0 / (1.0 / A[:-1] + 1.0 / A[1:]),
+        "maxloc": lambda: np.maximum(np.maximum(A[:-2], A[2:]), A[1:-1]),
+        "minloc": lambda: np.minimum(np.minimum(A[:-2], A[2:]), A[1:-1]),
+    }
+    return tab[name]()
+
+
+# Natural argument size of each macro relative to the result size R
+# (which field type the reference's KAT feeds it: test_FiniteDifferences3D.jl:38-57).
+FD3_ARG_OFFSET = {
+    "d_xa": (1, 0, 0), "d_ya": (0, 1, 0), "d_za": (0, 0, 1), "d_xi": (1, 2, 2), "d_yi": (2, 1, 2), "d_zi": (2, 2, 1),
+    "d2_xi": (2, 2, 2), "d2_yi": (2, 2, 2), "d2_zi": (2, 2, 2), "all": (0, 0, 0), "inn": (2, 2, 2), "inn_x": (2, 0, 0),
+    "inn_y": (0, 2, 0), "inn_z": (0, 0, 2), "inn_xy": (2, 2, 0), "inn_xz": (2, 0, 2), "inn_yz": (0, 2, 2),
+    "av": (1, 1, 1), "av_xa": (1, 0, 0), "av_ya": (0, 1, 0), "av_za": (0, 0, 1), "av_xi": (1, 2, 2), "av_yi": (2, 1, 2),
+    "av_zi": (2, 2, 1), "av_xya": (1, 1, 0), "av_xza": (1, 0, 1), "av_yza": (0, 1, 1), "av_xyi": (1, 1, 2),
+    "av_xzi": (1, 2, 1), "av_yzi": (2, 1, 1), "maxloc": (2, 2, 2), "minloc": (2, 2, 2),
+}
+for _k in list(FD3_ARG_OFFSET):
+    if _k.startswith("av"):
+        FD3_ARG_OFFSET["harm" + _k[2:]] = FD3_ARG_OFFSET[_k]
+
+FD2_ARG_OFFSET = {
+    "d_xa": (1, 0), "d_ya": (0, 1), "d_xi": (1, 2), "d_yi": (2, 1), "d2_xa": (2, 0), "d2_ya": (0, 2), "d2_xi": (2, 2),
+    "d2_yi": (2, 2), "all": (0, 0), "inn": (2, 2), "inn_x": (2, 0), "inn_y": (0, 2), "av": (1, 1), "av_xa": (1, 0),
+    "av_ya": (0, 1), "av_xi": (1, 2), "av_yi": (2, 1), "maxloc": (2, 2), "minloc": (2, 2),
+}
+for _k in list(FD2_ARG_OFFSET):
+    if _k.startswith("av"):
+        FD2_ARG_OFFSET["harm" + _k[2:]] = FD2_ARG_OFFSET[_k]
+
+FD1_ARG_OFFSET = {"d": 1, "d2": 2, "all": 0, "inn": 2, "av": 1, "harm": 1, "maxloc": 2, "minloc": 2}
+
+
+# ---------------------------------------------------------------- inputs
+def cubed_indices(shape):
+    """(ix + (iy-1)*nx + (iz-1)*nx*ny)^3 as Float64, the closed-form input of the
+    reference's KATs (test/test_parallel.jl:416, 443-446)."""
+    shape = tuple(shape) + (1,) * (3 - len(shape))
+    nx, ny, nz = shape
+    ix, iy, iz = np.meshgrid(np.arange(1, nx + 1), np.arange(1, ny + 1), np.arange(1, nz + 1), indexing="ij")
+    v = (ix + (iy - 1) * nx + (iz - 1) * nx * ny).astype(np.float64)
+    return np.asfortranarray(v ** 3)
+
+
+def linear_indices(shape):
+    shape = tuple(shape) + (1,) * (3 - len(shape))
+    nx, ny, nz = shape
+    ix, iy, iz = np.meshgrid(np.arange(1, nx + 1), np.arange(1, ny + 1), np.arange(1, nz + 1), indexing="ij")
+    return np.asfortranarray((ix + (iy - 1) * nx + (iz - 1) * nx * ny).astype(np.float64))
+
+
+def splitmix64_uniform(n, seed):
+    """n uniform [0,1) doubles from SplitMix64 (53-bit mantissa), deterministic across hosts."""
+    out = np.empty(n, dtype=np.float64)
+    x = np.uint64(seed)
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = x + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    out[:] = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return out
+
+
+def diffusion_inputs(n, seed=1, lx=10.0):
+    """3-D heat diffusion configuration: cells of size lx/n, T0 = 100 exp(-r^2) about the centre,
+    Cp = 1 + U[0,1), dt = min(dx^2,dy^2,dz^2)*cp_min/lam/8.1 (cp_min = 1, lam = 1)."""
+    dx = lx / n
+    x = (np.arange(1, n + 1) - 0.5) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    T = np.asfortranarray(100.0 * np.exp(-((X - lx / 2) ** 2 + (Y - lx / 2) ** 2 + (Z - lx / 2) ** 2)))
+    Cp = np.asfortranarray((1.0 + splitmix64_uniform(n ** 3, seed)).reshape((n, n, n), order="F"))
+    dt = min(dx * dx, dx * dx, dx * dx) * 1.0 / 1.0 / 8.1
+    return T, Cp, dict(lam=1.0, dt=dt, dx=dx, dy=dx, dz=dx)

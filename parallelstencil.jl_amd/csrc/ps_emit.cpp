@@ -1,0 +1,1159 @@
+// HIP C++ emission for gfx950 and the host-side bounds proof.
+//
+// The emitted code reproduces the arithmetic of the reference's kernels
+// bit for bit where IEEE allows it:
+//   - operators are emitted in the parser's (Julia's) association order and
+//     the code object is compiled without FMA contraction (Julia does not
+//     contract a*b+c unless asked to with muladd/@fastmath);
+//   - float literals are retyped to the kernel's number type
+//     (literaltypes, src/ParallelKernel/parallel.jl:400-425) and printed as
+//     exact hexadecimal literals;
+//   - literal integer powers follow Base.literal_pow (x^2 = x*x, x^3 = x*x*x,
+//     x^-1 = inv(x), x^-2 = inv(x)*inv(x));
+//   - max/min follow Julia's NaN and signed-zero rules;
+//   - Int/Int division yields Float64, as in Julia.
+// Thread geometry: one thread per (ix,iy,iz) of the launch ranges with an
+// early return outside them (add_threadids, src/ParallelKernel/parallel.jl:603-713),
+// or, for memopt kernels, one thread per column that streams `loopsize`
+// planes along the last parallel dimension with a register window per cached
+// stencil column (the register queues of src/memopt.jl:190-464).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <sstream>
+
+#include "ps_amd.h"
+#include "ps_lower.h"
+
+namespace ps {
+
+static const char* kIdx[3] = {"__ix", "__iy", "__iz"};
+static const char* kIdxC[3] = {"ix", "iy", "iz"};
+
+static Ty value_ty(int dtype) {
+  switch (dtype) {
+    case PS_F64: return Ty::F64;
+    case PS_F32: return Ty::F32;
+    case PS_BF16: return Ty::F32;
+    case PS_I32:
+    case PS_I64: return Ty::Int;
+    case PS_BOOL: return Ty::Bool;
+  }
+  throw PsError(PS_EARGUMENT, "unsupported element type");
+}
+static const char* elem_cty(int dtype) {
+  switch (dtype) {
+    case PS_F64: return "double";
+    case PS_F32: return "float";
+    case PS_BF16: return "unsigned short";
+    case PS_I32: return "int";
+    case PS_I64: return "long long";
+    case PS_BOOL: return "unsigned char";
+  }
+  throw PsError(PS_EARGUMENT, "unsupported element type");
+}
+static const char* cty(Ty t) {
+  switch (t) {
+    case Ty::Int: return "ps_int";
+    case Ty::F64: return "double";
+    case Ty::F32: return "float";
+    case Ty::Bool: return "bool";
+  }
+  return "void";
+}
+static bool is_float(Ty t) { return t == Ty::F64 || t == Ty::F32; }
+static Ty promote(Ty a, Ty b) {
+  if (a == Ty::F64 || b == Ty::F64) return Ty::F64;
+  if (a == Ty::F32 || b == Ty::F32) return Ty::F32;
+  return Ty::Int;
+}
+
+static std::string hexlit(double v, Ty t) {
+  char b[64];
+  if (t == Ty::F32) {
+    float f = (float)v;
+    if (std::isinf(f)) return f > 0 ? "__builtin_huge_valf()" : "(-__builtin_huge_valf())";
+    std::snprintf(b, sizeof b, "%af", (double)f);
+  } else {
+    if (std::isinf(v)) return v > 0 ? "__builtin_huge_val()" : "(-__builtin_huge_val())";
+    std::snprintf(b, sizeof b, "%a", v);
+  }
+  return b;
+}
+
+struct EV {
+  std::string c;
+  Ty t;
+};
+
+static std::string sanitize(const std::string& s) {
+  std::string o;
+  for (unsigned char c : s) {
+    if (std::isalnum(c) || c == '_') o += (char)c;
+    else {
+      char b[8];
+      std::snprintf(b, sizeof b, "_%02x", c);
+      o += b;
+    }
+  }
+  return o;
+}
+
+// device helpers prepended to every generated kernel
+static const char* kPrelude = R"PSD(
+#define PS_DI __device__ __forceinline__
+PS_DI float ps_bf16_ld(unsigned short v) { return __uint_as_float(((unsigned)v) << 16); }
+PS_DI unsigned short ps_bf16_st(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)0x7fc0u;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+PS_DI double ps_max(double x, double y) { bool py = (y > x) || (__builtin_signbit(y) < __builtin_signbit(x)); return py ? (x != x ? x : y) : (y != y ? y : x); }
+PS_DI float  ps_max(float x, float y)   { bool py = (y > x) || (__builtin_signbit(y) < __builtin_signbit(x)); return py ? (x != x ? x : y) : (y != y ? y : x); }
+PS_DI double ps_min(double x, double y) { bool py = (y < x) || (__builtin_signbit(y) > __builtin_signbit(x)); return py ? (x != x ? x : y) : (y != y ? y : x); }
+PS_DI float  ps_min(float x, float y)   { bool py = (y < x) || (__builtin_signbit(y) > __builtin_signbit(x)); return py ? (x != x ? x : y) : (y != y ? y : x); }
+PS_DI ps_int ps_max(ps_int x, ps_int y) { return x > y ? x : y; }
+PS_DI ps_int ps_min(ps_int x, ps_int y) { return x < y ? x : y; }
+PS_DI double ps_p2(double x) { return x * x; }
+PS_DI float  ps_p2(float x)  { return x * x; }
+PS_DI double ps_p3(double x) { return x * x * x; }
+PS_DI float  ps_p3(float x)  { return x * x * x; }
+PS_DI double ps_pm2(double x) { double i = 1.0 / x; return i * i; }
+PS_DI float  ps_pm2(float x)  { float i = 1.0f / x; return i * i; }
+PS_DI ps_int ps_ipow(ps_int b, ps_int e) { ps_int r = 1; while (e > 0) { if (e & 1) r *= b; b *= b; e >>= 1; } return r; }
+PS_DI bool ps_chk(ps_int i, ps_int j, ps_int k, ps_int nx, ps_int ny, ps_int nz, int* err) {
+  bool ok = i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz;
+  if (!ok) *err = 1;
+  return ok;
+}
+)PSD";
+
+class Emitter {
+ public:
+  Emitter(const Kernel& K, const std::vector<ArgKind>& kinds, const Variant& V) : K(K), kinds(kinds), V(V) {
+    NT = K.opt.numbertype == "Float32" ? Ty::F32 : Ty::F64;
+    if (kinds.size() != K.def.params.size())
+      throw PsError(PS_EARGUMENT, "kernel " + K.def.name + " expects " + std::to_string(K.def.params.size()) + " arguments, got " +
+                                      std::to_string(kinds.size()));
+    for (auto& a : K.accessed) {
+      int i = K.param_index.at(a);
+      if (!kinds[i].is_field) throw PsError(PS_EARGUMENT, "argument '" + a + "' is indexed in the kernel but is not an array");
+    }
+    for (auto& w : K.written) {
+      int i = K.param_index.at(w);
+      if (!kinds[i].is_field) throw PsError(PS_EARGUMENT, "argument '" + w + "' is assigned to but is not an array");
+    }
+  }
+
+  Emitted run();
+
+ private:
+  const Kernel& K;
+  const std::vector<ArgKind>& kinds;
+  Variant V;
+  Ty NT;
+  std::ostringstream o;
+  int ind = 1;
+  std::map<std::string, Ty> locals;
+  std::map<std::string, std::string> local_c;
+  std::vector<std::string> local_order;
+  bool pass1 = false;
+  bool in_loop = false;  // inside the memopt stream loop
+  // memopt register cache: key (arg, ox, oy, oz) -> register name
+  std::map<std::tuple<int, int, int, int>, std::string> cache;
+
+  std::string pad() const { return std::string(ind * 2, ' '); }
+  std::string an(int i) const { return "a" + std::to_string(i); }
+  int field_index(const std::string& name) const {
+    auto it = K.param_index.find(name);
+    if (it == K.param_index.end() || !kinds[it->second].is_field) return -1;
+    return it->second;
+  }
+
+  EV cast(const EV& v, Ty t) const {
+    if (v.t == t) return v;
+    return EV{std::string("((") + cty(t) + ")(" + v.c + "))", t};
+  }
+  EV lit(double x, Ty t) const { return EV{hexlit(x, t), t}; }
+
+  EV expr(const NodeP& n);
+  EV load(const NodeP& idxnode);
+  std::string offset(int ai, const std::vector<NodeP>& idx, std::string idxc[3], bool* affine_ok);
+  std::vector<NodeP> norm_indices(const NodeP& idxnode, int ai);
+  EV call(const NodeP& n);
+  EV binary(const std::string& op, EV a, EV b, const NodeP& rhsnode);
+  EV cmp(const NodeP& n);
+  void stmt(const NodeP& s);
+  void block(const NodeP& b);
+  void assign(const NodeP& s);
+  void emit_memopt_body(int L);
+};
+
+std::vector<NodeP> Emitter::norm_indices(const NodeP& n, int ai) {
+  std::vector<NodeP> idx(n->a.begin() + 1, n->a.end());
+  int fnd = kinds[ai].ndims;
+  if ((int)idx.size() > 3) {
+    // trailing literal-1 indices beyond the third are allowed
+    for (size_t i = 3; i < idx.size(); ++i)
+      if (!(idx[i]->k == NK::Num && idx[i]->is_int && idx[i]->ival == 1))
+        throw PsError(PS_EARGUMENT, "arrays with more than 3 dimensions are not supported");
+    idx.resize(3);
+  }
+  if (idx.size() == 1 && fnd > 1) {
+    // linear indexing of a multi-dimensional array is not supported by the stencil backend
+    throw PsError(PS_EARGUMENT, "linear indexing of multi-dimensional arrays is not supported");
+  }
+  while (idx.size() < 3) idx.push_back(mknum_int(1));
+  return idx;
+}
+
+// Element offset of A[i,j,k] and the C expressions of the three indices.
+std::string Emitter::offset(int ai, const std::vector<NodeP>& idx, std::string idxc[3], bool* affine_ok) {
+  static const char* szn[3] = {"_nx", "_ny", "_nz"};
+  std::string A = an(ai);
+  long long c[3] = {0, 0, 0};
+  bool aff = true;
+  for (int d = 0; d < 3; ++d) {
+    const NodeP& e = idx[d];
+    // affine analysis
+    bool ok = false;
+    int var = -1;
+    long long cc = 0;
+    std::function<bool(const NodeP&, int&, long long&)> af = [&](const NodeP& x, int& v, long long& k) -> bool {
+      if (x->k == NK::Num && x->is_int) {
+        k += x->ival;
+        return true;
+      }
+      if (x->k == NK::Ident) {
+        for (int q = 0; q < 3; ++q)
+          if (x->s == kIdx[q]) {
+            if (v != -1) return false;
+            v = q;
+            return true;
+          }
+        return false;
+      }
+      if (x->k == NK::EndKw) {
+        if (v != -1) return false;
+        v = 3;
+        return true;
+      }
+      if (x->k == NK::Binary && (x->s == "+" || x->s == "-")) {
+        if (!af(x->a[0], v, k)) return false;
+        if (x->s == "+") return af(x->a[1], v, k);
+        int v2 = -1;
+        long long k2 = 0;
+        if (!af(x->a[1], v2, k2) || v2 != -1) return false;
+        k -= k2;
+        return true;
+      }
+      return false;
+    };
+    ok = af(e, var, cc);
+    if (ok && var == 3) {  // `end` -> size of this dimension
+      idxc[d] = "(" + A + szn[d] + " + (" + std::to_string(cc) + "))";
+      aff = false;
+      continue;
+    }
+    if (ok && var >= 0) {
+      idxc[d] = std::string("(") + kIdxC[var] + (cc >= 0 ? " + " : " - ") + std::to_string(cc >= 0 ? cc : -cc) + ")";
+      if (var != d) aff = false;
+      c[d] = cc;
+      continue;
+    }
+    if (ok && var == -1) {
+      idxc[d] = "((ps_int)" + std::to_string(cc) + ")";
+      aff = false;
+      continue;
+    }
+    NodeP ee = e;
+    // replace `end` inside a general index expression
+    std::function<NodeP(const NodeP&)> rep = [&](const NodeP& x) -> NodeP {
+      if (x->k == NK::EndKw) return mk(NK::Call, "size", {mk(NK::Ident, K.def.params[ai].name), mknum_int(d + 1)});
+      auto y = std::make_shared<Node>(*x);
+      for (auto& ch : y->a) ch = rep(ch);
+      return y;
+    };
+    ee = rep(e);
+    EV v = expr(ee);
+    if (v.t != Ty::Int) throw PsError(PS_EARGUMENT, "array index must be an integer (got a " + std::string(cty(v.t)) + " expression)");
+    idxc[d] = "(" + v.c + ")";
+    aff = false;
+  }
+  if (affine_ok) *affine_ok = aff;
+  if (aff) {
+    std::string off = A + "_o";
+    std::string k;
+    if (c[0]) k += " + (" + std::to_string(c[0]) + ")";
+    if (c[1]) k += " + (" + std::to_string(c[1]) + ")*" + A + "_sx";
+    if (c[2]) k += " + (" + std::to_string(c[2]) + ")*" + A + "_sxy";
+    return "(" + off + k + ")";
+  }
+  return "((" + idxc[0] + " - 1) + (" + idxc[1] + " - 1)*" + A + "_sx + (" + idxc[2] + " - 1)*" + A + "_sxy)";
+}
+
+EV Emitter::load(const NodeP& n) {
+  if (n->a[0]->k != NK::Ident) throw PsError(PS_EARGUMENT, "unsupported indexing expression " + to_julia(n));
+  const std::string& name = n->a[0]->s;
+  int ai = field_index(name);
+  if (ai < 0) throw PsError(PS_EARGUMENT, "'" + name + "' is indexed but is not an array argument of the kernel");
+  auto idx = norm_indices(n, ai);
+  // memopt register cache
+  if (in_loop && !cache.empty()) {
+    long long off[3];
+    bool ok = true;
+    for (int d = 0; d < 3; ++d) {
+      int v = -1;
+      long long k = 0;
+      const NodeP& e = idx[d];
+      std::function<bool(const NodeP&)> af = [&](const NodeP& x) -> bool {
+        if (x->k == NK::Num && x->is_int) {
+          k += x->ival;
+          return true;
+        }
+        if (x->k == NK::Ident) {
+          for (int q = 0; q < 3; ++q)
+            if (x->s == kIdx[q] && v == -1) {
+              v = q;
+              return true;
+            }
+          return false;
+        }
+        if (x->k == NK::Binary && x->s == "+") return af(x->a[0]) && af(x->a[1]);
+        if (x->k == NK::Binary && x->s == "-" && x->a[1]->k == NK::Num && x->a[1]->is_int) {
+          bool r = af(x->a[0]);
+          k -= x->a[1]->ival;
+          return r;
+        }
+        return false;
+      };
+      if (d < K.nidx) {
+        if (!af(e) || v != d) ok = false;
+      } else {
+        if (!(e->k == NK::Num && e->is_int && e->ival == 1)) ok = false;
+        k = 0;
+      }
+      off[d] = k;
+    }
+    if (ok) {
+      auto it = cache.find(std::make_tuple(ai, (int)off[0], (int)off[1], (int)off[2]));
+      if (it != cache.end()) return EV{it->second, value_ty(kinds[ai].dtype)};
+    }
+  }
+  std::string ic[3];
+  bool aff = false;
+  std::string off = offset(ai, idx, ic, &aff);
+  std::string A = an(ai);
+  std::string raw = A + "[" + off + "]";
+  int dt = kinds[ai].dtype;
+  std::string val;
+  if (V.checked) {
+    std::string chk = "ps_chk(" + ic[0] + ", " + ic[1] + ", " + ic[2] + ", " + A + "_nx, " + A + "_ny, " + A + "_nz, ps_err)";
+    raw = "(" + chk + " ? " + raw + " : (" + elem_cty(dt) + ")0)";
+  }
+  if (dt == PS_BF16) val = "ps_bf16_ld(" + raw + ")";
+  else if (dt == PS_BOOL) val = "((bool)" + raw + ")";
+  else if (dt == PS_I64 || dt == PS_I32) val = "((ps_int)" + raw + ")";
+  else val = raw;
+  return EV{val, value_ty(dt)};
+}
+
+EV Emitter::binary(const std::string& op, EV a, EV b, const NodeP& rhsnode) {
+  if (op == "^") {
+    // literal integer exponent -> Base.literal_pow
+    long long k = 0;
+    bool literal = false;
+    if (rhsnode && rhsnode->k == NK::Num && rhsnode->is_int) {
+      k = rhsnode->ival;
+      literal = true;
+    } else if (rhsnode && rhsnode->k == NK::Unary && rhsnode->s == "-" && rhsnode->a[0]->k == NK::Num && rhsnode->a[0]->is_int) {
+      k = -rhsnode->a[0]->ival;
+      literal = true;
+    }
+    if (a.t == Ty::Bool) a = cast(a, Ty::Int);
+    if (literal) {
+      if (a.t == Ty::Int) {
+        if (k < 0) {
+          EV f = cast(a, Ty::F64);
+          return binary("^", f, b, rhsnode);
+        }
+        return EV{"ps_ipow(" + a.c + ", " + std::to_string(k) + ")", Ty::Int};
+      }
+      const char* one = a.t == Ty::F32 ? "1.0f" : "1.0";
+      switch (k) {
+        case 0: return EV{one, a.t};
+        case 1: return a;
+        case 2: return EV{"ps_p2(" + a.c + ")", a.t};
+        case 3: return EV{"ps_p3(" + a.c + ")", a.t};
+        case -1: return EV{"(" + std::string(one) + " / " + a.c + ")", a.t};
+        case -2: return EV{"ps_pm2(" + a.c + ")", a.t};
+        default:
+          return EV{std::string(a.t == Ty::F32 ? "powf(" : "pow(") + a.c + ", " + (a.t == Ty::F32 ? "(float)" : "(double)") + std::to_string(k) + ")", a.t};
+      }
+    }
+    Ty t = promote(a.t == Ty::Int ? Ty::F64 : a.t, b.t == Ty::Int ? (a.t == Ty::Int ? Ty::F64 : a.t) : b.t);
+    if (a.t == Ty::Int && b.t == Ty::Int) return EV{"ps_ipow(" + a.c + ", " + b.c + ")", Ty::Int};
+    EV x = cast(a, t), y = cast(b, t);
+    return EV{std::string(t == Ty::F32 ? "powf(" : "pow(") + x.c + ", " + y.c + ")", t};
+  }
+  if (a.t == Ty::Bool) a = cast(a, Ty::Int);
+  if (b.t == Ty::Bool) b = cast(b, Ty::Int);
+  if (op == "/") {
+    Ty t = promote(a.t, b.t);
+    if (t == Ty::Int) t = Ty::F64;
+    EV x = cast(a, t), y = cast(b, t);
+    return EV{"(" + x.c + " / " + y.c + ")", t};
+  }
+  if (op == "÷") {
+    if (a.t != Ty::Int || b.t != Ty::Int) throw PsError(PS_EARGUMENT, "÷ is only supported for integers");
+    return EV{"(" + a.c + " / " + b.c + ")", Ty::Int};
+  }
+  if (op == "%") {
+    Ty t = promote(a.t, b.t);
+    EV x = cast(a, t), y = cast(b, t);
+    if (t == Ty::Int) return EV{"(" + x.c + " % " + y.c + ")", t};
+    return EV{std::string(t == Ty::F32 ? "fmodf(" : "fmod(") + x.c + ", " + y.c + ")", t};
+  }
+  if (op == "+" || op == "-" || op == "*") {
+    Ty t = promote(a.t, b.t);
+    EV x = cast(a, t), y = cast(b, t);
+    return EV{"(" + x.c + " " + op + " " + y.c + ")", t};
+  }
+  if (op == "&" || op == "|") {
+    return EV{"(" + a.c + " " + op + " " + b.c + ")", Ty::Int};
+  }
+  throw PsError(PS_EARGUMENT, "unsupported operator " + op);
+}
+
+EV Emitter::cmp(const NodeP& n) {
+  std::vector<EV> v;
+  for (auto& x : n->a) v.push_back(expr(x));
+  std::string c;
+  for (size_t i = 0; i + 1 < v.size(); ++i) {
+    EV a = v[i], b = v[i + 1];
+    if (a.t == Ty::Bool && b.t != Ty::Bool) a = cast(a, Ty::Int);
+    if (b.t == Ty::Bool && a.t != Ty::Bool) b = cast(b, Ty::Int);
+    if (a.t != Ty::Bool) {
+      Ty t = promote(a.t, b.t);
+      a = cast(a, t);
+      b = cast(b, t);
+    }
+    std::string part = "(" + a.c + " " + n->ops[i] + " " + b.c + ")";
+    c = c.empty() ? part : "(" + c + " && " + part + ")";
+  }
+  return EV{c, Ty::Bool};
+}
+
+EV Emitter::call(const NodeP& n) {
+  const std::string& f = n->s;
+  auto argv = [&](size_t i) { return expr(n->a.at(i)); };
+  if (f == "size") {
+    if (n->a.empty() || n->a[0]->k != NK::Ident) throw PsError(PS_EARGUMENT, "size() expects an array argument");
+    int ai = field_index(n->a[0]->s);
+    if (ai < 0) throw PsError(PS_EARGUMENT, "size() of '" + n->a[0]->s + "': not an array argument");
+    if (n->a.size() != 2 || !(n->a[1]->k == NK::Num && n->a[1]->is_int)) throw PsError(PS_EARGUMENT, "size(A, d) requires a literal dimension d");
+    long long d = n->a[1]->ival;
+    if (d < 1) throw PsError(PS_EARGUMENT, "size(A, d): d must be >= 1");
+    if (d > 3) return EV{"((ps_int)1)", Ty::Int};
+    static const char* s[3] = {"_nx", "_ny", "_nz"};
+    return EV{an(ai) + s[d - 1], Ty::Int};
+  }
+  if (f == "length") {
+    int ai = field_index(n->a.at(0)->s);
+    if (ai < 0) throw PsError(PS_EARGUMENT, "length(): not an array argument");
+    std::string A = an(ai);
+    return EV{"(" + A + "_nx*" + A + "_ny*" + A + "_nz)", Ty::Int};
+  }
+  if (f == "firstindex") return EV{"((ps_int)1)", Ty::Int};
+  if (f == "lastindex") {
+    NodeP s = mk(NK::Call, "size", n->a);
+    return call(s);
+  }
+  if (f == "max" || f == "min") {
+    if (n->a.size() < 2) throw PsError(PS_EARGUMENT, f + " expects at least two arguments");
+    EV acc = argv(0);
+    for (size_t i = 1; i < n->a.size(); ++i) {
+      EV b = argv(i);
+      if (acc.t == Ty::Bool) acc = cast(acc, Ty::Int);
+      if (b.t == Ty::Bool) b = cast(b, Ty::Int);
+      Ty t = promote(acc.t, b.t);
+      acc = EV{"ps_" + f + "(" + cast(acc, t).c + ", " + cast(b, t).c + ")", t};
+    }
+    return acc;
+  }
+  if (f == "inv") {
+    EV a = argv(0);
+    if (!is_float(a.t)) a = cast(a, Ty::F64);
+    return EV{std::string("(") + (a.t == Ty::F32 ? "1.0f" : "1.0") + " / " + a.c + ")", a.t};
+  }
+  if (f == "abs") {
+    EV a = argv(0);
+    if (a.t == Ty::Int) return EV{"((" + a.c + ") < 0 ? -(" + a.c + ") : (" + a.c + "))", Ty::Int};
+    return EV{std::string(a.t == Ty::F32 ? "fabsf(" : "fabs(") + a.c + ")", a.t};
+  }
+  if (f == "Float64" || f == "Float32" || f == "Int" || f == "Int64" || f == "Int32") {
+    EV a = argv(0);
+    Ty t = f == "Float64" ? Ty::F64 : f == "Float32" ? Ty::F32 : Ty::Int;
+    return cast(a, t);
+  }
+  if (f == "one" || f == "zero") {
+    EV a = argv(0);
+    double v = f == "one" ? 1.0 : 0.0;
+    if (a.t == Ty::Int) return EV{f == "one" ? "((ps_int)1)" : "((ps_int)0)", Ty::Int};
+    return lit(v, a.t);
+  }
+  if (f == "ifelse") {
+    EV c = argv(0), a = argv(1), b = argv(2);
+    Ty t = promote(a.t, b.t);
+    return EV{"(" + c.c + " ? " + cast(a, t).c + " : " + cast(b, t).c + ")", t};
+  }
+  if (f == "muladd" || f == "fma") {
+    EV a = argv(0), b = argv(1), c = argv(2);
+    Ty t = promote(promote(a.t, b.t), c.t);
+    if (t == Ty::Int) return EV{"(" + a.c + "*" + b.c + " + " + c.c + ")", Ty::Int};
+    return EV{std::string(t == Ty::F32 ? "fmaf(" : "fma(") + cast(a, t).c + ", " + cast(b, t).c + ", " + cast(c, t).c + ")", t};
+  }
+  if (f == "sign") {
+    EV a = argv(0);
+    if (a.t == Ty::Int) return EV{"((ps_int)((" + a.c + ") > 0) - (ps_int)((" + a.c + ") < 0))", Ty::Int};
+    std::string z = a.t == Ty::F32 ? "0.0f" : "0.0", one = a.t == Ty::F32 ? "1.0f" : "1.0";
+    return EV{"((" + a.c + ") > " + z + " ? " + one + " : ((" + a.c + ") < " + z + " ? -" + one + " : (" + a.c + ")))", a.t};
+  }
+  if (f == "copysign" || f == "hypot" || (f == "atan" && n->a.size() == 2)) {
+    EV a = argv(0), b = argv(1);
+    Ty t = promote(a.t == Ty::Int ? Ty::F64 : a.t, b.t == Ty::Int ? Ty::F64 : b.t);
+    std::string fn = f == "atan" ? "atan2" : f;
+    return EV{fn + (t == Ty::F32 ? "f(" : "(") + cast(a, t).c + ", " + cast(b, t).c + ")", t};
+  }
+  static const char* unary_math[] = {"exp", "exp2", "exp10", "expm1", "log", "log2", "log10", "log1p", "sqrt", "cbrt", "sin", "cos", "tan",
+                                     "asin", "acos", "atan", "sinh", "cosh", "tanh", "floor", "ceil", "trunc", "erf", "erfc"};
+  for (auto um : unary_math) {
+    if (f == um) {
+      EV a = argv(0);
+      if (!is_float(a.t)) a = cast(a, Ty::F64);
+      return EV{std::string(um) + (a.t == Ty::F32 ? "f(" : "(") + a.c + ")", a.t};
+    }
+  }
+  if (f == "round") {
+    EV a = argv(0);
+    if (!is_float(a.t)) return a;
+    return EV{std::string(a.t == Ty::F32 ? "rintf(" : "rint(") + a.c + ")", a.t};
+  }
+  throw PsError(PS_EARGUMENT, "unsupported function '" + f + "' in kernel " + K.def.name);
+}
+
+EV Emitter::expr(const NodeP& n) {
+  switch (n->k) {
+    case NK::Num:
+      if (n->is_int) return EV{"((ps_int)" + std::to_string(n->ival) + ")", Ty::Int};
+      return lit(n->num, n->is_f32 ? Ty::F32 : NT);
+    case NK::Ident: {
+      for (int d = 0; d < 3; ++d)
+        if (n->s == kIdx[d]) return EV{kIdxC[d], Ty::Int};
+      if (n->s == "true" || n->s == "false") return EV{n->s, Ty::Bool};
+      if (n->s == "pi" || n->s == "π") return lit(M_PI, NT);
+      auto lt = locals.find(n->s);
+      if (lt != locals.end()) return EV{local_c.at(n->s), lt->second};
+      auto it = K.param_index.find(n->s);
+      if (it != K.param_index.end()) {
+        const ArgKind& k = kinds[it->second];
+        if (k.is_field) throw PsError(PS_EARGUMENT, "array '" + n->s + "' is used as a scalar (index it or use a finite-difference macro)");
+        Ty t = value_ty(k.dtype);
+        return EV{an(it->second), t};
+      }
+      throw PsError(PS_EARGUMENT, "UndefVarError: `" + n->s + "` not defined in kernel " + K.def.name);
+    }
+    case NK::Interp: throw PsError(PS_EARGUMENT, "unsupported interpolation $" + n->s);
+    case NK::EndKw: throw PsError(PS_EARGUMENT, "`end` is only valid inside array indexing");
+    case NK::Index: return load(n);
+    case NK::Call: return call(n);
+    case NK::Macro: throw PsError(PS_EPLUGIN, "macro @" + n->s + " is not available here");
+    case NK::Unary: {
+      EV a = expr(n->a[0]);
+      if (n->s == "+") return a;
+      if (n->s == "!") {
+        if (a.t != Ty::Bool) throw PsError(PS_EARGUMENT, "! expects a Bool");
+        return EV{"(!" + a.c + ")", Ty::Bool};
+      }
+      if (a.t == Ty::Bool) a = cast(a, Ty::Int);
+      return EV{"(-" + a.c + ")", a.t};
+    }
+    case NK::Binary: {
+      EV a = expr(n->a[0]);
+      if (n->s == "^") {
+        bool lit_exp = (n->a[1]->k == NK::Num && n->a[1]->is_int) ||
+                       (n->a[1]->k == NK::Unary && n->a[1]->s == "-" && n->a[1]->a[0]->k == NK::Num && n->a[1]->a[0]->is_int);
+        EV b = lit_exp ? EV{"0", Ty::Int} : expr(n->a[1]);
+        return binary("^", a, b, n->a[1]);
+      }
+      EV b = expr(n->a[1]);
+      return binary(n->s, a, b, nullptr);
+    }
+    case NK::Cmp: return cmp(n);
+    case NK::And:
+    case NK::Or: {
+      EV a = expr(n->a[0]), b = expr(n->a[1]);
+      if (a.t != Ty::Bool || b.t != Ty::Bool) throw PsError(PS_EARGUMENT, "&& and || expect Bool operands");
+      return EV{"(" + a.c + (n->k == NK::And ? " && " : " || ") + b.c + ")", Ty::Bool};
+    }
+    case NK::Tern: {
+      EV c = expr(n->a[0]), a = expr(n->a[1]), b = expr(n->a[2]);
+      if (c.t != Ty::Bool) throw PsError(PS_EARGUMENT, "the condition of ?: must be a Bool");
+      Ty t = (a.t == Ty::Bool && b.t == Ty::Bool) ? Ty::Bool : promote(a.t == Ty::Bool ? Ty::Int : a.t, b.t == Ty::Bool ? Ty::Int : b.t);
+      return EV{"(" + c.c + " ? " + cast(a, t).c + " : " + cast(b, t).c + ")", t};
+    }
+    case NK::Typed: return expr(n->a[0]);
+    default: throw PsError(PS_EARGUMENT, "unsupported expression: " + to_julia(n));
+  }
+}
+
+void Emitter::assign(const NodeP& s) {
+  const NodeP& lhs = s->a[0];
+  NodeP rhs = s->a[1];
+  if (!s->s.empty()) rhs = mk(NK::Binary, s->s, {clone(lhs), rhs});  // A op= x  ->  A = A op x
+  if (lhs->k == NK::Ident) {
+    const std::string& v = lhs->s;
+    if (K.param_index.count(v)) throw PsError(PS_EARGUMENT, "cannot assign to kernel argument '" + v + "' (arguments are passed by value)");
+    for (int d = 0; d < 3; ++d)
+      if (v == kIdx[d]) throw PsError(PS_EARGUMENT, "cannot assign to a parallel index");
+    EV r = expr(rhs);
+    if (!locals.count(v)) {
+      if (!pass1) throw PsError(PS_EINTERNAL, "local variable not hoisted: " + v);
+      locals[v] = r.t;
+      local_c[v] = "v_" + sanitize(v);
+      local_order.push_back(v);
+    }
+    o << pad() << local_c[v] << " = " << cast(r, locals[v]).c << ";\n";
+    return;
+  }
+  if (lhs->k != NK::Index || lhs->a[0]->k != NK::Ident) throw PsError(PS_EARGUMENT, "unsupported assignment target " + to_julia(lhs));
+  int ai = field_index(lhs->a[0]->s);
+  if (ai < 0) throw PsError(PS_EARGUMENT, "'" + lhs->a[0]->s + "' is assigned to but is not an array argument");
+  EV r = expr(rhs);
+  auto idx = norm_indices(lhs, ai);
+  std::string ic[3];
+  std::string off = offset(ai, idx, ic, nullptr);
+  int dt = kinds[ai].dtype;
+  std::string v;
+  if (dt == PS_BF16) v = "ps_bf16_st(" + cast(r, Ty::F32).c + ")";
+  else if (dt == PS_F64) v = cast(r, Ty::F64).c;
+  else if (dt == PS_F32) v = cast(r, Ty::F32).c;
+  else if (dt == PS_BOOL) v = "((unsigned char)(" + r.c + "))";
+  else v = std::string("((") + elem_cty(dt) + ")(" + r.c + "))";
+  std::string A = an(ai);
+  if (V.checked) {
+    o << pad() << "if (ps_chk(" << ic[0] << ", " << ic[1] << ", " << ic[2] << ", " << A << "_nx, " << A << "_ny, " << A
+      << "_nz, ps_err)) " << A << "[" << off << "] = " << v << ";\n";
+  } else {
+    o << pad() << A << "[" << off << "] = " << v << ";\n";
+  }
+}
+
+void Emitter::stmt(const NodeP& s) {
+  switch (s->k) {
+    case NK::Assign: assign(s); return;
+    case NK::If: {
+      EV c = expr(s->a[0]);
+      if (c.t != Ty::Bool) throw PsError(PS_EARGUMENT, "if-condition must be a Bool");
+      o << pad() << "if (" << c.c << ") {\n";
+      ++ind;
+      block(s->a[1]);
+      --ind;
+      if (s->a.size() > 2 && s->a[2]) {
+        o << pad() << "} else {\n";
+        ++ind;
+        if (s->a[2]->k == NK::If) stmt(s->a[2]);
+        else block(s->a[2]);
+        --ind;
+      }
+      o << pad() << "}\n";
+      return;
+    }
+    case NK::Block: block(s); return;
+    case NK::For: {
+      EV a = expr(s->a[0]->a[0]), b = expr(s->a[0]->a[1]);
+      if (a.t != Ty::Int || b.t != Ty::Int) throw PsError(PS_EARGUMENT, "for-loop bounds must be integers");
+      std::string v = s->s;
+      std::string cv = "f_" + sanitize(v);
+      bool had = locals.count(v);
+      Ty prev = had ? locals[v] : Ty::Int;
+      std::string prevc = had ? local_c[v] : "";
+      locals[v] = Ty::Int;
+      local_c[v] = cv;
+      o << pad() << "for (ps_int " << cv << " = " << a.c << "; " << cv << " <= " << b.c << "; ++" << cv << ") {\n";
+      ++ind;
+      block(s->a[1]);
+      --ind;
+      o << pad() << "}\n";
+      if (had) {
+        locals[v] = prev;
+        local_c[v] = prevc;
+      } else {
+        locals.erase(v);
+        local_c.erase(v);
+      }
+      return;
+    }
+    case NK::Return:
+      if (in_loop) throw PsError(PS_EARGUMENT, "early return is not supported in memopt kernels");
+      o << pad() << "return;\n";
+      return;
+    default: {
+      EV e = expr(s);
+      o << pad() << "(void)(" << e.c << ");\n";
+    }
+  }
+}
+
+void Emitter::block(const NodeP& b) {
+  for (auto& s : b->a) stmt(s);
+}
+
+// memopt: thread per column, stream along dimension L (0-based) in chunks of loopsize.
+void Emitter::emit_memopt_body(int L) {
+  const MemoptMeta& mm = K.mm;
+  // register windows per cached column
+  struct Col {
+    int ai;
+    int o[3];  // o[L] unused
+    int zlo, zhi;
+    std::string name;
+  };
+  std::vector<Col> cols;
+  for (auto& A : mm.optvars) {
+    int ai = K.param_index.at(A);
+    const auto& offs = mm.offsets.at(A);
+    bool explicit_r = mm.optranges.count(A) > 0;
+    for (auto& c : offs) {
+      int lo = 1 << 30, hi = -(1 << 30);
+      for (auto& z : c.second) {
+        if (explicit_r) {
+          const auto& r = mm.optranges.at(A);
+          int xy[2] = {c.first.first, c.first.second};
+          bool in = true;
+          for (int d = 0; d < L; ++d)
+            if (xy[d] < r[d].first || xy[d] > r[d].second) in = false;
+          if (z.first < r[L].first || z.first > r[L].second) in = false;
+          if (!in) continue;
+        }
+        lo = std::min(lo, z.first);
+        hi = std::max(hi, z.first);
+      }
+      if (lo > hi) continue;
+      if (!explicit_r && hi == lo) continue;  // no reuse along the stream: plain load
+      Col col;
+      col.ai = ai;
+      col.o[0] = c.first.first;
+      col.o[1] = L == 2 ? c.first.second : 0;
+      col.o[2] = 0;
+      col.zlo = lo;
+      col.zhi = hi;
+      col.name = "q" + std::to_string(ai) + "_" + std::to_string(cols.size());
+      cols.push_back(col);
+    }
+  }
+  std::string lc = kIdxC[L];
+  // per column: base offset (without the streamed dimension) and validity
+  for (auto& c : cols) {
+    std::string A = an(c.ai);
+    const char* etp = elem_cty(kinds[c.ai].dtype);
+    std::ostringstream base, ok;
+    base << "(ps_int)0";
+    ok << "true";
+    static const char* szn[3] = {"_nx", "_ny", "_nz"};
+    static const char* str[3] = {"1", "_sx", "_sxy"};
+    for (int d = 0; d < 3; ++d) {
+      if (d == L) continue;
+      std::string idx = d < K.nidx ? std::string("(") + kIdxC[d] + " + (" + std::to_string(c.o[d]) + "))" : std::string("((ps_int)1)");
+      base << " + (" << idx << " - 1)" << (d == 0 ? "" : "*" + A + str[d]);
+      ok << " && " << idx << " >= 1 && " << idx << " <= " << A << szn[d];
+    }
+    o << pad() << "const ps_int " << c.name << "_b = " << base.str() << ";\n";
+    o << pad() << "const bool " << c.name << "_ok = " << ok.str() << ";\n";
+    int w = c.zhi - c.zlo + 1;
+    Ty vt = value_ty(kinds[c.ai].dtype);
+    for (int k = 0; k < w; ++k) o << pad() << cty(vt) << " " << c.name << "_" << k << ";\n";
+    (void)etp;
+  }
+  auto ldq = [&](const Col& c, const std::string& z) {
+    std::string A = an(c.ai);
+    const char* szn = L == 2 ? "_nz" : "_ny";
+    const char* st = L == 2 ? "_sxy" : "_sx";
+    std::string raw = A + "[" + c.name + "_b + (" + z + " - 1)*" + A + st + "]";
+    std::string cond = "(" + c.name + "_ok && (" + z + ") >= 1 && (" + z + ") <= " + A + szn + ")";
+    int dt = kinds[c.ai].dtype;
+    std::string v = "(" + cond + " ? " + raw + " : (" + elem_cty(dt) + ")0)";
+    if (dt == PS_BF16) v = "ps_bf16_ld(" + v + ")";
+    return v;
+  };
+  // preload
+  for (auto& c : cols) {
+    int w = c.zhi - c.zlo + 1;
+    for (int k = 0; k + 1 < w; ++k)
+      o << pad() << c.name << "_" << k << " = " << ldq(c, "ps_zb + (" + std::to_string(c.zlo + k) + ")") << ";\n";
+  }
+  o << pad() << "for (ps_int " << lc << " = ps_zb; " << lc << " < ps_ze; ++" << lc << ") {\n";
+  ++ind;
+  for (auto& c : cols) {
+    int w = c.zhi - c.zlo + 1;
+    o << pad() << c.name << "_" << (w - 1) << " = " << ldq(c, lc + " + (" + std::to_string(c.zhi) + ")") << ";\n";
+  }
+  // center offsets of every field at this step
+  for (size_t i = 0; i < kinds.size(); ++i) {
+    if (!kinds[i].is_field) continue;
+    std::string A = an((int)i);
+    o << pad() << "const ps_int " << A << "_o = (ix - 1) + (iy - 1)*" << A << "_sx + (iz - 1)*" << A << "_sxy;\n";
+  }
+  cache.clear();
+  for (auto& c : cols) {
+    int w = c.zhi - c.zlo + 1;
+    for (int k = 0; k < w; ++k) {
+      int off[3] = {c.o[0], c.o[1], c.o[2]};
+      off[L] = c.zlo + k;
+      cache[std::make_tuple(c.ai, off[0], off[1], off[2])] = c.name + "_" + std::to_string(k);
+    }
+  }
+  in_loop = true;
+  block(K.body);
+  in_loop = false;
+  for (auto& c : cols) {
+    int w = c.zhi - c.zlo + 1;
+    for (int k = 0; k + 1 < w; ++k) o << pad() << c.name << "_" << k << " = " << c.name << "_" << (k + 1) << ";\n";
+  }
+  --ind;
+  o << pad() << "}\n";
+}
+
+Emitted Emitter::run() {
+  Emitted E;
+  std::string entry = "psk_" + sanitize(K.def.name);
+  E.entry = entry;
+  const char* I = V.idx64 ? "long long" : "int";
+  int ib = V.idx64 ? 8 : 4;
+  std::ostringstream sig;
+  for (size_t i = 0; i < kinds.size(); ++i) {
+    const ArgKind& k = kinds[i];
+    std::string A = an((int)i);
+    if (k.is_field) {
+      bool ro = !K.written.count(K.def.params[i].name);
+      sig << "    " << (ro ? "const " : "") << elem_cty(k.dtype) << "* " << (V.noalias ? "__restrict__ " : "") << A << ",  // "
+          << K.def.params[i].name << "\n";
+      sig << "    ps_int " << A << "_nx, ps_int " << A << "_ny, ps_int " << A << "_nz, ps_int " << A << "_sx, ps_int " << A << "_sxy,\n";
+      E.params.push_back({0, (int)i, 0, 8, 0});
+      for (int w = 0; w < 5; ++w) E.params.push_back({1, (int)i, w, ib, 0});
+    } else {
+      Ty t = value_ty(k.dtype);
+      std::string ct = t == Ty::Bool ? "int" : cty(t);
+      sig << "    " << ct << " " << A << ",  // " << K.def.params[i].name << "\n";
+      int bytes = t == Ty::F64 ? 8 : t == Ty::F32 ? 4 : t == Ty::Bool ? 4 : ib;
+      E.params.push_back({2, (int)i, 0, bytes, k.dtype});
+    }
+  }
+  sig << "    ps_int ps_r0x, ps_int ps_r0y, ps_int ps_r0z, ps_int ps_rlx, ps_int ps_rly, ps_int ps_rlz,\n";
+  for (int d = 0; d < 3; ++d) E.params.push_back({3, -1, d, ib, 0});
+  for (int d = 0; d < 3; ++d) E.params.push_back({4, -1, d, ib, 0});
+  sig << "    int* ps_err";
+  E.params.push_back({5, -1, 0, 8, 0});
+
+  // pass 1: discover local variables and their types
+  {
+    pass1 = true;
+    std::ostringstream saved;
+    saved.swap(o);
+    if (V.memopt) {
+      int L = K.nidx - 1;
+      (void)L;
+      in_loop = false;
+      block(K.body);
+    } else {
+      block(K.body);
+    }
+    o.swap(saved);
+    pass1 = false;
+    o.str("");
+  }
+
+  std::ostringstream src;
+  src << "// generated by libpsamd for kernel " << K.def.name << " (variant " << V.key() << ")\n";
+  src << "typedef " << I << " ps_int;\n" << kPrelude << "\n";
+  src << "extern \"C\" __global__ void __launch_bounds__(" << V.maxthreads << ") " << entry << "(\n" << sig.str() << ") {\n";
+  if (!V.memopt) {
+    o << "  const ps_int tx = (ps_int)blockIdx.x * (ps_int)blockDim.x + (ps_int)threadIdx.x;\n";
+    o << "  const ps_int ty = (ps_int)blockIdx.y * (ps_int)blockDim.y + (ps_int)threadIdx.y;\n";
+    o << "  const ps_int tz = (ps_int)blockIdx.z * (ps_int)blockDim.z + (ps_int)threadIdx.z;\n";
+    o << "  if (tx >= ps_rlx || ty >= ps_rly || tz >= ps_rlz) return;\n";
+    o << "  const ps_int ix = ps_r0x + tx;\n  const ps_int iy = ps_r0y + ty;\n  const ps_int iz = ps_r0z + tz;\n";
+    o << "  (void)ix; (void)iy; (void)iz;\n";
+    for (size_t i = 0; i < kinds.size(); ++i) {
+      if (!kinds[i].is_field) continue;
+      std::string A = an((int)i);
+      o << "  const ps_int " << A << "_o = (ix - 1) + (iy - 1)*" << A << "_sx + (iz - 1)*" << A << "_sxy; (void)" << A << "_o;\n";
+    }
+    for (auto& v : local_order) o << "  " << cty(locals[v]) << " " << local_c[v] << " = 0;\n";
+    ind = 1;
+    block(K.body);
+  } else {
+    int L = K.nidx - 1;  // streamed dimension
+    int ls = K.mm.loopsize;
+    // block indices, optionally remapped so that each XCD works on a contiguous part of the grid
+    if (K.opt.xcd_remap) {
+      o << "  const unsigned ps_nb = gridDim.x * gridDim.y * gridDim.z;\n";
+      o << "  const unsigned ps_lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);\n";
+      o << "  const unsigned ps_q = ps_nb / 8u, ps_r = ps_nb % 8u, ps_x = ps_lin % 8u, ps_l = ps_lin / 8u;\n";
+      o << "  const unsigned ps_id = (ps_x < ps_r ? ps_x * (ps_q + 1u) : ps_r * (ps_q + 1u) + (ps_x - ps_r) * ps_q) + ps_l;\n";
+      o << "  const ps_int ps_bx = (ps_int)(ps_id % gridDim.x);\n";
+      o << "  const ps_int ps_by = (ps_int)((ps_id / gridDim.x) % gridDim.y);\n";
+      o << "  const ps_int ps_bz = (ps_int)(ps_id / (gridDim.x * gridDim.y));\n";
+    } else {
+      o << "  const ps_int ps_bx = blockIdx.x, ps_by = blockIdx.y, ps_bz = blockIdx.z;\n";
+    }
+    const char* bl[3] = {"ps_bx", "ps_by", "ps_bz"};
+    const char* td[3] = {"threadIdx.x", "threadIdx.y", "threadIdx.z"};
+    const char* bd[3] = {"blockDim.x", "blockDim.y", "blockDim.z"};
+    const char* rl[3] = {"ps_rlx", "ps_rly", "ps_rlz"};
+    const char* r0[3] = {"ps_r0x", "ps_r0y", "ps_r0z"};
+    std::string guard;
+    for (int d = 0; d < 3; ++d) {
+      if (d == L) continue;
+      o << "  const ps_int t" << "xyz"[d] << " = " << bl[d] << " * (ps_int)" << bd[d] << " + (ps_int)" << td[d] << ";\n";
+      guard += std::string(guard.empty() ? "" : " || ") + "t" + "xyz"[d] + " >= " + rl[d];
+    }
+    o << "  if (" << guard << ") return;\n";
+    for (int d = 0; d < 3; ++d) {
+      if (d == L) continue;
+      o << "  const ps_int " << kIdxC[d] << " = " << r0[d] << " + t" << "xyz"[d] << ";\n";
+    }
+    o << "  const ps_int ps_zb = " << r0[L] << " + " << bl[L] << " * (ps_int)" << ls << ";\n";
+    o << "  const ps_int ps_ze = (ps_zb + (ps_int)" << ls << " < " << r0[L] << " + " << rl[L] << ") ? ps_zb + (ps_int)" << ls << " : "
+      << r0[L] << " + " << rl[L] << ";\n";
+    for (auto& v : local_order) o << "  " << cty(locals[v]) << " " << local_c[v] << " = 0;\n";
+    ind = 1;
+    emit_memopt_body(L);
+  }
+  src << o.str() << "}\n";
+  E.source = src.str();
+  return E;
+}
+
+Emitted lower_kernel(const Kernel& k, const std::vector<ArgKind>& kinds, const Variant& v) {
+  Emitter e(k, kinds, v);
+  return e.run();
+}
+
+// ---------------------------------------------------------------------------
+// Bounds proof (interval analysis over the launch box)
+// ---------------------------------------------------------------------------
+namespace {
+struct Prover {
+  const Kernel& K;
+  const LaunchShape& S;
+  std::string why;
+  bool ok = true;
+
+  bool eval(const NodeP& e, long long& v) {
+    switch (e->k) {
+      case NK::Num:
+        if (!e->is_int) return false;
+        v = e->ival;
+        return true;
+      case NK::Ident: {
+        auto it = K.param_index.find(e->s);
+        if (it == K.param_index.end()) return false;
+        int i = it->second;
+        if (S.kinds[i].is_field) return false;
+        if (S.kinds[i].dtype != PS_I32 && S.kinds[i].dtype != PS_I64) return false;
+        v = S.ivals[i];
+        return true;
+      }
+      case NK::Call: {
+        if (e->s == "size" && e->a.size() == 2 && e->a[0]->k == NK::Ident && e->a[1]->k == NK::Num && e->a[1]->is_int) {
+          auto it = K.param_index.find(e->a[0]->s);
+          if (it == K.param_index.end() || !S.kinds[it->second].is_field) return false;
+          long long d = e->a[1]->ival;
+          v = (d >= 1 && d <= 3) ? S.dims[it->second][d - 1] : 1;
+          return true;
+        }
+        return false;
+      }
+      case NK::Binary: {
+        long long a, b;
+        if (!eval(e->a[0], a) || !eval(e->a[1], b)) return false;
+        if (e->s == "+") v = a + b;
+        else if (e->s == "-") v = a - b;
+        else if (e->s == "*") v = a * b;
+        else if (e->s == "÷" && b != 0) v = a / b;
+        else return false;
+        return true;
+      }
+      case NK::Unary: {
+        long long a;
+        if (!eval(e->a[0], a)) return false;
+        if (e->s == "-") v = -a;
+        else if (e->s == "+") v = a;
+        else return false;
+        return true;
+      }
+      default: return false;
+    }
+  }
+  // var + c with var an index (0..2); returns false otherwise
+  bool idxaff(const NodeP& e, int& var, long long& c) {
+    var = -1;
+    c = 0;
+    std::function<bool(const NodeP&, int)> f = [&](const NodeP& x, int sign) -> bool {
+      if (x->k == NK::Ident) {
+        for (int d = 0; d < 3; ++d)
+          if (x->s == kIdx[d]) {
+            if (var != -1 || sign < 0) return false;
+            var = d;
+            return true;
+          }
+      }
+      if (x->k == NK::Binary && x->s == "+") return f(x->a[0], sign) && f(x->a[1], sign);
+      if (x->k == NK::Binary && x->s == "-") return f(x->a[0], sign) && f(x->a[1], -sign);
+      long long v;
+      if (eval(x, v)) {
+        c += sign * v;
+        return true;
+      }
+      return false;
+    };
+    return f(e, 1) && var >= 0;
+  }
+  void narrow(const NodeP& cond, long long lo[3], long long hi[3]) {
+    if (!cond) return;
+    if (cond->k == NK::And) {
+      narrow(cond->a[0], lo, hi);
+      narrow(cond->a[1], lo, hi);
+      return;
+    }
+    if (cond->k != NK::Cmp) return;
+    for (size_t i = 0; i + 1 < cond->a.size(); ++i) {
+      const NodeP& l = cond->a[i];
+      const NodeP& r = cond->a[i + 1];
+      std::string op = cond->ops[i];
+      int var;
+      long long c, b;
+      if (idxaff(l, var, c) && eval(r, b)) {
+        // var + c op b
+        if (op == "<") hi[var] = std::min(hi[var], b - c - 1);
+        else if (op == "<=") hi[var] = std::min(hi[var], b - c);
+        else if (op == ">") lo[var] = std::max(lo[var], b - c + 1);
+        else if (op == ">=") lo[var] = std::max(lo[var], b - c);
+        else if (op == "==") {
+          lo[var] = std::max(lo[var], b - c);
+          hi[var] = std::min(hi[var], b - c);
+        }
+      } else if (eval(l, b) && idxaff(r, var, c)) {
+        // b op var + c
+        if (op == "<") lo[var] = std::max(lo[var], b - c + 1);
+        else if (op == "<=") lo[var] = std::max(lo[var], b - c);
+        else if (op == ">") hi[var] = std::min(hi[var], b - c - 1);
+        else if (op == ">=") hi[var] = std::min(hi[var], b - c);
+        else if (op == "==") {
+          lo[var] = std::max(lo[var], b - c);
+          hi[var] = std::min(hi[var], b - c);
+        }
+      }
+    }
+  }
+  void check_access(const NodeP& n, long long lo[3], long long hi[3]) {
+    for (int d = 0; d < 3; ++d)
+      if (lo[d] > hi[d]) return;  // unreachable
+    auto it = K.param_index.find(n->a[0]->s);
+    if (it == K.param_index.end()) return;
+    int ai = it->second;
+    if (!S.kinds[ai].is_field) return;
+    size_t ni = n->a.size() - 1;
+    for (size_t d = 0; d < std::max<size_t>(ni, 3); ++d) {
+      long long dim = d < 3 ? S.dims[ai][d] : 1;
+      long long a, b;
+      if (d >= ni) {
+        a = b = 1;
+      } else {
+        NodeP e = n->a[1 + d];
+        // `end` -> size of this dimension
+        std::function<NodeP(const NodeP&)> rep = [&](const NodeP& x) -> NodeP {
+          if (x->k == NK::EndKw) return mknum_int(dim);
+          auto y = std::make_shared<Node>(*x);
+          for (auto& ch : y->a) ch = rep(ch);
+          return y;
+        };
+        e = rep(e);
+        int var;
+        long long c, v;
+        if (eval(e, v)) {
+          a = b = v;
+        } else if (idxaff(e, var, c)) {
+          a = lo[var] + c;
+          b = hi[var] + c;
+        } else {
+          ok = false;
+          why = "non-affine index in " + to_julia(n);
+          return;
+        }
+      }
+      if (a < 1 || b > dim) {
+        ok = false;
+        why = "access " + to_julia(n) + " may reach index " + std::to_string(a < 1 ? a : b) + " of dimension " + std::to_string(d + 1) +
+              " (size " + std::to_string(dim) + ")";
+        return;
+      }
+    }
+    for (size_t d = 1; d < n->a.size(); ++d) scan_expr(n->a[d], lo, hi);
+  }
+  void scan_expr(const NodeP& e, long long lo[3], long long hi[3]) {
+    if (!e || !ok) return;
+    if (e->k == NK::Index) {
+      check_access(e, lo, hi);
+      return;
+    }
+    for (auto& c : e->a) scan_expr(c, lo, hi);
+  }
+  void scan_block(const NodeP& b, long long lo[3], long long hi[3]) {
+    for (auto& s : b->a) {
+      if (!ok) return;
+      scan_stmt(s, lo, hi);
+    }
+  }
+  void scan_stmt(const NodeP& s, long long lo[3], long long hi[3]) {
+    switch (s->k) {
+      case NK::Assign:
+        scan_expr(s->a[0], lo, hi);
+        scan_expr(s->a[1], lo, hi);
+        return;
+      case NK::If: {
+        scan_expr(s->a[0], lo, hi);
+        long long l2[3] = {lo[0], lo[1], lo[2]}, h2[3] = {hi[0], hi[1], hi[2]};
+        narrow(s->a[0], l2, h2);
+        scan_block(s->a[1], l2, h2);
+        if (s->a.size() > 2 && s->a[2]) {
+          if (s->a[2]->k == NK::If) scan_stmt(s->a[2], lo, hi);
+          else scan_block(s->a[2], lo, hi);
+        }
+        return;
+      }
+      case NK::Block: scan_block(s, lo, hi); return;
+      case NK::For:
+        // accesses indexed by the loop variable are not affine in a parallel index
+        scan_block(s->a[1], lo, hi);
+        return;
+      default: scan_expr(s, lo, hi);
+    }
+  }
+};
+}  // namespace
+
+bool prove_in_bounds(const Kernel& k, const LaunchShape& s, bool memopt, std::string* why) {
+  (void)memopt;
+  Prover P{k, s, "", true};
+  long long lo[3] = {s.lo[0], s.lo[1], s.lo[2]}, hi[3] = {s.hi[0], s.hi[1], s.hi[2]};
+  P.scan_block(k.body, lo, hi);
+  if (why) *why = P.why;
+  return P.ok;
+}
+
+}  // namespace ps

@@ -1,0 +1,764 @@
+// Kernel definition: the passes that the reference runs at macro-expansion
+// time, restated over the parsed kernel text.
+//
+//   validate_body            src/shared.jl:51-57
+//   extract_onthefly_arrays! src/parallel.jl:789-819
+//   apply_masks / @within    src/parallel.jl:654-680, src/FiniteDifferences.jl:397-407
+//   FD macro expansion       src/FiniteDifferences.jl:54-61, 160-196, 328-396
+//   handle_padding (false)   src/ParallelKernel/parallel.jl:487-557  (ixi -> ix+1, ixd -> ix)
+//   insert_onthefly!         src/parallel.jl:821-859
+//   memopt offsets/metadata  src/memopt.jl:57-87, 606-700, 1179-1236
+#include "ps_lower.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <sstream>
+
+#include "ps_amd.h"
+
+namespace ps {
+
+static const char* kIdx[3] = {"__ix", "__iy", "__iz"};
+
+// ---------------------------------------------------------------------------
+// Finite-difference macro tables. Each entry restates the index algebra of
+// one macro of src/FiniteDifferences.jl with the reference's index roles:
+// ixa (all), ixi (inner, = ix+1 without padding), ixd (staggered, = ix).
+// ---------------------------------------------------------------------------
+struct MacroDef {
+  const char* name;
+  const char* tmpl;
+};
+
+static const MacroDef kFD1[] = {
+    {"d", "A[ixd+1] - A[ixd]"},
+    {"d2", "(A[ixi+1] - A[ixi]) - (A[ixi] - A[ixi-1])"},
+    {"all", "A[ixa]"},
+    {"inn", "A[ixi]"},
+    {"av", "(A[ixd] + A[ixd+1])*0.5"},
+    {"harm", "1.0/(1.0/A[ixd] + 1.0/A[ixd+1])*2.0"},
+    {"maxloc", "max(max(A[ixi-1], A[ixi+1]), A[ixi])"},
+    {"minloc", "min(min(A[ixi-1], A[ixi+1]), A[ixi])"},
+};
+
+static const MacroDef kFD2[] = {
+    {"d_xa", "A[ixd+1,iya] - A[ixd,iya]"},
+    {"d_ya", "A[ixa,iyd+1] - A[ixa,iyd]"},
+    {"d_xi", "A[ixd+1,iyi] - A[ixd,iyi]"},
+    {"d_yi", "A[ixi,iyd+1] - A[ixi,iyd]"},
+    {"d2_xa", "(A[ixi+1,iya] - A[ixi,iya]) - (A[ixi,iya] - A[ixi-1,iya])"},
+    {"d2_ya", "(A[ixa,iyi+1] - A[ixa,iyi]) - (A[ixa,iyi] - A[ixa,iyi-1])"},
+    {"d2_xi", "(A[ixi+1,iyi] - A[ixi,iyi]) - (A[ixi,iyi] - A[ixi-1,iyi])"},
+    {"d2_yi", "(A[ixi,iyi+1] - A[ixi,iyi]) - (A[ixi,iyi] - A[ixi,iyi-1])"},
+    {"all", "A[ixa,iya]"},
+    {"inn", "A[ixi,iyi]"},
+    {"inn_x", "A[ixi,iya]"},
+    {"inn_y", "A[ixa,iyi]"},
+    {"av", "(A[ixd,iyd] + A[ixd+1,iyd] + A[ixd,iyd+1] + A[ixd+1,iyd+1])*0.25"},
+    {"av_xa", "(A[ixd,iya] + A[ixd+1,iya])*0.5"},
+    {"av_ya", "(A[ixa,iyd] + A[ixa,iyd+1])*0.5"},
+    {"av_xi", "(A[ixd,iyi] + A[ixd+1,iyi])*0.5"},
+    {"av_yi", "(A[ixi,iyd] + A[ixi,iyd+1])*0.5"},
+    {"harm", "1.0/(1.0/A[ixd,iyd] + 1.0/A[ixd+1,iyd] + 1.0/A[ixd,iyd+1] + 1.0/A[ixd+1,iyd+1])*4.0"},
+    {"harm_xa", "1.0/(1.0/A[ixd,iya] + 1.0/A[ixd+1,iya])*2.0"},
+    {"harm_ya", "1.0/(1.0/A[ixa,iyd] + 1.0/A[ixa,iyd+1])*2.0"},
+    {"harm_xi", "1.0/(1.0/A[ixd,iyi] + 1.0/A[ixd+1,iyi])*2.0"},
+    {"harm_yi", "1.0/(1.0/A[ixi,iyd] + 1.0/A[ixi,iyd+1])*2.0"},
+    {"maxloc", "max(max(max(A[ixi-1,iyi], A[ixi+1,iyi]), A[ixi,iyi]), max(A[ixi,iyi-1], A[ixi,iyi+1]))"},
+    {"minloc", "min(min(min(A[ixi-1,iyi], A[ixi+1,iyi]), A[ixi,iyi]), min(A[ixi,iyi-1], A[ixi,iyi+1]))"},
+};
+
+static const MacroDef kFD3[] = {
+    {"d_xa", "A[ixd+1,iya,iza] - A[ixd,iya,iza]"},
+    {"d_ya", "A[ixa,iyd+1,iza] - A[ixa,iyd,iza]"},
+    {"d_za", "A[ixa,iya,izd+1] - A[ixa,iya,izd]"},
+    {"d_xi", "A[ixd+1,iyi,izi] - A[ixd,iyi,izi]"},
+    {"d_yi", "A[ixi,iyd+1,izi] - A[ixi,iyd,izi]"},
+    {"d_zi", "A[ixi,iyi,izd+1] - A[ixi,iyi,izd]"},
+    {"d2_xi", "(A[ixi+1,iyi,izi] - A[ixi,iyi,izi]) - (A[ixi,iyi,izi] - A[ixi-1,iyi,izi])"},
+    {"d2_yi", "(A[ixi,iyi+1,izi] - A[ixi,iyi,izi]) - (A[ixi,iyi,izi] - A[ixi,iyi-1,izi])"},
+    {"d2_zi", "(A[ixi,iyi,izi+1] - A[ixi,iyi,izi]) - (A[ixi,iyi,izi] - A[ixi,iyi,izi-1])"},
+    {"all", "A[ixa,iya,iza]"},
+    {"inn", "A[ixi,iyi,izi]"},
+    {"inn_x", "A[ixi,iya,iza]"},
+    {"inn_y", "A[ixa,iyi,iza]"},
+    {"inn_z", "A[ixa,iya,izi]"},
+    {"inn_xy", "A[ixi,iyi,iza]"},
+    {"inn_xz", "A[ixi,iya,izi]"},
+    {"inn_yz", "A[ixa,iyi,izi]"},
+    {"av", "(A[ixd,iyd,izd] + A[ixd+1,iyd,izd] + A[ixd,iyd+1,izd] + A[ixd+1,iyd+1,izd] + "
+           "A[ixd,iyd,izd+1] + A[ixd+1,iyd,izd+1] + A[ixd,iyd+1,izd+1] + A[ixd+1,iyd+1,izd+1])*0.125"},
+    {"av_xa", "(A[ixd,iya,iza] + A[ixd+1,iya,iza])*0.5"},
+    {"av_ya", "(A[ixa,iyd,iza] + A[ixa,iyd+1,iza])*0.5"},
+    {"av_za", "(A[ixa,iya,izd] + A[ixa,iya,izd+1])*0.5"},
+    {"av_xi", "(A[ixd,iyi,izi] + A[ixd+1,iyi,izi])*0.5"},
+    {"av_yi", "(A[ixi,iyd,izi] + A[ixi,iyd+1,izi])*0.5"},
+    {"av_zi", "(A[ixi,iyi,izd] + A[ixi,iyi,izd+1])*0.5"},
+    {"av_xya", "(A[ixd,iyd,iza] + A[ixd+1,iyd,iza] + A[ixd,iyd+1,iza] + A[ixd+1,iyd+1,iza])*0.25"},
+    {"av_xza", "(A[ixd,iya,izd] + A[ixd+1,iya,izd] + A[ixd,iya,izd+1] + A[ixd+1,iya,izd+1])*0.25"},
+    {"av_yza", "(A[ixa,iyd,izd] + A[ixa,iyd+1,izd] + A[ixa,iyd,izd+1] + A[ixa,iyd+1,izd+1])*0.25"},
+    {"av_xyi", "(A[ixd,iyd,izi] + A[ixd+1,iyd,izi] + A[ixd,iyd+1,izi] + A[ixd+1,iyd+1,izi])*0.25"},
+    {"av_xzi", "(A[ixd,iyi,izd] + A[ixd+1,iyi,izd] + A[ixd,iyi,izd+1] + A[ixd+1,iyi,izd+1])*0.25"},
+    {"av_yzi", "(A[ixi,iyd,izd] + A[ixi,iyd+1,izd] + A[ixi,iyd,izd+1] + A[ixi,iyd+1,izd+1])*0.25"},
+    {"harm", "1.0/(1.0/A[ixd,iyd,izd] + 1.0/A[ixd+1,iyd,izd] + 1.0/A[ixd,iyd+1,izd] + 1.0/A[ixd+1,iyd+1,izd] + "
+             "1.0/A[ixd,iyd,izd+1] + 1.0/A[ixd+1,iyd,izd+1] + 1.0/A[ixd,iyd+1,izd+1] + 1.0/A[ixd+1,iyd+1,izd+1])*8.0"},
+    {"harm_xa", "1.0/(1.0/A[ixd,iya,iza] + 1.0/A[ixd+1,iya,iza])*2.0"},
+    {"harm_ya", "1.0/(1.0/A[ixa,iyd,iza] + 1.0/A[ixa,iyd+1,iza])*2.0"},
+    {"harm_za", "1.0/(1.0/A[ixa,iya,izd] + 1.0/A[ixa,iya,izd+1])*2.0"},
+    {"harm_xi", "1.0/(1.0/A[ixd,iyi,izi] + 1.0/A[ixd+1,iyi,izi])*2.0"},
+    {"harm_yi", "1.0/(1.0/A[ixi,iyd,izi] + 1.0/A[ixi,iyd+1,izi])*2.0"},
+    {"harm_zi", "1.0/(1.0/A[ixi,iyi,izd] + 1.0/A[ixi,iyi,izd+1])*2.0"},
+    {"harm_xya", "1.0/(1.0/A[ixd,iyd,iza] + 1.0/A[ixd+1,iyd,iza] + 1.0/A[ixd,iyd+1,iza] + 1.0/A[ixd+1,iyd+1,iza])*4.0"},
+    {"harm_xza", "1.0/(1.0/A[ixd,iya,izd] + 1.0/A[ixd+1,iya,izd] + 1.0/A[ixd,iya,izd+1] + 1.0/A[ixd+1,iya,izd+1])*4.0"},
+    {"harm_yza", "1.0/(1.0/A[ixa,iyd,izd] + 1.0/A[ixa,iyd+1,izd] + 1.0/A[ixa,iyd,izd+1] + 1.0/A[ixa,iyd+1,izd+1])*4.0"},
+    {"harm_xyi", "1.0/(1.0/A[ixd,iyd,izi] + 1.0/A[ixd+1,iyd,izi] + 1.0/A[ixd,iyd+1,izi] + 1.0/A[ixd+1,iyd+1,izi])*4.0"},
+    {"harm_xzi", "1.0/(1.0/A[ixd,iyi,izd] + 1.0/A[ixd+1,iyi,izd] + 1.0/A[ixd,iyi,izd+1] + 1.0/A[ixd+1,iyi,izd+1])*4.0"},
+    {"harm_yzi", "1.0/(1.0/A[ixi,iyd,izd] + 1.0/A[ixi,iyd+1,izd] + 1.0/A[ixi,iyd,izd+1] + 1.0/A[ixi,iyd+1,izd+1])*4.0"},
+    {"maxloc", "max(max(max(max(A[ixi-1,iyi,izi], A[ixi+1,iyi,izi]), A[ixi,iyi,izi]), max(A[ixi,iyi-1,izi], A[ixi,iyi+1,izi])), "
+               "max(A[ixi,iyi,izi-1], A[ixi,iyi,izi+1]))"},
+    {"minloc", "min(min(min(min(A[ixi-1,iyi,izi], A[ixi+1,iyi,izi]), A[ixi,iyi,izi]), min(A[ixi,iyi-1,izi], A[ixi,iyi+1,izi])), "
+               "min(A[ixi,iyi,izi-1], A[ixi,iyi,izi+1]))"},
+};
+
+// @within(macroname, A): per dimension 'a' (firstindex <= ia <= lastindex) or
+// 'i' (firstindex < ii < lastindex), src/FiniteDifferences.jl:63-70,184-196,397-407.
+struct WithinDef {
+  const char* name;
+  const char* roles;
+};
+static const WithinDef kWithin1[] = {{"all", "a"}, {"inn", "i"}};
+static const WithinDef kWithin2[] = {{"all", "aa"}, {"inn", "ii"}, {"inn_x", "ia"}, {"inn_y", "ai"}};
+static const WithinDef kWithin3[] = {{"all", "aaa"}, {"inn", "iii"}, {"inn_x", "iaa"}, {"inn_y", "aia"},
+                                     {"inn_z", "aai"}, {"inn_xy", "iia"}, {"inn_xz", "iai"}, {"inn_yz", "aii"}};
+
+// ---------------------------------------------------------------------------
+// tree helpers
+// ---------------------------------------------------------------------------
+template <class F>
+static void walk(const NodeP& n, F&& f) {
+  if (!n) return;
+  f(n);
+  for (auto& c : n->a) walk(c, f);
+}
+
+template <class F>
+static NodeP rewrite(const NodeP& n, F&& f) {
+  if (!n) return n;
+  NodeP r = f(n);
+  if (r != n) return r;
+  for (auto& c : n->a) c = rewrite(c, f);
+  return n;
+}
+
+static bool refers_to(const NodeP& n, const std::string& name) {
+  bool hit = false;
+  walk(n, [&](const NodeP& x) {
+    if ((x->k == NK::Ident && x->s == name) || (x->k == NK::Macro && !x->a.empty() && x->a[0]->k == NK::Ident && x->a[0]->s == name))
+      hit = true;
+  });
+  return hit;
+}
+
+// Returns a substituted copy; `n` itself (e.g. a cached macro template) is left untouched.
+static NodeP subst_idents(const NodeP& n, const std::map<std::string, NodeP>& m) {
+  return rewrite(clone(n), [&](const NodeP& x) -> NodeP {
+    if (x->k == NK::Ident) {
+      auto it = m.find(x->s);
+      if (it != m.end()) return clone(it->second);
+    }
+    return x;
+  });
+}
+
+// Affine form of an index expression: var (0..2 index, 3 = `end`, -1 none) + c.
+struct Aff {
+  bool ok = false;
+  int var = -1;
+  long long c = 0;
+};
+
+static Aff affine(const NodeP& e) {
+  Aff r;
+  if (!e) return r;
+  switch (e->k) {
+    case NK::Num:
+      if (e->is_int) {
+        r.ok = true;
+        r.c = e->ival;
+      }
+      return r;
+    case NK::Ident:
+      for (int d = 0; d < 3; ++d)
+        if (e->s == kIdx[d]) {
+          r.ok = true;
+          r.var = d;
+        }
+      return r;
+    case NK::EndKw:
+      r.ok = true;
+      r.var = 3;
+      return r;
+    case NK::Binary: {
+      if (e->s != "+" && e->s != "-") return r;
+      Aff a = affine(e->a[0]), b = affine(e->a[1]);
+      if (!a.ok || !b.ok) return r;
+      if (e->s == "+") {
+        if (a.var >= 0 && b.var >= 0) return r;
+        r.ok = true;
+        r.var = a.var >= 0 ? a.var : b.var;
+        r.c = a.c + b.c;
+        return r;
+      }
+      if (b.var >= 0) return r;
+      r.ok = true;
+      r.var = a.var;
+      r.c = a.c - b.c;
+      return r;
+    }
+    case NK::Unary:
+      if (e->s == "+") return affine(e->a[0]);
+      if (e->s == "-") {
+        Aff a = affine(e->a[0]);
+        if (a.ok && a.var < 0) {
+          r.ok = true;
+          r.c = -a.c;
+        }
+      }
+      return r;
+    default:
+      return r;
+  }
+}
+
+static NodeP aff_node(const Aff& a) {
+  if (a.var < 0) return mknum_int(a.c);
+  NodeP v = a.var == 3 ? mk(NK::EndKw, "end") : mk(NK::Ident, kIdx[a.var]);
+  if (a.c == 0) return v;
+  if (a.c > 0) return mk(NK::Binary, "+", {v, mknum_int(a.c)});
+  return mk(NK::Binary, "-", {v, mknum_int(-a.c)});
+}
+
+// Canonicalise the index expressions of every array access to var +/- c.
+static NodeP canon_indices(const NodeP& n) {
+  return rewrite(n, [&](const NodeP& x) -> NodeP {
+    if (x->k == NK::Index) {
+      for (size_t i = 1; i < x->a.size(); ++i) {
+        x->a[i] = canon_indices(x->a[i]);
+        Aff a = affine(x->a[i]);
+        if (a.ok) x->a[i] = aff_node(a);
+      }
+      x->a[0] = canon_indices(x->a[0]);
+      return x;  // children handled
+    }
+    return x;
+  });
+}
+
+static std::vector<std::pair<int, int>> parse_optrange_tuple(const NodeP& t) {
+  // (a:b, c:d, e:f)
+  std::vector<std::pair<int, int>> r;
+  if (!t || t->k != NK::Tuple) throw PsError(PS_EKEYWORD, "optranges entries must be tuples of ranges");
+  for (auto& x : t->a) {
+    if (x->k != NK::Range) throw PsError(PS_EKEYWORD, "optranges entries must be tuples of ranges");
+    Aff lo = affine(x->a[0]), hi = affine(x->a[1]);
+    if (!lo.ok || !hi.ok || lo.var >= 0 || hi.var >= 0) throw PsError(PS_EKEYWORD, "optranges bounds must be integer literals");
+    r.emplace_back((int)lo.c, (int)hi.c);
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// options
+// ---------------------------------------------------------------------------
+static void read_options(Options& o, const JVal& j) {
+  if (j.t != JVal::Obj) throw PsError(PS_EARGUMENT, "kernel options must be a JSON object");
+  for (auto& kv : j.obj) {
+    const std::string& k = kv.first;
+    const JVal& v = kv.second;
+    if (v.t == JVal::Null) continue;
+    if (k == "kind") o.kind = v.s;
+    else if (k == "indices") {
+      o.indices.clear();
+      for (auto& x : v.arr) o.indices.push_back(x.s);
+    } else if (k == "ndims") o.ndims = (int)v.n;
+    else if (k == "numbertype") o.numbertype = v.s;
+    else if (k == "inbounds") o.inbounds = v.b;
+    else if (k == "padding") o.padding = v.b;
+    else if (k == "memopt") o.memopt = v.b;
+    else if (k == "loopsize") o.loopsize = (int)v.n;
+    else if (k == "loopdim") o.loopdim = (int)v.n;
+    else if (k == "optvars") {
+      o.optvars.clear();
+      if (v.t == JVal::Str) o.optvars.push_back(v.s);
+      else
+        for (auto& x : v.arr) o.optvars.push_back(x.s);
+    } else if (k == "optranges") {
+      for (auto& e : v.obj) {
+        std::vector<std::pair<int, int>> r;
+        for (auto& d : e.second.arr) r.emplace_back((int)d.arr.at(0).n, (int)d.arr.at(1).n);
+        o.optranges[e.first] = r;
+      }
+    } else if (k == "nthreads_max_memopt") o.nthreads_max_memopt = (int)v.n;
+    else if (k == "fp_contract") o.fp_contract = v.b;
+    else if (k == "lds") o.lds = v.b;
+    else if (k == "xcd_remap") o.xcd_remap = v.b;
+    else if (k == "useshmemhalos" || k == "optimize_halo_read" || k == "nonconst_metadata" || k == "metadata_module" || k == "N") {
+      // accepted for interface compatibility; they only steer the reference's shared-memory path
+    } else {
+      throw PsError(PS_EKEYWORD, "unknown keyword argument '" + k + "' in kernel definition");
+    }
+  }
+}
+
+static void apply_prefix_kwargs(Options& o, const KernelDef& kd) {
+  for (auto& kw : kd.prefix_kwargs) {
+    const std::string& k = kw.first;
+    const NodeP& v = kw.second;
+    auto as_bool = [&]() {
+      if (v->k == NK::Ident && (v->s == "true" || v->s == "false")) return v->s == "true";
+      throw PsError(PS_EKEYWORD, "keyword '" + k + "' expects true or false");
+    };
+    auto as_int = [&]() {
+      Aff a = affine(v);
+      if (!a.ok || a.var >= 0) throw PsError(PS_EKEYWORD, "keyword '" + k + "' expects an integer literal");
+      return (int)a.c;
+    };
+    if (k == "memopt") o.memopt = as_bool();
+    else if (k == "inbounds") o.inbounds = as_bool();
+    else if (k == "padding") o.padding = as_bool();
+    else if (k == "ndims") o.ndims = as_int();
+    else if (k == "loopsize") o.loopsize = as_int();
+    else if (k == "loopdim") o.loopdim = as_int();
+    else if (k == "optvars") {
+      o.optvars.clear();
+      if (v->k == NK::Ident) o.optvars.push_back(v->s);
+      else if (v->k == NK::Tuple)
+        for (auto& x : v->a) o.optvars.push_back(x->s);
+    } else if (k == "optranges") {
+      if (v->k != NK::Tuple) throw PsError(PS_EKEYWORD, "optranges must be a named tuple");
+      for (auto& e : v->a) {
+        if (e->k != NK::Kw) throw PsError(PS_EKEYWORD, "optranges must be a named tuple");
+        o.optranges[e->s] = parse_optrange_tuple(e->a[0]);
+      }
+    } else if (k == "useshmemhalos" || k == "optimize_halo_read") {
+    } else {
+      throw PsError(PS_EKEYWORD, "unknown keyword argument '" + k + "' in kernel definition");
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// macro expansion
+// ---------------------------------------------------------------------------
+static const MacroDef* find_macro(int nd, const std::string& name) {
+  const MacroDef* tab = nd == 1 ? kFD1 : nd == 2 ? kFD2 : kFD3;
+  size_t n = nd == 1 ? sizeof(kFD1) / sizeof(MacroDef) : nd == 2 ? sizeof(kFD2) / sizeof(MacroDef) : sizeof(kFD3) / sizeof(MacroDef);
+  for (size_t i = 0; i < n; ++i)
+    if (name == tab[i].name) return &tab[i];
+  return nullptr;
+}
+
+static std::map<std::string, NodeP>& template_cache() {
+  static std::map<std::string, NodeP> c;
+  return c;
+}
+
+static NodeP expand_macro(int nd, const std::string& name, const std::string& arr) {
+  const MacroDef* m = find_macro(nd, name);
+  if (!m) {
+    throw PsError(PS_EPLUGIN, "macro @" + name + " is not defined for " + std::to_string(nd) +
+                                  "-D kernels (FiniteDifferences" + std::to_string(nd) + "D)");
+  }
+  std::string key = std::to_string(nd) + ":" + name;
+  auto& cache = template_cache();
+  if (!cache.count(key)) cache[key] = parse_expression(m->tmpl);
+  std::map<std::string, NodeP> sub;
+  const char* dn[3] = {"x", "y", "z"};
+  for (int d = 0; d < 3; ++d) {
+    NodeP v = mk(NK::Ident, kIdx[d]);
+    sub[std::string("i") + dn[d] + "a"] = v;
+    sub[std::string("i") + dn[d] + "d"] = v;  // padding=false: ixd -> ix
+    sub[std::string("i") + dn[d] + "i"] = mk(NK::Binary, "+", {mk(NK::Ident, kIdx[d]), mknum_int(1)});  // ixi -> ix+1
+  }
+  sub["A"] = mk(NK::Ident, arr);
+  return subst_idents(cache[key], sub);
+}
+
+static NodeP expand_all_macros(int nd, const NodeP& n) {
+  return rewrite(n, [&](const NodeP& x) -> NodeP {
+    if (x->k == NK::Macro) {
+      if (x->a.size() != 1 || x->a[0]->k != NK::Ident)
+        throw PsError(PS_EARGUMENT, "finite-difference macro @" + x->s + " expects a single array argument");
+      return expand_macro(nd, x->s, x->a[0]->s);
+    }
+    return x;
+  });
+}
+
+static NodeP within_mask(int nd, const std::string& mname, const std::string& arr) {
+  const WithinDef* tab = nd == 1 ? kWithin1 : nd == 2 ? kWithin2 : kWithin3;
+  size_t n = nd == 1 ? 2 : nd == 2 ? 4 : 8;
+  const WithinDef* w = nullptr;
+  for (size_t i = 0; i < n; ++i)
+    if (mname == tab[i].name) w = &tab[i];
+  if (!w)
+    throw PsError(PS_EARGUMENT, "unkown macroname: @" + mname +
+                                    ". Only @all and the @inn family can be assigned to in @parallel kernels.");
+  NodeP cond;
+  for (int d = 0; d < nd; ++d) {
+    NodeP sz = mk(NK::Call, "size", {mk(NK::Ident, arr), mknum_int(d + 1)});
+    NodeP v = mk(NK::Ident, kIdx[d]);
+    NodeP lo, hi;
+    if (w->roles[d] == 'a') {  // 1 <= ix <= size(A,d)
+      lo = mk(NK::Cmp, "", {mknum_int(1), clone(v)});
+      lo->ops = {"<="};
+      hi = mk(NK::Cmp, "", {clone(v), sz});
+      hi->ops = {"<="};
+    } else {  // 1 < ix+1 < size(A,d)   <=>   1 <= ix <= size(A,d)-2
+      lo = mk(NK::Cmp, "", {mknum_int(1), clone(v)});
+      lo->ops = {"<="};
+      hi = mk(NK::Cmp, "", {clone(v), mk(NK::Binary, "-", {sz, mknum_int(2)})});
+      hi->ops = {"<="};
+    }
+    NodeP c = mk(NK::And, "&&", {lo, hi});
+    cond = cond ? mk(NK::And, "&&", {cond, c}) : c;
+  }
+  return cond;
+}
+
+// Substitutes on-the-fly arrays: A[i,j,k] -> expr_A with the parallel indices
+// replaced by (i,j,k) (src/parallel.jl:821-859, src/shared.jl:59-76).
+static NodeP insert_onthefly(const NodeP& n, const std::map<std::string, NodeP>& otf, int nd, int depth = 0) {
+  if (depth > 32) throw PsError(PS_EARGUMENT, "on-the-fly arrays are nested too deeply (cyclic definition?)");
+  return rewrite(n, [&](const NodeP& x) -> NodeP {
+    if (x->k == NK::Index && x->a[0]->k == NK::Ident) {
+      auto it = otf.find(x->a[0]->s);
+      if (it == otf.end()) return x;
+      if ((int)x->a.size() - 1 != nd)
+        throw PsError(PS_EARGUMENT, "unsupported kernel statements in @parallel kernel definition: wrong number of indices in " +
+                                        x->a[0]->s + " (expected " + std::to_string(nd) + " indices).");
+      std::map<std::string, NodeP> sub;
+      for (int d = 0; d < nd; ++d) sub[kIdx[d]] = insert_onthefly(x->a[1 + d], otf, nd, depth + 1);
+      NodeP e = subst_idents(it->second, sub);
+      return insert_onthefly(e, otf, nd, depth + 1);
+    }
+    return x;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// define
+// ---------------------------------------------------------------------------
+static void collect_access(const NodeP& n, std::set<std::string>& accessed, std::set<std::string>& written,
+                           const std::map<std::string, int>& params) {
+  walk(n, [&](const NodeP& x) {
+    if (x->k == NK::Index && x->a[0]->k == NK::Ident && params.count(x->a[0]->s)) accessed.insert(x->a[0]->s);
+    if (x->k == NK::Assign && x->a[0]->k == NK::Index && x->a[0]->a[0]->k == NK::Ident && params.count(x->a[0]->a[0]->s))
+      written.insert(x->a[0]->a[0]->s);
+  });
+}
+
+static void memopt_analysis(Kernel& K) {
+  MemoptMeta& mm = K.mm;
+  mm.enabled = true;
+  int nd = K.nidx;
+  if (nd != 2 && nd != 3)
+    throw PsError(PS_EINCOHERENT,
+                  "incoherent arguments memopt in @parallel[_indices] <kernel>: optimization can only be applied in 2-D and 3-D @parallel kernels and @parallel_indices kernels.");
+  int loopdim = K.opt.loopdim ? K.opt.loopdim : nd;
+  if (loopdim != nd)
+    throw PsError(PS_EINCOHERENT,
+                  "incoherent arguments memopt in @parallel[_indices] <kernel>: two-index kernels require `loopdim=2` and three-index kernels require `loopdim=3`.");
+  mm.loopdim = loopdim;
+  mm.loopsize = K.opt.loopsize;
+  if (mm.loopsize < 1) throw PsError(PS_EKEYWORD, "loopsize must be >= 1");
+  // read-only arrays in access order
+  std::vector<std::string> readonly;
+  walk(K.body, [&](const NodeP& x) {
+    if (x->k == NK::Index && x->a[0]->k == NK::Ident && K.param_index.count(x->a[0]->s) && !K.written.count(x->a[0]->s) &&
+        std::find(readonly.begin(), readonly.end(), x->a[0]->s) == readonly.end())
+      readonly.push_back(x->a[0]->s);
+  });
+  std::vector<std::string> optvars = K.opt.optvars;
+  if (optvars.empty()) optvars = readonly;
+  for (auto& A : optvars)
+    if (std::find(readonly.begin(), readonly.end(), A) == readonly.end())
+      throw PsError(PS_EINCOHERENT, "incoherent argument memopt in @parallel[_indices] <kernel>: optimization can only be applied to arrays that are only read within the kernel (not applicable to: " + A + ").");
+  // offsets of stencil accesses
+  for (auto& A : optvars) mm.offsets[A];
+  walk(K.body, [&](const NodeP& x) {
+    if (!(x->k == NK::Index && x->a[0]->k == NK::Ident && mm.offsets.count(x->a[0]->s))) return;
+    if ((int)x->a.size() - 1 < nd) return;
+    int off[3] = {0, 0, 0};
+    for (int d = 0; d < nd; ++d) {
+      Aff a = affine(x->a[1 + d]);
+      if (!a.ok || a.var != d) return;  // not a stencil access (e.g. A[1,iy])
+      off[d] = (int)a.c;
+    }
+    for (size_t d = nd; d + 1 < x->a.size(); ++d) {  // extra indices must be literal 1
+      Aff a = affine(x->a[1 + d]);
+      if (!a.ok || a.var >= 0 || a.c != 1) return;
+    }
+    std::pair<int, int> col = nd == 3 ? std::make_pair(off[0], off[1]) : std::make_pair(off[0], 0);
+    int oz = off[nd - 1];
+    mm.offsets[x->a[0]->s][col][oz] += 1;
+  });
+  // remove optvars that are accessed at a single point only (no reuse along the loop
+  // dimension) unless their optrange is given explicitly (src/memopt.jl:83, 1012-1025)
+  for (auto& A : optvars) {
+    auto& offs = mm.offsets[A];
+    bool multi = false;
+    int npoints = 0;
+    for (auto& c : offs) {
+      npoints += (int)c.second.size();
+      if (c.second.size() > 1) multi = true;
+    }
+    if (npoints == 0) continue;
+    if (npoints > 1 || K.opt.optranges.count(A)) {
+      (void)multi;
+      mm.optvars.push_back(A);
+    }
+  }
+  if (mm.optvars.empty())
+    throw PsError(PS_EINCOHERENT,
+                  "incoherent argument memopt in @parallel[_indices] <kernel>: optimization can only be applied if there is at least one array that is read-only within the kernel (and accessed with a multi-point stencil). Set memopt=false for this kernel.");
+  for (auto it = mm.offsets.begin(); it != mm.offsets.end();) {
+    if (std::find(mm.optvars.begin(), mm.optvars.end(), it->first) == mm.optvars.end()) it = mm.offsets.erase(it);
+    else ++it;
+  }
+  for (auto& A : mm.optvars) {
+    std::vector<std::pair<int, int>> sr(3, {0, 0});
+    bool first = true;
+    for (auto& c : mm.offsets[A]) {
+      for (auto& z : c.second) {
+        int o[3] = {c.first.first, nd == 3 ? c.first.second : z.first, nd == 3 ? z.first : 0};
+        for (int d = 0; d < 3; ++d) {
+          if (first) sr[d] = {o[d], o[d]};
+          else {
+            sr[d].first = std::min(sr[d].first, o[d]);
+            sr[d].second = std::max(sr[d].second, o[d]);
+          }
+        }
+        first = false;
+      }
+    }
+    if (K.opt.optranges.count(A)) {
+      auto r = K.opt.optranges.at(A);
+      while (r.size() < 3) r.push_back({0, 0});
+      mm.optranges[A] = r;
+      mm.stencilranges[A] = r;
+    } else {
+      mm.stencilranges[A] = sr;
+    }
+    bool xyhalo = nd == 3 && (mm.stencilranges[A][0].second > mm.stencilranges[A][0].first ||
+                              mm.stencilranges[A][1].second > mm.stencilranges[A][1].first);
+    if (xyhalo) mm.shmem_optvars.push_back(A);
+    mm.use_shmemhalos[A] = nd == 3;
+  }
+}
+
+Kernel define_kernel(const std::string& text, const JVal& opts) {
+  Kernel K;
+  K.def = parse_kernel(text);
+  read_options(K.opt, opts);
+  if (!K.def.kind.empty()) K.opt.kind = K.def.kind;
+  if (!K.def.indices.empty()) K.opt.indices = K.def.indices;
+  apply_prefix_kwargs(K.opt, K.def);
+  if (K.opt.numbertype != "Float64" && K.opt.numbertype != "Float32")
+    throw PsError(PS_EARGUMENT, "numbertype must be Float64 or Float32 (obtained: " + K.opt.numbertype + ")");
+  for (size_t i = 0; i < K.def.params.size(); ++i) K.param_index[K.def.params[i].name] = (int)i;
+
+  // `return` statements at the end of the body are dropped (add_return re-adds them).
+  auto strip_returns = [](NodeP body) {
+    auto b = mk(NK::Block);
+    for (auto& s : body->a) {
+      if (s->k == NK::Return) {
+        if (!s->a.empty() && !(s->a[0]->k == NK::Ident && s->a[0]->s == "nothing"))
+          throw PsError(PS_EARGUMENT, "kernels must return nothing");
+        continue;
+      }
+      b->a.push_back(s);
+    }
+    return b;
+  };
+
+  if (K.opt.kind == "parallel") {
+    int nd = K.opt.ndims;
+    if (nd < 1 || nd > 3)
+      throw PsError(PS_EKEYWORD, "@parallel: keyword argument 'ndims' is invalid or missing (valid values are 1, 2 or 3; 'ndims' an be set globally in @init_parallel_stencil and overwritten per kernel if needed).");
+    K.nidx = nd;
+    NodeP body = strip_returns(K.def.body);
+    // validate_body: exclusively macro array assignments
+    for (auto& s : body->a) {
+      if (!(s->k == NK::Assign && s->s.empty() && s->a[0]->k == NK::Macro))
+        throw PsError(PS_EARGUMENT,
+                      "unsupported kernel statements in @parallel kernel definition: @parallel is only applicable to kernels that contain exclusively array assignments using macros from FiniteDifferences{1|2|3}D or from another compatible computation submodule. @parallel_indices supports any kind of statements in the kernels.");
+      if (s->a[0]->a.size() != 1 || s->a[0]->a[0]->k != NK::Ident)
+        throw PsError(PS_EARGUMENT, "unsupported kernel statements in @parallel kernel definition: the left-hand side must be @macro(A) with A an array.");
+    }
+    // interpolated indices ($ix, $iy, $iz) name the parallel indices
+    body = rewrite(body, [&](const NodeP& x) -> NodeP {
+      if (x->k == NK::Interp) {
+        for (int d = 0; d < 3; ++d)
+          if (x->s == std::string(1, "xyz"[d]).insert(0, "i")) return mk(NK::Ident, kIdx[d]);
+      }
+      return x;
+    });
+    // write vars and on-the-fly extraction
+    std::vector<std::string> write_vars;
+    for (auto& s : body->a) {
+      const std::string& A = s->a[0]->a[0]->s;
+      if (K.param_index.count(A) && std::find(write_vars.begin(), write_vars.end(), A) == write_vars.end()) write_vars.push_back(A);
+    }
+    std::map<std::string, NodeP> otf;
+    auto remaining = mk(NK::Block);
+    for (auto& s : body->a) {
+      const std::string& m = s->a[0]->s;
+      const std::string& A = s->a[0]->a[0]->s;
+      if (K.param_index.count(A)) {
+        remaining->a.push_back(s);
+        continue;
+      }
+      if (m != "all")
+        throw PsError(PS_EARGUMENT, "unsupported kernel statements in @parallel kernel definition: partial assignments are not possible for arrays that are not stored in global memory (arrays that are not among the arguments of the kernel); use '@all' instead.");
+      if (refers_to(s->a[1], A))
+        throw PsError(PS_EARGUMENT, "unsupported kernel statements in @parallel kernel definition: auto-dependency is not possible for arrays that are not stored in global memory (arrays that are not among the arguments of the kernel).");
+      for (auto& w : write_vars)
+        if (refers_to(s->a[1], w))
+          throw PsError(PS_EARGUMENT, "unsupported kernel statements in @parallel kernel definition: the assignment of " + A +
+                                          " should be done on the fly as it is not among the arguments of the kernel; however, this is not possible because it depends on at least one variable that is not read-only within the scope of the kernel (any of: " + w + ").");
+      otf[A] = expand_all_macros(nd, s->a[1]);
+      K.otf_vars.push_back(A);
+    }
+    // masks, macro expansion, on-the-fly insertion
+    auto out = mk(NK::Block);
+    for (auto& s : remaining->a) {
+      const std::string& m = s->a[0]->s;
+      const std::string& A = s->a[0]->a[0]->s;
+      NodeP mask = within_mask(nd, m, A);
+      NodeP lhs = expand_macro(nd, m, A);
+      if (lhs->k != NK::Index) throw PsError(PS_EARGUMENT, "@" + m + " cannot be assigned to");
+      NodeP rhs = expand_all_macros(nd, s->a[1]);
+      rhs = insert_onthefly(rhs, otf, nd);
+      auto as = mk(NK::Assign, "", {canon_indices(lhs), canon_indices(rhs)});
+      auto blk = mk(NK::Block, "", {as});
+      out->a.push_back(mk(NK::If, "", {mask, blk, nullptr}));
+    }
+    K.body = out;
+  } else if (K.opt.kind == "parallel_indices") {
+    if (K.opt.indices.empty()) throw PsError(PS_EARGUMENT, "@parallel_indices: argument 'indices' must be a tuple of indices");
+    if (K.opt.indices.size() > 3) throw PsError(PS_EARGUMENT, "@parallel_indices: at most 3 parallel indices are supported");
+    K.nidx = (int)K.opt.indices.size();
+    std::map<std::string, NodeP> sub;
+    for (int d = 0; d < K.nidx; ++d) sub[K.opt.indices[d]] = mk(NK::Ident, kIdx[d]);
+    NodeP body = strip_returns(K.def.body);
+    body = subst_idents(body, sub);
+    body = rewrite(body, [&](const NodeP& x) -> NodeP {
+      if (x->k == NK::Interp) {
+        for (int d = 0; d < 3; ++d)
+          if (x->s == std::string(1, "xyz"[d]).insert(0, "i")) return mk(NK::Ident, kIdx[d]);
+      }
+      if (x->k == NK::Macro) throw PsError(PS_EARGUMENT, "@parallel_indices kernels index arrays explicitly; macro @" + x->s + " is not supported here");
+      return x;
+    });
+    // all parallel indices must be used (src/parallel.jl:696-704)
+    for (int d = 0; d < K.nidx; ++d) {
+      bool used = false;
+      walk(body, [&](const NodeP& x) {
+        if (x->k == NK::Ident && x->s == kIdx[d]) used = true;
+      });
+      if (!used) throw PsError(PS_EARGUMENT, "@parallel_indices: all parallel indices must be used in the kernel body (unused indices: " + K.opt.indices[d] + ").");
+    }
+    K.body = canon_indices(body);
+  } else {
+    throw PsError(PS_EARGUMENT, "unknown kernel kind '" + K.opt.kind + "'");
+  }
+  collect_access(K.body, K.accessed, K.written, K.param_index);
+  if (K.opt.memopt) memopt_analysis(K);
+  return K;
+}
+
+// ---------------------------------------------------------------------------
+// info
+// ---------------------------------------------------------------------------
+std::string kernel_info_json(const Kernel& K) {
+  std::ostringstream o;
+  o << "{\"name\":" << json_str(K.def.name) << ",\"kind\":" << json_str(K.opt.kind) << ",\"nb_parallel_indices\":" << K.nidx
+    << ",\"numbertype\":" << json_str(K.opt.numbertype) << ",\"memopt\":" << (K.opt.memopt ? "true" : "false") << ",\"params\":[";
+  for (size_t i = 0; i < K.def.params.size(); ++i)
+    o << (i ? "," : "") << "{\"name\":" << json_str(K.def.params[i].name) << ",\"annot\":" << json_str(K.def.params[i].annot) << "}";
+  o << "],\"written\":[";
+  bool f = true;
+  for (auto& w : K.written) {
+    o << (f ? "" : ",") << json_str(w);
+    f = false;
+  }
+  o << "],\"accessed\":[";
+  f = true;
+  for (auto& w : K.accessed) {
+    o << (f ? "" : ",") << json_str(w);
+    f = false;
+  }
+  o << "],\"onthefly\":[";
+  for (size_t i = 0; i < K.otf_vars.size(); ++i) o << (i ? "," : "") << json_str(K.otf_vars[i]);
+  o << "]";
+  if (K.mm.enabled) {
+    const auto& mm = K.mm;
+    o << ",\"loopdim\":" << mm.loopdim << ",\"loopsize\":" << mm.loopsize << ",\"loopsizes\":["
+      << (mm.loopdim == 3 ? "1,1," + std::to_string(mm.loopsize) : mm.loopdim == 2 ? "1," + std::to_string(mm.loopsize) + ",1" : std::to_string(mm.loopsize) + ",1,1")
+      << "],\"optvars\":[";
+    for (size_t i = 0; i < mm.optvars.size(); ++i) o << (i ? "," : "") << json_str(mm.optvars[i]);
+    o << "],\"stencilranges\":{";
+    for (size_t i = 0; i < mm.optvars.size(); ++i) {
+      auto& r = mm.stencilranges.at(mm.optvars[i]);
+      o << (i ? "," : "") << json_str(mm.optvars[i]) << ":[[" << r[0].first << "," << r[0].second << "],[" << r[1].first << ","
+        << r[1].second << "],[" << r[2].first << "," << r[2].second << "]]";
+    }
+    o << "},\"optranges\":{";
+    f = true;
+    for (auto& kv : mm.optranges) {
+      auto& r = kv.second;
+      o << (f ? "" : ",") << json_str(kv.first) << ":[[" << r[0].first << "," << r[0].second << "],[" << r[1].first << ","
+        << r[1].second << "],[" << r[2].first << "," << r[2].second << "]]";
+      f = false;
+    }
+    o << "},\"offsets\":{";
+    f = true;
+    for (auto& kv : mm.offsets) {
+      o << (f ? "" : ",") << json_str(kv.first) << ":[";
+      bool g = true;
+      for (auto& c : kv.second)
+        for (auto& z : c.second) {
+          o << (g ? "" : ",") << "[" << c.first.first << "," << c.first.second << "," << z.first << "," << z.second << "]";
+          g = false;
+        }
+      o << "]";
+      f = false;
+    }
+    o << "},\"shmem_optvars\":[";
+    for (size_t i = 0; i < mm.shmem_optvars.size(); ++i) o << (i ? "," : "") << json_str(mm.shmem_optvars[i]);
+    o << "],\"use_any_shmem\":" << (mm.shmem_optvars.empty() ? "false" : "true") << ",\"use_shmemhalos\":{";
+    f = true;
+    for (auto& kv : mm.use_shmemhalos) {
+      o << (f ? "" : ",") << json_str(kv.first) << ":" << (kv.second ? "true" : "false");
+      f = false;
+    }
+    o << "},\"shmem_dim1\":1,\"shmem_dim2\":" << (mm.loopdim == 3 ? 2 : 3) << ",\"shmem_spans\":{";
+    for (size_t i = 0; i < mm.optvars.size(); ++i) {
+      auto& r = mm.stencilranges.at(mm.optvars[i]);
+      int d2 = mm.loopdim == 3 ? 1 : 2;
+      o << (i ? "," : "") << json_str(mm.optvars[i]) << ":[" << (r[0].second - r[0].first) << "," << (r[d2].second - r[d2].first) << "]";
+    }
+    o << "}";
+  }
+  o << ",\"body\":" << json_str(to_julia(K.body)) << "}";
+  return o.str();
+}
+
+std::string Variant::key() const {
+  char b[96];
+  std::snprintf(b, sizeof b, "m%dc%di%da%dt%dl%d", memopt, checked, idx64, noalias, maxthreads, lds);
+  return b;
+}
+
+}  // namespace ps

@@ -1,0 +1,107 @@
+// Stencil IR and HIP lowering (host-only C++).
+#pragma once
+#include <array>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "ps_ast.h"
+#include "ps_json.h"
+
+namespace ps {
+
+enum class Ty { Int, F64, F32, Bool };
+
+struct ArgKind {
+  bool is_field = false;
+  int dtype = 0;   // PS_F64 ...
+  int ndims = 0;   // fields only
+};
+
+struct MemoptMeta {
+  bool enabled = false;
+  int loopdim = 0;      // 1-based dimension streamed by each thread
+  int loopsize = 32;
+  std::vector<std::string> optvars;
+  // per optvar: stencil range [min,max] per dimension (3 entries)
+  std::map<std::string, std::vector<std::pair<int, int>>> stencilranges;
+  std::map<std::string, std::vector<std::pair<int, int>>> optranges;  // explicit optranges
+  // per optvar: offsets keyed by (ox,oy) -> oz -> count (loopdim 3) or (ox) -> oy (loopdim 2)
+  std::map<std::string, std::map<std::pair<int, int>, std::map<int, int>>> offsets;
+  std::vector<std::string> shmem_optvars;
+  std::map<std::string, bool> use_shmemhalos;
+};
+
+struct Options {
+  std::string kind = "parallel";          // parallel | parallel_indices
+  std::vector<std::string> indices;       // parallel_indices names
+  int ndims = 3;                          // @parallel kernels
+  std::string numbertype = "Float64";
+  bool inbounds = false;
+  bool padding = false;
+  bool memopt = false;
+  int loopsize = 32;
+  int loopdim = 0;
+  std::vector<std::string> optvars;
+  std::map<std::string, std::vector<std::pair<int, int>>> optranges;
+  int nthreads_max_memopt = 256;
+  int nthreads_x_max = 64;
+  int nthreads_max = 256;
+  bool fp_contract = false;               // Julia semantics: no FMA contraction
+  bool lds = false;                       // stage x-y neighbours of optvars through LDS
+  bool xcd_remap = true;                  // XCD-aware block ordering for memopt kernels
+};
+
+struct Kernel {
+  KernelDef def;
+  Options opt;
+  int nidx = 3;                           // number of parallel indices
+  NodeP body;                             // processed statements
+  std::set<std::string> written;          // params assigned to
+  std::set<std::string> accessed;         // params indexed
+  std::vector<std::string> otf_vars;
+  std::map<std::string, int> param_index;
+  MemoptMeta mm;
+};
+
+struct Variant {
+  bool memopt = false;
+  bool checked = false;   // bounds-checked accesses + error flag
+  bool idx64 = false;     // 64-bit index arithmetic
+  bool noalias = true;    // __restrict__ on array pointers
+  int maxthreads = 256;   // __launch_bounds__
+  bool lds = false;
+  std::string key() const;
+};
+
+struct Emitted {
+  std::string source;
+  std::string entry;
+  // kernel parameter layout: one entry per parameter in order
+  struct P {
+    int kind;      // 0 field ptr, 1 int size, 2 scalar, 3 range start, 4 range len, 5 err ptr, 6 loop info
+    int arg;       // arg index (fields/scalars)
+    int which;     // for sizes: 0 nx 1 ny 2 nz 3 sx 4 sxy; for ranges: dim
+    int bytes;     // 4 or 8
+    int dtype;     // scalar dtype
+  };
+  std::vector<P> params;
+  int lds_bytes = 0;
+};
+
+Kernel define_kernel(const std::string& text, const JVal& opts);
+Emitted lower_kernel(const Kernel& k, const std::vector<ArgKind>& kinds, const Variant& v);
+std::string kernel_info_json(const Kernel& k);
+
+// Bounds proof: true if every array access of the kernel stays inside its
+// array for the given launch box and argument shapes.
+struct LaunchShape {
+  std::vector<ArgKind> kinds;
+  std::vector<std::array<long long, 3>> dims;  // per arg (fields)
+  std::vector<long long> ivals;                 // integer scalar values
+  long long lo[3], hi[3];                       // index box (1-based inclusive)
+};
+bool prove_in_bounds(const Kernel& k, const LaunchShape& s, bool memopt, std::string* why);
+
+}  // namespace ps

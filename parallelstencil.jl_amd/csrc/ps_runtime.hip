@@ -1,0 +1,1047 @@
+// libpsamd runtime: the C-ABI declared in include/ps_amd.h.
+//
+// Device memory, streams and events are plain HIP; generated kernels are
+// compiled with hiprtc for gfx950 and cached on disk by the hash of their
+// source and compile options, so a kernel compiles once per machine image.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <dlfcn.h>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <vector>
+
+#include "ps_amd.h"
+#include "ps_lower.h"
+
+using namespace ps;
+
+// ---------------------------------------------------------------------------
+// error state
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define PS_HIP(call)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) return fail(PS_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define PS_GUARD_BEGIN try {
+#define PS_GUARD_END                                       \
+  }                                                        \
+  catch (const PsError& e) {                               \
+    return fail(e.code, e.what());                         \
+  }                                                        \
+  catch (const std::exception& e) {                        \
+    return fail(PS_EARGUMENT, e.what());                   \
+  }
+
+extern "C" const char* ps_last_error(void) { return g_err.c_str(); }
+extern "C" int ps_version(void) { return 10000; }
+
+// ---------------------------------------------------------------------------
+// compiled-in helper kernels (fill, copy, reductions, error flag)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) ps_fill_k(T* p, long long nx, long long ny, long long nz, long long sx, long long sxy, T v) {
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  long long n = nx * ny * nz;
+  for (; i < n; i += (long long)gridDim.x * 256) {
+    long long x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+    p[x + y * sx + z * sxy] = v;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) ps_copy_k(T* __restrict__ d, const T* __restrict__ s, long long nx, long long ny, long long nz,
+                                                 long long dsx, long long dsxy, long long ssx, long long ssxy) {
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  long long n = nx * ny * nz;
+  for (; i < n; i += (long long)gridDim.x * 256) {
+    long long x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+    d[x + y * dsx + z * dsxy] = s[x + y * ssx + z * ssxy];
+  }
+}
+
+__global__ void ps_set_i32_k(int* p, int v) { *p = v; }
+
+template <class T>
+__device__ __forceinline__ double ps_to_f64(T v) { return (double)v; }
+template <>
+__device__ __forceinline__ double ps_to_f64<unsigned short>(unsigned short v) { return (double)__uint_as_float(((unsigned)v) << 16); }
+
+// Stage 1: per-block partial (sum, sum|x|, max|x|) over the logical box in a
+// fixed order, so repeated reductions of the same data give the same bits.
+template <class T>
+__global__ void __launch_bounds__(256) ps_reduce1_k(const T* __restrict__ p, long long nx, long long ny, long long nz, long long sx,
+                                                    long long sxy, double* __restrict__ part) {
+  __shared__ double s0[256], s1[256], s2[256];
+  long long n = nx * ny * nz;
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    long long x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+    double v = ps_to_f64<T>(p[x + y * sx + z * sxy]);
+    a += v;
+    b += fabs(v);
+    c = fmax(c, fabs(v));
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  s2[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x + 0] = s0[0];
+    part[3 * blockIdx.x + 1] = s1[0];
+    part[3 * blockIdx.x + 2] = s2[0];
+  }
+}
+
+__global__ void __launch_bounds__(256) ps_reduce2_k(const double* __restrict__ part, int nparts, double* __restrict__ out) {
+  __shared__ double s0[256], s1[256], s2[256];
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) {
+    a += part[3 * i];
+    b += part[3 * i + 1];
+    c = fmax(c, part[3 * i + 2]);
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  s2[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s0[0];
+    out[1] = s1[0];
+    out[2] = s2[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// global state
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Module {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  Emitted em;
+};
+
+struct KernelObj {
+  Kernel K;
+  std::mutex mu;
+  std::map<std::string, std::shared_ptr<Module>> variants;  // key: kinds+variant
+};
+
+struct State {
+  std::mutex mu;
+  bool inited = false;
+  int device = -1;
+  std::vector<hipStream_t> streams;       // pool 0
+  std::vector<hipStream_t> prio_streams;  // pool 1
+  hipStream_t stream0 = nullptr;
+  int* d_err = nullptr;
+  double* d_part = nullptr;
+  double* d_out = nullptr;
+  double* h_out = nullptr;
+  int nparts_cap = 0;
+  std::string cache_dir;
+};
+
+State& S() {
+  static State s;
+  return s;
+}
+
+size_t elsize(int dtype) {
+  switch (dtype) {
+    case PS_F64: return 8;
+    case PS_F32: return 4;
+    case PS_BF16: return 2;
+    case PS_I32: return 4;
+    case PS_I64: return 8;
+    case PS_BOOL: return 1;
+  }
+  return 0;
+}
+
+std::string default_cache_dir() {
+  if (const char* e = std::getenv("PS_AMD_CACHE")) return e;
+  Dl_info info;
+  if (dladdr((void*)&ps_last_error, &info) && info.dli_fname) {
+    std::string p = info.dli_fname;
+    size_t s = p.rfind('/');
+    if (s != std::string::npos) return p.substr(0, s) + "/kcache";
+  }
+  return "kcache";
+}
+
+const std::string& cache_dir() {
+  State& s = S();
+  if (s.cache_dir.empty()) s.cache_dir = default_cache_dir();
+  return s.cache_dir;
+}
+
+unsigned long long fnv1a(const std::string& x, unsigned long long h = 1469598103934665603ULL) {
+  for (unsigned char c : x) {
+    h ^= c;
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+std::string hash_hex(const std::string& x) {
+  char b[40];
+  std::snprintf(b, sizeof b, "%016llx%016llx", fnv1a(x), fnv1a(x, 0x84222325cbf29ce4ULL));
+  return b;
+}
+
+void mkdirs(const std::string& d) {
+  std::string cur;
+  std::stringstream ss(d);
+  std::string part;
+  if (!d.empty() && d[0] == '/') cur = "/";
+  while (std::getline(ss, part, '/')) {
+    if (part.empty()) continue;
+    cur += part + "/";
+    mkdir(cur.c_str(), 0755);
+  }
+}
+
+std::vector<std::string> compile_opts(const Kernel& K) {
+  std::vector<std::string> o = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  o.push_back(K.opt.fp_contract ? "-ffp-contract=fast" : "-ffp-contract=off");
+  return o;
+}
+
+// Compile (or fetch from the disk cache) the code object of an emitted kernel.
+std::vector<char> code_object(const Kernel& K, const Emitted& em) {
+  auto opts = compile_opts(K);
+  std::string key = em.source;
+  for (auto& x : opts) key += "\n//" + x;
+  int vmaj = 0, vmin = 0;
+  hiprtcVersion(&vmaj, &vmin);
+  key += "\n//hiprtc " + std::to_string(vmaj) + "." + std::to_string(vmin);
+  std::string path = cache_dir() + "/" + hash_hex(key) + ".hsaco";
+  {
+    std::ifstream f(path, std::ios::binary);
+    if (f) {
+      std::vector<char> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+      if (!buf.empty()) return buf;
+    }
+  }
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, em.source.c_str(), (K.def.name + ".hip").c_str(), 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+    throw PsError(PS_ECOMPILE, "hiprtcCreateProgram failed");
+  std::vector<const char*> copts;
+  for (auto& x : opts) copts.push_back(x.c_str());
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+  if (r != HIPRTC_SUCCESS) {
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string log(ls + 1, '\0');
+    hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    throw PsError(PS_ECOMPILE, "hiprtc failed to compile kernel " + K.def.name + ":\n" + log + "\n--- source ---\n" + em.source);
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  std::vector<char> code(cs);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  mkdirs(cache_dir());
+  std::string tmp = path + ".tmp" + std::to_string(getpid());
+  {
+    std::ofstream f(tmp, std::ios::binary);
+    f.write(code.data(), (std::streamsize)code.size());
+  }
+  std::rename(tmp.c_str(), path.c_str());
+  return code;
+}
+
+std::vector<ArgKind> arg_kinds(const ps_arg* args, int nargs) {
+  std::vector<ArgKind> k(nargs);
+  for (int i = 0; i < nargs; ++i) {
+    if (args[i].kind == PS_ARG_FIELD) {
+      k[i].is_field = true;
+      k[i].dtype = args[i].field.dtype;
+      k[i].ndims = args[i].field.ndims;
+    } else if (args[i].kind == PS_ARG_SCALAR) {
+      k[i].dtype = args[i].dtype;
+    } else {
+      throw PsError(PS_EARGUMENT, "argument " + std::to_string(i + 1) + " has an unknown kind");
+    }
+    if (elsize(k[i].dtype) == 0) throw PsError(PS_EARGUMENT, "argument " + std::to_string(i + 1) + " has an unsupported type");
+  }
+  return k;
+}
+
+std::string kinds_key(const std::vector<ArgKind>& k) {
+  std::string s;
+  for (auto& a : k) s += (a.is_field ? "F" : "S") + std::to_string(a.dtype) + (a.is_field ? std::to_string(a.ndims) : "") + ",";
+  return s;
+}
+
+std::shared_ptr<Module> get_module(KernelObj* ko, const std::vector<ArgKind>& kinds, const Variant& v, bool load) {
+  std::string key = kinds_key(kinds) + v.key();
+  std::lock_guard<std::mutex> lk(ko->mu);
+  auto it = ko->variants.find(key);
+  if (it != ko->variants.end() && (!load || it->second->fn)) return it->second;
+  auto m = std::make_shared<Module>();
+  m->em = lower_kernel(ko->K, kinds, v);
+  std::vector<char> code = code_object(ko->K, m->em);
+  if (load) {
+    hipError_t e = hipModuleLoadData(&m->mod, code.data());
+    if (e != hipSuccess) throw PsError(PS_EHIP, std::string("hipModuleLoadData: ") + hipGetErrorString(e));
+    e = hipModuleGetFunction(&m->fn, m->mod, m->em.entry.c_str());
+    if (e != hipSuccess) throw PsError(PS_EHIP, std::string("hipModuleGetFunction: ") + hipGetErrorString(e));
+  }
+  ko->variants[key] = m;
+  return m;
+}
+
+int require_init() {
+  if (!S().inited) return fail(PS_ENOTINIT, "ps_init has not been called (no device initialised)");
+  return 0;
+}
+
+hipStream_t stream_of(void* s) { return s ? (hipStream_t)s : S().stream0; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// device
+// ---------------------------------------------------------------------------
+extern "C" int ps_init(int device) {
+  State& s = S();
+  std::lock_guard<std::mutex> lk(s.mu);
+  if (s.inited) {
+    if (device != s.device) return fail(PS_EINCOHERENT, "ps_init: already initialised on another device");
+    return 0;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(PS_ENODEVICE, "no HIP device available");
+  if (device < 0 || device >= n) return fail(PS_EARGUMENT, "ps_init: invalid device index");
+  PS_HIP(hipSetDevice(device));
+  hipDeviceProp_t p;
+  PS_HIP(hipGetDeviceProperties(&p, device));
+  if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+    return fail(PS_ENODEVICE, std::string("this backend targets gfx950 (MI355X); found ") + p.gcnArchName);
+  PS_HIP(hipStreamCreateWithFlags(&s.stream0, hipStreamNonBlocking));
+  PS_HIP(hipMalloc(&s.d_err, 256));
+  PS_HIP(hipMemset(s.d_err, 0, 256));
+  PS_HIP(hipHostMalloc(&s.h_out, 4 * sizeof(double), hipHostMallocDefault));
+  PS_HIP(hipMalloc(&s.d_out, 4 * sizeof(double)));
+  s.device = device;
+  s.inited = true;
+  return 0;
+}
+
+extern "C" int ps_device_info(ps_devinfo* out) {
+  if (int r = require_init()) return r;
+  hipDeviceProp_t p;
+  PS_HIP(hipGetDeviceProperties(&p, S().device));
+  std::memset(out, 0, sizeof *out);
+  std::snprintf(out->name, sizeof out->name, "%s", p.name);
+  std::snprintf(out->arch, sizeof out->arch, "%s", p.gcnArchName);
+  out->cu_count = p.multiProcessorCount;
+  out->wave_size = p.warpSize;
+  out->lds_bytes_per_block = (int64_t)p.sharedMemPerBlock;
+  out->total_mem_bytes = (int64_t)p.totalGlobalMem;
+  out->l2_bytes = p.l2CacheSize;
+  out->clock_khz = p.clockRate;
+  out->mem_clock_khz = p.memoryClockRate;
+  out->mem_bus_width = p.memoryBusWidth;
+  out->hbm_peak_gbs = 2.0 * (double)p.memoryClockRate * 1e3 * (double)p.memoryBusWidth / 8.0 / 1e9;
+  return 0;
+}
+
+static int check_err_flag(hipStream_t st) {
+  State& s = S();
+  int h = 0;
+  PS_HIP(hipMemcpyAsync(&h, s.d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+  PS_HIP(hipStreamSynchronize(st));
+  if (h) {
+    hipLaunchKernelGGL(ps_set_i32_k, dim3(1), dim3(1), 0, st, s.d_err, 0);
+    PS_HIP(hipStreamSynchronize(st));
+    return fail(PS_EBOUNDS, "BoundsError: a kernel accessed an array outside its bounds (the access was suppressed)");
+  }
+  return 0;
+}
+
+extern "C" int ps_device_sync(void) {
+  if (int r = require_init()) return r;
+  PS_HIP(hipDeviceSynchronize());
+  return check_err_flag(S().stream0);
+}
+
+// ---------------------------------------------------------------------------
+// fields
+// ---------------------------------------------------------------------------
+extern "C" int ps_field_alloc(int ndims, const int64_t dims[3], int dtype, ps_field* out) {
+  if (int r = require_init()) return r;
+  if (ndims < 1 || ndims > 3) return fail(PS_EARGUMENT, "ps_field_alloc: ndims must be 1, 2 or 3");
+  size_t es = elsize(dtype);
+  if (!es) return fail(PS_EARGUMENT, "ps_field_alloc: unsupported element type");
+  int64_t d[3] = {1, 1, 1};
+  for (int i = 0; i < ndims; ++i) {
+    if (dims[i] < 0) return fail(PS_EARGUMENT, "ps_field_alloc: negative dimension");
+    d[i] = dims[i];
+  }
+  int64_t align_el = 256 / (int64_t)es;
+  int64_t pitch = ((std::max<int64_t>(d[0], 1) + align_el - 1) / align_el) * align_el;
+  if (d[1] == 1 && d[2] == 1) pitch = std::max<int64_t>(d[0], 1);  // 1-D fields are dense
+  int64_t plane = pitch * d[1];
+  int64_t bytes = std::max<int64_t>(plane * d[2] * (int64_t)es, 256);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, (size_t)bytes);
+  if (e != hipSuccess) return fail(PS_EOOM, "OutOfMemoryError: cannot allocate " + std::to_string(bytes) + " bytes of device memory");
+  PS_HIP(hipMemsetAsync(p, 0, (size_t)bytes, S().stream0));
+  PS_HIP(hipStreamSynchronize(S().stream0));
+  std::memset(out, 0, sizeof *out);
+  out->ptr = p;
+  out->base = p;
+  for (int i = 0; i < 3; ++i) out->dims[i] = d[i];
+  out->pitch = pitch;
+  out->plane = plane;
+  out->dtype = dtype;
+  out->ndims = ndims;
+  out->alloc_bytes = bytes;
+  return 0;
+}
+
+extern "C" int ps_field_free(ps_field* f) {
+  if (!f || !f->base) return 0;
+  if (int r = require_init()) return r;
+  PS_HIP(hipFree(f->base));
+  f->base = nullptr;
+  f->ptr = nullptr;
+  return 0;
+}
+
+extern "C" int ps_field_view(const ps_field* parent, const int64_t lo[3], const int64_t hi[3], ps_field* out) {
+  for (int i = 0; i < 3; ++i)
+    if (lo[i] < 1 || hi[i] > parent->dims[i] || hi[i] < lo[i] - 1) return fail(PS_EARGUMENT, "ps_field_view: index box outside the parent field");
+  *out = *parent;
+  size_t es = elsize(parent->dtype);
+  out->ptr = (char*)parent->ptr + ((lo[0] - 1) + (lo[1] - 1) * parent->pitch + (lo[2] - 1) * parent->plane) * (int64_t)es;
+  for (int i = 0; i < 3; ++i) out->dims[i] = hi[i] - lo[i] + 1;
+  out->base = nullptr;
+  out->alloc_bytes = 0;
+  return 0;
+}
+
+extern "C" int ps_field_h2d(const ps_field* f, const void* host, void* stream) {
+  if (int r = require_init()) return r;
+  size_t es = elsize(f->dtype);
+  hipStream_t st = stream_of(stream);
+  const char* h = (const char*)host;
+  size_t w = (size_t)f->dims[0] * es;
+  if (f->plane == f->pitch * f->dims[1]) {
+    PS_HIP(hipMemcpy2DAsync(f->ptr, (size_t)f->pitch * es, h, w, w, (size_t)(f->dims[1] * f->dims[2]), hipMemcpyHostToDevice, st));
+  } else {
+    for (int64_t z = 0; z < f->dims[2]; ++z)
+      PS_HIP(hipMemcpy2DAsync((char*)f->ptr + z * f->plane * es, (size_t)f->pitch * es, h + z * w * f->dims[1], w, w, (size_t)f->dims[1],
+                              hipMemcpyHostToDevice, st));
+  }
+  PS_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+extern "C" int ps_field_d2h(const ps_field* f, void* host, void* stream) {
+  if (int r = require_init()) return r;
+  size_t es = elsize(f->dtype);
+  hipStream_t st = stream_of(stream);
+  char* h = (char*)host;
+  size_t w = (size_t)f->dims[0] * es;
+  if (f->plane == f->pitch * f->dims[1]) {
+    PS_HIP(hipMemcpy2DAsync(h, w, f->ptr, (size_t)f->pitch * es, w, (size_t)(f->dims[1] * f->dims[2]), hipMemcpyDeviceToHost, st));
+  } else {
+    for (int64_t z = 0; z < f->dims[2]; ++z)
+      PS_HIP(hipMemcpy2DAsync(h + z * w * f->dims[1], w, (const char*)f->ptr + z * f->plane * es, (size_t)f->pitch * es, w, (size_t)f->dims[1],
+                              hipMemcpyDeviceToHost, st));
+  }
+  PS_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+static unsigned grid_for(long long n) {
+  long long b = (n + 255) / 256;
+  return (unsigned)std::max<long long>(1, std::min<long long>(b, 65536));
+}
+
+extern "C" int ps_field_fill(const ps_field* f, double value, void* stream) {
+  if (int r = require_init()) return r;
+  hipStream_t st = stream_of(stream);
+  long long n = f->dims[0] * f->dims[1] * f->dims[2];
+  if (n == 0) return 0;
+  dim3 g(grid_for(n)), b(256);
+  switch (f->dtype) {
+    case PS_F64: hipLaunchKernelGGL(ps_fill_k<double>, g, b, 0, st, (double*)f->ptr, f->dims[0], f->dims[1], f->dims[2], f->pitch, f->plane, value); break;
+    case PS_F32: hipLaunchKernelGGL(ps_fill_k<float>, g, b, 0, st, (float*)f->ptr, f->dims[0], f->dims[1], f->dims[2], f->pitch, f->plane, (float)value); break;
+    case PS_BF16: {
+      float fv = (float)value;
+      unsigned u;
+      std::memcpy(&u, &fv, 4);
+      unsigned short h = (u & 0x7fffffffu) > 0x7f800000u ? (unsigned short)0x7fc0u : (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+      hipLaunchKernelGGL(ps_fill_k<unsigned short>, g, b, 0, st, (unsigned short*)f->ptr, f->dims[0], f->dims[1], f->dims[2], f->pitch, f->plane, h);
+      break;
+    }
+    case PS_I32: hipLaunchKernelGGL(ps_fill_k<int>, g, b, 0, st, (int*)f->ptr, f->dims[0], f->dims[1], f->dims[2], f->pitch, f->plane, (int)value); break;
+    case PS_I64: hipLaunchKernelGGL(ps_fill_k<long long>, g, b, 0, st, (long long*)f->ptr, f->dims[0], f->dims[1], f->dims[2], f->pitch, f->plane, (long long)value); break;
+    case PS_BOOL: hipLaunchKernelGGL(ps_fill_k<unsigned char>, g, b, 0, st, (unsigned char*)f->ptr, f->dims[0], f->dims[1], f->dims[2], f->pitch, f->plane, (unsigned char)(value != 0)); break;
+    default: return fail(PS_EARGUMENT, "ps_field_fill: unsupported element type");
+  }
+  PS_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int ps_field_copy(const ps_field* dst, const ps_field* src, void* stream) {
+  if (int r = require_init()) return r;
+  for (int i = 0; i < 3; ++i)
+    if (dst->dims[i] != src->dims[i]) return fail(PS_EARGUMENT, "copy!: fields differ in size");
+  if (dst->dtype != src->dtype) return fail(PS_EARGUMENT, "copy!: fields differ in element type");
+  hipStream_t st = stream_of(stream);
+  long long n = dst->dims[0] * dst->dims[1] * dst->dims[2];
+  if (n == 0) return 0;
+  dim3 g(grid_for(n)), b(256);
+  size_t es = elsize(dst->dtype);
+#define PS_COPY(T) hipLaunchKernelGGL(ps_copy_k<T>, g, b, 0, st, (T*)dst->ptr, (const T*)src->ptr, dst->dims[0], dst->dims[1], dst->dims[2], dst->pitch, dst->plane, src->pitch, src->plane)
+  if (es == 8) PS_COPY(long long);
+  else if (es == 4) PS_COPY(int);
+  else if (es == 2) PS_COPY(unsigned short);
+  else PS_COPY(unsigned char);
+#undef PS_COPY
+  PS_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int ps_field_reduce(const ps_field* f, int op, double* out, void* stream) {
+  if (int r = require_init()) return r;
+  if (op < 0 || op > 3) return fail(PS_EARGUMENT, "ps_field_reduce: op must be 0..3");
+  State& s = S();
+  hipStream_t st = stream_of(stream);
+  long long n = f->dims[0] * f->dims[1] * f->dims[2];
+  int nb = (int)std::min<long long>(std::max<long long>((n + 255) / 256, 1), 2048);
+  {
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (s.nparts_cap < nb) {
+      if (s.d_part) PS_HIP(hipFree(s.d_part));
+      PS_HIP(hipMalloc(&s.d_part, sizeof(double) * 3 * 2048));
+      s.nparts_cap = 2048;
+    }
+  }
+  switch (f->dtype) {
+    case PS_F64: hipLaunchKernelGGL(ps_reduce1_k<double>, dim3(nb), dim3(256), 0, st, (const double*)f->ptr, f->dims[0], f->dims[1], f->dims[2], f->pitch, f->plane, s.d_part); break;
+    case PS_F32: hipLaunchKernelGGL(ps_reduce1_k<float>, dim3(nb), dim3(256), 0, st, (const float*)f->ptr, f->dims[0], f->dims[1], f->dims[2], f->pitch, f->plane, s.d_part); break;
+    case PS_BF16: hipLaunchKernelGGL(ps_reduce1_k<unsigned short>, dim3(nb), dim3(256), 0, st, (const unsigned short*)f->ptr, f->dims[0], f->dims[1], f->dims[2], f->pitch, f->plane, s.d_part); break;
+    default: return fail(PS_EARGUMENT, "ps_field_reduce: floating-point fields only");
+  }
+  hipLaunchKernelGGL(ps_reduce2_k, dim3(1), dim3(256), 0, st, s.d_part, nb, s.d_out);
+  PS_HIP(hipGetLastError());
+  double h[3];
+  PS_HIP(hipMemcpyAsync(h, s.d_out, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  PS_HIP(hipStreamSynchronize(st));
+  if (op == 0) *out = h[0];
+  else if (op == 1) *out = h[1];
+  else if (op == 2) *out = h[2];
+  else *out = n ? h[1] / (double)n : 0.0;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// streams and events
+// ---------------------------------------------------------------------------
+extern "C" int ps_stream_get(int id, int priority, void** out) {
+  if (int r = require_init()) return r;
+  if (id < 1) return fail(PS_EARGUMENT, "stream ids start at 1");
+  State& s = S();
+  std::lock_guard<std::mutex> lk(s.mu);
+  auto& pool = priority ? s.prio_streams : s.streams;
+  while ((int)pool.size() < id) {
+    hipStream_t st;
+    PS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    pool.push_back(st);
+  }
+  *out = (void*)pool[id - 1];
+  return 0;
+}
+
+extern "C" int ps_stream_sync(void* stream) {
+  if (int r = require_init()) return r;
+  hipStream_t st = stream_of(stream);
+  PS_HIP(hipStreamSynchronize(st));
+  return check_err_flag(st);
+}
+
+extern "C" int ps_event_create(void** ev) {
+  if (int r = require_init()) return r;
+  hipEvent_t e;
+  PS_HIP(hipEventCreate(&e));
+  *ev = (void*)e;
+  return 0;
+}
+extern "C" int ps_event_record(void* ev, void* stream) {
+  PS_HIP(hipEventRecord((hipEvent_t)ev, stream_of(stream)));
+  return 0;
+}
+extern "C" int ps_event_elapsed_ms(void* ev0, void* ev1, float* ms) {
+  PS_HIP(hipEventSynchronize((hipEvent_t)ev1));
+  PS_HIP(hipEventElapsedTime(ms, (hipEvent_t)ev0, (hipEvent_t)ev1));
+  return 0;
+}
+extern "C" int ps_event_destroy(void* ev) {
+  PS_HIP(hipEventDestroy((hipEvent_t)ev));
+  return 0;
+}
+
+extern "C" int ps_set_cache_dir(const char* dir) {
+  S().cache_dir = dir ? dir : "";
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// geometry helpers
+// ---------------------------------------------------------------------------
+extern "C" int ps_compute_nthreads(const int64_t maxsize[3], int nthreads_x_max, int nthreads_max, int flatdim, int32_t out[3]) {
+  int64_t nx = std::min<int64_t>(nthreads_x_max, flatdim == 1 ? 1 : maxsize[0]);
+  if (nx < 1) nx = 1;
+  int64_t ny = std::min<int64_t>((nthreads_max + nx - 1) / nx, flatdim == 2 ? 1 : maxsize[1]);
+  if (ny < 1) ny = 1;
+  int64_t nz = std::min<int64_t>((nthreads_max + nx * ny - 1) / (nx * ny), flatdim == 3 ? 1 : maxsize[2]);
+  if (nz < 1) nz = 1;
+  out[0] = (int32_t)nx;
+  out[1] = (int32_t)ny;
+  out[2] = (int32_t)nz;
+  return 0;
+}
+
+extern "C" int ps_compute_nblocks(const int64_t maxsize[3], const int32_t nthreads[3], int32_t out[3]) {
+  for (int i = 0; i < 3; ++i) {
+    if (nthreads[i] < 1) return fail(PS_EARGUMENT, "nthreads must be positive");
+    out[i] = (int32_t)((maxsize[i] + nthreads[i] - 1) / nthreads[i]);
+  }
+  return 0;
+}
+
+static int validate_hc(const int64_t r[3][2], const int64_t bw[3]) {
+  int64_t ms[3];
+  for (int i = 0; i < 3; ++i) {
+    if (r[i][0] != 1)
+      return fail(PS_EARGUMENT, "unsupported call to `@hide_communication boundary_width block`: automatic ranges determination for @hide_communication is not supported for ranges with start != 1. Use the explicit version `@hide_communication ranges_outer ranges_inner block` instead.");
+    ms[i] = r[i][1] - r[i][0] + 1;
+  }
+  for (int i = 0; i < 3; ++i)
+    if (ms[i] <= 0) return fail(PS_EINCOHERENT, "incoherent arguments in @hide_communication: the following must be true: all(length.(ranges) .> 0) ");
+  for (int i = 0; i < 3; ++i)
+    if (!(ms[i] > 2 * bw[i])) return fail(PS_EINCOHERENT, "incoherent arguments in @hide_communication: the following must be true: all(length.(ranges) .> 2 .* boundary_width) ");
+  for (int i = 0; i < 3; ++i)
+    if (ms[i] > 1 && bw[i] == 0) return fail(PS_EINCOHERENT, "incoherent arguments in @hide_communication: the following must be false: any((length.(ranges) .> 1) .& (boundary_width .== 0)) ");
+  if (ms[0] == 1 && ms[1] == 1 && ms[2] == 1) return fail(PS_EINCOHERENT, "incoherent arguments in @hide_communication: the following must be false: all(length.(ranges) .== 1)");
+  return 0;
+}
+
+extern "C" int ps_ranges_outer(const int64_t r[3][2], const int64_t bw[3], int64_t out[6][3][2], int* nout) {
+  if (int e = validate_hc(r, bw)) return e;
+  int64_t ms[3] = {r[0][1], r[1][1], r[2][1]};
+  int64_t boxes[6][3][2];
+  int n = 0;
+  auto add = [&](int64_t a0, int64_t a1, int64_t b0, int64_t b1, int64_t c0, int64_t c1) {
+    boxes[n][0][0] = a0; boxes[n][0][1] = a1;
+    boxes[n][1][0] = b0; boxes[n][1][1] = b1;
+    boxes[n][2][0] = c0; boxes[n][2][1] = c1;
+    ++n;
+  };
+  if (ms[2] > 1) {
+    add(1, bw[0], bw[1] + 1, ms[1] - bw[1], bw[2] + 1, ms[2] - bw[2]);
+    add(1, ms[0], 1, ms[1], 1, bw[2]);
+    add(1, ms[0], 1, ms[1], ms[2] - bw[2] + 1, ms[2]);
+    add(ms[0] - bw[0] + 1, ms[0], bw[1] + 1, ms[1] - bw[1], bw[2] + 1, ms[2] - bw[2]);
+    add(1, ms[0], 1, bw[1], bw[2] + 1, ms[2] - bw[2]);
+    add(1, ms[0], ms[1] - bw[1] + 1, ms[1], bw[2] + 1, ms[2] - bw[2]);
+  } else if (ms[1] > 1) {
+    add(ms[0] - bw[0] + 1, ms[0], bw[1] + 1, ms[1] - bw[1], 1, 1);
+    add(1, ms[0], 1, bw[1], 1, 1);
+    add(1, ms[0], ms[1] - bw[1] + 1, ms[1], 1, 1);
+    add(1, bw[0], bw[1] + 1, ms[1] - bw[1], 1, 1);
+  } else {
+    add(ms[0] - bw[0] + 1, ms[0], 1, 1, 1, 1);
+    add(1, bw[0], 1, 1, 1, 1);
+  }
+  int k = 0;
+  for (int i = 0; i < n; ++i) {
+    bool nonempty = true;
+    for (int d = 0; d < 3; ++d)
+      if (boxes[i][d][1] - boxes[i][d][0] + 1 == 0) nonempty = false;
+    if (!nonempty) continue;
+    std::memcpy(out[k], boxes[i], sizeof boxes[i]);
+    ++k;
+  }
+  *nout = k;
+  return 0;
+}
+
+extern "C" int ps_ranges_inner(const int64_t r[3][2], const int64_t bw[3], int64_t out[3][2]) {
+  if (int e = validate_hc(r, bw)) return e;
+  int64_t ms[3] = {r[0][1], r[1][1], r[2][1]};
+  for (int d = 0; d < 3; ++d) {
+    out[d][0] = bw[d] + 1;
+    out[d][1] = ms[d] - bw[d];
+  }
+  if (ms[2] <= 1) out[2][0] = out[2][1] = 1;
+  if (ms[1] <= 1) out[1][0] = out[1][1] = 1;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+extern "C" int ps_kernel_define(const char* text, const char* opts_json, ps_kernel* out) {
+  PS_GUARD_BEGIN
+  auto ko = new KernelObj();
+  try {
+    ko->K = define_kernel(text ? text : "", json_parse(opts_json ? opts_json : ""));
+  } catch (...) {
+    delete ko;
+    throw;
+  }
+  *out = (ps_kernel)ko;
+  return 0;
+  PS_GUARD_END
+}
+
+extern "C" int ps_kernel_free(ps_kernel k) {
+  KernelObj* ko = (KernelObj*)k;
+  if (!ko) return 0;
+  for (auto& v : ko->variants)
+    if (v.second->mod) (void)hipModuleUnload(v.second->mod);
+  delete ko;
+  return 0;
+}
+
+extern "C" int ps_kernel_info(ps_kernel k, char* out, size_t cap) {
+  PS_GUARD_BEGIN
+  std::string s = kernel_info_json(((KernelObj*)k)->K);
+  if (s.size() + 1 > cap) return fail(PS_EARGUMENT, "ps_kernel_info: buffer too small (need " + std::to_string(s.size() + 1) + ")");
+  std::memcpy(out, s.c_str(), s.size() + 1);
+  return 0;
+  PS_GUARD_END
+}
+
+static Variant variant_from_flags(const Kernel& K, int flags) {
+  Variant v;
+  v.memopt = (flags & 1) ? true : false;
+  if (v.memopt && !K.mm.enabled) throw PsError(PS_EINCOHERENT, "the kernel was not defined with memopt=true");
+  v.checked = (flags & 2) != 0;
+  v.idx64 = (flags & 4) != 0;
+  v.noalias = (flags & 8) == 0;
+  v.maxthreads = (flags & 16) ? 1024 : 256;
+  return v;
+}
+
+extern "C" int ps_kernel_source(ps_kernel k, const ps_arg* args, int nargs, int flags, char* out, size_t cap) {
+  PS_GUARD_BEGIN
+  KernelObj* ko = (KernelObj*)k;
+  Variant v = variant_from_flags(ko->K, flags);
+  Emitted em = lower_kernel(ko->K, arg_kinds(args, nargs), v);
+  if (em.source.size() + 1 > cap) return fail(PS_EARGUMENT, "ps_kernel_source: buffer too small (need " + std::to_string(em.source.size() + 1) + ")");
+  std::memcpy(out, em.source.c_str(), em.source.size() + 1);
+  return 0;
+  PS_GUARD_END
+}
+
+extern "C" int ps_kernel_precompile(ps_kernel k, const ps_arg* args, int nargs, int flags) {
+  PS_GUARD_BEGIN
+  KernelObj* ko = (KernelObj*)k;
+  Variant v = variant_from_flags(ko->K, flags);
+  get_module(ko, arg_kinds(args, nargs), v, false);
+  return 0;
+  PS_GUARD_END
+}
+
+namespace {
+
+struct Plan {
+  bool memopt = false;
+  int64_t ranges[3][2];
+  int32_t nthreads[3];
+  int32_t nblocks[3];
+  Variant v;
+};
+
+static const char* kErrAutoRanges =
+    "automatic detection of ranges not possible in @parallel <kernelcall>: some kernel arguments are neither arrays nor scalars nor any other bitstypes nor (named) tuple containing any of the former. Specify ranges or nthreads and nblocks manually.";
+static const char* kErrNbIdx =
+    "the number of parallel indices of the kernel does not match the number of dimensions of the kernel arguments (maximal number of dimensions of array arguments); specify the ranges explicitly or use @parallel_indices with the matching number of indices.";
+
+Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch* L, std::vector<ArgKind>& kinds) {
+  (void)kErrAutoRanges;
+  Plan P;
+  kinds = arg_kinds(args, nargs);
+  if (nargs != (int)K.def.params.size())
+    throw PsError(PS_EARGUMENT, "MethodError: kernel " + K.def.name + " takes " + std::to_string(K.def.params.size()) + " arguments (" +
+                                    std::to_string(nargs) + " given)");
+  bool memopt = K.mm.enabled;
+  if (L && L->memopt == 1 && !memopt) throw PsError(PS_EINCOHERENT, "@parallel memopt=true: the kernel was not defined with memopt=true");
+  if (L && L->memopt == 0) memopt = false;
+  P.memopt = memopt;
+  int64_t maxsize[3] = {1, 1, 1};
+  int maxnd = 1;
+  for (int i = 0; i < nargs; ++i) {
+    if (args[i].kind != PS_ARG_FIELD) continue;
+    for (int d = 0; d < 3; ++d) maxsize[d] = std::max<int64_t>(maxsize[d], args[i].field.dims[d]);
+    maxnd = std::max(maxnd, (int)args[i].field.ndims);
+  }
+  bool user_ranges = L && L->has_ranges;
+  if (user_ranges) {
+    for (int d = 0; d < 3; ++d) {
+      P.ranges[d][0] = L->ranges[d][0];
+      P.ranges[d][1] = L->ranges[d][1];
+    }
+  } else {
+    if (maxnd != K.nidx) throw PsError(PS_EARGUMENT, kErrNbIdx);
+    for (int d = 0; d < 3; ++d) {
+      P.ranges[d][0] = 1;
+      P.ranges[d][1] = maxsize[d];
+    }
+  }
+  int64_t len[3];
+  for (int d = 0; d < 3; ++d) {
+    len[d] = P.ranges[d][1] - P.ranges[d][0] + 1;
+    if (len[d] < 0) throw PsError(PS_EARGUMENT, "ranges must not be decreasing");
+  }
+  bool user_geom = L && (L->nthreads[0] || L->nblocks[0]);
+  if (memopt) {
+    const MemoptMeta& mm = K.mm;
+    int Ld = mm.loopdim - 1;
+    int64_t loopsizes[3] = {1, 1, 1};
+    loopsizes[Ld] = mm.loopsize;
+    if (K.opt.kind == "parallel" && !user_ranges) {
+      // get_ranges_memopt: pad to multiples of nthreads (src/parallel.jl:560-571)
+      int32_t nt[3];
+      ps_compute_nthreads(len, K.opt.nthreads_x_max, K.opt.nthreads_max_memopt, mm.loopdim, nt);
+      for (int d = 0; d < 3; ++d) {
+        int64_t rest = len[d] % nt[d];
+        if (rest) len[d] += nt[d] - rest;
+        P.ranges[d][1] = P.ranges[d][0] + len[d] - 1;
+      }
+    }
+    int64_t ms[3];
+    for (int d = 0; d < 3; ++d) ms[d] = (len[d] + loopsizes[d] - 1) / loopsizes[d];
+    if (user_geom) {
+      for (int d = 0; d < 3; ++d) {
+        P.nthreads[d] = L->nthreads[d] ? L->nthreads[d] : 1;
+        P.nblocks[d] = L->nblocks[d] ? L->nblocks[d] : (int32_t)((ms[d] + P.nthreads[d] - 1) / P.nthreads[d]);
+      }
+    } else {
+      ps_compute_nthreads(ms, K.opt.nthreads_x_max, K.opt.nthreads_max_memopt, mm.loopdim, P.nthreads);
+      // compute_nthreads_memopt checks (src/parallel.jl:548-558)
+      for (auto& A : mm.optvars) {
+        const auto& sr = mm.stencilranges.at(A);
+        int64_t he[3];
+        for (int d = 0; d < 3; ++d) he[d] = (d == Ld) ? 0 : (sr[d].second - sr[d].first);
+        int64_t p1 = (int64_t)P.nthreads[0] * P.nthreads[1] * P.nthreads[2];
+        int64_t p2 = (P.nthreads[0] + he[0]) * (P.nthreads[1] + he[1]) * (P.nthreads[2] + he[2]);
+        if (2 * p1 < p2)
+          throw PsError(PS_EARGUMENT, "@parallel <kernelcall>: the automatic determination of nthreads is not possible for this case. Please specify `nthreads` and `nblocks`.");
+      }
+      for (int d = 0; d < 3; ++d)
+        if (ms[d] % P.nthreads[d] != 0)
+          throw PsError(PS_EARGUMENT, "@parallel <kernelcall>: memopt optimization not possible for the given maximum array size in the kernel arguments (the maximum array size must be dividable without rest by the number of threads per block)");
+      ps_compute_nblocks(ms, P.nthreads, P.nblocks);
+    }
+  } else {
+    if (user_geom) {
+      for (int d = 0; d < 3; ++d) {
+        P.nthreads[d] = L->nthreads[d] ? L->nthreads[d] : 1;
+        P.nblocks[d] = L->nblocks[d] ? L->nblocks[d] : (int32_t)((len[d] + P.nthreads[d] - 1) / P.nthreads[d]);
+      }
+    } else {
+      ps_compute_nthreads(len, K.opt.nthreads_x_max, K.opt.nthreads_max, 0, P.nthreads);
+      ps_compute_nblocks(len, P.nthreads, P.nblocks);
+    }
+  }
+  for (int d = 0; d < 3; ++d)
+    if (P.nthreads[d] < 1 || P.nblocks[d] < 0) throw PsError(PS_EARGUMENT, "invalid nthreads / nblocks");
+  if ((int64_t)P.nthreads[0] * P.nthreads[1] * P.nthreads[2] > 1024) throw PsError(PS_EARGUMENT, "at most 1024 threads per block are supported");
+  if (P.nblocks[1] > 65535 || P.nblocks[2] > 65535) throw PsError(PS_EARGUMENT, "nblocks in y and z must not exceed 65535");
+  // variant
+  Variant v;
+  v.memopt = memopt;
+  v.maxthreads = ((int64_t)P.nthreads[0] * P.nthreads[1] * P.nthreads[2] > 256) ? 1024 : 256;
+  bool big = false;
+  for (int i = 0; i < nargs; ++i) {
+    if (args[i].kind == PS_ARG_FIELD) {
+      const ps_field& f = args[i].field;
+      if (f.plane * std::max<int64_t>(f.dims[2], 1) + f.pitch > (int64_t)INT32_MAX / 2) big = true;
+    } else if (args[i].dtype == PS_I32 || args[i].dtype == PS_I64) {
+      if (args[i].i > INT32_MAX / 2 || args[i].i < INT32_MIN / 2) big = true;
+    }
+  }
+  for (int d = 0; d < 3; ++d)
+    if (P.ranges[d][1] > INT32_MAX / 2 || P.ranges[d][0] < INT32_MIN / 2) big = true;
+  v.idx64 = big;
+  // aliasing: a written field that overlaps another field argument
+  for (int i = 0; i < nargs && v.noalias; ++i) {
+    if (args[i].kind != PS_ARG_FIELD) continue;
+    for (int j = i + 1; j < nargs; ++j) {
+      if (args[j].kind != PS_ARG_FIELD) continue;
+      const ps_field &a = args[i].field, &b = args[j].field;
+      size_t ea = elsize(a.dtype), eb = elsize(b.dtype);
+      const char* a0 = (const char*)a.ptr;
+      const char* a1 = a0 + (a.plane * std::max<int64_t>(a.dims[2], 1)) * (int64_t)ea;
+      const char* b0 = (const char*)b.ptr;
+      const char* b1 = b0 + (b.plane * std::max<int64_t>(b.dims[2], 1)) * (int64_t)eb;
+      if (a0 < b1 && b0 < a1) v.noalias = false;
+    }
+  }
+  // bounds proof over the index box the threads can reach
+  LaunchShape sh;
+  sh.kinds = kinds;
+  sh.dims.resize(nargs);
+  sh.ivals.resize(nargs);
+  for (int i = 0; i < nargs; ++i) {
+    if (args[i].kind == PS_ARG_FIELD)
+      for (int d = 0; d < 3; ++d) sh.dims[i][d] = args[i].field.dims[d];
+    else
+      sh.ivals[i] = args[i].i;
+  }
+  for (int d = 0; d < 3; ++d) {
+    sh.lo[d] = P.ranges[d][0];
+    int64_t reach = (int64_t)P.nthreads[d] * P.nblocks[d];
+    if (memopt && d == K.mm.loopdim - 1) reach *= K.mm.loopsize;
+    sh.hi[d] = std::min<int64_t>(P.ranges[d][1], P.ranges[d][0] + reach - 1);
+  }
+  std::string why;
+  v.checked = !prove_in_bounds(K, sh, memopt, &why);
+  P.v = v;
+  return P;
+}
+
+}  // namespace
+
+extern "C" int ps_kernel_geometry(ps_kernel k, const ps_arg* args, int nargs, const ps_launch* launch, int32_t nthreads[3], int32_t nblocks[3],
+                                  int64_t ranges[3][2], int64_t* lds_bytes) {
+  PS_GUARD_BEGIN
+  KernelObj* ko = (KernelObj*)k;
+  std::vector<ArgKind> kinds;
+  Plan P = plan_launch(ko->K, args, nargs, launch, kinds);
+  for (int d = 0; d < 3; ++d) {
+    nthreads[d] = P.nthreads[d];
+    nblocks[d] = P.nblocks[d];
+    ranges[d][0] = P.ranges[d][0];
+    ranges[d][1] = P.ranges[d][1];
+  }
+  if (lds_bytes) *lds_bytes = 0;
+  return (P.memopt ? 1 : 0) | (P.v.checked ? 2 : 0) | (P.v.idx64 ? 4 : 0);
+  PS_GUARD_END
+}
+
+extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, const ps_launch* launch) {
+  if (int r = require_init()) return r;
+  PS_GUARD_BEGIN
+  KernelObj* ko = (KernelObj*)k;
+  std::vector<ArgKind> kinds;
+  Plan P = plan_launch(ko->K, args, nargs, launch, kinds);
+  for (int d = 0; d < 3; ++d)
+    if (P.nblocks[d] == 0) return 0;  // empty range
+  auto m = get_module(ko, kinds, P.v, true);
+  // pack parameters
+  std::vector<unsigned long long> slots(m->em.params.size());
+  std::vector<void*> ptrs(m->em.params.size());
+  for (size_t i = 0; i < m->em.params.size(); ++i) {
+    const auto& p = m->em.params[i];
+    unsigned long long v = 0;
+    long long iv = 0;
+    bool isint = true;
+    switch (p.kind) {
+      case 0:
+        std::memcpy(&v, &args[p.arg].field.ptr, 8);
+        isint = false;
+        break;
+      case 1: {
+        const ps_field& f = args[p.arg].field;
+        iv = p.which == 0 ? f.dims[0] : p.which == 1 ? f.dims[1] : p.which == 2 ? f.dims[2] : p.which == 3 ? f.pitch : f.plane;
+        break;
+      }
+      case 2: {
+        const ps_arg& a = args[p.arg];
+        if (a.dtype == PS_F64) {
+          std::memcpy(&v, &a.f, 8);
+          isint = false;
+        } else if (a.dtype == PS_F32) {
+          float fv = (float)a.f;
+          std::memcpy(&v, &fv, 4);
+          isint = false;
+        } else if (a.dtype == PS_BF16) {
+          throw PsError(PS_EARGUMENT, "BFloat16 scalars are not supported; pass Float32");
+        } else {
+          iv = a.dtype == PS_BOOL ? (a.i != 0) : a.i;
+        }
+        break;
+      }
+      case 3: iv = P.ranges[p.which][0]; break;
+      case 4: iv = P.ranges[p.which][1] - P.ranges[p.which][0] + 1; break;
+      case 5: {
+        void* e = (void*)S().d_err;
+        std::memcpy(&v, &e, 8);
+        isint = false;
+        break;
+      }
+    }
+    if (isint) {
+      if (p.bytes == 4) {
+        int32_t x = (int32_t)iv;
+        std::memcpy(&v, &x, 4);
+      } else {
+        std::memcpy(&v, &iv, 8);
+      }
+    }
+    slots[i] = v;
+    ptrs[i] = &slots[i];
+  }
+  hipStream_t st = stream_of(launch ? launch->stream : nullptr);
+  hipError_t e = hipModuleLaunchKernel(m->fn, (unsigned)P.nblocks[0], (unsigned)P.nblocks[1], (unsigned)P.nblocks[2], (unsigned)P.nthreads[0],
+                                       (unsigned)P.nthreads[1], (unsigned)P.nthreads[2], 0, st, ptrs.data(), nullptr);
+  if (e != hipSuccess) return fail(PS_EHIP, std::string("hipModuleLaunchKernel: ") + hipGetErrorString(e));
+  if (!(launch && launch->async)) {
+    PS_HIP(hipStreamSynchronize(st));
+    if (P.v.checked) return check_err_flag(st);
+  }
+  return 0;
+  PS_GUARD_END
+}

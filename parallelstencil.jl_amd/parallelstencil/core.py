@@ -1,0 +1,840 @@
+"""Host-side mirror of the ParallelStencil API over libpsamd.so.
+
+Names, argument meaning and error behaviour follow the reference:
+  init_parallel_stencil     src/init_parallel_stencil.jl:20-151
+  parallel / parallel_indices / parallel_async
+                            src/parallel.jl:75-215, src/ParallelKernel/parallel.jl:149-164
+  synchronize               src/ParallelKernel/parallel.jl:175-184
+  zeros / ones / rand / fill / fill! / falses / trues
+                            src/ParallelKernel/allocators.jl:21-172
+  hide_communication        src/ParallelKernel/hide_communication.jl:147-258
+  get_stream / get_priority_stream
+                            src/ParallelKernel/hide_communication.jl:158-174
+Kernels are written in the FiniteDifferences macro language of the reference
+and passed as text; all parsing, lowering, compilation and launching happens
+in the shared library. There is no CPU execution path: without the library
+or without a GPU the calls raise.
+"""
+import contextlib
+import ctypes
+import inspect
+import json
+import threading
+import warnings
+
+import numpy as np
+
+from . import _lib
+from ._lib import (PS_ARG_FIELD, PS_ARG_SCALAR, PS_BF16, PS_BOOL, PS_F32, PS_F64, PS_I32, PS_I64, ArgumentError,
+                   IncoherentArgumentError, KeywordArgumentError, NotInitializedError, check, lib, ps_arg, ps_devinfo,
+                   ps_field, ps_launch)
+
+AMDGPU = "AMDGPU"
+Float64 = "Float64"
+Float32 = "Float32"
+BFloat16 = "BFloat16"
+Int32 = "Int32"
+Int64 = "Int64"
+Bool = "Bool"
+
+_DTYPES = {
+    "Float64": PS_F64, "Float32": PS_F32, "BFloat16": PS_BF16, "Int32": PS_I32, "Int64": PS_I64, "Int": PS_I64,
+    "Bool": PS_BOOL,
+}
+_NP = {PS_F64: np.float64, PS_F32: np.float32, PS_BF16: np.uint16, PS_I32: np.int32, PS_I64: np.int64, PS_BOOL: np.bool_}
+_NP_TO_CODE = {np.dtype(np.float64): PS_F64, np.dtype(np.float32): PS_F32, np.dtype(np.int32): PS_I32,
+               np.dtype(np.int64): PS_I64, np.dtype(np.bool_): PS_BOOL}
+_CODE_NAME = {v: k for k, v in _DTYPES.items() if k != "Int"}
+
+
+def dtype_code(t):
+    if isinstance(t, int) and t in _NP:
+        return t
+    if isinstance(t, str):
+        if t in _DTYPES:
+            return _DTYPES[t]
+        raise ArgumentError(f"unsupported element type {t}")
+    try:
+        return _NP_TO_CODE[np.dtype(t)]
+    except (TypeError, KeyError):
+        raise ArgumentError(f"unsupported element type {t}") from None
+
+
+# ---------------------------------------------------------------------------
+# initialisation (per caller module, like the reference's metadata module)
+# ---------------------------------------------------------------------------
+class _Config:
+    __slots__ = ("package", "numbertype", "ndims", "inbounds", "padding", "memopt", "nonconst_metadata")
+
+    def __init__(self, **kw):
+        for k in self.__slots__:
+            setattr(self, k, kw[k])
+
+    def astuple(self):
+        return tuple(getattr(self, k) for k in self.__slots__)
+
+
+_configs = {}
+_device_ready = False
+_lock = threading.Lock()
+
+
+def _caller(depth):
+    f = inspect.currentframe()
+    for _ in range(depth + 1):
+        f = f.f_back
+    return f.f_globals.get("__name__", "__main__")
+
+
+def init_parallel_stencil(package=AMDGPU, numbertype=Float64, ndims=3, *, inbounds=False, padding=False, memopt=False,
+                          nonconst_metadata=False, module=None):
+    """@init_parallel_stencil(package, numbertype, ndims; inbounds, padding, memopt, nonconst_metadata)."""
+    mod = module or _caller(1)
+    if package not in ("AMDGPU", "HIP"):
+        raise ArgumentError(f"unsupported package {package}: this backend runs on AMD MI355X GPUs (package AMDGPU)")
+    if numbertype not in ("Float64", "Float32"):
+        raise ArgumentError(f"unsupported numbertype {numbertype} (Float64 or Float32)")
+    if ndims not in (1, 2, 3):
+        raise ArgumentError(f"ndims must be evaluatable at parse time (e.g. literal or constant) and has to be one of the following Integers: 1, 2, 3 (obtained: {ndims}).")
+    for name, val in (("memopt", memopt), ("nonconst_metadata", nonconst_metadata), ("inbounds", inbounds), ("padding", padding)):
+        if not isinstance(val, bool):
+            raise ArgumentError(f"{name} must be evaluatable at parse time (e.g. literal or constant) and has to be of type Bool.")
+    if padding:
+        raise ArgumentError("padding=true is not supported by the gfx950 backend: fields are pitch-aligned instead")
+    cfg = _Config(package="AMDGPU", numbertype=numbertype, ndims=ndims, inbounds=inbounds, padding=padding, memopt=memopt,
+                  nonconst_metadata=nonconst_metadata)
+    if mod in _configs:
+        if _configs[mod].astuple() == cfg.astuple():
+            warnings.warn(f"ParallelStencil has already been initialized for the module {mod}, with the same arguments.")
+            return Data(mod)
+        raise IncoherentCallError(
+            f"ParallelStencil has already been initialized for the module {mod}, with different arguments. "
+            "Call reset_parallel_stencil() first.")
+    _configs[mod] = cfg
+    return Data(mod)
+
+
+class IncoherentCallError(IncoherentArgumentError):
+    pass
+
+
+def is_initialized(module=None):
+    return (module or _caller(1)) in _configs
+
+
+def reset_parallel_stencil(module=None):
+    _configs.pop(module or _caller(1), None)
+
+
+def _config(mod):
+    if mod not in _configs:
+        raise NotInitializedError(f"no ParallelStencil macro or function can be called before @init_parallel_stencil in each module (missing call in {mod}).")
+    return _configs[mod]
+
+
+def ensure_device(device=0):
+    """Initialises the GPU once (ps_init); raises if none is present."""
+    global _device_ready
+    with _lock:
+        if not _device_ready:
+            check(lib.ps_init(device))
+            _device_ready = True
+
+
+def device_info():
+    ensure_device()
+    d = ps_devinfo()
+    check(lib.ps_device_info(ctypes.byref(d)))
+    return {f: (getattr(d, f).decode() if isinstance(getattr(d, f), bytes) else getattr(d, f)) for f, _ in d._fields_}
+
+
+class Data:
+    """The Data module of the reference (src/ParallelKernel/Data.jl): Number is the numbertype."""
+
+    def __init__(self, mod):
+        self.module = mod
+        self.Number = _configs[mod].numbertype
+        self.Array = DeviceArray
+        self.Field = DeviceArray
+
+
+# ---------------------------------------------------------------------------
+# device arrays
+# ---------------------------------------------------------------------------
+class DeviceArray:
+    """A pitch-aligned device field (x fastest, 1-based in kernels)."""
+
+    __slots__ = ("f", "_owner", "_parent", "__weakref__")
+
+    def __init__(self, f, owner=True, parent=None):
+        self.f = f
+        self._owner = owner
+        self._parent = parent
+
+    @property
+    def ndim(self):
+        return self.f.ndims
+
+    @property
+    def shape(self):
+        return tuple(int(self.f.dims[i]) for i in range(self.f.ndims))
+
+    @property
+    def dtype_code(self):
+        return self.f.dtype
+
+    @property
+    def eltype(self):
+        return _CODE_NAME[self.f.dtype]
+
+    @property
+    def pitch(self):
+        return int(self.f.pitch)
+
+    @property
+    def ptr(self):
+        return int(self.f.ptr or 0)
+
+    def __len__(self):
+        return self.shape[0]
+
+    def __repr__(self):
+        return f"DeviceArray({self.eltype}, {self.shape}, pitch={self.pitch})"
+
+    def to_host(self, stream=None):
+        """Array(A): a Fortran-ordered numpy copy (BFloat16 comes back as raw uint16 bits)."""
+        out = np.empty(self.shape, dtype=_NP[self.f.dtype], order="F")
+        if out.size:
+            check(lib.ps_field_d2h(ctypes.byref(self.f), out.ctypes.data_as(ctypes.c_void_p), _stream_handle(stream)))
+        return out
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.to_host()
+        return a if dtype is None else a.astype(dtype)
+
+    def copy_from(self, host, stream=None):
+        """copy!(A, host)."""
+        h = np.asarray(host)
+        if self.f.dtype == PS_BF16 and h.dtype != np.uint16:
+            h = f32_to_bf16(h)
+        h = np.asfortranarray(h, dtype=_NP[self.f.dtype])
+        if h.shape != self.shape:
+            if h.size == int(np.prod(self.shape)) and h.ndim != self.ndim:
+                h = np.asfortranarray(h.reshape(self.shape, order="F"))
+            else:
+                raise ArgumentError(f"copy!: shape mismatch (device {self.shape}, host {h.shape})")
+        if h.size:
+            check(lib.ps_field_h2d(ctypes.byref(self.f), h.ctypes.data_as(ctypes.c_void_p), _stream_handle(stream)))
+        return self
+
+    def fill_(self, value, stream=None):
+        """fill!(A, x)."""
+        check(lib.ps_field_fill(ctypes.byref(self.f), float(value), _stream_handle(stream)))
+        if stream is None:
+            check(lib.ps_stream_sync(None))
+        return self
+
+    def copy_(self, other, stream=None):
+        check(lib.ps_field_copy(ctypes.byref(self.f), ctypes.byref(other.f), _stream_handle(stream)))
+        if stream is None:
+            check(lib.ps_stream_sync(None))
+        return self
+
+    def view(self, lo, hi):
+        """A view of the 1-based inclusive index box lo..hi."""
+        lo3 = (ctypes.c_int64 * 3)(*(list(lo) + [1] * (3 - len(lo))))
+        hi3 = (ctypes.c_int64 * 3)(*(list(hi) + [1] * (3 - len(hi))))
+        v = ps_field()
+        check(lib.ps_field_view(ctypes.byref(self.f), lo3, hi3, ctypes.byref(v)))
+        v.ndims = self.f.ndims
+        return DeviceArray(v, owner=False, parent=self)
+
+    def reduce(self, op, stream=None):
+        out = ctypes.c_double()
+        code = {"sum": 0, "sum_abs": 1, "max_abs": 2, "mean_abs": 3}[op]
+        check(lib.ps_field_reduce(ctypes.byref(self.f), code, ctypes.byref(out), _stream_handle(stream)))
+        return out.value
+
+    def free(self):
+        if self._owner and self.f.base:
+            check(lib.ps_field_free(ctypes.byref(self.f)))
+
+    def __del__(self):
+        try:
+            if self._owner and self.f.base and lib is not None:
+                lib.ps_field_free(ctypes.byref(self.f))
+        except Exception:
+            pass
+
+
+def Array(A):
+    """Array(A): host copy of a device array."""
+    return A.to_host()
+
+
+def f32_to_bf16(x):
+    """Round float32 values to BFloat16 bits (nearest even; NaN -> 0x7fc0)."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = a.view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    r[nan] = 0x7FC0
+    return np.asarray(r).reshape(np.shape(x))
+
+
+def bf16_to_f32(b):
+    return (np.asarray(b, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+class ArgSpec:
+    """Shape and element type of a field, without memory (for ahead-of-time compilation)."""
+
+    def __init__(self, shape, eltype=Float64):
+        self.shape = tuple(shape)
+        self.f = ps_field()
+        for i in range(3):
+            self.f.dims[i] = self.shape[i] if i < len(self.shape) else 1
+        self.f.ndims = len(self.shape)
+        self.f.dtype = dtype_code(eltype)
+        self.f.pitch = self.shape[0] if self.shape else 1
+        self.f.plane = self.f.pitch * (self.shape[1] if len(self.shape) > 1 else 1)
+
+
+def _numbertype_of(mod):
+    return _configs[mod].numbertype if mod in _configs else Float64
+
+
+def _alloc(dims, eltype):
+    ensure_device()
+    if len(dims) == 1 and isinstance(dims[0], (tuple, list)):
+        dims = tuple(dims[0])
+    dims = tuple(int(d) for d in dims)
+    if not 1 <= len(dims) <= 3:
+        raise ArgumentError("fields have 1, 2 or 3 dimensions")
+    f = ps_field()
+    d3 = (ctypes.c_int64 * 3)(*(list(dims) + [1] * (3 - len(dims))))
+    check(lib.ps_field_alloc(len(dims), d3, dtype_code(eltype), ctypes.byref(f)))
+    return DeviceArray(f)
+
+
+def zeros(*dims, eltype=None):
+    """@zeros(dims...; eltype=numbertype)."""
+    return _alloc(dims, eltype or _numbertype_of(_caller(1)))
+
+
+def ones(*dims, eltype=None):
+    A = _alloc(dims, eltype or _numbertype_of(_caller(1)))
+    return A.fill_(1.0)
+
+
+def fill(x, *dims, eltype=None):
+    A = _alloc(dims, eltype or _numbertype_of(_caller(1)))
+    return A.fill_(x)
+
+
+def fill_(A, x):
+    return A.fill_(x)
+
+
+def falses(*dims):
+    return _alloc(dims, Bool)
+
+
+def trues(*dims):
+    return _alloc(dims, Bool).fill_(1)
+
+
+_rng = np.random.default_rng(0)
+
+
+def seed(s):
+    """Seeds the host generator behind rand()."""
+    global _rng
+    _rng = np.random.default_rng(s)
+
+
+def rand(*dims, eltype=None):
+    """@rand: uniform [0,1) values drawn on the host (as the reference's AMDGPU allocator does) and copied."""
+    A = _alloc(dims, eltype or _numbertype_of(_caller(1)))
+    code = A.dtype_code
+    if code in (PS_F64, PS_F32, PS_BF16):
+        h = _rng.random(A.shape)
+        h = f32_to_bf16(h.astype(np.float32)) if code == PS_BF16 else h.astype(_NP[code])
+    elif code == PS_BOOL:
+        h = _rng.random(A.shape) < 0.5
+    else:
+        h = _rng.integers(0, 2**31 - 1, size=A.shape).astype(_NP[code])
+    return A.copy_from(np.asfortranarray(h))
+
+
+# ---------------------------------------------------------------------------
+# streams and events
+# ---------------------------------------------------------------------------
+class Stream:
+    __slots__ = ("handle", "id", "priority")
+
+    def __init__(self, handle, id_, priority):
+        self.handle = handle
+        self.id = id_
+        self.priority = priority
+
+    def synchronize(self):
+        check(lib.ps_stream_sync(self.handle))
+
+    def __repr__(self):
+        return f"Stream(id={self.id}, {'priority' if self.priority else 'normal'} pool)"
+
+
+_streams = {}
+
+
+def get_stream(id_):
+    """get_rocstream(id) equivalent: stream `id` of the normal pool."""
+    return _get_stream(id_, 0)
+
+
+def get_priority_stream(id_):
+    """get_priority_rocstream(id) equivalent. Streams of both pools run at the default
+    hardware priority; the pools only separate boundary and inner work."""
+    return _get_stream(id_, 1)
+
+
+def _get_stream(id_, prio):
+    ensure_device()
+    key = (int(id_), prio)
+    if key not in _streams:
+        h = ctypes.c_void_p()
+        check(lib.ps_stream_get(int(id_), prio, ctypes.byref(h)))
+        _streams[key] = Stream(h.value, int(id_), prio)
+    return _streams[key]
+
+
+_tls = threading.local()
+
+
+def _default_stream():
+    return getattr(_tls, "stream", None)
+
+
+def _stream_handle(s):
+    if s is None:
+        s = _default_stream()
+    if s is None:
+        return None
+    return s.handle if isinstance(s, Stream) else s
+
+
+@contextlib.contextmanager
+def stream_context(s):
+    """Makes `s` the default stream of @parallel calls in this thread."""
+    prev = getattr(_tls, "stream", None)
+    _tls.stream = s
+    try:
+        yield s
+    finally:
+        _tls.stream = prev
+
+
+def synchronize(stream=None):
+    """@synchronize() / @synchronize(stream)."""
+    ensure_device()
+    if stream is None:
+        check(lib.ps_device_sync())
+    else:
+        check(lib.ps_stream_sync(_stream_handle(stream)))
+
+
+class Event:
+    def __init__(self):
+        ensure_device()
+        h = ctypes.c_void_p()
+        check(lib.ps_event_create(ctypes.byref(h)))
+        self.h = h
+
+    def record(self, stream=None):
+        check(lib.ps_event_record(self.h, _stream_handle(stream)))
+        return self
+
+    def elapsed_ms(self, later):
+        ms = ctypes.c_float()
+        check(lib.ps_event_elapsed_ms(self.h, later.h, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            lib.ps_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# kernels
+# ---------------------------------------------------------------------------
+def _range3(r):
+    """Normalises ranges: (start, stop) pairs (Julia start:stop) per dimension, 1-D/2-D promoted."""
+    if isinstance(r, UnitRange):
+        r = (r,)
+    if len(r) == 2 and all(isinstance(x, (int, np.integer)) for x in r):
+        r = ((r[0], r[1]),)
+    out = []
+    for x in r:
+        if isinstance(x, UnitRange):
+            out.append((x.start, x.stop))
+        elif isinstance(x, range):
+            if x.step != 1:
+                raise ArgumentError("ranges must have unit step")
+            out.append((x.start, x.stop - 1))
+        else:
+            a, b = x
+            out.append((int(a), int(b)))
+    if not 1 <= len(out) <= 3:
+        raise ArgumentError(f"ranges must be a Tuple of UnitRange of size 1, 2 or 3 (obtained: {r}).")
+    while len(out) < 3:
+        out.append((1, 1))
+    return out
+
+
+class UnitRange:
+    """Julia's start:stop (inclusive, 1-based)."""
+    __slots__ = ("start", "stop")
+
+    def __init__(self, start, stop):
+        self.start, self.stop = int(start), int(stop)
+
+    def __len__(self):
+        return max(0, self.stop - self.start + 1)
+
+    def __eq__(self, o):
+        return isinstance(o, UnitRange) and (self.start, self.stop) == (o.start, o.stop)
+
+    def __repr__(self):
+        return f"{self.start}:{self.stop}"
+
+
+def _triple(x, name):
+    if x is None:
+        return None
+    x = tuple(int(v) for v in (x if isinstance(x, (tuple, list)) else (x,)))
+    if not 1 <= len(x) <= 3:
+        raise ArgumentError(f"{name} must be an Array or Tuple of size 3 (obtained: {x}).")
+    return x + (1,) * (3 - len(x))
+
+
+class Geometry(tuple):
+    """(memopt, checked, nthreads, nblocks, ranges) of a launch; `checked` is True when the host could
+    not prove every access in bounds and the bounds-checked code runs."""
+    __slots__ = ()
+
+    def __new__(cls, memopt, checked, nthreads, nblocks, ranges):
+        return tuple.__new__(cls, (memopt, checked, nthreads, nblocks, ranges))
+
+    memopt = property(lambda s: s[0])
+    checked = property(lambda s: s[1])
+    nthreads = property(lambda s: s[2])
+    nblocks = property(lambda s: s[3])
+    ranges = property(lambda s: s[4])
+
+
+class Kernel:
+    """A kernel defined with @parallel / @parallel_indices; calling it is `@parallel kernel(args...)`."""
+
+    def __init__(self, text, opts):
+        self._h = ctypes.c_void_p()
+        self.opts = opts
+        self.text = text
+        check(lib.ps_kernel_define(text.encode(), json.dumps(opts).encode(), ctypes.byref(self._h)))
+        buf = ctypes.create_string_buffer(1 << 20)
+        check(lib.ps_kernel_info(self._h, buf, len(buf)))
+        self.info = json.loads(buf.value.decode())
+        self.name = self.info["name"]
+        self.params = [p["name"] for p in self.info["params"]]
+        self._annots = [p["annot"] for p in self.info["params"]]
+        self.numbertype = self.info["numbertype"]
+
+    def __del__(self):
+        try:
+            lib.ps_kernel_free(self._h)
+        except Exception:
+            pass
+
+    def __repr__(self):
+        return f"Kernel({self.name}, {self.info['kind']}, params={self.params})"
+
+    @property
+    def metadata(self):
+        return self.info
+
+    def _scalar_code(self, i, v):
+        ann = self._annots[i]
+        if isinstance(v, (bool, np.bool_)):
+            return PS_BOOL
+        if ann.endswith("Number") or ann.endswith("Data.Number"):
+            return dtype_code(self.numbertype)
+        if ann in ("Float64", "Float32", "Int32", "Int64", "Int", "Bool"):
+            return dtype_code(ann)
+        if ann in ("Integer",):
+            return PS_I64
+        if isinstance(v, np.generic):
+            return dtype_code(v.dtype)
+        if isinstance(v, (int, np.integer)):
+            return PS_I64
+        if isinstance(v, (float, np.floating)):
+            return PS_F64
+        raise ArgumentError(f"argument {self.params[i]}: unsupported type {type(v).__name__}")
+
+    def _args(self, args):
+        if len(args) != len(self.params):
+            raise ArgumentError(f"MethodError: kernel {self.name} takes {len(self.params)} arguments ({len(args)} given)")
+        arr = (ps_arg * max(1, len(args)))()
+        for i, v in enumerate(args):
+            a = arr[i]
+            if isinstance(v, (DeviceArray, ArgSpec)):
+                a.kind = PS_ARG_FIELD
+                a.field = v.f
+                a.dtype = v.f.dtype
+            else:
+                a.kind = PS_ARG_SCALAR
+                code = self._scalar_code(i, v)
+                a.dtype = code
+                if code in (PS_F64, PS_F32):
+                    a.f = float(v)
+                else:
+                    a.i = int(v)
+        return arr
+
+    def _launch_struct(self, ranges, nblocks, nthreads, stream, async_, memopt):
+        L = ps_launch()
+        if ranges is not None:
+            for d, (a, b) in enumerate(_range3(ranges)):
+                L.ranges[d][0] = a
+                L.ranges[d][1] = b
+            L.has_ranges = 1
+        nb, nt = _triple(nblocks, "nblocks"), _triple(nthreads, "nthreads")
+        if (nb is None) != (nt is None):
+            raise ArgumentError("nblocks and nthreads must be given together")
+        if nb:
+            for d in range(3):
+                L.nblocks[d], L.nthreads[d] = nb[d], nt[d]
+        L.stream = _stream_handle(stream)
+        L.async_ = 1 if async_ else 0
+        L.memopt = -1 if memopt is None else int(bool(memopt))
+        return L
+
+    def __call__(self, *args, ranges=None, nblocks=None, nthreads=None, stream=None, async_=False, memopt=None):
+        ensure_device()
+        arr = self._args(args)
+        L = self._launch_struct(ranges, nblocks, nthreads, stream, async_, memopt)
+        check(lib.ps_kernel_launch(self._h, arr, len(args), ctypes.byref(L)))
+
+    def geometry(self, *args, ranges=None, nblocks=None, nthreads=None, memopt=None):
+        """(memopt, nthreads, nblocks, ranges) the launch would use."""
+        arr = self._args(args)
+        L = self._launch_struct(ranges, nblocks, nthreads, None, True, memopt)
+        nt = (ctypes.c_int32 * 3)()
+        nb = (ctypes.c_int32 * 3)()
+        rg = ((ctypes.c_int64 * 2) * 3)()
+        lds = ctypes.c_int64()
+        m = check(lib.ps_kernel_geometry(self._h, arr, len(args), ctypes.byref(L), nt, nb,
+                                         ctypes.cast(rg, ctypes.POINTER(ctypes.c_int64)), ctypes.byref(lds)))
+        return Geometry(bool(m & 1), bool(m & 2), tuple(nt), tuple(nb), tuple((rg[d][0], rg[d][1]) for d in range(3)))
+
+    def source(self, *args, memopt=None, checked=False, idx64=False):
+        """The generated HIP source for these argument kinds."""
+        arr = self._args(args)
+        mo = self.info["memopt"] if memopt is None else memopt
+        flags = (1 if mo else 0) | (2 if checked else 0) | (4 if idx64 else 0)
+        buf = ctypes.create_string_buffer(1 << 22)
+        check(lib.ps_kernel_source(self._h, arr, len(args), flags, buf, len(buf)))
+        return buf.value.decode()
+
+    def precompile(self, *args, memopt=None, checked=False, idx64=False, maxthreads=256):
+        """Compiles the code object for these argument kinds into the disk cache (no GPU needed)."""
+        arr = self._args(args)
+        mo = self.info["memopt"] if memopt is None else memopt
+        flags = (1 if mo else 0) | (2 if checked else 0) | (4 if idx64 else 0) | (16 if maxthreads > 256 else 0)
+        check(lib.ps_kernel_precompile(self._h, arr, len(args), flags))
+
+
+_KW_DEF = ("ndims", "N", "inbounds", "padding", "memopt", "optvars", "loopdim", "loopsize", "optranges", "useshmemhalos",
+           "optimize_halo_read", "fp_contract", "lds", "xcd_remap", "nthreads_max_memopt")
+
+
+def _define(kind, text, kw, mod, indices=None):
+    bad = set(kw) - set(_KW_DEF)
+    if bad:
+        raise KeywordArgumentError(f"invalid keyword argument(s) in @{kind} <kernel>: {', '.join(sorted(bad))}")
+    cfg = _config(mod)
+    opts = {"kind": kind, "ndims": kw.get("ndims", cfg.ndims), "numbertype": cfg.numbertype,
+            "inbounds": kw.get("inbounds", cfg.inbounds), "padding": kw.get("padding", cfg.padding),
+            "memopt": kw.get("memopt", cfg.memopt)}
+    if indices is not None:
+        opts["indices"] = list(indices)
+    for k in ("optvars", "loopdim", "loopsize", "fp_contract", "lds", "xcd_remap", "nthreads_max_memopt"):
+        if k in kw:
+            v = kw[k]
+            opts[k] = list(v) if k == "optvars" and isinstance(v, (tuple, list)) else v
+    if "optranges" in kw and kw["optranges"] is not None:
+        opts["optranges"] = {k: [list(_range3((r,))[0]) if not isinstance(r, (tuple, list)) else list(r) for r in v]
+                             for k, v in kw["optranges"].items()}
+    return Kernel(text, opts)
+
+
+def _split_call(args):
+    """(ranges, nblocks, nthreads, kernel, kernel_args) from the positional forms of @parallel."""
+    for i, a in enumerate(args):
+        if isinstance(a, Kernel):
+            pre, k, rest = args[:i], a, args[i + 1:]
+            break
+    else:
+        raise ArgumentError("@parallel: the last argument must be a kernel call")
+    ranges = nblocks = nthreads = None
+    if len(pre) == 1:
+        (ranges,) = pre
+    elif len(pre) == 2:
+        nblocks, nthreads = pre
+    elif len(pre) == 3:
+        ranges, nblocks, nthreads = pre
+    elif len(pre) > 3:
+        raise ArgumentError("too many positional arguments before the kernel call")
+    return ranges, nblocks, nthreads, k, rest
+
+
+def parallel(*args, **kw):
+    """@parallel: `parallel(text, **kw)` defines a kernel; `parallel([ranges,] [nblocks, nthreads,] kernel, args...)` runs it."""
+    if args and isinstance(args[0], str):
+        if len(args) != 1:
+            raise ArgumentError("@parallel <kernel>: the kernel text must be the only positional argument")
+        return _define("parallel", args[0], kw, _caller(1))
+    ranges, nblocks, nthreads, k, rest = _split_call(args)
+    ranges = kw.pop("ranges", ranges)
+    return k(*rest, ranges=ranges, nblocks=nblocks, nthreads=nthreads, **kw)
+
+
+def parallel_async(*args, **kw):
+    """@parallel_async: launch without synchronising the stream."""
+    ranges, nblocks, nthreads, k, rest = _split_call(args)
+    ranges = kw.pop("ranges", ranges)
+    return k(*rest, ranges=ranges, nblocks=nblocks, nthreads=nthreads, async_=True, **kw)
+
+
+def parallel_indices(indices_or_text, text=None, **kw):
+    """@parallel_indices (ix,iy,...) kernel."""
+    mod = _caller(1)
+    if text is None:
+        return _define("parallel_indices", indices_or_text, kw, mod)
+    idx = indices_or_text
+    if isinstance(idx, str):
+        idx = [s.strip() for s in idx.strip("() ").split(",") if s.strip()]
+    return _define("parallel_indices", text, kw, mod, indices=idx)
+
+
+# ---------------------------------------------------------------------------
+# @hide_communication
+# ---------------------------------------------------------------------------
+def _promote_bw(boundary_width):
+    """promote_boundary_width: an Int or 1-3-tuple -> (bx, by, bz) padded with 0."""
+    bwt = tuple(boundary_width) if isinstance(boundary_width, (tuple, list)) else (boundary_width,)
+    if not 1 <= len(bwt) <= 3 or not all(isinstance(b, (int, np.integer)) for b in bwt):
+        raise ArgumentError(f"@hide_communication: boundary_width must be a Tuple of Integer of size 1, 2 or 3 (obtained: {boundary_width}).")
+    return tuple(int(bwt[i]) if i < len(bwt) else 0 for i in range(3))
+
+
+def _c_ranges(r):
+    return ((ctypes.c_int64 * 2) * 3)(*[(ctypes.c_int64 * 2)(a, b) for a, b in r])
+
+
+def get_ranges_outer(boundary_width, ranges):
+    """get_ranges_outer: the boundary slabs computed first (src/ParallelKernel/hide_communication.jl:350-381)."""
+    rr = _c_ranges(_range3(ranges))
+    out = (((ctypes.c_int64 * 2) * 3) * 6)()
+    n = ctypes.c_int()
+    check(lib.ps_ranges_outer(ctypes.cast(rr, ctypes.POINTER(ctypes.c_int64)), (ctypes.c_int64 * 3)(*_promote_bw(boundary_width)),
+                              ctypes.cast(out, ctypes.POINTER(ctypes.c_int64)), ctypes.byref(n)))
+    return tuple(tuple((out[i][d][0], out[i][d][1]) for d in range(3)) for i in range(n.value))
+
+
+def get_ranges_inner(boundary_width, ranges):
+    """get_ranges_inner (src/ParallelKernel/hide_communication.jl:383-394)."""
+    rr = _c_ranges(_range3(ranges))
+    out = ((ctypes.c_int64 * 2) * 3)()
+    check(lib.ps_ranges_inner(ctypes.cast(rr, ctypes.POINTER(ctypes.c_int64)), (ctypes.c_int64 * 3)(*_promote_bw(boundary_width)),
+                              ctypes.cast(out, ctypes.POINTER(ctypes.c_int64))))
+    return (tuple((out[d][0], out[d][1]) for d in range(3)),)
+
+
+def _maxsize_ranges(args):
+    ms = [1, 1, 1]
+    for a in args:
+        if isinstance(a, DeviceArray):
+            for d in range(3):
+                ms[d] = max(ms[d], int(a.f.dims[d]))
+    return tuple((1, m) for m in ms)
+
+
+def hide_communication(boundary_width, computations, communication, ranges_inner=None, computation_calls=None):
+    """@hide_communication boundary_width begin <computation calls>; <boundary/communication code> end.
+
+    computations: a list of (kernel, args) or (kernel, args, kwargs) tuples (the first
+    `computation_calls` statements of the reference's block). communication: a callable
+    running the boundary-condition / communication code; its @parallel calls go to
+    priority stream 1. With `ranges_inner` given, `boundary_width` is the tuple of outer ranges
+    (the explicit form `@hide_communication ranges_outer ranges_inner block`).
+    """
+    ensure_device()
+    if callable(computations) or isinstance(computations, tuple) and computations and isinstance(computations[0], Kernel):
+        computations = [computations]
+    calls = []
+    for c in computations:
+        k, a = c[0], c[1]
+        kw = dict(c[2]) if len(c) > 2 else {}
+        if "stream" in kw:
+            raise ArgumentError("@hide_communication: the keyword argument 'stream' is not allowed in computation calls")
+        if "ranges" in kw:
+            raise ArgumentError("@hide_communication: computation calls must not be given explicit ranges")
+        calls.append((k, tuple(a), kw))
+    if computation_calls is not None and computation_calls < 1:
+        raise KeywordArgumentError("Invalid keyword argument in @hide_communication: computation_calls must be >= 1.")
+    if ranges_inner is None:
+        rngs = [_maxsize_ranges(a) for _, a, _ in calls]
+        if any(r != rngs[0] for r in rngs):
+            raise ArgumentError(f"@hide_communication: the ranges of the computation calls are not all equal (obtained: {rngs}).")
+        outer = get_ranges_outer(boundary_width, rngs[0])
+        inner = get_ranges_inner(boundary_width, rngs[0])
+    else:
+        outer, inner = boundary_width, ranges_inner
+    for i, r in enumerate(outer):
+        for k, a, kw in calls:
+            k(*a, ranges=r, stream=get_priority_stream(i + 1), async_=True, **kw)
+    for i, r in enumerate(inner):
+        for k, a, kw in calls:
+            k(*a, ranges=r, stream=get_stream(i + 1), async_=True, **kw)
+    for i in range(len(outer)):
+        get_priority_stream(i + 1).synchronize()
+    with stream_context(get_priority_stream(1)):
+        communication()
+    get_priority_stream(1).synchronize()
+    for i in range(len(inner)):
+        get_stream(i + 1).synchronize()
+
+
+# ---------------------------------------------------------------------------
+# launch-geometry helpers (host, mirror src/ParallelKernel/parallel.jl:773-819)
+# ---------------------------------------------------------------------------
+def compute_nthreads(maxsize, nthreads_x_max=64, nthreads_max=256, flatdim=0):
+    ms = _triple(maxsize, "maxsize")
+    out = (ctypes.c_int32 * 3)()
+    check(lib.ps_compute_nthreads((ctypes.c_int64 * 3)(*ms), nthreads_x_max, nthreads_max, flatdim, out))
+    return tuple(out)
+
+
+def compute_nblocks(maxsize, nthreads):
+    ms = _triple(maxsize, "maxsize")
+    nt = _triple(nthreads, "nthreads")
+    out = (ctypes.c_int32 * 3)()
+    check(lib.ps_compute_nblocks((ctypes.c_int64 * 3)(*ms), (ctypes.c_int32 * 3)(*nt), out))
+    return tuple(out)
+
+
+def get_ranges(*args):
+    """get_ranges: 1:maxsize over the array arguments."""
+    return tuple(UnitRange(a, b) for a, b in _maxsize_ranges(args))

@@ -1,0 +1,288 @@
+"""Pins the CPU oracle before it is trusted as the parity checker.
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped term,
+a wrong sign or offset, a transposed operand) fails it: closed forms, invariants,
+brute-force single-point evaluation and the reference's own known-answer tests.
+"""
+import numpy as np
+import pytest
+
+from oracle import ps_oracle as O
+
+
+def F(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------ FD macro KATs
+@pytest.mark.parametrize("name", sorted(O.FD3_ARG_OFFSET))
+def test_fd3_shapes_match_reference_field_types(name):
+    # R is an IField of nxyz=(9,7,8) in test_FiniteDifferences3D.jl:58; each macro's argument is the
+    # field type whose size minus the macro's footprint gives R's size.
+    R = (7, 5, 6)
+    off = O.FD3_ARG_OFFSET[name]
+    A = np.random.default_rng(1).random(tuple(R[i] + off[i] for i in range(3))) + 0.5
+    assert O.fd3(name, F(A)).shape == R
+
+
+def test_fd3_closed_forms():
+    nx, ny, nz = 9, 7, 8
+    ix, iy, iz = np.meshgrid(np.arange(1, nx + 1), np.arange(1, ny + 1), np.arange(1, nz + 1), indexing="ij")
+    lin = F(3.0 * ix - 2.0 * iy + 5.0 * iz)
+    assert np.all(O.fd3("d_xa", lin) == 3.0)
+    assert np.all(O.fd3("d_ya", lin) == -2.0)
+    assert np.all(O.fd3("d_za", lin) == 5.0)
+    assert np.all(O.fd3("d_xi", lin) == 3.0) and O.fd3("d_xi", lin).shape == (8, 5, 6)
+    assert np.all(O.fd3("d_zi", lin) == 5.0) and O.fd3("d_zi", lin).shape == (7, 5, 7)
+    quad = F(1.0 * ix ** 2 + 3.0 * iy ** 2 - 2.0 * iz ** 2)
+    assert np.all(O.fd3("d2_xi", quad) == 2.0)
+    assert np.all(O.fd3("d2_yi", quad) == 6.0)
+    assert np.all(O.fd3("d2_zi", quad) == -4.0)
+    # averages of a linear field are its value at the staggered point
+    assert np.all(O.fd3("av_xa", lin) == (3.0 * (np.arange(1, nx) + 0.5))[:, None, None] - 2.0 * iy[:-1, :, :] + 5.0 * iz[:-1, :, :])
+    c = F(np.full((nx, ny, nz), 4.0))
+    for n in ("av", "av_xya", "av_yzi", "harm", "harm_xza", "harm_zi"):
+        assert np.all(O.fd3(n, c) == 4.0), n
+    # harmonic mean of (1, 3) along x is 1.5
+    a = F(np.where(ix % 2 == 1, 1.0, 3.0))
+    assert np.all(O.fd3("harm_xa", a) == 1.5)
+    # maxloc/minloc of a single peak / pit
+    p = np.zeros((nx, ny, nz))
+    p[4, 3, 4] = 7.0
+    mx = O.fd3("maxloc", F(p))
+    assert mx[3, 2, 3] == 7.0 and mx[2, 2, 3] == 7.0 and mx[3, 2, 2] == 7.0 and mx[2, 1, 2] == 0.0
+    assert np.all(O.fd3("minloc", F(-p)) == -mx)
+    # selection: inner
+    assert np.array_equal(O.fd3("inn", lin), lin[1:-1, 1:-1, 1:-1])
+    assert np.array_equal(O.fd3("inn_yz", lin), lin[:, 1:-1, 1:-1])
+
+
+def test_fd2_fd1_closed_forms():
+    nx, ny = 9, 7
+    ix, iy = np.meshgrid(np.arange(1, nx + 1), np.arange(1, ny + 1), indexing="ij")
+    lin = F(3.0 * ix - 2.0 * iy)
+    assert np.all(O.fd2("d_xa", lin) == 3.0) and np.all(O.fd2("d_yi", lin) == -2.0)
+    quad = F(1.0 * ix ** 2 - 3.0 * iy ** 2)
+    assert np.all(O.fd2("d2_xa", quad) == 2.0) and np.all(O.fd2("d2_yi", quad) == -6.0)
+    assert np.all(O.fd2("av", F(np.full((nx, ny), 2.5))) == 2.5)
+    assert np.all(O.fd2("harm_yi", F(np.full((nx, ny), 2.5))) == 2.5)
+    x = F(np.arange(1, 10, dtype=np.float64) ** 2)
+    assert np.all(O.fd1("d2", x) == 2.0)
+    assert np.all(O.fd1("d", F(np.arange(5.0))) == 1.0)
+    assert np.all(O.fd1("av", F(np.arange(5.0))) == np.arange(4) + 0.5)
+
+
+# ------------------------------------------------------------ diffusion
+def test_otf_diffusion_matches_reference_kat():
+    # test/test_parallel.jl:411-434: integer-cubed T, unit constants, Ci = 1, exact ==.
+    nx, ny, nz = 32, 8, 8
+    T = O.cubed_indices((nx, ny, nz))
+    Ci = F(np.ones((nx, ny, nz)))
+    T2 = F(np.zeros((nx, ny, nz)))
+    O.diffusion3d_otf_test(T2, T, Ci, 1.0, 1.0, 1.0, 1.0, 1.0)
+    ref = np.zeros_like(T)
+    c = T[1:-1, 1:-1, 1:-1]
+    ref[1:-1, 1:-1, 1:-1] = c + 1.0 * (1.0 * Ci[1:-1, 1:-1, 1:-1] * (
+        ((T[2:, 1:-1, 1:-1] - c) - (c - T[:-2, 1:-1, 1:-1])) * 1.0
+        + ((T[1:-1, 2:, 1:-1] - c) - (c - T[1:-1, :-2, 1:-1])) * 1.0
+        + ((T[1:-1, 1:-1, 2:] - c) - (c - T[1:-1, 1:-1, :-2])) * 1.0))
+    assert np.array_equal(T2, ref)
+
+
+def test_diffusion_linear_field_is_stationary():
+    n = 12
+    x = np.arange(1, n + 1, dtype=np.float64)
+    T = F(np.broadcast_to((2.0 * x - 7.0)[:, None, None], (n, n, n)))
+    Cp = F(1.0 + np.random.default_rng(3).random((n, n, n)))
+    for f in (O.diffusion3d_flux, O.diffusion3d_d2):
+        T2 = F(np.full((n, n, n), -1.0))
+        f(T2, T, Cp, 1.0, 0.01, 0.5, 0.5, 0.5)
+        assert np.array_equal(T2[1:-1, 1:-1, 1:-1], T[1:-1, 1:-1, 1:-1])
+        assert np.all(T2[0] == -1.0) and np.all(T2[:, -1] == -1.0)  # boundaries untouched
+
+
+def test_diffusion_flux_single_point_and_d2_agreement():
+    rng = np.random.default_rng(7)
+    n = 10
+    T = F(rng.random((n, n, n)))
+    Cp = F(1.0 + rng.random((n, n, n)))
+    lam, dt, dx, dy, dz = 1.3, 0.01, 0.5, 0.6, 0.7
+    T2 = F(np.zeros_like(T))
+    O.diffusion3d_flux(T2, T, Cp, lam, dt, dx, dy, dz)
+    i, j, k = 4, 6, 3  # 0-based interior point
+    qxp = -lam * (T[i + 1, j, k] - T[i, j, k]) / dx
+    qxm = -lam * (T[i, j, k] - T[i - 1, j, k]) / dx
+    qyp = -lam * (T[i, j + 1, k] - T[i, j, k]) / dy
+    qym = -lam * (T[i, j, k] - T[i, j - 1, k]) / dy
+    qzp = -lam * (T[i, j, k + 1] - T[i, j, k]) / dz
+    qzm = -lam * (T[i, j, k] - T[i, j, k - 1]) / dz
+    ref = T[i, j, k] + dt / Cp[i, j, k] * (-(qxp - qxm) / dx - (qyp - qym) / dy - (qzp - qzm) / dz)
+    assert abs(T2[i, j, k] - ref) <= 1e-15 * abs(ref)
+    # same physics in the d2 form with Ci = 1/Cp
+    T3 = F(np.zeros_like(T))
+    O.diffusion3d_d2(T3, T, F(1.0 / Cp), lam, dt, dx, dy, dz)
+    assert np.allclose(T3[1:-1, 1:-1, 1:-1], T2[1:-1, 1:-1, 1:-1], rtol=1e-13, atol=0)
+
+
+def test_diffusion_conserves_heat_with_no_flux():
+    # sum(Cp*T) changes only through boundary fluxes: a field constant on the boundary
+    # layers has zero boundary flux in the first step -> interior heat is conserved.
+    rng = np.random.default_rng(11)
+    n = 12
+    T = np.full((n, n, n), 2.0)
+    T[2:-2, 2:-2, 2:-2] += rng.random((n - 4, n - 4, n - 4))
+    T = F(T)
+    Cp = F(1.0 + rng.random((n, n, n)))
+    T2 = F(T.copy())
+    O.diffusion3d_flux(T2, T, Cp, 1.0, 1e-3, 1.0, 1.0, 1.0)
+    s0 = np.sum((Cp * T)[1:-1, 1:-1, 1:-1])
+    s1 = np.sum((Cp * T2)[1:-1, 1:-1, 1:-1])
+    assert abs(s1 - s0) < 1e-11 * abs(s0)
+
+
+# ------------------------------------------------------------ acoustic
+def _acoustic3d_state(n, rng):
+    P = F(rng.random((n, n, n)))
+    Vx = F(np.zeros((n + 1, n, n)))
+    Vy = F(np.zeros((n, n + 1, n)))
+    Vz = F(np.zeros((n, n, n + 1)))
+    return P, Vx, Vy, Vz
+
+
+def test_acoustic3d_pressure_is_conserved():
+    rng = np.random.default_rng(5)
+    n = 10
+    P, Vx, Vy, Vz = _acoustic3d_state(n, rng)
+    s0 = P.sum()
+    for _ in range(5):
+        O.acoustic3d_v(Vx, Vy, Vz, P, 0.1, 1.0, 1.0, 1.0, 1.0)
+        O.acoustic3d_p(P, Vx, Vy, Vz, 0.1, 1.0, 1.0, 1.0, 1.0)
+    # velocities on the outer faces are never updated (masks) -> telescoping divergence
+    assert np.all(Vx[0] == 0) and np.all(Vx[-1] == 0) and np.all(Vy[:, 0] == 0) and np.all(Vz[:, :, -1] == 0)
+    assert abs(P.sum() - s0) < 1e-12 * abs(s0)
+
+
+def test_acoustic3d_single_point():
+    rng = np.random.default_rng(6)
+    n = 8
+    P, Vx, Vy, Vz = _acoustic3d_state(n, rng)
+    Vx[:] = rng.random(Vx.shape)
+    Vy[:] = rng.random(Vy.shape)
+    Vz[:] = rng.random(Vz.shape)
+    dt, rho, k, dx, dy, dz = 0.1, 1.2, 0.9, 0.5, 0.6, 0.7
+    Vx0, Vy0, Vz0, P0 = Vx.copy(), Vy.copy(), Vz.copy(), P.copy()
+    O.acoustic3d_v(Vx, Vy, Vz, P, dt, rho, dx, dy, dz)
+    # Vx[ix+1,iy+1,iz+1] updated for ix in 1..n-1: 0-based i in 1..n-1
+    i, j, k3 = 3, 2, 5
+    assert Vx[i, j, k3] == Vx0[i, j, k3] - dt / rho * (P[i, j, k3] - P[i - 1, j, k3]) / dx
+    assert Vy[i, j, k3] == Vy0[i, j, k3] - dt / rho * (P[i, j, k3] - P[i, j - 1, k3]) / dy
+    assert Vz[i, j, k3] == Vz0[i, j, k3] - dt / rho * (P[i, j, k3] - P[i, j, k3 - 1]) / dz
+    assert Vx[n, j, k3] == Vx0[n, j, k3]  # last face untouched
+    O.acoustic3d_p(P, Vx, Vy, Vz, dt, k, dx, dy, dz)
+    ref = P0[i, j, k3] - dt * k * ((Vx[i + 1, j, k3] - Vx[i, j, k3]) / dx + (Vy[i, j + 1, k3] - Vy[i, j, k3]) / dy
+                                    + (Vz[i, j, k3 + 1] - Vz[i, j, k3]) / dz)
+    assert P[i, j, k3] == ref
+
+
+def test_acoustic_precisions_track_f64():
+    rng = np.random.default_rng(9)
+    n = 12
+    x = (np.arange(n) - n / 2) / 2
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    P0 = np.exp(-(X ** 2 + Y ** 2 + Z ** 2))
+    res = {}
+    for prec, dt_ in (("f64", np.float64), ("f32", np.float32), ("bf16", np.uint16)):
+        if prec == "bf16":
+            from parallelstencil.core import f32_to_bf16, bf16_to_f32  # host conversion helpers only
+            P = np.asfortranarray(f32_to_bf16(P0.astype(np.float32)))
+            Vs = [np.asfortranarray(np.zeros(s, dtype=np.uint16)) for s in ((n + 1, n, n), (n, n + 1, n), (n, n, n + 1))]
+        else:
+            P = np.asfortranarray(P0.astype(dt_))
+            Vs = [np.asfortranarray(np.zeros(s, dtype=dt_)) for s in ((n + 1, n, n), (n, n + 1, n), (n, n, n + 1))]
+        for _ in range(10):
+            O.acoustic3d_v(*Vs, P, 0.08, 1.0, 0.5, 0.5, 0.5, prec=prec)
+            O.acoustic3d_p(P, *Vs, 0.08, 1.0, 0.5, 0.5, 0.5, prec=prec)
+        res[prec] = bf16_to_f32(P).astype(np.float64) if prec == "bf16" else P.astype(np.float64)
+    e32 = np.abs(res["f32"] - res["f64"]).max()
+    e16 = np.abs(res["bf16"] - res["f64"]).max()
+    assert 0 < e32 < 1e-5 and e32 < e16 < 5e-2
+
+
+def test_acoustic2d_pressure_is_conserved():
+    rng = np.random.default_rng(12)
+    n = 16
+    P = F(rng.random((n, n)))
+    Vx, Vy = F(np.zeros((n + 1, n))), F(np.zeros((n, n + 1)))
+    s0 = P.sum()
+    for _ in range(4):
+        O.acoustic2d_v(Vx, Vy, P, 0.1, 1.0, 1.0, 1.0)
+        O.acoustic2d_p(P, Vx, Vy, 0.1, 1.0, 1.0, 1.0)
+    assert abs(P.sum() - s0) < 1e-12 * abs(s0)
+
+
+# ------------------------------------------------------------ Stokes
+def _stokes_arrays(nx, ny):
+    z = lambda *s: F(np.zeros(s))  # noqa: E731
+    return dict(Pt=z(nx, ny), dtPt=z(nx, ny), divV=z(nx, ny), Vx=z(nx + 1, ny), Vy=z(nx, ny + 1), txx=z(nx, ny),
+                tyy=z(nx, ny), txy=z(nx - 1, ny - 1), Rx=z(nx - 1, ny - 2), Ry=z(nx - 2, ny - 1), dVxdt=z(nx - 1, ny - 2),
+                dVydt=z(nx - 2, ny - 1), dtVx=z(nx - 1, ny - 2), dtVy=z(nx - 2, ny - 1))
+
+
+def test_stokes_uniform_velocity_has_no_strain():
+    nx, ny = 12, 10
+    a = _stokes_arrays(nx, ny)
+    a["Vx"][:] = 0.7
+    a["Vy"][:] = -0.3
+    Mus = F(1.0 + np.random.default_rng(2).random((nx, ny)))
+    O.stokes2d_p(a["divV"], a["Pt"], a["Vx"], a["Vy"], a["dtPt"], 0.1, 0.1)
+    O.stokes2d_tau(a["divV"], a["txx"], a["tyy"], a["txy"], a["Vx"], a["Vy"], Mus, 0.1, 0.1)
+    assert np.all(a["divV"] == 0) and np.all(a["txx"] == 0) and np.all(a["txy"] == 0)
+
+
+def test_stokes_single_points():
+    rng = np.random.default_rng(4)
+    nx, ny = 9, 8
+    a = _stokes_arrays(nx, ny)
+    for k in a:
+        a[k][:] = rng.random(a[k].shape)
+    Mus = F(1.0 + rng.random((nx, ny)))
+    Rog = F(rng.random((nx, ny)))
+    dx, dy = 0.3, 0.4
+    b = {k: v.copy() for k, v in a.items()}
+    O.stokes2d_timesteps(a["dtVx"], a["dtVy"], a["dtPt"], Mus, 1.0, 0.25, min(dx, dy) ** 2, max(nx, ny))
+    i, j = 2, 3
+    assert a["dtVx"][i, j] == 1.0 * min(dx, dy) ** 2 / ((Mus[i, j + 1] + Mus[i + 1, j + 1]) * 0.5) / 4.1
+    assert a["dtPt"][i, j] == 0.25 * 4.1 * Mus[i, j] / max(nx, ny)
+    O.stokes2d_p(a["divV"], a["Pt"], a["Vx"], a["Vy"], a["dtPt"], dx, dy)
+    dv = (a["Vx"][i + 1, j] - a["Vx"][i, j]) / dx + (a["Vy"][i, j + 1] - a["Vy"][i, j]) / dy
+    assert a["divV"][i, j] == dv and a["Pt"][i, j] == b["Pt"][i, j] - a["dtPt"][i, j] * dv
+    O.stokes2d_tau(a["divV"], a["txx"], a["tyy"], a["txy"], a["Vx"], a["Vy"], Mus, dx, dy)
+    assert a["txx"][i, j] == 2.0 * Mus[i, j] * ((a["Vx"][i + 1, j] - a["Vx"][i, j]) / dx - 1.0 / 3.0 * a["divV"][i, j])
+    av = (Mus[i, j] + Mus[i + 1, j] + Mus[i, j + 1] + Mus[i + 1, j + 1]) * 0.25
+    assert a["txy"][i, j] == 2.0 * av * (0.5 * ((a["Vx"][i + 1, j + 1] - a["Vx"][i + 1, j]) / dy
+                                                 + (a["Vy"][i + 1, j + 1] - a["Vy"][i, j + 1]) / dx))
+    O.stokes2d_dv(a["Rx"], a["Ry"], a["dVxdt"], a["dVydt"], a["Pt"], Rog, a["txx"], a["tyy"], a["txy"], 0.9, 0.8, dx, dy)
+    rx = (a["txx"][i + 1, j + 1] - a["txx"][i, j + 1]) / dx + (a["txy"][i, j + 1] - a["txy"][i, j]) / dy \
+        - (a["Pt"][i + 1, j + 1] - a["Pt"][i, j + 1]) / dx
+    assert a["Rx"][i, j] == rx and a["dVxdt"][i, j] == 0.9 * b["dVxdt"][i, j] + rx
+    ry = (a["tyy"][i + 1, j + 1] - a["tyy"][i + 1, j]) / dy + (a["txy"][i + 1, j] - a["txy"][i, j]) / dx \
+        - (a["Pt"][i + 1, j + 1] - a["Pt"][i + 1, j]) / dy + (Rog[i + 1, j] + Rog[i + 1, j + 1]) * 0.5
+    assert a["Ry"][i, j] == ry
+    O.stokes2d_v(a["Vx"], a["Vy"], a["dVxdt"], a["dVydt"], a["dtVx"], a["dtVy"])
+    assert a["Vx"][i + 1, j + 1] == b["Vx"][i + 1, j + 1] + a["dtVx"][i, j] * a["dVxdt"][i, j]
+    assert a["Vx"][0, j] == b["Vx"][0, j]
+
+
+def test_bc_kernels():
+    A = O.linear_indices((5, 4))[:, :, 0].copy(order="F")
+    B = A.copy(order="F")
+    O.bc_x(B)
+    assert np.array_equal(B[0], A[1]) and np.array_equal(B[-1], A[-2]) and np.array_equal(B[1:-1], A[1:-1])
+    C = A.copy(order="F")
+    O.bc_y(C)
+    assert np.array_equal(C[:, 0], A[:, 1]) and np.array_equal(C[:, -1], A[:, -2])
+
+
+def test_mean_abs():
+    a = F(np.array([[1.0, -2.0], [3.0, -4.0]]))
+    assert O.mean_abs(a) == 2.5
