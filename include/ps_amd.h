@@ -170,6 +170,9 @@ int ps_kernel_source(ps_kernel k, const ps_arg* args, int nargs, int flags, char
 /* Compiles (or loads from the cache) the code object for the given argument kinds
  * without touching a GPU. Used to fill the kernel cache ahead of time. */
 int ps_kernel_precompile(ps_kernel k, const ps_arg* args, int nargs, int flags);
+/* Same, for exactly the variant that ps_kernel_launch would pick for these arguments (field
+ * pointers may be NULL) and launch parameters: geometry, bounds proof, LDS tiling. */
+int ps_kernel_precompile_launch(ps_kernel k, const ps_arg* args, int nargs, const ps_launch* launch);
 /* Launches the kernel. Replaces parallel_call_gpu_or_xpu / create_gpu_or_xpu_call /
  * AMDGPU.@roc (src/ParallelKernel/parallel.jl:303-334, 883-927) and, for memopt
  * kernels, parallel_call_memopt (src/parallel.jl:451-476). Out-of-bounds accesses

@@ -124,6 +124,29 @@ PS_DI float  ps_p3(float x)  { return x * x * x; }
 PS_DI double ps_pm2(double x) { double i = 1.0 / x; return i * i; }
 PS_DI float  ps_pm2(float x)  { float i = 1.0f / x; return i * i; }
 PS_DI ps_int ps_ipow(ps_int b, ps_int e) { ps_int r = 1; while (e > 0) { if (e & 1) r *= b; b *= b; e >>= 1; } return r; }
+// Correctly rounded a/b for a positive divisor b that is uniform over the launch (a scalar
+// argument or a literal; a negative divisor is handled as (-a)/(-b)), from r = RN(1/b):
+// q0 = RN(a*r), e = q0*b - a (exact by fma), q = RN(q0 - e*r)
+// (P. Markstein, IBM J. Res. Dev. 34(1), 1990). The host accepts b only in [2^-400, 2^400] with a
+// significand that is not all ones; results outside [2^-500, 2^500] (other than exact zeros,
+// whose sign this form gets right), infinities and NaN set `bad`, and the statement is then
+// recomputed with IEEE division.
+PS_DI double ps_divp(double a, double b, double r, bool& bad) {
+  double q0 = a * r;
+  double e = __builtin_fma(q0, b, -a);
+  double q = __builtin_fma(-e, r, q0);
+  double aq = __builtin_fabs(q);
+  bad |= !(aq <= 0x1p500) | ((aq < 0x1p-500) & (aq != 0.0));
+  return q;
+}
+PS_DI float ps_divp(float a, float b, float r, bool& bad) {
+  float q0 = a * r;
+  float e = __builtin_fmaf(q0, b, -a);
+  float q = __builtin_fmaf(-e, r, q0);
+  float aq = __builtin_fabsf(q);
+  bad |= !(aq <= 0x1p50f) | ((aq < 0x1p-50f) & (aq != 0.0f));
+  return q;
+}
 PS_DI bool ps_chk(ps_int i, ps_int j, ps_int k, ps_int nx, ps_int ny, ps_int nz, int* err) {
   bool ok = i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz;
   if (!ok) *err = 1;
@@ -164,6 +187,14 @@ class Emitter {
   bool in_loop = false;  // inside the memopt stream loop
   // memopt register cache: key (arg, ox, oy, oz) -> register name
   std::map<std::tuple<int, int, int, int>, std::string> cache;
+  // scalar arguments used as uniform divisors: their reciprocal travels as a hidden argument
+  std::map<int, Ty> rcp_params;
+  bool fast_div = false;   // emitting the reciprocal form of a statement
+  bool used_fast = false;  // the current statement used it
+  int tmp_counter = 0;
+  int E_lds_bytes = 0;  // static LDS of the emitted kernel
+  EV divide(const NodeP& n);
+  EV rhs_value(const NodeP& rhs, bool* opened);
 
   std::string pad() const { return std::string(ind * 2, ' '); }
   std::string an(int i) const { return "a" + std::to_string(i); }
@@ -428,6 +459,57 @@ EV Emitter::binary(const std::string& op, EV a, EV b, const NodeP& rhsnode) {
   throw PsError(PS_EARGUMENT, "unsupported operator " + op);
 }
 
+bool rcp_safe64(double b) {
+  double ab = std::fabs(b);
+  if (!(ab >= 0x1p-400 && ab <= 0x1p400)) return false;
+  unsigned long long bits;
+  std::memcpy(&bits, &b, 8);
+  return (bits & ((1ULL << 52) - 1)) != ((1ULL << 52) - 1);
+}
+bool rcp_safe32(float b) {
+  float ab = std::fabs(b);
+  if (!(ab >= 0x1p-40f && ab <= 0x1p40f)) return false;
+  unsigned bits;
+  std::memcpy(&bits, &b, 4);
+  return (bits & 0x7fffffu) != 0x7fffffu;
+}
+
+// a / b. Uniform float divisors (scalar arguments the host accepted, literals) are divided
+// through their reciprocal (ps_divp) when the enclosing statement provides a fallback.
+EV Emitter::divide(const NodeP& n) {
+  EV a = expr(n->a[0]);
+  const NodeP& d = n->a[1];
+  EV b = expr(d);
+  if (a.t == Ty::Bool) a = cast(a, Ty::Int);
+  if (b.t == Ty::Bool) b = cast(b, Ty::Int);
+  Ty t = promote(a.t, b.t);
+  if (t == Ty::Int) t = Ty::F64;
+  if (fast_div && d->k == NK::Ident && !locals.count(d->s)) {
+    auto it = K.param_index.find(d->s);
+    int k = it == K.param_index.end() ? -1 : it->second;
+    if (k >= 0 && !kinds[k].is_field && value_ty(kinds[k].dtype) == t) {
+      char mode = (size_t)k < V.divmode.size() ? V.divmode[k] : 'x';
+      if (mode == 'p' || mode == 'n') {
+        rcp_params[k] = t;
+        used_fast = true;
+        std::string A = an(k);
+        if (mode == 'p') return EV{"ps_divp(" + cast(a, t).c + ", " + A + ", " + A + "_r, ps_bad)", t};
+        return EV{"ps_divp(-(" + cast(a, t).c + "), -" + A + ", " + A + "_r, ps_bad)", t};
+      }
+    }
+  }
+  if (fast_div && d->k == NK::Num && !d->is_int) {
+    double v = (d->is_f32 || NT == Ty::F32) ? (double)(float)d->num : d->num;
+    bool ok = t == Ty::F32 ? rcp_safe32((float)v) : rcp_safe64(v);
+    if (ok && v > 0) {
+      used_fast = true;
+      double r = t == Ty::F32 ? (double)(1.0f / (float)v) : 1.0 / v;
+      return EV{"ps_divp(" + cast(a, t).c + ", " + hexlit(v, t) + ", " + hexlit(r, t) + ", ps_bad)", t};
+    }
+  }
+  return binary("/", a, b, nullptr);
+}
+
 EV Emitter::cmp(const NodeP& n) {
   std::vector<EV> v;
   for (auto& x : n->a) v.push_back(expr(x));
@@ -582,6 +664,7 @@ EV Emitter::expr(const NodeP& n) {
       return EV{"(-" + a.c + ")", a.t};
     }
     case NK::Binary: {
+      if (n->s == "/") return divide(n);
       EV a = expr(n->a[0]);
       if (n->s == "^") {
         bool lit_exp = (n->a[1]->k == NK::Num && n->a[1]->is_int) ||
@@ -619,7 +702,8 @@ void Emitter::assign(const NodeP& s) {
     if (K.param_index.count(v)) throw PsError(PS_EARGUMENT, "cannot assign to kernel argument '" + v + "' (arguments are passed by value)");
     for (int d = 0; d < 3; ++d)
       if (v == kIdx[d]) throw PsError(PS_EARGUMENT, "cannot assign to a parallel index");
-    EV r = expr(rhs);
+    bool blk = false;
+    EV r = rhs_value(rhs, &blk);
     if (!locals.count(v)) {
       if (!pass1) throw PsError(PS_EINTERNAL, "local variable not hoisted: " + v);
       locals[v] = r.t;
@@ -627,12 +711,17 @@ void Emitter::assign(const NodeP& s) {
       local_order.push_back(v);
     }
     o << pad() << local_c[v] << " = " << cast(r, locals[v]).c << ";\n";
+    if (blk) {
+      --ind;
+      o << pad() << "}\n";
+    }
     return;
   }
   if (lhs->k != NK::Index || lhs->a[0]->k != NK::Ident) throw PsError(PS_EARGUMENT, "unsupported assignment target " + to_julia(lhs));
   int ai = field_index(lhs->a[0]->s);
   if (ai < 0) throw PsError(PS_EARGUMENT, "'" + lhs->a[0]->s + "' is assigned to but is not an array argument");
-  EV r = expr(rhs);
+  bool blk = false;
+  EV r = rhs_value(rhs, &blk);
   auto idx = norm_indices(lhs, ai);
   std::string ic[3];
   std::string off = offset(ai, idx, ic, nullptr);
@@ -648,8 +737,34 @@ void Emitter::assign(const NodeP& s) {
     o << pad() << "if (ps_chk(" << ic[0] << ", " << ic[1] << ", " << ic[2] << ", " << A << "_nx, " << A << "_ny, " << A
       << "_nz, ps_err)) " << A << "[" << off << "] = " << v << ";\n";
   } else {
-    o << pad() << A << "[" << off << "] = " << v << ";\n";
+    if (K.opt.nt_store) o << pad() << "__builtin_nontemporal_store(" << v << ", &" << A << "[" << off << "]);\n";
+    else o << pad() << A << "[" << off << "] = " << v << ";\n";
   }
+  if (blk) {
+    --ind;
+    o << pad() << "}\n";
+  }
+}
+
+// Value of a right-hand side. When it divides by uniform divisors, the fast (reciprocal) form
+// is evaluated first and the IEEE form only if a quotient left the exact range; the emitted
+// code then opens a block that the caller closes after using the value.
+EV Emitter::rhs_value(const NodeP& rhs, bool* opened) {
+  fast_div = true;
+  used_fast = false;
+  EV f = expr(rhs);
+  fast_div = false;
+  *opened = false;
+  if (!used_fast) return f;
+  EV e = expr(rhs);
+  std::string var = "ps_v" + std::to_string(++tmp_counter);
+  o << pad() << "{\n";
+  ++ind;
+  o << pad() << "bool ps_bad = false;\n";
+  o << pad() << cty(f.t) << " " << var << " = " << f.c << ";\n";
+  o << pad() << "if (ps_bad) " << var << " = " << cast(e, f.t).c << ";\n";
+  *opened = true;
+  return EV{var, f.t};
 }
 
 void Emitter::stmt(const NodeP& s) {
@@ -789,6 +904,98 @@ void Emitter::emit_memopt_body(int L) {
     if (dt == PS_BF16) v = "ps_bf16_ld(" + v + ")";
     return v;
   };
+  // LDS tiles: planes of optvars read at several (ox, oy) columns are staged in LDS once per
+  // step; each thread contributes its register-window value, a few lanes load the halo runs.
+  struct Run {
+    bool horiz;
+    int x0, y0, len;
+  };
+  struct Tile {
+    int ai, oz, xlo, xhi, ylo, yhi, W, H;
+    int own = -1;  // index into cols of the register column that supplies the interior
+    std::vector<Run> runs;
+    int nhalo = 0;
+    std::string name;
+    std::set<std::pair<int, int>> cols_at;
+  };
+  std::vector<Tile> tiles;
+  const int BX = V.bx, BY = V.by;
+  if (V.lds && L == 2) {
+    for (auto& A : mm.shmem_optvars) {
+      int ai = K.param_index.at(A);
+      const auto& offs = mm.offsets.at(A);
+      std::set<int> planes;
+      for (auto& c : offs)
+        for (auto& z : c.second) planes.insert(z.first);
+      for (int oz : planes) {
+        Tile t;
+        t.ai = ai;
+        t.oz = oz;
+        for (auto& c : offs)
+          if (c.second.count(oz)) t.cols_at.insert(c.first);
+        if (t.cols_at.size() < 2) continue;
+        t.xlo = t.ylo = 1 << 30;
+        t.xhi = t.yhi = -(1 << 30);
+        for (auto& p : t.cols_at) {
+          t.xlo = std::min(t.xlo, p.first);
+          t.xhi = std::max(t.xhi, p.first);
+          t.ylo = std::min(t.ylo, p.second);
+          t.yhi = std::max(t.yhi, p.second);
+        }
+        t.W = BX + t.xhi - t.xlo;
+        t.H = BY + t.yhi - t.ylo;
+        for (size_t ci = 0; ci < cols.size(); ++ci) {
+          const Col& c = cols[ci];
+          if (c.ai == ai && t.cols_at.count({c.o[0], c.o[1]}) && c.zlo <= oz && oz <= c.zhi) {
+            t.own = (int)ci;
+            break;
+          }
+        }
+        std::vector<char> need((size_t)t.W * t.H, 0);
+        for (auto& p : t.cols_at)
+          for (int ty = 0; ty < BY; ++ty)
+            for (int tx = 0; tx < BX; ++tx) need[(size_t)(ty + p.second - t.ylo) * t.W + (tx + p.first - t.xlo)] = 1;
+        if (t.own >= 0) {
+          const Col& c = cols[t.own];
+          for (int ty = 0; ty < BY; ++ty)
+            for (int tx = 0; tx < BX; ++tx) need[(size_t)(ty + c.o[1] - t.ylo) * t.W + (tx + c.o[0] - t.xlo)] = 0;
+        }
+        // horizontal runs of length >= 2, then vertical runs of what is left
+        for (int ly = 0; ly < t.H; ++ly)
+          for (int lx = 0; lx < t.W;) {
+            int e = lx;
+            while (e < t.W && need[(size_t)ly * t.W + e] == 1) ++e;
+            if (e - lx >= 2) {
+              t.runs.push_back({true, lx, ly, e - lx});
+              for (int q = lx; q < e; ++q) need[(size_t)ly * t.W + q] = 2;
+            }
+            lx = e > lx ? e : lx + 1;
+          }
+        for (int lx = 0; lx < t.W; ++lx)
+          for (int ly = 0; ly < t.H;) {
+            int e = ly;
+            while (e < t.H && need[(size_t)e * t.W + lx] == 1) ++e;
+            if (e > ly) {
+              t.runs.push_back({false, lx, ly, e - ly});
+              for (int q = ly; q < e; ++q) need[(size_t)q * t.W + lx] = 2;
+            }
+            ly = e > ly ? e : ly + 1;
+          }
+        for (auto& r : t.runs) t.nhalo += r.len;
+        t.name = "ps_t" + std::to_string(ai) + "_" + std::to_string(tiles.size());
+        tiles.push_back(t);
+      }
+    }
+  }
+  for (auto& t : tiles) {
+    Ty vt = value_ty(kinds[t.ai].dtype);
+    E_lds_bytes += 2 * t.W * t.H * (vt == Ty::F64 ? 8 : 4);
+    o << pad() << "__shared__ " << cty(vt) << " " << t.name << "[2][" << t.W * t.H << "];\n";
+  }
+  if (!tiles.empty()) {
+    o << pad() << "const int ps_lx = threadIdx.x, ps_ly = threadIdx.y, ps_tid = ps_lx + ps_ly * " << BX << ";\n";
+    o << pad() << "const ps_int ps_ix0 = ix - ps_lx, ps_iy0 = iy - ps_ly;\n";
+  }
   // preload
   for (auto& c : cols) {
     int w = c.zhi - c.zlo + 1;
@@ -800,6 +1007,40 @@ void Emitter::emit_memopt_body(int L) {
   for (auto& c : cols) {
     int w = c.zhi - c.zlo + 1;
     o << pad() << c.name << "_" << (w - 1) << " = " << ldq(c, lc + " + (" + std::to_string(c.zhi) + ")") << ";\n";
+  }
+  if (!tiles.empty()) {
+    o << pad() << "const int ps_buf = (int)((" << lc << " - ps_zb) & 1);\n";
+    for (auto& t : tiles) {
+      std::string A = an(t.ai);
+      int dt = kinds[t.ai].dtype;
+      if (t.own >= 0) {
+        const Col& c = cols[t.own];
+        o << pad() << t.name << "[ps_buf][(ps_ly + " << (c.o[1] - t.ylo) << ") * " << t.W << " + ps_lx + " << (c.o[0] - t.xlo)
+          << "] = " << c.name << "_" << (t.oz - c.zlo) << ";\n";
+      }
+      if (t.nhalo == 0) continue;
+      o << pad() << "for (int ps_c = ps_tid; ps_c < " << t.nhalo << "; ps_c += " << BX * BY << ") {\n";
+      ++ind;
+      o << pad() << "int ps_hx, ps_hy;\n";
+      int base = 0;
+      for (size_t r = 0; r < t.runs.size(); ++r) {
+        const Run& run = t.runs[r];
+        o << pad() << (r ? "else " : "") << "if (ps_c < " << base + run.len << ") { ";
+        if (run.horiz) o << "ps_hx = " << run.x0 << " + (ps_c - " << base << "); ps_hy = " << run.y0 << "; }\n";
+        else o << "ps_hx = " << run.x0 << "; ps_hy = " << run.y0 << " + (ps_c - " << base << "); }\n";
+        base += run.len;
+      }
+      o << pad() << "const ps_int ps_gx = ps_ix0 + ps_hx + (" << t.xlo << "), ps_gy = ps_iy0 + ps_hy + (" << t.ylo << "), ps_gz = " << lc
+        << " + (" << t.oz << ");\n";
+      std::string cond = "ps_gx >= 1 && ps_gx <= " + A + "_nx && ps_gy >= 1 && ps_gy <= " + A + "_ny && ps_gz >= 1 && ps_gz <= " + A + "_nz";
+      std::string raw = A + "[(ps_gx - 1) + (ps_gy - 1)*" + A + "_sx + (ps_gz - 1)*" + A + "_sxy]";
+      std::string v = "((" + cond + ") ? " + raw + " : (" + elem_cty(dt) + ")0)";
+      if (dt == PS_BF16) v = "ps_bf16_ld(" + v + ")";
+      o << pad() << t.name << "[ps_buf][ps_hy * " << t.W << " + ps_hx] = " << v << ";\n";
+      --ind;
+      o << pad() << "}\n";
+    }
+    o << pad() << "__syncthreads();\n";
   }
   // center offsets of every field at this step
   for (size_t i = 0; i < kinds.size(); ++i) {
@@ -816,8 +1057,23 @@ void Emitter::emit_memopt_body(int L) {
       cache[std::make_tuple(c.ai, off[0], off[1], off[2])] = c.name + "_" + std::to_string(k);
     }
   }
+  for (auto& t : tiles)
+    for (auto& p : t.cols_at) {
+      auto key = std::make_tuple(t.ai, p.first, p.second, t.oz);
+      if (cache.count(key)) continue;  // the thread's own register value
+      cache[key] = t.name + "[ps_buf][(ps_ly + " + std::to_string(p.second - t.ylo) + ") * " + std::to_string(t.W) + " + ps_lx + " +
+                   std::to_string(p.first - t.xlo) + "]";
+    }
   in_loop = true;
+  if (V.lds) {
+    o << pad() << "if (ps_act) {\n";
+    ++ind;
+  }
   block(K.body);
+  if (V.lds) {
+    --ind;
+    o << pad() << "}\n";
+  }
   in_loop = false;
   for (auto& c : cols) {
     int w = c.zhi - c.zlo + 1;
@@ -833,6 +1089,17 @@ Emitted Emitter::run() {
   E.entry = entry;
   const char* I = V.idx64 ? "long long" : "int";
   int ib = V.idx64 ? 8 : 4;
+  // pass 1: discover local variables, their types and the uniform divisors
+  {
+    pass1 = true;
+    std::ostringstream saved;
+    saved.swap(o);
+    in_loop = false;
+    block(K.body);
+    o.swap(saved);
+    pass1 = false;
+    o.str("");
+  }
   std::ostringstream sig;
   for (size_t i = 0; i < kinds.size(); ++i) {
     const ArgKind& k = kinds[i];
@@ -857,23 +1124,9 @@ Emitted Emitter::run() {
   for (int d = 0; d < 3; ++d) E.params.push_back({4, -1, d, ib, 0});
   sig << "    int* ps_err";
   E.params.push_back({5, -1, 0, 8, 0});
-
-  // pass 1: discover local variables and their types
-  {
-    pass1 = true;
-    std::ostringstream saved;
-    saved.swap(o);
-    if (V.memopt) {
-      int L = K.nidx - 1;
-      (void)L;
-      in_loop = false;
-      block(K.body);
-    } else {
-      block(K.body);
-    }
-    o.swap(saved);
-    pass1 = false;
-    o.str("");
+  for (auto& rp : rcp_params) {  // reciprocals of uniform divisors (0 = use IEEE division)
+    sig << ",\n    " << cty(rp.second) << " " << an(rp.first) << "_r";
+    E.params.push_back({6, rp.first, 0, rp.second == Ty::F64 ? 8 : 4, rp.second == Ty::F64 ? PS_F64 : PS_F32});
   }
 
   std::ostringstream src;
@@ -921,7 +1174,9 @@ Emitted Emitter::run() {
       o << "  const ps_int t" << "xyz"[d] << " = " << bl[d] << " * (ps_int)" << bd[d] << " + (ps_int)" << td[d] << ";\n";
       guard += std::string(guard.empty() ? "" : " || ") + "t" + "xyz"[d] + " >= " + rl[d];
     }
-    o << "  if (" << guard << ") return;\n";
+    // with LDS tiles every thread of the block must reach the barriers: no early return
+    if (V.lds) o << "  const bool ps_act = !(" << guard << ");\n";
+    else o << "  if (" << guard << ") return;\n";
     for (int d = 0; d < 3; ++d) {
       if (d == L) continue;
       o << "  const ps_int " << kIdxC[d] << " = " << r0[d] << " + t" << "xyz"[d] << ";\n";
@@ -935,6 +1190,7 @@ Emitted Emitter::run() {
   }
   src << o.str() << "}\n";
   E.source = src.str();
+  E.lds_bytes = E_lds_bytes;
   return E;
 }
 

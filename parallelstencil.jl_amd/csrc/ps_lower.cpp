@@ -302,6 +302,7 @@ static void read_options(Options& o, const JVal& j) {
     else if (k == "fp_contract") o.fp_contract = v.b;
     else if (k == "lds") o.lds = v.b;
     else if (k == "xcd_remap") o.xcd_remap = v.b;
+    else if (k == "nt_store") o.nt_store = v.b;
     else if (k == "useshmemhalos" || k == "optimize_halo_read" || k == "nonconst_metadata" || k == "metadata_module" || k == "N") {
       // accepted for interface compatibility; they only steer the reference's shared-memory path
     } else {
@@ -757,8 +758,8 @@ std::string kernel_info_json(const Kernel& K) {
 
 std::string Variant::key() const {
   char b[96];
-  std::snprintf(b, sizeof b, "m%dc%di%da%dt%dl%d", memopt, checked, idx64, noalias, maxthreads, lds);
-  return b;
+  std::snprintf(b, sizeof b, "m%dc%di%da%dt%dl%d_%dx%d", memopt, checked, idx64, noalias, maxthreads, lds, bx, by);
+  return std::string(b) + "d" + divmode;
 }
 
 }  // namespace ps

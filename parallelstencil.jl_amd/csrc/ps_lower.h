@@ -49,8 +49,9 @@ struct Options {
   int nthreads_x_max = 64;
   int nthreads_max = 256;
   bool fp_contract = false;               // Julia semantics: no FMA contraction
-  bool lds = false;                       // stage x-y neighbours of optvars through LDS
+  bool lds = true;                        // memopt: stage x-y neighbours of optvars through LDS
   bool xcd_remap = true;                  // XCD-aware block ordering for memopt kernels
+  bool nt_store = false;                  // nontemporal stores for array writes
 };
 
 struct Kernel {
@@ -71,7 +72,11 @@ struct Variant {
   bool idx64 = false;     // 64-bit index arithmetic
   bool noalias = true;    // __restrict__ on array pointers
   int maxthreads = 256;   // __launch_bounds__
-  bool lds = false;
+  bool lds = false;       // memopt: stage x-y neighbours of optvars through LDS tiles
+  int bx = 0, by = 0;     // block shape baked into LDS variants
+  // per argument: 'p' / 'n' a positive / negative float scalar accepted as reciprocal divisor,
+  // 'x' a float scalar that must be divided by exactly, '-' anything else
+  std::string divmode;
   std::string key() const;
 };
 
@@ -89,6 +94,10 @@ struct Emitted {
   std::vector<P> params;
   int lds_bytes = 0;
 };
+
+// Divisors for which the reciprocal-based division of the generated code is exact.
+bool rcp_safe64(double b);
+bool rcp_safe32(float b);
 
 Kernel define_kernel(const std::string& text, const JVal& opts);
 Emitted lower_kernel(const Kernel& k, const std::vector<ArgKind>& kinds, const Variant& v);

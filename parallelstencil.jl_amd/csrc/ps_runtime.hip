@@ -56,6 +56,12 @@ static int fail(int code, const std::string& msg) {
   }
 
 extern "C" const char* ps_last_error(void) { return g_err.c_str(); }
+
+// extra row padding (bytes) applied to fields whose row stride is a multiple of 4 KiB
+static int64_t g_pitch_pad = [] {
+  const char* e = std::getenv("PS_AMD_PITCH_PAD");
+  return e ? (int64_t)std::atoll(e) : (int64_t)0;
+}();
 extern "C" int ps_version(void) { return 10000; }
 
 // ---------------------------------------------------------------------------
@@ -425,6 +431,9 @@ extern "C" int ps_field_alloc(int ndims, const int64_t dims[3], int dtype, ps_fi
   int64_t align_el = 256 / (int64_t)es;
   int64_t pitch = ((std::max<int64_t>(d[0], 1) + align_el - 1) / align_el) * align_el;
   if (d[1] == 1 && d[2] == 1) pitch = std::max<int64_t>(d[0], 1);  // 1-D fields are dense
+  // Rows whose byte stride is a multiple of 4 KiB put the neighbours of a stencil (y +/- 1,
+  // z +/- 1) on the same HBM channels; a pad of g_pitch_pad bytes breaks that regularity.
+  if (d[1] > 1 && g_pitch_pad > 0 && ((pitch * (int64_t)es) % 4096) == 0) pitch += g_pitch_pad / (int64_t)es;
   int64_t plane = pitch * d[1];
   int64_t bytes = std::max<int64_t>(plane * d[2] * (int64_t)es, 256);
   void* p = nullptr;
@@ -762,6 +771,21 @@ extern "C" int ps_kernel_info(ps_kernel k, char* out, size_t cap) {
   PS_GUARD_END
 }
 
+// Division mode of every argument (see Variant::divmode): float scalars whose magnitude the
+// reciprocal division handles exactly are 'p' / 'n' by sign, other float scalars 'x'.
+static std::string divmode_of(const ps_arg* args, int nargs) {
+  std::string m(nargs, '-');
+  for (int i = 0; i < nargs; ++i) {
+    if (args[i].kind != PS_ARG_SCALAR) continue;
+    if (args[i].dtype == PS_F64) m[i] = rcp_safe64(args[i].f) ? (args[i].f > 0 ? 'p' : 'n') : 'x';
+    else if (args[i].dtype == PS_F32) {
+      float b = (float)args[i].f;
+      m[i] = rcp_safe32(b) ? (b > 0 ? 'p' : 'n') : 'x';
+    }
+  }
+  return m;
+}
+
 static Variant variant_from_flags(const Kernel& K, int flags) {
   Variant v;
   v.memopt = (flags & 1) ? true : false;
@@ -770,6 +794,12 @@ static Variant variant_from_flags(const Kernel& K, int flags) {
   v.idx64 = (flags & 4) != 0;
   v.noalias = (flags & 8) == 0;
   v.maxthreads = (flags & 16) ? 1024 : 256;
+  if (flags & 32) {  // LDS-tiled memopt variant for the block shape in bits 8..19 / 20..31
+    v.lds = true;
+    v.bx = (flags >> 8) & 0xfff;
+    v.by = (flags >> 20) & 0xfff;
+    if (!v.memopt || v.bx < 1 || v.by < 1) throw PsError(PS_EARGUMENT, "LDS variants need memopt and a block shape");
+  }
   return v;
 }
 
@@ -777,6 +807,7 @@ extern "C" int ps_kernel_source(ps_kernel k, const ps_arg* args, int nargs, int 
   PS_GUARD_BEGIN
   KernelObj* ko = (KernelObj*)k;
   Variant v = variant_from_flags(ko->K, flags);
+  v.divmode = divmode_of(args, nargs);
   Emitted em = lower_kernel(ko->K, arg_kinds(args, nargs), v);
   if (em.source.size() + 1 > cap) return fail(PS_EARGUMENT, "ps_kernel_source: buffer too small (need " + std::to_string(em.source.size() + 1) + ")");
   std::memcpy(out, em.source.c_str(), em.source.size() + 1);
@@ -788,6 +819,7 @@ extern "C" int ps_kernel_precompile(ps_kernel k, const ps_arg* args, int nargs, 
   PS_GUARD_BEGIN
   KernelObj* ko = (KernelObj*)k;
   Variant v = variant_from_flags(ko->K, flags);
+  v.divmode = divmode_of(args, nargs);
   get_module(ko, arg_kinds(args, nargs), v, false);
   return 0;
   PS_GUARD_END
@@ -902,6 +934,11 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   // variant
   Variant v;
   v.memopt = memopt;
+  if (memopt && K.opt.lds && !K.mm.shmem_optvars.empty() && K.mm.loopdim == 3 && P.nthreads[2] == 1) {
+    v.lds = true;
+    v.bx = P.nthreads[0];
+    v.by = P.nthreads[1];
+  }
   v.maxthreads = ((int64_t)P.nthreads[0] * P.nthreads[1] * P.nthreads[2] > 256) ? 1024 : 256;
   bool big = false;
   for (int i = 0; i < nargs; ++i) {
@@ -915,6 +952,7 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   for (int d = 0; d < 3; ++d)
     if (P.ranges[d][1] > INT32_MAX / 2 || P.ranges[d][0] < INT32_MIN / 2) big = true;
   v.idx64 = big;
+  v.divmode = divmode_of(args, nargs);
   // aliasing: a written field that overlaps another field argument
   for (int i = 0; i < nargs && v.noalias; ++i) {
     if (args[i].kind != PS_ARG_FIELD) continue;
@@ -971,6 +1009,16 @@ extern "C" int ps_kernel_geometry(ps_kernel k, const ps_arg* args, int nargs, co
   PS_GUARD_END
 }
 
+extern "C" int ps_kernel_precompile_launch(ps_kernel k, const ps_arg* args, int nargs, const ps_launch* launch) {
+  PS_GUARD_BEGIN
+  KernelObj* ko = (KernelObj*)k;
+  std::vector<ArgKind> kinds;
+  Plan P = plan_launch(ko->K, args, nargs, launch, kinds);
+  get_module(ko, kinds, P.v, false);
+  return 0;
+  PS_GUARD_END
+}
+
 extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, const ps_launch* launch) {
   if (int r = require_init()) return r;
   PS_GUARD_BEGIN
@@ -1019,6 +1067,18 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
       case 5: {
         void* e = (void*)S().d_err;
         std::memcpy(&v, &e, 8);
+        isint = false;
+        break;
+      }
+      case 6: {  // RN(1/b) of a uniform divisor, 0 when the fast division is not exact for b
+        const ps_arg& a = args[p.arg];
+        if (p.dtype == PS_F64) {
+          double b = std::fabs(a.f), r = rcp_safe64(b) ? 1.0 / b : 0.0;
+          std::memcpy(&v, &r, 8);
+        } else {
+          float b = std::fabs((float)a.f), r = rcp_safe32(b) ? 1.0f / b : 0.0f;
+          std::memcpy(&v, &r, 4);
+        }
         isint = false;
         break;
       }

@@ -157,6 +157,7 @@ _SIGS = {
     "ps_kernel_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
     "ps_kernel_source": (ctypes.c_int, [ctypes.c_void_p, _P(ps_arg), ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
     "ps_kernel_precompile": (ctypes.c_int, [ctypes.c_void_p, _P(ps_arg), ctypes.c_int, ctypes.c_int]),
+    "ps_kernel_precompile_launch": (ctypes.c_int, [ctypes.c_void_p, _P(ps_arg), ctypes.c_int, _P(ps_launch)]),
     "ps_kernel_launch": (ctypes.c_int, [ctypes.c_void_p, _P(ps_arg), ctypes.c_int, _P(ps_launch)]),
     "ps_kernel_geometry": (ctypes.c_int, [ctypes.c_void_p, _P(ps_arg), ctypes.c_int, _P(ps_launch), _P(ctypes.c_int32),
                                           _P(ctypes.c_int32), _P(ctypes.c_int64), _P(ctypes.c_int64)]),

@@ -298,6 +298,11 @@ class ArgSpec:
         self.f.dtype = dtype_code(eltype)
         self.f.pitch = self.shape[0] if self.shape else 1
         self.f.plane = self.f.pitch * (self.shape[1] if len(self.shape) > 1 else 1)
+        # distinct fake addresses (never dereferenced) so that aliasing analysis sees separate fields
+        ArgSpec._next += 1
+        self.f.ptr = ArgSpec._next << 40
+
+    _next = 0
 
 
 def _numbertype_of(mod):
@@ -654,9 +659,16 @@ class Kernel:
         flags = (1 if mo else 0) | (2 if checked else 0) | (4 if idx64 else 0) | (16 if maxthreads > 256 else 0)
         check(lib.ps_kernel_precompile(self._h, arr, len(args), flags))
 
+    def precompile_launch(self, *args, ranges=None, nblocks=None, nthreads=None, memopt=None):
+        """Compiles exactly the variant a launch with these arguments would use (no GPU needed);
+        fields may be ArgSpecs."""
+        arr = self._args(args)
+        L = self._launch_struct(ranges, nblocks, nthreads, None, True, memopt)
+        check(lib.ps_kernel_precompile_launch(self._h, arr, len(args), ctypes.byref(L)))
+
 
 _KW_DEF = ("ndims", "N", "inbounds", "padding", "memopt", "optvars", "loopdim", "loopsize", "optranges", "useshmemhalos",
-           "optimize_halo_read", "fp_contract", "lds", "xcd_remap", "nthreads_max_memopt")
+           "optimize_halo_read", "fp_contract", "lds", "xcd_remap", "nt_store", "nthreads_max_memopt")
 
 
 def _define(kind, text, kw, mod, indices=None):
@@ -669,7 +681,7 @@ def _define(kind, text, kw, mod, indices=None):
             "memopt": kw.get("memopt", cfg.memopt)}
     if indices is not None:
         opts["indices"] = list(indices)
-    for k in ("optvars", "loopdim", "loopsize", "fp_contract", "lds", "xcd_remap", "nthreads_max_memopt"):
+    for k in ("optvars", "loopdim", "loopsize", "fp_contract", "lds", "xcd_remap", "nt_store", "nthreads_max_memopt"):
         if k in kw:
             v = kw[k]
             opts[k] = list(v) if k == "optvars" and isinstance(v, (tuple, list)) else v
