@@ -147,6 +147,19 @@ PS_DI float ps_divp(float a, float b, float r, bool& bad) {
   bad |= !(aq <= 0x1p50f) | ((aq < 0x1p-50f) & (aq != 0.0f));
   return q;
 }
+// Guarded load without a branch: element 0 of the array stands in for an index outside it
+// (the runtime only launches these kernels on non-empty arrays), and the value is then zeroed.
+template <class T>
+PS_DI T ps_ldz(const T* p, bool c, ps_int i) {
+  T v = p[c ? i : (ps_int)0];
+  return c ? v : (T)0;
+}
+// Same without zeroing, for kernels whose accesses are proven in bounds: every use of such a
+// value happens where the access is inside the array, so the stand-in value is never used.
+template <class T>
+PS_DI T ps_ldc(const T* p, bool c, ps_int i) {
+  return p[c ? i : (ps_int)0];
+}
 PS_DI bool ps_chk(ps_int i, ps_int j, ps_int k, ps_int nx, ps_int ny, ps_int nz, int* err) {
   bool ok = i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz;
   if (!ok) *err = 1;
@@ -828,22 +841,26 @@ void Emitter::block(const NodeP& b) {
 }
 
 // memopt: thread per column, stream along dimension L (0-based) in chunks of loopsize.
+// The loop is software-pipelined: every value a step takes from a register column or from a
+// halo of an LDS plane tile is loaded during the previous step, so each load has a whole step
+// to arrive and the barrier that publishes a tile only waits for LDS writes.
 void Emitter::emit_memopt_body(int L) {
   const MemoptMeta& mm = K.mm;
-  // register windows per cached column
-  struct Col {
+  // points read per read-only array: (ox, oy) -> planes oz (oy = 0 when streaming dimension 2)
+  struct Pts {
     int ai;
-    int o[3];  // o[L] unused
-    int zlo, zhi;
     std::string name;
+    bool optvar;
+    std::map<std::pair<int, int>, std::set<int>> cols;
   };
-  std::vector<Col> cols;
-  for (auto& A : mm.optvars) {
-    int ai = K.param_index.at(A);
-    const auto& offs = mm.offsets.at(A);
-    bool explicit_r = mm.optranges.count(A) > 0;
-    for (auto& c : offs) {
-      int lo = 1 << 30, hi = -(1 << 30);
+  std::vector<Pts> pts;
+  auto collect = [&](const std::string& A, const std::map<std::pair<int, int>, std::map<int, int>>& offs, bool optvar) {
+    Pts P;
+    P.ai = K.param_index.at(A);
+    P.name = A;
+    P.optvar = optvar;
+    bool explicit_r = optvar && mm.optranges.count(A) > 0;
+    for (auto& c : offs)
       for (auto& z : c.second) {
         if (explicit_r) {
           const auto& r = mm.optranges.at(A);
@@ -854,112 +871,71 @@ void Emitter::emit_memopt_body(int L) {
           if (z.first < r[L].first || z.first > r[L].second) in = false;
           if (!in) continue;
         }
-        lo = std::min(lo, z.first);
-        hi = std::max(hi, z.first);
+        P.cols[c.first].insert(z.first);
       }
-      if (lo > hi) continue;
-      if (!explicit_r && hi == lo) continue;  // no reuse along the stream: plain load
-      Col col;
-      col.ai = ai;
-      col.o[0] = c.first.first;
-      col.o[1] = L == 2 ? c.first.second : 0;
-      col.o[2] = 0;
-      col.zlo = lo;
-      col.zhi = hi;
-      col.name = "q" + std::to_string(ai) + "_" + std::to_string(cols.size());
-      cols.push_back(col);
-    }
-  }
-  std::string lc = kIdxC[L];
-  // per column: base offset (without the streamed dimension) and validity
-  for (auto& c : cols) {
-    std::string A = an(c.ai);
-    const char* etp = elem_cty(kinds[c.ai].dtype);
-    std::ostringstream base, ok;
-    base << "(ps_int)0";
-    ok << "true";
-    static const char* szn[3] = {"_nx", "_ny", "_nz"};
-    static const char* str[3] = {"1", "_sx", "_sxy"};
-    for (int d = 0; d < 3; ++d) {
-      if (d == L) continue;
-      std::string idx = d < K.nidx ? std::string("(") + kIdxC[d] + " + (" + std::to_string(c.o[d]) + "))" : std::string("((ps_int)1)");
-      base << " + (" << idx << " - 1)" << (d == 0 ? "" : "*" + A + str[d]);
-      ok << " && " << idx << " >= 1 && " << idx << " <= " << A << szn[d];
-    }
-    o << pad() << "const ps_int " << c.name << "_b = " << base.str() << ";\n";
-    o << pad() << "const bool " << c.name << "_ok = " << ok.str() << ";\n";
-    int w = c.zhi - c.zlo + 1;
-    Ty vt = value_ty(kinds[c.ai].dtype);
-    for (int k = 0; k < w; ++k) o << pad() << cty(vt) << " " << c.name << "_" << k << ";\n";
-    (void)etp;
-  }
-  auto ldq = [&](const Col& c, const std::string& z) {
-    std::string A = an(c.ai);
-    const char* szn = L == 2 ? "_nz" : "_ny";
-    const char* st = L == 2 ? "_sxy" : "_sx";
-    std::string raw = A + "[" + c.name + "_b + (" + z + " - 1)*" + A + st + "]";
-    std::string cond = "(" + c.name + "_ok && (" + z + ") >= 1 && (" + z + ") <= " + A + szn + ")";
-    int dt = kinds[c.ai].dtype;
-    std::string v = "(" + cond + " ? " + raw + " : (" + elem_cty(dt) + ")0)";
-    if (dt == PS_BF16) v = "ps_bf16_ld(" + v + ")";
-    return v;
+    pts.push_back(P);
   };
+  for (auto& A : mm.optvars) collect(A, mm.offsets.at(A), true);
+  // read-only arrays outside optvars are only prefetched into registers; a bounds-checked
+  // variant keeps their plain (checked) loads
+  if (!V.checked)
+    for (auto& kv : mm.ro_offsets) collect(kv.first, kv.second, false);
+
   // LDS tiles: planes of optvars read at several (ox, oy) columns are staged in LDS once per
-  // step; each thread contributes its register-window value, a few lanes load the halo runs.
+  // step; each thread contributes its register value ("own" column), the halo runs are loaded
+  // by the first lanes of the block.
   struct Run {
     bool horiz;
     int x0, y0, len;
   };
   struct Tile {
     int ai, oz, xlo, xhi, ylo, yhi, W, H;
-    int own = -1;  // index into cols of the register column that supplies the interior
+    std::pair<int, int> own;
+    int own_col = -1;
     std::vector<Run> runs;
-    int nhalo = 0;
+    int nhalo = 0, nslots = 0;
     std::string name;
     std::set<std::pair<int, int>> cols_at;
   };
   std::vector<Tile> tiles;
   const int BX = V.bx, BY = V.by;
   if (V.lds && L == 2) {
-    for (auto& A : mm.shmem_optvars) {
-      int ai = K.param_index.at(A);
-      const auto& offs = mm.offsets.at(A);
+    for (auto& P : pts) {
+      if (!P.optvar || std::find(mm.shmem_optvars.begin(), mm.shmem_optvars.end(), P.name) == mm.shmem_optvars.end()) continue;
+      auto orig = P.cols;
       std::set<int> planes;
-      for (auto& c : offs)
-        for (auto& z : c.second) planes.insert(z.first);
+      for (auto& c : orig)
+        for (int z : c.second) planes.insert(z);
       for (int oz : planes) {
         Tile t;
-        t.ai = ai;
+        t.ai = P.ai;
         t.oz = oz;
-        for (auto& c : offs)
+        for (auto& c : orig)
           if (c.second.count(oz)) t.cols_at.insert(c.first);
         if (t.cols_at.size() < 2) continue;
         t.xlo = t.ylo = 1 << 30;
         t.xhi = t.yhi = -(1 << 30);
+        size_t best = 0;
         for (auto& p : t.cols_at) {
           t.xlo = std::min(t.xlo, p.first);
           t.xhi = std::max(t.xhi, p.first);
           t.ylo = std::min(t.ylo, p.second);
           t.yhi = std::max(t.yhi, p.second);
-        }
-        t.W = BX + t.xhi - t.xlo;
-        t.H = BY + t.yhi - t.ylo;
-        for (size_t ci = 0; ci < cols.size(); ++ci) {
-          const Col& c = cols[ci];
-          if (c.ai == ai && t.cols_at.count({c.o[0], c.o[1]}) && c.zlo <= oz && oz <= c.zhi) {
-            t.own = (int)ci;
-            break;
+          if (orig.at(p).size() > best) {
+            best = orig.at(p).size();
+            t.own = p;
           }
         }
+        for (auto& p : t.cols_at)
+          if (p != t.own) P.cols[p].erase(oz);  // served by the tile
+        t.W = BX + t.xhi - t.xlo;
+        t.H = BY + t.yhi - t.ylo;
         std::vector<char> need((size_t)t.W * t.H, 0);
         for (auto& p : t.cols_at)
           for (int ty = 0; ty < BY; ++ty)
             for (int tx = 0; tx < BX; ++tx) need[(size_t)(ty + p.second - t.ylo) * t.W + (tx + p.first - t.xlo)] = 1;
-        if (t.own >= 0) {
-          const Col& c = cols[t.own];
-          for (int ty = 0; ty < BY; ++ty)
-            for (int tx = 0; tx < BX; ++tx) need[(size_t)(ty + c.o[1] - t.ylo) * t.W + (tx + c.o[0] - t.xlo)] = 0;
-        }
+        for (int ty = 0; ty < BY; ++ty)
+          for (int tx = 0; tx < BX; ++tx) need[(size_t)(ty + t.own.second - t.ylo) * t.W + (tx + t.own.first - t.xlo)] = 0;
         // horizontal runs of length >= 2, then vertical runs of what is left
         for (int ly = 0; ly < t.H; ++ly)
           for (int lx = 0; lx < t.W;) {
@@ -982,11 +958,73 @@ void Emitter::emit_memopt_body(int L) {
             ly = e > ly ? e : ly + 1;
           }
         for (auto& r : t.runs) t.nhalo += r.len;
-        t.name = "ps_t" + std::to_string(ai) + "_" + std::to_string(tiles.size());
+        t.nslots = (t.nhalo + BX * BY - 1) / (BX * BY);
+        t.name = "ps_t" + std::to_string(P.ai) + "_" + std::to_string(tiles.size());
         tiles.push_back(t);
       }
     }
   }
+  // register columns: a window of planes [zlo, zhi] per (array, ox, oy) plus the next plane
+  struct Col {
+    int ai;
+    int o[3];  // o[L] unused
+    int zlo, zhi;
+    std::string name;
+  };
+  std::vector<Col> cols;
+  for (auto& P : pts)
+    for (auto& c : P.cols) {
+      if (c.second.empty()) continue;
+      Col col;
+      col.ai = P.ai;
+      col.o[0] = c.first.first;
+      col.o[1] = L == 2 ? c.first.second : 0;
+      col.o[2] = 0;
+      col.zlo = *c.second.begin();
+      col.zhi = *c.second.rbegin();
+      col.name = "q" + std::to_string(P.ai) + "_" + std::to_string(cols.size());
+      cols.push_back(col);
+    }
+  for (auto& t : tiles)
+    for (size_t ci = 0; ci < cols.size(); ++ci)
+      if (cols[ci].ai == t.ai && cols[ci].o[0] == t.own.first && cols[ci].o[1] == t.own.second && cols[ci].zlo <= t.oz &&
+          t.oz <= cols[ci].zhi)
+        t.own_col = (int)ci;
+  for (auto& t : tiles)
+    if (t.own_col < 0) throw PsError(PS_EINTERNAL, "memopt: tile without register column");
+
+  std::string lc = kIdxC[L];
+  static const char* szn[3] = {"_nx", "_ny", "_nz"};
+  static const char* str[3] = {"1", "_sx", "_sxy"};
+  // per column: base offset (without the streamed dimension) and validity
+  for (auto& c : cols) {
+    std::string A = an(c.ai);
+    std::ostringstream base, ok;
+    base << "(ps_int)0";
+    ok << "true";
+    for (int d = 0; d < 3; ++d) {
+      if (d == L) continue;
+      std::string idx = d < K.nidx ? std::string("(") + kIdxC[d] + " + (" + std::to_string(c.o[d]) + "))" : std::string("((ps_int)1)");
+      base << " + (" << idx << " - 1)" << (d == 0 ? "" : "*" + A + str[d]);
+      ok << " && " << idx << " >= 1 && " << idx << " <= " << A << szn[d];
+    }
+    o << pad() << "const ps_int " << c.name << "_b = " << base.str() << ";\n";
+    o << pad() << "const bool " << c.name << "_ok = " << ok.str() << ";\n";
+    int w = c.zhi - c.zlo + 1;
+    Ty vt = value_ty(kinds[c.ai].dtype);
+    for (int k = 0; k < w; ++k) o << pad() << cty(vt) << " " << c.name << "_" << k << ";\n";
+    o << pad() << cty(vt) << " " << c.name << "_n;\n";
+  }
+  // guarded load of column c at plane z (an expression); `extra` is an additional condition
+  auto ldq = [&](const Col& c, const std::string& z, const std::string& extra) {
+    std::string A = an(c.ai);
+    std::string idx = c.name + "_b + (" + z + " - 1)*" + A + str[L];
+    std::string cond = extra + c.name + "_ok && (" + z + ") >= 1 && (" + z + ") <= " + A + szn[L];
+    int dt = kinds[c.ai].dtype;
+    std::string v = std::string(V.checked ? "ps_ldz(" : "ps_ldc(") + A + ", " + cond + ", " + idx + ")";
+    if (dt == PS_BF16) v = "ps_bf16_ld(" + v + ")";
+    return v;
+  };
   for (auto& t : tiles) {
     Ty vt = value_ty(kinds[t.ai].dtype);
     E_lds_bytes += 2 * t.W * t.H * (vt == Ty::F64 ? 8 : 4);
@@ -996,49 +1034,72 @@ void Emitter::emit_memopt_body(int L) {
     o << pad() << "const int ps_lx = threadIdx.x, ps_ly = threadIdx.y, ps_tid = ps_lx + ps_ly * " << BX << ";\n";
     o << pad() << "const ps_int ps_ix0 = ix - ps_lx, ps_iy0 = iy - ps_ly;\n";
   }
-  // preload
+  // halo slots: slot j of a thread covers halo cell ps_tid + j*BX*BY of its tile; the position
+  // in the tile and the x-y part of the global offset are fixed for the whole stream
+  for (auto& t : tiles) {
+    std::string A = an(t.ai);
+    Ty vt = value_ty(kinds[t.ai].dtype);
+    for (int j = 0; j < t.nslots; ++j) {
+      std::string S = t.name + "_s" + std::to_string(j);
+      o << pad() << "int " << S << "_x = 0, " << S << "_y = 0;\n";
+      o << pad() << "const int " << S << "_c = ps_tid + " << j * BX * BY << ";\n";
+      o << pad() << "const bool " << S << "_v = " << S << "_c < " << t.nhalo << ";\n";
+      int base = 0;
+      for (size_t r = 0; r < t.runs.size(); ++r) {
+        const Run& run = t.runs[r];
+        o << pad() << (r ? "else " : "") << "if (" << S << "_c < " << base + run.len << ") { ";
+        if (run.horiz) o << S << "_x = " << run.x0 << " + (" << S << "_c - " << base << "); " << S << "_y = " << run.y0 << "; }\n";
+        else o << S << "_x = " << run.x0 << "; " << S << "_y = " << run.y0 << " + (" << S << "_c - " << base << "); }\n";
+        base += run.len;
+      }
+      o << pad() << "const ps_int " << S << "_gx = ps_ix0 + " << S << "_x + (" << t.xlo << "), " << S << "_gy = ps_iy0 + " << S << "_y + ("
+        << t.ylo << ");\n";
+      o << pad() << "const bool " << S << "_ok = " << S << "_v && " << S << "_gx >= 1 && " << S << "_gx <= " << A << "_nx && " << S
+        << "_gy >= 1 && " << S << "_gy <= " << A << "_ny;\n";
+      o << pad() << "const ps_int " << S << "_b = (" << S << "_gx - 1) + (" << S << "_gy - 1)*" << A << "_sx;\n";
+      o << pad() << "const int " << S << "_l = " << S << "_y * " << t.W << " + " << S << "_x;\n";
+      o << pad() << cty(vt) << " " << S << "_h;\n";
+    }
+  }
+  auto ldh = [&](const Tile& t, int j, const std::string& z, const std::string& extra) {
+    std::string A = an(t.ai);
+    std::string S = t.name + "_s" + std::to_string(j);
+    std::string cond = extra + S + "_ok && (" + z + ") >= 1 && (" + z + ") <= " + A + "_nz";
+    std::string idx = S + "_b + (" + z + " - 1)*" + A + "_sxy";
+    int dt = kinds[t.ai].dtype;
+    std::string v = std::string(V.checked ? "ps_ldz(" : "ps_ldc(") + A + ", " + cond + ", " + idx + ")";
+    if (dt == PS_BF16) v = "ps_bf16_ld(" + v + ")";
+    return v;
+  };
+  // prologue: the window below the first step, and the values of the first step
   for (auto& c : cols) {
     int w = c.zhi - c.zlo + 1;
     for (int k = 0; k + 1 < w; ++k)
-      o << pad() << c.name << "_" << k << " = " << ldq(c, "ps_zb + (" + std::to_string(c.zlo + k) + ")") << ";\n";
+      o << pad() << c.name << "_" << k << " = " << ldq(c, "ps_zb + (" + std::to_string(c.zlo + k) + ")", "") << ";\n";
+    o << pad() << c.name << "_n = " << ldq(c, "ps_zb + (" + std::to_string(c.zhi) + ")", "") << ";\n";
   }
+  for (auto& t : tiles)
+    for (int j = 0; j < t.nslots; ++j)
+      o << pad() << t.name << "_s" << j << "_h = " << ldh(t, j, "ps_zb + (" + std::to_string(t.oz) + ")", "") << ";\n";
   o << pad() << "for (ps_int " << lc << " = ps_zb; " << lc << " < ps_ze; ++" << lc << ") {\n";
   ++ind;
+  o << pad() << "const bool ps_more = " << lc << " + 1 < ps_ze;\n";
   for (auto& c : cols) {
     int w = c.zhi - c.zlo + 1;
-    o << pad() << c.name << "_" << (w - 1) << " = " << ldq(c, lc + " + (" + std::to_string(c.zhi) + ")") << ";\n";
+    o << pad() << c.name << "_" << (w - 1) << " = " << c.name << "_n;\n";
+    o << pad() << c.name << "_n = " << ldq(c, lc + " + (" + std::to_string(c.zhi + 1) + ")", "ps_more && ") << ";\n";
   }
   if (!tiles.empty()) {
     o << pad() << "const int ps_buf = (int)((" << lc << " - ps_zb) & 1);\n";
     for (auto& t : tiles) {
-      std::string A = an(t.ai);
-      int dt = kinds[t.ai].dtype;
-      if (t.own >= 0) {
-        const Col& c = cols[t.own];
-        o << pad() << t.name << "[ps_buf][(ps_ly + " << (c.o[1] - t.ylo) << ") * " << t.W << " + ps_lx + " << (c.o[0] - t.xlo)
-          << "] = " << c.name << "_" << (t.oz - c.zlo) << ";\n";
+      const Col& c = cols[t.own_col];
+      o << pad() << t.name << "[ps_buf][(ps_ly + " << (c.o[1] - t.ylo) << ") * " << t.W << " + ps_lx + " << (c.o[0] - t.xlo)
+        << "] = " << c.name << "_" << (t.oz - c.zlo) << ";\n";
+      for (int j = 0; j < t.nslots; ++j) {
+        std::string S = t.name + "_s" + std::to_string(j);
+        o << pad() << "if (" << S << "_v) " << t.name << "[ps_buf][" << S << "_l] = " << S << "_h;\n";
+        o << pad() << S << "_h = " << ldh(t, j, lc + " + (" + std::to_string(t.oz + 1) + ")", "ps_more && ") << ";\n";
       }
-      if (t.nhalo == 0) continue;
-      o << pad() << "for (int ps_c = ps_tid; ps_c < " << t.nhalo << "; ps_c += " << BX * BY << ") {\n";
-      ++ind;
-      o << pad() << "int ps_hx, ps_hy;\n";
-      int base = 0;
-      for (size_t r = 0; r < t.runs.size(); ++r) {
-        const Run& run = t.runs[r];
-        o << pad() << (r ? "else " : "") << "if (ps_c < " << base + run.len << ") { ";
-        if (run.horiz) o << "ps_hx = " << run.x0 << " + (ps_c - " << base << "); ps_hy = " << run.y0 << "; }\n";
-        else o << "ps_hx = " << run.x0 << "; ps_hy = " << run.y0 << " + (ps_c - " << base << "); }\n";
-        base += run.len;
-      }
-      o << pad() << "const ps_int ps_gx = ps_ix0 + ps_hx + (" << t.xlo << "), ps_gy = ps_iy0 + ps_hy + (" << t.ylo << "), ps_gz = " << lc
-        << " + (" << t.oz << ");\n";
-      std::string cond = "ps_gx >= 1 && ps_gx <= " + A + "_nx && ps_gy >= 1 && ps_gy <= " + A + "_ny && ps_gz >= 1 && ps_gz <= " + A + "_nz";
-      std::string raw = A + "[(ps_gx - 1) + (ps_gy - 1)*" + A + "_sx + (ps_gz - 1)*" + A + "_sxy]";
-      std::string v = "((" + cond + ") ? " + raw + " : (" + elem_cty(dt) + ")0)";
-      if (dt == PS_BF16) v = "ps_bf16_ld(" + v + ")";
-      o << pad() << t.name << "[ps_buf][ps_hy * " << t.W << " + ps_hx] = " << v << ";\n";
-      --ind;
-      o << pad() << "}\n";
     }
     o << pad() << "__syncthreads();\n";
   }
@@ -1060,7 +1121,7 @@ void Emitter::emit_memopt_body(int L) {
   for (auto& t : tiles)
     for (auto& p : t.cols_at) {
       auto key = std::make_tuple(t.ai, p.first, p.second, t.oz);
-      if (cache.count(key)) continue;  // the thread's own register value
+      if (cache.count(key)) continue;  // a register of the thread's own column
       cache[key] = t.name + "[ps_buf][(ps_ly + " + std::to_string(p.second - t.ylo) + ") * " + std::to_string(t.W) + " + ps_lx + " +
                    std::to_string(p.first - t.xlo) + "]";
     }

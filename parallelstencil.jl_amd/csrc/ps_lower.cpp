@@ -299,6 +299,7 @@ static void read_options(Options& o, const JVal& j) {
         o.optranges[e.first] = r;
       }
     } else if (k == "nthreads_max_memopt") o.nthreads_max_memopt = (int)v.n;
+    else if (k == "nthreads_x_max_memopt") o.nthreads_x_max_memopt = (int)v.n;
     else if (k == "fp_contract") o.fp_contract = v.b;
     else if (k == "lds") o.lds = v.b;
     else if (k == "xcd_remap") o.xcd_remap = v.b;
@@ -485,8 +486,8 @@ static void memopt_analysis(Kernel& K) {
   for (auto& A : optvars)
     if (std::find(readonly.begin(), readonly.end(), A) == readonly.end())
       throw PsError(PS_EINCOHERENT, "incoherent argument memopt in @parallel[_indices] <kernel>: optimization can only be applied to arrays that are only read within the kernel (not applicable to: " + A + ").");
-  // offsets of stencil accesses
-  for (auto& A : optvars) mm.offsets[A];
+  // offsets of stencil accesses (of every read-only array)
+  for (auto& A : readonly) mm.offsets[A];
   walk(K.body, [&](const NodeP& x) {
     if (!(x->k == NK::Index && x->a[0]->k == NK::Ident && mm.offsets.count(x->a[0]->s))) return;
     if ((int)x->a.size() - 1 < nd) return;
@@ -524,8 +525,12 @@ static void memopt_analysis(Kernel& K) {
     throw PsError(PS_EINCOHERENT,
                   "incoherent argument memopt in @parallel[_indices] <kernel>: optimization can only be applied if there is at least one array that is read-only within the kernel (and accessed with a multi-point stencil). Set memopt=false for this kernel.");
   for (auto it = mm.offsets.begin(); it != mm.offsets.end();) {
-    if (std::find(mm.optvars.begin(), mm.optvars.end(), it->first) == mm.optvars.end()) it = mm.offsets.erase(it);
-    else ++it;
+    if (std::find(mm.optvars.begin(), mm.optvars.end(), it->first) == mm.optvars.end()) {
+      if (!it->second.empty()) mm.ro_offsets[it->first] = it->second;
+      it = mm.offsets.erase(it);
+    } else {
+      ++it;
+    }
   }
   for (auto& A : mm.optvars) {
     std::vector<std::pair<int, int>> sr(3, {0, 0});

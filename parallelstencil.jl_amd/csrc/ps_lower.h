@@ -29,6 +29,8 @@ struct MemoptMeta {
   std::map<std::string, std::vector<std::pair<int, int>>> optranges;  // explicit optranges
   // per optvar: offsets keyed by (ox,oy) -> oz -> count (loopdim 3) or (ox) -> oy (loopdim 2)
   std::map<std::string, std::map<std::pair<int, int>, std::map<int, int>>> offsets;
+  // same for the read-only arrays that are not optvars (register prefetch only)
+  std::map<std::string, std::map<std::pair<int, int>, std::map<int, int>>> ro_offsets;
   std::vector<std::string> shmem_optvars;
   std::map<std::string, bool> use_shmemhalos;
 };
@@ -45,7 +47,10 @@ struct Options {
   int loopdim = 0;
   std::vector<std::string> optvars;
   std::map<std::string, std::vector<std::pair<int, int>>> optranges;
-  int nthreads_max_memopt = 256;
+  // memopt block shape limits: 128 x 4 threads streaming along z measured fastest on gfx950
+  // (wider rows of HBM per wave, smaller LDS halo share than 64 x 4)
+  int nthreads_max_memopt = 512;
+  int nthreads_x_max_memopt = 128;
   int nthreads_x_max = 64;
   int nthreads_max = 256;
   bool fp_contract = false;               // Julia semantics: no FMA contraction

@@ -850,6 +850,10 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   bool memopt = K.mm.enabled;
   if (L && L->memopt == 1 && !memopt) throw PsError(PS_EINCOHERENT, "@parallel memopt=true: the kernel was not defined with memopt=true");
   if (L && L->memopt == 0) memopt = false;
+  // memopt kernels load branch-free from a clamped index (element 0 of the array stands in for
+  // an access outside it), which needs every array to hold at least one element
+  for (int i = 0; i < nargs && memopt; ++i)
+    if (args[i].kind == PS_ARG_FIELD && args[i].field.dims[0] * args[i].field.dims[1] * args[i].field.dims[2] == 0) memopt = false;
   P.memopt = memopt;
   int64_t maxsize[3] = {1, 1, 1};
   int maxnd = 1;
@@ -885,7 +889,7 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
     if (K.opt.kind == "parallel" && !user_ranges) {
       // get_ranges_memopt: pad to multiples of nthreads (src/parallel.jl:560-571)
       int32_t nt[3];
-      ps_compute_nthreads(len, K.opt.nthreads_x_max, K.opt.nthreads_max_memopt, mm.loopdim, nt);
+      ps_compute_nthreads(len, K.opt.nthreads_x_max_memopt, K.opt.nthreads_max_memopt, mm.loopdim, nt);
       for (int d = 0; d < 3; ++d) {
         int64_t rest = len[d] % nt[d];
         if (rest) len[d] += nt[d] - rest;
@@ -900,7 +904,11 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
         P.nblocks[d] = L->nblocks[d] ? L->nblocks[d] : (int32_t)((ms[d] + P.nthreads[d] - 1) / P.nthreads[d]);
       }
     } else {
-      ps_compute_nthreads(ms, K.opt.nthreads_x_max, K.opt.nthreads_max_memopt, mm.loopdim, P.nthreads);
+      ps_compute_nthreads(ms, K.opt.nthreads_x_max_memopt, K.opt.nthreads_max_memopt, mm.loopdim, P.nthreads);
+      // sizes that the tuned block does not divide get the reference's AMDGPU block limits
+      // (64 threads along x, 256 per block: src/shared.jl:29, src/ParallelKernel/shared.jl:33)
+      if (ms[0] % P.nthreads[0] || ms[1] % P.nthreads[1] || ms[2] % P.nthreads[2])
+        ps_compute_nthreads(ms, K.opt.nthreads_x_max, 256, mm.loopdim, P.nthreads);
       // compute_nthreads_memopt checks (src/parallel.jl:548-558)
       for (auto& A : mm.optvars) {
         const auto& sr = mm.stencilranges.at(A);
