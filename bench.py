@@ -39,6 +39,21 @@ def time_kernel(ps, launch, steps, warmup):
     return (t1 - t0) * 1e3 / steps, e0.elapsed_ms(e1) / steps
 
 
+def _traffic(n, head):
+    """HBM bytes per launch of the head kernel, from the committed rocprofv3 counter summary
+    (FETCH_SIZE scaled by the triad calibration + WRITE_SIZE); None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"diffusion3d_{n}_bench.md")
+    if head == "triad" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 3 and p[0] == "hbm_bytes_per_launch" and p[1].startswith("psk_diffusion3D") and \
+                    (("_memopt" in p[1]) == (head == "memopt")):
+                return int(p[2])
+    return None
+
+
 def cpu_baseline(n, budget_s=12.0):
     """The oracle's diffusion step (Threads-backend loop nest, OpenMP over z) on the host cores."""
     from oracle import ps_oracle as O
@@ -120,11 +135,7 @@ def main():
     ms, kms = res[head]
     teff_wall = bytes_step / (ms * 1e-3) / 1e9
     achieved = bytes_step / (kms * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"diffusion3d_{n}_{head}_traffic.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic = _traffic(n, head)
     out = {
         "metric": "T_eff GB/s, %HBM peak (diffusion3D 512³); Stokes2D time-to-tol: warm vs plain",
         "value": round(teff_wall, 2),

@@ -207,6 +207,17 @@ int ps_event_record(void* ev, void* stream);
 int ps_event_elapsed_ms(void* ev0, void* ev1, float* ms);
 int ps_event_destroy(void* ev);
 
+/* ---- graphs (no reference counterpart: the reference launches each @parallel call on its
+ * own, src/ParallelKernel/parallel.jl:883-927; these batch launch-bound loops such as the
+ * pseudo-transient iteration of miniapps/Stokes2D.jl:109-125) ----
+ * ps_graph_begin starts capturing the launches issued on `stream` (launches must be
+ * asynchronous and their kernels already compiled: a blocking launch fails during capture);
+ * ps_graph_end instantiates the captured work; ps_graph_launch replays it on a stream. */
+int ps_graph_begin(void* stream);
+int ps_graph_end(void* stream, void** exec);
+int ps_graph_launch(void* exec, void* stream);
+int ps_graph_destroy(void* exec);
+
 /* ---- kernel cache ---- */
 /* Directory of the content-addressed code-object cache (default: $PS_AMD_CACHE or
  * the kcache/ folder beside the library). */

@@ -1146,7 +1146,8 @@ void Emitter::emit_memopt_body(int L) {
 
 Emitted Emitter::run() {
   Emitted E;
-  std::string entry = "psk_" + sanitize(K.def.name);
+  // variants carry their execution path in the symbol so that profiles tell them apart
+  std::string entry = "psk_" + sanitize(K.def.name) + (V.memopt ? (V.lds ? "_memopt_lds" : "_memopt") : "") + (V.checked ? "_checked" : "");
   E.entry = entry;
   const char* I = V.idx64 ? "long long" : "int";
   int ib = V.idx64 ? 8 : 4;

@@ -17,6 +17,7 @@
 #include <fstream>
 #include <map>
 #include <memory>
+#include <unordered_map>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -167,10 +168,16 @@ struct Module {
   Emitted em;
 };
 
+struct LaunchEntry;
+
 struct KernelObj {
   Kernel K;
   std::mutex mu;
   std::map<std::string, std::shared_ptr<Module>> variants;  // key: kinds+variant
+  // launch plans (geometry, variant, module) by argument signature: repeated launches of the
+  // same shapes skip the planning, the bounds proof and the variant lookup
+  std::mutex lmu;
+  std::unordered_map<std::string, std::shared_ptr<LaunchEntry>> launches;
 };
 
 struct State {
@@ -644,6 +651,38 @@ extern "C" int ps_set_cache_dir(const char* dir) {
 }
 
 // ---------------------------------------------------------------------------
+// graphs: capture the asynchronous launches issued on a stream, replay them as one launch
+// ---------------------------------------------------------------------------
+extern "C" int ps_graph_begin(void* stream) {
+  if (int r = require_init()) return r;
+  PS_HIP(hipStreamBeginCapture(stream_of(stream), hipStreamCaptureModeRelaxed));
+  return 0;
+}
+
+extern "C" int ps_graph_end(void* stream, void** exec) {
+  if (int r = require_init()) return r;
+  hipGraph_t g = nullptr;
+  PS_HIP(hipStreamEndCapture(stream_of(stream), &g));
+  hipGraphExec_t x = nullptr;
+  hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(PS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  *exec = (void*)x;
+  return 0;
+}
+
+extern "C" int ps_graph_launch(void* exec, void* stream) {
+  if (int r = require_init()) return r;
+  PS_HIP(hipGraphLaunch((hipGraphExec_t)exec, stream_of(stream)));
+  return 0;
+}
+
+extern "C" int ps_graph_destroy(void* exec) {
+  PS_HIP(hipGraphExecDestroy((hipGraphExec_t)exec));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // geometry helpers
 // ---------------------------------------------------------------------------
 extern "C" int ps_compute_nthreads(const int64_t maxsize[3], int nthreads_x_max, int nthreads_max, int flatdim, int32_t out[3]) {
@@ -998,6 +1037,63 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   return P;
 }
 
+struct LaunchEntry {
+  Plan P;
+  std::shared_ptr<Module> m;  // null for an empty launch box
+};
+
+// Everything plan_launch and the variant depend on: argument kinds and types, field shapes and
+// pitches, integer scalar values (bounds proof), the division mode of float scalars, which
+// field arguments overlap in memory (aliasing), and the launch request.
+std::string launch_key(const ps_arg* args, int nargs, const ps_launch* L) {
+  std::string k;
+  k.reserve(32 + 48 * (size_t)nargs);
+  auto put = [&](const void* p, size_t n) { k.append((const char*)p, n); };
+  for (int i = 0; i < nargs; ++i) {
+    const ps_arg& a = args[i];
+    put(&a.kind, 4);
+    put(&a.dtype, 4);
+    if (a.kind == PS_ARG_FIELD) {
+      put(a.field.dims, 24);
+      put(&a.field.pitch, 8);
+      put(&a.field.plane, 8);
+      put(&a.field.ndims, 4);
+    } else if (a.dtype == PS_F64 || a.dtype == PS_F32) {
+      char m = 'x';
+      if (a.dtype == PS_F64) m = rcp_safe64(a.f) ? (a.f > 0 ? 'p' : 'n') : 'x';
+      else m = rcp_safe32((float)a.f) ? ((float)a.f > 0 ? 'p' : 'n') : 'x';
+      k.push_back(m);
+    } else {
+      put(&a.i, 8);
+    }
+  }
+  for (int i = 0; i < nargs; ++i) {
+    if (args[i].kind != PS_ARG_FIELD) continue;
+    const ps_field& a = args[i].field;
+    const char* a0 = (const char*)a.ptr;
+    const char* a1 = a0 + (a.plane * std::max<int64_t>(a.dims[2], 1)) * (int64_t)elsize(a.dtype);
+    for (int j = i + 1; j < nargs; ++j) {
+      if (args[j].kind != PS_ARG_FIELD) continue;
+      const ps_field& b = args[j].field;
+      const char* b0 = (const char*)b.ptr;
+      const char* b1 = b0 + (b.plane * std::max<int64_t>(b.dims[2], 1)) * (int64_t)elsize(b.dtype);
+      if (a0 < b1 && b0 < a1) {
+        int16_t pr[2] = {(int16_t)i, (int16_t)j};
+        put(pr, 4);
+      }
+    }
+  }
+  k.push_back('|');
+  if (L) {
+    put(L->ranges, sizeof(L->ranges));
+    put(&L->has_ranges, 4);
+    put(L->nthreads, 12);
+    put(L->nblocks, 12);
+    put(&L->memopt, 4);
+  }
+  return k;
+}
+
 }  // namespace
 
 extern "C" int ps_kernel_geometry(ps_kernel k, const ps_arg* args, int nargs, const ps_launch* launch, int32_t nthreads[3], int32_t nblocks[3],
@@ -1031,11 +1127,28 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
   if (int r = require_init()) return r;
   PS_GUARD_BEGIN
   KernelObj* ko = (KernelObj*)k;
-  std::vector<ArgKind> kinds;
-  Plan P = plan_launch(ko->K, args, nargs, launch, kinds);
-  for (int d = 0; d < 3; ++d)
-    if (P.nblocks[d] == 0) return 0;  // empty range
-  auto m = get_module(ko, kinds, P.v, true);
+  std::string key = launch_key(args, nargs, launch);
+  std::shared_ptr<LaunchEntry> le;
+  {
+    std::lock_guard<std::mutex> g(ko->lmu);
+    auto it = ko->launches.find(key);
+    if (it != ko->launches.end()) le = it->second;
+  }
+  if (!le) {
+    le = std::make_shared<LaunchEntry>();
+    std::vector<ArgKind> kinds;
+    le->P = plan_launch(ko->K, args, nargs, launch, kinds);
+    bool empty = false;
+    for (int d = 0; d < 3; ++d)
+      if (le->P.nblocks[d] == 0) empty = true;
+    if (!empty) le->m = get_module(ko, kinds, le->P.v, true);
+    std::lock_guard<std::mutex> g(ko->lmu);
+    if (ko->launches.size() >= 256) ko->launches.clear();
+    ko->launches[key] = le;
+  }
+  if (!le->m) return 0;  // empty range
+  const Plan& P = le->P;
+  const auto& m = le->m;
   // pack parameters
   std::vector<unsigned long long> slots(m->em.params.size());
   std::vector<void*> ptrs(m->em.params.size());

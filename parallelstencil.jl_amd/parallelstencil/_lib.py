@@ -170,6 +170,10 @@ _SIGS = {
     "ps_event_elapsed_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _P(ctypes.c_float)]),
     "ps_event_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_set_cache_dir": (ctypes.c_int, [ctypes.c_char_p]),
+    "ps_graph_begin": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_graph_end": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_void_p)]),
+    "ps_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "ps_graph_destroy": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 

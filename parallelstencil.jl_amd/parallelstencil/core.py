@@ -472,9 +472,70 @@ class Event:
             pass
 
 
+class Graph:
+    """HIP graph of the asynchronous launches issued on a stream inside `with graph:`; `launch()`
+    replays them with one launch. Kernels must have run once (compiled) before the capture."""
+
+    def __init__(self, stream=None):
+        ensure_device()
+        self.stream = stream
+        self.h = None
+
+    def __enter__(self):
+        check(lib.ps_graph_begin(_stream_handle(self.stream)))
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = ctypes.c_void_p()
+        rc = lib.ps_graph_end(_stream_handle(self.stream), ctypes.byref(h))
+        if exc_type is None:
+            check(rc)
+            self.h = h
+        elif rc == 0:
+            lib.ps_graph_destroy(h)
+        return False
+
+    def launch(self, stream=None):
+        if self.h is None:
+            raise ArgumentError("the graph has not been captured")
+        check(lib.ps_graph_launch(self.h, _stream_handle(self.stream if stream is None else stream)))
+
+    def __del__(self):
+        try:
+            if self.h is not None:
+                lib.ps_graph_destroy(self.h)
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------------
 # kernels
 # ---------------------------------------------------------------------------
+class BoundLaunch:
+    """A kernel launch with a fixed argument block (see Kernel.launcher)."""
+
+    def __init__(self, kernel, args, launch):
+        self.kernel = kernel
+        self.args = list(args)
+        self._launch = launch
+        self._build()
+
+    def _build(self):
+        k = self.kernel
+        self._arr = k._args(self.args)
+        self._L = k._launch_struct(**self._launch)
+        self._Lp = ctypes.byref(self._L)
+        self._n = len(self.args)
+
+    def set(self, i, value):
+        """Replaces argument i (a field or a scalar)."""
+        self.args[i] = value
+        self._build()
+
+    def __call__(self):
+        rc = lib.ps_kernel_launch(self.kernel._h, self._arr, self._n, self._Lp)
+        if rc < 0:
+            check(rc)
 def _range3(r):
     """Normalises ranges: (start, stop) pairs (Julia start:stop) per dimension, 1-D/2-D promoted."""
     if isinstance(r, UnitRange):
@@ -630,6 +691,14 @@ class Kernel:
         arr = self._args(args)
         L = self._launch_struct(ranges, nblocks, nthreads, stream, async_, memopt)
         check(lib.ps_kernel_launch(self._h, arr, len(args), ctypes.byref(L)))
+
+    def launcher(self, *args, ranges=None, nblocks=None, nthreads=None, stream=None, async_=True, memopt=None):
+        """A prepared launch of this kernel with these arguments (asynchronous by default):
+        calling it launches again without rebuilding the argument block, for launch-bound loops
+        and graph capture. It keeps its arguments alive; `set(i, value)` replaces one."""
+        ensure_device()
+        return BoundLaunch(self, args, dict(ranges=ranges, nblocks=nblocks, nthreads=nthreads, stream=stream,
+                                            async_=async_, memopt=memopt))
 
     def geometry(self, *args, ranges=None, nblocks=None, nthreads=None, memopt=None):
         """(memopt, nthreads, nblocks, ranges) the launch would use."""

@@ -90,3 +90,287 @@ def define_diffusion3d(memopt, module, **kw):
 
 def define_triad3d(module):
     return core._define("parallel", TRIAD3D, {}, module)
+
+
+# ------------------------------------------------------------------ Stokes 2-D (pseudo-transient)
+# miniapps/Stokes2D.jl:11-42 (2-D module). The boundary kernels of :44-54 declare (ix,iy) and use
+# one index, which src/parallel.jl:706-713 rejects; they are written with the one index they use
+# and launched over that dimension of the array.
+STOKES2D_TIMESTEPS = """
+@parallel function compute_timesteps!(dτVx::Data.Array, dτVy::Data.Array, dτPt::Data.Array, Mus::Data.Array, Vsc::Data.Number, Ptsc::Data.Number, min_dxy2::Data.Number, max_nxy::Int)
+    @all(dτVx) = Vsc*min_dxy2/@av_xi(Mus)/4.1
+    @all(dτVy) = Vsc*min_dxy2/@av_yi(Mus)/4.1
+    @all(dτPt) = Ptsc*4.1*@all(Mus)/max_nxy
+    return
+end
+"""
+STOKES2D_P = """
+@parallel function compute_P!(∇V::Data.Array, Pt::Data.Array, Vx::Data.Array, Vy::Data.Array, dτPt::Data.Array, dx::Data.Number, dy::Data.Number)
+    @all(∇V)  = @d_xa(Vx)/dx + @d_ya(Vy)/dy
+    @all(Pt)  = @all(Pt) - @all(dτPt)*@all(∇V)
+    return
+end
+"""
+STOKES2D_TAU = """
+@parallel function compute_τ!(∇V::Data.Array, τxx::Data.Array, τyy::Data.Array, τxy::Data.Array, Vx::Data.Array, Vy::Data.Array, Mus::Data.Array, dx::Data.Number, dy::Data.Number)
+    @all(τxx) = 2.0*@all(Mus)*(@d_xa(Vx)/dx - 1.0/3.0*@all(∇V))
+    @all(τyy) = 2.0*@all(Mus)*(@d_ya(Vy)/dy - 1.0/3.0*@all(∇V))
+    @all(τxy) = 2.0*@av(Mus)*(0.5*(@d_yi(Vx)/dy + @d_xi(Vy)/dx))
+    return
+end
+"""
+STOKES2D_DV = """
+@parallel function compute_dV!(Rx::Data.Array, Ry::Data.Array, dVxdτ::Data.Array, dVydτ::Data.Array, Pt::Data.Array, Rog::Data.Array, τxx::Data.Array, τyy::Data.Array, τxy::Data.Array, dampX::Data.Number, dampY::Data.Number, dx::Data.Number, dy::Data.Number)
+    @all(Rx)    = @d_xi(τxx)/dx + @d_ya(τxy)/dy - @d_xi(Pt)/dx
+    @all(Ry)    = @d_yi(τyy)/dy + @d_xa(τxy)/dx - @d_yi(Pt)/dy + @av_yi(Rog)
+    @all(dVxdτ) = dampX*@all(dVxdτ) + @all(Rx)
+    @all(dVydτ) = dampY*@all(dVydτ) + @all(Ry)
+    return
+end
+"""
+STOKES2D_V = """
+@parallel function compute_V!(Vx::Data.Array, Vy::Data.Array, dVxdτ::Data.Array, dVydτ::Data.Array, dτVx::Data.Array, dτVy::Data.Array)
+    @inn(Vx) = @inn(Vx) + @all(dτVx)*@all(dVxdτ)
+    @inn(Vy) = @inn(Vy) + @all(dτVy)*@all(dVydτ)
+    return
+end
+"""
+STOKES2D_BC_X = """
+@parallel_indices (iy) function bc_x!(A::Data.Array)
+    A[1  , iy] = A[2    , iy]
+    A[end, iy] = A[end-1, iy]
+    return
+end
+"""
+STOKES2D_BC_Y = """
+@parallel_indices (ix) function bc_y!(A::Data.Array)
+    A[ix, 1  ] = A[ix, 2    ]
+    A[ix, end] = A[ix, end-1]
+    return
+end
+"""
+
+
+def stokes2d_inclusions(nx, ny, seed, lx=10.0, ly=10.0, mu0=1.0, log10_contrast=3.0):
+    """Viscosity and body force of one Stokes sample on the miniapp's node grid
+    (x = (ix-1)dx, dx = lx/(nx-1)): 1-8 circular inclusions with centres uniform in the box,
+    radii uniform [0.3, 1.5], log10 viscosity contrast uniform [-c, c] (c = log10_contrast),
+    rho*g = 1 inside and 0 outside (later inclusions overwrite earlier ones). SplitMix64 stream
+    `seed`. The miniapp's pseudo-transient iteration converges for contrasts up to about one
+    order of magnitude (c = 1) and diverges from about 10^1.5 on."""
+    import numpy as np
+    u = splitmix64_uniform(1 + 4 * 8, seed)
+    ninc = 1 + int(u[0] * 8)
+    dx, dy = lx / (nx - 1), ly / (ny - 1)
+    x = (np.arange(nx) * dx)[:, None]
+    y = (np.arange(ny) * dy)[None, :]
+    Mus = np.full((nx, ny), mu0, order="F")
+    Rog = np.zeros((nx, ny), order="F")
+    for i in range(ninc):
+        cx, cy, r = u[1 + 4 * i] * lx, u[2 + 4 * i] * ly, 0.3 + 1.2 * u[3 + 4 * i]
+        c = log10_contrast * (2.0 * u[4 + 4 * i] - 1.0)
+        m = (x - cx) ** 2 + (y - cy) ** 2 < r * r
+        Mus[m] = mu0 * 10.0 ** c
+        Rog[m] = 1.0
+    return Mus, Rog
+
+
+STOKES2D_MAXLOC_X = """
+@parallel_indices (ix,iy) function maxloc_x!(Mx::Data.Array, Mus::Data.Array)
+    Mx[ix,iy] = max(max(Mus[max(ix-1,1),iy], Mus[ix,iy]), Mus[min(ix+1,size(Mus,1)),iy])
+    return
+end
+"""
+STOKES2D_MAXLOC_Y = """
+@parallel_indices (ix,iy) function maxloc_y!(My::Data.Array, Mx::Data.Array)
+    My[ix,iy] = max(max(Mx[ix,max(iy-1,1)], Mx[ix,iy]), Mx[ix,min(iy+1,size(Mx,2))])
+    return
+end
+"""
+
+
+class Stokes2D:
+    """The pseudo-transient Stokes solver of miniapps/Stokes2D.jl:62-125 on the GPU.
+
+    Launches are prepared once (Kernel.launcher) and issued asynchronously; with `graph=True`
+    each block of `nout` iterations between two residual checks is one HIP graph replay. The
+    residual check (mean |Rx|, mean |Ry|, mean |∇V|, :117-123) is a deterministic FP64 reduction
+    on the GPU.
+
+    Pseudo-time steps: `eta_ts="av"` (default) is the miniapp's compute_timesteps! on the
+    viscosity (two-point averages @av_xi / @av_yi, :11-16). `eta_ts="maxloc"` feeds the same
+    kernel the 3x3 local maximum of the viscosity instead. Neither makes the iteration converge
+    for inclusion contrasts beyond about one order of magnitude."""
+
+    def __init__(self, module, nx, ny, lx=10.0, ly=10.0, Vdmp=4.0, Vsc=1.0, Ptsc=0.25, eps=1e-8, nout=200, itermax=500000,
+                 eta_ts="av", graph=True):
+        self.nx, self.ny = nx, ny
+        self.dx, self.dy = lx / (nx - 1), ly / (ny - 1)
+        self.min_dxy2 = min(self.dx, self.dy) ** 2
+        self.max_nxy = max(nx, ny)
+        self.dampX, self.dampY = 1.0 - Vdmp / nx, 1.0 - Vdmp / ny
+        self.Vsc, self.Ptsc, self.eps, self.nout, self.itermax = Vsc, Ptsc, eps, nout, itermax
+        self.eta_ts, self.use_graph = eta_ts, graph
+        d = lambda text, kind="parallel": core._define(kind, text, {}, module)
+        self.k_ts, self.k_p, self.k_tau = d(STOKES2D_TIMESTEPS), d(STOKES2D_P), d(STOKES2D_TAU)
+        self.k_dv, self.k_v = d(STOKES2D_DV), d(STOKES2D_V)
+        self.k_bcx, self.k_bcy = d(STOKES2D_BC_X, "parallel_indices"), d(STOKES2D_BC_Y, "parallel_indices")
+        self.k_mx, self.k_my = d(STOKES2D_MAXLOC_X, "parallel_indices"), d(STOKES2D_MAXLOC_Y, "parallel_indices")
+        z = lambda *s: core.zeros(*s, eltype=core.Float64)
+        self.Pt, self.dtPt, self.divV = z(nx, ny), z(nx, ny), z(nx, ny)
+        self.Vx, self.Vy = z(nx + 1, ny), z(nx, ny + 1)
+        self.txx, self.tyy, self.txy = z(nx, ny), z(nx, ny), z(nx - 1, ny - 1)
+        self.Rx, self.Ry = z(nx - 1, ny - 2), z(nx - 2, ny - 1)
+        self.dVxdt, self.dVydt = z(nx - 1, ny - 2), z(nx - 2, ny - 1)
+        self.dtVx, self.dtVy = z(nx - 1, ny - 2), z(nx - 2, ny - 1)
+        self.Mus, self.Rog, self.Mts, self.Mtmp = z(nx, ny), z(nx, ny), z(nx, ny), z(nx, ny)
+        self._steps = [
+            self.k_p.launcher(self.divV, self.Pt, self.Vx, self.Vy, self.dtPt, self.dx, self.dy),
+            self.k_tau.launcher(self.divV, self.txx, self.tyy, self.txy, self.Vx, self.Vy, self.Mus, self.dx, self.dy),
+            self.k_dv.launcher(self.Rx, self.Ry, self.dVxdt, self.dVydt, self.Pt, self.Rog, self.txx, self.tyy, self.txy,
+                               self.dampX, self.dampY, self.dx, self.dy),
+            self.k_v.launcher(self.Vx, self.Vy, self.dVxdt, self.dVydt, self.dtVx, self.dtVy),
+            self.k_bcy.launcher(self.Vx, ranges=((1, nx + 1),)),
+            self.k_bcx.launcher(self.Vy, ranges=((1, ny + 1),)),
+        ]
+        self._graph = None
+        # compile and load every kernel now (all fields are zero: the iteration maps zeros to
+        # zeros), so that graphs can be captured later
+        self.iterate(1)
+        core.synchronize()
+
+    def set_material(self, Mus, Rog):
+        self.Mus.copy_from(Mus)
+        self.Rog.copy_from(Rog)
+        M = self.Mus
+        if self.eta_ts == "maxloc":
+            self.k_mx(self.Mtmp, self.Mus)
+            self.k_my(self.Mts, self.Mtmp)
+            M = self.Mts
+        self.k_ts(self.dtVx, self.dtVy, self.dtPt, M, self.Vsc, self.Ptsc, self.min_dxy2, self.max_nxy)
+
+    def reset(self, Vx=None, Vy=None, Pt=None):
+        """Zero state, or a warm start (host arrays of the shapes of Vx, Vy, Pt)."""
+        for A in (self.Pt, self.Vx, self.Vy, self.divV, self.txx, self.tyy, self.txy, self.Rx, self.Ry, self.dVxdt, self.dVydt):
+            A.fill_(0.0)
+        if Vx is not None:
+            self.Vx.copy_from(Vx)
+            self.Vy.copy_from(Vy)
+            self.Pt.copy_from(Pt)
+
+    def iterate(self, n):
+        """n PT iterations (miniapps/Stokes2D.jl:111-116), asynchronous."""
+        steps = self._steps
+        for _ in range(n):
+            for s in steps:
+                s()
+
+    def iterate_block(self):
+        """nout iterations: one graph replay (captured on first use) or nout eager iterations."""
+        if not self.use_graph:
+            self.iterate(self.nout)
+            return
+        if self._graph is None:
+            core.synchronize()
+            g = core.Graph()
+            with g:
+                self.iterate(self.nout)
+            self._graph = g
+        self._graph.launch()
+
+    def error(self):
+        """max(mean |Rx|, mean |Ry|, mean |∇V|), FP64 on the GPU."""
+        return max(self.Rx.reduce("mean_abs"), self.Ry.reduce("mean_abs"), self.divV.reduce("mean_abs"))
+
+    def solve(self):
+        """Iterates until the error check (every nout iterations) is below eps; returns the
+        number of iterations and the final error."""
+        it, err = 0, float("inf")
+        while it < self.itermax:
+            self.iterate_block()
+            it += self.nout
+            err = self.error()
+            if err <= self.eps or not err < 1e30:  # converged, or diverged (inf / NaN)
+                break
+        return it, err
+
+
+# ------------------------------------------------------------------ acoustic waves
+def acoustic3d_inputs(n, lx=40.0, k=1.0, rho=1.0):
+    """Initial state of miniapps/acoustic3D.jl:24-40 on n^3 pressure nodes (dx = lx/(n-1)):
+    P0 = exp(-((ix-1)dx - lx/2)^2 - ...), V0 = 0, dt = min(dx,dy,dz)/sqrt(k/rho)/6.1."""
+    import numpy as np
+    dx = lx / (n - 1)
+    x = np.arange(n) * dx - 0.5 * lx
+    X2 = x * x
+    P = np.empty((n, n, n), order="F")
+    for kz in range(n):
+        # exp(-a^2 - b^2 - c^2), left to right as the miniapp writes it
+        P[:, :, kz] = np.exp(((-X2[:, None]) - X2[None, :]) - X2[kz])
+    dt = min(dx, dx, dx) / np.sqrt(k / rho) / 6.1
+    return P, dict(dt=dt, rho=rho, k=k, dx=dx, dy=dx, dz=dx)
+
+
+def acoustic2d_inputs(n, lx=40.0, k=1.0, rho=1.0):
+    """Initial state of miniapps/acoustic2D.jl (same construction as the 3-D case)."""
+    import numpy as np
+    dx = lx / (n - 1)
+    x = np.arange(n) * dx - 0.5 * lx
+    P = np.asfortranarray(np.exp((-(x * x)[:, None]) - (x * x)[None, :]))
+    dt = min(dx, dx) / np.sqrt(k / rho) / 6.1
+    return P, dict(dt=dt, rho=rho, k=k, dx=dx, dy=dx)
+
+
+def define_acoustic3d(module, memopt=False, **kw):
+    """(compute_V!, compute_P!) of miniapps/acoustic3D.jl:11-21."""
+    kw = dict(kw, memopt=memopt)
+    return core._define("parallel", ACOUSTIC3D_V, kw, module), core._define("parallel", ACOUSTIC3D_P, kw, module)
+
+
+def define_acoustic2d(module, memopt=False, **kw):
+    """(compute_V!, compute_P!) of miniapps/acoustic2D.jl:11-20."""
+    kw = dict(kw, memopt=memopt)
+    return core._define("parallel", ACOUSTIC2D_V, kw, module), core._define("parallel", ACOUSTIC2D_P, kw, module)
+
+
+def acoustic3d_run(module, P0, prm, steps, eltype, memopt=False):
+    """Runs `steps` acoustic time steps (V then P, miniapps/acoustic3D.jl:48-52) on the GPU from
+    P0 (float64 host array) in storage type `eltype` and returns host copies (P, Vx, Vy, Vz) as
+    float64 (BFloat16 widened)."""
+    import numpy as np
+    n1, n2, n3 = P0.shape
+    kV, kP = define_acoustic3d(module, memopt)
+    P = core.zeros(n1, n2, n3, eltype=eltype)
+    Vx, Vy, Vz = (core.zeros(n1 + 1, n2, n3, eltype=eltype), core.zeros(n1, n2 + 1, n3, eltype=eltype),
+                  core.zeros(n1, n2, n3 + 1, eltype=eltype))
+    P.copy_from(_to_storage(P0, eltype))
+    for _ in range(steps):
+        kV(Vx, Vy, Vz, P, prm["dt"], prm["rho"], prm["dx"], prm["dy"], prm["dz"], async_=True)
+        kP(P, Vx, Vy, Vz, prm["dt"], prm["k"], prm["dx"], prm["dy"], prm["dz"], async_=True)
+    core.synchronize()
+    return tuple(_from_storage(A.to_host(), eltype) for A in (P, Vx, Vy, Vz))
+
+
+def _to_storage(a, eltype):
+    import numpy as np
+    if eltype == core.BFloat16:
+        return np.asfortranarray(core.f32_to_bf16(np.asarray(a, dtype=np.float32)))
+    if eltype == core.Float32:
+        return np.asfortranarray(a, dtype=np.float32)
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def _from_storage(a, eltype):
+    import numpy as np
+    if eltype == core.BFloat16:
+        return np.asfortranarray(core.bf16_to_f32(a).astype(np.float64))
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def field_errors(got, ref):
+    """(max |got - ref| / max |ref|, ||got - ref||_2 / ||ref||_2) of one field."""
+    import numpy as np
+    d = got - ref
+    mref = float(np.abs(ref).max()) or 1.0
+    nref = float(np.linalg.norm(ref.ravel())) or 1.0
+    return float(np.abs(d).max()) / mref, float(np.linalg.norm(d.ravel())) / nref
