@@ -368,3 +368,68 @@ def diffusion_inputs(n, seed=1, lx=10.0):
     Cp = np.asfortranarray((1.0 + splitmix64_uniform(n ** 3, seed)).reshape((n, n, n), order="F"))
     dt = min(dx * dx, dx * dx, dx * dx) * 1.0 / 1.0 / 8.1
     return T, Cp, dict(lam=1.0, dt=dt, dx=dx, dy=dx, dz=dx)
+
+
+# ------------------------------------------------- acoustic 2-D inversion gradient (adjoint)
+# Forward: miniapps/acoustic2D.jl:11-20 with a bulk-modulus field K (P -= dt*K*div V), a point
+# source added after the pressure update and receivers sampling P^{n+1}. The adjoint has no
+# reference counterpart (the reference differentiates with Enzyme, not available here); it is
+# pinned in tests/test_oracle.py by the dot-product identity <A x, y> = <x, A^T y> of one step and
+# by central finite differences of the misfit along random directions.
+def _ac2_fwd_step(P, Vx, Vy, K, dt, rho, dx, dy):
+    nx, ny = P.shape
+    Vx[1:nx, 1:ny - 1] = Vx[1:nx, 1:ny - 1] - ((dt / rho) * (P[1:nx, 1:ny - 1] - P[0:nx - 1, 1:ny - 1])) / dx
+    Vy[1:nx - 1, 1:ny] = Vy[1:nx - 1, 1:ny] - ((dt / rho) * (P[1:nx - 1, 1:ny] - P[1:nx - 1, 0:ny - 1])) / dy
+    P[:, :] = P - (dt * K) * (((Vx[1:, :] - Vx[:-1, :]) / dx) + ((Vy[:, 1:] - Vy[:, :-1]) / dy))
+
+
+def _ac2_adj_step(lP, lVx, lVy, K, dt, rho, dx, dy):
+    nx, ny = lP.shape
+    W = K * lP
+    lVx[1:nx, 1:ny - 1] = lVx[1:nx, 1:ny - 1] + (dt * (W[1:nx, 1:ny - 1] - W[0:nx - 1, 1:ny - 1])) / dx
+    lVy[1:nx - 1, 1:ny] = lVy[1:nx - 1, 1:ny] + (dt * (W[1:nx - 1, 1:ny] - W[1:nx - 1, 0:ny - 1])) / dy
+    lP[:, :] = lP + (dt / rho) * (((lVx[1:, :] - lVx[:-1, :]) / dx) + ((lVy[:, 1:] - lVy[:, :-1]) / dy))
+
+
+def acoustic2d_step(P, Vx, Vy, K, dt, rho, dx, dy):
+    """One source-free forward step (in place): the linear map A of the adjoint tests."""
+    _ac2_fwd_step(P, Vx, Vy, K, dt, rho, dx, dy)
+
+
+def acoustic2d_step_adjoint(lP, lVx, lVy, K, dt, rho, dx, dy):
+    """A^T of acoustic2d_step (in place)."""
+    _ac2_adj_step(lP, lVx, lVy, K, dt, rho, dx, dy)
+
+
+def acoustic2d_traces(K, nt, dt, rho, dx, dy, src, isx, isy, rx0, rdx, ry, nrec, keep=False):
+    """Forward run from rest; traces D (nrec x nt) of P^{n+1} at the receivers (1-based grid
+    positions rx0 + (r-1) rdx, ry), optionally with every V^{n+1} kept."""
+    nx, ny = K.shape
+    P, Vx, Vy = np.zeros((nx, ny)), np.zeros((nx + 1, ny)), np.zeros((nx, ny + 1))
+    rx = rx0 - 1 + np.arange(nrec) * rdx
+    D = np.zeros((nrec, nt))
+    Vs = []
+    for n in range(nt):
+        _ac2_fwd_step(P, Vx, Vy, K, dt, rho, dx, dy)
+        P[isx - 1, isy - 1] = P[isx - 1, isy - 1] + src[n]
+        D[:, n] = P[rx, ry - 1]
+        if keep:
+            Vs.append((Vx.copy(), Vy.copy()))
+    return (D, Vs) if keep else D
+
+
+def acoustic2d_misfit_gradient(K, Dobs, nt, dt, rho, dx, dy, src, isx, isy, rx0, rdx, ry, nrec):
+    """(J, dJ/dK) for J = 1/2 sum (D - Dobs)^2: every state kept, adjoint swept backwards."""
+    nx, ny = K.shape
+    D, Vs = acoustic2d_traces(K, nt, dt, rho, dx, dy, src, isx, isy, rx0, rdx, ry, nrec, keep=True)
+    r = D - Dobs
+    J = 0.5 * float(np.sum(r * r))
+    rx = rx0 - 1 + np.arange(nrec) * rdx
+    lP, lVx, lVy = np.zeros((nx, ny)), np.zeros((nx + 1, ny)), np.zeros((nx, ny + 1))
+    gK = np.zeros((nx, ny))
+    for n in reversed(range(nt)):
+        lP[rx, ry - 1] = lP[rx, ry - 1] + (D[:, n] - Dobs[:, n])
+        Vx1, Vy1 = Vs[n]
+        gK[:, :] = gK - (dt * lP) * (((Vx1[1:, :] - Vx1[:-1, :]) / dx) + ((Vy1[:, 1:] - Vy1[:, :-1]) / dy))
+        _ac2_adj_step(lP, lVx, lVy, K, dt, rho, dx, dy)
+    return J, gK

@@ -286,3 +286,56 @@ def test_bc_kernels():
 def test_mean_abs():
     a = F(np.array([[1.0, -2.0], [3.0, -4.0]]))
     assert O.mean_abs(a) == 2.5
+
+
+# ------------------------------------------------- acoustic 2-D adjoint (no reference counterpart)
+def _ac2_case(nx=22, ny=19, nt=50):
+    rng = np.random.default_rng(5)
+    K = 1.0 + 0.1 * rng.random((nx, ny))
+    prm = dict(dt=0.05, rho=1.0, dx=0.3, dy=0.35)
+    t = (np.arange(nt) + 1) * prm["dt"] - 0.8
+    src = prm["dt"] * (1 - 2 * (3 * t) ** 2) * np.exp(-(3 * t) ** 2)
+    geo = dict(isx=9, isy=7, rx0=3, rdx=2, ry=15, nrec=8)
+    return K, prm, src, geo, nt
+
+
+def _zero_boundary_v(Vx, Vy):
+    Vx[0], Vx[-1], Vx[:, 0], Vx[:, -1] = 0, 0, 0, 0
+    Vy[0], Vy[-1], Vy[:, 0], Vy[:, -1] = 0, 0, 0, 0
+
+
+def test_acoustic2d_adjoint_step_dot_product():
+    """<A x, y> = <x, A^T y> for one step (velocity boundary nodes zero: they are constant)."""
+    nx, ny = 22, 19
+    rng = np.random.default_rng(1)
+    K = 1.0 + 0.1 * rng.random((nx, ny))
+    x = [rng.standard_normal((nx, ny)), rng.standard_normal((nx + 1, ny)), rng.standard_normal((nx, ny + 1))]
+    y = [rng.standard_normal((nx, ny)), rng.standard_normal((nx + 1, ny)), rng.standard_normal((nx, ny + 1))]
+    _zero_boundary_v(x[1], x[2])
+    _zero_boundary_v(y[1], y[2])
+    Ax, ATy = [a.copy() for a in x], [a.copy() for a in y]
+    O.acoustic2d_step(*Ax, K, 0.05, 1.0, 0.3, 0.35)
+    O.acoustic2d_step_adjoint(*ATy, K, 0.05, 1.0, 0.3, 0.35)
+    lhs = sum(float(np.sum(a * b)) for a, b in zip(Ax, y))
+    rhs = sum(float(np.sum(a * b)) for a, b in zip(x, ATy))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    # a plausible mistake (sign of the adjoint velocity update) breaks the identity
+    bad = [a.copy() for a in y]
+    O.acoustic2d_step_adjoint(bad[0], -bad[1], -bad[2], K, 0.05, 1.0, 0.3, 0.35)
+    assert abs(sum(float(np.sum(a * b)) for a, b in zip(x, bad)) - lhs) > 1e-6 * abs(lhs)
+
+
+def test_acoustic2d_gradient_finite_differences():
+    K, prm, src, geo, nt = _ac2_case()
+    args = (nt, prm["dt"], prm["rho"], prm["dx"], prm["dy"], src, geo["isx"], geo["isy"], geo["rx0"], geo["rdx"], geo["ry"],
+            geo["nrec"])
+    Dobs = O.acoustic2d_traces(np.ones_like(K), *args)
+    J, g = O.acoustic2d_misfit_gradient(K, Dobs, *args)
+    assert J > 0
+    rng = np.random.default_rng(2)
+    for _ in range(4):
+        d = rng.standard_normal(K.shape)
+        h = 1e-4
+        r = lambda Kp: 0.5 * float(np.sum((O.acoustic2d_traces(Kp, *args) - Dobs) ** 2))
+        fd = (r(K + h * d) - r(K - h * d)) / (2 * h)
+        assert abs(fd - float(np.sum(g * d))) <= 1e-6 * abs(fd)
