@@ -1,0 +1,425 @@
+// Neural-network kernels of the Stokes warm-start U-Net (include/ps_nn.h).
+//
+// BF16 storage, FP32 accumulation. Activations are NHWC (channels fastest), so a 3x3
+// convolution is im2col (rows = pixels, columns = (ky, kx, c)) followed by a GEMM with the
+// weights stored [9*Cin][Cout]. The GEMM runs on the gfx950 matrix cores:
+// v_mfma_f32_16x16x32_bf16, lane l holding A[l%16][(l/16)*8 .. +7], B[(l/16)*8 .. +7][l%16] and
+// D[(l/16)*4 + r][l%16], r = 0..3. A workgroup of four waves computes a 64x64 tile, each wave a
+// 32x32 quarter as 2x2 MFMA blocks, over K steps of 32 staged through LDS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+
+#include "ps_amd.h"
+#include "ps_internal.h"
+#include "ps_nn.h"
+
+namespace {
+
+typedef unsigned short bf16_t;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(8 * sizeof(short)))) short s16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float((unsigned)b << 16); }
+__device__ __forceinline__ bf16_t f2bf(float f) {  // round to nearest even, NaN -> 0x7fc0
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)0x7fc0u;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+
+constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;
+
+// C[m, n] = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn].
+// blockIdx.z splits K into chunks of kchunk (a multiple of TK).
+template <int EPI>
+__global__ void __launch_bounds__(256) nn_gemm_k(int M, int N, int K, const bf16_t* __restrict__ A, long long sam, long long sak,
+                                                 const bf16_t* __restrict__ B, long long sbk, long long sbn, void* __restrict__ C,
+                                                 long long ldc, const float* __restrict__ bias, int kchunk) {
+  __shared__ bf16_t As[TM * LDK];
+  __shared__ bf16_t Bs[TN * LDK];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const long long m0 = (long long)blockIdx.x * TM, n0 = (long long)blockIdx.y * TN;
+  const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+  const bool a_kfast = sak == 1, b_kfast = sbk == 1;
+  f32x4 acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kb; k0 < ke; k0 += TK) {
+    // stage the A and B tiles (K fastest in LDS); consecutive threads walk the unit-stride index
+    for (int e = tid; e < TM * TK; e += 256) {
+      int r, c;
+      if (a_kfast) { r = e / TK; c = e % TK; } else { r = e % TM; c = e / TM; }
+      const long long m = m0 + r;
+      const int k = k0 + c;
+      As[r * LDK + c] = (m < M && k < ke) ? A[m * sam + (long long)k * sak] : (bf16_t)0;
+    }
+    for (int e = tid; e < TN * TK; e += 256) {
+      int r, c;
+      if (b_kfast) { r = e / TK; c = e % TK; } else { r = e % TN; c = e / TN; }
+      const long long n = n0 + r;
+      const int k = k0 + c;
+      Bs[r * LDK + c] = (n < N && k < ke) ? B[(long long)k * sbk + n * sbn] : (bf16_t)0;
+    }
+    __syncthreads();
+    const int i = lane & 15, kq = (lane >> 4) * 8;
+    s16x8 a[2], b[2];
+    for (int q = 0; q < 2; ++q) {
+      const bf16_t* pa = &As[(wm * 32 + q * 16 + i) * LDK + kq];
+      const bf16_t* pb = &Bs[(wn * 32 + q * 16 + i) * LDK + kq];
+      for (int t = 0; t < 8; ++t) {
+        a[q][t] = (short)pa[t];
+        b[q][t] = (short)pb[t];
+      }
+    }
+    for (int bi = 0; bi < 2; ++bi)
+      for (int bj = 0; bj < 2; ++bj)
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[bi]), __builtin_bit_cast(bf16x8, b[bj]),
+                                                              acc[bi][bj], 0, 0, 0);
+    __syncthreads();
+  }
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
+      for (int r = 0; r < 4; ++r) {
+        const long long m = m0 + wm * 32 + bi * 16 + (lane >> 4) * 4 + r;
+        const long long n = n0 + wn * 32 + bj * 16 + (lane & 15);
+        if (m >= M || n >= N) continue;
+        float v = acc[bi][bj][r];
+        if (EPI == PS_NN_EPI_BF16) {
+          ((bf16_t*)C)[m * ldc + n] = f2bf(v);
+        } else if (EPI == PS_NN_EPI_BF16_BIAS_RELU) {
+          v += bias[n];
+          ((bf16_t*)C)[m * ldc + n] = f2bf(v > 0.f ? v : 0.f);
+        } else if (EPI == PS_NN_EPI_BF16_BIAS) {
+          ((bf16_t*)C)[m * ldc + n] = f2bf(v + bias[n]);
+        } else if (EPI == PS_NN_EPI_F32_ADD) {
+          atomicAdd(&((float*)C)[m * ldc + n], v);
+        } else {
+          ((float*)C)[m * ldc + n] = v;
+        }
+      }
+}
+
+unsigned grid_for(long long n) { return (unsigned)std::min<long long>((n + 255) / 256, 1 << 20); }
+
+#define GS_LOOP(e, n) for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < (n); e += (long long)gridDim.x * 256)
+
+__global__ void __launch_bounds__(256) nn_im2col_k(const bf16_t* __restrict__ x, int B, int H, int W, int C, bf16_t* __restrict__ col) {
+  const long long n = (long long)B * H * W * 9 * C;
+  GS_LOOP(e, n) {
+    const int c = (int)(e % C);
+    const long long t = e / C;
+    const int kk = (int)(t % 9);
+    const long long p = t / 9;
+    const int xx = (int)(p % W);
+    const long long t2 = p / W;
+    const int yy = (int)(t2 % H), b = (int)(t2 / H);
+    const int sy = yy + kk / 3 - 1, sx = xx + kk % 3 - 1;
+    col[e] = (sy >= 0 && sy < H && sx >= 0 && sx < W) ? x[(((long long)b * H + sy) * W + sx) * C + c] : (bf16_t)0;
+  }
+}
+
+// dx[b, y, x, c] = sum over the 3x3 taps of the columns that read (b, y, x, c)
+__global__ void __launch_bounds__(256) nn_col2im_k(const bf16_t* __restrict__ dcol, int B, int H, int W, int C, bf16_t* __restrict__ dx) {
+  const long long n = (long long)B * H * W * C;
+  GS_LOOP(e, n) {
+    const int c = (int)(e % C);
+    const long long p = e / C;
+    const int xx = (int)(p % W);
+    const long long t2 = p / W;
+    const int yy = (int)(t2 % H), b = (int)(t2 / H);
+    float s = 0.f;
+    for (int kk = 0; kk < 9; ++kk) {
+      const int oy = yy - (kk / 3 - 1), ox = xx - (kk % 3 - 1);  // output pixel whose tap kk reads (yy, xx)
+      if (oy < 0 || oy >= H || ox < 0 || ox >= W) continue;
+      s += bf2f(dcol[((((long long)b * H + oy) * W + ox) * 9 + kk) * C + c]);
+    }
+    dx[e] = f2bf(s);
+  }
+}
+
+__global__ void __launch_bounds__(256) nn_pool2_k(const bf16_t* __restrict__ x, int B, int H, int W, int C, bf16_t* __restrict__ y) {
+  const int Ho = H / 2, Wo = W / 2;
+  const long long n = (long long)B * Ho * Wo * C;
+  GS_LOOP(e, n) {
+    const int c = (int)(e % C);
+    const long long p = e / C;
+    const int X = (int)(p % Wo);
+    const long long t = p / Wo;
+    const int Y = (int)(t % Ho), b = (int)(t / Ho);
+    const long long base = (((long long)b * H + 2 * Y) * W + 2 * X) * C + c;
+    const float s = (bf2f(x[base]) + bf2f(x[base + C])) + (bf2f(x[base + (long long)W * C]) + bf2f(x[base + (long long)W * C + C]));
+    y[e] = f2bf(0.25f * s);
+  }
+}
+
+__global__ void __launch_bounds__(256) nn_pool2_bwd_k(const bf16_t* __restrict__ dy, int B, int H, int W, int C, bf16_t* __restrict__ dx) {
+  const long long n = (long long)B * H * W * C;
+  GS_LOOP(e, n) {
+    const int c = (int)(e % C);
+    const long long p = e / C;
+    const int xx = (int)(p % W);
+    const long long t = p / W;
+    const int yy = (int)(t % H), b = (int)(t / H);
+    dx[e] = f2bf(0.25f * bf2f(dy[(((long long)b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c]));
+  }
+}
+
+__global__ void __launch_bounds__(256) nn_up2_k(const bf16_t* __restrict__ x, int B, int H, int W, int C, bf16_t* __restrict__ y) {
+  const int Ho = 2 * H, Wo = 2 * W;
+  const long long n = (long long)B * Ho * Wo * C;
+  GS_LOOP(e, n) {
+    const int c = (int)(e % C);
+    const long long p = e / C;
+    const int X = (int)(p % Wo);
+    const long long t = p / Wo;
+    const int Y = (int)(t % Ho), b = (int)(t / Ho);
+    y[e] = x[(((long long)b * H + Y / 2) * W + X / 2) * C + c];
+  }
+}
+
+__global__ void __launch_bounds__(256) nn_up2_bwd_k(const bf16_t* __restrict__ dy, int B, int H, int W, int C, bf16_t* __restrict__ dx) {
+  const long long n = (long long)B * H * W * C;
+  const int Wo = 2 * W;
+  GS_LOOP(e, n) {
+    const int c = (int)(e % C);
+    const long long p = e / C;
+    const int xx = (int)(p % W);
+    const long long t = p / W;
+    const int yy = (int)(t % H), b = (int)(t / H);
+    const long long base = (((long long)b * 2 * H + 2 * yy) * Wo + 2 * xx) * C + c;
+    const float s = (bf2f(dy[base]) + bf2f(dy[base + C])) + (bf2f(dy[base + (long long)Wo * C]) + bf2f(dy[base + (long long)Wo * C + C]));
+    dx[e] = f2bf(s);
+  }
+}
+
+__global__ void __launch_bounds__(256) nn_concat_k(const bf16_t* __restrict__ a, int Ca, const bf16_t* __restrict__ b, int Cb, long long P,
+                                                   bf16_t* __restrict__ y) {
+  const int C = Ca + Cb;
+  GS_LOOP(e, P * C) {
+    const int c = (int)(e % C);
+    const long long p = e / C;
+    y[e] = c < Ca ? a[p * Ca + c] : b[p * Cb + (c - Ca)];
+  }
+}
+
+__global__ void __launch_bounds__(256) nn_split_k(const bf16_t* __restrict__ dy, int Ca, int Cb, long long P, bf16_t* __restrict__ da,
+                                                  bf16_t* __restrict__ db) {
+  const int C = Ca + Cb;
+  GS_LOOP(e, P * C) {
+    const int c = (int)(e % C);
+    const long long p = e / C;
+    if (c < Ca) da[p * Ca + c] = dy[e];
+    else db[p * Cb + (c - Ca)] = dy[e];
+  }
+}
+
+__global__ void __launch_bounds__(256) nn_relu_bwd_k(bf16_t* __restrict__ dy, const bf16_t* __restrict__ y, long long n) {
+  GS_LOOP(e, n) {
+    if (!(bf2f(y[e]) > 0.f)) dy[e] = 0;
+  }
+}
+
+// db[n] += sum over rows of dy[m, n] (FP32); rows split over blockIdx.x in chunks of 256
+__global__ void __launch_bounds__(256) nn_bias_grad_k(const bf16_t* __restrict__ dy, long long M, int N, float* __restrict__ db) {
+  const long long r0 = (long long)blockIdx.x * 256, r1 = std::min<long long>(M, r0 + 256);
+  for (int n = threadIdx.x; n < N; n += 256) {
+    float s = 0.f;
+    for (long long m = r0; m < r1; ++m) s += bf2f(dy[m * N + n]);
+    atomicAdd(&db[n], s);
+  }
+}
+
+// loss += scale * mask * (pred - target)^2 ; dpred = 2 * scale * mask * (pred - target)
+__global__ void __launch_bounds__(256) nn_mse_k(const bf16_t* __restrict__ pred, const float* __restrict__ target, const float* __restrict__ mask,
+                                                long long n, float scale, bf16_t* __restrict__ dpred, float* __restrict__ loss) {
+  __shared__ float part[256];
+  float s = 0.f;
+  GS_LOOP(e, n) {
+    const float d = bf2f(pred[e]) - target[e];
+    const float mk = mask ? mask[e] : 1.f;
+    s += scale * mk * d * d;
+    dpred[e] = f2bf(2.f * scale * mk * d);
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, part[0]);
+}
+
+__global__ void __launch_bounds__(256) nn_adam_k(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                                                 long long n, float lr, float b1, float b2, float eps, float c1, float c2,
+                                                 bf16_t* __restrict__ w16) {
+  GS_LOOP(e, n) {
+    const float ge = g[e];
+    const float me = b1 * m[e] + (1.f - b1) * ge;
+    const float ve = b2 * v[e] + (1.f - b2) * ge * ge;
+    m[e] = me;
+    v[e] = ve;
+    const float we = w[e] - lr * (me * c1) / (sqrtf(ve * c2) + eps);
+    w[e] = we;
+    if (w16) w16[e] = f2bf(we);
+  }
+}
+
+__global__ void __launch_bounds__(256) nn_f32_to_bf16_k(const float* __restrict__ x, long long n, bf16_t* __restrict__ y) {
+  GS_LOOP(e, n) y[e] = f2bf(x[e]);
+}
+__global__ void __launch_bounds__(256) nn_bf16_to_f32_k(const bf16_t* __restrict__ x, long long n, float* __restrict__ y) {
+  GS_LOOP(e, n) y[e] = bf2f(x[e]);
+}
+__global__ void __launch_bounds__(256) nn_fill_f32_k(float* __restrict__ x, long long n, float v) {
+  GS_LOOP(e, n) x[e] = v;
+}
+__global__ void __launch_bounds__(256) nn_add_bf16_k(bf16_t* __restrict__ y, const bf16_t* __restrict__ x, long long n) {
+  GS_LOOP(e, n) y[e] = f2bf(bf2f(y[e]) + bf2f(x[e]));
+}
+
+int launched(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ps_internal_fail(PS_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+}  // namespace
+
+#define NN_BEGIN                                   \
+  if (int r_ = ps_internal_require_init()) return r_; \
+  hipStream_t st = ps_internal_stream(stream);
+
+extern "C" int ps_nn_gemm(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B, int64_t sbk, int64_t sbn, void* C,
+                          int64_t ldc, const float* bias, int epi, int splitk, void* stream) {
+  NN_BEGIN
+  if (M < 0 || N < 0 || K < 0 || splitk < 1) return ps_internal_fail(PS_EARGUMENT, "ps_nn_gemm: negative size or splitk < 1");
+  if (M == 0 || N == 0) return 0;
+  if ((epi == PS_NN_EPI_BF16_BIAS_RELU || epi == PS_NN_EPI_BF16_BIAS) && !bias) return ps_internal_fail(PS_EARGUMENT, "ps_nn_gemm: bias missing");
+  if (splitk > 1 && epi != PS_NN_EPI_F32_ADD) return ps_internal_fail(PS_EARGUMENT, "ps_nn_gemm: split-K needs the accumulating FP32 epilogue");
+  int kchunk = (K + splitk - 1) / splitk;
+  kchunk = ((kchunk + TK - 1) / TK) * TK;
+  if (kchunk < TK) kchunk = TK;
+  const int nz = (K + kchunk - 1) / kchunk;
+  dim3 grid((unsigned)((M + TM - 1) / TM), (unsigned)((N + TN - 1) / TN), (unsigned)std::max(1, nz));
+  if (grid.y > 65535 || grid.z > 65535) return ps_internal_fail(PS_EARGUMENT, "ps_nn_gemm: N or split too large");
+  const bf16_t* a = (const bf16_t*)A;
+  const bf16_t* b = (const bf16_t*)B;
+  switch (epi) {
+    case PS_NN_EPI_BF16: hipLaunchKernelGGL(nn_gemm_k<PS_NN_EPI_BF16>, grid, dim3(256), 0, st, M, N, K, a, sam, sak, b, sbk, sbn, C, ldc, bias, kchunk); break;
+    case PS_NN_EPI_BF16_BIAS_RELU:
+      hipLaunchKernelGGL(nn_gemm_k<PS_NN_EPI_BF16_BIAS_RELU>, grid, dim3(256), 0, st, M, N, K, a, sam, sak, b, sbk, sbn, C, ldc, bias, kchunk);
+      break;
+    case PS_NN_EPI_BF16_BIAS:
+      hipLaunchKernelGGL(nn_gemm_k<PS_NN_EPI_BF16_BIAS>, grid, dim3(256), 0, st, M, N, K, a, sam, sak, b, sbk, sbn, C, ldc, bias, kchunk);
+      break;
+    case PS_NN_EPI_F32_ADD:
+      hipLaunchKernelGGL(nn_gemm_k<PS_NN_EPI_F32_ADD>, grid, dim3(256), 0, st, M, N, K, a, sam, sak, b, sbk, sbn, C, ldc, bias, kchunk);
+      break;
+    case PS_NN_EPI_F32: hipLaunchKernelGGL(nn_gemm_k<PS_NN_EPI_F32>, grid, dim3(256), 0, st, M, N, K, a, sam, sak, b, sbk, sbn, C, ldc, bias, kchunk); break;
+    default: return ps_internal_fail(PS_EARGUMENT, "ps_nn_gemm: unknown epilogue");
+  }
+  return launched("ps_nn_gemm");
+}
+
+#define NN_ELEMENTWISE(name, n, ...)                                            \
+  NN_BEGIN                                                                      \
+  if ((n) <= 0) return 0;                                                       \
+  hipLaunchKernelGGL(name, dim3(grid_for(n)), dim3(256), 0, st, __VA_ARGS__); \
+  return launched(#name);
+
+extern "C" int ps_nn_im2col(const void* x, int B, int H, int W, int C, void* col, void* stream) {
+  NN_ELEMENTWISE(nn_im2col_k, (long long)B * H * W * 9 * C, (const bf16_t*)x, B, H, W, C, (bf16_t*)col)
+}
+extern "C" int ps_nn_col2im(const void* dcol, int B, int H, int W, int C, void* dx, void* stream) {
+  NN_ELEMENTWISE(nn_col2im_k, (long long)B * H * W * C, (const bf16_t*)dcol, B, H, W, C, (bf16_t*)dx)
+}
+extern "C" int ps_nn_pool2(const void* x, int B, int H, int W, int C, void* y, void* stream) {
+  if ((H | W) & 1) return ps_internal_fail(PS_EARGUMENT, "ps_nn_pool2: H and W must be even");
+  NN_ELEMENTWISE(nn_pool2_k, (long long)B * (H / 2) * (W / 2) * C, (const bf16_t*)x, B, H, W, C, (bf16_t*)y)
+}
+extern "C" int ps_nn_pool2_bwd(const void* dy, int B, int H, int W, int C, void* dx, void* stream) {
+  if ((H | W) & 1) return ps_internal_fail(PS_EARGUMENT, "ps_nn_pool2_bwd: H and W must be even");
+  NN_ELEMENTWISE(nn_pool2_bwd_k, (long long)B * H * W * C, (const bf16_t*)dy, B, H, W, C, (bf16_t*)dx)
+}
+extern "C" int ps_nn_up2(const void* x, int B, int H, int W, int C, void* y, void* stream) {
+  NN_ELEMENTWISE(nn_up2_k, (long long)B * 4 * H * W * C, (const bf16_t*)x, B, H, W, C, (bf16_t*)y)
+}
+extern "C" int ps_nn_up2_bwd(const void* dy, int B, int H, int W, int C, void* dx, void* stream) {
+  NN_ELEMENTWISE(nn_up2_bwd_k, (long long)B * H * W * C, (const bf16_t*)dy, B, H, W, C, (bf16_t*)dx)
+}
+extern "C" int ps_nn_concat(const void* a, int Ca, const void* b, int Cb, int64_t P, void* y, void* stream) {
+  NN_ELEMENTWISE(nn_concat_k, (long long)P * (Ca + Cb), (const bf16_t*)a, Ca, (const bf16_t*)b, Cb, (long long)P, (bf16_t*)y)
+}
+extern "C" int ps_nn_split(const void* dy, int Ca, int Cb, int64_t P, void* da, void* db, void* stream) {
+  NN_ELEMENTWISE(nn_split_k, (long long)P * (Ca + Cb), (const bf16_t*)dy, Ca, Cb, (long long)P, (bf16_t*)da, (bf16_t*)db)
+}
+extern "C" int ps_nn_relu_bwd(void* dy, const void* y, int64_t n, void* stream) {
+  NN_ELEMENTWISE(nn_relu_bwd_k, (long long)n, (bf16_t*)dy, (const bf16_t*)y, (long long)n)
+}
+extern "C" int ps_nn_bias_grad(const void* dy, int64_t M, int N, float* db, void* stream) {
+  NN_BEGIN
+  if (M <= 0 || N <= 0) return 0;
+  hipLaunchKernelGGL(nn_bias_grad_k, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st, (const bf16_t*)dy, (long long)M, N, db);
+  return launched("ps_nn_bias_grad");
+}
+extern "C" int ps_nn_mse(const void* pred, const float* target, const float* mask, int64_t n, float scale, void* dpred, float* loss,
+                         void* stream) {
+  NN_ELEMENTWISE(nn_mse_k, (long long)n, (const bf16_t*)pred, target, mask, (long long)n, scale, (bf16_t*)dpred, loss)
+}
+extern "C" int ps_nn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2, float eps, float c1, float c2,
+                          void* w16, void* stream) {
+  NN_ELEMENTWISE(nn_adam_k, (long long)n, w, g, m, v, (long long)n, lr, b1, b2, eps, c1, c2, (bf16_t*)w16)
+}
+extern "C" int ps_nn_f32_to_bf16(const float* x, int64_t n, void* y, void* stream) {
+  NN_ELEMENTWISE(nn_f32_to_bf16_k, (long long)n, x, (long long)n, (bf16_t*)y)
+}
+extern "C" int ps_nn_bf16_to_f32(const void* x, int64_t n, float* y, void* stream) {
+  NN_ELEMENTWISE(nn_bf16_to_f32_k, (long long)n, (const bf16_t*)x, (long long)n, y)
+}
+extern "C" int ps_nn_fill_f32(float* x, int64_t n, float v, void* stream) { NN_ELEMENTWISE(nn_fill_f32_k, (long long)n, x, (long long)n, v) }
+extern "C" int ps_nn_add_bf16(void* y, const void* x, int64_t n, void* stream) {
+  NN_ELEMENTWISE(nn_add_bf16_k, (long long)n, (bf16_t*)y, (const bf16_t*)x, (long long)n)
+}
+
+extern "C" int ps_nn_alloc(int64_t bytes, void** out) {
+  if (int r = ps_internal_require_init()) return r;
+  if (bytes < 0) return ps_internal_fail(PS_EARGUMENT, "ps_nn_alloc: negative size");
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, (size_t)std::max<int64_t>(bytes, 256));
+  if (e != hipSuccess) return ps_internal_fail(e == hipErrorOutOfMemory ? PS_EOOM : PS_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemset(p, 0, (size_t)std::max<int64_t>(bytes, 256));
+  if (e != hipSuccess) return ps_internal_fail(PS_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  *out = p;
+  return 0;
+}
+extern "C" int ps_nn_free(void* p) {
+  if (!p) return 0;
+  hipError_t e = hipFree(p);
+  if (e != hipSuccess) return ps_internal_fail(PS_EHIP, std::string("hipFree: ") + hipGetErrorString(e));
+  return 0;
+}
+extern "C" int ps_nn_h2d(void* dst, const void* src, int64_t bytes, void* stream) {
+  NN_BEGIN
+  hipError_t e = hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return ps_internal_fail(PS_EHIP, std::string("ps_nn_h2d: ") + hipGetErrorString(e));
+  return 0;
+}
+extern "C" int ps_nn_d2h(void* dst, const void* src, int64_t bytes, void* stream) {
+  NN_BEGIN
+  hipError_t e = hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return ps_internal_fail(PS_EHIP, std::string("ps_nn_d2h: ") + hipGetErrorString(e));
+  return 0;
+}
+extern "C" int ps_nn_d2d(void* dst, const void* src, int64_t bytes, void* stream) {
+  NN_BEGIN
+  hipError_t e = hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return ps_internal_fail(PS_EHIP, std::string("ps_nn_d2d: ") + hipGetErrorString(e));
+  return 0;
+}

@@ -356,6 +356,11 @@ hipStream_t stream_of(void* s) { return s ? (hipStream_t)s : S().stream0; }
 
 }  // namespace
 
+// helpers shared with the other translation units of the library (csrc/ps_internal.h)
+int ps_internal_fail(int code, const std::string& msg) { return fail(code, msg); }
+int ps_internal_require_init() { return require_init(); }
+hipStream_t ps_internal_stream(void* s) { return stream_of(s); }
+
 // ---------------------------------------------------------------------------
 // device
 // ---------------------------------------------------------------------------

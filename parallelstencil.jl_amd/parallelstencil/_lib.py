@@ -176,6 +176,34 @@ _SIGS = {
     "ps_graph_destroy": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
+# include/ps_nn.h (kernels of the Stokes warm-start network)
+_V, _I64, _F = ctypes.c_void_p, ctypes.c_int64, ctypes.c_float
+_I = ctypes.c_int
+_SIGS.update({
+    "ps_nn_gemm": (_I, [_I, _I, _I, _V, _I64, _I64, _V, _I64, _I64, _V, _I64, _V, _I, _I, _V]),
+    "ps_nn_im2col": (_I, [_V, _I, _I, _I, _I, _V, _V]),
+    "ps_nn_col2im": (_I, [_V, _I, _I, _I, _I, _V, _V]),
+    "ps_nn_pool2": (_I, [_V, _I, _I, _I, _I, _V, _V]),
+    "ps_nn_pool2_bwd": (_I, [_V, _I, _I, _I, _I, _V, _V]),
+    "ps_nn_up2": (_I, [_V, _I, _I, _I, _I, _V, _V]),
+    "ps_nn_up2_bwd": (_I, [_V, _I, _I, _I, _I, _V, _V]),
+    "ps_nn_concat": (_I, [_V, _I, _V, _I, _I64, _V, _V]),
+    "ps_nn_split": (_I, [_V, _I, _I, _I64, _V, _V, _V]),
+    "ps_nn_relu_bwd": (_I, [_V, _V, _I64, _V]),
+    "ps_nn_bias_grad": (_I, [_V, _I64, _I, _V, _V]),
+    "ps_nn_mse": (_I, [_V, _V, _V, _I64, _F, _V, _V, _V]),
+    "ps_nn_adam": (_I, [_V, _V, _V, _V, _I64, _F, _F, _F, _F, _F, _F, _V, _V]),
+    "ps_nn_f32_to_bf16": (_I, [_V, _I64, _V, _V]),
+    "ps_nn_bf16_to_f32": (_I, [_V, _I64, _V, _V]),
+    "ps_nn_fill_f32": (_I, [_V, _I64, _F, _V]),
+    "ps_nn_add_bf16": (_I, [_V, _V, _I64, _V]),
+    "ps_nn_alloc": (_I, [_I64, _P(_V)]),
+    "ps_nn_free": (_I, [_V]),
+    "ps_nn_h2d": (_I, [_V, _V, _I64, _V]),
+    "ps_nn_d2h": (_I, [_V, _V, _I64, _V]),
+    "ps_nn_d2d": (_I, [_V, _V, _I64, _V]),
+})
+
 
 def _load():
     if not os.path.exists(LIB_PATH):

@@ -250,10 +250,17 @@ class Stokes2D:
         self.k_ts(self.dtVx, self.dtVy, self.dtPt, M, self.Vsc, self.Ptsc, self.min_dxy2, self.max_nxy)
 
     def reset(self, Vx=None, Vy=None, Pt=None):
-        """Zero state, or a warm start (host arrays of the shapes of Vx, Vy, Pt)."""
+        """Zero state, or a warm start (host arrays of the shapes of Vx, Vy, Pt). The iteration never
+        updates the wall-normal velocity nodes (Vx at the x walls, Vy at the y walls), so a warm
+        start gets them set to the zero of the free-slip condition."""
+        import numpy as np
         for A in (self.Pt, self.Vx, self.Vy, self.divV, self.txx, self.tyy, self.txy, self.Rx, self.Ry, self.dVxdt, self.dVydt):
             A.fill_(0.0)
         if Vx is not None:
+            Vx = np.array(Vx, dtype=np.float64, order="F")
+            Vy = np.array(Vy, dtype=np.float64, order="F")
+            Vx[0, :] = Vx[-1, :] = 0.0
+            Vy[:, 0] = Vy[:, -1] = 0.0
             self.Vx.copy_from(Vx)
             self.Vy.copy_from(Vy)
             self.Pt.copy_from(Pt)
