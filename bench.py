@@ -128,6 +128,63 @@ def stokes_warm_start(args):
     print(json.dumps(out), flush=True)
 
 
+def acoustic_study(args):
+    """acoustic3D n^3 (miniapps/acoustic3D.jl), `steps` time steps from the miniapp's initial
+    state in FP64, FP32 and BFloat16 storage with FP32 arithmetic, plain and memopt kernels:
+    T_eff per step (reference A_eff = 8 field sweeps, miniapps/acoustic3D.jl:62, and the 12 sweeps
+    the two unfused kernels need) and the per-field error of each run against the FP64 run."""
+    import numpy as np
+    import parallelstencil as ps
+    from parallelstencil import workloads as W
+
+    n, steps = args.acoustic_n, args.acoustic_steps
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3, module="bench.a64")
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float32, 3, module="bench.a32")
+    P0, prm = W.acoustic3d_inputs(n)
+    runs, ref = {}, None
+    for prec, mod, elt, size in (("f64", "bench.a64", ps.Float64, 8), ("f32", "bench.a32", ps.Float32, 4),
+                                 ("bf16", "bench.a32", ps.BFloat16, 2)):
+        for memopt in (False, True):
+            kV, kP = W.define_acoustic3d(mod, memopt)
+            P, Vx, Vy, Vz = (ps.zeros(n, n, n, eltype=elt), ps.zeros(n + 1, n, n, eltype=elt), ps.zeros(n, n + 1, n, eltype=elt),
+                             ps.zeros(n, n, n + 1, eltype=elt))
+            a = (prm["dt"], prm["rho"], prm["dx"], prm["dy"], prm["dz"])
+            b = (prm["dt"], prm["k"], prm["dx"], prm["dy"], prm["dz"])
+            for _ in range(3):  # compile and warm up on the zero state
+                kV(Vx, Vy, Vz, P, *a, async_=True)
+                kP(P, Vx, Vy, Vz, *b, async_=True)
+            for A in (Vx, Vy, Vz):
+                A.fill_(0.0)
+            P.copy_from(W._to_storage(P0, elt))
+            ps.synchronize()
+            e0, e1 = ps.Event(), ps.Event()
+            e0.record()
+            for _ in range(steps):
+                kV(Vx, Vy, Vz, P, *a, async_=True)
+                kP(P, Vx, Vy, Vz, *b, async_=True)
+            e1.record()
+            ps.synchronize()
+            ms = e0.elapsed_ms(e1) / steps
+            fields = tuple(W._from_storage(A.to_host(), elt) for A in (P, Vx, Vy, Vz))
+            if prec == "f64" and not memopt:
+                ref = fields
+            errs = {name: W.field_errors(f, r) for name, f, r in zip(("P", "Vx", "Vy", "Vz"), fields, ref)}
+            key = f"{prec}_{'memopt' if memopt else 'plain'}"
+            runs[key] = {"ms_per_step": round(ms, 4), "T_eff_gbs_ref": round(8 * size * n ** 3 / ms / 1e6, 1),
+                         "T_eff_gbs_12sweeps": round(12 * size * n ** 3 / ms / 1e6, 1),
+                         "err_vs_f64 (max, l2)": {k: [float(f"{v[0]:.3e}"), float(f"{v[1]:.3e}")] for k, v in errs.items()}}
+            log(f"acoustic3D {n}^3 {key}: {ms:.3f} ms/step, T_eff {runs[key]['T_eff_gbs_ref']} GB/s (ref A_eff), "
+                f"P err max {errs['P'][0]:.2e} l2 {errs['P'][1]:.2e}")
+            del P, Vx, Vy, Vz
+    best = min(runs.values(), key=lambda r: r["ms_per_step"])
+    out = {"metric": "T_eff GB/s (acoustic3D, reference A_eff = 8 field sweeps)", "value": runs["f64_memopt"]["T_eff_gbs_ref"],
+           "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": 3, "ms_per_step": runs["f64_memopt"]["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 / f32 / bf16-storage", "data": "synthetic",
+           "config": {"workload": f"acoustic3D {n}^3, {steps} steps from the miniapp initial state (P0 = exp(-r^2), V0 = 0)"},
+           "runs": runs, "fastest_ms_per_step": best["ms_per_step"]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,7 +193,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start)")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic")
+    ap.add_argument("--acoustic-n", type=int, default=384)
+    ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--stokes-n", type=int, default=127)
     ap.add_argument("--train", type=int, default=256, help="stokes: training samples")
     ap.add_argument("--eval", type=int, default=24, help="stokes: held-out samples")
@@ -147,6 +206,9 @@ def main():
         sys.exit(2)
     if args.workload == "stokes":
         stokes_warm_start(args)
+        return
+    if args.workload == "acoustic":
+        acoustic_study(args)
         return
 
     import parallelstencil as ps

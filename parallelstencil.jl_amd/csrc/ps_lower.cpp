@@ -689,7 +689,8 @@ Kernel define_kernel(const std::string& text, const JVal& opts) {
 // ---------------------------------------------------------------------------
 std::string kernel_info_json(const Kernel& K) {
   std::ostringstream o;
-  o << "{\"name\":" << json_str(K.def.name) << ",\"kind\":" << json_str(K.opt.kind) << ",\"nb_parallel_indices\":" << K.nidx
+  o << "{\"name\":" << json_str(K.def.name) << ",\"kind\":" << json_str(K.opt.kind)
+    << ",\"is_parallel_kernel\":" << (K.opt.kind == "parallel" ? "true" : "false") << ",\"nb_parallel_indices\":" << K.nidx
     << ",\"numbertype\":" << json_str(K.opt.numbertype) << ",\"memopt\":" << (K.opt.memopt ? "true" : "false") << ",\"params\":[";
   for (size_t i = 0; i < K.def.params.size(); ++i)
     o << (i ? "," : "") << "{\"name\":" << json_str(K.def.params[i].name) << ",\"annot\":" << json_str(K.def.params[i].annot) << "}";

@@ -930,10 +930,13 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
     int Ld = mm.loopdim - 1;
     int64_t loopsizes[3] = {1, 1, 1};
     loopsizes[Ld] = mm.loopsize;
+    // x block width: the tuned width unless half of it pads the x range to fewer threads
+    int xmax = K.opt.nthreads_x_max_memopt;
+    if (xmax >= 128 && (len[0] + 63) / 64 * 64 < (len[0] + xmax - 1) / xmax * xmax) xmax = 64;
     if (K.opt.kind == "parallel" && !user_ranges) {
       // get_ranges_memopt: pad to multiples of nthreads (src/parallel.jl:560-571)
       int32_t nt[3];
-      ps_compute_nthreads(len, K.opt.nthreads_x_max_memopt, K.opt.nthreads_max_memopt, mm.loopdim, nt);
+      ps_compute_nthreads(len, xmax, K.opt.nthreads_max_memopt, mm.loopdim, nt);
       for (int d = 0; d < 3; ++d) {
         int64_t rest = len[d] % nt[d];
         if (rest) len[d] += nt[d] - rest;
@@ -948,7 +951,7 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
         P.nblocks[d] = L->nblocks[d] ? L->nblocks[d] : (int32_t)((ms[d] + P.nthreads[d] - 1) / P.nthreads[d]);
       }
     } else {
-      ps_compute_nthreads(ms, K.opt.nthreads_x_max_memopt, K.opt.nthreads_max_memopt, mm.loopdim, P.nthreads);
+      ps_compute_nthreads(ms, xmax, K.opt.nthreads_max_memopt, mm.loopdim, P.nthreads);
       // sizes that the tuned block does not divide get the reference's AMDGPU block limits
       // (64 threads along x, 256 per block: src/shared.jl:29, src/ParallelKernel/shared.jl:33)
       if (ms[0] % P.nthreads[0] || ms[1] % P.nthreads[1] || ms[2] % P.nthreads[2])

@@ -1,0 +1,170 @@
+"""Host-side behaviour without a GPU: the C-ABI library exports every declared symbol, kernel
+definition (parsing, FD-macro lowering, memopt analysis and metadata as in
+test/test_parallel.jl:1418-1528), definition-time errors, launch geometry helpers, the
+@hide_communication range split and the FieldAllocators sizes (test_allocators.jl:899-960)."""
+import ctypes
+import itertools
+import os
+import re
+
+import numpy as np
+import pytest
+
+import parallelstencil as ps
+from parallelstencil import FieldAllocators as FA
+from parallelstencil._lib import LIB_PATH
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3, module=__name__)
+MOD2 = __name__ + ".2d"
+ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2, module=MOD2)
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ps_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header", ["ps_amd.h", "ps_nn.h"])
+def test_library_exports_every_declared_symbol(header):
+    lib = ctypes.CDLL(LIB_PATH)
+    names = _declared(header)
+    assert len(names) >= (30 if header == "ps_amd.h" else 20)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"declared in include/{header} but not exported: {missing}"
+
+
+def test_version_and_no_device_errors():
+    lib = ctypes.CDLL(LIB_PATH)
+    assert lib.ps_version() > 0
+    # launching or allocating without an initialised device fails with a typed error, never silently
+    with pytest.raises((ps.NoDeviceError, ps.NotInitializedError)):
+        ps.zeros(4, 4, 4)
+
+
+def _k(text, **kw):
+    return ps.core._define("parallel_indices" if text.lstrip().startswith("@parallel_indices") else "parallel", text, kw, __name__)
+
+
+def test_metadata_standard():
+    k = _k("@parallel_indices (ix, iy, iz) function metadata_probe!(A, B, D)\n A[ix, iy, iz] = 2.0 * B[ix, iy, iz]\n return\nend")
+    assert k.info["memopt"] is False and k.info["nb_parallel_indices"] == 3
+    assert k.info["is_parallel_kernel"] is False
+
+
+def test_metadata_memopt():
+    k = _k("@parallel_indices (ix, iy, iz) memopt=true loopsize=3 optvars=B optranges=(B=(0:0,0:0,0:0),) "
+           "function metadata_memopt_probe!(A, B, D)\n A[ix, iy, iz] = 2.0 * B[ix, iy, iz]\n return\nend")
+    m = k.info
+    assert (m["loopdim"], m["loopsize"], m["loopsizes"], m["memopt"], m["nb_parallel_indices"]) == (3, 3, [1, 1, 3], True, 3)
+    assert m["optvars"] == ["B"] and m["optranges"]["B"] == [[0, 0], [0, 0], [0, 0]]
+    assert m["offsets"]["B"] == [[0, 0, 0, 1]]
+    assert (m["shmem_dim1"], m["shmem_dim2"], m["shmem_optvars"], m["use_any_shmem"]) == (1, 2, [], False)
+    assert m["shmem_spans"] == {"B": [0, 0]} and m["stencilranges"]["B"] == [[0, 0], [0, 0], [0, 0]]
+    assert m["use_shmemhalos"]["B"] is True
+
+
+def test_metadata_memopt_z_and_full_stencils():
+    z = _k("@parallel_indices (ix, iy, iz) memopt=true loopsize=3 optvars=B optranges=(B=(0:0,0:0,-1:1),) "
+           "function metadata_memopt_zstencil_probe!(A, B, D)\n A[ix, iy, iz] = B[ix, iy, iz-1] + B[ix, iy, iz] + B[ix, iy, iz+1] + D[ix, iy, iz, 1]\n return\nend").info
+    assert z["shmem_optvars"] == [] and z["shmem_spans"] == {"B": [0, 0]} and z["use_any_shmem"] is False
+    assert z["stencilranges"]["B"] == [[0, 0], [0, 0], [-1, 1]]
+    f = _k("@parallel_indices (ix, iy, iz) memopt=true loopsize=3 optvars=B function metadata_memopt_fullstencil_probe!(A, B, D)\n"
+           " A[ix, iy, iz] = B[ix-1, iy-1, iz-1] + B[ix, iy, iz] + B[ix+1, iy+1, iz+1] + D[ix, iy, iz, 1]\n return\nend").info
+    # the GPU branch of the reference test
+    assert f["shmem_optvars"] == ["B"] and f["shmem_spans"] == {"B": [2, 2]} and f["use_any_shmem"] is True
+    assert f["stencilranges"]["B"] == [[-1, 1], [-1, 1], [-1, 1]]
+
+
+def test_metadata_memopt_2d():
+    k = ps.core._define("parallel_indices", "@parallel_indices (ix, iy) ndims=2 memopt=true loopdim=2 loopsize=5 optvars=B "
+                        "optranges=(B=(0:0,-1:1,0:0),) function metadata_memopt_2d_probe!(A, B)\n if (1 < iy < size(A, 2))\n"
+                        " A[ix, iy] = B[ix, iy - 1] + B[ix, iy] + B[ix, iy + 1]\n end\n return\nend", {}, MOD2)
+    m = k.info
+    assert (m["nb_parallel_indices"], m["loopdim"], m["loopsize"], m["loopsizes"]) == (2, 2, 5, [1, 5, 1])
+    assert m["optranges"]["B"] == [[0, 0], [-1, 1], [0, 0]] and (m["shmem_dim1"], m["shmem_dim2"]) == (1, 3)
+    assert m["shmem_optvars"] == [] and m["use_any_shmem"] is False and m["use_shmemhalos"]["B"] is False
+
+
+def test_onthefly_and_fd_lowering():
+    """The flux-form diffusion kernel: qTx, qTy, qTz become on-the-fly arrays, and the lowered
+    body reads T at the seven-point stencil and Cp at the centre."""
+    from parallelstencil import workloads as W
+    k = W.define_diffusion3d(False, __name__)
+    assert k.info["onthefly"] == ["qTx", "qTy", "qTz"]
+    assert k.info["written"] == ["T2"] and set(k.info["accessed"]) >= {"T", "Cp", "T2"}
+    A = [ps.ArgSpec((9, 7, 8)) for _ in range(3)]
+    src = k.source(*A, 1.0, 0.1, 0.3, 0.4, 0.5)
+    assert "psk_diffusion3D_step" in src and "__global__" in src
+    g = k.geometry(*A, 1.0, 0.1, 0.3, 0.4, 0.5)
+    assert g.memopt is False and g.checked is False  # the bounds proof clears every access
+    assert g.ranges == ((1, 9), (1, 7), (1, 8))
+
+
+@pytest.mark.parametrize("text,exc,match", [
+    ("@parallel function f!(A, B)\n a = B + 1\n return\nend", ps.ArgumentError, None),
+    ("@parallel_indices (ix, iy, iz) function write_xy_plane!(A)\n A[ix, iy, 1] = 2.0 * A[ix, iy, 1]\n return\nend",
+     ps.ArgumentError, "all parallel indices must be used"),
+    ("@parallel_indices (ix, iy, iz) memopt=true function w!(A, B)\n A[ix, iy, 1] = 2.0 * A[ix, iy, 1] + B[ix, iy, 1]\n return\nend",
+     ps.ArgumentError, "all parallel indices must be used"),
+    ("@parallel_indices (ix, iy, iz) memopt=true optvars=A function w!(A, B)\n A[ix, iy, iz] = B[ix, iy, iz]\n return\nend",
+     ps.IncoherentArgumentError, "only read"),
+    ("@parallel_indices (ix) memopt=true function w!(A, B)\n A[ix] = B[ix] + B[ix+1]\n return\nend",
+     ps.IncoherentArgumentError, "2-D and 3-D"),
+    ("@parallel function f!(A, B)\n @all(A) = @nosuchmacro(B)\n return\nend", ps.PSException, None),
+    ("@parallel function f!(A, B)\n @all(A) = (@all(B) + 1\n return\nend", ps.PSException, None),
+])
+def test_definition_errors(text, exc, match):
+    with pytest.raises(exc, match=match):
+        _k(text)
+
+
+def test_compute_nthreads_and_nblocks():
+    assert ps.compute_nthreads((512, 512, 512)) == (64, 4, 1)
+    assert ps.compute_nblocks((512, 512, 512), (64, 4, 1)) == (8, 128, 512)
+    nt = ps.compute_nthreads((20, 3, 1))
+    assert nt[0] == 20 and nt[1] <= 3 and nt[0] * nt[1] <= 256
+
+
+def _cells(r):
+    return set(itertools.product(*[range(a, b + 1) for a, b in r]))
+
+
+@pytest.mark.parametrize("bw,ms", [((2, 2, 3), (6, 7, 8)), ((1, 1, 2), (6, 7, 8)), ((2, 2), (6, 7)), ((2,), (9,))])
+def test_hide_communication_ranges_partition_the_box(bw, ms):
+    full = tuple((1, m) for m in ms)
+    outer = ps.get_ranges_outer(bw, full)
+    inner = ps.get_ranges_inner(bw, full)
+    box = _cells(tuple(full) + ((1, 1),) * (3 - len(ms)))
+    parts = [_cells(r) for r in outer] + [_cells(r) for r in inner]
+    assert sum(len(p) for p in parts) == len(box) and set().union(*parts) == box  # disjoint cover
+    bw3 = tuple(bw) + (0,) * (3 - len(bw))
+    ms3 = tuple(ms) + (1,) * (3 - len(ms))
+    exp_inner = tuple((b + 1, m - b) if m > 1 else (1, 1) for b, m in zip(bw3, ms3))
+    assert inner == (exp_inner,)
+
+
+def test_hide_communication_errors():
+    with pytest.raises(ps.IncoherentArgumentError):
+        ps.get_ranges_outer((3, 2, 3), ((1, 6), (1, 7), (1, 8)))
+
+
+@pytest.mark.parametrize("grid", [(9, 7, 8), (9, 7), (9,)])
+def test_field_allocator_sizes(grid):
+    n = len(grid)
+    exp = {None: (0, 0, 0), "X": (-1, -2, -2), "Y": (-2, -1, -2), "Z": (-2, -2, -1), "BX": (1, 0, 0), "BY": (0, 1, 0),
+           "BZ": (0, 0, 1), "XX": (0, -2, -2), "YY": (-2, 0, -2), "ZZ": (-2, -2, 0), "XY": (-1, -1, -2), "XZ": (-1, -2, -1),
+           "YZ": (-2, -1, -1)}
+    for t, off in exp.items():
+        if n < 3 and t in ("BZ",):
+            continue
+        assert FA.field_size(grid, t) == tuple(grid[i] + off[i] for i in range(n)), t
+
+
+def test_bf16_conversion_rounds_to_nearest_even():
+    x = np.array([1.0, 1.00390625, 1.01171875, -2.5, 3.0e38, np.inf, np.nan], np.float32)
+    b = ps.f32_to_bf16(x)
+    assert b[0] == 0x3F80 and b[1] == 0x3F80 and b[2] == 0x3F82  # ties to even
+    assert b[5] == 0x7F80 and b[6] == 0x7FC0
+    assert np.array_equal(ps.bf16_to_f32(b[:4]), np.array([1.0, 1.0, 1.015625, -2.5], np.float32))
