@@ -185,6 +185,56 @@ def acoustic_study(args):
     print(json.dumps(out), flush=True)
 
 
+def adjoint_study(args):
+    """Acoustic 2-D inversion gradient at n^2 over `steps` steps (sqrt(N) checkpointing): time of
+    the forward run and of the checkpointed gradient in FP32 and FP64, the FP32 gradient against
+    the FP64 one, and the FP64 gradient against central finite differences along 8 random
+    directions."""
+    import numpy as np
+    import parallelstencil as ps
+    from parallelstencil import acoustic_adjoint as AA
+
+    n, nt = args.adjoint_n, args.adjoint_steps
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float32, 2, module="bench.adj32")
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2, module="bench.adj64")
+    s = AA.Setup(n, n, nt, seed=1, nrec=64, sigma=100.0)
+    res = {}
+    grads = {}
+    for prec, mod in (("f32", "bench.adj32"), ("f64", "bench.adj64")):
+        inv = AA.Acoustic2DInversion(mod, s)
+        Dobs = inv.forward(s.K0)  # observed data from the background model
+        inv.set_observed(Dobs)
+        inv.forward(s.K)  # warm-up
+        t0 = time.perf_counter()
+        inv.forward(s.K)
+        t1 = time.perf_counter()
+        J, g = inv.gradient(s.K)
+        t2 = time.perf_counter()
+        grads[prec] = g
+        res[prec] = {"forward_s": round(t1 - t0, 4), "gradient_s": round(t2 - t1, 4), "J": J, "checkpoints": len(inv.ck),
+                     "segment": inv.seg}
+        log(f"adjoint {n}^2 x {nt} {prec}: forward {t1 - t0:.3f} s, gradient {t2 - t1:.3f} s ({len(inv.ck)} checkpoints), J {J:.6e}")
+        if prec == "f64":
+            rng = np.random.default_rng(11)
+            fd = []
+            for _ in range(8):
+                d = rng.standard_normal(s.K.shape) * np.exp(-((np.arange(n)[:, None] - n / 2) ** 2 + (np.arange(n)[None, :] - n / 2) ** 2) / (2 * (n / 4) ** 2))
+                h = 1e-4
+                f = (inv.misfit(s.K + h * d) - inv.misfit(s.K - h * d)) / (2 * h)
+                a = float(np.sum(g * d))
+                fd.append(abs(f - a) / max(abs(f), 1e-300))
+            res[prec]["fd_rel_err_max"] = float(max(fd))
+            log(f"finite differences (8 directions): max relative error {max(fd):.2e}")
+    rel = float(np.linalg.norm(grads["f32"] - grads["f64"]) / np.linalg.norm(grads["f64"]))
+    out = {"metric": "acoustic2D inversion gradient time (sqrt(N) checkpointed adjoint)", "value": res["f32"]["gradient_s"],
+           "unit": "s per gradient", "n_gpus": 1, "steps": nt, "warmup": 1, "ms_per_step": round(1e3 * res["f32"]["gradient_s"] / nt, 4),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 check)", "data": "synthetic",
+           "config": {"workload": f"acoustic2D adjoint {n}x{n}, {nt} steps, 64 receivers, Ricker source, K = 1 + 0.1 Gaussian (sigma 100 cells)"},
+           "runs": res, "f32_vs_f64_gradient_rel_l2": rel}
+    log(f"FP32 vs FP64 gradient: relative L2 {rel:.2e}")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +246,8 @@ def main():
     ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
+    ap.add_argument("--adjoint-n", type=int, default=2048)
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--stokes-n", type=int, default=127)
     ap.add_argument("--train", type=int, default=256, help="stokes: training samples")
     ap.add_argument("--eval", type=int, default=24, help="stokes: held-out samples")
@@ -209,6 +261,9 @@ def main():
         return
     if args.workload == "acoustic":
         acoustic_study(args)
+        return
+    if args.workload == "adjoint":
+        adjoint_study(args)
         return
 
     import parallelstencil as ps

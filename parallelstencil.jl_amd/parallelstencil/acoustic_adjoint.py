@@ -97,26 +97,29 @@ def ricker(nt, dt, f0):
 
 class Setup:
     """Geometry of the inversion experiment on an nx x ny pressure grid (lx = ly = 40 as the
-    miniapp): K = 1 + amp * exp(-r^2 / (2 sigma^2)) (sigma in cells, centre from `seed`), a Ricker
-    source with `ppw` cells per wavelength at a random interior point, `nrec` receivers on a
-    horizontal line, dt = min(dx,dy)/sqrt(max K/rho)/4.1."""
+    miniapp): K = 1 + amp * exp(-r^2 / (2 sigma^2)) (sigma in cells), a Ricker source with `ppw`
+    cells per wavelength at a random interior point, `nrec` receivers on a horizontal line,
+    dt = min(dx,dy)/sqrt(max K/rho)/4.1. A wave crosses about nt/4.1 cells during the run, so the
+    receiver line lies `reach` = 0.6 nt/4.1 cells from the source (at most ny/3), spans 2 reach
+    cells, and the blob centre is drawn between the two (random from `seed`)."""
 
     def __init__(self, nx, ny, nt, seed=1, nrec=64, sigma=100.0, amp=0.1, ppw=10.0, lx=40.0, ly=40.0, rho=1.0):
         u = _uniform(8, seed)
         self.nx, self.ny, self.nt, self.rho = nx, ny, nt, rho
         self.dx, self.dy = lx / (nx - 1), ly / (ny - 1)
-        cx, cy = (0.25 + 0.5 * u[0]) * nx, (0.25 + 0.5 * u[1]) * ny
+        reach = max(4, min(int(0.6 * nt / 4.1), ny // 3))
+        self.isx = int(nx // 2 + (u[2] - 0.5) * (nx // 4))
+        self.isy = int(ny // 3 + (u[3] - 0.5) * (ny // 8))
+        self.nrec = nrec
+        self.rdx = max(1, min(2 * reach // nrec, (nx - 4) // nrec))
+        self.rx0 = min(max(2, self.isx - (nrec - 1) * self.rdx // 2), nx - 1 - (nrec - 1) * self.rdx)
+        self.ry = min(self.isy + reach, ny - 2)
+        assert 2 <= self.rx0 and self.rx0 + (nrec - 1) * self.rdx <= nx - 1
+        cx, cy = self.isx + (u[0] - 0.5) * reach, self.isy + (0.3 + 0.4 * u[1]) * reach
         ix = np.arange(1, nx + 1)[:, None]
         iy = np.arange(1, ny + 1)[None, :]
         self.K = np.asfortranarray(1.0 + amp * np.exp(-((ix - cx) ** 2 + (iy - cy) ** 2) / (2.0 * sigma * sigma)))
         self.K0 = np.asfortranarray(np.ones((nx, ny)))
-        self.isx = int(nx // 4 + u[2] * (nx // 2))
-        self.isy = int(ny // 4 + u[3] * (ny // 4))
-        self.nrec = nrec
-        self.rdx = max(1, (nx - 2) // nrec)
-        self.rx0 = max(2, (nx - (nrec - 1) * self.rdx) // 2)
-        self.ry = max(2, (3 * ny) // 4)
-        assert self.rx0 + (nrec - 1) * self.rdx <= nx - 1
         self.dt = min(self.dx, self.dy) / math.sqrt(float(self.K.max()) / rho) / 4.1
         f0 = 1.0 / (ppw * self.dx)  # wave speed 1 (K = rho = 1 background)
         self.src = self.dt * ricker(nt, self.dt, f0)
@@ -171,7 +174,7 @@ class Acoustic2DInversion:
         for n in range(self.s.nt):
             if checkpoint and n % self.seg == 0:
                 for A, B in zip(self.ck[n // self.seg], (self.P, self.Vx, self.Vy)):
-                    A.copy_(B)
+                    A.copy_(B, async_=True)
             self._step(n)
         core.synchronize()
         return self.D.to_host().astype(np.float64)
@@ -198,11 +201,11 @@ class Acoustic2DInversion:
         for c in reversed(range(len(self.ck))):
             n0, n1 = c * seg, min(nt, (c + 1) * seg)
             for A, B in zip((self.P, self.Vx, self.Vy), self.ck[c]):
-                A.copy_(B)
+                A.copy_(B, async_=True)
             for n in range(n0, n1):  # recompute the segment, keep V^{n+1}
                 self._step(n, record=False)
-                self.snap[n - n0][0].copy_(self.Vx)
-                self.snap[n - n0][1].copy_(self.Vy)
+                self.snap[n - n0][0].copy_(self.Vx, async_=True)
+                self.snap[n - n0][1].copy_(self.Vy, async_=True)
             for n in reversed(range(n0, n1)):
                 Vx1, Vy1 = self.snap[n - n0]
                 self.k_res(self.lP, self.D, self.Dobs, s.rx0, s.rdx, s.ry, n + 1, ranges=((1, s.nrec),), async_=True)

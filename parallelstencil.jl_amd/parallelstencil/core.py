@@ -234,9 +234,11 @@ class DeviceArray:
             check(lib.ps_stream_sync(None))
         return self
 
-    def copy_(self, other, stream=None):
+    def copy_(self, other, stream=None, async_=False):
+        """self .= other on the device; blocks unless a stream is given or async_ is set (then it
+        is ordered with the kernels of that stream, the library stream by default)."""
         check(lib.ps_field_copy(ctypes.byref(self.f), ctypes.byref(other.f), _stream_handle(stream)))
-        if stream is None:
+        if stream is None and not async_:
             check(lib.ps_stream_sync(None))
         return self
 
