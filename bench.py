@@ -49,7 +49,7 @@ def _traffic(n, head):
         for line in f:
             p = line.split()
             if len(p) == 3 and p[0] == "hbm_bytes_per_launch" and p[1].startswith("psk_diffusion3D") and \
-                    (("_memopt" in p[1]) == (head == "memopt")):
+                    (("_memopt" in p[1]) == (head == "memopt")) and "_fastmath" not in p[1]:
                 return int(p[2])
     return None
 
@@ -235,6 +235,54 @@ def adjoint_study(args):
     print(json.dumps(out), flush=True)
 
 
+def gemm_study(args):
+    """BF16 MFMA GEMM of the warm-start network (ps_nn_gemm), HIP-event timed: the three products
+    of a 3x3 convolution layer of the U-Net at batch 8 x 128^2 (forward, data gradient, weight
+    gradient) and a square 4096^3 product, as TFLOP/s against the BF16 dense MFMA peak."""
+    import numpy as np
+    import parallelstencil as ps
+    from parallelstencil import unet as U
+
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3, module="bench.gemm")
+    peak = 2500.0  # TFLOP/s, MI355X BF16 dense MFMA
+    rng = np.random.default_rng(0)
+    cases = []
+    P, cin, cout = 8 * 128 * 128, 32, 32
+    K = 9 * cin
+    cases.append(("conv 32->32 forward (col . W)", P, cout, K, (P, K), K, 1, (K, cout), cout, 1, U.EPI_BIAS_RELU, 1))
+    cases.append(("conv 32->32 data gradient (dY . W^T)", P, K, cout, (P, cout), cout, 1, (K, cout), 1, cout, U.EPI_BF16, 1))
+    cases.append(("conv 32->32 weight gradient (col^T . dY)", K, cout, P, (P, K), 1, K, (P, cout), cout, 1, U.EPI_F32_ADD, U._splitk(K, cout, P)))
+    cases.append(("square 4096^3", 4096, 4096, 4096, (4096, 4096), 4096, 1, (4096, 4096), 4096, 1, U.EPI_F32, 1))
+    res = []
+    for name, M, N, Kd, ash, sam, sak, bsh, sbk, sbn, epi, split in cases:
+        A = U.Buf(ash).upload(rng.standard_normal(ash).astype(np.float32))
+        B = U.Buf(bsh).upload(rng.standard_normal(bsh).astype(np.float32))
+        C = U.Buf((M, N), "f32" if epi in (U.EPI_F32, U.EPI_F32_ADD) else "bf16")
+        bias = U.Buf((N,), "f32")
+        for _ in range(3):
+            U.gemm(M, N, Kd, A, sam, sak, B, sbk, sbn, C, N, bias=bias, epi=epi, splitk=split)
+        ps.synchronize()
+        reps = 20
+        e0, e1 = ps.Event(), ps.Event()
+        e0.record()
+        for _ in range(reps):
+            U.gemm(M, N, Kd, A, sam, sak, B, sbk, sbn, C, N, bias=bias, epi=epi, splitk=split)
+        e1.record()
+        ps.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        tf = 2.0 * M * N * Kd / (ms * 1e-3) / 1e12
+        res.append({"case": name, "M": M, "N": N, "K": Kd, "ms": round(ms, 4), "tflops": round(tf, 2), "frac_of_peak": round(tf / peak, 4)})
+        log(f"{name}: M={M} N={N} K={Kd}: {ms:.4f} ms, {tf:.1f} TFLOP/s ({100 * tf / peak:.2f}% of BF16 MFMA peak)")
+    out = {"metric": "BF16 MFMA GEMM TFLOP/s (U-Net convolution products)", "value": res[0]["tflops"], "unit": "TFLOP/s",
+           "n_gpus": 1, "steps": 20, "warmup": 3, "ms_per_step": res[0]["ms"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
+           "config": {"workload": "ps_nn_gemm on the U-Net's 3x3 convolution products and a square 4096^3 product"},
+           "roofline": {"bound": "mfma", "achieved": res[0]["tflops"], "peak": peak, "unit": "TFLOP/s",
+                        "frac": res[0]["frac_of_peak"], "traffic": None},
+           "cases": res}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +313,9 @@ def main():
     if args.workload == "adjoint":
         adjoint_study(args)
         return
+    if args.workload == "gemm":
+        gemm_study(args)
+        return
 
     import parallelstencil as ps
     from parallelstencil import workloads as W
@@ -293,10 +344,10 @@ def main():
     pa = (p["lam"], p["dt"], p["dx"], p["dy"], p["dz"])
     T, T2, Cpd = ps.zeros(n, n, n), ps.zeros(n, n, n), ps.zeros(n, n, n)
     Cpd.copy_from(Cp)
-    for name, memopt in (("plain", False), ("memopt", True)):
+    for name, memopt, fast in (("plain", False, False), ("memopt", True, False), ("memopt_fastmath", True, True)):
         if args.only and args.only != name:
             continue
-        k = W.define_diffusion3d(memopt, __name__)
+        k = W.define_diffusion3d(memopt, __name__, fastmath=fast)
         T.copy_from(T0)
         T2.copy_from(T0)
         st = [T, T2]

@@ -147,6 +147,20 @@ PS_DI float ps_divp(float a, float b, float r, bool& bad) {
   bad |= !(aq <= 0x1p50f) | ((aq < 0x1p-50f) & (aq != 0.0f));
   return q;
 }
+// fastmath reciprocal: hardware estimate refined by Newton steps r += r (1 - b r) (two for
+// double, one for float); the raw estimate is kept where the refinement is not finite
+// (b = 0, b = inf, NaN)
+PS_DI double ps_frcp(double b) {
+  const double r0 = __builtin_amdgcn_rcp(b);
+  double r = __builtin_fma(r0, __builtin_fma(-b, r0, 1.0), r0);
+  r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+  return r == r ? r : r0;
+}
+PS_DI float ps_frcp(float b) {
+  const float r0 = __builtin_amdgcn_rcpf(b);
+  const float r = __builtin_fmaf(r0, __builtin_fmaf(-b, r0, 1.0f), r0);
+  return r == r ? r : r0;
+}
 // Guarded load without a branch: element 0 of the array stands in for an index outside it
 // (the runtime only launches these kernels on non-empty arrays), and the value is then zeroed.
 template <class T>
@@ -497,6 +511,24 @@ EV Emitter::divide(const NodeP& n) {
   if (b.t == Ty::Bool) b = cast(b, Ty::Int);
   Ty t = promote(a.t, b.t);
   if (t == Ty::Int) t = Ty::F64;
+  if (K.opt.fastmath && (t == Ty::F64 || t == Ty::F32)) {
+    // a * RN(1/b): the host reciprocal of a scalar argument, the rounded reciprocal of a
+    // literal, otherwise the hardware reciprocal refined by Newton steps
+    if (d->k == NK::Ident && !locals.count(d->s)) {
+      auto it = K.param_index.find(d->s);
+      int k = it == K.param_index.end() ? -1 : it->second;
+      if (k >= 0 && !kinds[k].is_field && value_ty(kinds[k].dtype) == t) {
+        rcp_params[k] = t;
+        return EV{"(" + cast(a, t).c + " * " + an(k) + "_r)", t};
+      }
+    }
+    if (d->k == NK::Num && !d->is_int) {
+      double v = (d->is_f32 || NT == Ty::F32) ? (double)(float)d->num : d->num;
+      double r = t == Ty::F32 ? (double)(1.0f / (float)v) : 1.0 / v;
+      return EV{"(" + cast(a, t).c + " * " + hexlit(r, t) + ")", t};
+    }
+    return EV{"(" + cast(a, t).c + " * ps_frcp(" + cast(b, t).c + "))", t};
+  }
   if (fast_div && d->k == NK::Ident && !locals.count(d->s)) {
     auto it = K.param_index.find(d->s);
     int k = it == K.param_index.end() ? -1 : it->second;
@@ -846,6 +878,7 @@ void Emitter::block(const NodeP& b) {
 // to arrive and the barrier that publishes a tile only waits for LDS writes.
 void Emitter::emit_memopt_body(int L) {
   const MemoptMeta& mm = K.mm;
+  const int PD = std::max(1, K.opt.prefetch);  // prefetch distance in steps
   // points read per read-only array: (ox, oy) -> planes oz (oy = 0 when streaming dimension 2)
   struct Pts {
     int ai;
@@ -1013,7 +1046,7 @@ void Emitter::emit_memopt_body(int L) {
     int w = c.zhi - c.zlo + 1;
     Ty vt = value_ty(kinds[c.ai].dtype);
     for (int k = 0; k < w; ++k) o << pad() << cty(vt) << " " << c.name << "_" << k << ";\n";
-    o << pad() << cty(vt) << " " << c.name << "_n;\n";
+    for (int d = 0; d < PD; ++d) o << pad() << cty(vt) << " " << c.name << "_n" << d << ";\n";
   }
   // guarded load of column c at plane z (an expression); `extra` is an additional condition
   auto ldq = [&](const Col& c, const std::string& z, const std::string& extra) {
@@ -1058,7 +1091,7 @@ void Emitter::emit_memopt_body(int L) {
         << "_gy >= 1 && " << S << "_gy <= " << A << "_ny;\n";
       o << pad() << "const ps_int " << S << "_b = (" << S << "_gx - 1) + (" << S << "_gy - 1)*" << A << "_sx;\n";
       o << pad() << "const int " << S << "_l = " << S << "_y * " << t.W << " + " << S << "_x;\n";
-      o << pad() << cty(vt) << " " << S << "_h;\n";
+      for (int d = 0; d < PD; ++d) o << pad() << cty(vt) << " " << S << "_h" << d << ";\n";
     }
   }
   auto ldh = [&](const Tile& t, int j, const std::string& z, const std::string& extra) {
@@ -1071,23 +1104,28 @@ void Emitter::emit_memopt_body(int L) {
     if (dt == PS_BF16) v = "ps_bf16_ld(" + v + ")";
     return v;
   };
-  // prologue: the window below the first step, and the values of the first step
+  // prologue: the window below the first step, and the values of the first PD steps
+  auto within = [&](int d) { return d ? "(ps_zb + " + std::to_string(d) + " < ps_ze) && " : std::string(); };
   for (auto& c : cols) {
     int w = c.zhi - c.zlo + 1;
     for (int k = 0; k + 1 < w; ++k)
       o << pad() << c.name << "_" << k << " = " << ldq(c, "ps_zb + (" + std::to_string(c.zlo + k) + ")", "") << ";\n";
-    o << pad() << c.name << "_n = " << ldq(c, "ps_zb + (" + std::to_string(c.zhi) + ")", "") << ";\n";
+    for (int d = 0; d < PD; ++d)
+      o << pad() << c.name << "_n" << d << " = " << ldq(c, "ps_zb + (" + std::to_string(c.zhi + d) + ")", within(d)) << ";\n";
   }
   for (auto& t : tiles)
     for (int j = 0; j < t.nslots; ++j)
-      o << pad() << t.name << "_s" << j << "_h = " << ldh(t, j, "ps_zb + (" + std::to_string(t.oz) + ")", "") << ";\n";
+      for (int d = 0; d < PD; ++d)
+        o << pad() << t.name << "_s" << j << "_h" << d << " = " << ldh(t, j, "ps_zb + (" + std::to_string(t.oz + d) + ")", within(d)) << ";\n";
   o << pad() << "for (ps_int " << lc << " = ps_zb; " << lc << " < ps_ze; ++" << lc << ") {\n";
   ++ind;
-  o << pad() << "const bool ps_more = " << lc << " + 1 < ps_ze;\n";
+  // the loads issued in this step serve step lc + PD
+  o << pad() << "const bool ps_more = " << lc << " + " << PD << " < ps_ze;\n";
   for (auto& c : cols) {
     int w = c.zhi - c.zlo + 1;
-    o << pad() << c.name << "_" << (w - 1) << " = " << c.name << "_n;\n";
-    o << pad() << c.name << "_n = " << ldq(c, lc + " + (" + std::to_string(c.zhi + 1) + ")", "ps_more && ") << ";\n";
+    o << pad() << c.name << "_" << (w - 1) << " = " << c.name << "_n0;\n";
+    for (int d = 0; d + 1 < PD; ++d) o << pad() << c.name << "_n" << d << " = " << c.name << "_n" << d + 1 << ";\n";
+    o << pad() << c.name << "_n" << PD - 1 << " = " << ldq(c, lc + " + (" + std::to_string(c.zhi + PD) + ")", "ps_more && ") << ";\n";
   }
   if (!tiles.empty()) {
     o << pad() << "const int ps_buf = (int)((" << lc << " - ps_zb) & 1);\n";
@@ -1097,8 +1135,9 @@ void Emitter::emit_memopt_body(int L) {
         << "] = " << c.name << "_" << (t.oz - c.zlo) << ";\n";
       for (int j = 0; j < t.nslots; ++j) {
         std::string S = t.name + "_s" + std::to_string(j);
-        o << pad() << "if (" << S << "_v) " << t.name << "[ps_buf][" << S << "_l] = " << S << "_h;\n";
-        o << pad() << S << "_h = " << ldh(t, j, lc + " + (" + std::to_string(t.oz + 1) + ")", "ps_more && ") << ";\n";
+        o << pad() << "if (" << S << "_v) " << t.name << "[ps_buf][" << S << "_l] = " << S << "_h0;\n";
+        for (int d = 0; d + 1 < PD; ++d) o << pad() << S << "_h" << d << " = " << S << "_h" << d + 1 << ";\n";
+        o << pad() << S << "_h" << PD - 1 << " = " << ldh(t, j, lc + " + (" + std::to_string(t.oz + PD) + ")", "ps_more && ") << ";\n";
       }
     }
     o << pad() << "__syncthreads();\n";
@@ -1147,7 +1186,8 @@ void Emitter::emit_memopt_body(int L) {
 Emitted Emitter::run() {
   Emitted E;
   // variants carry their execution path in the symbol so that profiles tell them apart
-  std::string entry = "psk_" + sanitize(K.def.name) + (V.memopt ? (V.lds ? "_memopt_lds" : "_memopt") : "") + (V.checked ? "_checked" : "");
+  std::string entry = "psk_" + sanitize(K.def.name) + (V.memopt ? (V.lds ? "_memopt_lds" : "_memopt") : "") + (V.checked ? "_checked" : "") +
+                      (K.opt.fastmath ? "_fastmath" : "");
   E.entry = entry;
   const char* I = V.idx64 ? "long long" : "int";
   int ib = V.idx64 ? 8 : 4;
@@ -1186,9 +1226,9 @@ Emitted Emitter::run() {
   for (int d = 0; d < 3; ++d) E.params.push_back({4, -1, d, ib, 0});
   sig << "    int* ps_err";
   E.params.push_back({5, -1, 0, 8, 0});
-  for (auto& rp : rcp_params) {  // reciprocals of uniform divisors (0 = use IEEE division)
+  for (auto& rp : rcp_params) {  // reciprocals of uniform divisors (exact mode: 0 = use IEEE division)
     sig << ",\n    " << cty(rp.second) << " " << an(rp.first) << "_r";
-    E.params.push_back({6, rp.first, 0, rp.second == Ty::F64 ? 8 : 4, rp.second == Ty::F64 ? PS_F64 : PS_F32});
+    E.params.push_back({K.opt.fastmath ? 7 : 6, rp.first, 0, rp.second == Ty::F64 ? 8 : 4, rp.second == Ty::F64 ? PS_F64 : PS_F32});
   }
 
   std::ostringstream src;

@@ -301,7 +301,11 @@ static void read_options(Options& o, const JVal& j) {
     } else if (k == "nthreads_max_memopt") o.nthreads_max_memopt = (int)v.n;
     else if (k == "nthreads_x_max_memopt") o.nthreads_x_max_memopt = (int)v.n;
     else if (k == "fp_contract") o.fp_contract = v.b;
-    else if (k == "lds") o.lds = v.b;
+    else if (k == "fastmath") o.fastmath = v.b;
+    else if (k == "prefetch") {
+      o.prefetch = (int)v.n;
+      if (o.prefetch < 1 || o.prefetch > 4) throw PsError(PS_EKEYWORD, "prefetch must be 1, 2, 3 or 4");
+    } else if (k == "lds") o.lds = v.b;
     else if (k == "xcd_remap") o.xcd_remap = v.b;
     else if (k == "nt_store") o.nt_store = v.b;
     else if (k == "useshmemhalos" || k == "optimize_halo_read" || k == "nonconst_metadata" || k == "metadata_module" || k == "N") {
@@ -326,6 +330,7 @@ static void apply_prefix_kwargs(Options& o, const KernelDef& kd) {
       return (int)a.c;
     };
     if (k == "memopt") o.memopt = as_bool();
+    else if (k == "fastmath") o.fastmath = as_bool();
     else if (k == "inbounds") o.inbounds = as_bool();
     else if (k == "padding") o.padding = as_bool();
     else if (k == "ndims") o.ndims = as_int();

@@ -54,6 +54,10 @@ struct Options {
   int nthreads_x_max = 64;
   int nthreads_max = 256;
   bool fp_contract = false;               // Julia semantics: no FMA contraction
+  // FMA contraction, division by uniform divisors as multiplication by the host reciprocal and
+  // by other values through a refined hardware reciprocal (within a few ulp per operation)
+  bool fastmath = false;
+  int prefetch = 1;                       // memopt: loads are issued this many steps ahead
   bool lds = true;                        // memopt: stage x-y neighbours of optvars through LDS
   bool xcd_remap = true;                  // XCD-aware block ordering for memopt kernels
   bool nt_store = false;                  // nontemporal stores for array writes
@@ -90,7 +94,8 @@ struct Emitted {
   std::string entry;
   // kernel parameter layout: one entry per parameter in order
   struct P {
-    int kind;      // 0 field ptr, 1 int size, 2 scalar, 3 range start, 4 range len, 5 err ptr, 6 loop info
+    int kind;      // 0 field ptr, 1 int size, 2 scalar, 3 range start, 4 range len, 5 err ptr,
+                   // 6 RN(1/|b|) of a uniform divisor or 0 (exact division), 7 RN(1/b) (fastmath)
     int arg;       // arg index (fields/scalars)
     int which;     // for sizes: 0 nx 1 ny 2 nz 3 sx 4 sxy; for ranges: dim
     int bytes;     // 4 or 8

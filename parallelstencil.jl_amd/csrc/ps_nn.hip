@@ -39,42 +39,75 @@ template <int EPI>
 __global__ void __launch_bounds__(256) nn_gemm_k(int M, int N, int K, const bf16_t* __restrict__ A, long long sam, long long sak,
                                                  const bf16_t* __restrict__ B, long long sbk, long long sbn, void* __restrict__ C,
                                                  long long ldc, const float* __restrict__ bias, int kchunk) {
-  __shared__ bf16_t As[TM * LDK];
-  __shared__ bf16_t Bs[TN * LDK];
+  __shared__ __attribute__((aligned(16))) bf16_t As[TM * LDK];
+  __shared__ __attribute__((aligned(16))) bf16_t Bs[TN * LDK];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const long long m0 = (long long)blockIdx.x * TM, n0 = (long long)blockIdx.y * TN;
   const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
-  const bool a_kfast = sak == 1, b_kfast = sbk == 1;
   f32x4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = kb; k0 < ke; k0 += TK) {
-    // stage the A and B tiles (K fastest in LDS); consecutive threads walk the unit-stride index
-    for (int e = tid; e < TM * TK; e += 256) {
-      int r, c;
-      if (a_kfast) { r = e / TK; c = e % TK; } else { r = e % TM; c = e / TM; }
-      const long long m = m0 + r;
-      const int k = k0 + c;
-      As[r * LDK + c] = (m < M && k < ke) ? A[m * sam + (long long)k * sak] : (bf16_t)0;
+  // Each thread moves one 8-element piece of each 64x32 tile per K step: 8 consecutive k of one
+  // row when k has unit stride, else 8 consecutive rows of one k (unit stride along m / n).
+  // Whole, aligned pieces travel as one 16-byte load; edge or strided pieces element by element.
+  // The next step's pieces are loaded while the current tile is multiplied.
+  const bool a_kfast = sak == 1, b_kfast = sbk == 1;
+  const int ar = a_kfast ? tid / 4 : (tid % 8) * 8, ac = a_kfast ? (tid % 4) * 8 : tid / 8;
+  const int br = b_kfast ? tid / 4 : (tid % 8) * 8, bc = b_kfast ? (tid % 4) * 8 : tid / 8;
+  // A piece starts at an element index that is a multiple of 8 when the stride across pieces is
+  // (k0, ac, ar, m0 are all multiples of 8), so the 16-byte load is aligned
+  const bool a_vec = (a_kfast ? sam : sak) % 8 == 0 && ((uintptr_t)A & 15) == 0 && (a_kfast || sam == 1);
+  const bool b_vec = (b_kfast ? sbn : sbk) % 8 == 0 && ((uintptr_t)B & 15) == 0 && (b_kfast || sbn == 1);
+  s16x8 pa, pb;
+  auto fetch = [&](int k0) {
+    {  // A piece
+      const long long m = m0 + ar;
+      const int k = k0 + ac;
+      const bool full = a_kfast ? (m < M && k + 7 < ke) : (m + 7 < M && k < ke);
+      if (a_vec && full)
+        pa = *(const s16x8*)(A + m * sam + (long long)k * sak);
+      else
+        for (int t = 0; t < 8; ++t) {
+          const long long mm = a_kfast ? m : m + t;
+          const int kk = a_kfast ? k + t : k;
+          pa[t] = (mm < M && kk < ke) ? (short)A[mm * sam + (long long)kk * sak] : (short)0;
+        }
     }
-    for (int e = tid; e < TN * TK; e += 256) {
-      int r, c;
-      if (b_kfast) { r = e / TK; c = e % TK; } else { r = e % TN; c = e / TN; }
-      const long long n = n0 + r;
-      const int k = k0 + c;
-      Bs[r * LDK + c] = (n < N && k < ke) ? B[(long long)k * sbk + n * sbn] : (bf16_t)0;
+    {  // B piece
+      const long long n = n0 + br;
+      const int k = k0 + bc;
+      const bool full = b_kfast ? (n < N && k + 7 < ke) : (n + 7 < N && k < ke);
+      if (b_vec && full)
+        pb = *(const s16x8*)(B + (long long)k * sbk + n * sbn);
+      else
+        for (int t = 0; t < 8; ++t) {
+          const long long nn = b_kfast ? n : n + t;
+          const int kk = b_kfast ? k + t : k;
+          pb[t] = (nn < N && kk < ke) ? (short)B[(long long)kk * sbk + nn * sbn] : (short)0;
+        }
+    }
+  };
+  fetch(kb);
+  for (int k0 = kb; k0 < ke; k0 += TK) {
+    // tiles K-fastest in LDS
+    if (a_kfast) {
+      *(s16x8*)&As[ar * LDK + ac] = pa;
+    } else {
+      for (int t = 0; t < 8; ++t) As[(ar + t) * LDK + ac] = (bf16_t)pa[t];
+    }
+    if (b_kfast) {
+      *(s16x8*)&Bs[br * LDK + bc] = pb;
+    } else {
+      for (int t = 0; t < 8; ++t) Bs[(br + t) * LDK + bc] = (bf16_t)pb[t];
     }
     __syncthreads();
+    if (k0 + TK < ke) fetch(k0 + TK);
     const int i = lane & 15, kq = (lane >> 4) * 8;
     s16x8 a[2], b[2];
-    for (int q = 0; q < 2; ++q) {
-      const bf16_t* pa = &As[(wm * 32 + q * 16 + i) * LDK + kq];
-      const bf16_t* pb = &Bs[(wn * 32 + q * 16 + i) * LDK + kq];
-      for (int t = 0; t < 8; ++t) {
-        a[q][t] = (short)pa[t];
-        b[q][t] = (short)pb[t];
-      }
+    for (int q = 0; q < 2; ++q) {  // rows are LDK * 2 = 80 bytes apart, kq * 2 a multiple of 16
+      a[q] = *(const s16x8*)&As[(wm * 32 + q * 16 + i) * LDK + kq];
+      b[q] = *(const s16x8*)&Bs[(wn * 32 + q * 16 + i) * LDK + kq];
     }
     for (int bi = 0; bi < 2; ++bi)
       for (int bj = 0; bj < 2; ++bj)

@@ -257,7 +257,7 @@ void mkdirs(const std::string& d) {
 
 std::vector<std::string> compile_opts(const Kernel& K) {
   std::vector<std::string> o = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-  o.push_back(K.opt.fp_contract ? "-ffp-contract=fast" : "-ffp-contract=off");
+  o.push_back((K.opt.fp_contract || K.opt.fastmath) ? "-ffp-contract=fast" : "-ffp-contract=off");
   return o;
 }
 
@@ -1206,6 +1206,18 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
           std::memcpy(&v, &r, 8);
         } else {
           float b = std::fabs((float)a.f), r = rcp_safe32(b) ? 1.0f / b : 0.0f;
+          std::memcpy(&v, &r, 4);
+        }
+        isint = false;
+        break;
+      }
+      case 7: {  // fastmath: RN(1/b) of a uniform divisor
+        const ps_arg& a = args[p.arg];
+        if (p.dtype == PS_F64) {
+          double r = 1.0 / a.f;
+          std::memcpy(&v, &r, 8);
+        } else {
+          float r = 1.0f / (float)a.f;
           std::memcpy(&v, &r, 4);
         }
         isint = false;

@@ -739,7 +739,7 @@ class Kernel:
 
 
 _KW_DEF = ("ndims", "N", "inbounds", "padding", "memopt", "optvars", "loopdim", "loopsize", "optranges", "useshmemhalos",
-           "optimize_halo_read", "fp_contract", "lds", "xcd_remap", "nt_store", "nthreads_max_memopt",
+           "optimize_halo_read", "fp_contract", "fastmath", "prefetch", "lds", "xcd_remap", "nt_store", "nthreads_max_memopt",
            "nthreads_x_max_memopt")
 
 
@@ -753,7 +753,7 @@ def _define(kind, text, kw, mod, indices=None):
             "memopt": kw.get("memopt", cfg.memopt)}
     if indices is not None:
         opts["indices"] = list(indices)
-    for k in ("optvars", "loopdim", "loopsize", "fp_contract", "lds", "xcd_remap", "nt_store", "nthreads_max_memopt",
+    for k in ("optvars", "loopdim", "loopsize", "fp_contract", "fastmath", "prefetch", "lds", "xcd_remap", "nt_store", "nthreads_max_memopt",
               "nthreads_x_max_memopt"):
         if k in kw:
             v = kw[k]

@@ -68,8 +68,11 @@ def gemm(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias=None, epi=EPI_BF16, spl
     check(lib.ps_nn_gemm(M, N, K, A.p, sam, sak, B.p, sbk, sbn, C.p, ldc, bias.p if bias is not None else None, epi, splitk, None))
 
 
-def _splitk(K):
-    return max(1, min(64, K // 8192))
+def _splitk(M, N, K):
+    """K chunks for a product whose M x N tiles (64 x 64) alone do not fill the GPU: enough
+    chunks for about 1024 workgroups, each chunk at least 256 deep."""
+    tiles = -(-M // 64) * -(-N // 64)
+    return max(1, min(-(-1024 // tiles), K // 256))
 
 
 class Conv:
@@ -121,7 +124,7 @@ class Conv:
             check(lib.ps_nn_relu_bwd(dy.p, self.y.p, dy.n, None))
         check(lib.ps_nn_bias_grad(dy.p, M, self.cout, self.gb.p, None))
         # gw[kk, co] += sum_pixels col[p, kk] dy[p, co]
-        gemm(self.K, self.cout, M, self.x_col, 1, self.K, dy, self.cout, 1, self.gw, self.cout, epi=EPI_F32_ADD, splitk=_splitk(M))
+        gemm(self.K, self.cout, M, self.x_col, 1, self.K, dy, self.cout, 1, self.gw, self.cout, epi=EPI_F32_ADD, splitk=_splitk(self.K, self.cout, M))
         if not need_dx:
             return None
         dcol = self._buf("dcol", (M, self.K))
