@@ -85,6 +85,10 @@ typedef struct ps_launch {
   void*   stream;        /* hipStream_t, NULL = the library's stream 0      */
   int32_t async;         /* 0: block on the stream after the launch         */
   int32_t memopt;        /* -1: kernel default, 0 / 1: override              */
+  int32_t reserved;
+  int64_t shmem;         /* bytes of dynamic LDS for @sharedMem arrays: the
+                            shmem keyword of @parallel
+                            (src/ParallelKernel/parallel.jl:151,883-895)    */
 } ps_launch;
 
 typedef struct ps_devinfo {

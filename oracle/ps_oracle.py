@@ -35,6 +35,15 @@ that parity is not anchored on a formula merely retyped:
                     kernel text; a uniform velocity field gives zero strain rates and
                     zero divergence; residual definition of miniapps/Stokes2D.jl:117-123.
   bc_x/bc_y         KAT on an index-valued array.
+  diffusion2d_shmem_step  a linear field is stationary (flux differences are exactly 0);
+                    single-cell brute force in Python floats (diffusion2d_shmem_cell);
+                    a wrong sign or swapped neighbour changes a quadratic field's
+                    exactly known discrete Laplacian (2 on integer grids).
+  wave_*            closed forms in tests/test_oracle.py: the xor butterfly over
+                    masks 32..1 leaves every lane with the wave sum; shfl_up then
+                    shfl_down by the same delta restores the lanes away from the
+                    partition edges; shfl with lane = own lane is the identity;
+                    the ballot of "lane is even" is 0x5555...; popcounts.
 """
 import ctypes
 import os
@@ -433,3 +442,62 @@ def acoustic2d_misfit_gradient(K, Dobs, nt, dt, rho, dx, dy, src, isx, isy, rx0,
         gK[:, :] = gK - (dt * lP) * (((Vx1[1:, :] - Vx1[:-1, :]) / dx) + ((Vy1[:, 1:] - Vy1[:, :-1]) / dy))
         _ac2_adj_step(lP, lVx, lVy, K, dt, rho, dx, dy)
     return J, gK
+
+
+# ------------------------------------------------- kernel language (@parallel_indices)
+# examples/diffusion2D_shmem_novis.jl:11-27: the shared-memory tile T_l holds exactly the values of
+# T at the block's cells and halo, so on the CPU (one thread per block, kernel_language.jl:622-628)
+# and on the GPU the update is the 5-point flux form below, in the example's operator order, on the
+# cells 1 < ix < nx, 1 < iy < ny; T2 keeps its other values.
+def diffusion2d_shmem_step(T2, T, Ci, lam, dt, _dx, _dy):
+    c = T[1:-1, 1:-1]
+    qxp = (-lam * (T[2:, 1:-1] - c)) * _dx
+    qxm = (-lam * (c - T[:-2, 1:-1])) * _dx
+    qyp = (-lam * (T[1:-1, 2:] - c)) * _dy
+    qym = (-lam * (c - T[1:-1, :-2])) * _dy
+    T2[1:-1, 1:-1] = c + dt * (Ci[1:-1, 1:-1] * ((-(qxp - qxm)) * _dx - (qyp - qym) * _dy))
+
+
+def diffusion2d_shmem_cell(T, Ci, lam, dt, _dx, _dy, ix, iy):
+    """One cell of the same update in plain Python floats (0-based ix, iy): the brute-force check."""
+    t = lambda i, j: float(T[i, j])
+    a = ((-lam * (t(ix + 1, iy) - t(ix, iy))) * _dx) - ((-lam * (t(ix, iy) - t(ix - 1, iy))) * _dx)
+    b = ((-lam * (t(ix, iy + 1) - t(ix, iy))) * _dy) - ((-lam * (t(ix, iy) - t(ix, iy - 1))) * _dy)
+    return t(ix, iy) + dt * (float(Ci[ix, iy]) * (-a * _dx - b * _dy))
+
+
+# Wavefront model of the warp-level primitives (kernel_language.jl:123-193 docstrings), for one
+# wavefront of `w` lanes holding values v[0..w-1] (lane l = laneid() - 1). Lanes without a valid
+# partner keep their own value.
+def wave_shfl(v, lane, width):
+    """@shfl_sync(mask, v, lane, width): lane-1 of the lane's own width-partition (modulo width)."""
+    n = len(v)
+    return np.array([v[(l // width) * width + ((lane[l] - 1) % width)] for l in range(n)], dtype=np.asarray(v).dtype)
+
+
+def wave_shfl_up(v, delta, width):
+    n = len(v)
+    return np.array([v[l - delta] if (l % width) >= delta else v[l] for l in range(n)], dtype=np.asarray(v).dtype)
+
+
+def wave_shfl_down(v, delta, width):
+    n = len(v)
+    return np.array([v[l + delta] if (l % width) + delta < width else v[l] for l in range(n)], dtype=np.asarray(v).dtype)
+
+
+def wave_shfl_xor(v, lane_mask, width):
+    n = len(v)
+    out = []
+    for l in range(n):
+        p = l ^ lane_mask
+        out.append(v[p] if (p // width) == (l // width) else v[l])
+    return np.array(out, dtype=np.asarray(v).dtype)
+
+
+def wave_ballot(pred):
+    """Bit l set iff lane l's predicate holds (64-bit mask as int64 bits)."""
+    m = 0
+    for l, p in enumerate(pred):
+        if p:
+            m |= 1 << l
+    return np.array([m], dtype=np.uint64).view(np.int64)[0]

@@ -174,6 +174,29 @@ template <class T>
 PS_DI T ps_ldc(const T* p, bool c, ps_int i) {
   return p[c ? i : (ps_int)0];
 }
+// Wavefront shuffles within partitions of w lanes (w a power of two <= 64); a lane whose partner
+// lies outside its partition keeps its own value. Every form reads through one full-wave
+// permute, so any index wraps inside the wavefront.
+template <class T>
+PS_DI T ps_shfl(T v, int lane, int w) {
+  const int s = __lane_id();
+  return __shfl(v, (s & ~(w - 1)) + ((lane - 1) & (w - 1)), 64);
+}
+template <class T>
+PS_DI T ps_shfl_up(T v, unsigned d, int w) {
+  const int s = __lane_id();
+  return __shfl(v, (d >= (unsigned)w || (unsigned)(s & (w - 1)) < d) ? s : s - (int)d, 64);
+}
+template <class T>
+PS_DI T ps_shfl_down(T v, unsigned d, int w) {
+  const int s = __lane_id();
+  return __shfl(v, (d >= (unsigned)w || (unsigned)(s & (w - 1)) + d >= (unsigned)w) ? s : s + (int)d, 64);
+}
+template <class T>
+PS_DI T ps_shfl_xor(T v, int m, int w) {
+  const int s = __lane_id(), i = s ^ m;
+  return __shfl(v, ((i ^ s) & ~(w - 1)) ? s : i, 64);
+}
 PS_DI bool ps_chk(ps_int i, ps_int j, ps_int k, ps_int nx, ps_int ny, ps_int nz, int* err) {
   bool ok = i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz;
   if (!ok) *err = 1;
@@ -222,6 +245,15 @@ class Emitter {
   int E_lds_bytes = 0;  // static LDS of the emitted kernel
   EV divide(const NodeP& n);
   EV rhs_value(const NodeP& rhs, bool* opened);
+  // kernel language: element dtype of each @sharedMem array; the extent `end` stands for while
+  // an index of such an array is emitted
+  std::map<std::string, int> sm_dtype;
+  std::string end_subst;
+  int type_dtype(const NodeP& t) const;
+  EV kl_expr(const NodeP& n);
+  void kl_stmt(const NodeP& n);
+  void sm_create(const std::string& v, const NodeP& mac);
+  std::string sm_address(const NodeP& n, std::string* check);
 
   std::string pad() const { return std::string(ind * 2, ' '); }
   std::string an(int i) const { return "a" + std::to_string(i); }
@@ -356,6 +388,16 @@ std::string Emitter::offset(int ai, const std::vector<NodeP>& idx, std::string i
 EV Emitter::load(const NodeP& n) {
   if (n->a[0]->k != NK::Ident) throw PsError(PS_EARGUMENT, "unsupported indexing expression " + to_julia(n));
   const std::string& name = n->a[0]->s;
+  if (K.shared_arrays.count(name)) {  // @sharedMem array in LDS
+    std::string chk;
+    std::string addr = sm_address(n, &chk);
+    int dt = sm_dtype.at(name);
+    std::string raw = chk.empty() ? addr : "(" + chk + " ? " + addr + " : (" + elem_cty(dt) + ")0)";
+    if (dt == PS_BF16) return EV{"ps_bf16_ld(" + raw + ")", Ty::F32};
+    if (dt == PS_BOOL) return EV{"((bool)" + raw + ")", Ty::Bool};
+    if (dt == PS_I64 || dt == PS_I32) return EV{"((ps_int)" + raw + ")", Ty::Int};
+    return EV{raw, value_ty(dt)};
+  }
   int ai = field_index(name);
   if (ai < 0) throw PsError(PS_EARGUMENT, "'" + name + "' is indexed but is not an array argument of the kernel");
   auto idx = norm_indices(n, ai);
@@ -694,10 +736,17 @@ EV Emitter::expr(const NodeP& n) {
       throw PsError(PS_EARGUMENT, "UndefVarError: `" + n->s + "` not defined in kernel " + K.def.name);
     }
     case NK::Interp: throw PsError(PS_EARGUMENT, "unsupported interpolation $" + n->s);
-    case NK::EndKw: throw PsError(PS_EARGUMENT, "`end` is only valid inside array indexing");
+    case NK::EndKw:
+      if (!end_subst.empty()) return EV{end_subst, Ty::Int};
+      throw PsError(PS_EARGUMENT, "`end` is only valid inside array indexing");
     case NK::Index: return load(n);
     case NK::Call: return call(n);
-    case NK::Macro: throw PsError(PS_EPLUGIN, "macro @" + n->s + " is not available here");
+    case NK::Macro:
+      if (K.kernel_language) return kl_expr(n);
+      throw PsError(PS_EPLUGIN, "macro @" + n->s + " is not available here");
+    case NK::Dot:
+      if (K.kernel_language && n->a[0]->k == NK::Macro) return kl_expr(n);
+      throw PsError(PS_EARGUMENT, "unsupported expression: " + to_julia(n));
     case NK::Unary: {
       EV a = expr(n->a[0]);
       if (n->s == "+") return a;
@@ -742,8 +791,33 @@ void Emitter::assign(const NodeP& s) {
   const NodeP& lhs = s->a[0];
   NodeP rhs = s->a[1];
   if (!s->s.empty()) rhs = mk(NK::Binary, s->s, {clone(lhs), rhs});  // A op= x  ->  A = A op x
+  if (lhs->k == NK::Ident && rhs->k == NK::Macro && rhs->s == "sharedMem") {
+    if (!s->s.empty()) throw PsError(PS_EARGUMENT, "@sharedMem: use A = @sharedMem(T, dims)");
+    sm_create(lhs->s, rhs);
+    return;
+  }
+  if (lhs->k == NK::Index && lhs->a[0]->k == NK::Ident && K.shared_arrays.count(lhs->a[0]->s)) {
+    bool blk = false;
+    EV r = rhs_value(rhs, &blk);
+    std::string chk;
+    std::string addr = sm_address(lhs, &chk);
+    int dt = sm_dtype.at(lhs->a[0]->s);
+    std::string v;
+    if (dt == PS_BF16) v = "ps_bf16_st(" + cast(r, Ty::F32).c + ")";
+    else if (dt == PS_F64) v = cast(r, Ty::F64).c;
+    else if (dt == PS_F32) v = cast(r, Ty::F32).c;
+    else if (dt == PS_BOOL) v = "((unsigned char)(" + r.c + "))";
+    else v = std::string("((") + elem_cty(dt) + ")(" + r.c + "))";
+    o << pad() << (chk.empty() ? "" : "if (" + chk + ") ") << addr << " = " << v << ";\n";
+    if (blk) {
+      --ind;
+      o << pad() << "}\n";
+    }
+    return;
+  }
   if (lhs->k == NK::Ident) {
     const std::string& v = lhs->s;
+    if (K.shared_arrays.count(v)) throw PsError(PS_EARGUMENT, "'" + v + "' is a @sharedMem array; assign to its elements");
     if (K.param_index.count(v)) throw PsError(PS_EARGUMENT, "cannot assign to kernel argument '" + v + "' (arguments are passed by value)");
     for (int d = 0; d < 3; ++d)
       if (v == kIdx[d]) throw PsError(PS_EARGUMENT, "cannot assign to a parallel index");
@@ -862,6 +936,10 @@ void Emitter::stmt(const NodeP& s) {
       o << pad() << "return;\n";
       return;
     default: {
+      if (K.kernel_language && s->k == NK::Macro && (s->s == "sync_threads" || s->s == "pk_println" || s->s == "pk_show")) {
+        kl_stmt(s);
+        return;
+      }
       EV e = expr(s);
       o << pad() << "(void)(" << e.c << ");\n";
     }
@@ -870,6 +948,251 @@ void Emitter::stmt(const NodeP& s) {
 
 void Emitter::block(const NodeP& b) {
   for (auto& s : b->a) stmt(s);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel language (src/ParallelKernel/kernel_language.jl:9-193, AMDGPU mappings :300-580)
+// ---------------------------------------------------------------------------
+static int dtype_bytes(int dt) {
+  switch (dt) {
+    case PS_F64:
+    case PS_I64: return 8;
+    case PS_F32:
+    case PS_I32: return 4;
+    case PS_BF16: return 2;
+    default: return 1;
+  }
+}
+
+// Element type of @sharedMem(T, dims): eltype(A) of an array argument or a type name.
+int Emitter::type_dtype(const NodeP& t) const {
+  if (t->k == NK::Call && t->s == "eltype" && t->a.size() == 1 && t->a[0]->k == NK::Ident) {
+    int ai = field_index(t->a[0]->s);
+    if (ai < 0) throw PsError(PS_EARGUMENT, "eltype(" + t->a[0]->s + "): not an array argument of the kernel");
+    return kinds[ai].dtype;
+  }
+  std::string s;
+  if (t->k == NK::Ident) s = t->s;
+  else if (t->k == NK::Dot && t->a[0]->k == NK::Ident && t->a[0]->s == "Data") s = "Data." + t->s;
+  if (s == "Float64") return PS_F64;
+  if (s == "Float32") return PS_F32;
+  if (s == "BFloat16") return PS_BF16;
+  if (s == "Int64" || s == "Int") return PS_I64;
+  if (s == "Int32") return PS_I32;
+  if (s == "Bool") return PS_BOOL;
+  if (s == "Data.Number") return NT == Ty::F32 ? PS_F32 : PS_F64;
+  if (s == "Data.Index") return PS_I64;
+  throw PsError(PS_EARGUMENT, "@sharedMem: unsupported element type " + to_julia(t));
+}
+
+// A = @sharedMem(T, dims[, offset]): a column-major view of the launch's dynamic LDS starting
+// `offset` bytes in. The offset is honoured (CUDA.jl's @cuDynamicSharedMem semantics), so several
+// arrays can share the dynamic LDS without aliasing.
+void Emitter::sm_create(const std::string& v, const NodeP& mac) {
+  int dt = type_dtype(mac->a[0]);
+  auto it = sm_dtype.find(v);
+  if (it != sm_dtype.end() && it->second != dt) throw PsError(PS_EARGUMENT, "@sharedMem: '" + v + "' is created with different element types");
+  sm_dtype[v] = dt;
+  const NodeP& dn = mac->a[1];
+  std::vector<NodeP> dims = dn->k == NK::Tuple ? dn->a : std::vector<NodeP>{dn};
+  std::string d[3] = {"(ps_int)1", "(ps_int)1", "(ps_int)1"};
+  for (size_t i = 0; i < dims.size() && i < 3; ++i) {
+    EV e = expr(dims[i]);
+    if (e.t != Ty::Int) throw PsError(PS_EARGUMENT, "@sharedMem: dimensions must be integers");
+    d[i] = e.c;
+  }
+  std::string off = "(ps_int)0";
+  if (mac->a.size() == 3) {
+    EV e = expr(mac->a[2]);
+    if (e.t != Ty::Int) throw PsError(PS_EARGUMENT, "@sharedMem: the offset must be an integer (bytes)");
+    off = e.c;
+  }
+  const std::string P = "sm_" + sanitize(v);
+  o << pad() << P << "_n1 = " << d[0] << "; " << P << "_n2 = " << d[1] << "; " << P << "_n3 = " << d[2] << ";\n";
+  o << pad() << P << " = (" << elem_cty(dt) << "*)(ps_dyn_lds + (" << off << "));\n";
+  if (V.checked) {
+    const int es = dtype_bytes(dt);
+    // an array that does not fit the launch's dynamic LDS gets no elements: every access fails
+    // the bounds check instead of reaching past the allocation
+    o << pad() << "if ((" << off << ") < 0 || (" << off << ") % " << es << " != 0 || " << P << "_n1 < 0 || " << P << "_n2 < 0 || " << P
+      << "_n3 < 0 || (" << off << ") + " << P << "_n1 * " << P << "_n2 * " << P << "_n3 * " << es << " > ps_shmem) { *ps_err = 1; " << P
+      << "_n1 = 0; }\n";
+  }
+}
+
+// Address of an element of a @sharedMem array (1-based, column-major; one index = linear).
+std::string Emitter::sm_address(const NodeP& n, std::string* check) {
+  const std::string& v = n->a[0]->s;
+  if (!sm_dtype.count(v)) throw PsError(PS_EARGUMENT, "'" + v + "' is used before its @sharedMem");
+  const int nd = K.shared_arrays.at(v);
+  const std::string P = "sm_" + sanitize(v);
+  const std::string N[3] = {P + "_n1", P + "_n2", P + "_n3"};
+  const size_t ni = n->a.size() - 1;
+  if (ni != 1 && (int)ni != nd)
+    throw PsError(PS_EARGUMENT, "'" + v + "' has " + std::to_string(nd) + " dimensions; index it with 1 or " + std::to_string(nd) + " indices");
+  const std::string saved = end_subst;
+  std::string ic[3] = {"(ps_int)1", "(ps_int)1", "(ps_int)1"};
+  auto index = [&](const NodeP& e, const std::string& extent) {
+    end_subst = extent;
+    EV r = expr(e);
+    end_subst = saved;
+    if (r.t != Ty::Int) throw PsError(PS_EARGUMENT, "indices of '" + v + "' must be integers");
+    return r.c;
+  };
+  check->clear();
+  if (ni == 1 && nd > 1) {
+    const std::string len = "(" + N[0] + " * " + N[1] + " * " + N[2] + ")";
+    ic[0] = index(n->a[1], len);
+    if (V.checked) *check = "ps_chk(" + ic[0] + ", 1, 1, " + len + ", 1, 1, ps_err)";
+    return P + "[(" + ic[0] + ") - 1]";
+  }
+  for (int d = 0; d < nd; ++d) ic[d] = index(n->a[1 + d], N[d]);
+  std::string off = "((" + ic[0] + ") - 1)";
+  if (nd > 1) off += " + ((" + ic[1] + ") - 1) * " + N[0];
+  if (nd > 2) off += " + ((" + ic[2] + ") - 1) * " + N[0] + " * " + N[1];
+  if (V.checked) *check = "ps_chk(" + ic[0] + ", " + ic[1] + ", " + ic[2] + ", " + N[0] + ", " + N[1] + ", " + N[2] + ", ps_err)";
+  return P + "[" + off + "]";
+}
+
+// Value-producing kernel-language macros. Thread and block indices are 1-based, as AMDGPU.jl's
+// workitemIdx / workgroupIdx; wavefront operations work on gfx950's 64 lanes.
+EV Emitter::kl_expr(const NodeP& n) {
+  if (n->k == NK::Dot) {
+    const std::string& m = n->a[0]->s;
+    const std::string& c = n->s;
+    if (!n->a[0]->a.empty()) throw PsError(PS_EARGUMENT, "@" + m + ": no arguments allowed.");
+    if (c != "x" && c != "y" && c != "z") throw PsError(PS_EARGUMENT, "@" + m + "()." + c + ": only the fields x, y and z exist");
+    if (m == "threadIdx") return EV{"((ps_int)threadIdx." + c + " + 1)", Ty::Int};
+    if (m == "blockIdx") return EV{"((ps_int)blockIdx." + c + " + 1)", Ty::Int};
+    if (m == "blockDim") return EV{"((ps_int)blockDim." + c + ")", Ty::Int};
+    if (m == "gridDim") return EV{"((ps_int)gridDim." + c + ")", Ty::Int};
+    throw PsError(PS_EARGUMENT, "unsupported expression: " + to_julia(n));
+  }
+  const std::string& m = n->s;
+  if (m == "threadIdx" || m == "blockIdx" || m == "blockDim" || m == "gridDim")
+    throw PsError(PS_EARGUMENT, "@" + m + "() returns a triple; use @" + m + "().x, .y or .z");
+  if (m == "warpsize") return EV{"((ps_int)64)", Ty::Int};
+  if (m == "laneid") return EV{"((ps_int)__lane_id() + 1)", Ty::Int};
+  if (m == "active_mask") return EV{"((ps_int)__ballot(1))", Ty::Int};
+  auto mask = [&]() {
+    EV k = expr(n->a[0]);
+    if (k.t != Ty::Int) throw PsError(PS_EARGUMENT, "@" + m + ": the mask must be an integer");
+    return "((unsigned long long)(" + k.c + "))";
+  };
+  auto intarg = [&](size_t i, const char* what) {
+    EV e = expr(n->a[i]);
+    if (e.t != Ty::Int) throw PsError(PS_EARGUMENT, "@" + m + ": " + what + " must be an integer");
+    if (n->a[i]->k == NK::Num && std::string(what) == "width") {
+      long long w = n->a[i]->ival;
+      if (w < 1 || w > 64 || (w & (w - 1))) throw PsError(PS_EARGUMENT, "@" + m + ": width must be a power of two between 1 and 64");
+    }
+    return "((int)(" + e.c + "))";
+  };
+  if (m == "shfl_sync" || m == "shfl_up_sync" || m == "shfl_down_sync" || m == "shfl_xor_sync") {
+    (void)mask();  // every lane of the wavefront takes part; the mask names no subset here
+    EV v = expr(n->a[1]);
+    const std::string w = n->a.size() == 4 ? intarg(3, "width") : "64";
+    std::string fn, sel;
+    if (m == "shfl_sync") {
+      fn = "ps_shfl";
+      sel = intarg(2, "lane");  // 1-based source lane within the partition
+    } else if (m == "shfl_up_sync") {
+      fn = "ps_shfl_up";
+      sel = "(unsigned int)" + intarg(2, "delta");
+    } else if (m == "shfl_down_sync") {
+      fn = "ps_shfl_down";
+      sel = "(unsigned int)" + intarg(2, "delta");
+    } else {
+      fn = "ps_shfl_xor";
+      sel = intarg(2, "lane_mask");  // partner lane = (laneid - 1) xor lane_mask
+    }
+    const std::string tail = ", " + sel + ", " + w + ")";
+    switch (v.t) {
+      case Ty::Bool: return EV{"((bool)" + fn + "((int)(" + v.c + ")" + tail + ")", Ty::Bool};
+      case Ty::Int: return EV{"((ps_int)" + fn + "((ps_int)(" + v.c + ")" + tail + ")", Ty::Int};
+      case Ty::F32: return EV{fn + "((float)(" + v.c + ")" + tail, Ty::F32};
+      default: return EV{fn + "((double)(" + v.c + ")" + tail, Ty::F64};
+    }
+  }
+  if (m == "vote_any_sync" || m == "vote_all_sync" || m == "vote_ballot_sync") {
+    const std::string k = mask();
+    EV p = expr(n->a[1]);
+    if (p.t != Ty::Bool) throw PsError(PS_EARGUMENT, "@" + m + ": the predicate must be a Bool");
+    const std::string b = "__ballot((int)(" + p.c + "))";
+    if (m == "vote_any_sync") return EV{"((" + b + " & " + k + ") != 0ull)", Ty::Bool};
+    if (m == "vote_all_sync") return EV{"((" + b + " & " + k + ") == (" + k + " & __ballot(1)))", Ty::Bool};
+    return EV{"((ps_int)(" + b + " & " + k + "))", Ty::Int};
+  }
+  throw PsError(PS_EARGUMENT, "@" + m + " is a statement, not a value");
+}
+
+// Statement macros: @sync_threads() is a workgroup barrier; @pk_println(args...) prints its
+// arguments concatenated (Base.println); @pk_show(args...) prints `expr = value` per argument.
+// Floats print with 16 (Float64) or 8 (Float32) significant digits.
+void Emitter::kl_stmt(const NodeP& s) {
+  if (s->s == "sync_threads") {
+    if (!s->a.empty()) throw PsError(PS_EARGUMENT, "@sync_threads: no arguments allowed.");
+    o << pad() << "__syncthreads();\n";
+    return;
+  }
+  auto esc = [](const std::string& t) {
+    std::string r;
+    for (char c : t) {
+      if (c == '%') r += "%%";
+      else if (c == '"') r += "\\\"";
+      else r += c;
+    }
+    return r;
+  };
+  auto add = [&](const NodeP& a, std::string& f, std::vector<std::string>& vals) {
+    if (a->k == NK::Str) {
+      f += esc(a->s);
+      return;
+    }
+    EV e = expr(a);
+    switch (e.t) {
+      case Ty::Int:
+        f += "%lld";
+        vals.push_back("(long long)(" + e.c + ")");
+        break;
+      case Ty::F64:
+        f += "%.16g";
+        vals.push_back("(double)(" + e.c + ")");
+        break;
+      case Ty::F32:
+        f += "%.8g";
+        vals.push_back("(double)(" + e.c + ")");
+        break;
+      case Ty::Bool:
+        f += "%s";
+        vals.push_back("((" + e.c + ") ? \"true\" : \"false\")");
+        break;
+    }
+  };
+  auto emit = [&](const std::string& f, const std::vector<std::string>& vals) {
+    o << pad() << "printf(\"" << f << "\\n\"";
+    for (auto& v : vals) o << ", " << v;
+    o << ");\n";
+  };
+  if (s->s == "pk_println") {
+    std::string f;
+    std::vector<std::string> vals;
+    for (auto& a : s->a) add(a, f, vals);
+    emit(f, vals);
+    return;
+  }
+  for (auto& a : s->a) {
+    std::string text = to_julia(a);
+    for (int d = 0; d < 3 && d < (int)K.opt.indices.size(); ++d) {  // the kernel's own index names
+      const std::string from = kIdx[d];
+      for (size_t p = text.find(from); p != std::string::npos; p = text.find(from, p + K.opt.indices[d].size()))
+        text.replace(p, from.size(), K.opt.indices[d]);
+    }
+    std::string f = esc(text) + " = ";
+    std::vector<std::string> vals;
+    add(a, f, vals);
+    emit(f, vals);
+  }
 }
 
 // memopt: thread per column, stream along dimension L (0-based) in chunks of loopsize.
@@ -1226,6 +1549,10 @@ Emitted Emitter::run() {
   for (int d = 0; d < 3; ++d) E.params.push_back({4, -1, d, ib, 0});
   sig << "    int* ps_err";
   E.params.push_back({5, -1, 0, 8, 0});
+  if (!K.shared_arrays.empty()) {  // bytes of dynamic LDS the launch provides (@sharedMem)
+    sig << ",\n    ps_int ps_shmem";
+    E.params.push_back({8, -1, 0, ib, 0});
+  }
   for (auto& rp : rcp_params) {  // reciprocals of uniform divisors (exact mode: 0 = use IEEE division)
     sig << ",\n    " << cty(rp.second) << " " << an(rp.first) << "_r";
     E.params.push_back({K.opt.fastmath ? 7 : 6, rp.first, 0, rp.second == Ty::F64 ? 8 : 4, rp.second == Ty::F64 ? PS_F64 : PS_F32});
@@ -1234,6 +1561,7 @@ Emitted Emitter::run() {
   std::ostringstream src;
   src << "// generated by libpsamd for kernel " << K.def.name << " (variant " << V.key() << ")\n";
   src << "typedef " << I << " ps_int;\n" << kPrelude << "\n";
+  if (!K.shared_arrays.empty()) src << "extern __shared__ __attribute__((aligned(16))) unsigned char ps_dyn_lds[];\n";
   src << "extern \"C\" __global__ void __launch_bounds__(" << V.maxthreads << ") " << entry << "(\n" << sig.str() << ") {\n";
   if (!V.memopt) {
     o << "  const ps_int tx = (ps_int)blockIdx.x * (ps_int)blockDim.x + (ps_int)threadIdx.x;\n";
@@ -1248,6 +1576,11 @@ Emitted Emitter::run() {
       o << "  const ps_int " << A << "_o = (ix - 1) + (iy - 1)*" << A << "_sx + (iz - 1)*" << A << "_sxy; (void)" << A << "_o;\n";
     }
     for (auto& v : local_order) o << "  " << cty(locals[v]) << " " << local_c[v] << " = 0;\n";
+    for (auto& sm : sm_dtype) {
+      const std::string P = "sm_" + sanitize(sm.first);
+      o << "  " << elem_cty(sm.second) << "* " << P << " = nullptr; ps_int " << P << "_n1 = 0, " << P << "_n2 = 1, " << P << "_n3 = 1;\n";
+      o << "  (void)" << P << "; (void)" << P << "_n1; (void)" << P << "_n2; (void)" << P << "_n3;\n";
+    }
     ind = 1;
     block(K.body);
   } else {
@@ -1422,7 +1755,13 @@ struct Prover {
     for (int d = 0; d < 3; ++d)
       if (lo[d] > hi[d]) return;  // unreachable
     auto it = K.param_index.find(n->a[0]->s);
-    if (it == K.param_index.end()) return;
+    if (it == K.param_index.end()) {
+      if (K.shared_arrays.count(n->a[0]->s)) {  // extents known only on the device
+        ok = false;
+        why = "access " + to_julia(n) + " to a @sharedMem array";
+      }
+      return;
+    }
     int ai = it->second;
     if (!S.kinds[ai].is_field) return;
     size_t ni = n->a.size() - 1;

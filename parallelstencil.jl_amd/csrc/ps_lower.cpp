@@ -568,6 +568,36 @@ static void memopt_analysis(Kernel& K) {
   }
 }
 
+// Kernel language of @parallel_indices kernels (src/ParallelKernel/kernel_language.jl:9-193, argument
+// checks :264-295). Validates the argument count of each macro call; the lowering happens in the
+// emitter.
+static void check_kernel_language(Kernel& K, const NodeP& x) {
+  const std::string& m = x->s;
+  const size_t n = x->a.size();
+  auto nargs = [&](size_t lo, size_t hi) {
+    if (n < lo || n > hi) throw PsError(PS_EARGUMENT, "@" + m + ": " + (hi == 0 ? "no arguments allowed." : "wrong number of arguments."));
+  };
+  if (m == "gridDim" || m == "blockIdx" || m == "blockDim" || m == "threadIdx" || m == "sync_threads" || m == "warpsize" || m == "laneid") {
+    nargs(0, 0);
+  } else if (m == "active_mask") {
+    nargs(0, 0);
+    K.needs_idx64 = true;
+  } else if (m == "sharedMem") {
+    nargs(2, 3);
+  } else if (m == "shfl_sync" || m == "shfl_up_sync" || m == "shfl_down_sync" || m == "shfl_xor_sync") {
+    nargs(3, 4);
+  } else if (m == "vote_any_sync" || m == "vote_all_sync") {
+    nargs(2, 2);
+  } else if (m == "vote_ballot_sync") {
+    nargs(2, 2);
+    K.needs_idx64 = true;
+  } else if (m == "pk_println" || m == "pk_show") {
+  } else {
+    throw PsError(PS_EARGUMENT, "@parallel_indices kernels index arrays explicitly; macro @" + m + " is not supported here");
+  }
+  K.kernel_language = true;
+}
+
 Kernel define_kernel(const std::string& text, const JVal& opts) {
   Kernel K;
   K.def = parse_kernel(text);
@@ -669,9 +699,32 @@ Kernel define_kernel(const std::string& text, const JVal& opts) {
         for (int d = 0; d < 3; ++d)
           if (x->s == std::string(1, "xyz"[d]).insert(0, "i")) return mk(NK::Ident, kIdx[d]);
       }
-      if (x->k == NK::Macro) throw PsError(PS_EARGUMENT, "@parallel_indices kernels index arrays explicitly; macro @" + x->s + " is not supported here");
+      if (x->k == NK::Macro) check_kernel_language(K, x);
       return x;
     });
+    // local arrays created by @sharedMem
+    walk(body, [&](const NodeP& x) {
+      if (x->k == NK::Assign && x->a[0]->k == NK::Ident && x->a[1]->k == NK::Macro && x->a[1]->s == "sharedMem") {
+        const std::string& v = x->a[0]->s;
+        if (K.param_index.count(v)) throw PsError(PS_EARGUMENT, "@sharedMem: cannot assign to kernel argument '" + v + "'");
+        const NodeP& dims = x->a[1]->a[1];
+        int nd = dims->k == NK::Tuple ? (int)dims->a.size() : 1;
+        if (nd < 1 || nd > 3) throw PsError(PS_EARGUMENT, "@sharedMem: arrays of 1 to 3 dimensions are supported");
+        auto it = K.shared_arrays.find(v);
+        if (it != K.shared_arrays.end() && it->second != nd)
+          throw PsError(PS_EARGUMENT, "@sharedMem: '" + v + "' is created with different numbers of dimensions");
+        K.shared_arrays[v] = nd;
+      } else if (x->k == NK::Macro && x->s == "sharedMem") {
+        // a @sharedMem anywhere but as the whole right-hand side of a local assignment
+        bool ok = false;
+        walk(body, [&](const NodeP& y) {
+          if (y->k == NK::Assign && y->a[1].get() == x.get() && y->a[0]->k == NK::Ident) ok = true;
+        });
+        if (!ok) throw PsError(PS_EARGUMENT, "@sharedMem must be assigned to a local variable: A = @sharedMem(T, dims)");
+      }
+    });
+    if (K.kernel_language && K.opt.memopt)
+      throw PsError(PS_EARGUMENT, "memopt kernels stage their own tiles through LDS; the kernel language (@sharedMem, @sync_threads, @threadIdx, ...) is only available with memopt=false");
     // all parallel indices must be used (src/parallel.jl:696-704)
     for (int d = 0; d < K.nidx; ++d) {
       bool used = false;
@@ -696,6 +749,7 @@ std::string kernel_info_json(const Kernel& K) {
   std::ostringstream o;
   o << "{\"name\":" << json_str(K.def.name) << ",\"kind\":" << json_str(K.opt.kind)
     << ",\"is_parallel_kernel\":" << (K.opt.kind == "parallel" ? "true" : "false") << ",\"nb_parallel_indices\":" << K.nidx
+    << ",\"kernel_language\":" << (K.kernel_language ? "true" : "false") << ",\"shared_arrays\":" << K.shared_arrays.size()
     << ",\"numbertype\":" << json_str(K.opt.numbertype) << ",\"memopt\":" << (K.opt.memopt ? "true" : "false") << ",\"params\":[";
   for (size_t i = 0; i < K.def.params.size(); ++i)
     o << (i ? "," : "") << "{\"name\":" << json_str(K.def.params[i].name) << ",\"annot\":" << json_str(K.def.params[i].annot) << "}";

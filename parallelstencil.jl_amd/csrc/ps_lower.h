@@ -73,6 +73,11 @@ struct Kernel {
   std::vector<std::string> otf_vars;
   std::map<std::string, int> param_index;
   MemoptMeta mm;
+  // kernel language (@parallel_indices kernels): local arrays created by @sharedMem (name ->
+  // number of dimensions), and whether the kernel needs 64-bit integers (wavefront masks)
+  std::map<std::string, int> shared_arrays;
+  bool kernel_language = false;
+  bool needs_idx64 = false;
 };
 
 struct Variant {
@@ -95,7 +100,8 @@ struct Emitted {
   // kernel parameter layout: one entry per parameter in order
   struct P {
     int kind;      // 0 field ptr, 1 int size, 2 scalar, 3 range start, 4 range len, 5 err ptr,
-                   // 6 RN(1/|b|) of a uniform divisor or 0 (exact division), 7 RN(1/b) (fastmath)
+                   // 6 RN(1/|b|) of a uniform divisor or 0 (exact division), 7 RN(1/b) (fastmath),
+                   // 8 bytes of dynamic LDS of the launch (@sharedMem)
     int arg;       // arg index (fields/scalars)
     int which;     // for sizes: 0 nx 1 ny 2 nz 3 sx 4 sxy; for ranges: dim
     int bytes;     // 4 or 8

@@ -891,6 +891,11 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   if (nargs != (int)K.def.params.size())
     throw PsError(PS_EARGUMENT, "MethodError: kernel " + K.def.name + " takes " + std::to_string(K.def.params.size()) + " arguments (" +
                                     std::to_string(nargs) + " given)");
+  const int64_t shmem = L ? L->shmem : 0;
+  if (shmem < 0) throw PsError(PS_EARGUMENT, "@parallel <kernelcall>: shmem must be a non-negative number of bytes");
+  if (!K.shared_arrays.empty() && shmem == 0)
+    throw PsError(PS_EARGUMENT, "@parallel <kernelcall>: kernel " + K.def.name +
+                                    " creates @sharedMem arrays; the amount of shared memory needs to be specified when launching the kernel (keyword argument `shmem`)");
   bool memopt = K.mm.enabled;
   if (L && L->memopt == 1 && !memopt) throw PsError(PS_EINCOHERENT, "@parallel memopt=true: the kernel was not defined with memopt=true");
   if (L && L->memopt == 0) memopt = false;
@@ -1006,7 +1011,7 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   }
   for (int d = 0; d < 3; ++d)
     if (P.ranges[d][1] > INT32_MAX / 2 || P.ranges[d][0] < INT32_MIN / 2) big = true;
-  v.idx64 = big;
+  v.idx64 = big || K.needs_idx64;
   v.divmode = divmode_of(args, nargs);
   // aliasing: a written field that overlaps another field argument
   for (int i = 0; i < nargs && v.noalias; ++i) {
@@ -1098,6 +1103,7 @@ std::string launch_key(const ps_arg* args, int nargs, const ps_launch* L) {
     put(L->nthreads, 12);
     put(L->nblocks, 12);
     put(&L->memopt, 4);
+    put(&L->shmem, 8);
   }
   return k;
 }
@@ -1211,6 +1217,7 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
         isint = false;
         break;
       }
+      case 8: iv = launch ? launch->shmem : 0; break;
       case 7: {  // fastmath: RN(1/b) of a uniform divisor
         const ps_arg& a = args[p.arg];
         if (p.dtype == PS_F64) {
@@ -1236,8 +1243,16 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
     ptrs[i] = &slots[i];
   }
   hipStream_t st = stream_of(launch ? launch->stream : nullptr);
+  const unsigned shmem = launch ? (unsigned)launch->shmem : 0u;
+  if (shmem) {
+    static int lds_max = -1;
+    if (lds_max < 0) PS_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, S().device));
+    if ((int64_t)shmem + m->em.lds_bytes > lds_max)
+      return fail(PS_EARGUMENT, "@parallel <kernelcall>: shmem=" + std::to_string(shmem) + " bytes exceeds the " + std::to_string(lds_max) +
+                                    " bytes of LDS a workgroup can use");
+  }
   hipError_t e = hipModuleLaunchKernel(m->fn, (unsigned)P.nblocks[0], (unsigned)P.nblocks[1], (unsigned)P.nblocks[2], (unsigned)P.nthreads[0],
-                                       (unsigned)P.nthreads[1], (unsigned)P.nthreads[2], 0, st, ptrs.data(), nullptr);
+                                       (unsigned)P.nthreads[1], (unsigned)P.nthreads[2], shmem, st, ptrs.data(), nullptr);
   if (e != hipSuccess) return fail(PS_EHIP, std::string("hipModuleLaunchKernel: ") + hipGetErrorString(e));
   if (!(launch && launch->async)) {
     PS_HIP(hipStreamSynchronize(st));

@@ -11,6 +11,6 @@ from .core import (AMDGPU, Float64, Float32, BFloat16, Int32, Int64, Bool, Array
                    IncoherentCallError, Kernel, Stream, UnitRange, bf16_to_f32, compute_nblocks, compute_nthreads,
                    device_info, ensure_device, f32_to_bf16, falses, fill, fill_, get_priority_stream, get_ranges,
                    get_ranges_inner, get_ranges_outer, get_stream, hide_communication, init_parallel_stencil,
-                   is_initialized, ones, parallel, parallel_async, parallel_indices, rand, reset_parallel_stencil, seed,
+                   is_initialized, ones, overlap, parallel, parallel_async, parallel_indices, rand, reset_parallel_stencil, seed,
                    stream_context, synchronize, trues, zeros)
 from . import FieldAllocators, FiniteDifferences1D, FiniteDifferences2D, FiniteDifferences3D  # noqa: E402

@@ -115,6 +115,8 @@ class ps_launch(ctypes.Structure):
         ("stream", ctypes.c_void_p),
         ("async_", ctypes.c_int32),
         ("memopt", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("shmem", ctypes.c_int64),
     ]
 
 

@@ -444,6 +444,8 @@ def stream_context(s):
 
 def synchronize(stream=None):
     """@synchronize() / @synchronize(stream)."""
+    if getattr(_tls, "overlap", None) is not None:
+        raise ArgumentError(ERRMSG_OVERLAP)
     ensure_device()
     if stream is None:
         check(lib.ps_device_sync())
@@ -670,7 +672,7 @@ class Kernel:
                     a.i = int(v)
         return arr
 
-    def _launch_struct(self, ranges, nblocks, nthreads, stream, async_, memopt):
+    def _launch_struct(self, ranges, nblocks, nthreads, stream, async_, memopt, shmem=None):
         L = ps_launch()
         if ranges is not None:
             for d, (a, b) in enumerate(_range3(ranges)):
@@ -686,26 +688,40 @@ class Kernel:
         L.stream = _stream_handle(stream)
         L.async_ = 1 if async_ else 0
         L.memopt = -1 if memopt is None else int(bool(memopt))
+        if shmem is not None:
+            if int(shmem) < 0:
+                raise ArgumentError("shmem must be a non-negative number of bytes")
+            L.shmem = int(shmem)
         return L
 
-    def __call__(self, *args, ranges=None, nblocks=None, nthreads=None, stream=None, async_=False, memopt=None):
+    def __call__(self, *args, ranges=None, nblocks=None, nthreads=None, stream=None, async_=False, memopt=None, shmem=None):
+        """Launches the kernel (@parallel); `shmem` is the number of bytes of dynamic LDS that
+        @sharedMem arrays of the kernel use."""
+        ov = getattr(_tls, "overlap", None)
+        if ov is not None:  # inside an @overlap block: stream i for the i-th call
+            if stream is not None:
+                raise ArgumentError(ERRMSG_INVALID_STREAM_OVERLAP)
+            if async_:
+                raise ArgumentError(ERRMSG_OVERLAP)
+            stream, async_ = get_stream(len(ov) + 1), True
+            ov.append(stream)
         ensure_device()
         arr = self._args(args)
-        L = self._launch_struct(ranges, nblocks, nthreads, stream, async_, memopt)
+        L = self._launch_struct(ranges, nblocks, nthreads, stream, async_, memopt, shmem)
         check(lib.ps_kernel_launch(self._h, arr, len(args), ctypes.byref(L)))
 
-    def launcher(self, *args, ranges=None, nblocks=None, nthreads=None, stream=None, async_=True, memopt=None):
+    def launcher(self, *args, ranges=None, nblocks=None, nthreads=None, stream=None, async_=True, memopt=None, shmem=None):
         """A prepared launch of this kernel with these arguments (asynchronous by default):
         calling it launches again without rebuilding the argument block, for launch-bound loops
         and graph capture. It keeps its arguments alive; `set(i, value)` replaces one."""
         ensure_device()
         return BoundLaunch(self, args, dict(ranges=ranges, nblocks=nblocks, nthreads=nthreads, stream=stream,
-                                            async_=async_, memopt=memopt))
+                                            async_=async_, memopt=memopt, shmem=shmem))
 
-    def geometry(self, *args, ranges=None, nblocks=None, nthreads=None, memopt=None):
+    def geometry(self, *args, ranges=None, nblocks=None, nthreads=None, memopt=None, shmem=None):
         """(memopt, nthreads, nblocks, ranges) the launch would use."""
         arr = self._args(args)
-        L = self._launch_struct(ranges, nblocks, nthreads, None, True, memopt)
+        L = self._launch_struct(ranges, nblocks, nthreads, None, True, memopt, shmem)
         nt = (ctypes.c_int32 * 3)()
         nb = (ctypes.c_int32 * 3)()
         rg = ((ctypes.c_int64 * 2) * 3)()
@@ -730,11 +746,11 @@ class Kernel:
         flags = (1 if mo else 0) | (2 if checked else 0) | (4 if idx64 else 0) | (16 if maxthreads > 256 else 0)
         check(lib.ps_kernel_precompile(self._h, arr, len(args), flags))
 
-    def precompile_launch(self, *args, ranges=None, nblocks=None, nthreads=None, memopt=None):
+    def precompile_launch(self, *args, ranges=None, nblocks=None, nthreads=None, memopt=None, shmem=None):
         """Compiles exactly the variant a launch with these arguments would use (no GPU needed);
         fields may be ArgSpecs."""
         arr = self._args(args)
-        L = self._launch_struct(ranges, nblocks, nthreads, None, True, memopt)
+        L = self._launch_struct(ranges, nblocks, nthreads, None, True, memopt, shmem)
         check(lib.ps_kernel_precompile_launch(self._h, arr, len(args), ctypes.byref(L)))
 
 
@@ -811,6 +827,63 @@ def parallel_indices(indices_or_text, text=None, **kw):
     if isinstance(idx, str):
         idx = [s.strip() for s in idx.strip("() ").split(",") if s.strip()]
     return _define("parallel_indices", text, kw, mod, indices=idx)
+
+
+# ---------------------------------------------------------------------------
+# @overlap (src/ParallelKernel/overlap.jl)
+# ---------------------------------------------------------------------------
+ERRMSG_OVERLAP = "@overlap: the code block must only contain @parallel calls. Type `?@overlap` for more information."
+ERRMSG_INVALID_STREAM_OVERLAP = ("@overlap: invalid keyword argument in @parallel <kernelcall>: stream. Specifying the stream is not "
+                                 "allowed in a @parallel call inside the code block argument of @overlap.")
+
+
+class _OverlapBlock:
+    def __enter__(self):
+        if getattr(_tls, "overlap", None) is not None:
+            raise ArgumentError(ERRMSG_OVERLAP)
+        self.streams = _tls.overlap = []
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        _tls.overlap = None
+        if exc_type is not None:
+            for st in self.streams:  # leave no launch of the block running behind the error
+                st.synchronize()
+            return False
+        if not self.streams:
+            raise ArgumentError(ERRMSG_OVERLAP)
+        for st in self.streams:
+            st.synchronize()
+        return False
+
+
+def overlap(*calls):
+    """@overlap (src/ParallelKernel/overlap.jl:47-58): runs the i-th @parallel call of the block
+    asynchronously on stream i of the normal pool (get_stream(i)) and synchronises those streams
+    at the end of the block. Two forms:
+
+        with overlap():
+            kernel1(A)
+            kernel2(B)
+
+        overlap((kernel1, (A,)), (kernel2, (B,), {"nblocks": ..., "nthreads": ...}))
+
+    Inside the block, a call may not name a stream, be asynchronous or synchronise."""
+    if not calls:
+        return _OverlapBlock()
+    norm = []
+    for c in calls:
+        if not (isinstance(c, tuple) and 2 <= len(c) <= 3 and isinstance(c[0], Kernel)):
+            raise ArgumentError(ERRMSG_OVERLAP)
+        kw = dict(c[2]) if len(c) == 3 else {}
+        if "stream" in kw:
+            raise ArgumentError(ERRMSG_INVALID_STREAM_OVERLAP)
+        if kw.get("async_"):
+            raise ArgumentError(ERRMSG_OVERLAP)
+        norm.append((c[0], tuple(c[1]), kw))
+    with _OverlapBlock():
+        for k, args, kw in norm:
+            k(*args, **kw)
 
 
 # ---------------------------------------------------------------------------

@@ -168,3 +168,83 @@ def test_bf16_conversion_rounds_to_nearest_even():
     assert b[0] == 0x3F80 and b[1] == 0x3F80 and b[2] == 0x3F82  # ties to even
     assert b[5] == 0x7F80 and b[6] == 0x7FC0
     assert np.array_equal(ps.bf16_to_f32(b[:4]), np.array([1.0, 1.0, 1.015625, -2.5], np.float32))
+
+
+# ---------------------------------------------------------------- kernel language and @overlap
+KL_SHMEM = """
+@parallel_indices (ix,iy) function sm!(B, A)
+    T_l = @sharedMem(eltype(A), (@blockDim().x+2, @blockDim().y))
+    T_l[@threadIdx().x+1, @threadIdx().y] = A[ix,iy]
+    @sync_threads()
+    B[ix,iy] = T_l[@threadIdx().x+1, @threadIdx().y] + T_l[end, 1]
+    return
+end
+"""
+
+
+def test_kernel_language_lowering():
+    k = _kl(KL_SHMEM)
+    assert k.info["kernel_language"] and k.info["shared_arrays"] == 1
+    A = ps.ArgSpec((64, 8))
+    src = k.source(A, A, checked=True)
+    assert "extern __shared__" in src and "__syncthreads();" in src and "ps_shmem" in src
+    assert "((ps_int)threadIdx.x + 1)" in src and "((ps_int)blockDim.y)" in src
+    # `end` inside a @sharedMem index is that dimension's extent
+    assert "sm_T_l[((sm_T_l_n1) - 1)" in src
+    # without shmem the launch is refused before anything runs (no device needed)
+    with pytest.raises(ps.ArgumentError, match="shmem"):
+        k.geometry(A, A, nblocks=(2, 2), nthreads=(32, 4))
+    g = k.geometry(A, A, nblocks=(2, 2), nthreads=(32, 4), shmem=34 * 4 * 8)
+    assert g.checked  # shared-memory accesses are not provable on the host: bounds-checked variant
+
+
+def _kl(text):
+    return ps.parallel_indices(text)
+
+
+@pytest.mark.parametrize("body,match", [
+    ("B[ix] = @threadIdx(1).x", "no arguments"),
+    ("B[ix] = @threadIdx().w", "x, y and z"),
+    ("B[ix] = @threadIdx()", "use @threadIdx"),
+    ("B[ix] = @shfl_sync(1, 2.0)", "wrong number"),
+    ("B[ix] = @shfl_sync(@active_mask(), 2.0, 1, 48)", "power of two"),
+    ("B[ix] = @vote_any_sync(@active_mask(), 2.0)", "Bool"),
+    ("B[ix] = @sharedMem(Float64, 4)[1]", "local variable"),
+    ("S = @sharedMem(Float64, (1,2,3,4))\n    B[ix] = 1.0", "1 to 3"),
+    ("S = @sharedMem(Float64, 4)\n    B[ix] = S[1,1]", "1 or 1 indices"),
+    ("B[ix] = S[1]\n    S = @sharedMem(Float64, 4)", "before its @sharedMem"),
+    ("S = @sharedMem(Complex, 4)\n    B[ix] = 1.0", "element type"),
+    ("B[ix] = @sync_threads()", "statement"),
+    ("@foo(B)\n    B[ix] = 1.0", "not supported"),
+])
+def test_kernel_language_errors(body, match):
+    text = f"@parallel_indices (ix) function kl!(B)\n    {body}\n    return\nend"
+    with pytest.raises(ps.PSException, match=match):
+        k = ps.parallel_indices(text)
+        k.source(ps.ArgSpec((64,)))
+
+
+def test_kernel_language_not_with_memopt():
+    with pytest.raises(ps.ArgumentError, match="memopt"):
+        ps.parallel_indices("@parallel_indices (ix,iy,iz) function m!(B, A)\n"
+                            "    B[ix,iy,iz] = A[ix,iy,iz-1] + A[ix,iy,iz+1] + @threadIdx().x\n    return\nend", memopt=True)
+
+
+def test_overlap_errors_on_the_host():
+    k = ps.parallel_indices("@parallel_indices (ix) function f!(A)\n    A[ix] = 1.0\n    return\nend")
+    with pytest.raises(ps.ArgumentError, match="stream"):
+        with ps.overlap():
+            k(None, stream=1)
+    with pytest.raises(ps.ArgumentError, match="only contain @parallel"):
+        with ps.overlap():
+            pass
+    with pytest.raises(ps.ArgumentError, match="only contain @parallel"):
+        with ps.overlap():
+            ps.synchronize()
+    with pytest.raises(ps.ArgumentError, match="only contain @parallel"):
+        with ps.overlap():
+            k(None, async_=True)
+    with pytest.raises(ps.ArgumentError, match="stream"):
+        ps.overlap((k, (None,), {"stream": 2}))
+    with pytest.raises(ps.ArgumentError, match="only contain @parallel"):
+        ps.overlap((k,))

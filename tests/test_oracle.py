@@ -339,3 +339,63 @@ def test_acoustic2d_gradient_finite_differences():
         r = lambda Kp: 0.5 * float(np.sum((O.acoustic2d_traces(Kp, *args) - Dobs) ** 2))
         fd = (r(K + h * d) - r(K - h * d)) / (2 * h)
         assert abs(fd - float(np.sum(g * d))) <= 1e-6 * abs(fd)
+
+
+# ------------------------------------------------------------ kernel language
+def test_diffusion2d_shmem_linear_field_is_stationary():
+    nx, ny = 13, 9
+    ix, iy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    T = F(3.0 * ix - 2.0 * iy + 7.0)
+    Ci = F(np.random.default_rng(2).random((nx, ny)) + 0.5)
+    T2 = F(np.full((nx, ny), -1.0))
+    O.diffusion2d_shmem_step(T2, T, Ci, 1.3, 0.01, 4.0, 2.0)
+    assert np.array_equal(T2[1:-1, 1:-1], T[1:-1, 1:-1])
+    assert np.all(T2[0, :] == -1.0) and np.all(T2[:, -1] == -1.0)  # boundary untouched
+
+
+def test_diffusion2d_shmem_quadratic_field():
+    # T = x^2 + 3 y^2 on the integer grid: second differences are exactly 2 and 6, so with lam=1,
+    # _dx=_dy=1, Ci=1 the update is T + dt*(2 + 6) (a sign or neighbour mistake changes it)
+    nx, ny = 11, 10
+    ix, iy = np.meshgrid(np.arange(nx, dtype=float), np.arange(ny, dtype=float), indexing="ij")
+    T = F(ix * ix + 3.0 * iy * iy)
+    T2 = F(np.zeros((nx, ny)))
+    O.diffusion2d_shmem_step(T2, T, F(np.ones((nx, ny))), 1.0, 0.125, 1.0, 1.0)
+    assert np.array_equal(T2[1:-1, 1:-1], T[1:-1, 1:-1] + 0.125 * 8.0)
+
+
+def test_diffusion2d_shmem_brute_force():
+    rng = np.random.default_rng(4)
+    T, Ci = F(rng.random((8, 7))), F(rng.random((8, 7)) + 0.5)
+    T2 = F(np.zeros((8, 7)))
+    O.diffusion2d_shmem_step(T2, T, Ci, 0.7, 0.003, 7.0, 6.0)
+    for i in range(1, 7):
+        for j in range(1, 6):
+            assert T2[i, j] == O.diffusion2d_shmem_cell(T, Ci, 0.7, 0.003, 7.0, 6.0, i, j)
+
+
+def test_wavefront_model_closed_forms():
+    rng = np.random.default_rng(9)
+    v = rng.integers(-100, 100, 64).astype(np.float64)
+    s = v.copy()
+    for m in (32, 16, 8, 4, 2, 1):  # xor butterfly: every lane ends with the wave sum
+        s = s + O.wave_shfl_xor(s, m, 64)
+    assert np.all(s == v.sum())
+    for w in (8, 16, 64):  # a partial butterfly sums each partition
+        s = v.copy()
+        m = w // 2
+        while m:
+            s = s + O.wave_shfl_xor(s, m, w)
+            m //= 2
+        assert np.array_equal(s, np.repeat(v.reshape(-1, w).sum(axis=1), w))
+    lanes = np.arange(64)
+    assert np.array_equal(O.wave_shfl(v, lanes % 16 + 1, 16), v)  # own lane: identity
+    assert np.array_equal(O.wave_shfl(v, np.ones(64, dtype=int), 16), np.repeat(v[::16], 16))  # broadcast lane 1
+    up = O.wave_shfl_up(v, 3, 32)
+    assert np.array_equal(up[3:32], v[:29]) and np.array_equal(up[:3], v[:3]) and np.array_equal(up[32:35], v[32:35])
+    dn = O.wave_shfl_down(v, 5, 32)
+    assert np.array_equal(dn[:27], v[5:32]) and np.array_equal(dn[27:32], v[27:32])
+    assert np.array_equal(O.wave_shfl_xor(v, 24, 16), v)  # partner in another partition: own value
+    assert O.wave_ballot(lanes % 2 == 0) == np.array([0x5555555555555555], dtype=np.uint64).view(np.int64)[0]
+    assert O.wave_ballot(np.ones(64, bool)) == -1 and O.wave_ballot(np.zeros(64, bool)) == 0
+    assert bin(int(np.array([O.wave_ballot(lanes < 10)]).view(np.uint64)[0])).count("1") == 10
