@@ -283,6 +283,71 @@ def gemm_study(args):
     print(json.dumps(out), flush=True)
 
 
+def shmem_study(args):
+    """diffusion2D written with the kernel language (examples/diffusion2D_shmem_novis.jl: an
+    @sharedMem tile with halo and @sync_threads) at the example's 16384^2, against the same update
+    written with FiniteDifferences2D macros (plain and memopt) and a 2-D triad over the same three
+    arrays. T_eff per step uses the example's A_eff = 3 arrays x 8 B x n^2; the fields of the
+    kernel-language and macro kernels are compared bit for bit."""
+    import numpy as np
+    import parallelstencil as ps
+    from parallelstencil import workloads as W
+
+    n, K, Wm = args.shmem_n, args.steps, args.warmup
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2, module="bench.shmem")
+    mod = "bench.shmem"
+    kl, kli = W.define_diffusion2d_shmem(mod), W.define_diffusion2d_shmem(mod, inbounds=True)
+    fd, fdm = W.define_diffusion2d_fd(mod), W.define_diffusion2d_fd(mod, memopt=True)
+    tri = ps.core._define("parallel", "@parallel function copy2D!(T2, T, Ci)\n    @all(T2) = @all(T) + @all(Ci)\n    return\nend", {}, mod)
+    lx = ly = 1.0
+    dx, dy = lx / (n - 1), ly / (n - 1)
+    lam, c0 = 1.0, 2.0
+    _dx, _dy = 1.0 / dx, 1.0 / dy
+    dt = min(dx * dx, dy * dy) / lam / (1.0 / c0) / 4.1
+    bytes_step = 3 * 8 * n * n
+    rng = np.random.default_rng(1)
+    T0 = np.asfortranarray(1.7 + 0.1 * rng.random((n, n)))
+    Ci = ps.fill(1.0 / c0, n, n)
+    runs = {}
+    finals = {}
+    cases = [("kernel language, threads (32,8) (the example)", "kl", (32, 8)), ("kernel language, threads (64,4)", "kl", (64, 4)),
+             ("kernel language, threads (128,2)", "kl", (128, 2)), ("kernel language inbounds=true, threads (32,8)", "kli", (32, 8)),
+             ("kernel language inbounds=true, threads (128,2)", "kli", (128, 2)), ("FiniteDifferences2D", "fd", None),
+             ("FiniteDifferences2D memopt", "fdm", None), ("triad", "tri", None)]
+    for name, kind, th in cases:
+        T, T2 = ps.zeros(n, n), ps.zeros(n, n)
+        T.copy_from(T0)
+        T2.copy_from(T0)
+        st = [T, T2]
+        if kind in ("kl", "kli"):
+            kw = dict(nblocks=(n // th[0], n // th[1]), nthreads=th, shmem=(th[0] + 2) * (th[1] + 2) * 8)
+            k = kl if kind == "kl" else kli
+            launch = lambda i, k=k, kw=kw: (k(st[1], st[0], Ci, lam, dt, _dx, _dy, async_=True, **kw), st.reverse())
+        elif kind in ("fd", "fdm"):
+            k = fd if kind == "fd" else fdm
+            launch = lambda i, k=k: (k(st[1], st[0], Ci, lam, dt, _dx, _dy, async_=True), st.reverse())
+        else:
+            launch = lambda i: tri(st[1], st[0], Ci, async_=True)
+        wall, ms = time_kernel(ps, launch, K, Wm)
+        runs[name] = {"ms_per_step": round(ms, 4), "T_eff_gbs": round(bytes_step / ms / 1e6, 1)}
+        log(f"diffusion2D {n}^2 {name}: {ms:.4f} ms/step, T_eff {runs[name]['T_eff_gbs']} GB/s")
+        if kind != "tri":
+            finals[name] = st[0].to_host()
+        del T, T2, st
+    names = [c[0] for c in cases if c[1] != "tri"]
+    same = all(np.array_equal(finals[names[0]], finals[m]) for m in names[1:])
+    log(f"kernel-language and FiniteDifferences2D fields after {K + Wm} steps bit-identical: {same}")
+    ex = runs[cases[0][0]]
+    out = {"metric": "T_eff GB/s (diffusion2D, kernel language: @sharedMem + @sync_threads)", "value": ex["T_eff_gbs"], "unit": "GB/s",
+           "n_gpus": 1, "steps": K, "warmup": Wm, "ms_per_step": ex["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"examples/diffusion2D_shmem_novis.jl at {n}^2, threads (32,8), shmem 34*10*8 B"},
+           "roofline": {"bound": "hbm", "achieved": ex["T_eff_gbs"], "peak": 8000.0, "unit": "GB/s",
+                        "frac": round(ex["T_eff_gbs"] / 8000.0, 4), "traffic": None},
+           "fields_bit_identical": bool(same), "runs": runs}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,7 +356,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--adjoint-n", type=int, default=2048)
@@ -300,6 +365,7 @@ def main():
     ap.add_argument("--train", type=int, default=256, help="stokes: training samples")
     ap.add_argument("--eval", type=int, default=24, help="stokes: held-out samples")
     ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--shmem-n", type=int, default=16384)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built", file=sys.stderr)
@@ -315,6 +381,9 @@ def main():
         return
     if args.workload == "gemm":
         gemm_study(args)
+        return
+    if args.workload == "shmem":
+        shmem_study(args)
         return
 
     import parallelstencil as ps

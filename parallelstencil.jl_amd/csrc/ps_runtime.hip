@@ -1044,8 +1044,10 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
     if (memopt && d == K.mm.loopdim - 1) reach *= K.mm.loopsize;
     sh.hi[d] = std::min<int64_t>(P.ranges[d][1], P.ranges[d][0] + reach - 1);
   }
+  // inbounds=true is @inbounds over the body (src/ParallelKernel/parallel.jl:209): the caller
+  // asserts that every access is inside its array and no check is emitted
   std::string why;
-  v.checked = !prove_in_bounds(K, sh, memopt, &why);
+  v.checked = !K.opt.inbounds && !prove_in_bounds(K, sh, memopt, &why);
   P.v = v;
   return P;
 }

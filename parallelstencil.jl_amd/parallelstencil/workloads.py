@@ -15,6 +15,40 @@ DIFFUSION3D = """
 end
 """
 
+# examples/diffusion2D_shmem_novis.jl:11-27: the kernel language (@sharedMem tile with a one-cell
+# halo, @sync_threads); launched with explicit nblocks/nthreads and shmem=prod(nthreads .+ 2)*8
+DIFFUSION2D_SHMEM = """
+@parallel_indices (ix,iy) function diffusion2D_step!(T2, T, Ci, lam, dt, _dx, _dy)
+    tx  = @threadIdx().x + 1
+    ty  = @threadIdx().y + 1
+    T_l = @sharedMem(eltype(T), (@blockDim().x+2, @blockDim().y+2))
+    T_l[tx,ty] = T[ix,iy]
+    if (1<ix<size(T2,1) && 1<iy<size(T2,2))
+        if (@threadIdx().x == 1)             T_l[tx-1,ty] = T[ix-1,iy] end
+        if (@threadIdx().x == @blockDim().x) T_l[tx+1,ty] = T[ix+1,iy] end
+        if (@threadIdx().y == 1)             T_l[tx,ty-1] = T[ix,iy-1] end
+        if (@threadIdx().y == @blockDim().y) T_l[tx,ty+1] = T[ix,iy+1] end
+        @sync_threads()
+        T2[ix,iy] = T_l[tx,ty] + dt*(Ci[ix,iy]*(
+                    - ((-lam*(T_l[tx+1,ty] - T_l[tx,ty])*_dx) - (-lam*(T_l[tx,ty] - T_l[tx-1,ty])*_dx))*_dx
+                    - ((-lam*(T_l[tx,ty+1] - T_l[tx,ty])*_dy) - (-lam*(T_l[tx,ty] - T_l[tx,ty-1])*_dy))*_dy
+                    ));
+    end
+    return
+end
+"""
+
+# The same update with FiniteDifferences2D macros and on-the-fly fluxes: the same operations in
+# the same order, so both kernels give bit-identical fields.
+DIFFUSION2D_FD = """
+@parallel function diffusion2D_fd!(T2, T, Ci, lam, dt, _dx, _dy)
+    @all(qx) = -lam*@d_xi(T)*_dx
+    @all(qy) = -lam*@d_yi(T)*_dy
+    @inn(T2) = @inn(T) + dt*(@inn(Ci)*(-@d_xa(qx)*_dx - @d_ya(qy)*_dy))
+    return
+end
+"""
+
 # examples/memcopy3D.jl:10-13
 TRIAD3D = """
 @parallel function copy3D!(T2, T, Ci)
@@ -86,6 +120,14 @@ def diffusion_inputs(n, seed=1, lx=10.0):
 
 def define_diffusion3d(memopt, module, **kw):
     return core._define("parallel", DIFFUSION3D, dict(kw, memopt=memopt), module)
+
+
+def define_diffusion2d_shmem(module, inbounds=False):
+    return core._define("parallel_indices", DIFFUSION2D_SHMEM, {"inbounds": inbounds}, module)
+
+
+def define_diffusion2d_fd(module, memopt=False):
+    return core._define("parallel", DIFFUSION2D_FD, {"memopt": memopt}, module)
 
 
 def define_triad3d(module):
