@@ -67,6 +67,14 @@ typedef struct ps_field {
   int32_t ndims;
   void*   base;        /* allocation base (NULL for views)                  */
   int64_t alloc_bytes;
+  /* CellArray (ncomp > 0): dims, pitch and plane describe the dense grid of cells;
+   * value j (0-based, column-major over celldims) of the cell with linear index I
+   * (0-based) lives at ptr + (I / blocklen) * blocklen * ncomp + j * blocklen + I % blocklen,
+   * the layout of CellArrays.jl (data of size (blocklen, ncomp, ceil(N / blocklen))). */
+  int32_t ncomp;        /* values per cell; 0 for a plain field               */
+  int32_t celldims[4];  /* cell dimensions, 1 for absent ones                  */
+  int64_t blocklen;     /* values of one cell field stored contiguously        */
+  const char* cellnames;/* comma-separated field names of a @CellType, or NULL */
 } ps_field;
 
 typedef struct ps_arg {
@@ -128,12 +136,19 @@ int ps_device_sync(void);
  * (src/ParallelKernel/FieldAllocators.jl:569-637). The x-pitch is rounded up
  * to a multiple of 256 bytes so that every x-row starts on a 256-byte boundary. */
 int ps_field_alloc(int ndims, const int64_t dims[3], int dtype, ps_field* out);
+/* A CellArray: @zeros(dims...; celldims, blocklength) and @zeros(dims...; celltype)
+ * (src/ParallelKernel/allocators.jl:1-17,436-454; the CellArray type of CellArrays.jl).
+ * blocklength 0 stores each cell field contiguously over all cells (the reference's GPU
+ * default); 1 stores cells whole. h2d/d2h/fill/copy of a CellArray move its data array. */
+int ps_cellarray_alloc(int ndims, const int64_t dims[3], int cellndims, const int32_t celldims[4], int64_t blocklength, int dtype,
+                       ps_field* out);
 int ps_field_free(ps_field* f);
 /* A view of the index box lo..hi (1-based, inclusive) of a parent field; replaces
  * the subarray views of padded Fields (FieldAllocators.jl:622-634). */
 int ps_field_view(const ps_field* parent, const int64_t lo[3], const int64_t hi[3], ps_field* out);
 /* Host <-> device copies of a dense column-major host array of the field's logical
- * size (Array(A) / copy!(A, host) in the reference). stream NULL = stream 0; both block. */
+ * size (Array(A) / copy!(A, host) in the reference); for a CellArray, of its data array.
+ * stream NULL = stream 0; both block. */
 int ps_field_h2d(const ps_field* f, const void* host, void* stream);
 int ps_field_d2h(const ps_field* f, void* host, void* stream);
 /* fill!(A, x): replaces fill!_amdgpu (AMDGPUExt/allocators.jl). Asynchronous on stream. */

@@ -39,6 +39,12 @@ that parity is not anchored on a formula merely retyped:
                     single-cell brute force in Python floats (diffusion2d_shmem_cell);
                     a wrong sign or swapped neighbour changes a quadratic field's
                     exactly known discrete Laplacian (2 on integer grids).
+  cellarray_data_offset / cells_to_data  closed forms: B = 1 stores cells whole
+                    (I*nc + j), B >= N stores each value contiguously (j*B + I); a
+                    blocked case against numpy's own reshape/transpose of the
+                    (B, nblocks, nc) view; every offset hit exactly once.
+  copy3d_cells      addition of the cells (a retyped formula: pinned only through
+                    the layout checks and integer-valued inputs, exact).
   wave_*            closed forms in tests/test_oracle.py: the xor butterfly over
                     masks 32..1 leaves every lane with the wave sum; shfl_up then
                     shfl_down by the same delta restores the lanes away from the
@@ -501,3 +507,35 @@ def wave_ballot(pred):
         if p:
             m |= 1 << l
     return np.array([m], dtype=np.uint64).view(np.int64)[0]
+
+
+# ------------------------------------------------- CellArrays
+# CellArrays.jl (a dependency of the reference, Project.toml; not vendored) stores a CellArray of
+# N cells with nc values each in a data array of size (B, nc, ceil(N/B)), B the block length
+# (src/ParallelKernel/allocators.jl:449: CellArray{T,N,B}(rand(eltype(T), blocklen, prod(size(T)),
+# ceil(Int, prod(dims)/blocklen)), dims)): value j of cell I sits at data[I % B, j, I // B]
+# (0-based). Restated here element by element.
+def cellarray_data_offset(I, j, B, nc):
+    """0-based offset in the column-major data array of value j of cell I."""
+    return (I % B) + j * B + (I // B) * B * nc
+
+
+def cells_to_data(cells, celldims, B):
+    """Host cells (shape dims + celldims, column-major) -> the data array (B, nc, nblocks), by loops."""
+    cells = np.asarray(cells)
+    nd = cells.ndim - len(celldims)
+    dims = cells.shape[:nd]
+    N = int(np.prod(dims))
+    nc = int(np.prod(celldims))
+    nb = -(-N // B) if N else 0
+    flat = cells.reshape((N, nc), order="F")
+    data = np.zeros(B * nc * nb, dtype=cells.dtype)
+    for I in range(N):
+        for j in range(nc):
+            data[cellarray_data_offset(I, j, B, nc)] = flat[I, j]
+    return data.reshape((B, nc, nb), order="F")
+
+
+# examples/memcopyCellArray3D.jl:10-27: T2 = T + Ci cell by cell (every value of the cells).
+def copy3d_cells(T, Ci):
+    return np.asarray(T) + np.asarray(Ci)

@@ -219,6 +219,8 @@ class Emitter {
       int i = K.param_index.at(w);
       if (!kinds[i].is_field) throw PsError(PS_EARGUMENT, "argument '" + w + "' is assigned to but is not an array");
     }
+    for (auto& k : kinds) any_cell = any_cell || k.ncomp > 0;
+    if (any_cell && V.memopt) throw PsError(PS_EARGUMENT, "memopt=true is not supported with CellArray arguments");
   }
 
   Emitted run();
@@ -254,6 +256,17 @@ class Emitter {
   void kl_stmt(const NodeP& n);
   void sm_create(const std::string& v, const NodeP& mac);
   std::string sm_address(const NodeP& n, std::string* check);
+  // CellArrays: the component (0-based) of the cell assignment being emitted, -1 outside one,
+  // and the number of values per cell of its left-hand side
+  int comp_ctx = -1;
+  int comp_nc = 0;
+  bool any_cell = false;
+  bool is_cell(int ai) const { return ai >= 0 && kinds[ai].ncomp > 0; }
+  int cell_shape(const NodeP& n);
+  bool cell_ref(const NodeP& n, int& ai, NodeP& cellidx, std::string& comp, std::string& compchk);
+  std::string cell_offset(int ai, const std::string& I, const std::string& j) const;
+  EV cell_load(int ai, const NodeP& cellidx, const std::string& comp, const std::string& compchk);
+  void cell_store(int ai, const NodeP& cellidx, const std::string& comp, const std::string& compchk, const EV& r);
 
   std::string pad() const { return std::string(ind * 2, ' '); }
   std::string an(int i) const { return "a" + std::to_string(i); }
@@ -386,6 +399,22 @@ std::string Emitter::offset(int ai, const std::vector<NodeP>& idx, std::string i
 }
 
 EV Emitter::load(const NodeP& n) {
+  if (any_cell) {
+    int cai;
+    NodeP cidx;
+    std::string comp, cchk;
+    if (cell_ref(n, cai, cidx, comp, cchk)) return cell_load(cai, cidx, comp, cchk);
+    if (n->a[0]->k == NK::Ident && is_cell(field_index(n->a[0]->s))) {
+      const int ai = field_index(n->a[0]->s);
+      if (comp_ctx < 0)
+        throw PsError(PS_EARGUMENT, "a cell of CellArray '" + n->a[0]->s + "' is used where a number is expected; select one value with field(" +
+                                        n->a[0]->s + ", i)[...] or " + n->a[0]->s + "[...][i]");
+      if (kinds[ai].ncomp != comp_nc)
+        throw PsError(PS_EARGUMENT, "CellArrays with cells of " + std::to_string(kinds[ai].ncomp) + " and " + std::to_string(comp_nc) +
+                                        " values meet in one cell assignment");
+      return cell_load(ai, n, std::to_string(comp_ctx), "");
+    }
+  }
   if (n->a[0]->k != NK::Ident) throw PsError(PS_EARGUMENT, "unsupported indexing expression " + to_julia(n));
   const std::string& name = n->a[0]->s;
   if (K.shared_arrays.count(name)) {  // @sharedMem array in LDS
@@ -636,6 +665,15 @@ EV Emitter::call(const NodeP& n) {
     std::string A = an(ai);
     return EV{"(" + A + "_nx*" + A + "_ny*" + A + "_nz)", Ty::Int};
   }
+  if (f == "cellsize") {
+    if (n->a.size() != 2 || n->a[0]->k != NK::Ident || !(n->a[1]->k == NK::Num && n->a[1]->is_int))
+      throw PsError(PS_EARGUMENT, "cellsize(A, d) requires a CellArray argument and a literal dimension d");
+    int ai = field_index(n->a[0]->s);
+    if (!is_cell(ai)) throw PsError(PS_EARGUMENT, "cellsize(" + n->a[0]->s + ", d): not a CellArray argument");
+    long long d = n->a[1]->ival;
+    if (d < 1) throw PsError(PS_EARGUMENT, "cellsize(A, d): d must be >= 1");
+    return EV{"((ps_int)" + std::to_string(d <= 3 ? kinds[ai].celldims[d - 1] : 1) + ")", Ty::Int};
+  }
   if (f == "firstindex") return EV{"((ps_int)1)", Ty::Int};
   if (f == "lastindex") {
     NodeP s = mk(NK::Call, "size", n->a);
@@ -746,6 +784,12 @@ EV Emitter::expr(const NodeP& n) {
       throw PsError(PS_EPLUGIN, "macro @" + n->s + " is not available here");
     case NK::Dot:
       if (K.kernel_language && n->a[0]->k == NK::Macro) return kl_expr(n);
+      if (any_cell) {
+        int cai;
+        NodeP cidx;
+        std::string comp, cchk;
+        if (cell_ref(n, cai, cidx, comp, cchk)) return cell_load(cai, cidx, comp, cchk);
+      }
       throw PsError(PS_EARGUMENT, "unsupported expression: " + to_julia(n));
     case NK::Unary: {
       EV a = expr(n->a[0]);
@@ -791,6 +835,42 @@ void Emitter::assign(const NodeP& s) {
   const NodeP& lhs = s->a[0];
   NodeP rhs = s->a[1];
   if (!s->s.empty()) rhs = mk(NK::Binary, s->s, {clone(lhs), rhs});  // A op= x  ->  A = A op x
+  if (any_cell) {
+    int cai;
+    NodeP cidx;
+    std::string comp, cchk;
+    if (lhs->k == NK::Index && cell_ref(lhs, cai, cidx, comp, cchk)) {  // one value of a cell
+      if (cell_shape(rhs)) throw PsError(PS_EARGUMENT, "a cell is assigned to one value of CellArray '" + K.def.params[cai].name + "'");
+      bool blk = false;
+      EV r = rhs_value(rhs, &blk);
+      cell_store(cai, cidx, comp, cchk, r);
+      if (blk) {
+        --ind;
+        o << pad() << "}\n";
+      }
+      return;
+    }
+    if (lhs->k == NK::Index && lhs->a[0]->k == NK::Ident && is_cell(field_index(lhs->a[0]->s))) {  // a whole cell
+      const int ai = field_index(lhs->a[0]->s);
+      if (!cell_shape(rhs))
+        throw PsError(PS_EARGUMENT, "a number is assigned to a cell of CellArray '" + lhs->a[0]->s + "'; assign a cell, or one value with field(" +
+                                        lhs->a[0]->s + ", i)[...]");
+      comp_nc = kinds[ai].ncomp;
+      for (int j = 0; j < comp_nc; ++j) {  // the statement once per value of the cell
+        comp_ctx = j;
+        bool blk = false;
+        EV r = rhs_value(rhs, &blk);
+        cell_store(ai, lhs, std::to_string(j), "", r);
+        if (blk) {
+          --ind;
+          o << pad() << "}\n";
+        }
+      }
+      comp_ctx = -1;
+      return;
+    }
+    if (cell_shape(rhs)) throw PsError(PS_EARGUMENT, "a cell is used where a number is expected in " + to_julia(s));
+  }
   if (lhs->k == NK::Ident && rhs->k == NK::Macro && rhs->s == "sharedMem") {
     if (!s->s.empty()) throw PsError(PS_EARGUMENT, "@sharedMem: use A = @sharedMem(T, dims)");
     sm_create(lhs->s, rhs);
@@ -948,6 +1028,197 @@ void Emitter::stmt(const NodeP& s) {
 
 void Emitter::block(const NodeP& b) {
   for (auto& s : b->a) stmt(s);
+}
+
+// ---------------------------------------------------------------------------
+// CellArrays (src/ParallelKernel/allocators.jl:1-17,436-454 and the CellArray type of
+// CellArrays.jl): a cell assignment is emitted once per value of the cell; field(A, i...)[I],
+// A[I][i...] and A[I].name select one value.
+// ---------------------------------------------------------------------------
+
+// Offset of value j of the cell with linear index I, in the layout of the argument.
+std::string Emitter::cell_offset(int ai, const std::string& I, const std::string& j) const {
+  const std::string A = an(ai), nc = std::to_string(kinds[ai].ncomp);
+  switch (kinds[ai].cell_layout) {
+    case 1: return "(" + I + ") + (" + j + ") * " + A + "_B";
+    case 2: return "(" + I + ") * " + nc + " + (" + j + ")";
+    default:
+      return "((" + I + ") / " + A + "_B) * (" + A + "_B * " + nc + ") + (" + j + ") * " + A + "_B + (" + I + ") % " + A + "_B";
+  }
+}
+
+// Recognises the forms that select one value of a cell: field(A, i...)[I], A[I][i...] and
+// A[I].name. Returns the argument, the cell access A[I], the 0-based value index and, for
+// value indices not known at compile time, the check of that index (checked variants).
+bool Emitter::cell_ref(const NodeP& n, int& ai, NodeP& cellidx, std::string& comp, std::string& compchk) {
+  std::vector<NodeP> cs;
+  if (n->k == NK::Index && n->a[0]->k == NK::Call && n->a[0]->s == "field") {
+    const NodeP& f = n->a[0];
+    if (f->a.size() < 2 || f->a[0]->k != NK::Ident) throw PsError(PS_EARGUMENT, "field(A, i...) expects a CellArray argument and value indices");
+    ai = field_index(f->a[0]->s);
+    if (!is_cell(ai)) throw PsError(PS_EARGUMENT, "field(" + f->a[0]->s + ", ...): not a CellArray argument");
+    std::vector<NodeP> a{f->a[0]};
+    a.insert(a.end(), n->a.begin() + 1, n->a.end());
+    cellidx = mk(NK::Index, "", a);
+    cs.assign(f->a.begin() + 1, f->a.end());
+  } else if (n->k == NK::Index && n->a[0]->k == NK::Index && n->a[0]->a[0]->k == NK::Ident && is_cell(field_index(n->a[0]->a[0]->s))) {
+    ai = field_index(n->a[0]->a[0]->s);
+    cellidx = n->a[0];
+    cs.assign(n->a.begin() + 1, n->a.end());
+  } else if (n->k == NK::Dot && n->a[0]->k == NK::Index && n->a[0]->a[0]->k == NK::Ident && is_cell(field_index(n->a[0]->a[0]->s))) {
+    ai = field_index(n->a[0]->a[0]->s);
+    cellidx = n->a[0];
+    const std::string& names = kinds[ai].cellnames;
+    int j = 0;
+    size_t p = 0;
+    for (;;) {
+      size_t q = names.find(',', p);
+      if (names.substr(p, q == std::string::npos ? std::string::npos : q - p) == n->s) break;
+      if (q == std::string::npos) {
+        throw PsError(PS_EARGUMENT, "CellArray '" + n->a[0]->a[0]->s + "' has no cell field named " + n->s +
+                                        (names.empty() ? " (its cells have no field names)" : " (fields: " + names + ")"));
+      }
+      p = q + 1;
+      ++j;
+    }
+    if (j >= kinds[ai].ncomp) throw PsError(PS_EARGUMENT, "cell field " + n->s + " lies outside the cell");
+    comp = std::to_string(j);
+    compchk.clear();
+    return true;
+  } else {
+    return false;
+  }
+  const int* cd = kinds[ai].celldims;
+  if (cs.empty() || cs.size() > 4) throw PsError(PS_EARGUMENT, "a cell value is selected with 1 to 4 indices");
+  // extents per given index; the last index runs over the remaining cell dimensions (Julia's
+  // trailing linear indexing), so one index runs over the whole cell
+  long long ext[4] = {1, 1, 1, 1};
+  for (size_t d = 0; d < cs.size(); ++d) ext[d] = cd[d];
+  for (size_t d = cs.size(); d < 4; ++d) ext[cs.size() - 1] *= cd[d];
+  std::string ic[4] = {"((ps_int)1)", "((ps_int)1)", "((ps_int)1)", "((ps_int)1)"};
+  bool literal = true;
+  long long lin = 0, stride = 1;
+  comp.clear();
+  for (size_t d = 0; d < cs.size(); ++d) {
+    EV e = expr(cs[d]);
+    if (e.t != Ty::Int) throw PsError(PS_EARGUMENT, "cell value indices must be integers");
+    ic[d] = e.c;
+    if (cs[d]->k == NK::Num && cs[d]->is_int) {
+      const long long v = cs[d]->ival;
+      if (v < 1 || v > ext[d])
+        throw PsError(PS_EBOUNDS, "BoundsError: value index " + std::to_string(v) + " of a cell of CellArray '" + K.def.params[ai].name +
+                                      "' is outside 1:" + std::to_string(ext[d]));
+      lin += (v - 1) * stride;
+    } else {
+      literal = false;
+    }
+    comp += std::string(comp.empty() ? "" : " + ") + "((" + e.c + ") - 1) * " + std::to_string(stride);
+    stride *= ext[d];
+  }
+  if (literal) {
+    comp = std::to_string(lin);
+    compchk.clear();
+  } else {
+    comp = "(" + comp + ")";
+    compchk = V.checked ? "(ps_chk(" + ic[0] + ", " + ic[1] + ", " + ic[2] + ", " + std::to_string(ext[0]) + ", " + std::to_string(ext[1]) + ", " +
+                              std::to_string(ext[2]) + ", ps_err) && ps_chk(" + ic[3] + ", 1, 1, " + std::to_string(ext[3]) + ", 1, 1, ps_err))"
+                        : "";
+  }
+  return true;
+}
+
+EV Emitter::cell_load(int ai, const NodeP& cellidx, const std::string& comp, const std::string& compchk) {
+  auto idx = norm_indices(cellidx, ai);
+  std::string ic[3];
+  bool aff = false;
+  const std::string I = offset(ai, idx, ic, &aff);
+  const std::string A = an(ai);
+  std::string raw = A + "[" + cell_offset(ai, I, comp) + "]";
+  const int dt = kinds[ai].dtype;
+  if (V.checked) {
+    std::string chk = "ps_chk(" + ic[0] + ", " + ic[1] + ", " + ic[2] + ", " + A + "_nx, " + A + "_ny, " + A + "_nz, ps_err)";
+    if (!compchk.empty()) chk = "(" + chk + " && " + compchk + ")";
+    raw = "(" + chk + " ? " + raw + " : (" + elem_cty(dt) + ")0)";
+  }
+  if (dt == PS_BF16) return EV{"ps_bf16_ld(" + raw + ")", Ty::F32};
+  if (dt == PS_BOOL) return EV{"((bool)" + raw + ")", Ty::Bool};
+  if (dt == PS_I64 || dt == PS_I32) return EV{"((ps_int)" + raw + ")", Ty::Int};
+  return EV{raw, value_ty(dt)};
+}
+
+void Emitter::cell_store(int ai, const NodeP& cellidx, const std::string& comp, const std::string& compchk, const EV& r) {
+  auto idx = norm_indices(cellidx, ai);
+  std::string ic[3];
+  const std::string I = offset(ai, idx, ic, nullptr);
+  const std::string A = an(ai);
+  const int dt = kinds[ai].dtype;
+  std::string v;
+  if (dt == PS_BF16) v = "ps_bf16_st(" + cast(r, Ty::F32).c + ")";
+  else if (dt == PS_F64) v = cast(r, Ty::F64).c;
+  else if (dt == PS_F32) v = cast(r, Ty::F32).c;
+  else if (dt == PS_BOOL) v = "((unsigned char)(" + r.c + "))";
+  else v = std::string("((") + elem_cty(dt) + ")(" + r.c + "))";
+  std::string chk;
+  if (V.checked) {
+    chk = "ps_chk(" + ic[0] + ", " + ic[1] + ", " + ic[2] + ", " + A + "_nx, " + A + "_ny, " + A + "_nz, ps_err)";
+    if (!compchk.empty()) chk += " && " + compchk;
+  }
+  o << pad() << (chk.empty() ? "" : "if (" + chk + ") ") << A << "[" << cell_offset(ai, I, comp) << "] = " << v << ";\n";
+}
+
+// 1 if the expression is a whole cell, 0 if it is a number. Cells combine as the static arrays
+// of CellArrays.jl do: cell +/- cell, number * cell, cell * number, cell / number and -cell;
+// anything else (cell + number, cell * cell, functions of cells, cells in conditions) is an
+// error, as the corresponding Julia method does not exist or is not elementwise.
+int Emitter::cell_shape(const NodeP& n) {
+  switch (n->k) {
+    case NK::Index: {
+      int ai;
+      NodeP ci;
+      std::string c, cc;
+      if ((n->a[0]->k == NK::Call && n->a[0]->s == "field") ||
+          (n->a[0]->k == NK::Index && n->a[0]->a[0]->k == NK::Ident && is_cell(field_index(n->a[0]->a[0]->s))))
+        return 0;
+      (void)ai;
+      (void)ci;
+      (void)c;
+      (void)cc;
+      return (n->a[0]->k == NK::Ident && is_cell(field_index(n->a[0]->s))) ? 1 : 0;
+    }
+    case NK::Binary: {
+      const int a = cell_shape(n->a[0]), b = cell_shape(n->a[1]);
+      const std::string& op = n->s;
+      if (op == "+" || op == "-") {
+        if (a != b) throw PsError(PS_EARGUMENT, "MethodError: a cell and a number cannot be combined with " + op + " in " + to_julia(n));
+        return a;
+      }
+      if (op == "*") {
+        if (a && b) throw PsError(PS_EARGUMENT, "products of two cells are not supported (" + to_julia(n) + ")");
+        return a | b;
+      }
+      if (op == "/") {
+        if (b) throw PsError(PS_EARGUMENT, "division by a cell is not supported (" + to_julia(n) + ")");
+        return a;
+      }
+      if (a || b) throw PsError(PS_EARGUMENT, "operator " + op + " is not defined for cells (" + to_julia(n) + ")");
+      return 0;
+    }
+    case NK::Unary: {
+      const int a = cell_shape(n->a[0]);
+      if (a && n->s == "!") throw PsError(PS_EARGUMENT, "! is not defined for cells");
+      return a;
+    }
+    case NK::Dot:
+    case NK::Num:
+    case NK::Ident:
+    case NK::Str:
+    case NK::EndKw:
+      return 0;
+    default:
+      for (auto& c : n->a)
+        if (c && cell_shape(c)) throw PsError(PS_EARGUMENT, "a cell is used where a number is expected in " + to_julia(n));
+      return 0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1536,6 +1807,10 @@ Emitted Emitter::run() {
       sig << "    ps_int " << A << "_nx, ps_int " << A << "_ny, ps_int " << A << "_nz, ps_int " << A << "_sx, ps_int " << A << "_sxy,\n";
       E.params.push_back({0, (int)i, 0, 8, 0});
       for (int w = 0; w < 5; ++w) E.params.push_back({1, (int)i, w, ib, 0});
+      if (k.ncomp > 0) {  // CellArray: values of one cell field stored contiguously
+        sig << "    ps_int " << A << "_B,\n";
+        E.params.push_back({9, (int)i, 0, ib, 0});
+      }
     } else {
       Ty t = value_ty(k.dtype);
       std::string ct = t == Ty::Bool ? "int" : cty(t);
@@ -1754,6 +2029,36 @@ struct Prover {
   void check_access(const NodeP& n, long long lo[3], long long hi[3]) {
     for (int d = 0; d < 3; ++d)
       if (lo[d] > hi[d]) return;  // unreachable
+    // one value of a CellArray cell: field(A, i...)[I] or A[I][i...]; value indices must be
+    // literals (their range is checked when the kernel is lowered), the cell access as usual
+    if (n->a[0]->k == NK::Call && n->a[0]->s == "field") {
+      const NodeP& f = n->a[0];
+      for (size_t i = 1; i < f->a.size(); ++i) {
+        long long v;
+        if (!eval(f->a[i], v)) {
+          ok = false;
+          why = "cell value index of " + to_julia(n) + " is not a literal";
+          return;
+        }
+      }
+      if (f->a.empty()) return;
+      std::vector<NodeP> a{f->a[0]};
+      a.insert(a.end(), n->a.begin() + 1, n->a.end());
+      check_access(mk(NK::Index, "", a), lo, hi);
+      return;
+    }
+    if (n->a[0]->k == NK::Index) {
+      for (size_t i = 1; i < n->a.size(); ++i) {
+        long long v;
+        if (!eval(n->a[i], v)) {
+          ok = false;
+          why = "cell value index of " + to_julia(n) + " is not a literal";
+          return;
+        }
+      }
+      check_access(n->a[0], lo, hi);
+      return;
+    }
     auto it = K.param_index.find(n->a[0]->s);
     if (it == K.param_index.end()) {
       if (K.shared_arrays.count(n->a[0]->s)) {  // extents known only on the device

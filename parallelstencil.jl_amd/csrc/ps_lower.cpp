@@ -456,12 +456,26 @@ static NodeP insert_onthefly(const NodeP& n, const std::map<std::string, NodeP>&
 // ---------------------------------------------------------------------------
 // define
 // ---------------------------------------------------------------------------
+// The argument an indexing expression reads: A[I], field(A, i...)[I] (CellArrays) or A[I][i...].
+static std::string indexed_param(const NodeP& x, const std::map<std::string, int>& params) {
+  const NodeP& b = x->a[0];
+  if (b->k == NK::Ident) return params.count(b->s) ? b->s : "";
+  if (b->k == NK::Call && b->s == "field" && !b->a.empty() && b->a[0]->k == NK::Ident) return params.count(b->a[0]->s) ? b->a[0]->s : "";
+  if (b->k == NK::Index) return indexed_param(b, params);
+  return "";
+}
+
 static void collect_access(const NodeP& n, std::set<std::string>& accessed, std::set<std::string>& written,
                            const std::map<std::string, int>& params) {
   walk(n, [&](const NodeP& x) {
-    if (x->k == NK::Index && x->a[0]->k == NK::Ident && params.count(x->a[0]->s)) accessed.insert(x->a[0]->s);
-    if (x->k == NK::Assign && x->a[0]->k == NK::Index && x->a[0]->a[0]->k == NK::Ident && params.count(x->a[0]->a[0]->s))
-      written.insert(x->a[0]->a[0]->s);
+    if (x->k == NK::Index) {
+      std::string p = indexed_param(x, params);
+      if (!p.empty()) accessed.insert(p);
+    }
+    if (x->k == NK::Assign && x->a[0]->k == NK::Index) {
+      std::string p = indexed_param(x->a[0], params);
+      if (!p.empty()) written.insert(p);
+    }
   });
 }
 

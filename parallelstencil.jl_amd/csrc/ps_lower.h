@@ -17,6 +17,13 @@ struct ArgKind {
   bool is_field = false;
   int dtype = 0;   // PS_F64 ...
   int ndims = 0;   // fields only
+  // CellArray fields: values per cell (0: plain field), cell dimensions, storage layout
+  // (1 one block of all cells, 2 cells stored whole, 3 blocks of blocklen cells) and the
+  // comma-separated field names of a @CellType
+  int ncomp = 0;
+  int celldims[4] = {1, 1, 1, 1};
+  int cell_layout = 0;
+  std::string cellnames;
 };
 
 struct MemoptMeta {
@@ -101,7 +108,7 @@ struct Emitted {
   struct P {
     int kind;      // 0 field ptr, 1 int size, 2 scalar, 3 range start, 4 range len, 5 err ptr,
                    // 6 RN(1/|b|) of a uniform divisor or 0 (exact division), 7 RN(1/b) (fastmath),
-                   // 8 bytes of dynamic LDS of the launch (@sharedMem)
+                   // 8 bytes of dynamic LDS of the launch (@sharedMem), 9 blocklen of a CellArray
     int arg;       // arg index (fields/scalars)
     int which;     // for sizes: 0 nx 1 ny 2 nz 3 sx 4 sxy; for ranges: dim
     int bytes;     // 4 or 8

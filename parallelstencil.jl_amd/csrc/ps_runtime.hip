@@ -306,6 +306,37 @@ std::vector<char> code_object(const Kernel& K, const Emitted& em) {
   return code;
 }
 
+// The data array of a CellArray as a dense 1-D field (h2d, d2h, fill, copy and reductions move
+// all values of all cells); a plain field is returned unchanged.
+ps_field data_of(const ps_field& f) {
+  if (f.ncomp <= 0) return f;
+  ps_field d = f;
+  const int64_t N = f.dims[0] * f.dims[1] * f.dims[2];
+  const int64_t B = std::max<int64_t>(f.blocklen, 1);
+  const int64_t total = (N + B - 1) / B * B * f.ncomp;
+  d.dims[0] = total;
+  d.dims[1] = d.dims[2] = 1;
+  d.pitch = d.plane = std::max<int64_t>(total, 1);
+  d.ndims = 1;
+  d.ncomp = 0;
+  return d;
+}
+
+// Bytes a field argument spans from its first element (aliasing analysis).
+int64_t extent_bytes(const ps_field& f) {
+  const ps_field d = data_of(f);
+  return d.plane * std::max<int64_t>(d.dims[2], 1) * (int64_t)elsize(d.dtype);
+}
+
+// 1: one block holds every cell (blocklen >= cells), 2: cells stored whole (blocklen 1),
+// 3: blocks of blocklen cells
+int cell_layout(const ps_field& f) {
+  const int64_t N = f.dims[0] * f.dims[1] * f.dims[2];
+  if (f.blocklen >= N) return 1;
+  if (f.blocklen == 1) return 2;
+  return 3;
+}
+
 std::vector<ArgKind> arg_kinds(const ps_arg* args, int nargs) {
   std::vector<ArgKind> k(nargs);
   for (int i = 0; i < nargs; ++i) {
@@ -313,6 +344,15 @@ std::vector<ArgKind> arg_kinds(const ps_arg* args, int nargs) {
       k[i].is_field = true;
       k[i].dtype = args[i].field.dtype;
       k[i].ndims = args[i].field.ndims;
+      if (args[i].field.ncomp > 0) {
+        const ps_field& f = args[i].field;
+        if (f.pitch != f.dims[0] || f.plane != f.dims[0] * f.dims[1] || f.blocklen < 1)
+          throw PsError(PS_EARGUMENT, "argument " + std::to_string(i + 1) + ": a CellArray must be dense with blocklen >= 1");
+        k[i].ncomp = f.ncomp;
+        for (int d = 0; d < 4; ++d) k[i].celldims[d] = f.celldims[d];
+        k[i].cell_layout = cell_layout(f);
+        k[i].cellnames = f.cellnames ? f.cellnames : "";
+      }
     } else if (args[i].kind == PS_ARG_SCALAR) {
       k[i].dtype = args[i].dtype;
     } else {
@@ -325,7 +365,14 @@ std::vector<ArgKind> arg_kinds(const ps_arg* args, int nargs) {
 
 std::string kinds_key(const std::vector<ArgKind>& k) {
   std::string s;
-  for (auto& a : k) s += (a.is_field ? "F" : "S") + std::to_string(a.dtype) + (a.is_field ? std::to_string(a.ndims) : "") + ",";
+  for (auto& a : k) {
+    s += (a.is_field ? "F" : "S") + std::to_string(a.dtype) + (a.is_field ? std::to_string(a.ndims) : "");
+    if (a.ncomp > 0)
+      s += "c" + std::to_string(a.celldims[0]) + "x" + std::to_string(a.celldims[1]) + "x" + std::to_string(a.celldims[2]) + "x" +
+           std::to_string(a.celldims[3]) + "l" +
+           std::to_string(a.cell_layout) + "[" + a.cellnames + "]";
+    s += ",";
+  }
   return s;
 }
 
@@ -465,6 +512,51 @@ extern "C" int ps_field_alloc(int ndims, const int64_t dims[3], int dtype, ps_fi
   return 0;
 }
 
+extern "C" int ps_cellarray_alloc(int ndims, const int64_t dims[3], int cellndims, const int32_t celldims[4], int64_t blocklength, int dtype,
+                                  ps_field* out) {
+  if (int r = require_init()) return r;
+  if (ndims < 1 || ndims > 3) return fail(PS_EARGUMENT, "ps_cellarray_alloc: ndims must be 1, 2 or 3");
+  if (cellndims < 1 || cellndims > 4) return fail(PS_EARGUMENT, "ps_cellarray_alloc: celldims must have 1 to 4 dimensions");
+  if (blocklength < 0) return fail(PS_EARGUMENT, "ps_cellarray_alloc: blocklength must be >= 0");
+  size_t es = elsize(dtype);
+  if (!es) return fail(PS_EARGUMENT, "ps_cellarray_alloc: unsupported element type");
+  int64_t d[3] = {1, 1, 1};
+  for (int i = 0; i < ndims; ++i) {
+    if (dims[i] < 0) return fail(PS_EARGUMENT, "ps_cellarray_alloc: negative dimension");
+    d[i] = dims[i];
+  }
+  int32_t cd[4] = {1, 1, 1, 1};
+  int64_t nc = 1;
+  for (int i = 0; i < cellndims; ++i) {
+    if (celldims[i] < 1) return fail(PS_EARGUMENT, "ps_cellarray_alloc: cell dimensions must be >= 1");
+    cd[i] = celldims[i];
+    nc *= celldims[i];
+  }
+  if (nc > 4096) return fail(PS_EARGUMENT, "ps_cellarray_alloc: at most 4096 values per cell are supported");
+  const int64_t N = d[0] * d[1] * d[2];
+  const int64_t B = blocklength == 0 ? std::max<int64_t>(N, 1) : blocklength;
+  const int64_t nblk = (N + B - 1) / B;
+  const int64_t bytes = std::max<int64_t>(nblk * B * nc * (int64_t)es, 256);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, (size_t)bytes);
+  if (e != hipSuccess) return fail(PS_EOOM, "OutOfMemoryError: cannot allocate " + std::to_string(bytes) + " bytes of device memory");
+  PS_HIP(hipMemsetAsync(p, 0, (size_t)bytes, S().stream0));
+  PS_HIP(hipStreamSynchronize(S().stream0));
+  std::memset(out, 0, sizeof *out);
+  out->ptr = p;
+  out->base = p;
+  for (int i = 0; i < 3; ++i) out->dims[i] = d[i];
+  out->pitch = d[0];
+  out->plane = d[0] * d[1];
+  out->dtype = dtype;
+  out->ndims = ndims;
+  out->alloc_bytes = bytes;
+  out->ncomp = (int32_t)nc;
+  for (int i = 0; i < 4; ++i) out->celldims[i] = cd[i];
+  out->blocklen = B;
+  return 0;
+}
+
 extern "C" int ps_field_free(ps_field* f) {
   if (!f || !f->base) return 0;
   if (int r = require_init()) return r;
@@ -475,6 +567,7 @@ extern "C" int ps_field_free(ps_field* f) {
 }
 
 extern "C" int ps_field_view(const ps_field* parent, const int64_t lo[3], const int64_t hi[3], ps_field* out) {
+  if (parent->ncomp > 0) return fail(PS_EARGUMENT, "ps_field_view: views of CellArrays are not supported");
   for (int i = 0; i < 3; ++i)
     if (lo[i] < 1 || hi[i] > parent->dims[i] || hi[i] < lo[i] - 1) return fail(PS_EARGUMENT, "ps_field_view: index box outside the parent field");
   *out = *parent;
@@ -486,8 +579,10 @@ extern "C" int ps_field_view(const ps_field* parent, const int64_t lo[3], const 
   return 0;
 }
 
-extern "C" int ps_field_h2d(const ps_field* f, const void* host, void* stream) {
+extern "C" int ps_field_h2d(const ps_field* f_in, const void* host, void* stream) {
   if (int r = require_init()) return r;
+  const ps_field fd = data_of(*f_in);
+  const ps_field* f = &fd;
   size_t es = elsize(f->dtype);
   hipStream_t st = stream_of(stream);
   const char* h = (const char*)host;
@@ -503,8 +598,10 @@ extern "C" int ps_field_h2d(const ps_field* f, const void* host, void* stream) {
   return 0;
 }
 
-extern "C" int ps_field_d2h(const ps_field* f, void* host, void* stream) {
+extern "C" int ps_field_d2h(const ps_field* f_in, void* host, void* stream) {
   if (int r = require_init()) return r;
+  const ps_field fd = data_of(*f_in);
+  const ps_field* f = &fd;
   size_t es = elsize(f->dtype);
   hipStream_t st = stream_of(stream);
   char* h = (char*)host;
@@ -525,8 +622,10 @@ static unsigned grid_for(long long n) {
   return (unsigned)std::max<long long>(1, std::min<long long>(b, 65536));
 }
 
-extern "C" int ps_field_fill(const ps_field* f, double value, void* stream) {
+extern "C" int ps_field_fill(const ps_field* f_in, double value, void* stream) {
   if (int r = require_init()) return r;
+  const ps_field fd = data_of(*f_in);
+  const ps_field* f = &fd;
   hipStream_t st = stream_of(stream);
   long long n = f->dims[0] * f->dims[1] * f->dims[2];
   if (n == 0) return 0;
@@ -551,8 +650,13 @@ extern "C" int ps_field_fill(const ps_field* f, double value, void* stream) {
   return 0;
 }
 
-extern "C" int ps_field_copy(const ps_field* dst, const ps_field* src, void* stream) {
+extern "C" int ps_field_copy(const ps_field* dst_in, const ps_field* src_in, void* stream) {
   if (int r = require_init()) return r;
+  if ((dst_in->ncomp > 0) != (src_in->ncomp > 0) ||
+      (dst_in->ncomp > 0 && (dst_in->ncomp != src_in->ncomp || dst_in->blocklen != src_in->blocklen)))
+    return fail(PS_EARGUMENT, "copy!: CellArrays differ in cell size or block length");
+  const ps_field dd = data_of(*dst_in), sd = data_of(*src_in);
+  const ps_field *dst = &dd, *src = &sd;
   for (int i = 0; i < 3; ++i)
     if (dst->dims[i] != src->dims[i]) return fail(PS_EARGUMENT, "copy!: fields differ in size");
   if (dst->dtype != src->dtype) return fail(PS_EARGUMENT, "copy!: fields differ in element type");
@@ -571,8 +675,10 @@ extern "C" int ps_field_copy(const ps_field* dst, const ps_field* src, void* str
   return 0;
 }
 
-extern "C" int ps_field_reduce(const ps_field* f, int op, double* out, void* stream) {
+extern "C" int ps_field_reduce(const ps_field* f_in, int op, double* out, void* stream) {
   if (int r = require_init()) return r;
+  const ps_field fd = data_of(*f_in);
+  const ps_field* f = &fd;
   if (op < 0 || op > 3) return fail(PS_EARGUMENT, "ps_field_reduce: op must be 0..3");
   State& s = S();
   hipStream_t st = stream_of(stream);
@@ -903,6 +1009,9 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   // an access outside it), which needs every array to hold at least one element
   for (int i = 0; i < nargs && memopt; ++i)
     if (args[i].kind == PS_ARG_FIELD && args[i].field.dims[0] * args[i].field.dims[1] * args[i].field.dims[2] == 0) memopt = false;
+  for (int i = 0; i < nargs && memopt; ++i)
+    if (args[i].kind == PS_ARG_FIELD && args[i].field.ncomp > 0)
+      throw PsError(PS_EARGUMENT, "@parallel <kernelcall>: memopt=true is not supported with CellArray arguments (argument " + std::to_string(i + 1) + ")");
   P.memopt = memopt;
   int64_t maxsize[3] = {1, 1, 1};
   int maxnd = 1;
@@ -1004,7 +1113,7 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   for (int i = 0; i < nargs; ++i) {
     if (args[i].kind == PS_ARG_FIELD) {
       const ps_field& f = args[i].field;
-      if (f.plane * std::max<int64_t>(f.dims[2], 1) + f.pitch > (int64_t)INT32_MAX / 2) big = true;
+      if (extent_bytes(f) / (int64_t)elsize(f.dtype) + f.pitch > (int64_t)INT32_MAX / 2) big = true;
     } else if (args[i].dtype == PS_I32 || args[i].dtype == PS_I64) {
       if (args[i].i > INT32_MAX / 2 || args[i].i < INT32_MIN / 2) big = true;
     }
@@ -1019,11 +1128,10 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
     for (int j = i + 1; j < nargs; ++j) {
       if (args[j].kind != PS_ARG_FIELD) continue;
       const ps_field &a = args[i].field, &b = args[j].field;
-      size_t ea = elsize(a.dtype), eb = elsize(b.dtype);
       const char* a0 = (const char*)a.ptr;
-      const char* a1 = a0 + (a.plane * std::max<int64_t>(a.dims[2], 1)) * (int64_t)ea;
+      const char* a1 = a0 + extent_bytes(a);
       const char* b0 = (const char*)b.ptr;
-      const char* b1 = b0 + (b.plane * std::max<int64_t>(b.dims[2], 1)) * (int64_t)eb;
+      const char* b1 = b0 + extent_bytes(b);
       if (a0 < b1 && b0 < a1) v.noalias = false;
     }
   }
@@ -1073,6 +1181,13 @@ std::string launch_key(const ps_arg* args, int nargs, const ps_launch* L) {
       put(&a.field.pitch, 8);
       put(&a.field.plane, 8);
       put(&a.field.ndims, 4);
+      if (a.field.ncomp > 0) {
+        put(&a.field.ncomp, 4);
+        put(a.field.celldims, 16);
+        put(&a.field.blocklen, 8);
+        if (a.field.cellnames) k.append(a.field.cellnames);
+        k.push_back(';');
+      }
     } else if (a.dtype == PS_F64 || a.dtype == PS_F32) {
       char m = 'x';
       if (a.dtype == PS_F64) m = rcp_safe64(a.f) ? (a.f > 0 ? 'p' : 'n') : 'x';
@@ -1086,12 +1201,12 @@ std::string launch_key(const ps_arg* args, int nargs, const ps_launch* L) {
     if (args[i].kind != PS_ARG_FIELD) continue;
     const ps_field& a = args[i].field;
     const char* a0 = (const char*)a.ptr;
-    const char* a1 = a0 + (a.plane * std::max<int64_t>(a.dims[2], 1)) * (int64_t)elsize(a.dtype);
+    const char* a1 = a0 + extent_bytes(a);
     for (int j = i + 1; j < nargs; ++j) {
       if (args[j].kind != PS_ARG_FIELD) continue;
       const ps_field& b = args[j].field;
       const char* b0 = (const char*)b.ptr;
-      const char* b1 = b0 + (b.plane * std::max<int64_t>(b.dims[2], 1)) * (int64_t)elsize(b.dtype);
+      const char* b1 = b0 + extent_bytes(b);
       if (a0 < b1 && b0 < a1) {
         int16_t pr[2] = {(int16_t)i, (int16_t)j};
         put(pr, 4);
@@ -1220,6 +1335,7 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
         break;
       }
       case 8: iv = launch ? launch->shmem : 0; break;
+      case 9: iv = args[p.arg].field.blocklen; break;
       case 7: {  // fastmath: RN(1/b) of a uniform divisor
         const ps_arg& a = args[p.arg];
         if (p.dtype == PS_F64) {

@@ -7,7 +7,8 @@ launches them. See `core` for the API mirror.
 from ._lib import (ArgumentError, BoundsError, CompileError, HIPError, IncoherentArgumentError, KeywordArgumentError,
                    MethodPluginError, ModuleInternalError, NoDeviceError, NotInitializedError, OutOfMemoryError,
                    ParseError, PSException, LIB_PATH)
-from .core import (AMDGPU, Float64, Float32, BFloat16, Int32, Int64, Bool, Array, ArgSpec, BoundLaunch, Data, DeviceArray, Event, Graph,
+from .core import (AMDGPU, Float64, Float32, BFloat16, Int32, Int64, Bool, Array, ArgSpec, BoundLaunch, CellArray, CellType, Data,
+                   DeviceArray, Event, Graph,
                    IncoherentCallError, Kernel, Stream, UnitRange, bf16_to_f32, compute_nblocks, compute_nthreads,
                    device_info, ensure_device, f32_to_bf16, falses, fill, fill_, get_priority_stream, get_ranges,
                    get_ranges_inner, get_ranges_outer, get_stream, hide_communication, init_parallel_stencil,

@@ -93,6 +93,10 @@ class ps_field(ctypes.Structure):
         ("ndims", ctypes.c_int32),
         ("base", ctypes.c_void_p),
         ("alloc_bytes", ctypes.c_int64),
+        ("ncomp", ctypes.c_int32),
+        ("celldims", ctypes.c_int32 * 4),
+        ("blocklen", ctypes.c_int64),
+        ("cellnames", ctypes.c_char_p),
     ]
 
 
@@ -145,6 +149,8 @@ _SIGS = {
     "ps_device_info": (ctypes.c_int, [_P(ps_devinfo)]),
     "ps_device_sync": (ctypes.c_int, []),
     "ps_field_alloc": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_int64), ctypes.c_int, _P(ps_field)]),
+    "ps_cellarray_alloc": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_int64), ctypes.c_int, _P(ctypes.c_int32), ctypes.c_int64, ctypes.c_int,
+                                          _P(ps_field)]),
     "ps_field_free": (ctypes.c_int, [_P(ps_field)]),
     "ps_field_view": (ctypes.c_int, [_P(ps_field), _P(ctypes.c_int64), _P(ctypes.c_int64), _P(ps_field)]),
     "ps_field_h2d": (ctypes.c_int, [_P(ps_field), ctypes.c_void_p, ctypes.c_void_p]),

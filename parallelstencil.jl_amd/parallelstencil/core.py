@@ -289,9 +289,10 @@ def bf16_to_f32(b):
 
 
 class ArgSpec:
-    """Shape and element type of a field, without memory (for ahead-of-time compilation)."""
+    """Shape and element type of a field, without memory (for ahead-of-time compilation); with
+    `celldims` (or a `celltype`) that of a CellArray."""
 
-    def __init__(self, shape, eltype=Float64):
+    def __init__(self, shape, eltype=Float64, celldims=None, blocklength=0, celltype=None):
         self.shape = tuple(shape)
         self.f = ps_field()
         for i in range(3):
@@ -300,6 +301,17 @@ class ArgSpec:
         self.f.dtype = dtype_code(eltype)
         self.f.pitch = self.shape[0] if self.shape else 1
         self.f.plane = self.f.pitch * (self.shape[1] if len(self.shape) > 1 else 1)
+        if celltype is not None:
+            celldims = celltype.dims
+            self._names = ",".join(celltype.fieldnames).encode()
+            self.f.cellnames = self._names
+        if celldims is not None:
+            cd = tuple(celldims) if isinstance(celldims, (tuple, list)) else (celldims,)
+            self.f.ncomp = int(np.prod(cd))
+            for i in range(4):
+                self.f.celldims[i] = cd[i] if i < len(cd) else 1
+            n = int(np.prod(self.shape))
+            self.f.blocklen = max(n, 1) if not blocklength else blocklength
         # distinct fake addresses (never dereferenced) so that aliasing analysis sees separate fields
         ArgSpec._next += 1
         self.f.ptr = ArgSpec._next << 40
@@ -311,7 +323,138 @@ def _numbertype_of(mod):
     return _configs[mod].numbertype if mod in _configs else Float64
 
 
-def _alloc(dims, eltype):
+# ---------------------------------------------------------------------------
+# CellArrays (src/ParallelKernel/allocators.jl:1-17,176-224,389-454; CellArrays.jl)
+# ---------------------------------------------------------------------------
+class CellType:
+    """@CellType(name; fieldnames, dims=length(fieldnames), eltype, parametric): a cell of named
+    values for the `celltype` keyword of the allocators. Values can always also be addressed by
+    index. A parametric cell type takes its element type as `T[eltype]`."""
+
+    def __init__(self, name, fieldnames=None, dims=None, eltype=None, parametric=False):
+        if fieldnames is None:
+            raise ArgumentError("@CellType: the keyword argument 'fieldnames' is mandatory.")
+        if isinstance(fieldnames, str):
+            fieldnames = (fieldnames,)
+        self.name = str(name)
+        self.fieldnames = tuple(str(f) for f in fieldnames)
+        self.dims = (len(self.fieldnames),) if dims is None else tuple(int(d) for d in (dims if isinstance(dims, (tuple, list)) else (dims,)))
+        if int(np.prod(self.dims)) != len(self.fieldnames):
+            raise ArgumentError(f"@CellType: prod(dims) == length(fieldnames) must hold (dims {self.dims}, {len(self.fieldnames)} fieldnames).")
+        if not 1 <= len(self.dims) <= 4:
+            raise ArgumentError("@CellType: cells of 1 to 4 dimensions are supported.")
+        if parametric and eltype is not None:
+            raise ArgumentError("@CellType: the keyword argument 'eltype' is invalid when `parametric=true` is set.")
+        self.parametric = bool(parametric)
+        self.eltype = eltype
+
+    def __getitem__(self, eltype):
+        if not self.parametric:
+            raise ArgumentError(f"@CellType {self.name} is not parametric")
+        return CellType(self.name, self.fieldnames, self.dims, eltype)
+
+    def __repr__(self):
+        return f"CellType({self.name}, fieldnames={self.fieldnames}, dims={self.dims})"
+
+
+class CellArray(DeviceArray):
+    """An array of cells (a CellArray of CellArrays.jl): `shape` is the grid of cells, `celldims`
+    the dimensions of each cell. The values of one cell field are stored in blocks of
+    `blocklength` cells; the default 0 stores each cell field contiguously over all cells."""
+
+    __slots__ = ("celltype", "_names")
+
+    def __init__(self, f, celltype=None):
+        super().__init__(f)
+        self.celltype = celltype
+        self._names = None
+        if celltype is not None:
+            self._names = ",".join(celltype.fieldnames).encode()
+            self.f.cellnames = self._names
+
+    @property
+    def celldims(self):
+        nd = 4
+        while nd > 1 and self.f.celldims[nd - 1] == 1:
+            nd -= 1
+        return tuple(int(self.f.celldims[i]) for i in range(nd))
+
+    @property
+    def blocklength(self):
+        return int(self.f.blocklen)
+
+    @property
+    def data(self):
+        """The data array, of size (blocklength, prod(celldims), ceil(length / blocklength))."""
+        n = int(np.prod(self.shape))
+        B, nc = int(self.f.blocklen), int(self.f.ncomp)
+        v = ps_field()
+        v.ptr, v.dtype, v.ndims = self.f.ptr, self.f.dtype, 3
+        for i, d in enumerate((B, nc, -(-n // B) if n else 0)):
+            v.dims[i] = d
+        v.pitch, v.plane = B, B * nc
+        return DeviceArray(v, owner=False, parent=self)
+
+    def _raw_shape(self):
+        n = int(np.prod(self.shape))
+        B = int(self.f.blocklen)
+        return (B, int(self.f.ncomp), -(-n // B) if n else 0)
+
+    def to_host(self, stream=None):
+        """The cells as a Fortran-ordered numpy array of shape shape + celldims."""
+        raw = self.data.to_host(stream)
+        B, nc, nb = self._raw_shape()
+        n = int(np.prod(self.shape))
+        cells = raw.transpose(0, 2, 1).reshape((B * nb, nc), order="F")[:n]
+        return np.asfortranarray(cells.reshape(self.shape + self.celldims, order="F"))
+
+    def copy_from(self, host, stream=None):
+        """copy!(A, host) with host of shape shape + celldims."""
+        h = np.asarray(host)
+        if h.shape != self.shape + self.celldims:
+            raise ArgumentError(f"copy!: shape mismatch (CellArray {self.shape} with cells {self.celldims}, host {h.shape})")
+        B, nc, nb = self._raw_shape()
+        n = int(np.prod(self.shape))
+        cells = np.zeros((B * nb, nc), dtype=h.dtype)
+        cells[:n] = h.reshape((n, nc), order="F")
+        raw = np.asfortranarray(cells.reshape((B, nb, nc), order="F").transpose(0, 2, 1))
+        self.data.copy_from(raw, stream)
+        return self
+
+    def fill_(self, value, stream=None):
+        self.data.fill_(value, stream)
+        return self
+
+    def copy_(self, other, stream=None, async_=False):
+        check(lib.ps_field_copy(ctypes.byref(self.f), ctypes.byref(other.f), _stream_handle(stream)))
+        if not async_ and stream is None:
+            check(lib.ps_stream_sync(None))
+        return self
+
+    def view(self, lo, hi):
+        raise ArgumentError("views of CellArrays are not supported")
+
+    def reduce(self, op, stream=None):
+        return self.data.reduce(op, stream)
+
+
+def _cell_spec(mod, eltype, celldims, celltype):
+    """(eltype, celldims, celltype) after the checks of determine_celltype (allocators.jl:415-430)."""
+    if celltype is not None:
+        if celldims is not None or eltype is not None:
+            raise ArgumentError("the keyword argument 'celltype' is incompatible with the other keyword arguments.")
+        if not isinstance(celltype, CellType):
+            raise ArgumentError("celltype must be created with CellType (@CellType)")
+        if celltype.parametric:
+            raise ArgumentError(f"the parametric cell type {celltype.name} needs an element type: {celltype.name}[eltype]")
+        return celltype.eltype or _numbertype_of(mod), celltype.dims, celltype
+    if celldims is None:
+        return eltype or _numbertype_of(mod), None, None
+    cd = tuple(int(d) for d in (celldims if isinstance(celldims, (tuple, list)) else (celldims,)))
+    return eltype or _numbertype_of(mod), cd, None
+
+
+def _alloc(dims, eltype, celldims=None, celltype=None, blocklength=None):
     ensure_device()
     if len(dims) == 1 and isinstance(dims[0], (tuple, list)):
         dims = tuple(dims[0])
@@ -320,35 +463,49 @@ def _alloc(dims, eltype):
         raise ArgumentError("fields have 1, 2 or 3 dimensions")
     f = ps_field()
     d3 = (ctypes.c_int64 * 3)(*(list(dims) + [1] * (3 - len(dims))))
-    check(lib.ps_field_alloc(len(dims), d3, dtype_code(eltype), ctypes.byref(f)))
-    return DeviceArray(f)
+    if celldims is None:  # blocklength only matters for cells (allocators.jl:15)
+        check(lib.ps_field_alloc(len(dims), d3, dtype_code(eltype), ctypes.byref(f)))
+        return DeviceArray(f)
+    if not 1 <= len(celldims) <= 4:
+        raise ArgumentError("celldims must have 1 to 4 dimensions")
+    c4 = (ctypes.c_int32 * 4)(*(list(celldims) + [1] * (4 - len(celldims))))
+    check(lib.ps_cellarray_alloc(len(dims), d3, len(celldims), c4, int(blocklength or 0), dtype_code(eltype), ctypes.byref(f)))
+    return CellArray(f, celltype)
 
 
-def zeros(*dims, eltype=None):
-    """@zeros(dims...; eltype=numbertype)."""
-    return _alloc(dims, eltype or _numbertype_of(_caller(1)))
+def _allocate(dims, kw, depth=2):
+    eltype, celldims, celltype = _cell_spec(_caller(depth), kw.pop("eltype", None), kw.pop("celldims", None), kw.pop("celltype", None))
+    blocklength = kw.pop("blocklength", None)
+    if kw:
+        raise KeywordArgumentError(f"invalid keyword argument(s): {', '.join(sorted(kw))}")
+    return _alloc(dims, eltype, celldims, celltype, blocklength)
 
 
-def ones(*dims, eltype=None):
-    A = _alloc(dims, eltype or _numbertype_of(_caller(1)))
-    return A.fill_(1.0)
+def zeros(*dims, **kw):
+    """@zeros(dims...; eltype=numbertype, celldims, celltype, blocklength)."""
+    return _allocate(dims, kw, 2)
 
 
-def fill(x, *dims, eltype=None):
-    A = _alloc(dims, eltype or _numbertype_of(_caller(1)))
-    return A.fill_(x)
+def ones(*dims, **kw):
+    return _allocate(dims, kw, 2).fill_(1.0)
+
+
+def fill(x, *dims, **kw):
+    return _allocate(dims, kw, 2).fill_(x)
 
 
 def fill_(A, x):
     return A.fill_(x)
 
 
-def falses(*dims):
-    return _alloc(dims, Bool)
+def falses(*dims, **kw):
+    kw.setdefault("eltype", Bool)
+    return _allocate(dims, kw, 2)
 
 
-def trues(*dims):
-    return _alloc(dims, Bool).fill_(1)
+def trues(*dims, **kw):
+    kw.setdefault("eltype", Bool)
+    return _allocate(dims, kw, 2).fill_(1)
 
 
 _rng = np.random.default_rng(0)
@@ -360,17 +517,18 @@ def seed(s):
     _rng = np.random.default_rng(s)
 
 
-def rand(*dims, eltype=None):
+def rand(*dims, **kw):
     """@rand: uniform [0,1) values drawn on the host (as the reference's AMDGPU allocator does) and copied."""
-    A = _alloc(dims, eltype or _numbertype_of(_caller(1)))
+    A = _allocate(dims, kw, 2)
     code = A.dtype_code
+    shape = A.shape + (A.celldims if isinstance(A, CellArray) else ())
     if code in (PS_F64, PS_F32, PS_BF16):
-        h = _rng.random(A.shape)
+        h = _rng.random(shape)
         h = f32_to_bf16(h.astype(np.float32)) if code == PS_BF16 else h.astype(_NP[code])
     elif code == PS_BOOL:
-        h = _rng.random(A.shape) < 0.5
+        h = _rng.random(shape) < 0.5
     else:
-        h = _rng.integers(0, 2**31 - 1, size=A.shape).astype(_NP[code])
+        h = _rng.integers(0, 2**31 - 1, size=shape).astype(_NP[code])
     return A.copy_from(np.asfortranarray(h))
 
 

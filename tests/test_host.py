@@ -248,3 +248,64 @@ def test_overlap_errors_on_the_host():
         ps.overlap((k, (None,), {"stream": 2}))
     with pytest.raises(ps.ArgumentError, match="only contain @parallel"):
         ps.overlap((k,))
+
+
+# ---------------------------------------------------------------- CellArrays
+def _cellspec(shape, celldims, blocklength=0, celltype=None):
+    return ps.ArgSpec(shape, celldims=celldims, blocklength=blocklength, celltype=celltype)
+
+
+def test_celltype_checks():
+    with pytest.raises(ps.ArgumentError, match="fieldnames"):
+        ps.CellType("C")
+    with pytest.raises(ps.ArgumentError, match="prod"):
+        ps.CellType("C", fieldnames=("a", "b", "c"), dims=(2, 2))
+    with pytest.raises(ps.ArgumentError, match="parametric"):
+        ps.CellType("C", fieldnames=("a", "b"), eltype=ps.Float32, parametric=True)
+    T = ps.CellType("T", fieldnames=("xx", "yy", "xy"))
+    assert T.dims == (3,)
+    P = ps.CellType("P", fieldnames=("xx", "yy"), parametric=True)
+    assert P[ps.Float32].eltype == ps.Float32
+
+
+def test_cellarray_lowering():
+    k = ps.parallel("@parallel function c!(T2, T, Ci)\n    @all(T2) = @all(T) + 2.0*@all(Ci)\n    return\nend")
+    A = _cellspec((8, 8, 8), (4, 4))
+    src = k.source(A, A, A)
+    assert src.count("a0[") == 16  # the statement once per value of the 4x4 cells
+    assert "a1[((a1_o)) + (15) * a1_B]" in src  # value 16 of a cell field stored contiguously
+    B = _cellspec((8, 8, 8), (4, 4), blocklength=1)
+    src = k.source(B, B, B)
+    assert "a0[((a0_o)) * 16 + (15)]" in src  # cells stored whole
+    assert not k.geometry(A, A, A).checked  # whole-cell accesses are proven like field accesses
+
+
+@pytest.mark.parametrize("body,match", [
+    ("T2[ix,iy,iz] = T[ix,iy,iz] + 1.0", "cannot be combined"),
+    ("T2[ix,iy,iz] = T[ix,iy,iz] * T[ix,iy,iz]", "products of two cells"),
+    ("T2[ix,iy,iz] = 1.0 / T[ix,iy,iz]", "division by a cell"),
+    ("T2[ix,iy,iz] = 3.0", "number is assigned"),
+    ("field(T2,1,1)[ix,iy,iz] = T[ix,iy,iz]", "cell is assigned"),
+    ("T2[ix,iy,iz] = sqrt(T[ix,iy,iz])", "where a number is expected"),
+    ("field(T2,3,1)[ix,iy,iz] = 1.0", "BoundsError"),
+    ("field(T2,1,1)[ix,iy,iz] = T[ix,iy,iz].zz", "no cell field"),
+])
+def test_cellarray_errors(body, match):
+    k = ps.parallel_indices(f"@parallel_indices (ix,iy,iz) function c!(T2, T)\n    {body}\n    return\nend")
+    Tn = ps.CellType("Tn", fieldnames=("xx", "yx", "xy", "yy"), dims=(2, 2))
+    with pytest.raises(ps.PSException, match=match):
+        k.source(_cellspec((4, 4, 4), (2, 2)), _cellspec((4, 4, 4), None, celltype=Tn))
+
+
+def test_cellarray_value_forms_are_checked_unless_literal():
+    k = ps.parallel_indices("""
+    @parallel_indices (ix,iy,iz) function c!(T2, T)
+        for c = 1:cellsize(T2,1)
+            field(T2,c,1)[ix,iy,iz] = T[ix,iy,iz][c,2]
+        end
+        return
+    end""")
+    A = _cellspec((4, 4, 4), (2, 2))
+    assert k.geometry(A, A).checked
+    k2 = ps.parallel_indices("@parallel_indices (ix,iy,iz) function c!(T2, T)\n    field(T2,2,1)[ix,iy,iz] = T[ix,iy,iz][1,2]\n    return\nend")
+    assert not k2.geometry(A, A).checked

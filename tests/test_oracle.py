@@ -399,3 +399,31 @@ def test_wavefront_model_closed_forms():
     assert O.wave_ballot(lanes % 2 == 0) == np.array([0x5555555555555555], dtype=np.uint64).view(np.int64)[0]
     assert O.wave_ballot(np.ones(64, bool)) == -1 and O.wave_ballot(np.zeros(64, bool)) == 0
     assert bin(int(np.array([O.wave_ballot(lanes < 10)]).view(np.uint64)[0])).count("1") == 10
+
+
+# ------------------------------------------------------------ CellArrays
+def test_cellarray_layout_closed_forms():
+    N, nc = 10, 6
+    for I in range(N):
+        for j in range(nc):
+            assert O.cellarray_data_offset(I, j, 1, nc) == I * nc + j  # cells stored whole
+            assert O.cellarray_data_offset(I, j, N, nc) == j * N + I  # each value contiguous
+    for B in (1, 3, 4, 10, 16):
+        nb = -(-N // B)
+        offs = sorted(O.cellarray_data_offset(I, j, B, nc) for I in range(N) for j in range(nc))
+        assert len(set(offs)) == N * nc and offs[-1] < B * nc * nb  # one slot per value, inside the data
+
+
+def test_cells_to_data_against_numpy_views():
+    rng = np.random.default_rng(8)
+    dims, celldims = (5, 3, 2), (2, 3)
+    cells = np.asfortranarray(rng.integers(0, 1000, dims + celldims).astype(np.float64))
+    N, nc = 30, 6
+    flat = cells.reshape((N, nc), order="F")
+    for B in (1, 4, 30):
+        nb = -(-N // B)
+        data = O.cells_to_data(cells, celldims, B)
+        pad = np.zeros((B * nb, nc))
+        pad[:N] = flat
+        # (B, nb, nc) view of the cells, values moved to the middle axis
+        assert np.array_equal(data, pad.reshape((B, nb, nc), order="F").transpose(0, 2, 1))
