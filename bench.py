@@ -348,6 +348,57 @@ def shmem_study(args):
     print(json.dumps(out), flush=True)
 
 
+def cellarray_study(args):
+    """examples/memcopyCellArray3D.jl: T2 = T + Ci over CellArrays of 4x4 FP64 cells at
+    128 x 128 x 1024 cells, in the example's three kernel forms (FiniteDifferences3D @all,
+    @parallel_indices over whole cells, per-field loops with field(A, i, j)), with each cell field
+    stored contiguously (blocklength 0, the GPU default) and with cells stored whole (blocklength
+    1); against a triad over plain fields of the same byte volume. T_eff uses the example's
+    A_eff = 3 x 8 B x cells x 16."""
+    import parallelstencil as ps
+
+    nx, ny, nz = 128, 128, 1024
+    K, Wm = args.steps, args.warmup
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3, module="bench.cells")
+    mod = "bench.cells"
+    d = lambda kind, text: ps.core._define(kind, text, {}, mod)
+    kfd = d("parallel", "@parallel function copy3D!(T2, T, Ci)\n    @all(T2) = @all(T) + @all(Ci)\n    return\nend")
+    kex = d("parallel_indices", "@parallel_indices (ix,iy,iz) function copy3D_explicit!(T2, T, Ci)\n"
+                                "    T2[ix,iy,iz] = T[ix,iy,iz] + Ci[ix,iy,iz]\n    return\nend")
+    klg = d("parallel_indices", "@parallel_indices (ix,iy,iz) function copy3D_large_cells!(T2, T, Ci)\n"
+                                "    for cy = 1:cellsize(T2,2)\n        for cx = 1:cellsize(T2,1)\n"
+                                "            field(T2,cx,cy)[ix,iy,iz] = field(T,cx,cy)[ix,iy,iz] + field(Ci,cx,cy)[ix,iy,iz]\n"
+                                "        end\n    end\n    return\nend")
+    tri = d("parallel", "@parallel function copy3D!(T2, T, Ci)\n    @all(T2) = @all(T) + @all(Ci)\n    return\nend")
+    bytes_step = 3 * 8 * nx * ny * nz * 16
+    runs = {}
+    for bl in (0, 1):
+        T = ps.fill(1.7, nx, ny, nz, celldims=(4, 4), blocklength=bl)
+        T2 = ps.fill(1.7, nx, ny, nz, celldims=(4, 4), blocklength=bl)
+        Ci = ps.fill(0.5, nx, ny, nz, celldims=(4, 4), blocklength=bl)
+        for name, k in (("@all (FiniteDifferences3D)", kfd), ("whole cells (@parallel_indices)", kex), ("field(A,i,j) loops", klg)):
+            st = [T, T2]
+            launch = lambda i, k=k: (k(st[1], st[0], Ci, async_=True), st.reverse())
+            wall, ms = time_kernel(ps, launch, K, Wm)
+            key = f"blocklength {bl}: {name}"
+            runs[key] = {"ms_per_step": round(ms, 4), "T_eff_gbs": round(bytes_step / ms / 1e6, 1)}
+            log(f"memcopyCellArray3D {nx}x{ny}x{nz} 4x4 cells, {key}: {ms:.4f} ms/step, T_eff {runs[key]['T_eff_gbs']} GB/s")
+        del T, T2, Ci
+    A, B, C = (ps.fill(1.0, nx, ny, nz * 16) for _ in range(3))
+    st = [A, B]
+    wall, ms = time_kernel(ps, lambda i: tri(st[1], st[0], C, async_=True), K, Wm)
+    runs["triad over plain fields of the same bytes"] = {"ms_per_step": round(ms, 4), "T_eff_gbs": round(bytes_step / ms / 1e6, 1)}
+    log(f"triad {nx}x{ny}x{nz * 16} plain: {ms:.4f} ms/step, T_eff {runs['triad over plain fields of the same bytes']['T_eff_gbs']} GB/s")
+    main_key = "blocklength 0: @all (FiniteDifferences3D)"
+    out = {"metric": "T_eff GB/s (memcopyCellArray3D, 4x4 FP64 cells)", "value": runs[main_key]["T_eff_gbs"], "unit": "GB/s", "n_gpus": 1,
+           "steps": K, "warmup": Wm, "ms_per_step": runs[main_key]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"examples/memcopyCellArray3D.jl at {nx}x{ny}x{nz} cells of 4x4"},
+           "roofline": {"bound": "hbm", "achieved": runs[main_key]["T_eff_gbs"], "peak": 8000.0, "unit": "GB/s",
+                        "frac": round(runs[main_key]["T_eff_gbs"] / 8000.0, 4), "traffic": None}, "runs": runs}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -356,7 +407,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--adjoint-n", type=int, default=2048)
@@ -384,6 +435,9 @@ def main():
         return
     if args.workload == "shmem":
         shmem_study(args)
+        return
+    if args.workload == "cellarray":
+        cellarray_study(args)
         return
 
     import parallelstencil as ps

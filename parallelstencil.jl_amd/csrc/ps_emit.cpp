@@ -260,6 +260,7 @@ class Emitter {
   // and the number of values per cell of its left-hand side
   int comp_ctx = -1;
   int comp_nc = 0;
+  std::set<std::string> loop_vars;  // variables of the enclosing for loops
   bool any_cell = false;
   bool is_cell(int ai) const { return ai >= 0 && kinds[ai].ncomp > 0; }
   int cell_shape(const NodeP& n);
@@ -898,6 +899,8 @@ void Emitter::assign(const NodeP& s) {
   if (lhs->k == NK::Ident) {
     const std::string& v = lhs->s;
     if (K.shared_arrays.count(v)) throw PsError(PS_EARGUMENT, "'" + v + "' is a @sharedMem array; assign to its elements");
+    // the bounds proof relies on a loop variable taking exactly the values of its range
+    if (loop_vars.count(v)) throw PsError(PS_EARGUMENT, "assignment to the for-loop variable '" + v + "' is not supported");
     if (K.param_index.count(v)) throw PsError(PS_EARGUMENT, "cannot assign to kernel argument '" + v + "' (arguments are passed by value)");
     for (int d = 0; d < 3; ++d)
       if (v == kIdx[d]) throw PsError(PS_EARGUMENT, "cannot assign to a parallel index");
@@ -999,7 +1002,10 @@ void Emitter::stmt(const NodeP& s) {
       local_c[v] = cv;
       o << pad() << "for (ps_int " << cv << " = " << a.c << "; " << cv << " <= " << b.c << "; ++" << cv << ") {\n";
       ++ind;
+      const bool outer = loop_vars.count(v) != 0;
+      loop_vars.insert(v);
       block(s->a[1]);
+      if (!outer) loop_vars.erase(v);
       --ind;
       o << pad() << "}\n";
       if (had) {
@@ -1918,6 +1924,43 @@ struct Prover {
   const LaunchShape& S;
   std::string why;
   bool ok = true;
+  // for-loop variables with constant bounds -> [first, last] (in scope while their body is scanned)
+  std::map<std::string, std::pair<long long, long long>> loopvars;
+
+  // The range of a cell value index: a literal or a loop variable with constant bounds.
+  bool value_range(const NodeP& e, long long& a, long long& b) {
+    long long v;
+    if (eval(e, v)) {
+      a = b = v;
+      return true;
+    }
+    if (e->k == NK::Ident) {
+      auto it = loopvars.find(e->s);
+      if (it != loopvars.end()) {
+        a = it->second.first;
+        b = it->second.second;
+        return true;
+      }
+    }
+    return false;
+  }
+  // Value indices of a cell of argument ai (extents as the emitter counts them: the last index
+  // runs over the remaining cell dimensions) lie inside the cell.
+  bool values_inside(int ai, const std::vector<NodeP>& cs, const NodeP& n) {
+    if (ai < 0 || ai >= (int)S.kinds.size() || S.kinds[ai].ncomp <= 0 || cs.empty() || cs.size() > 4) return false;
+    const int* cd = S.kinds[ai].celldims;
+    long long ext[4] = {1, 1, 1, 1};
+    for (size_t d = 0; d < cs.size(); ++d) ext[d] = cd[d];
+    for (size_t d = cs.size(); d < 4; ++d) ext[cs.size() - 1] *= cd[d];
+    for (size_t d = 0; d < cs.size(); ++d) {
+      long long a, b;
+      if (!value_range(cs[d], a, b) || a < 1 || b > ext[d]) {
+        why = "cell value index of " + to_julia(n) + " is not provably inside the cell";
+        return false;
+      }
+    }
+    return true;
+  }
 
   bool eval(const NodeP& e, long long& v) {
     switch (e->k) {
@@ -1940,6 +1983,14 @@ struct Prover {
           if (it == K.param_index.end() || !S.kinds[it->second].is_field) return false;
           long long d = e->a[1]->ival;
           v = (d >= 1 && d <= 3) ? S.dims[it->second][d - 1] : 1;
+          return true;
+        }
+        if (e->s == "cellsize" && e->a.size() == 2 && e->a[0]->k == NK::Ident && e->a[1]->k == NK::Num && e->a[1]->is_int) {
+          auto it = K.param_index.find(e->a[0]->s);
+          if (it == K.param_index.end() || S.kinds[it->second].ncomp <= 0) return false;
+          long long d = e->a[1]->ival;
+          if (d < 1) return false;
+          v = d <= 4 ? S.kinds[it->second].celldims[d - 1] : 1;
           return true;
         }
         return false;
@@ -2033,30 +2084,25 @@ struct Prover {
     // literals (their range is checked when the kernel is lowered), the cell access as usual
     if (n->a[0]->k == NK::Call && n->a[0]->s == "field") {
       const NodeP& f = n->a[0];
-      for (size_t i = 1; i < f->a.size(); ++i) {
-        long long v;
-        if (!eval(f->a[i], v)) {
-          ok = false;
-          why = "cell value index of " + to_julia(n) + " is not a literal";
-          return;
-        }
+      if (f->a.empty() || f->a[0]->k != NK::Ident) return;
+      auto it = K.param_index.find(f->a[0]->s);
+      if (!values_inside(it == K.param_index.end() ? -1 : it->second, std::vector<NodeP>(f->a.begin() + 1, f->a.end()), n)) {
+        ok = false;
+        return;
       }
-      if (f->a.empty()) return;
       std::vector<NodeP> a{f->a[0]};
       a.insert(a.end(), n->a.begin() + 1, n->a.end());
       check_access(mk(NK::Index, "", a), lo, hi);
       return;
     }
     if (n->a[0]->k == NK::Index) {
-      for (size_t i = 1; i < n->a.size(); ++i) {
-        long long v;
-        if (!eval(n->a[i], v)) {
-          ok = false;
-          why = "cell value index of " + to_julia(n) + " is not a literal";
-          return;
-        }
+      const NodeP& c = n->a[0];
+      auto it = c->a[0]->k == NK::Ident ? K.param_index.find(c->a[0]->s) : K.param_index.end();
+      if (!values_inside(it == K.param_index.end() ? -1 : it->second, std::vector<NodeP>(n->a.begin() + 1, n->a.end()), n)) {
+        ok = false;
+        return;
       }
-      check_access(n->a[0], lo, hi);
+      check_access(c, lo, hi);
       return;
     }
     auto it = K.param_index.find(n->a[0]->s);
@@ -2139,10 +2185,18 @@ struct Prover {
         return;
       }
       case NK::Block: scan_block(s, lo, hi); return;
-      case NK::For:
-        // accesses indexed by the loop variable are not affine in a parallel index
+      case NK::For: {
+        // accesses indexed by the loop variable are not affine in a parallel index; a loop with
+        // constant bounds gives its variable a known range (cell value indices)
+        long long a, b;
+        const bool known = eval(s->a[0]->a[0], a) && eval(s->a[0]->a[1], b);
+        auto saved = loopvars;
+        if (known) loopvars[s->s] = {a, b};
+        else loopvars.erase(s->s);
         scan_block(s->a[1], lo, hi);
+        loopvars = saved;
         return;
+      }
       default: scan_expr(s, lo, hi);
     }
   }

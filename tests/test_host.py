@@ -306,6 +306,15 @@ def test_cellarray_value_forms_are_checked_unless_literal():
         return
     end""")
     A = _cellspec((4, 4, 4), (2, 2))
-    assert k.geometry(A, A).checked
+    assert not k.geometry(A, A).checked  # c runs over 1:cellsize(T2,1) = 1:2: inside the 2x2 cells
     k2 = ps.parallel_indices("@parallel_indices (ix,iy,iz) function c!(T2, T)\n    field(T2,2,1)[ix,iy,iz] = T[ix,iy,iz][1,2]\n    return\nend")
     assert not k2.geometry(A, A).checked
+    k3 = ps.parallel_indices("@parallel_indices (ix,iy,iz) function c!(T2, T)\n    for c = 1:3\n"
+                             "        field(T2,c,1)[ix,iy,iz] = T[ix,iy,iz][c,1]\n    end\n    return\nend")
+    assert k3.geometry(A, A).checked  # c = 3 lies outside the cells
+    k4 = ps.parallel_indices("@parallel_indices (ix,iy,iz) function c!(T2, T, n)\n    for c = 1:n\n"
+                             "        field(T2,c,1)[ix,iy,iz] = T[ix,iy,iz][c,1]\n    end\n    return\nend")
+    assert not k4.geometry(A, A, 2).checked and k4.geometry(A, A, 3).checked  # integer arguments enter the proof
+    with pytest.raises(ps.ArgumentError, match="loop variable"):
+        ps.parallel_indices("@parallel_indices (ix) function c!(A)\n    for c = 1:3\n        c = 2\n"
+                            "        A[ix] = c\n    end\n    return\nend").source(ps.ArgSpec((8,)))
