@@ -49,6 +49,7 @@ enum class NK {
   Block,    // a... statements
   Return,   // a[0] optional
   For,      // s = loop var, a[0] = Range, a[1] = Block
+  FuncDef,  // nested function: s = name, a[0] = Tuple of parameter Idents, a[1] = Block
 };
 
 struct Node {

@@ -11,6 +11,7 @@
 #include "ps_lower.h"
 
 #include <algorithm>
+#include <functional>
 #include <cstdio>
 #include <sstream>
 
@@ -582,6 +583,132 @@ static void memopt_analysis(Kernel& K) {
   }
 }
 
+// Nested functions of @parallel_indices kernels (test/ParallelKernel/test_parallel.jl:387-433): a
+// function defined at the top of the body is inlined where it is called. Called as a statement,
+// its body runs in place (a final `return` ends it); called as a value, its body must be a single
+// `return expr`. Parameters are replaced by the call's arguments, and variables the body assigns
+// get names of their own, as in Julia's function scope.
+namespace {
+struct NestedFn {
+  std::vector<std::string> params;
+  NodeP body;
+};
+}  // namespace
+
+static NodeP inline_nested_functions(const NodeP& body) {
+  std::map<std::string, NestedFn> fns;
+  auto rest = mk(NK::Block);
+  for (auto& st : body->a) {
+    if (st->k == NK::FuncDef) {
+      NestedFn f;
+      for (auto& q : st->a[0]->a) f.params.push_back(q->s);
+      f.body = st->a[1];
+      fns[st->s] = f;
+    } else {
+      rest->a.push_back(st);
+    }
+  }
+  walk(rest, [&](const NodeP& x) {
+    if (x->k == NK::FuncDef) throw PsError(PS_EARGUMENT, "nested functions must be defined at the top level of the kernel body (" + x->s + ")");
+  });
+  if (fns.empty()) return body;
+  for (auto& f : fns)
+    walk(f.second.body, [&](const NodeP& x) {
+      if (x->k == NK::Assign && x->a[0]->k == NK::Ident &&
+          std::find(f.second.params.begin(), f.second.params.end(), x->a[0]->s) != f.second.params.end())
+        throw PsError(PS_EARGUMENT, "nested function " + f.first + ": assignment to its parameter " + x->a[0]->s + " is not supported");
+      if (x->k == NK::FuncDef) throw PsError(PS_EARGUMENT, "nested function " + f.first + " defines a function itself");
+    });
+  int counter = 0;
+  auto instantiate = [&](const std::string& name, const std::vector<NodeP>& args) {
+    const NestedFn& f = fns.at(name);
+    if (args.size() != f.params.size())
+      throw PsError(PS_EARGUMENT, "MethodError: nested function " + name + " takes " + std::to_string(f.params.size()) + " arguments (" +
+                                      std::to_string(args.size()) + " given)");
+    std::map<std::string, NodeP> sub;
+    for (size_t i = 0; i < args.size(); ++i) sub[f.params[i]] = args[i];
+    ++counter;
+    std::map<std::string, std::string> ren;
+    walk(f.body, [&](const NodeP& x) {
+      if (x->k == NK::Assign && x->a[0]->k == NK::Ident) ren[x->a[0]->s] = x->a[0]->s + "__" + name + "_" + std::to_string(counter);
+      if (x->k == NK::For) ren[x->s] = x->s + "__" + name + "_" + std::to_string(counter);
+    });
+    for (auto& r : ren) sub[r.first] = mk(NK::Ident, r.second);
+    NodeP b = subst_idents(f.body, sub);
+    walk(b, [&](const NodeP& x) {
+      if (x->k == NK::For && ren.count(x->s)) x->s = ren[x->s];
+    });
+    return b;
+  };
+  std::function<NodeP(const NodeP&, int)> ex = [&](const NodeP& n, int depth) -> NodeP {
+    return rewrite(n, [&](const NodeP& x) -> NodeP {
+      if (x->k == NK::Call && fns.count(x->s)) {
+        if (depth > 16) throw PsError(PS_EARGUMENT, "nested function " + x->s + " calls itself (recursion is not supported)");
+        std::vector<NodeP> args;
+        for (auto& a : x->a) args.push_back(ex(a, depth));
+        NodeP b = instantiate(x->s, args);
+        if (b->a.size() != 1 || b->a[0]->k != NK::Return || b->a[0]->a.empty())
+          throw PsError(PS_EARGUMENT, "nested function " + x->s + " is used as a value, so its body must be a single `return expr`");
+        return ex(b->a[0]->a[0], depth + 1);
+      }
+      return x;
+    });
+  };
+  std::function<void(const NodeP&, NodeP&, int)> st;
+  auto blk = [&](const NodeP& b, int depth) {
+    auto out = mk(NK::Block);
+    for (auto& t : b->a) st(t, out, depth);
+    return out;
+  };
+  st = [&](const NodeP& t, NodeP& out, int depth) {
+    if (t->k == NK::Call && fns.count(t->s)) {
+      if (depth > 16) throw PsError(PS_EARGUMENT, "nested function " + t->s + " calls itself (recursion is not supported)");
+      std::vector<NodeP> args;
+      for (auto& a : t->a) args.push_back(ex(a, depth));
+      NodeP b = instantiate(t->s, args);
+      for (size_t i = 0; i < b->a.size(); ++i) {
+        if (b->a[i]->k == NK::Return) {
+          if (i + 1 != b->a.size())
+            throw PsError(PS_EARGUMENT, "nested function " + t->s + ": a return before its last statement is not supported");
+          break;  // the value of a function called as a statement is discarded
+        }
+        walk(b->a[i], [&](const NodeP& x) {
+          if (x->k == NK::Return) throw PsError(PS_EARGUMENT, "nested function " + t->s + ": a return inside a branch or loop is not supported");
+        });
+        st(b->a[i], out, depth + 1);
+      }
+      return;
+    }
+    if (t->k == NK::If) {
+      NodeP els;
+      if (t->a.size() > 2 && t->a[2]) {
+        if (t->a[2]->k == NK::If) {
+          auto tmp = mk(NK::Block);
+          st(t->a[2], tmp, depth);
+          els = tmp->a.size() == 1 ? tmp->a[0] : tmp;
+        } else {
+          els = blk(t->a[2], depth);
+        }
+      }
+      out->a.push_back(mk(NK::If, "", {ex(t->a[0], depth), blk(t->a[1], depth), els}));
+      return;
+    }
+    if (t->k == NK::For) {
+      auto r = mk(NK::Range, t->a[0]->s, {ex(t->a[0]->a[0], depth), ex(t->a[0]->a[1], depth)});
+      auto f = mk(NK::For, t->s, {r, blk(t->a[1], depth)});
+      f->line = t->line;
+      out->a.push_back(f);
+      return;
+    }
+    if (t->k == NK::Block) {
+      out->a.push_back(blk(t, depth));
+      return;
+    }
+    out->a.push_back(ex(t, depth));
+  };
+  return blk(rest, 0);
+}
+
 // Kernel language of @parallel_indices kernels (src/ParallelKernel/kernel_language.jl:9-193, argument
 // checks :264-295). Validates the argument count of each macro call; the lowering happens in the
 // emitter.
@@ -623,17 +750,25 @@ Kernel define_kernel(const std::string& text, const JVal& opts) {
     throw PsError(PS_EARGUMENT, "numbertype must be Float64 or Float32 (obtained: " + K.opt.numbertype + ")");
   for (size_t i = 0; i < K.def.params.size(); ++i) K.param_index[K.def.params[i].name] = (int)i;
 
-  // `return` statements at the end of the body are dropped (add_return re-adds them).
+  // remove_return (src/ParallelKernel/shared.jl:200-209): the last statement must return
+  // nothing and is dropped (the generated kernel returns at its end); no other return statement
+  // may appear, except inside nested function definitions.
   auto strip_returns = [](NodeP body) {
+    auto nothing_ret = [](const NodeP& s) {
+      if (s->k == NK::Return) return s->a.empty() || (s->a[0]->k == NK::Ident && s->a[0]->s == "nothing");
+      return s->k == NK::Ident && s->s == "nothing";
+    };
+    if (body->a.empty() || !nothing_ret(body->a.back()))
+      throw PsError(PS_EARGUMENT, "invalid kernel in @parallel kernel definition: the last statement must be a `return nothing` statement ('return' or 'return nothing' or 'nothing') as required for any GPU kernels.");
     auto b = mk(NK::Block);
-    for (auto& s : body->a) {
-      if (s->k == NK::Return) {
-        if (!s->a.empty() && !(s->a[0]->k == NK::Ident && s->a[0]->s == "nothing"))
-          throw PsError(PS_EARGUMENT, "kernels must return nothing");
-        continue;
-      }
-      b->a.push_back(s);
-    }
+    b->a.assign(body->a.begin(), body->a.end() - 1);
+    std::function<void(const NodeP&)> scan = [&](const NodeP& x) {
+      if (!x || x->k == NK::FuncDef) return;
+      if (x->k == NK::Return)
+        throw PsError(PS_EARGUMENT, "invalid kernel in @parallel kernel definition: only one return statement is allowed in the kernel (exception: nested function definitions) and it must return nothing and be the last statement (required to ensure equal behaviour with different packages for parallellization).");
+      for (auto& c : x->a) scan(c);
+    };
+    scan(b);
     return b;
   };
 
@@ -706,7 +841,7 @@ Kernel define_kernel(const std::string& text, const JVal& opts) {
     K.nidx = (int)K.opt.indices.size();
     std::map<std::string, NodeP> sub;
     for (int d = 0; d < K.nidx; ++d) sub[K.opt.indices[d]] = mk(NK::Ident, kIdx[d]);
-    NodeP body = strip_returns(K.def.body);
+    NodeP body = inline_nested_functions(strip_returns(K.def.body));
     body = subst_idents(body, sub);
     body = rewrite(body, [&](const NodeP& x) -> NodeP {
       if (x->k == NK::Interp) {

@@ -524,25 +524,49 @@ struct Parser {
     if (is_kw("local")) ++p;
     if (is_kw("for")) {
       ++p;
-      if (cur().k != TK::Ident) fail("expected loop variable");
-      std::string v = cur().s;
-      ++p;
-      if (is_op("=") || is_kw("in") || is_op("∈")) ++p;
-      else fail("expected '=' or 'in' in for loop");
-      NodeP r = expr();
-      if (r->k != NK::Range) fail("for loops must iterate over a range a:b");
+      // for a = r1, b = r2 ... end  is  for a = r1; for b = r2 ... end; end
+      std::vector<std::pair<std::string, NodeP>> loops;
+      for (;;) {
+        if (cur().k != TK::Ident) fail("expected loop variable");
+        std::string v = cur().s;
+        ++p;
+        if (is_op("=") || is_kw("in") || is_op("∈")) ++p;
+        else fail("expected '=' or 'in' in for loop");
+        NodeP r = expr();
+        if (r->k != NK::Range) fail("for loops must iterate over a range a:b");
+        loops.emplace_back(v, r);
+        if (!is_op(",")) break;
+        ++p;
+      }
       auto b = block();
       if (!is_kw("end")) fail("expected 'end' closing 'for'");
       ++p;
-      auto f = mk(NK::For, v, {r, b});
-      f->line = line;
+      NodeP f = b;
+      for (size_t i = loops.size(); i-- > 0;) {
+        f = mk(NK::For, loops[i].first, {loops[i].second, i + 1 == loops.size() ? f : mk(NK::Block, "", {f})});
+        f->line = line;
+      }
       return f;
+    }
+    if (is_kw("function")) {  // nested function (long form)
+      ++p;
+      NodeP sig = expr();
+      if (sig->k != NK::Call) fail("expected a function signature name(args...)");
+      auto b = block();
+      if (!is_kw("end")) fail("expected 'end' closing 'function'");
+      ++p;
+      return funcdef(sig, b);
     }
     if (is_op("@") && peek().k == TK::Ident && (peek().s == "inbounds" || peek().s == "inline" || peek().s == "fastmath") && !(peek(2).k == TK::Op && peek(2).s == "(")) {
       p += 2;  // statement-level annotations carry no meaning for generated code
       return statement();
     }
     NodeP lhs = expr();
+    if (lhs->k == NK::Call && is_op("=")) {  // nested function (short form): name(args...) = body
+      ++p;
+      skip_nl_only();
+      return funcdef(lhs, short_body());
+    }
     if (is_op("=") || is_op("+=") || is_op("-=") || is_op("*=") || is_op("/=")) {
       std::string o = cur().s;
       ++p;
@@ -553,6 +577,37 @@ struct Parser {
       return a;
     }
     return lhs;  // expression statement
+  }
+  // Body of a short-form function: `return e`, `(stmt; stmt; ...)` or an expression (its value).
+  NodeP short_body() {
+    if (is_kw("return")) return mk(NK::Block, "", {statement()});
+    if (is_op("(")) {
+      ++p;
+      int saved = bracket;
+      bracket = 0;
+      auto b = block();
+      bracket = saved;
+      if (!is_op(")")) fail("expected ')'");
+      ++p;
+      if (b->a.size() == 1) {
+        NK k = b->a[0]->k;
+        if (k != NK::Assign && k != NK::Return && k != NK::If && k != NK::For && k != NK::Block && k != NK::FuncDef)
+          return mk(NK::Block, "", {mk(NK::Return, "", {b->a[0]})});
+      }
+      return b;
+    }
+    return mk(NK::Block, "", {mk(NK::Return, "", {expr()})});
+  }
+  NodeP funcdef(const NodeP& sig, const NodeP& body) {
+    auto params = mk(NK::Tuple);
+    for (auto& a : sig->a) {
+      NodeP x = a->k == NK::Typed ? a->a[0] : a;
+      if (x->k != NK::Ident) fail("nested function parameters must be names");
+      params->a.push_back(x);
+    }
+    auto f = mk(NK::FuncDef, sig->s, {params, body});
+    f->line = sig->line;
+    return f;
   }
   NodeP if_rest() {
     NodeP c = expr();
@@ -713,6 +768,9 @@ void emit(const NodeP& n, std::string& o) {
       break;
     case NK::For:
       o += "for " + n->s + " = "; emit(n->a[0], o); o += "\n"; emit(n->a[1], o); o += "end";
+      break;
+    case NK::FuncDef:
+      o += "function " + n->s; emit(n->a[0], o); o += "\n"; emit(n->a[1], o); o += "end";
       break;
   }
 }

@@ -25,10 +25,8 @@ end
 """
 COPY_LARGE = """
 @parallel_indices (ix,iy,iz) function copy3D_large_cells!(T2, T, Ci)
-    for cy = 1:cellsize(T2,2)
-        for cx = 1:cellsize(T2,1)
-            field(T2,cx,cy)[ix,iy,iz] = field(T,cx,cy)[ix,iy,iz] + field(Ci,cx,cy)[ix,iy,iz]
-        end
+    for cy = 1:cellsize(T2,2), cx = 1:cellsize(T2,1)
+        field(T2,cx,cy)[ix,iy,iz] = field(T,cx,cy)[ix,iy,iz] + field(Ci,cx,cy)[ix,iy,iz]
     end
     return
 end

@@ -233,3 +233,49 @@ def test_hide_communication_kat():
     ref[:, :, 0] = ref[:, :, -2]
     ref[:, :, -1] = ref[:, :, 1]
     assert np.array_equal(A.to_host(), ref)
+
+
+# --------------------------------------------------------------- nested functions (ParallelKernel/test_parallel.jl:368-433)
+NESTED = {
+    "long_modify": """@parallel_indices (ix,iy,iz) function write_indices!(A)
+    function compute_indices!(A)
+        A[ix,iy,iz] = ix + (iy-1)*size(A,1) + (iz-1)*size(A,1)*size(A,2);
+        return
+    end
+    compute_indices!(A)
+    return
+end""",
+    "short_modify": """@parallel_indices (ix,iy,iz) function write_indices!(A)
+    compute_indices!(A) = (A[ix,iy,iz] = ix + (iy-1)*size(A,1) + (iz-1)*size(A,1)*size(A,2); return)
+    compute_indices!(A)
+    return
+end""",
+    "long_value": """@parallel_indices (ix,iy,iz) function write_indices!(A)
+    function compute_indices(A)
+        return ix + (iy-1)*size(A,1) + (iz-1)*size(A,1)*size(A,2)
+    end
+    A[ix,iy,iz] = compute_indices(A)
+    return
+end""",
+    "short_value": """@parallel_indices (ix,iy,iz) function write_indices!(A)
+    compute_indices(A) = return ix + (iy-1)*size(A,1) + (iz-1)*size(A,1)*size(A,2)
+    A[ix,iy,iz] = compute_indices(A)
+    return
+end""",
+}
+
+
+@pytest.mark.parametrize("form", sorted(NESTED))
+def test_nested_functions(form):
+    A = ps.zeros(4, 5, 6)
+    ps.parallel_indices(NESTED[form])(A)
+    assert np.array_equal(A.to_host(), O.linear_indices((4, 5, 6)))
+
+
+def test_write_indices_1d_2d():
+    # ParallelKernel/test_parallel.jl:368-386
+    A1, A2 = ps.zeros(4), ps.zeros(4, 5)
+    ps.parallel_indices("@parallel_indices (ix) function write_indices!(A)\n    A[ix] = ix;\n    return\nend")(A1)
+    ps.parallel_indices("@parallel_indices (ix,iy) function write_indices!(A)\n    A[ix,iy] = ix + (iy-1)*size(A,1);\n    return\nend")(A2)
+    assert np.array_equal(A1.to_host(), np.arange(1, 5, dtype=float))
+    assert np.array_equal(A2.to_host(), O.linear_indices((4, 5)).reshape(4, 5, order="F"))

@@ -318,3 +318,62 @@ def test_cellarray_value_forms_are_checked_unless_literal():
     with pytest.raises(ps.ArgumentError, match="loop variable"):
         ps.parallel_indices("@parallel_indices (ix) function c!(A)\n    for c = 1:3\n        c = 2\n"
                             "        A[ix] = c\n    end\n    return\nend").source(ps.ArgSpec((8,)))
+
+
+# ---------------------------------------------------------------- nested functions
+@pytest.mark.parametrize("body,match", [
+    ("f(A) = (A[ix] = 1.0; return)\n    g(A) = f(A)\n    A[ix] = g(A)", "single `return expr`"),
+    ("f(A) = f(A)\n    A[ix] = f(A)", "recursion"),
+    ("function f(x)\n        x = x + 1\n        return x\n    end\n    A[ix] = f(1.0)", "parameter"),
+    ("function f(A)\n        if ix > 2\n            return\n        end\n        A[ix] = 1.0\n    end\n    f(A)", "inside a branch"),
+    ("f(a, b) = a + b\n    A[ix] = f(1.0)", "MethodError"),
+    ("if ix > 1\n        f(A) = 2.0\n    end\n    A[ix] = 1.0", "top level"),
+])
+def test_nested_function_errors(body, match):
+    with pytest.raises(ps.PSException, match=match):
+        k = ps.parallel_indices(f"@parallel_indices (ix) function k!(A)\n    {body}\n    return\nend")
+        k.source(ps.ArgSpec((8,)))
+
+
+def test_nested_function_locals_get_their_own_scope():
+    with pytest.raises(ps.ArgumentError, match="single `return expr`"):  # refused rather than misread
+        ps.parallel_indices("""@parallel_indices (ix) function k!(A)
+    function twice(a)
+        t = 2.0*a
+        return t
+    end
+    t = 1.0
+    A[ix] = twice(A[ix]) + t
+    return
+end""")
+    k2 = ps.parallel_indices("""@parallel_indices (ix) function k!(A, B)
+    function step!(A, B)
+        t = 2.0*B[ix]
+        A[ix] = t
+    end
+    t = 1.0
+    step!(A, B)
+    A[ix] = A[ix] + t
+    return
+end""")
+    src = k2.source(ps.ArgSpec((8,)), ps.ArgSpec((8,)))
+    assert "v_t__step_21" in src and "v_t = " in src  # the function's t is not the kernel's t
+
+
+def test_for_with_several_variables():
+    k = ps.parallel_indices("@parallel_indices (ix) function k!(A)\n    for a = 1:2, b = 1:3\n        A[ix] = A[ix] + a*b\n    end\n    return\nend")
+    src = k.source(ps.ArgSpec((8,)))
+    assert "for (ps_int f_a" in src and "for (ps_int f_b" in src
+
+
+@pytest.mark.parametrize("text,match", [
+    ("@parallel_indices (ix) function f!(A)\n    A[ix] = 1.0\nend", "last statement must be a `return nothing`"),
+    ("@parallel_indices (ix) function f!(A)\n    A[ix] = 1.0\n    return A\nend", "last statement must be a `return nothing`"),
+    ("@parallel_indices (ix) function f!(A)\n    if ix > 1\n        return\n    end\n    A[ix] = 1.0\n    return\nend", "only one return statement"),
+])
+def test_return_statement_rules(text, match):
+    # remove_return, src/ParallelKernel/shared.jl:200-209
+    with pytest.raises(ps.ArgumentError, match=match):
+        ps.parallel_indices(text)
+    ps.parallel_indices("@parallel_indices (ix) function f!(A)\n    A[ix] = 1.0\n    nothing\nend")
+    ps.parallel_indices("@parallel_indices (ix) function f!(A)\n    A[ix] = 1.0\n    return nothing\nend")
