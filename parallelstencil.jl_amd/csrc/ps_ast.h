@@ -50,6 +50,7 @@ enum class NK {
   Return,   // a[0] optional
   For,      // s = loop var, a[0] = Range, a[1] = Block
   FuncDef,  // nested function: s = name, a[0] = Tuple of parameter Idents, a[1] = Block
+  Forall,   // @∀ x ∈ X statement: a[0] = variable (Ident or Tuple of Idents), a[1] = X, a[2] = statement
 };
 
 struct Node {

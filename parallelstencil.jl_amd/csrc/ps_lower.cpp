@@ -309,6 +309,15 @@ static void read_options(Options& o, const JVal& j) {
     } else if (k == "lds") o.lds = v.b;
     else if (k == "xcd_remap") o.xcd_remap = v.b;
     else if (k == "nt_store") o.nt_store = v.b;
+    else if (k == "tuple_params") {
+      o.tuple_params.clear();
+      for (auto& e : v.obj) {
+        std::vector<std::string> names;
+        for (auto& x : e.second.arr) names.push_back(x.s);
+        if (names.empty()) throw PsError(PS_EARGUMENT, "tuple argument '" + e.first + "' has no components");
+        o.tuple_params.emplace_back(e.first, names);
+      }
+    }
     else if (k == "useshmemhalos" || k == "optimize_halo_read" || k == "nonconst_metadata" || k == "metadata_module" || k == "N") {
       // accepted for interface compatibility; they only steer the reference's shared-memory path
     } else {
@@ -583,6 +592,102 @@ static void memopt_analysis(Kernel& K) {
   }
 }
 
+// @∀ x ∈ X statement (src/ParallelKernel/kernel_language.jl:197-237, 543-566): the statement once per
+// element of X, with x replaced by the element: as a name (also as a field name, `C.x`), a
+// number (X = a:b with literal bounds) or, for a tuple of variables, element-wise.
+static NodeP expand_forall(const NodeP& n);
+static void expand_forall_into(const NodeP& st, NodeP& out) {
+  if (st->k != NK::Forall) {
+    out->a.push_back(expand_forall(st));
+    return;
+  }
+  const NodeP& var = st->a[0];
+  const NodeP& coll = st->a[1];
+  std::vector<NodeP> elems;
+  if (coll->k == NK::Range) {
+    Aff a = affine(coll->a[0]), b = affine(coll->a[1]);
+    if (!a.ok || a.var >= 0 || !b.ok || b.var >= 0) throw PsError(PS_EARGUMENT, "@∀: a range must have literal bounds");
+    for (long long v = a.c; v <= b.c; ++v) elems.push_back(mknum_int(v));
+  } else if (coll->k == NK::Tuple) {
+    elems = coll->a;
+  } else if (coll->k == NK::Ident) {
+    elems = {coll};
+  } else {
+    throw PsError(PS_EARGUMENT, "@∀: the set must be a tuple of names or a range a:b");
+  }
+  std::vector<std::string> vars;
+  if (var->k == NK::Ident) vars.push_back(var->s);
+  else
+    for (auto& v : var->a) {
+      if (v->k != NK::Ident) throw PsError(PS_EARGUMENT, "@∀: variables must be names");
+      vars.push_back(v->s);
+    }
+  for (auto& e : elems) {
+    std::vector<NodeP> vals;
+    if (vars.size() == 1) vals.push_back(e);
+    else {
+      if (e->k != NK::Tuple || e->a.size() != vars.size())
+        throw PsError(PS_EARGUMENT, "@∀: each element must be a tuple of " + std::to_string(vars.size()) + " values");
+      vals = e->a;
+    }
+    std::map<std::string, NodeP> sub;
+    for (size_t i = 0; i < vars.size(); ++i) sub[vars[i]] = vals[i];
+    NodeP b = subst_idents(st->a[2], sub);
+    walk(b, [&](const NodeP& x) {  // field names: C.i -> C.x
+      if (x->k == NK::Dot) {
+        auto it = sub.find(x->s);
+        if (it != sub.end()) {
+          if (it->second->k != NK::Ident) throw PsError(PS_EARGUMENT, "@∀: field ." + x->s + " needs a name, not a number");
+          x->s = it->second->s;
+        }
+      }
+    });
+    expand_forall_into(b, out);
+  }
+}
+static NodeP expand_forall(const NodeP& n) {
+  if (!n) return n;
+  if (n->k == NK::Block) {
+    auto out = mk(NK::Block);
+    for (auto& st : n->a) expand_forall_into(st, out);
+    return out;
+  }
+  auto c = std::make_shared<Node>(*n);
+  for (auto& x : c->a) x = expand_forall(x);
+  return c;
+}
+
+// Tuple and NamedTuple arguments: C.x and C[k] (k a literal) name the component parameter.
+static NodeP expand_tuple_args(const NodeP& body, const Options& o) {
+  std::map<std::string, const std::vector<std::string>*> tp;
+  for (auto& t : o.tuple_params) tp[t.first] = &t.second;
+  if (tp.empty()) return body;
+  NodeP b = rewrite(clone(body), [&](const NodeP& x) -> NodeP {
+    if (x->k == NK::Dot && x->a[0]->k == NK::Ident && tp.count(x->a[0]->s)) {
+      const auto& names = *tp[x->a[0]->s];
+      if (std::find(names.begin(), names.end(), x->s) == names.end())
+        throw PsError(PS_EARGUMENT, "tuple argument '" + x->a[0]->s + "' has no component ." + x->s);
+      return mk(NK::Ident, x->a[0]->s + "." + x->s);
+    }
+    if (x->k == NK::Index && x->a[0]->k == NK::Ident && tp.count(x->a[0]->s) && x->a.size() == 2 && x->a[1]->k == NK::Num && x->a[1]->is_int) {
+      const auto& names = *tp[x->a[0]->s];
+      long long k = x->a[1]->ival;
+      if (k < 1 || k > (long long)names.size())
+        throw PsError(PS_EBOUNDS, "BoundsError: tuple argument '" + x->a[0]->s + "' has " + std::to_string(names.size()) + " components (index " +
+                                      std::to_string(k) + ")");
+      return mk(NK::Ident, x->a[0]->s + "." + names[(size_t)k - 1]);
+    }
+    if (x->k == NK::Call && x->s == "length" && x->a.size() == 1 && x->a[0]->k == NK::Ident && tp.count(x->a[0]->s))
+      return mknum_int((long long)tp[x->a[0]->s]->size());
+    return x;
+  });
+  walk(b, [&](const NodeP& x) {
+    if (x->k == NK::Ident && tp.count(x->s))
+      throw PsError(PS_EARGUMENT, "tuple argument '" + x->s + "' must be used through its components (" + x->s + ".name or " + x->s + "[k])");
+  });
+  return b;
+}
+
 // Nested functions of @parallel_indices kernels (test/ParallelKernel/test_parallel.jl:387-433): a
 // function defined at the top of the body is inlined where it is called. Called as a statement,
 // its body runs in place (a final `return` ends it); called as a value, its body must be a single
@@ -748,6 +853,45 @@ Kernel define_kernel(const std::string& text, const JVal& opts) {
   apply_prefix_kwargs(K.opt, K.def);
   if (K.opt.numbertype != "Float64" && K.opt.numbertype != "Float32")
     throw PsError(PS_EARGUMENT, "numbertype must be Float64 or Float32 (obtained: " + K.opt.numbertype + ")");
+  K.def.body = expand_forall(K.def.body);
+  for (auto& prm : K.def.params) K.def_params.push_back(prm.name);
+  if (K.opt.tuple_params.empty()) {
+    // without the call's structure, parameters used as P.name or P[k][...] are taken to be tuples
+    // of the components the body names (the launch defines the kernel again with the structure
+    // of its arguments)
+    std::set<std::string> pn;
+    for (auto& prm : K.def.params) pn.insert(prm.name);
+    std::map<std::string, std::vector<std::string>> inferred;
+    std::vector<std::string> order;
+    auto add = [&](const std::string& P, const std::string& c) {
+      if (!inferred.count(P)) order.push_back(P);
+      auto& v = inferred[P];
+      if (std::find(v.begin(), v.end(), c) == v.end()) v.push_back(c);
+    };
+    walk(K.def.body, [&](const NodeP& x) {
+      if (x->k == NK::Dot && x->a[0]->k == NK::Ident && pn.count(x->a[0]->s)) add(x->a[0]->s, x->s);
+      if (x->k == NK::Index && x->a[0]->k == NK::Index && x->a[0]->a[0]->k == NK::Ident && pn.count(x->a[0]->a[0]->s) &&
+          x->a[0]->a.size() == 2 && x->a[0]->a[1]->k == NK::Num && x->a[0]->a[1]->is_int)
+        for (long long i = 1; i <= x->a[0]->a[1]->ival; ++i) add(x->a[0]->a[0]->s, std::to_string(i));
+    });
+    for (auto& P : order) K.opt.tuple_params.emplace_back(P, inferred[P]);
+  }
+  if (!K.opt.tuple_params.empty()) {  // one parameter per component, in order
+    std::vector<Param> ps;
+    for (auto& prm : K.def.params) {
+      auto it = std::find_if(K.opt.tuple_params.begin(), K.opt.tuple_params.end(), [&](const auto& t) { return t.first == prm.name; });
+      if (it == K.opt.tuple_params.end()) {
+        ps.push_back(prm);
+        continue;
+      }
+      for (auto& c : it->second) ps.push_back(Param{prm.name + "." + c, prm.annot});
+    }
+    for (auto& t : K.opt.tuple_params)
+      if (std::none_of(K.def.params.begin(), K.def.params.end(), [&](const Param& q) { return q.name == t.first; }))
+        throw PsError(PS_EARGUMENT, "tuple argument '" + t.first + "' is not a parameter of kernel " + K.def.name);
+    K.def.params = ps;
+    K.def.body = expand_tuple_args(K.def.body, K.opt);
+  }
   for (size_t i = 0; i < K.def.params.size(); ++i) K.param_index[K.def.params[i].name] = (int)i;
 
   // remove_return (src/ParallelKernel/shared.jl:200-209): the last statement must return
@@ -899,6 +1043,11 @@ std::string kernel_info_json(const Kernel& K) {
   o << "{\"name\":" << json_str(K.def.name) << ",\"kind\":" << json_str(K.opt.kind)
     << ",\"is_parallel_kernel\":" << (K.opt.kind == "parallel" ? "true" : "false") << ",\"nb_parallel_indices\":" << K.nidx
     << ",\"kernel_language\":" << (K.kernel_language ? "true" : "false") << ",\"shared_arrays\":" << K.shared_arrays.size()
+    << ",\"def_params\":[" << [&] {
+         std::string r;
+         for (size_t i = 0; i < K.def_params.size(); ++i) r += (i ? "," : "") + json_str(K.def_params[i]);
+         return r;
+       }() << "]"
     << ",\"numbertype\":" << json_str(K.opt.numbertype) << ",\"memopt\":" << (K.opt.memopt ? "true" : "false") << ",\"params\":[";
   for (size_t i = 0; i < K.def.params.size(); ++i)
     o << (i ? "," : "") << "{\"name\":" << json_str(K.def.params[i].name) << ",\"annot\":" << json_str(K.def.params[i].annot) << "}";

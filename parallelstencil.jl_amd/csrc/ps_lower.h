@@ -68,6 +68,10 @@ struct Options {
   bool lds = true;                        // memopt: stage x-y neighbours of optvars through LDS
   bool xcd_remap = true;                  // XCD-aware block ordering for memopt kernels
   bool nt_store = false;                  // nontemporal stores for array writes
+  // arguments that are tuples / NamedTuples of arrays (Data.ArrayTuple, Data.NamedArrayTuple):
+  // name -> component names ("1", "2", ... for plain tuples); each component becomes a
+  // parameter of its own, named name.component
+  std::vector<std::pair<std::string, std::vector<std::string>>> tuple_params;
 };
 
 struct Kernel {
@@ -85,6 +89,7 @@ struct Kernel {
   std::map<std::string, int> shared_arrays;
   bool kernel_language = false;
   bool needs_idx64 = false;
+  std::vector<std::string> def_params;  // parameters as written (tuple arguments not yet split)
 };
 
 struct Variant {

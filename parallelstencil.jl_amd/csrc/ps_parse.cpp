@@ -58,6 +58,7 @@ const char* kUniOps[][2] = {
     {"\xE2\x89\xA4", "<="},     // ≤
     {"\xE2\x89\xA5", ">="},     // ≥
     {"\xE2\x89\xA0", "!="},     // ≠
+    {"\xE2\x88\x88", "∈"},      // ∈
 };
 
 int match_uniop(const std::string& t, size_t i, std::string* canon) {
@@ -548,6 +549,18 @@ struct Parser {
       }
       return f;
     }
+    if (is_op("@") && peek().k == TK::Ident && peek().s == "∀") {  // @∀ x ∈ X statement
+      p += 2;
+      NodeP var = atom();
+      if (!(var->k == NK::Ident || var->k == NK::Tuple)) fail("@∀: expected a variable name or a tuple of names");
+      if (is_op("∈") || is_kw("in") || is_op("=")) ++p;
+      else fail("@∀: the first argument must be of the form `x ∈ X`, `x in X` or `x = X`");
+      NodeP coll = expr();
+      NodeP body = statement();
+      auto f = mk(NK::Forall, "", {var, coll, body});
+      f->line = line;
+      return f;
+    }
     if (is_kw("function")) {  // nested function (long form)
       ++p;
       NodeP sig = expr();
@@ -771,6 +784,9 @@ void emit(const NodeP& n, std::string& o) {
       break;
     case NK::FuncDef:
       o += "function " + n->s; emit(n->a[0], o); o += "\n"; emit(n->a[1], o); o += "end";
+      break;
+    case NK::Forall:
+      o += "@∀ "; emit(n->a[0], o); o += " ∈ "; emit(n->a[1], o); o += " "; emit(n->a[2], o);
       break;
   }
 }

@@ -763,6 +763,21 @@ class Geometry(tuple):
     ranges = property(lambda s: s[4])
 
 
+def _tuple_names(v):
+    """Component names of a tuple-like kernel argument (None for arrays and scalars)."""
+    if isinstance(v, dict):
+        return [str(n) for n in v]
+    if isinstance(v, tuple) and hasattr(v, "_fields"):
+        return list(v._fields)
+    if isinstance(v, (tuple, list)):
+        return [str(i + 1) for i in range(len(v))]
+    return None
+
+
+def _tuple_values(v):
+    return list(v.values()) if isinstance(v, dict) else list(v)
+
+
 class Kernel:
     """A kernel defined with @parallel / @parallel_indices; calling it is `@parallel kernel(args...)`."""
 
@@ -778,6 +793,30 @@ class Kernel:
         self.params = [p["name"] for p in self.info["params"]]
         self._annots = [p["annot"] for p in self.info["params"]]
         self.numbertype = self.info["numbertype"]
+        self._expanded = {}
+
+    def _resolve(self, args):
+        """(kernel, arguments) with tuple and NamedTuple arguments (Data.ArrayTuple,
+        Data.NamedArrayTuple) split into their components; each distinct structure of such
+        arguments gets a kernel of its own, whose parameters are the components."""
+        spec, flat = {}, []
+        dp = self.info.get("def_params", self.params)
+        for i, v in enumerate(args):
+            names = _tuple_names(v)
+            if names is None or i >= len(dp):
+                flat.append(v)
+                continue
+            spec[dp[i]] = names
+            flat.extend(_tuple_values(v))
+        if not spec:
+            if len(dp) != len(self.params):  # the body uses tuple arguments the call does not pass
+                raise ArgumentError(f"kernel {self.name} takes tuple arguments ({', '.join(p for p in dp if p + '.' in ' '.join(self.params))})")
+            return self, args
+        key = tuple((p, tuple(n)) for p, n in spec.items())
+        k = self._expanded.get(key)
+        if k is None:
+            k = self._expanded[key] = Kernel(self.text, dict(self.opts, tuple_params=spec))
+        return k, tuple(flat)
 
     def __del__(self):
         try:
@@ -864,52 +903,58 @@ class Kernel:
             stream, async_ = get_stream(len(ov) + 1), True
             ov.append(stream)
         ensure_device()
-        arr = self._args(args)
-        L = self._launch_struct(ranges, nblocks, nthreads, stream, async_, memopt, shmem)
-        check(lib.ps_kernel_launch(self._h, arr, len(args), ctypes.byref(L)))
+        k, args = self._resolve(args)
+        arr = k._args(args)
+        L = k._launch_struct(ranges, nblocks, nthreads, stream, async_, memopt, shmem)
+        check(lib.ps_kernel_launch(k._h, arr, len(args), ctypes.byref(L)))
 
     def launcher(self, *args, ranges=None, nblocks=None, nthreads=None, stream=None, async_=True, memopt=None, shmem=None):
         """A prepared launch of this kernel with these arguments (asynchronous by default):
         calling it launches again without rebuilding the argument block, for launch-bound loops
         and graph capture. It keeps its arguments alive; `set(i, value)` replaces one."""
         ensure_device()
-        return BoundLaunch(self, args, dict(ranges=ranges, nblocks=nblocks, nthreads=nthreads, stream=stream,
+        k, args = self._resolve(args)
+        return BoundLaunch(k, args, dict(ranges=ranges, nblocks=nblocks, nthreads=nthreads, stream=stream,
                                             async_=async_, memopt=memopt, shmem=shmem))
 
     def geometry(self, *args, ranges=None, nblocks=None, nthreads=None, memopt=None, shmem=None):
         """(memopt, nthreads, nblocks, ranges) the launch would use."""
-        arr = self._args(args)
-        L = self._launch_struct(ranges, nblocks, nthreads, None, True, memopt, shmem)
+        k, args = self._resolve(args)
+        arr = k._args(args)
+        L = k._launch_struct(ranges, nblocks, nthreads, None, True, memopt, shmem)
         nt = (ctypes.c_int32 * 3)()
         nb = (ctypes.c_int32 * 3)()
         rg = ((ctypes.c_int64 * 2) * 3)()
         lds = ctypes.c_int64()
-        m = check(lib.ps_kernel_geometry(self._h, arr, len(args), ctypes.byref(L), nt, nb,
+        m = check(lib.ps_kernel_geometry(k._h, arr, len(args), ctypes.byref(L), nt, nb,
                                          ctypes.cast(rg, ctypes.POINTER(ctypes.c_int64)), ctypes.byref(lds)))
         return Geometry(bool(m & 1), bool(m & 2), tuple(nt), tuple(nb), tuple((rg[d][0], rg[d][1]) for d in range(3)))
 
     def source(self, *args, memopt=None, checked=False, idx64=False):
         """The generated HIP source for these argument kinds."""
-        arr = self._args(args)
-        mo = self.info["memopt"] if memopt is None else memopt
+        k, args = self._resolve(args)
+        arr = k._args(args)
+        mo = k.info["memopt"] if memopt is None else memopt
         flags = (1 if mo else 0) | (2 if checked else 0) | (4 if idx64 else 0)
         buf = ctypes.create_string_buffer(1 << 22)
-        check(lib.ps_kernel_source(self._h, arr, len(args), flags, buf, len(buf)))
+        check(lib.ps_kernel_source(k._h, arr, len(args), flags, buf, len(buf)))
         return buf.value.decode()
 
     def precompile(self, *args, memopt=None, checked=False, idx64=False, maxthreads=256):
         """Compiles the code object for these argument kinds into the disk cache (no GPU needed)."""
-        arr = self._args(args)
-        mo = self.info["memopt"] if memopt is None else memopt
+        k, args = self._resolve(args)
+        arr = k._args(args)
+        mo = k.info["memopt"] if memopt is None else memopt
         flags = (1 if mo else 0) | (2 if checked else 0) | (4 if idx64 else 0) | (16 if maxthreads > 256 else 0)
-        check(lib.ps_kernel_precompile(self._h, arr, len(args), flags))
+        check(lib.ps_kernel_precompile(k._h, arr, len(args), flags))
 
     def precompile_launch(self, *args, ranges=None, nblocks=None, nthreads=None, memopt=None, shmem=None):
         """Compiles exactly the variant a launch with these arguments would use (no GPU needed);
         fields may be ArgSpecs."""
-        arr = self._args(args)
-        L = self._launch_struct(ranges, nblocks, nthreads, None, True, memopt, shmem)
-        check(lib.ps_kernel_precompile_launch(self._h, arr, len(args), ctypes.byref(L)))
+        k, args = self._resolve(args)
+        arr = k._args(args)
+        L = k._launch_struct(ranges, nblocks, nthreads, None, True, memopt, shmem)
+        check(lib.ps_kernel_precompile_launch(k._h, arr, len(args), ctypes.byref(L)))
 
 
 _KW_DEF = ("ndims", "N", "inbounds", "padding", "memopt", "optvars", "loopdim", "loopsize", "optranges", "useshmemhalos",
@@ -1079,11 +1124,15 @@ def get_ranges_inner(boundary_width, ranges):
 
 
 def _maxsize_ranges(args):
+    """1:maxsize per dimension over the array arguments, tuples and NamedTuples of arrays included
+    (maxsize, src/ParallelKernel/parallel.jl:773-803)."""
     ms = [1, 1, 1]
     for a in args:
-        if isinstance(a, DeviceArray):
-            for d in range(3):
-                ms[d] = max(ms[d], int(a.f.dims[d]))
+        parts = _tuple_values(a) if _tuple_names(a) is not None else [a]
+        for x in parts:
+            if isinstance(x, DeviceArray):
+                for d in range(3):
+                    ms[d] = max(ms[d], int(x.f.dims[d]))
     return tuple((1, m) for m in ms)
 
 

@@ -279,3 +279,53 @@ def test_write_indices_1d_2d():
     ps.parallel_indices("@parallel_indices (ix,iy) function write_indices!(A)\n    A[ix,iy] = ix + (iy-1)*size(A,1);\n    return\nend")(A2)
     assert np.array_equal(A1.to_host(), np.arange(1, 5, dtype=float))
     assert np.array_equal(A2.to_host(), O.linear_indices((4, 5)).reshape(4, 5, order="F"))
+
+
+# --------------------------------------------------------------- tuples and NamedTuples of arrays, @∀
+def test_named_array_tuple_example():
+    # examples/multimemcopyNamedArrayTuple3D.jl:10-15 (staggered components)
+    import collections
+    n = 17
+    rng = np.random.default_rng(12)
+    k = ps.parallel("""@parallel function copy3D!(V2, V, dV)
+    @all(V2.x) = @all(V.x) + @all(dV.x)
+    @all(V2.y) = @all(V.y) + @all(dV.y)
+    @all(V2.z) = @all(V.z) + @all(dV.z)
+    return
+end""")
+    shapes = {"x": (n + 1, n, n), "y": (n, n + 1, n), "z": (n, n, n + 1)}
+    h = {c: np.asfortranarray(rng.integers(-99, 99, s).astype(float)) for c, s in shapes.items()}
+    d = {c: np.asfortranarray(rng.integers(-99, 99, s).astype(float)) for c, s in shapes.items()}
+    V = {c: _dev(h[c]) for c in shapes}
+    NT = collections.namedtuple("NT", "x y z")
+    dV = NT(*(_dev(d[c]) for c in "xyz"))
+    V2 = {c: ps.zeros(*shapes[c]) for c in shapes}
+    k(V2, V, dV)
+    for c in shapes:
+        assert np.array_equal(V2[c].to_host(), h[c] + d[c])
+    assert ps.get_ranges(V2, V) == (ps.UnitRange(1, n + 1),) * 3  # maxsize over the components
+
+
+def test_forall_over_names_ranges_and_tuples():
+    # kernel_language.jl:197-237 (@∀ forms) with NamedTuple and plain-tuple arguments
+    import collections
+    n = (9, 8, 7)
+    rng = np.random.default_rng(13)
+    k = ps.parallel_indices("""@parallel_indices (ix,iy,iz) function f!(C, A, B, T)
+    @∀ i ∈ (x,z) C.i[ix,iy,iz] = A.i[ix,iy,iz] + B.i[ix,iy,iz]
+    @∀ i ∈ 1:2 T[i][ix,iy,iz] = T[i][ix,iy,iz] * 2.0
+    @∀ (ij,i,j) ∈ ((x,x,y), (z,y,z)) C.ij[ix,iy,iz] = C.ij[ix,iy,iz] + A.i[ix,iy,iz] * B.j[ix,iy,iz]
+    return
+end""")
+    NT = collections.namedtuple("NT", "x y z")
+    ha = NT(*(rng.integers(-9, 9, n).astype(float) for _ in range(3)))
+    hb = NT(*(rng.integers(-9, 9, n).astype(float) for _ in range(3)))
+    ht = [rng.integers(-9, 9, n).astype(float) for _ in range(2)]
+    A, B = NT(*map(_dev, ha)), NT(*map(_dev, hb))
+    C = NT(*(ps.zeros(*n) for _ in range(3)))
+    T = (_dev(ht[0]), _dev(ht[1]))
+    k(C, A, B, T)
+    assert np.array_equal(C.x.to_host(), ha.x + hb.x + ha.x * hb.y)
+    assert np.array_equal(C.y.to_host(), np.zeros(n))
+    assert np.array_equal(C.z.to_host(), ha.z + hb.z + ha.y * hb.z)
+    assert np.array_equal(T[0].to_host(), 2 * ht[0]) and np.array_equal(T[1].to_host(), 2 * ht[1])

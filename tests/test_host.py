@@ -377,3 +377,33 @@ def test_return_statement_rules(text, match):
         ps.parallel_indices(text)
     ps.parallel_indices("@parallel_indices (ix) function f!(A)\n    A[ix] = 1.0\n    nothing\nend")
     ps.parallel_indices("@parallel_indices (ix) function f!(A)\n    A[ix] = 1.0\n    return nothing\nend")
+
+
+# ---------------------------------------------------------------- tuple arguments and @∀
+def test_tuple_arguments_and_forall_lowering():
+    k = ps.parallel_indices("""@parallel_indices (ix) function f!(C, A)
+    @∀ i ∈ (x,y) C.i[ix] = 2.0*A.i[ix]
+    return
+end""")
+    assert k.info["def_params"] == ["C", "A"]
+    s = lambda: {"x": ps.ArgSpec((8,)), "y": ps.ArgSpec((8,))}
+    src = k.source(s(), s())
+    assert re.search(r"a0\[[^\n]*\] = \(0x1p\+1 \* a2\[", src) and re.search(r"a1\[[^\n]*\] = \(0x1p\+1 \* a3\[", src)
+    # the structure of the call decides the components: a third component is a parameter too
+    # (C.x, C.y, C.z are a0..a2, A.x, A.y, A.z a3..a5)
+    s3 = lambda: {"x": ps.ArgSpec((8,)), "y": ps.ArgSpec((8,)), "z": ps.ArgSpec((8,))}
+    assert re.search(r"a1\[[^\n]*\] = \(0x1p\+1 \* a4\[", k.source(s3(), s3()))
+    with pytest.raises(ps.ArgumentError, match="has no component"):
+        k.source({"x": ps.ArgSpec((8,))}, s())
+    with pytest.raises(ps.ArgumentError, match="tuple arguments"):
+        k.source(ps.ArgSpec((8,)), ps.ArgSpec((8,)))
+
+
+@pytest.mark.parametrize("text,match", [
+    ("@∀ i ∈ 1:n A[ix] = 1.0", "literal bounds"),
+    ("@∀ (i,j) ∈ ((x,y), (z,)) A[ix] = 1.0", "tuple of 2"),
+    ("@∀ i ⊂ (x,y) A[ix] = 1.0", "x ∈ X"),
+])
+def test_forall_errors(text, match):
+    with pytest.raises(ps.PSException, match=match):
+        ps.parallel_indices(f"@parallel_indices (ix) function f!(A, n)\n    {text}\n    return\nend")
