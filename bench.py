@@ -399,6 +399,46 @@ def cellarray_study(args):
     print(json.dumps(out), flush=True)
 
 
+def ad_study(args):
+    """Derivative kernels of the diffusion step at n^3 FP64 (@parallel ∇=(T2->T̄2, T->T̄)):
+    the primal (memopt) beside the forward-mode (tangent) and reverse-mode (adjoint) kernels,
+    launched directly. Algorithmic bytes per interior point: primal and tangent read two fields
+    and write one (24 B); the adjoint reads the seed T̄2 and Cp, clears the seed and adds into
+    T̄ (read + write), 40 B. The adjoint's additions into T̄ are atomics: each interior point
+    adds into its 7 stencil neighbours."""
+    import parallelstencil as ps
+    from parallelstencil import workloads as W
+
+    n, K, Wm = args.n, args.steps, args.warmup
+    mod = "bench.ad"
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3, module=mod)
+    k = W.define_diffusion3d(True, mod)
+    hT, hCp, c = W.diffusion_inputs(n)
+    T, T2, Cp = (ps.zeros(n, n, n).copy_from(h) for h in (hT, hT, hCp))
+    del hT, hCp
+    sc = (c["lam"], c["dt"], c["dx"], c["dy"], c["dz"])
+    dup = {"T2": "DuplicatedNoNeed", "T": "DuplicatedNoNeed"}
+    tan, adj = k._derivative("Forward", dup), k._derivative("Reverse", dup)
+    S1, S2 = ps.fill(0.0, n, n, n), ps.fill(1e-3, n, n, n)
+    runs = {}
+    pts = (n - 2) ** 3
+    for name, launch, bpp in (
+            ("primal (memopt)", lambda i: k(T2, T, Cp, *sc, async_=True), 24),
+            ("forward mode (tangent)", lambda i: tan(T2, T, Cp, *sc, S1, S2, async_=True), 24),
+            ("reverse mode (adjoint)", lambda i: adj(T2, T, Cp, *sc, S2, S1, async_=True), 40)):
+        wall, ms = time_kernel(ps, launch, K, Wm)
+        runs[name] = {"ms_per_launch": round(ms, 4), "bytes_per_point": bpp, "gbs": round(bpp * pts / ms / 1e6, 1)}
+        log(f"diffusion3D {n}^3 {name}: {ms:.4f} ms/launch, {runs[name]['gbs']} GB/s algorithmic")
+    head = runs["reverse mode (adjoint)"]
+    out = {"metric": "GB/s algorithmic (reverse-mode derivative of diffusion3D)", "value": head["gbs"], "unit": "GB/s", "n_gpus": 1,
+           "steps": K, "warmup": Wm, "ms_per_step": head["ms_per_launch"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"@parallel ∇=(T2->T̄2, T->T̄) diffusion3D_step! at {n}^3"},
+           "roofline": {"bound": "hbm", "achieved": head["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(head["gbs"] / HBM_PEAK_GBS, 4), "traffic": None}, "runs": runs}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,7 +447,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--adjoint-n", type=int, default=2048)
@@ -438,6 +478,9 @@ def main():
         return
     if args.workload == "cellarray":
         cellarray_study(args)
+        return
+    if args.workload == "ad":
+        ad_study(args)
         return
 
     import parallelstencil as ps

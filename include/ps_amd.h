@@ -178,6 +178,17 @@ int ps_stream_sync(void* stream);
  * loopsize, optvars, optranges, nthreads_max_memopt. */
 int ps_kernel_define(const char* text, const char* opts_json, ps_kernel* out);
 int ps_kernel_free(ps_kernel k);
+/* Derivative kernel of `@parallel ∇=(A->Ā, ...) ad_mode=... ad_annotations=... f!(A, ...)`
+ * (replaces parallel_call_ad, src/ParallelKernel/parallel.jl:265-301, and the Enzyme call
+ * AD.autodiff_deferred! it expands to, src/ParallelKernel/EnzymeExt/autodiff_gpu.jl).
+ * spec_json: {"mode": "Reverse"|"Forward" (Enzyme. prefix allowed; default Reverse),
+ * "annotations": {param: "Const"|"Duplicated"|"DuplicatedNoNeed"}} with unnamed parameters
+ * Const (defaults of src/ParallelKernel/shared.jl:73-76 are applied by the caller). The new
+ * kernel takes the primal's arguments followed by one shadow array per Duplicated* parameter,
+ * in parameter order, and is launched with the primal's ranges. Reverse mode adds into the
+ * shadows of read arrays and consumes (zeroes) the shadows of written ones; forward mode writes
+ * the tangents of written arrays. Writes to DuplicatedNoNeed arrays are dropped. */
+int ps_kernel_derivative(ps_kernel primal, const char* spec_json, ps_kernel* out);
 /* JSON description of a defined kernel: name, params, written/on-the-fly arrays,
  * nb_parallel_indices and the memopt metadata of src/memopt.jl:1179-1236
  * (stencilranges, loopsize(s), loopdim, optvars). */

@@ -50,6 +50,12 @@ that parity is not anchored on a formula merely retyped:
                     shfl_down by the same delta restores the lanes away from the
                     partition edges; shfl with lane = own lane is the identity;
                     the ballot of "lane is even" is 0x5555...; popcounts.
+  linear_jacobian   the 1-D second difference gives the tridiagonal (1, -2, 1)
+                    matrix; a map and its transpose give transposed matrices.
+  complex_step_*    closed forms: d exp = exp, d x^3 = 3x^2, d log(1+x^2) = 2x/(1+x^2),
+                    d sqrt, a product rule; the vjp of a linear map equals J^T y with J
+                    from linear_jacobian; a dropped imaginary part or wrong h scaling
+                    fails them.
 """
 import ctypes
 import os
@@ -539,3 +545,50 @@ def cells_to_data(cells, celldims, B):
 # examples/memcopyCellArray3D.jl:10-27: T2 = T + Ci cell by cell (every value of the cells).
 def copy3d_cells(T, Ci):
     return np.asarray(T) + np.asarray(Ci)
+
+
+# ------------------------------------------------- derivatives (@parallel ∇=...)
+# The reference checks its derivative kernels against Enzyme on a CPU loop restatement of the
+# kernel (test/ParallelKernel/test_parallel.jl:236-283: A[ix] += a*B[ix]*100.65, Ā = 1 and
+# B̄ = 1 + a*100.65). Enzyme is not available here; derivatives are obtained from primal
+# evaluations instead, independently of any derivative formula:
+#   * a linear kernel's Jacobian, column by column, from the oracle's own primal step applied
+#     to unit vectors (exact up to the step's rounding);
+#   * a nonlinear kernel's directional derivative by the complex step (Squire & Trapp 1998):
+#     f'(x) v = Im f(x + i h v) / h, free of cancellation, so exact to rounding for tiny h, for
+#     restatements written with analytic operations and real-part comparisons.
+def linear_jacobian(step, n_in):
+    """Dense Jacobian of a linear map given as step(flat_input) -> flat_output."""
+    cols = []
+    for j in range(n_in):
+        e = np.zeros(n_in)
+        e[j] = 1.0
+        cols.append(np.asarray(step(e), dtype=np.float64).ravel(order="F"))
+    return np.stack(cols, axis=1)
+
+
+COMPLEX_STEP = 1e-200
+
+
+def complex_step_jvp(f, xs, vs, h=COMPLEX_STEP):
+    """Directional derivative of f(*xs) along vs (one tangent per input, None for none)."""
+    zs = [np.asarray(x, dtype=np.complex128) + (0 if v is None else 1j * h * np.asarray(v)) for x, v in zip(xs, vs)]
+    return np.imag(f(*zs)) / h
+
+
+def complex_step_vjp(f, xs, wrt, y, h=COMPLEX_STEP):
+    """Jacobian-transpose product: for each input index in wrt, the array sum_i y_i df_i/dx_j,
+    one column of the Jacobian (one complex evaluation) per element x_j."""
+    y = np.asarray(y, dtype=np.float64)
+    out = []
+    for k in wrt:
+        x = np.asarray(xs[k])
+        g = np.zeros(x.shape, order="F")
+        for j in range(x.size):
+            v = np.zeros(x.shape, order="F")
+            v.flat[j] = 1.0
+            vs = [None] * len(xs)
+            vs[k] = v
+            g.flat[j] = np.sum(y * complex_step_jvp(f, xs, vs, h))
+        out.append(g)
+    return out

@@ -291,6 +291,7 @@ class Emitter {
   EV binary(const std::string& op, EV a, EV b, const NodeP& rhsnode);
   EV cmp(const NodeP& n);
   void stmt(const NodeP& s);
+  void atomic_add(const NodeP& s);
   void block(const NodeP& b);
   void assign(const NodeP& s);
   void emit_memopt_body(int L);
@@ -948,6 +949,34 @@ void Emitter::assign(const NodeP& s) {
   }
 }
 
+// A[i...] += v as one atomic read-modify-write of the element (the shadow accumulations of
+// derivative kernels, where several threads add into one element).
+void Emitter::atomic_add(const NodeP& s) {
+  const NodeP& lhs = s->a.at(0);
+  if (lhs->k != NK::Index || lhs->a[0]->k != NK::Ident) throw PsError(PS_EINTERNAL, "@ps_atomic_add expects an array element");
+  int ai = field_index(lhs->a[0]->s);
+  if (ai < 0 || is_cell(ai)) throw PsError(PS_EINTERNAL, "@ps_atomic_add: '" + lhs->a[0]->s + "' is not a plain array argument");
+  int dt = kinds[ai].dtype;
+  if (dt != PS_F64 && dt != PS_F32)
+    throw PsError(PS_EARGUMENT, "∇: shadow array '" + K.def.params[ai].name + "' must hold Float64 or Float32 values");
+  bool blk = false;
+  EV r = rhs_value(s->a.at(1), &blk);
+  auto idx = norm_indices(lhs, ai);
+  std::string ic[3];
+  std::string off = offset(ai, idx, ic, nullptr);
+  std::string A = an(ai);
+  std::string v = cast(r, dt == PS_F64 ? Ty::F64 : Ty::F32).c;
+  std::string op = "unsafeAtomicAdd(&" + A + "[" + off + "], " + v + ");\n";
+  if (V.checked)
+    o << pad() << "if (ps_chk(" << ic[0] << ", " << ic[1] << ", " << ic[2] << ", " << A << "_nx, " << A << "_ny, " << A << "_nz, ps_err)) " << op;
+  else
+    o << pad() << op;
+  if (blk) {
+    --ind;
+    o << pad() << "}\n";
+  }
+}
+
 // Value of a right-hand side. When it divides by uniform divisors, the fast (reciprocal) form
 // is evaluated first and the IEEE form only if a quotient left the exact range; the emitted
 // code then opens a block that the caller closes after using the value.
@@ -1022,6 +1051,10 @@ void Emitter::stmt(const NodeP& s) {
       o << pad() << "return;\n";
       return;
     default: {
+      if (s->k == NK::Macro && s->s == "ps_atomic_add") {
+        atomic_add(s);
+        return;
+      }
       if (K.kernel_language && s->k == NK::Macro && (s->s == "sync_threads" || s->s == "pk_println" || s->s == "pk_show")) {
         kl_stmt(s);
         return;

@@ -482,11 +482,19 @@ static void collect_access(const NodeP& n, std::set<std::string>& accessed, std:
       std::string p = indexed_param(x, params);
       if (!p.empty()) accessed.insert(p);
     }
-    if (x->k == NK::Assign && x->a[0]->k == NK::Index) {
+    if ((x->k == NK::Assign || (x->k == NK::Macro && x->s == "ps_atomic_add")) && !x->a.empty() && x->a[0]->k == NK::Index) {
       std::string p = indexed_param(x->a[0], params);
       if (!p.empty()) written.insert(p);
     }
   });
+}
+
+void refresh_kernel(Kernel& K) {
+  K.param_index.clear();
+  for (size_t i = 0; i < K.def.params.size(); ++i) K.param_index[K.def.params[i].name] = (int)i;
+  K.accessed.clear();
+  K.written.clear();
+  collect_access(K.body, K.accessed, K.written, K.param_index);
 }
 
 static void memopt_analysis(Kernel& K) {

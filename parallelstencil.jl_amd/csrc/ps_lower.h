@@ -128,6 +128,13 @@ bool rcp_safe64(double b);
 bool rcp_safe32(float b);
 
 Kernel define_kernel(const std::string& text, const JVal& opts);
+// Recomputes the parameter index and the accessed / written sets after the body or the
+// parameters of a kernel changed.
+void refresh_kernel(Kernel& k);
+// Derivative kernel of `@parallel ∇=...` (reverse or forward mode): spec is
+// {"mode": "Reverse"|"Forward", "annotations": {param: "Const"|"Duplicated"|"DuplicatedNoNeed"}};
+// the derivative takes the primal's parameters followed by one shadow per duplicated array.
+Kernel derivative_kernel(const Kernel& k, const JVal& spec);
 Emitted lower_kernel(const Kernel& k, const std::vector<ArgKind>& kinds, const Variant& v);
 std::string kernel_info_json(const Kernel& k);
 

@@ -903,6 +903,22 @@ extern "C" int ps_kernel_define(const char* text, const char* opts_json, ps_kern
   PS_GUARD_END
 }
 
+extern "C" int ps_kernel_derivative(ps_kernel primal, const char* spec_json, ps_kernel* out) {
+  PS_GUARD_BEGIN
+  KernelObj* p = (KernelObj*)primal;
+  if (!p || !out) throw PsError(PS_EARGUMENT, "ps_kernel_derivative: null kernel");
+  auto ko = new KernelObj();
+  try {
+    ko->K = derivative_kernel(p->K, json_parse(spec_json ? spec_json : ""));
+  } catch (...) {
+    delete ko;
+    throw;
+  }
+  *out = (ps_kernel)ko;
+  return 0;
+  PS_GUARD_END
+}
+
 extern "C" int ps_kernel_free(ps_kernel k) {
   KernelObj* ko = (KernelObj*)k;
   if (!ko) return 0;

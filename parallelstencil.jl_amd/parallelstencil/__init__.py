@@ -14,4 +14,6 @@ from .core import (AMDGPU, Float64, Float32, BFloat16, Int32, Int64, Bool, Array
                    get_ranges_inner, get_ranges_outer, get_stream, hide_communication, init_parallel_stencil,
                    is_initialized, ones, overlap, parallel, parallel_async, parallel_indices, rand, reset_parallel_stencil, seed,
                    stream_context, synchronize, trues, zeros)
+from . import ad as AD  # noqa: E402
+from .ad import Forward, Reverse  # noqa: E402
 from . import FieldAllocators, FiniteDifferences1D, FiniteDifferences2D, FiniteDifferences3D  # noqa: E402

@@ -161,6 +161,7 @@ _SIGS = {
     "ps_stream_get": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _P(ctypes.c_void_p)]),
     "ps_stream_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_kernel_define": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P(ctypes.c_void_p)]),
+    "ps_kernel_derivative": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_void_p)]),
     "ps_kernel_free": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_kernel_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
     "ps_kernel_source": (ctypes.c_int, [ctypes.c_void_p, _P(ps_arg), ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),

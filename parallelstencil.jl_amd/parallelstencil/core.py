@@ -25,6 +25,7 @@ import warnings
 import numpy as np
 
 from . import _lib
+from . import ad as _ad
 from ._lib import (PS_ARG_FIELD, PS_ARG_SCALAR, PS_BF16, PS_BOOL, PS_F32, PS_F64, PS_I32, PS_I64, ArgumentError,
                    IncoherentArgumentError, KeywordArgumentError, NotInitializedError, check, lib, ps_arg, ps_devinfo,
                    ps_field, ps_launch)
@@ -700,7 +701,7 @@ class BoundLaunch:
             check(rc)
 def _range3(r):
     """Normalises ranges: (start, stop) pairs (Julia start:stop) per dimension, 1-D/2-D promoted."""
-    if isinstance(r, UnitRange):
+    if isinstance(r, (UnitRange, range)):
         r = (r,)
     if len(r) == 2 and all(isinstance(x, (int, np.integer)) for x in r):
         r = ((r[0], r[1]),)
@@ -794,6 +795,7 @@ class Kernel:
         self._annots = [p["annot"] for p in self.info["params"]]
         self.numbertype = self.info["numbertype"]
         self._expanded = {}
+        self._derivs = {}
 
     def _resolve(self, args):
         """(kernel, arguments) with tuple and NamedTuple arguments (Data.ArrayTuple,
@@ -891,9 +893,31 @@ class Kernel:
             L.shmem = int(shmem)
         return L
 
-    def __call__(self, *args, ranges=None, nblocks=None, nthreads=None, stream=None, async_=False, memopt=None, shmem=None):
+    def _derivative(self, mode, annots):
+        """The derivative kernel (reverse or forward mode) for these annotations {param: kind}."""
+        key = (mode, tuple(sorted(annots.items())))
+        d = self._derivs.get(key)
+        if d is None:
+            d = self._derivs[key] = _DerivativeKernel(self, mode, annots)
+        return d
+
+    def __call__(self, *args, ranges=None, nblocks=None, nthreads=None, stream=None, async_=False, memopt=None, shmem=None, **adkw):
         """Launches the kernel (@parallel); `shmem` is the number of bytes of dynamic LDS that
-        @sharedMem arrays of the kernel use."""
+        @sharedMem arrays of the kernel use. With `∇` (or `grad`), `ad_mode` and
+        `ad_annotations` the derivative of the kernel is launched instead (`parallelstencil.ad`)."""
+        if adkw:
+            bad = set(adkw) - {"∇", "grad", "ad_mode", "ad_annotations"}
+            if bad:
+                raise KeywordArgumentError(f"invalid keyword argument(s) in @parallel <kernelcall>: {', '.join(sorted(bad))}")
+            grad = adkw.get("∇", adkw.get("grad"))
+            if "∇" in adkw and "grad" in adkw:
+                raise KeywordArgumentError("∇ and grad are the same keyword argument")
+            if grad is None:
+                _ad.incoherent()
+            if memopt is not None:
+                raise KeywordArgumentError("memopt cannot be chosen for a derivative launch (derivative kernels are plain kernels)")
+            return _ad.call_with_gradient(self, args, grad, adkw.get("ad_mode"), adkw.get("ad_annotations"), ranges=ranges,
+                                          nblocks=nblocks, nthreads=nthreads, stream=stream, async_=async_, shmem=shmem)
         ov = getattr(_tls, "overlap", None)
         if ov is not None:  # inside an @overlap block: stream i for the i-th call
             if stream is not None:
@@ -955,6 +979,31 @@ class Kernel:
         arr = k._args(args)
         L = k._launch_struct(ranges, nblocks, nthreads, None, True, memopt, shmem)
         check(lib.ps_kernel_precompile_launch(k._h, arr, len(args), ctypes.byref(L)))
+
+
+class _DerivativeKernel(Kernel):
+    """A derivative kernel made by ps_kernel_derivative from a defined kernel."""
+
+    def __init__(self, primal, mode, annots):
+        self._h = ctypes.c_void_p()
+        self.opts = dict(primal.opts, memopt=False)
+        self.text = primal.text
+        self.primal = primal
+        self.mode = mode
+        check(lib.ps_kernel_derivative(primal._h, _ad.spec_json(mode, annots).encode(), ctypes.byref(self._h)))
+        buf = ctypes.create_string_buffer(1 << 20)
+        check(lib.ps_kernel_info(self._h, buf, len(buf)))
+        self.info = json.loads(buf.value.decode())
+        self.info["def_params"] = [p["name"] for p in self.info["params"]]
+        self.name = self.info["name"]
+        self.params = [p["name"] for p in self.info["params"]]
+        self._annots = [p["annot"] for p in self.info["params"]]
+        self.numbertype = self.info["numbertype"]
+        self._expanded = {}
+        self._derivs = {}
+
+    def _resolve(self, args):
+        return self, args
 
 
 _KW_DEF = ("ndims", "N", "inbounds", "padding", "memopt", "optvars", "loopdim", "loopsize", "optranges", "useshmemhalos",

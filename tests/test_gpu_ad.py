@@ -1,0 +1,347 @@
+"""GPU tests of derivative kernels (@parallel ∇=(A->Ā, ...) f!(...), ad_mode, ad_annotations and
+the generic AD.autodiff_deferred! form), test/ParallelKernel/test_parallel.jl:224-283.
+
+Expected derivatives never come from a derivative formula: the reference's closed form for
+A[ix] += a*B[ix]*100.65, a dense Jacobian built from the oracle's primal diffusion step applied
+to unit vectors, and complex-step derivatives of numpy restatements of the test kernels."""
+import numpy as np
+import pytest
+
+import parallelstencil as ps
+from oracle import ps_oracle as O
+from parallelstencil import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+MOD = __name__
+ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3, module=MOD)
+
+F_REF = """
+@parallel_indices (ix) function f!(A, B, a)
+    A[ix] += a * B[ix] * 100.65
+    return
+end"""
+
+
+def _dev(h):
+    d = ps.zeros(*h.shape)
+    return d.copy_from(np.asfortranarray(h))
+
+
+def _close(got, want, rtol=1e-12):
+    scale = max(1.0, float(np.max(np.abs(want)))) if np.size(want) else 1.0
+    np.testing.assert_allclose(got, want, rtol=rtol, atol=rtol * scale)
+
+
+# ---------------------------------------------------------------- reference closed form
+def test_reference_ad_numerical():
+    """test/ParallelKernel/test_parallel.jl:236-283: Ā stays 1, B̄ = 1 + a*100.65."""
+    print("ad: reference closed form N=16", flush=True)
+    N, a = 16, 6.5
+    ps.seed(3)
+    k = ps.core._define("parallel_indices", F_REF, {}, MOD)
+    A, B = ps.rand(N), ps.rand(N)
+    hA, hB = A.to_host(), B.to_host()
+    Ab, Bb = ps.ones(N), ps.ones(N)
+    k(A, B, a, **{"∇": [(A, Ab), (B, Bb)]})
+    assert np.allclose(Ab.to_host(), np.ones(N), rtol=1e-15)
+    assert np.allclose(Bb.to_host(), 1 + a * 100.65, rtol=1e-15)
+    assert np.array_equal(A.to_host(), hA)  # DuplicatedNoNeed: primal update dropped
+    # the generic form, with the scalar promoted to Const
+    Ab, Bb = ps.ones(N), ps.ones(N)
+    ps.AD.autodiff_deferred_(ps.Reverse, k, ps.AD.DuplicatedNoNeed(A, Ab), ps.AD.DuplicatedNoNeed(B, Bb), ps.AD.Const(a))
+    assert np.allclose(Ab.to_host(), 1.0, rtol=1e-15) and np.allclose(Bb.to_host(), 1 + a * 100.65, rtol=1e-15)
+    Ab, Bb = ps.ones(N), ps.ones(N)
+    ps.AD.autodiff_deferred_(ps.Reverse, k, ps.AD.DuplicatedNoNeed(A, Ab), ps.AD.DuplicatedNoNeed(B, Bb), a)
+    assert np.allclose(Bb.to_host(), 1 + a * 100.65, rtol=1e-15)
+    # ∇=B->B̄ only: A is Const, so its update is kept and nothing reaches B̄
+    Bb = ps.ones(N)
+    k(A, B, a, grad=(B, Bb))
+    assert np.array_equal(Bb.to_host(), np.ones(N))
+    assert np.array_equal(A.to_host(), hA + a * hB * 100.65)
+    # Duplicated keeps the primal update as well
+    A.copy_from(hA)
+    Ab, Bb = ps.ones(N), ps.zeros(N)
+    k(A, B, a, grad={"A": Ab, "B": Bb}, ad_annotations={"Duplicated": ("A", "B")})
+    assert np.array_equal(A.to_host(), hA + a * hB * 100.65)
+    assert np.allclose(Bb.to_host(), a * 100.65, rtol=1e-15)
+    with pytest.raises(ps.ArgumentError, match="explicit insertion of the return type activity"):
+        ps.AD.autodiff_deferred_(ps.Reverse, k, ps.AD.Const, ps.AD.DuplicatedNoNeed(A, Ab), ps.AD.DuplicatedNoNeed(B, Bb), a)
+
+
+def test_reference_ad_forward_mode():
+    """ad_mode=Enzyme.Forward: Ȧ = Ȧ + a*100.65*Ḃ."""
+    N, a = 16, 6.5
+    k = ps.core._define("parallel_indices", F_REF, {}, MOD)
+    rng = np.random.default_rng(4)
+    hA, hB, hAd, hBd = (rng.standard_normal(N) for _ in range(4))
+    A, B, Ad, Bd = _dev(hA), _dev(hB), _dev(hAd), _dev(hBd)
+    k(A, B, a, **{"∇": [(A, Ad), (B, Bd)]}, ad_mode="Enzyme.Forward")
+    _close(Ad.to_host(), hAd + a * 100.65 * hBd, rtol=1e-15)
+    assert np.array_equal(Bd.to_host(), hBd) and np.array_equal(A.to_host(), hA)
+
+
+# ---------------------------------------------------------------- headline kernel (linear)
+def _diffusion_case(shape, seed):
+    rng = np.random.default_rng(seed)
+    T = np.asfortranarray(rng.standard_normal(shape))
+    Cp = np.asfortranarray(1.0 + rng.random(shape))
+    return T, Cp, dict(lam=0.7, dt=0.05, dx=0.9, dy=1.1, dz=1.3)
+
+
+@pytest.mark.parametrize("memopt", [False, True])
+def test_diffusion3d_reverse_against_oracle_jacobian(memopt):
+    """The diffusion step is linear in T: its Jacobian is the oracle's step applied to unit
+    vectors. Reverse mode must give T̄ += Jᵀ T̄2 (inner points) and zero T̄2 where T2 is written."""
+    print(f"ad: diffusion3D reverse memopt={memopt}", flush=True)
+    shape = (9, 8, 7)
+    T, Cp, c = _diffusion_case(shape, 11)
+    n = T.size
+
+    def step(x):
+        out = np.zeros(shape, order="F")
+        O.diffusion3d_flux(out, np.asfortranarray(x.reshape(shape, order="F")), Cp, c["lam"], c["dt"], c["dx"], c["dy"], c["dz"])
+        return out
+
+    J = O.linear_jacobian(step, n)
+    inner = np.zeros(shape, dtype=bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    rng = np.random.default_rng(12)
+    y = np.asfortranarray(rng.standard_normal(shape))
+    z = np.asfortranarray(rng.standard_normal(shape))
+    k = W.define_diffusion3d(memopt, MOD)
+    T2d, Td, Cpd = _dev(T), _dev(T), _dev(Cp)
+    T2b, Tb = _dev(y), _dev(z)
+    k(T2d, Td, Cpd, c["lam"], c["dt"], c["dx"], c["dy"], c["dz"], grad=[(T2d, T2b), (Td, Tb)])
+    want_Tb = z.ravel(order="F") + J.T @ np.where(inner, y, 0.0).ravel(order="F")
+    _close(Tb.to_host().ravel(order="F"), want_Tb)
+    assert np.array_equal(T2b.to_host(), np.where(inner, 0.0, y))
+    assert np.array_equal(T2d.to_host(), T)  # T2 DuplicatedNoNeed: not updated
+
+
+def test_diffusion3d_forward_against_oracle_jacobian():
+    shape = (9, 8, 7)
+    T, Cp, c = _diffusion_case(shape, 13)
+
+    def step(x):
+        out = np.zeros(shape, order="F")
+        O.diffusion3d_flux(out, np.asfortranarray(x.reshape(shape, order="F")), Cp, c["lam"], c["dt"], c["dx"], c["dy"], c["dz"])
+        return out
+
+    J = O.linear_jacobian(step, T.size)
+    inner = np.zeros(shape, dtype=bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    rng = np.random.default_rng(14)
+    x = np.asfortranarray(rng.standard_normal(shape))
+    w = np.asfortranarray(rng.standard_normal(shape))
+    k = W.define_diffusion3d(True, MOD)
+    T2d, Td, Cpd, T2t, Tt = _dev(np.zeros(shape)), _dev(T), _dev(Cp), _dev(w), _dev(x)
+    k(T2d, Td, Cpd, c["lam"], c["dt"], c["dx"], c["dy"], c["dz"], grad=[(T2d, T2t), (Td, Tt)], ad_mode=ps.Forward,
+      ad_annotations={"Duplicated": T2d})
+    Jx = (J @ x.ravel(order="F")).reshape(shape, order="F")
+    _close(T2t.to_host(), np.where(inner, Jx, w))
+    # Duplicated T2: the primal update is kept and equals the primal kernel's, bit for bit
+    P = _dev(np.zeros(shape))
+    k(P, Td, Cpd, c["lam"], c["dt"], c["dx"], c["dy"], c["dz"])
+    assert np.array_equal(T2d.to_host(), P.to_host())
+
+
+def test_diffusion3d_adjoint_dot_product_full_size():
+    """At a production size the adjoint identity <J x, y> = <x, Jᵀ y> ties forward and reverse
+    mode together (size-independent property; 1e-12 relative)."""
+    n = 128
+    print(f"ad: diffusion3D dot-product test {n}^3", flush=True)
+    T, Cp, c = W.diffusion_inputs(n)
+    rng = np.random.default_rng(15)
+    x = np.asfortranarray(rng.standard_normal(T.shape))
+    y = np.asfortranarray(rng.standard_normal(T.shape))
+    k = W.define_diffusion3d(True, MOD)
+    T2d, Td, Cpd = _dev(T), _dev(T), _dev(Cp)
+    args = (T2d, Td, Cpd, c["lam"], c["dt"], c["dx"], c["dy"], c["dz"])
+    Jx, Xt = _dev(np.zeros(T.shape)), _dev(x)
+    k(*args, grad=[(T2d, Jx), (Td, Xt)], ad_mode=ps.Forward)
+    Yb, Tb = _dev(y), _dev(np.zeros(T.shape))
+    k(*args, grad=[(T2d, Yb), (Td, Tb)])
+    inner = np.zeros(T.shape, dtype=bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    lhs = float(np.sum(np.where(inner, Jx.to_host(), 0.0) * y))
+    rhs = float(np.sum(x * Tb.to_host()))
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), abs(rhs))
+
+
+# ---------------------------------------------------------------- nonlinear kernel
+NL = """
+@parallel_indices (ix, iy) function nl!(R, U, V, K, s)
+    if ix > 1 && ix < size(U,1) && iy > 1 && iy < size(U,2)
+        u  = U[ix,iy]
+        gx = (U[ix+1,iy] - U[ix-1,iy]) * 0.5
+        k  = exp(-u*u) * K[ix,iy] + sqrt(V[ix,iy]^2 + 1.0)
+        w  = u > 0.3 ? log(1.0 + u*u) : tanh(u)
+        R[ix,iy] = k*gx/(1.0 + abs(u)) + s*max(w, V[ix,iy-1]) + u^3
+    else
+        R[ix,iy] = 2.0*U[ix,iy] - min(V[ix,iy], 0.5)
+    end
+    return
+end"""
+
+
+def nl_ref(U, V, K, s):
+    """numpy restatement of nl! with analytic operations (comparisons on real parts)."""
+    R = np.zeros(U.shape, dtype=np.result_type(U, V, K))
+    u = U[1:-1, 1:-1]
+    gx = (U[2:, 1:-1] - U[:-2, 1:-1]) * 0.5
+    kk = np.exp(-u * u) * K[1:-1, 1:-1] + np.sqrt(V[1:-1, 1:-1] ** 2 + 1.0)
+    w = np.where(u.real > 0.3, np.log(1.0 + u * u), np.tanh(u))
+    au = np.where(u.real >= 0, u, -u)
+    vm = V[1:-1, :-2]
+    mx = np.where(w.real >= vm.real, w, vm)
+    R[...] = 2.0 * U - np.where(V.real <= 0.5, V, 0.5)
+    R[1:-1, 1:-1] = kk * gx / (1.0 + au) + s * mx + u ** 3
+    return R
+
+
+def _nl_inputs(shape, seed):
+    rng = np.random.default_rng(seed)
+    return (np.asfortranarray(rng.uniform(-1, 1, shape)), np.asfortranarray(rng.uniform(-1, 1.5, shape)),
+            np.asfortranarray(rng.uniform(0.5, 2, shape)), 0.8)
+
+
+def _init(suffix, numbertype, ndims):
+    if not ps.is_initialized(MOD + suffix):
+        ps.init_parallel_stencil(ps.AMDGPU, numbertype, ndims, module=MOD + suffix)
+
+
+def _nl_kernel():
+    _init(".nl", ps.Float64, 2)
+    return ps.core._define("parallel_indices", NL, {}, MOD + ".nl")
+
+
+def test_nonlinear_forward_against_complex_step():
+    shape = (12, 10)
+    U, V, K, s = _nl_inputs(shape, 21)
+    rng = np.random.default_rng(22)
+    dU, dV, dK = (np.asfortranarray(rng.standard_normal(shape)) for _ in range(3))
+    k = _nl_kernel()
+    R = _dev(np.zeros(shape))
+    Rt = _dev(np.zeros(shape))
+    Ud, Vd, Kd = _dev(U), _dev(V), _dev(K)
+    Ut, Vt, Kt = _dev(dU), _dev(dV), _dev(dK)
+    k(R, Ud, Vd, Kd, s, grad=[(R, Rt), (Ud, Ut), (Vd, Vt), (Kd, Kt)], ad_mode=ps.Forward)
+    want = O.complex_step_jvp(lambda a, b, c: nl_ref(a, b, c, s), [U, V, K], [dU, dV, dK])
+    _close(Rt.to_host(), want)
+    assert np.array_equal(R.to_host(), np.zeros(shape))
+
+
+def test_nonlinear_reverse_against_complex_step():
+    shape = (12, 10)
+    U, V, K, s = _nl_inputs(shape, 23)
+    rng = np.random.default_rng(24)
+    y = np.asfortranarray(rng.standard_normal(shape))
+    zU, zV, zK = (np.asfortranarray(rng.standard_normal(shape)) for _ in range(3))
+    k = _nl_kernel()
+    R, Ud, Vd, Kd = _dev(np.zeros(shape)), _dev(U), _dev(V), _dev(K)
+    Rb, Ub, Vb, Kb = _dev(y), _dev(zU), _dev(zV), _dev(zK)
+    k(R, Ud, Vd, Kd, s, grad={"R": Rb, "U": Ub, "V": Vb, "K": Kb}, ad_annotations={"Duplicated": "R", "Active": "s"})
+    gU, gV, gK = O.complex_step_vjp(lambda a, b, c: nl_ref(a, b, c, s), [U, V, K], [0, 1, 2], y)
+    _close(Ub.to_host(), zU + gU)
+    _close(Vb.to_host(), zV + gV)
+    _close(Kb.to_host(), zK + gK)
+    assert np.array_equal(Rb.to_host(), np.zeros(shape))
+    # Duplicated R: the kept primal update equals the primal kernel's, bit for bit
+    P = _dev(np.zeros(shape))
+    k(P, Ud, Vd, Kd, s)
+    assert np.array_equal(R.to_host(), P.to_host())
+    _close(P.to_host(), nl_ref(U, V, K, s))
+
+
+# ---------------------------------------------------------------- loops
+ACC = """
+@parallel_indices (ix) function acc!(A, B, C)
+    s = 0.0
+    for k = 1:size(B,2)
+        t = B[ix, k] * C[ix, k]
+        s += t * t
+    end
+    A[ix] = sqrt(s) + (ix > 2 ? B[ix,1] : C[ix,2])
+    return
+end"""
+
+OVERWRITE = """
+@parallel_indices (ix) function ow!(A, B)
+    for k = 1:3
+        A[ix] = B[ix, k] * 2.0
+    end
+    return
+end"""
+
+
+def acc_ref(B, C):
+    t = B * C
+    ix = np.arange(1, B.shape[0] + 1)
+    return np.sqrt(np.sum(t * t, axis=1)) + np.where(ix > 2, B[:, 0], C[:, 1])
+
+
+def test_loop_accumulation_both_modes():
+    n = 37
+    rng = np.random.default_rng(31)
+    B, C = (np.asfortranarray(rng.uniform(0.5, 1.5, (n, 4))) for _ in range(2))
+    _init(".acc", ps.Float64, 1)
+    k = ps.core._define("parallel_indices", ACC, {}, MOD + ".acc")
+    A, Bd, Cd = _dev(np.zeros(n)), _dev(B), _dev(C)
+    y = rng.standard_normal(n)
+    Ab, Bb, Cb = _dev(y), _dev(np.zeros((n, 4))), _dev(np.zeros((n, 4)))
+    k(A, Bd, Cd, grad=[(A, Ab), (Bd, Bb), (Cd, Cb)], ranges=range(1, n + 1))
+    gB, gC = O.complex_step_vjp(acc_ref, [B, C], [0, 1], y)
+    _close(Bb.to_host(), gB)
+    _close(Cb.to_host(), gC)
+    dB, dC = (np.asfortranarray(rng.standard_normal((n, 4))) for _ in range(2))
+    At, Bt, Ct = _dev(np.zeros(n)), _dev(dB), _dev(dC)
+    k(A, Bd, Cd, grad=[(A, At), (Bd, Bt), (Cd, Ct)], ad_mode=ps.Forward, ranges=range(1, n + 1))
+    _close(At.to_host(), O.complex_step_jvp(acc_ref, [B, C], [dB, dC]))
+
+
+def test_loop_overwrite_takes_the_last_iteration():
+    """A[ix] is written in every iteration; only the last write survives, so only B[:, 3] gets
+    an adjoint (the reverse sweep runs the iterations backwards)."""
+    n = 20
+    rng = np.random.default_rng(32)
+    B = np.asfortranarray(rng.standard_normal((n, 3)))
+    y = rng.standard_normal(n)
+    _init(".ow", ps.Float64, 1)
+    k = ps.core._define("parallel_indices", OVERWRITE, {}, MOD + ".ow")
+    A, Bd, Ab, Bb = _dev(np.zeros(n)), _dev(B), _dev(y), _dev(np.zeros((n, 3)))
+    k(A, Bd, grad=[(A, Ab), (Bd, Bb)], ranges=range(1, n + 1))
+    want = np.zeros((n, 3))
+    want[:, 2] = 2.0 * y
+    assert np.array_equal(Bb.to_host(), want)
+    assert np.array_equal(Ab.to_host(), np.zeros(n))
+
+
+def test_float32_stencil_adjoint():
+    """Float32 arrays: atomics on 32-bit shadows; against the float64 Jacobian (1e-5)."""
+    _init(".f32", ps.Float32, 2)
+    k = ps.core._define("parallel", """
+    @parallel function lap!(B, A, c)
+        @inn(B) = c*(@d2_xi(A) + @d2_yi(A))
+        return
+    end""", {}, MOD + ".f32")
+    shape = (17, 13)
+    rng = np.random.default_rng(33)
+    y = np.asfortranarray(rng.standard_normal(shape).astype(np.float32))
+    c = 0.25
+
+    def lap(x):
+        a = x.reshape(shape, order="F")
+        o = np.zeros(shape)
+        o[1:-1, 1:-1] = c * ((a[2:, 1:-1] - 2 * a[1:-1, 1:-1] + a[:-2, 1:-1]) + (a[1:-1, 2:] - 2 * a[1:-1, 1:-1] + a[1:-1, :-2]))
+        return o
+
+    J = O.linear_jacobian(lap, int(np.prod(shape)))
+    B, A = ps.zeros(*shape, eltype=ps.Float32), ps.zeros(*shape, eltype=ps.Float32)
+    Bb = ps.zeros(*shape, eltype=ps.Float32).copy_from(y)
+    Ab = ps.zeros(*shape, eltype=ps.Float32)
+    k(B, A, c, grad=[(B, Bb), (A, Ab)])
+    inner = np.zeros(shape, dtype=bool)
+    inner[1:-1, 1:-1] = True
+    want = (J.T @ np.where(inner, y, 0).astype(np.float64).ravel(order="F")).reshape(shape, order="F")
+    np.testing.assert_allclose(Ab.to_host(), want, rtol=1e-5, atol=1e-5)

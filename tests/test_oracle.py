@@ -427,3 +427,39 @@ def test_cells_to_data_against_numpy_views():
         pad[:N] = flat
         # (B, nb, nc) view of the cells, values moved to the middle axis
         assert np.array_equal(data, pad.reshape((B, nb, nc), order="F").transpose(0, 2, 1))
+
+
+def test_linear_jacobian_closed_form():
+    n = 7
+
+    def d2(x):  # interior second difference, boundary rows zero
+        y = np.zeros_like(x)
+        y[1:-1] = x[2:] - 2 * x[1:-1] + x[:-2]
+        return y
+
+    J = O.linear_jacobian(d2, n)
+    want = np.zeros((n, n))
+    for i in range(1, n - 1):
+        want[i, i - 1:i + 2] = (1, -2, 1)
+    assert np.array_equal(J, want)
+    M = np.arange(12.0).reshape(3, 4)
+    assert np.array_equal(O.linear_jacobian(lambda x: M @ x, 4), M)
+    assert np.array_equal(O.linear_jacobian(lambda x: M.T @ x, 3), M.T)
+
+
+def test_complex_step_closed_forms():
+    x = np.linspace(-1.3, 1.7, 11)
+    one = np.ones_like(x)
+    assert np.allclose(O.complex_step_jvp(np.exp, [x], [one]), np.exp(x), rtol=1e-15, atol=0)
+    assert np.allclose(O.complex_step_jvp(lambda z: z ** 3, [x], [one]), 3 * x ** 2, rtol=1e-15, atol=1e-300)
+    assert np.allclose(O.complex_step_jvp(lambda z: np.log(1 + z * z), [x], [one]), 2 * x / (1 + x * x), rtol=1e-15, atol=1e-300)
+    xp = x + 2.0
+    assert np.allclose(O.complex_step_jvp(np.sqrt, [xp], [one]), 0.5 / np.sqrt(xp), rtol=1e-15, atol=0)
+    v = np.cos(x)
+    got = O.complex_step_jvp(lambda a, b: a * b, [x, xp], [v, None])
+    assert np.allclose(got, v * xp, rtol=1e-15, atol=1e-300)
+    # vjp of a linear map is J^T y
+    M = np.random.default_rng(5).standard_normal((5, 4))
+    y = np.arange(1.0, 6.0)
+    (g,) = O.complex_step_vjp(lambda a: M @ a, [np.zeros(4)], [0], y)
+    assert np.allclose(g, O.linear_jacobian(lambda a: M @ a, 4).T @ y, rtol=1e-14, atol=1e-14)
