@@ -404,8 +404,9 @@ def ad_study(args):
     the primal (memopt) beside the forward-mode (tangent) and reverse-mode (adjoint) kernels,
     launched directly. Algorithmic bytes per interior point: primal and tangent read two fields
     and write one (24 B); the adjoint reads the seed T̄2 and Cp, clears the seed and adds into
-    T̄ (read + write), 40 B. The adjoint's additions into T̄ are atomics: each interior point
-    adds into its 7 stencil neighbours."""
+    T̄ (read + write), 40 B. The adjoint runs in its gather form (each point sums the adjoints
+    of its 7 stencil neighbours, then a second kernel clears the seeds) and in the atomic
+    scatter form (each point adds into its 7 neighbours)."""
     import parallelstencil as ps
     from parallelstencil import workloads as W
 
@@ -422,14 +423,21 @@ def ad_study(args):
     S1, S2 = ps.fill(0.0, n, n, n), ps.fill(1e-3, n, n, n)
     runs = {}
     pts = (n - 2) ** 3
+
+    def scatter(i):
+        adj.use_gather = False
+        adj(T2, T, Cp, *sc, S2, S1, async_=True)
+        adj.use_gather = True
+
     for name, launch, bpp in (
             ("primal (memopt)", lambda i: k(T2, T, Cp, *sc, async_=True), 24),
             ("forward mode (tangent)", lambda i: tan(T2, T, Cp, *sc, S1, S2, async_=True), 24),
-            ("reverse mode (adjoint)", lambda i: adj(T2, T, Cp, *sc, S2, S1, async_=True), 40)):
+            ("reverse mode (adjoint, gather + clear)", lambda i: adj(T2, T, Cp, *sc, S2, S1, async_=True), 40),
+            ("reverse mode (adjoint, atomic scatter)", scatter, 40)):
         wall, ms = time_kernel(ps, launch, K, Wm)
         runs[name] = {"ms_per_launch": round(ms, 4), "bytes_per_point": bpp, "gbs": round(bpp * pts / ms / 1e6, 1)}
         log(f"diffusion3D {n}^3 {name}: {ms:.4f} ms/launch, {runs[name]['gbs']} GB/s algorithmic")
-    head = runs["reverse mode (adjoint)"]
+    head = runs["reverse mode (adjoint, gather + clear)"]
     out = {"metric": "GB/s algorithmic (reverse-mode derivative of diffusion3D)", "value": head["gbs"], "unit": "GB/s", "n_gpus": 1,
            "steps": K, "warmup": Wm, "ms_per_step": head["ms_per_launch"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",

@@ -107,7 +107,8 @@ bool flat_function(const std::string& f) {
 
 class Ad {
  public:
-  Ad(const Kernel& K, bool reverse, const std::map<std::string, std::string>& annot) : K(K), reverse(reverse) {
+  Ad(const Kernel& K, bool reverse, const std::map<std::string, std::string>& annot, const std::string& form = "scatter")
+      : K(K), reverse(reverse), form(form) {
     for (auto& p : K.def.params) {
       auto it = annot.find(p.name);
       if (it == annot.end() || it->second == "Const") continue;
@@ -141,7 +142,7 @@ class Ad {
     find_private_shadows();
 
     Kernel D = K;
-    D.def.name = K.def.name + (reverse ? "_adjoint" : "_tangent");
+    D.def.name = K.def.name + (reverse ? "_adjoint" : "_tangent") + (form == "scatter" ? "" : "_" + form);
     for (auto& p : K.def.params)
       if (shadow.count(p.name)) D.def.params.push_back(Param{shadow[p.name], p.annot});
     D.opt.memopt = false;
@@ -150,7 +151,10 @@ class Ad {
     D.mm = MemoptMeta{};
     D.opt.tuple_params.clear();
     auto body = mk(NK::Block);
-    if (reverse) {
+    if (form != "scatter") {
+      check_gather();
+      body = form == "gather" ? gather_body() : clear_body();
+    } else if (reverse) {
       append(body, rev_block(K.body, {}));
       bool tail = false;
       for (auto& w : K.written)
@@ -168,6 +172,8 @@ class Ad {
  private:
   const Kernel& K;
   bool reverse;
+  std::string form;                           // scatter | gather | clear
+  bool no_temps = false;                      // contributions as pure expressions (gather form)
   std::map<std::string, std::string> shadow;  // duplicated argument -> shadow parameter
   std::set<std::string> noneed;               // DuplicatedNoNeed arguments
   std::set<std::string> locals, active;
@@ -505,7 +511,7 @@ class Ad {
 
   // ---- reverse mode --------------------------------------------------------------------------
   NodeP share(const NodeP& v, const NodeP& out) {
-    if (v->k == NK::Ident || v->k == NK::Num) return v;
+    if (v->k == NK::Ident || v->k == NK::Num || no_temps) return v;
     std::string t = fresh("a");
     out->a.push_back(assign(ident(t), v));
     return ident(t);
@@ -696,6 +702,118 @@ class Ad {
     return mk(NK::If, "", {clone(s->a[0]), primal_tail(s->a[1]), e});
   }
 
+  // ---- reverse mode, gather form (@parallel kernels) ------------------------------------------
+  // Every statement of a @parallel kernel is `if mask(i): W[i + w] = e(i)`. The adjoint that a
+  // read R[i + o] of e receives is g(i) * ∂e/∂R[i + o](i) with g(i) = W̄[i + w]; the point
+  // j = i + o of R̄ thus collects, over the statements and read offsets o, the contribution
+  // evaluated at i = j - o where mask(j - o) holds. Summed by the thread of point j, without
+  // atomics. The seeds W̄ are read by several threads, so they are cleared by a second kernel
+  // (the clear form), which also applies the kept primal updates.
+  static bool offset_of(const NodeP& e, int d, long long& c) {
+    if (e->k == NK::Ident && e->s == kPIdx[d]) {
+      c = 0;
+      return true;
+    }
+    if (e->k == NK::Binary && (e->s == "+" || e->s == "-") && e->a[0]->k == NK::Ident && e->a[0]->s == kPIdx[d] && e->a[1]->k == NK::Num &&
+        e->a[1]->is_int) {
+      c = e->s == "+" ? e->a[1]->ival : -e->a[1]->ival;
+      return true;
+    }
+    return false;
+  }
+  void check_gather() const {
+    auto no = [&](const std::string& why) {
+      throw PsError(PS_EARGUMENT, "∇: the gather form does not apply to kernel " + K.def.name + ": " + why);
+    };
+    if (!reverse) no("forward mode");
+    if (K.opt.kind != "parallel") no("not a @parallel kernel");
+    std::map<std::string, int> writes;
+    for (auto& st : K.body->a) {
+      if (!(st->k == NK::If && st->a[1]->k == NK::Block && st->a[1]->a.size() == 1 && st->a[1]->a[0]->k == NK::Assign &&
+            (st->a.size() < 3 || !st->a[2])))
+        no("unexpected statement form");
+      const NodeP& as = st->a[1]->a[0];
+      if (!as->s.empty() || !is_param_array(as->a[0])) no("unexpected assignment form");
+      writes[as->a[0]->a[0]->s]++;
+      walk(as->a[1], [&](const NodeP& x) {
+        if (!is_param_array(x) || !shadow.count(x->a[0]->s)) return;
+        if (K.written.count(x->a[0]->s)) no("duplicated array '" + x->a[0]->s + "' is read and written");
+        if ((int)x->a.size() - 1 != K.nidx) no("'" + x->a[0]->s + "' has other dimensions than the kernel");
+        long long c;
+        for (int d = 0; d < K.nidx; ++d)
+          if (!offset_of(x->a[d + 1], d, c)) no("'" + x->a[0]->s + "' is read at a non-constant offset");
+      });
+    }
+    for (auto& w : writes)
+      if (shadow.count(w.first) && w.second > 1) no("duplicated array '" + w.first + "' is written by several statements");
+  }
+  // Copy of e at the point i = j - o: parallel index d becomes (index - o_d).
+  NodeP shift(const NodeP& e, const long long o[3]) const {
+    std::function<NodeP(const NodeP&)> f = [&](const NodeP& n) -> NodeP {
+      if (n->k == NK::Ident)
+        for (int d = 0; d < 3; ++d)
+          if (n->s == kPIdx[d] && o[d] != 0) return mk(NK::Binary, o[d] > 0 ? "-" : "+", {ident(n->s), mknum_int(o[d] > 0 ? o[d] : -o[d])});
+      auto c = std::make_shared<Node>(*n);
+      for (auto& x : c->a) x = x ? f(x) : x;
+      return c;
+    };
+    return canonical_indices(f(e));
+  }
+  NodeP gather_body() {
+    no_temps = true;
+    auto body = mk(NK::Block);
+    for (auto& prm : K.def.params) {
+      const std::string& R = prm.name;
+      if (!shadow.count(R) || K.written.count(R) || !K.accessed.count(R)) continue;
+      std::string acc = fresh("acc");
+      body->a.push_back(assign(ident(acc), fnum(0.0)));
+      for (auto& st : K.body->a) {
+        const NodeP& as = st->a[1]->a[0];
+        const std::string& W = as->a[0]->a[0]->s;
+        if (!shadow.count(W)) continue;
+        std::vector<NodeP> ia{ident(shadow.at(W))};
+        for (size_t i = 1; i < as->a[0]->a.size(); ++i) ia.push_back(clone(as->a[0]->a[i]));
+        auto sink = mk(NK::Block);
+        back(as->a[1], mk(NK::Index, "", ia), sink);
+        for (auto& p : pending) {
+          if (p.array != R) continue;
+          long long o[3] = {0, 0, 0};
+          for (int d = 0; d < K.nidx; ++d) offset_of(p.target->a[d + 1], d, o[d]);
+          auto then = mk(NK::Block, "", {assign(ident(acc), add(ident(acc), shift(p.sum, o)))});
+          body->a.push_back(mk(NK::If, "", {shift(st->a[0], o), then, nullptr}));
+        }
+        pending.clear();
+      }
+      NodeP guard;
+      std::vector<NodeP> ja{ident(shadow.at(R))};
+      for (int d = 0; d < K.nidx; ++d) {
+        NodeP c = cmp(ident(kPIdx[d]), "<=", call("size", {ident(R), mknum_int(d + 1)}));
+        guard = guard ? mk(NK::And, "&&", {guard, c}) : c;
+        ja.push_back(ident(kPIdx[d]));
+      }
+      NodeP target = mk(NK::Index, "", ja);
+      body->a.push_back(mk(NK::If, "", {guard, mk(NK::Block, "", {assign(target, add(clone(target), ident(acc)))}), nullptr}));
+    }
+    no_temps = false;
+    return body;
+  }
+  NodeP clear_body() {
+    auto body = mk(NK::Block);
+    for (auto& st : K.body->a) {
+      const NodeP& as = st->a[1]->a[0];
+      const std::string& W = as->a[0]->a[0]->s;
+      auto blk = mk(NK::Block);
+      if (shadow.count(W)) {
+        std::vector<NodeP> ia{ident(shadow.at(W))};
+        for (size_t i = 1; i < as->a[0]->a.size(); ++i) ia.push_back(clone(as->a[0]->a[i]));
+        blk->a.push_back(assign(mk(NK::Index, "", ia), fnum(0.0)));
+      }
+      if (!noneed.count(W)) blk->a.push_back(clone(as));
+      if (!blk->a.empty()) body->a.push_back(mk(NK::If, "", {clone(st->a[0]), blk, nullptr}));
+    }
+    return body;
+  }
+
   // ---- forward mode --------------------------------------------------------------------------
   NodeP tangent(const NodeP& e) {
     if (!dep(e)) return fnum(0.0);
@@ -781,7 +899,11 @@ Kernel derivative_kernel(const Kernel& K, const JVal& spec) {
       annot[kv.first] = v;
     }
   }
-  return Ad(K, mode == "Reverse", annot).run();
+  std::string form = "scatter";
+  if (auto* f = spec.get("form")) form = f->s;
+  if (form != "scatter" && form != "gather" && form != "clear")
+    throw PsError(PS_EARGUMENT, "∇: unknown derivative form '" + form + "' (scatter, gather or clear)");
+  return Ad(K, mode == "Reverse", annot, form).run();
 }
 
 }  // namespace ps

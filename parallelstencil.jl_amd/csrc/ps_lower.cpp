@@ -255,6 +255,8 @@ static NodeP canon_indices(const NodeP& n) {
   });
 }
 
+NodeP canonical_indices(const NodeP& n) { return canon_indices(n); }
+
 static std::vector<std::pair<int, int>> parse_optrange_tuple(const NodeP& t) {
   // (a:b, c:d, e:f)
   std::vector<std::pair<int, int>> r;

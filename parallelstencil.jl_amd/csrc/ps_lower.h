@@ -131,9 +131,14 @@ Kernel define_kernel(const std::string& text, const JVal& opts);
 // Recomputes the parameter index and the accessed / written sets after the body or the
 // parameters of a kernel changed.
 void refresh_kernel(Kernel& k);
+// Rewrites the index expressions of every array access to the canonical form var +/- c.
+NodeP canonical_indices(const NodeP& n);
 // Derivative kernel of `@parallel ∇=...` (reverse or forward mode): spec is
-// {"mode": "Reverse"|"Forward", "annotations": {param: "Const"|"Duplicated"|"DuplicatedNoNeed"}};
-// the derivative takes the primal's parameters followed by one shadow per duplicated array.
+// {"mode": "Reverse"|"Forward", "annotations": {param: "Const"|"Duplicated"|"DuplicatedNoNeed"},
+// "form": "scatter"|"gather"|"clear"}; the derivative takes the primal's parameters followed by
+// one shadow per duplicated array. Reverse mode of @parallel kernels also has a two-kernel form:
+// "gather" (each point sums the adjoints that reach it) then "clear" (seeds and kept primal
+// updates); it raises an ArgumentError where it does not apply.
 Kernel derivative_kernel(const Kernel& k, const JVal& spec);
 Emitted lower_kernel(const Kernel& k, const std::vector<ArgKind>& kinds, const Variant& v);
 std::string kernel_info_json(const Kernel& k);

@@ -193,8 +193,8 @@ def _expand(cls, a, da):
     return [cls(x, y) for x, y in zip(ca, cd)]
 
 
-def spec_json(mode, annots):
-    return json.dumps({"mode": mode, "annotations": annots})
+def spec_json(mode, annots, form="scatter"):
+    return json.dumps({"mode": mode, "annotations": annots, "form": form})
 
 
 def incoherent():

@@ -982,15 +982,21 @@ class Kernel:
 
 
 class _DerivativeKernel(Kernel):
-    """A derivative kernel made by ps_kernel_derivative from a defined kernel."""
+    """A derivative kernel made by ps_kernel_derivative from a defined kernel. Reverse mode of a
+    @parallel kernel also gets the two-kernel gather form (each point sums the adjoints that
+    reach it, then the seeds are cleared): it is used for launches with the default geometry,
+    the atomic (scatter) form otherwise or where the gather form does not apply."""
 
-    def __init__(self, primal, mode, annots):
+    def __init__(self, primal, mode, annots, form="scatter"):
         self._h = ctypes.c_void_p()
         self.opts = dict(primal.opts, memopt=False)
         self.text = primal.text
         self.primal = primal
         self.mode = mode
-        check(lib.ps_kernel_derivative(primal._h, _ad.spec_json(mode, annots).encode(), ctypes.byref(self._h)))
+        self.form = form
+        self._gather = None
+        self.use_gather = True
+        check(lib.ps_kernel_derivative(primal._h, _ad.spec_json(mode, annots, form).encode(), ctypes.byref(self._h)))
         buf = ctypes.create_string_buffer(1 << 20)
         check(lib.ps_kernel_info(self._h, buf, len(buf)))
         self.info = json.loads(buf.value.decode())
@@ -1001,9 +1007,22 @@ class _DerivativeKernel(Kernel):
         self.numbertype = self.info["numbertype"]
         self._expanded = {}
         self._derivs = {}
+        if form == "scatter" and mode == "Reverse" and primal.info["kind"] == "parallel":
+            try:
+                self._gather = (_DerivativeKernel(primal, mode, annots, "gather"), _DerivativeKernel(primal, mode, annots, "clear"))
+            except ArgumentError:
+                self._gather = None
 
     def _resolve(self, args):
         return self, args
+
+    def __call__(self, *args, ranges=None, nblocks=None, nthreads=None, stream=None, async_=False, memopt=None, shmem=None):
+        if self._gather is not None and self.use_gather and ranges is None and nblocks is None and nthreads is None:
+            g, c = self._gather
+            Kernel.__call__(g, *args, stream=stream, async_=True, memopt=memopt, shmem=shmem)
+            return Kernel.__call__(c, *args, stream=stream, async_=async_, memopt=memopt, shmem=shmem)
+        return Kernel.__call__(self, *args, ranges=ranges, nblocks=nblocks, nthreads=nthreads, stream=stream, async_=async_,
+                               memopt=memopt, shmem=shmem)
 
 
 _KW_DEF = ("ndims", "N", "inbounds", "padding", "memopt", "optvars", "loopdim", "loopsize", "optranges", "useshmemhalos",

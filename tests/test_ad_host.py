@@ -183,3 +183,32 @@ def test_keyword_errors():
         k(A, B, 6.5, grad=(A, Ab), ad_mode="Sideways")
     with pytest.raises(ps.KeywordArgumentError, match="invalid keyword"):
         k(A, B, 6.5, gradient=(A, Ab))
+
+
+def test_gather_form_of_parallel_kernels():
+    k = W.define_diffusion3d(True, MOD)
+    d = k._derivative("Reverse", {"T2": "DuplicatedNoNeed", "T": "DuplicatedNoNeed"})
+    assert d._gather is not None
+    g, c = d._gather
+    assert g.name.endswith("_adjoint_gather") and c.name.endswith("_adjoint_clear")
+    args = (S3, S3, S3, 1.0, 0.1, 1.0, 1.0, 1.0, S3, S3)
+    gs, cs = _body(g.source(*args)), _body(c.source(*args))
+    assert "unsafeAtomicAdd" not in gs + cs
+    # the gather kernel reads the seeds and never writes them; the clear kernel zeroes them
+    assert not re.search(r"\ba8\[[^\]]*\] =", gs) and re.search(r"\ba9\[[^\]]*\] =", gs)
+    assert re.search(r"\ba8\[[^\]]*\] = 0x0p\+0;", cs) and not re.search(r"\ba9\[", cs)
+    # 7 neighbour contributions (one per read offset of T), each under the shifted mask
+    assert gs.count("if (") >= 8
+    g.precompile(*args)
+    c.precompile(*args)
+    # no gather form: @parallel_indices kernels, forward mode, arrays read and written
+    assert _k(F_REF)._derivative("Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"})._gather is None
+    assert k._derivative("Forward", {"T2": "DuplicatedNoNeed", "T": "DuplicatedNoNeed"})._gather is None
+    inplace = _k("""
+    @parallel function upd!(A, B)
+        @inn(A) = @inn(A) + @d2(B)
+        return
+    end""")
+    assert inplace._derivative("Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"})._gather is None
+    with pytest.raises(ps.ArgumentError, match="read and written"):
+        ps.core._DerivativeKernel(inplace, "Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"}, "gather")

@@ -89,11 +89,13 @@ def _diffusion_case(shape, seed):
     return T, Cp, dict(lam=0.7, dt=0.05, dx=0.9, dy=1.1, dz=1.3)
 
 
+@pytest.mark.parametrize("form", ["gather", "scatter"])
 @pytest.mark.parametrize("memopt", [False, True])
-def test_diffusion3d_reverse_against_oracle_jacobian(memopt):
+def test_diffusion3d_reverse_against_oracle_jacobian(memopt, form):
     """The diffusion step is linear in T: its Jacobian is the oracle's step applied to unit
-    vectors. Reverse mode must give T̄ += Jᵀ T̄2 (inner points) and zero T̄2 where T2 is written."""
-    print(f"ad: diffusion3D reverse memopt={memopt}", flush=True)
+    vectors. Reverse mode must give T̄ += Jᵀ T̄2 (inner points) and zero T̄2 where T2 is written,
+    in the gather form (two kernels, no atomics) and in the atomic scatter form."""
+    print(f"ad: diffusion3D reverse memopt={memopt} form={form}", flush=True)
     shape = (9, 8, 7)
     T, Cp, c = _diffusion_case(shape, 11)
     n = T.size
@@ -110,9 +112,13 @@ def test_diffusion3d_reverse_against_oracle_jacobian(memopt):
     y = np.asfortranarray(rng.standard_normal(shape))
     z = np.asfortranarray(rng.standard_normal(shape))
     k = W.define_diffusion3d(memopt, MOD)
+    d = k._derivative("Reverse", {"T2": "DuplicatedNoNeed", "T": "DuplicatedNoNeed"})
+    assert d._gather is not None
+    d.use_gather = form == "gather"
     T2d, Td, Cpd = _dev(T), _dev(T), _dev(Cp)
     T2b, Tb = _dev(y), _dev(z)
     k(T2d, Td, Cpd, c["lam"], c["dt"], c["dx"], c["dy"], c["dz"], grad=[(T2d, T2b), (Td, Tb)])
+    d.use_gather = True
     want_Tb = z.ravel(order="F") + J.T @ np.where(inner, y, 0.0).ravel(order="F")
     _close(Tb.to_host().ravel(order="F"), want_Tb)
     assert np.array_equal(T2b.to_host(), np.where(inner, 0.0, y))
@@ -317,8 +323,10 @@ def test_loop_overwrite_takes_the_last_iteration():
     assert np.array_equal(Ab.to_host(), np.zeros(n))
 
 
-def test_float32_stencil_adjoint():
-    """Float32 arrays: atomics on 32-bit shadows; against the float64 Jacobian (1e-5)."""
+@pytest.mark.parametrize("form", ["gather", "scatter"])
+def test_float32_stencil_adjoint(form):
+    """Float32 arrays (atomics on 32-bit shadows in the scatter form); against the float64
+    Jacobian (1e-5)."""
     _init(".f32", ps.Float32, 2)
     k = ps.core._define("parallel", """
     @parallel function lap!(B, A, c)
@@ -340,8 +348,59 @@ def test_float32_stencil_adjoint():
     B, A = ps.zeros(*shape, eltype=ps.Float32), ps.zeros(*shape, eltype=ps.Float32)
     Bb = ps.zeros(*shape, eltype=ps.Float32).copy_from(y)
     Ab = ps.zeros(*shape, eltype=ps.Float32)
+    d = k._derivative("Reverse", {"B": "DuplicatedNoNeed", "A": "DuplicatedNoNeed"})
+    d.use_gather = form == "gather"
     k(B, A, c, grad=[(B, Bb), (A, Ab)])
+    d.use_gather = True
     inner = np.zeros(shape, dtype=bool)
     inner[1:-1, 1:-1] = True
     want = (J.T @ np.where(inner, y, 0).astype(np.float64).ravel(order="F")).reshape(shape, order="F")
     np.testing.assert_allclose(Ab.to_host(), want, rtol=1e-5, atol=1e-5)
+
+
+def test_staggered_grids_gather_scatter_and_oracle():
+    """Arrays of different sizes (the staggered velocities of miniapps/acoustic2D.jl). With the
+    default ranges the gather form runs, with explicit ranges the scatter form; both against the
+    dense Jacobian of the oracle's compute_V! (zero initial velocities). The kernel of the
+    miniapp itself reads Vx and Vy before updating them: it has the scatter form only, and its
+    seeds pass through unchanged (dVx_new/dVx_old = 1)."""
+    _init(".ac", ps.Float64, 2)
+    grad_p = ps.core._define("parallel", """
+    @parallel function grad_P!(Vx, Vy, P, dt, rho, dx, dy)
+        @inn(Vx) = -dt/rho*@d_xi(P)/dx
+        @inn(Vy) = -dt/rho*@d_yi(P)/dy
+        return
+    end""", {}, MOD + ".ac")
+    compute_v = ps.core._define("parallel", W.ACOUSTIC2D_V, {}, MOD + ".ac")
+    nx, ny, sc = 23, 17, (0.1, 2.0, 0.5, 0.7)
+    rng = np.random.default_rng(41)
+    hP = np.asfortranarray(rng.standard_normal((nx, ny)))
+    yx = np.asfortranarray(rng.standard_normal((nx + 1, ny)))
+    yy = np.asfortranarray(rng.standard_normal((nx, ny + 1)))
+
+    def step(p):
+        Vx = np.zeros((nx + 1, ny), order="F")
+        Vy = np.zeros((nx, ny + 1), order="F")
+        O.acoustic2d_v(Vx, Vy, np.asfortranarray(p.reshape((nx, ny), order="F")), *sc)
+        return np.concatenate([Vx.ravel(order="F"), Vy.ravel(order="F")])
+
+    J = O.linear_jacobian(step, nx * ny)
+    inx = np.zeros((nx + 1, ny), dtype=bool)
+    inx[1:-1, 1:-1] = True
+    iny = np.zeros((nx, ny + 1), dtype=bool)
+    iny[1:-1, 1:-1] = True
+    want = J.T @ np.concatenate([np.where(inx, yx, 0).ravel(order="F"), np.where(iny, yy, 0).ravel(order="F")])
+    annots = {"P": "DuplicatedNoNeed", "Vx": "DuplicatedNoNeed", "Vy": "DuplicatedNoNeed"}
+    assert grad_p._derivative("Reverse", annots)._gather is not None
+    assert compute_v._derivative("Reverse", annots)._gather is None
+    for k, explicit in ((grad_p, False), (grad_p, True), (compute_v, False)):
+        print(f"ad: staggered {k.name} explicit_ranges={explicit}", flush=True)
+        Vx, Vy, P = _dev(np.zeros((nx + 1, ny))), _dev(np.zeros((nx, ny + 1))), _dev(hP)
+        Vxb, Vyb, Pb = _dev(yx), _dev(yy), _dev(np.zeros((nx, ny)))
+        kw = {"ranges": ((1, nx + 1), (1, ny + 1))} if explicit else {}
+        k(Vx, Vy, P, *sc, grad=[(P, Pb), (Vx, Vxb), (Vy, Vyb)], **kw)
+        _close(Pb.to_host().ravel(order="F"), want)
+        if k is compute_v:
+            assert np.array_equal(Vxb.to_host(), yx) and np.array_equal(Vyb.to_host(), yy)
+        else:
+            assert np.array_equal(Vxb.to_host(), np.where(inx, 0.0, yx)) and np.array_equal(Vyb.to_host(), np.where(iny, 0.0, yy))
