@@ -51,6 +51,7 @@ enum class NK {
   For,      // s = loop var, a[0] = Range, a[1] = Block
   FuncDef,  // nested function: s = name, a[0] = Tuple of parameter Idents, a[1] = Block
   Forall,   // @∀ x ∈ X statement: a[0] = variable (Ident or Tuple of Idents), a[1] = X, a[2] = statement
+  Splat,    // a[0]...  (the parallel indices I... of @parallel_indices (I...))
 };
 
 struct Node {

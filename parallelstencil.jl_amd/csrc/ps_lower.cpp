@@ -827,6 +827,83 @@ static NodeP inline_nested_functions(const NodeP& body) {
 // Kernel language of @parallel_indices kernels (src/ParallelKernel/kernel_language.jl:9-193, argument
 // checks :264-295). Validates the argument count of each macro call; the lowering happens in the
 // emitter.
+// Tuples of index expressions (the splatted parallel indices I of @parallel_indices (I...)):
+// broadcast arithmetic (.+ .- .* ./ .^, and + - * / with a scalar), unary minus, sum, prod,
+// length, maximum, minimum and literal indexing are evaluated at definition, so that only
+// scalar expressions reach the code generator (src/ParallelKernel/parallel.jl:189-217).
+static NodeP fold_tuples(const NodeP& n) {
+  if (!n) return n;
+  for (auto& c : n->a) c = fold_tuples(c);
+  auto is_tuple = [](const NodeP& x) { return x && x->k == NK::Tuple && std::none_of(x->a.begin(), x->a.end(), [](const NodeP& y) { return y->k == NK::Kw; }); };
+  if (n->k == NK::Binary) {
+    std::string op = n->s;
+    const bool dotted = op.size() == 2 && op[0] == '.';
+    if (dotted) op = op.substr(1);
+    const NodeP &l = n->a[0], &r = n->a[1];
+    if (!is_tuple(l) && !is_tuple(r)) {
+      if (dotted) return mk(NK::Binary, op, {l, r});
+      return n;
+    }
+    if (!dotted && op != "+" && op != "-" && !(is_tuple(l) != is_tuple(r)))
+      throw PsError(PS_EARGUMENT, "tuple arithmetic needs broadcasting operators (.+ .- .* ./): " + to_julia(n));
+    size_t nl = is_tuple(l) ? l->a.size() : 1, nr = is_tuple(r) ? r->a.size() : 1;
+    if (nl != nr && nl != 1 && nr != 1) throw PsError(PS_EARGUMENT, "tuples of different lengths in " + to_julia(n));
+    size_t m = std::max(nl, nr);
+    auto t = mk(NK::Tuple);
+    for (size_t i = 0; i < m; ++i) {
+      NodeP a = is_tuple(l) ? clone(l->a[nl == 1 ? 0 : i]) : clone(l);
+      NodeP b = is_tuple(r) ? clone(r->a[nr == 1 ? 0 : i]) : clone(r);
+      t->a.push_back(mk(NK::Binary, op, {a, b}));
+    }
+    return t;
+  }
+  if (n->k == NK::Unary && is_tuple(n->a[0])) {
+    auto t = mk(NK::Tuple);
+    for (auto& x : n->a[0]->a) t->a.push_back(mk(NK::Unary, n->s, {x}));
+    return t;
+  }
+  if (n->k == NK::Call && n->a.size() == 1 && is_tuple(n->a[0])) {
+    const auto& xs = n->a[0]->a;
+    if (n->s == "length") return mknum_int((long long)xs.size());
+    if ((n->s == "sum" || n->s == "prod") && xs.empty()) return mknum_int(n->s == "sum" ? 0 : 1);
+    if (n->s == "sum" || n->s == "prod") {
+      NodeP acc = xs[0];
+      for (size_t i = 1; i < xs.size(); ++i) acc = mk(NK::Binary, n->s == "sum" ? "+" : "*", {acc, xs[i]});
+      return acc;
+    }
+    if ((n->s == "maximum" || n->s == "minimum") && !xs.empty())
+      return xs.size() == 1 ? xs[0] : mk(NK::Call, n->s == "maximum" ? "max" : "min", xs);
+  }
+  if (n->k == NK::Index && n->a.size() == 2 && is_tuple(n->a[0]) && n->a[1]->k == NK::Num && n->a[1]->is_int) {
+    long long i = n->a[1]->ival;
+    if (i < 1 || i > (long long)n->a[0]->a.size()) throw PsError(PS_EARGUMENT, "tuple index out of range in " + to_julia(n));
+    return n->a[0]->a[i - 1];
+  }
+  return n;
+}
+
+// @parallel_indices (I...): I stands for the kernel's ndims parallel indices; A[I...] and
+// f(I...) spread them, a bare I is their tuple.
+static NodeP expand_splat_indices(const NodeP& n, const std::string& I, int nd) {
+  if (!n) return n;
+  auto idx = [&] {
+    std::vector<NodeP> v;
+    for (int d = 0; d < nd; ++d) v.push_back(mk(NK::Ident, kIdx[d]));
+    return v;
+  };
+  if (n->k == NK::Ident && n->s == I) return mk(NK::Tuple, "", idx());
+  std::vector<NodeP> out;
+  for (auto& c : n->a) {
+    if (c && c->k == NK::Splat && c->a[0]->k == NK::Ident && c->a[0]->s == I && (n->k == NK::Index || n->k == NK::Call || n->k == NK::Tuple)) {
+      for (auto& x : idx()) out.push_back(x);
+    } else {
+      out.push_back(expand_splat_indices(c, I, nd));
+    }
+  }
+  n->a = out;
+  return n;
+}
+
 static void check_kernel_language(Kernel& K, const NodeP& x) {
   const std::string& m = x->s;
   const size_t n = x->a.size();
@@ -991,11 +1068,21 @@ Kernel define_kernel(const std::string& text, const JVal& opts) {
     K.body = out;
   } else if (K.opt.kind == "parallel_indices") {
     if (K.opt.indices.empty()) throw PsError(PS_EARGUMENT, "@parallel_indices: argument 'indices' must be a tuple of indices");
+    std::string splat;
+    if (K.opt.indices.size() == 1 && K.opt.indices[0].size() > 3 && K.opt.indices[0].compare(K.opt.indices[0].size() - 3, 3, "...") == 0) {
+      splat = K.opt.indices[0].substr(0, K.opt.indices[0].size() - 3);
+      if (K.opt.ndims < 1 || K.opt.ndims > 3)
+        throw PsError(PS_EKEYWORD, "@parallel_indices (I...): keyword argument 'ndims' is invalid or missing (valid values are 1, 2 or 3).");
+      K.opt.indices.clear();
+      for (int d = 0; d < K.opt.ndims; ++d) K.opt.indices.push_back(kIdx[d]);
+    }
     if (K.opt.indices.size() > 3) throw PsError(PS_EARGUMENT, "@parallel_indices: at most 3 parallel indices are supported");
     K.nidx = (int)K.opt.indices.size();
     std::map<std::string, NodeP> sub;
     for (int d = 0; d < K.nidx; ++d) sub[K.opt.indices[d]] = mk(NK::Ident, kIdx[d]);
     NodeP body = inline_nested_functions(strip_returns(K.def.body));
+    if (!splat.empty()) body = expand_splat_indices(body, splat, K.nidx);
+    body = fold_tuples(body);
     body = subst_idents(body, sub);
     body = rewrite(body, [&](const NodeP& x) -> NodeP {
       if (x->k == NK::Interp) {

@@ -150,7 +150,13 @@ std::vector<Tok> lex(const std::string& t) {
       i = j + 1;
       continue;
     }
-    static const char* ops2[] = {"&&", "||", "==", "!=", "<=", ">=", "::", "+=", "-=", "*=", "/=", "^=", "->", "<:"};
+    if (t.compare(i, 3, "...") == 0) {
+      push(TK::Op, "...");
+      i += 3;
+      continue;
+    }
+    static const char* ops2[] = {"&&", "||", "==", "!=", "<=", ">=", "::", "+=", "-=", "*=", "/=", "^=", "->", "<:",
+                                 ".+", ".-", ".*", "./", ".^"};
     bool done = false;
     for (auto o : ops2) {
       if (t.compare(i, 2, o) == 0) {
@@ -265,7 +271,7 @@ struct Parser {
   bool bracket_or_paren_ok() const { return true; }
   NodeP add() {
     NodeP l = mul();
-    while ((is_op("+") || is_op("-") || is_op("|")) && !(cur().k == TK::Op && peek().k == TK::NL && false)) {
+    while ((is_op("+") || is_op("-") || is_op("|") || is_op(".+") || is_op(".-")) && !(cur().k == TK::Op && peek().k == TK::NL && false)) {
       std::string o = cur().s;
       ++p;
       skip_nl_only();
@@ -275,7 +281,7 @@ struct Parser {
   }
   NodeP mul() {
     NodeP l = unary();
-    while (is_op("*") || is_op("/") || is_op("÷") || is_op("%") || is_op("&")) {
+    while (is_op("*") || is_op("/") || is_op("÷") || is_op("%") || is_op("&") || is_op(".*") || is_op("./")) {
       std::string o = cur().s;
       ++p;
       skip_nl_only();
@@ -294,7 +300,8 @@ struct Parser {
   }
   NodeP power() {
     NodeP b = postfix();
-    if (is_op("^")) {
+    if (is_op("^") || is_op(".^")) {
+      std::string op = cur().s;
       ++p;
       NodeP e;
       if (is_op("-") || is_op("+")) {
@@ -304,7 +311,7 @@ struct Parser {
       } else {
         e = power();
       }
-      return mk(NK::Binary, "^", {b, e});
+      return mk(NK::Binary, op, {b, e});
     }
     return b;
   }
@@ -362,6 +369,9 @@ struct Parser {
         ++p;
         NodeP ty = postfix_type();
         e = mk(NK::Typed, "", {e, ty});
+      } else if (is_op("...")) {
+        ++p;
+        e = mk(NK::Splat, "...", {e});
       } else {
         break;
       }
@@ -649,8 +659,9 @@ struct Parser {
       kd.kind = cur().s == "parallel_indices" ? "parallel_indices" : "parallel";
       ++p;
       if (kd.kind == "parallel_indices") {
-        NodeP idx = atom();
+        NodeP idx = postfix();
         if (idx->k == NK::Ident) kd.indices.push_back(idx->s);
+        else if (idx->k == NK::Splat && idx->a[0]->k == NK::Ident) kd.indices.push_back(idx->a[0]->s + "...");
         else if (idx->k == NK::Tuple) {
           for (auto& x : idx->a) {
             if (x->k != NK::Ident) fail("parallel indices must be identifiers");
@@ -787,6 +798,9 @@ void emit(const NodeP& n, std::string& o) {
       break;
     case NK::Forall:
       o += "@∀ "; emit(n->a[0], o); o += " ∈ "; emit(n->a[1], o); o += " "; emit(n->a[2], o);
+      break;
+    case NK::Splat:
+      emit(n->a[0], o); o += "...";
       break;
   }
 }

@@ -13,7 +13,8 @@ from .core import (AMDGPU, Float64, Float32, BFloat16, Int32, Int64, Bool, Array
                    device_info, ensure_device, f32_to_bf16, falses, fill, fill_, get_priority_stream, get_ranges,
                    get_ranges_inner, get_ranges_outer, get_stream, hide_communication, init_parallel_stencil,
                    is_initialized, ones, overlap, parallel, parallel_async, parallel_indices, rand, reset_parallel_stencil, seed,
-                   stream_context, synchronize, trues, zeros)
+                   stream_context, synchronize, trues, zeros, select_hardware, current_hardware, get_hardware, handle,
+                   KernelMethods)
 from . import ad as AD  # noqa: E402
 from .ad import Forward, Reverse  # noqa: E402
 from . import FieldAllocators, FiniteDifferences1D, FiniteDifferences2D, FiniteDifferences3D  # noqa: E402

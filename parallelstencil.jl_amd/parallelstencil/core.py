@@ -19,6 +19,7 @@ import contextlib
 import ctypes
 import inspect
 import json
+import re
 import threading
 import warnings
 
@@ -87,15 +88,17 @@ def _caller(depth):
     return f.f_globals.get("__name__", "__main__")
 
 
-def init_parallel_stencil(package=AMDGPU, numbertype=Float64, ndims=3, *, inbounds=False, padding=False, memopt=False,
+def init_parallel_stencil(package=AMDGPU, numbertype=None, ndims=None, *, inbounds=False, padding=False, memopt=False,
                           nonconst_metadata=False, module=None):
-    """@init_parallel_stencil(package, numbertype, ndims; inbounds, padding, memopt, nonconst_metadata)."""
+    """@init_parallel_stencil(package, numbertype, ndims; inbounds, padding, memopt, nonconst_metadata).
+    numbertype and ndims may be omitted (src/ParallelKernel/init_parallel_kernel.jl:114): kernels
+    then give `ndims` themselves and allocations their `eltype`."""
     mod = module or _caller(1)
     if package not in ("AMDGPU", "HIP"):
         raise ArgumentError(f"unsupported package {package}: this backend runs on AMD MI355X GPUs (package AMDGPU)")
-    if numbertype not in ("Float64", "Float32"):
+    if numbertype is not None and numbertype not in ("Float64", "Float32"):
         raise ArgumentError(f"unsupported numbertype {numbertype} (Float64 or Float32)")
-    if ndims not in (1, 2, 3):
+    if ndims is not None and ndims not in (1, 2, 3):
         raise ArgumentError(f"ndims must be evaluatable at parse time (e.g. literal or constant) and has to be one of the following Integers: 1, 2, 3 (obtained: {ndims}).")
     for name, val in (("memopt", memopt), ("nonconst_metadata", nonconst_metadata), ("inbounds", inbounds), ("padding", padding)):
         if not isinstance(val, bool):
@@ -124,7 +127,50 @@ def is_initialized(module=None):
 
 
 def reset_parallel_stencil(module=None):
-    _configs.pop(module or _caller(1), None)
+    mod = module or _caller(1)
+    _configs.pop(mod, None)
+    _hardware.pop(mod, None)
+
+
+# Runtime hardware selection (src/ParallelKernel/select_hardware.jl:28-105): the AMDGPU package
+# has one hardware symbol, gpu_amd; the selection exists for code written against backends with
+# several (KernelAbstractions).
+SUPPORTED_HARDWARE = {"AMDGPU": ("gpu_amd",)}
+_hardware = {}
+
+
+def _hw_name(h):
+    return str(h).lstrip(":")
+
+
+def select_hardware(hardware, module=None):
+    """@select_hardware(hardware): only :gpu_amd is valid for this backend."""
+    mod = module or _caller(1)
+    if mod not in _configs:
+        raise ArgumentError(f"ParallelKernel has not been initialized in module {mod} (missing @init_parallel_kernel call).")
+    supported = SUPPORTED_HARDWARE[_configs[mod].package]
+    if _hw_name(hardware) not in supported:
+        raise ArgumentError(f"unsupported hardware selection for package {_configs[mod].package} (obtained: {hardware}). Supported hardware "
+                            f"symbols are: {', '.join(supported)}.")
+    _hardware[mod] = _hw_name(hardware)
+
+
+def current_hardware(module=None):
+    """@current_hardware(): the selected hardware symbol (gpu_amd)."""
+    mod = module or _caller(1)
+    _config(mod)
+    return _hardware.get(mod, SUPPORTED_HARDWARE[_configs[mod].package][0])
+
+
+def get_hardware(module=None):
+    """@get_hardware(): the hardware of the module (the same as current_hardware here)."""
+    return current_hardware(module or _caller(1))
+
+
+def handle(hardware, package=AMDGPU):
+    """Backend handle of a hardware symbol: defined for KernelAbstractions only (an ArgumentError
+    for every other package, as in the reference)."""
+    raise ArgumentError(f"handle: hardware handles exist for the KernelAbstractions package only (obtained: {hardware}, package {package}).")
 
 
 def _config(mod):
@@ -321,7 +367,10 @@ class ArgSpec:
 
 
 def _numbertype_of(mod):
-    return _configs[mod].numbertype if mod in _configs else Float64
+    nt = _configs[mod].numbertype if mod in _configs else Float64
+    if nt is None:
+        raise ArgumentError("the keyword argument 'eltype' is mandatory in @zeros, @ones, @rand and @fill when no default is set.")
+    return nt
 
 
 # ---------------------------------------------------------------------------
@@ -450,9 +499,9 @@ def _cell_spec(mod, eltype, celldims, celltype):
             raise ArgumentError(f"the parametric cell type {celltype.name} needs an element type: {celltype.name}[eltype]")
         return celltype.eltype or _numbertype_of(mod), celltype.dims, celltype
     if celldims is None:
-        return eltype or _numbertype_of(mod), None, None
+        return eltype if eltype is not None else _numbertype_of(mod), None, None
     cd = tuple(int(d) for d in (celldims if isinstance(celldims, (tuple, list)) else (celldims,)))
-    return eltype or _numbertype_of(mod), cd, None
+    return eltype if eltype is not None else _numbertype_of(mod), cd, None
 
 
 def _alloc(dims, eltype, celldims=None, celltype=None, blocklength=None):
@@ -1030,12 +1079,123 @@ _KW_DEF = ("ndims", "N", "inbounds", "padding", "memopt", "optvars", "loopdim", 
            "nthreads_x_max_memopt")
 
 
+_PREFIX_RE = re.compile(r"^(\s*@parallel(?:_indices)?\b\s*(?:\([^()]*\)|\w+\.\.\.)?)((?:\s*\w+\s*=\s*(?:\([^()]*\)|[^\s(]+))*)", re.S)
+_KW_PAIR_RE = re.compile(r"(\w+)\s*=\s*(\([^()]*\)|[^\s(]+)")
+
+
+def _prefix_ndims(text, kw):
+    """Moves `ndims=...` and `N=...` of the text prefix (`@parallel ndims=(1,3) N=ndims.+1 f(...)`)
+    into the keyword arguments: both select kernel methods, which the definition expands here."""
+    m = _PREFIX_RE.match(text)
+    if not m or not m.group(2).strip():
+        return text, kw
+    kept, kw = [], dict(kw)
+    for k, v in _KW_PAIR_RE.findall(m.group(2)):
+        if k in ("ndims", "N"):
+            if k in kw:
+                raise KeywordArgumentError(f"keyword argument '{k}' is given twice")
+            kw[k] = _literal(v) if k == "ndims" else v
+        else:
+            kept.append(f"{k}={v}")
+    return m.group(1) + (" " + " ".join(kept) if kept else "") + text[m.end():], kw
+
+
+def _literal(v):
+    v = v.strip()
+    if re.fullmatch(r"\d+", v):
+        return int(v)
+    if re.fullmatch(r"\(\s*\d+\s*(,\s*\d+\s*)*,?\s*\)", v):
+        return tuple(int(x) for x in re.findall(r"\d+", v))
+    raise KeywordArgumentError(f"keyword argument 'ndims' must be an integer or a tuple of integers (obtained: {v}).")
+
+
+def _eval_N(N, ndims, macro):
+    """The value(s) of N (an integer, a tuple, or an expression of ndims such as ndims+2 or
+    ndims.+1, broadcast over a tuple of ndims), src/ParallelKernel/parallel.jl:334-359."""
+    if isinstance(N, (int, np.integer)) and not isinstance(N, bool):
+        vals = int(N)
+    elif isinstance(N, (tuple, list)):
+        vals = tuple(int(x) for x in N)
+    elif isinstance(N, str):
+        expr = N.strip()
+        if "ndims" not in expr:
+            ndims = 0 if not isinstance(ndims, tuple) else ndims
+        if not re.fullmatch(r"[0-9ndims+\-*/(). ,]*", expr):
+            raise KeywordArgumentError(f"{macro}: keyword argument 'N' must be an integer expression of ndims (obtained: {N}).")
+        broadcast = isinstance(ndims, tuple)
+        if broadcast and expr != "ndims" and re.search(r"(?<!\.)[+\-*/]", expr.replace("ndims", "")):
+            raise KeywordArgumentError(f"{macro}: keyword argument 'N' must be a tuple of integers of the same length as 'ndims' when "
+                                       f"'ndims' is a tuple (use broadcasting operators, e.g. ndims.+1; obtained: N={N}, ndims={ndims}).")
+        plain = expr.replace(".+", "+").replace(".-", "-").replace(".*", "*").replace("./", "//")
+        ev = lambda d: eval(plain, {"__builtins__": {}}, {"ndims": d})  # noqa: S307 (digits, ndims and operators only)
+        vals = tuple(ev(d) for d in ndims) if broadcast and "ndims" in expr else ev(ndims)
+    else:
+        raise KeywordArgumentError(f"{macro}: keyword argument 'N' must be an integer (or a tuple if 'ndims' is a tuple; obtained: {N}).")
+    return vals
+
+
+def _subst_N(text, n, macro):
+    """N substituted in the type parameters of the kernel signature (`Data.Array{T,N}`)."""
+    if re.search(r"\bwhere\b[^=\n]*\bN\b", text):
+        raise IncoherentArgumentError(f"{macro}: 'N' must not appear in the where clause of the kernel signature when the keyword argument "
+                                      "'N' is used.")
+    return re.sub(r"\{[^{}]*\}", lambda m: re.sub(r"\bN\b", str(n), m.group(0)), text)
+
+
+class KernelMethods:
+    """A kernel defined for several ndims (`@parallel ndims=(1,3) N=ndims.+1 f(...)`): one
+    method per ndims, chosen at the call by the number of dimensions of the array arguments
+    (the value of N of each method)."""
+
+    def __init__(self, methods):
+        self.methods = dict(methods)  # ndims -> (N, Kernel)
+        self.name = next(iter(self.methods.values()))[1].name
+
+    def __repr__(self):
+        return f"KernelMethods({self.name}, ndims={tuple(self.methods)})"
+
+    def method(self, *args):
+        nds = {len(a.shape) for a in args if isinstance(a, (DeviceArray, ArgSpec))}
+        for nd, (n, k) in self.methods.items():
+            if nds and max(nds) == n:
+                return k
+        raise ArgumentError(f"MethodError: no method of kernel {self.name} for arrays of {max(nds) if nds else 0} dimensions "
+                            f"(methods for N = {', '.join(str(n) for n, _ in self.methods.values())})")
+
+    def __call__(self, *args, **kw):
+        return self.method(*args)(*args, **kw)
+
+
 def _define(kind, text, kw, mod, indices=None):
     bad = set(kw) - set(_KW_DEF)
     if bad:
         raise KeywordArgumentError(f"invalid keyword argument(s) in @{kind} <kernel>: {', '.join(sorted(bad))}")
     cfg = _config(mod)
-    opts = {"kind": kind, "ndims": kw.get("ndims", cfg.ndims), "numbertype": cfg.numbertype,
+    macro = f"@{kind}"
+    text, kw = _prefix_ndims(text, kw)
+    nd = kw.get("ndims", cfg.ndims)
+    if isinstance(nd, (tuple, list)):
+        nd = tuple(nd)
+        if not all(isinstance(d, (int, np.integer)) for d in nd):
+            raise KeywordArgumentError(f"{macro}: keyword argument 'ndims' must be an integer or a tuple of integers (obtained: {nd}).")
+        if "N" not in kw:
+            raise KeywordArgumentError(f"{macro}: keyword argument 'N' is mandatory when 'ndims' is a tuple ('N' must also be present as type "
+                                       "parameter in the function signature enabling to dispatch on). ")
+        Ns = _eval_N(kw["N"], nd, macro)
+        if not isinstance(Ns, tuple) or len(Ns) != len(nd):
+            raise KeywordArgumentError(f"{macro}: keyword argument 'N' must be a tuple of integers of the same length as 'ndims' when 'ndims' "
+                                       f"is a tuple (obtained: N={Ns}, ndims={nd}).")
+        return KernelMethods({d: (n, _define(kind, text, dict(kw, ndims=d, N=n), mod, indices)) for d, n in zip(nd, Ns)})
+    if "N" in kw:
+        if nd is None or not 1 <= nd <= 3:
+            raise KeywordArgumentError(f"{macro}: keyword argument 'ndims' is invalid or missing (valid values are 1, 2 or 3; 'ndims' an be set "
+                                       "globally in @init_parallel_stencil and overwritten per kernel if needed).")
+        n = _eval_N(kw["N"], nd, macro)
+        if not isinstance(n, int):
+            raise KeywordArgumentError(f"{macro}: keyword argument 'N' must be an integer (or a tuple if 'ndims' is a tuple; obtained: {n}).")
+        text = _subst_N(text, n, macro)
+        kw = {k: v for k, v in kw.items() if k != "N"}
+    opts = {"kind": kind, "ndims": nd if nd is not None else 0, "numbertype": cfg.numbertype or "Float64",
             "inbounds": kw.get("inbounds", cfg.inbounds), "padding": kw.get("padding", cfg.padding),
             "memopt": kw.get("memopt", cfg.memopt)}
     if indices is not None:
@@ -1054,7 +1214,7 @@ def _define(kind, text, kw, mod, indices=None):
 def _split_call(args):
     """(ranges, nblocks, nthreads, kernel, kernel_args) from the positional forms of @parallel."""
     for i, a in enumerate(args):
-        if isinstance(a, Kernel):
+        if isinstance(a, (Kernel, KernelMethods)):
             pre, k, rest = args[:i], a, args[i + 1:]
             break
     else:
