@@ -145,7 +145,7 @@ class Ad {
     D.def.name = K.def.name + (reverse ? "_adjoint" : "_tangent") + (form == "scatter" ? "" : "_" + form);
     for (auto& p : K.def.params)
       if (shadow.count(p.name)) D.def.params.push_back(Param{shadow[p.name], p.annot});
-    D.opt.memopt = false;
+    D.opt.memopt = false;  // set again below for the forms that stream
     D.opt.optvars.clear();
     D.opt.optranges.clear();
     D.mm = MemoptMeta{};
@@ -166,6 +166,12 @@ class Ad {
     }
     D.body = body;
     refresh_kernel(D);
+    // the gather form and the tangent read the seeds, tangents and primal arrays at constant
+    // offsets of their own point: a memopt primal gives memopt (z-streaming) derivative kernels
+    if ((form == "gather" || !reverse) && K.opt.memopt && K.nidx >= 2) {
+      D.opt.memopt = true;
+      analyse_memopt(D);
+    }
     return D;
   }
 

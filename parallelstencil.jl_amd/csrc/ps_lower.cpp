@@ -605,6 +605,11 @@ static void memopt_analysis(Kernel& K) {
 // @∀ x ∈ X statement (src/ParallelKernel/kernel_language.jl:197-237, 543-566): the statement once per
 // element of X, with x replaced by the element: as a name (also as a field name, `C.x`), a
 // number (X = a:b with literal bounds) or, for a tuple of variables, element-wise.
+void analyse_memopt(Kernel& K) {
+  K.mm = MemoptMeta{};
+  memopt_analysis(K);
+}
+
 static NodeP expand_forall(const NodeP& n);
 static void expand_forall_into(const NodeP& st, NodeP& out) {
   if (st->k != NK::Forall) {

@@ -131,6 +131,8 @@ Kernel define_kernel(const std::string& text, const JVal& opts);
 // Recomputes the parameter index and the accessed / written sets after the body or the
 // parameters of a kernel changed.
 void refresh_kernel(Kernel& k);
+// memopt metadata (z-streaming plan) of a kernel whose body is final.
+void analyse_memopt(Kernel& k);
 // Rewrites the index expressions of every array access to the canonical form var +/- c.
 NodeP canonical_indices(const NodeP& n);
 // Derivative kernel of `@parallel ∇=...` (reverse or forward mode): spec is

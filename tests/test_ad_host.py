@@ -212,3 +212,17 @@ def test_gather_form_of_parallel_kernels():
     assert inplace._derivative("Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"})._gather is None
     with pytest.raises(ps.ArgumentError, match="read and written"):
         ps.core._DerivativeKernel(inplace, "Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"}, "gather")
+
+
+def test_memopt_primal_gives_streaming_derivatives():
+    """A memopt primal gets a memopt gather kernel and a memopt tangent (both read their inputs at
+    constant offsets of their own point); the atomic scatter form stays a plain kernel."""
+    for memopt in (True, False):
+        k = W.define_diffusion3d(memopt, MOD)
+        r = k._derivative("Reverse", {"T2": "DuplicatedNoNeed", "T": "DuplicatedNoNeed"})
+        f = k._derivative("Forward", {"T2": "DuplicatedNoNeed", "T": "DuplicatedNoNeed"})
+        assert r._gather[0].info["memopt"] is memopt and f.info["memopt"] is memopt
+        assert r.info["memopt"] is False and r._gather[1].info["memopt"] is False
+    g = W.define_diffusion3d(True, MOD)._derivative("Reverse", {"T2": "DuplicatedNoNeed", "T": "DuplicatedNoNeed"})._gather[0]
+    assert set(g.info["optvars"]) == {"T2#shadow", "Cp"}
+    assert g.info["stencilranges"]["T2#shadow"] == [[-1, 1], [-1, 1], [-1, 1]]
