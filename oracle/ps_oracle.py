@@ -52,6 +52,10 @@ that parity is not anchored on a formula merely retyped:
                     the ballot of "lane is even" is 0x5555...; popcounts.
   linear_jacobian   the 1-D second difference gives the tridiagonal (1, -2, 1)
                     matrix; a map and its transpose give transposed matrices.
+  update_halo_periodic  a periodic function sampled on the grid, with the halo cells set to
+                    NaN, is restored exactly at every cell (its periodic images), for overlap 2
+                    / halo width 1, overlap 4 / halo width 2, staggered arrays (overlap 3)
+                    and all three dimensions; non-periodic dimensions are left alone.
   complex_step_*    closed forms: d exp = exp, d x^3 = 3x^2, d log(1+x^2) = 2x/(1+x^2),
                     d sqrt, a product rule; the vjp of a linear map equals J^T y with J
                     from linear_jacobian; a dropped imaginary part or wrong h scaling
@@ -592,3 +596,31 @@ def complex_step_vjp(f, xs, wrt, y, h=COMPLEX_STEP):
             g.flat[j] = np.sum(y * complex_step_jvp(f, xs, vs, h))
         out.append(g)
     return out
+
+
+# ------------------------------------------------- one-process halo update (ImplicitGlobalGrid)
+# update_halo! of ImplicitGlobalGrid (a dependency of the reference's multi-XPU examples, e.g.
+# examples/diffusion3D_multigpucpu_hidecomm_novis.jl:59-62; not vendored) on one process: in a
+# periodic dimension the process is its own neighbour, so the halo cells 1:hw take the cells
+# s-ol+1 : s-ol+hw and s-hw+1 : s take ol-hw+1 : ol, dimension after dimension; ol is the
+# array's overlap (overlap + size - nx, larger for staggered arrays).
+def update_halo_periodic(A, nxyz, overlaps=(2, 2, 2), halowidths=(1, 1, 1), periods=(1, 1, 1)):
+    A = np.array(A, order="F", copy=True)
+    shape = A.shape + (1,) * (3 - A.ndim)
+    B = A.reshape(shape, order="F")
+    for d in range(3):
+        if not periods[d] or nxyz[d] <= 1 or shape[d] <= 1:
+            continue
+        s, ol, hw = shape[d], overlaps[d] + shape[d] - nxyz[d], halowidths[d]
+        for i in range(hw):
+            src_l, dst_l = s - ol + i, i          # 0-based: cell s-ol+1+i -> cell 1+i
+            src_r, dst_r = ol - hw + i, s - hw + i  # cell ol-hw+1+i -> cell s-hw+1+i
+            sl = [slice(None)] * 3
+            sl[d] = dst_l
+            sr = [slice(None)] * 3
+            sr[d] = src_l
+            B[tuple(sl)] = B[tuple(sr)]
+            sl[d] = dst_r
+            sr[d] = src_r
+            B[tuple(sl)] = B[tuple(sr)]
+    return B.reshape(A.shape, order="F")

@@ -16,5 +16,6 @@ from .core import (AMDGPU, Float64, Float32, BFloat16, Int32, Int64, Bool, Array
                    stream_context, synchronize, trues, zeros, select_hardware, current_hardware, get_hardware, handle,
                    KernelMethods)
 from . import ad as AD  # noqa: E402
+from . import igg  # noqa: E402
 from .ad import Forward, Reverse  # noqa: E402
 from . import FieldAllocators, FiniteDifferences1D, FiniteDifferences2D, FiniteDifferences3D  # noqa: E402

@@ -81,7 +81,7 @@ def autodiff_deferred_(mode, f, *args, ranges=None, nblocks=None, nthreads=None,
     """AD.autodiff_deferred!(mode, f!, args...) launched over the ranges of the call f!(args...):
     arguments that are not annotations are promoted to Const. Passing the return activity as
     third argument (the class Const itself) raises an ArgumentError, as in the reference."""
-    from .core import DeviceArray, Kernel
+    from .core import CellArray, DeviceArray, Kernel
     if not isinstance(f, Kernel):
         raise ArgumentError("AD.autodiff_deferred!: the second argument must be a kernel")
     if args and isinstance(args[0], type) and issubclass(args[0], Annotation):
@@ -97,6 +97,8 @@ def autodiff_deferred_(mode, f, *args, ranges=None, nblocks=None, nthreads=None,
             comps.extend(type(a)(x, a.dval) for x in _components(a.val))
     if len(comps) != len(k.params):
         raise ArgumentError(f"MethodError: kernel {k.name} takes {len(k.params)} arguments ({len(comps)} given)")
+    if any(isinstance(x, CellArray) for a in comps for x in (a.val, a.dval)):
+        raise ArgumentError(f"∇: kernel {k.name} has CellArray arguments; derivative kernels take plain arrays")
     annots, shadows = {}, []
     for name, a in zip(k.params, comps):
         if isinstance(a, (Duplicated, DuplicatedNoNeed)):

@@ -404,3 +404,36 @@ def test_staggered_grids_gather_scatter_and_oracle():
             assert np.array_equal(Vxb.to_host(), yx) and np.array_equal(Vyb.to_host(), yy)
         else:
             assert np.array_equal(Vxb.to_host(), np.where(inx, 0.0, yx)) and np.array_equal(Vyb.to_host(), np.where(iny, 0.0, yy))
+
+
+def test_named_tuple_arguments_async_and_cellarray_error():
+    """∇ through NamedTuple arguments (the components are differentiated; test_parallel.jl:230
+    differentiates V.x, V.y) launched with @parallel_async, and the error for CellArrays."""
+    import collections
+    _init(".nt", ps.Float64, 2)
+    k = ps.core._define("parallel_indices", """
+    @parallel_indices (ix, iy) function sq!(R, V, a)
+        R[ix, iy] = a * V.x[ix, iy] * V.y[ix, iy]
+        return
+    end""", {}, MOD + ".nt")
+    VT = collections.namedtuple("VT", "x y")
+    shape = (19, 11)
+    rng = np.random.default_rng(51)
+    hx, hy, y = (np.asfortranarray(rng.standard_normal(shape)) for _ in range(3))
+    V = VT(_dev(hx), _dev(hy))
+    Vb = VT(_dev(np.zeros(shape)), _dev(np.zeros(shape)))
+    R, Rb = _dev(np.zeros(shape)), _dev(y)
+    ps.parallel_async(k, R, V, 1.5, grad={"R": Rb, "V": Vb})
+    ps.synchronize()
+    _close(Vb.x.to_host(), 1.5 * y * hy)
+    _close(Vb.y.to_host(), 1.5 * y * hx)
+    assert np.array_equal(Rb.to_host(), np.zeros(shape))
+    C = ps.zeros(*shape, celldims=(2,))
+    Cb = ps.zeros(*shape, celldims=(2,))
+    kc = ps.core._define("parallel_indices", """
+    @parallel_indices (ix, iy) function c!(R, C)
+        R[ix, iy] = C[ix, iy][1]
+        return
+    end""", {}, MOD + ".nt")
+    with pytest.raises(ps.ArgumentError, match="CellArray"):
+        kc(R, C, grad=[(C, Cb)])

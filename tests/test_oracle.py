@@ -463,3 +463,46 @@ def test_complex_step_closed_forms():
     y = np.arange(1.0, 6.0)
     (g,) = O.complex_step_vjp(lambda a: M @ a, [np.zeros(4)], [0], y)
     assert np.allclose(g, O.linear_jacobian(lambda a: M @ a, 4).T @ y, rtol=1e-14, atol=1e-14)
+
+
+def _periodic_samples(shape, nxyz, overlaps, halowidths, periods):
+    """f at the periodic image of every cell: cell i of a dimension of size s (overlap
+    ol = overlap + s - nx) sits at global position i - 1 - (s - nx)/2 - 1 (staggered arrays are
+    shifted by half a cell), taken modulo the global size nx - overlap."""
+    pos = []
+    for d in range(3):
+        s = shape[d] if d < len(shape) else 1
+        i = np.arange(1, s + 1, dtype=float)
+        if periods[d] and s > 1:
+            ng = nxyz[d] - overlaps[d]
+            p = i - 1 - 0.5 * (s - nxyz[d]) - 1
+            pos.append(np.mod(p, ng) / ng)
+        else:
+            pos.append(i / 1000.0)
+    X, Y, Z = np.meshgrid(*pos, indexing="ij")
+    f = np.sin(2 * np.pi * X) + 0.5 * np.cos(2 * np.pi * Y) + 0.25 * np.sin(2 * np.pi * Z + 0.3) + X * 0 + 3.0
+    return f.reshape(shape, order="F")
+
+
+@pytest.mark.parametrize("case", ["ol2", "ol4", "staggered", "nonperiodic_y"])
+def test_update_halo_periodic_restores_periodic_images(case):
+    nxyz = (14, 12, 10)
+    overlaps, halowidths, periods, shape = (2, 2, 2), (1, 1, 1), (1, 1, 1), nxyz
+    if case == "ol4":
+        overlaps, halowidths = (4, 4, 4), (2, 2, 2)
+    if case == "staggered":
+        shape = (15, 12, 11)
+    if case == "nonperiodic_y":
+        periods = (1, 0, 1)
+    want = _periodic_samples(shape, nxyz, overlaps, halowidths, periods)
+    A = want.copy()
+    for d in range(3):  # the halo cells of the periodic dimensions are unknown before the update
+        if not periods[d]:
+            continue
+        s, hw = shape[d], halowidths[d]
+        sl = [slice(None)] * 3
+        for i in list(range(hw)) + list(range(s - hw, s)):
+            sl[d] = i
+            A[tuple(sl)] = np.nan
+    got = O.update_halo_periodic(A, nxyz, overlaps, halowidths, periods)
+    assert np.array_equal(got, want)
