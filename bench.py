@@ -447,6 +447,67 @@ def ad_study(args):
     print(json.dumps(out), flush=True)
 
 
+def hidecomm_study(args):
+    """examples/diffusion3D_multigpucpu_hidecomm_novis.jl on one GPU at n^3 FP64 (periodic in
+    every dimension, so that update_halo! exchanges with the process itself): the example's loop
+    (@hide_communication around the step and update_halo!) beside the same step and halo update
+    in sequence, and the step alone. T_eff is the example's (A_eff = 3 x 8 B x n^3 per step, 100
+    steps of which the first 10 are not timed)."""
+    import numpy as np
+    import parallelstencil as ps
+    from parallelstencil import igg
+
+    n, nt = args.n, 100
+    mod = "bench.hidecomm"
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3, module=mod)
+    text = ("@parallel function diffusion3D_step!(T2, T, Ci, lam, dt, _dx, _dy, _dz)\n"
+            "    @inn(T2) = @inn(T) + dt*(lam*@inn(Ci)*(@d2_xi(T)*_dx^2 + @d2_yi(T)*_dy^2 + @d2_zi(T)*_dz^2));\n"
+            "    return\nend")
+    igg.init_global_grid(n, n, n, periodx=1, periody=1, periodz=1)
+    dx = dy = dz = 1.0 / (igg.nx_g() - 1)
+    lam, c0 = 1.0, 2.0
+    dt = min(dx ** 2, dy ** 2, dz ** 2) / lam / (1 / c0) / 8.1
+    sc = (lam, dt, 1 / dx, 1 / dy, 1 / dz)
+    a_eff = 3 * 8 * n ** 3
+    runs = {}
+    for memopt in (False, True):
+        k = ps.core._define("parallel", text, {"memopt": memopt}, mod)
+        T, T2, Ci = ps.fill(1.7, n, n, n), ps.fill(1.7, n, n, n), ps.fill(1 / c0, n, n, n)
+        variants = (
+            ("step only", lambda: k(T2, T, Ci, *sc, async_=True)),
+            ("step, then update_halo!", lambda: (k(T2, T, Ci, *sc, async_=True), igg.update_halo_(T2, async_=True))),
+            ("@hide_communication (16,4,4) step + update_halo!",
+             lambda: ps.hide_communication((16, 4, 4), [(k, (T2, T, Ci) + sc)], lambda: igg.update_halo_(T2))))
+        for name, step in variants:
+            key = f"{'memopt' if memopt else 'plain'}: {name}"
+            try:
+                for it in range(nt):
+                    if it == 10:
+                        ps.synchronize()
+                        t0 = time.perf_counter()
+                    step()
+                    T, T2 = T2, T
+            except ps.ArgumentError as e:  # memopt launches need ranges divisible by the block (as in the reference)
+                ps.synchronize()
+                runs[key] = {"error": str(e)}
+                log(f"hidecomm example {n}^3 {key}: {e}")
+                continue
+            ps.synchronize()
+            t_it = (time.perf_counter() - t0) / (nt - 10)
+            runs[key] = {"ms_per_step": round(1e3 * t_it, 4), "T_eff_gbs": round(a_eff / t_it / 1e9, 1)}
+            log(f"hidecomm example {n}^3 {key}: {1e3 * t_it:.4f} ms/step, T_eff {runs[key]['T_eff_gbs']} GB/s")
+        del T, T2, Ci
+    igg.finalize_global_grid()
+    head = runs["plain: @hide_communication (16,4,4) step + update_halo!"]
+    out = {"metric": "T_eff GB/s (diffusion3D_multigpucpu_hidecomm_novis, one GPU, periodic halos)", "value": head["T_eff_gbs"],
+           "unit": "GB/s", "n_gpus": 1, "steps": nt - 10, "warmup": 10, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"examples/diffusion3D_multigpucpu_hidecomm_novis.jl at {n}^3, periodic in x, y, z"},
+           "roofline": {"bound": "hbm", "achieved": head["T_eff_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(head["T_eff_gbs"] / HBM_PEAK_GBS, 4), "traffic": None}, "runs": runs}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -455,7 +516,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad | hidecomm")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--adjoint-n", type=int, default=2048)
@@ -489,6 +550,9 @@ def main():
         return
     if args.workload == "ad":
         ad_study(args)
+        return
+    if args.workload == "hidecomm":
+        hidecomm_study(args)
         return
 
     import parallelstencil as ps

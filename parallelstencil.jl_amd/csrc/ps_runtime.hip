@@ -455,7 +455,12 @@ extern "C" int ps_device_info(ps_devinfo* out) {
   return 0;
 }
 
+// Set once a bounds-checked variant has been launched (only those write the error flag): until
+// then a synchronisation needs no device-to-host read of the flag.
+static std::atomic<bool> g_err_armed{false};
+
 static int check_err_flag(hipStream_t st) {
+  if (!g_err_armed.load(std::memory_order_relaxed)) return 0;
   State& s = S();
   int h = 0;
   PS_HIP(hipMemcpyAsync(&h, s.d_err, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1385,6 +1390,7 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
       return fail(PS_EARGUMENT, "@parallel <kernelcall>: shmem=" + std::to_string(shmem) + " bytes exceeds the " + std::to_string(lds_max) +
                                     " bytes of LDS a workgroup can use");
   }
+  if (P.v.checked) g_err_armed.store(true, std::memory_order_relaxed);
   hipError_t e = hipModuleLaunchKernel(m->fn, (unsigned)P.nblocks[0], (unsigned)P.nblocks[1], (unsigned)P.nblocks[2], (unsigned)P.nthreads[0],
                                        (unsigned)P.nthreads[1], (unsigned)P.nthreads[2], shmem, st, ptrs.data(), nullptr);
   if (e != hipSuccess) return fail(PS_EHIP, std::string("hipModuleLaunchKernel: ") + hipGetErrorString(e));

@@ -1364,6 +1364,15 @@ def _maxsize_ranges(args):
     return tuple((1, m) for m in ms)
 
 
+_HC_PLANS = {}
+
+
+def _hc_key(calls, boundary_width, ranges_inner):
+    def arg_key(x):
+        return ("v", x) if isinstance(x, (bool, int, float, np.number)) else ("o", id(x))
+    return (tuple((id(k), tuple(arg_key(x) for x in a), tuple(sorted(kw.items()))) for k, a, kw in calls), repr(boundary_width), repr(ranges_inner))
+
+
 def hide_communication(boundary_width, computations, communication, ranges_inner=None, computation_calls=None):
     """@hide_communication boundary_width begin <computation calls>; <boundary/communication code> end.
 
@@ -1372,6 +1381,12 @@ def hide_communication(boundary_width, computations, communication, ranges_inner
     running the boundary-condition / communication code; its @parallel calls go to
     priority stream 1. With `ranges_inner` given, `boundary_width` is the tuple of outer ranges
     (the explicit form `@hide_communication ranges_outer ranges_inner block`).
+
+    The boundary slabs all run on priority stream 1 and the inner region on stream 1 (the
+    reference gives each slab a stream of its own): a process has few hardware queues (4 by
+    default), and streams beyond them share queues, which serialises the boundary work behind
+    the inner region. The launches of a call are prepared once and replayed when the same
+    kernels are called with the same arguments.
     """
     ensure_device()
     if callable(computations) or isinstance(computations, tuple) and computations and isinstance(computations[0], Kernel):
@@ -1387,27 +1402,33 @@ def hide_communication(boundary_width, computations, communication, ranges_inner
         calls.append((k, tuple(a), kw))
     if computation_calls is not None and computation_calls < 1:
         raise KeywordArgumentError("Invalid keyword argument in @hide_communication: computation_calls must be >= 1.")
-    if ranges_inner is None:
-        rngs = [_maxsize_ranges(a) for _, a, _ in calls]
-        if any(r != rngs[0] for r in rngs):
-            raise ArgumentError(f"@hide_communication: the ranges of the computation calls are not all equal (obtained: {rngs}).")
-        outer = get_ranges_outer(boundary_width, rngs[0])
-        inner = get_ranges_inner(boundary_width, rngs[0])
-    else:
-        outer, inner = boundary_width, ranges_inner
-    for i, r in enumerate(outer):
-        for k, a, kw in calls:
-            k(*a, ranges=r, stream=get_priority_stream(i + 1), async_=True, **kw)
-    for i, r in enumerate(inner):
-        for k, a, kw in calls:
-            k(*a, ranges=r, stream=get_stream(i + 1), async_=True, **kw)
-    for i in range(len(outer)):
-        get_priority_stream(i + 1).synchronize()
-    with stream_context(get_priority_stream(1)):
+    key = _hc_key(calls, boundary_width, ranges_inner)
+    plan = _HC_PLANS.get(key)
+    if plan is None:
+        if ranges_inner is None:
+            rngs = [_maxsize_ranges(a) for _, a, _ in calls]
+            if any(r != rngs[0] for r in rngs):
+                raise ArgumentError(f"@hide_communication: the ranges of the computation calls are not all equal (obtained: {rngs}).")
+            outer = get_ranges_outer(boundary_width, rngs[0])
+            inner = get_ranges_inner(boundary_width, rngs[0])
+        else:
+            outer, inner = boundary_width, ranges_inner
+        bs, ins = get_priority_stream(1), get_stream(1)
+        plan = ([k.launcher(*a, ranges=r, stream=bs, async_=True, **kw) for r in outer for k, a, kw in calls],
+                [k.launcher(*a, ranges=r, stream=ins, async_=True, **kw) for r in inner for k, a, kw in calls], bs, ins)
+        if len(_HC_PLANS) >= 64:
+            _HC_PLANS.pop(next(iter(_HC_PLANS)))
+        _HC_PLANS[key] = plan
+    outer_l, inner_l, bs, ins = plan
+    for L in outer_l:
+        L()
+    for L in inner_l:
+        L()
+    bs.synchronize()
+    with stream_context(bs):
         communication()
-    get_priority_stream(1).synchronize()
-    for i in range(len(inner)):
-        get_stream(i + 1).synchronize()
+    bs.synchronize()
+    ins.synchronize()
 
 
 # ---------------------------------------------------------------------------

@@ -74,6 +74,7 @@ def finalize_global_grid(finalize_MPI=True):
     global _grid
     _g()
     _grid = None
+    _halo_views.clear()
 
 
 def select_device():
@@ -140,6 +141,30 @@ def _halo_boxes(shape, d, ol, hw):
     return [(box(1, hw), box(s - ol + 1, s - ol + hw)), (box(s - hw + 1, s), box(ol - hw + 1, ol))]
 
 
+_halo_views = {}  # id(A) -> (A, [(destination view, source view)]): prepared once per array
+
+
+def _halo_pairs(A, g):
+    ent = _halo_views.get(id(A))
+    if ent is not None and ent[0] is A:
+        return ent[1]
+    shape = tuple(A.shape) + (1,) * (3 - len(A.shape))
+    pairs = []
+    for d in range(3):
+        if not g.periods[d] or g.nxyz[d] <= 1 or shape[d] <= 1:
+            continue
+        ol = g.overlaps[d] + shape[d] - g.nxyz[d]
+        hw = g.halowidths[d]
+        if ol < 2 * hw:
+            raise ArgumentError(f"update_halo!: the overlap of the array in dimension {d + 1} ({ol}) is smaller than 2 x halowidth ({hw}).")
+        for dst, src in _halo_boxes(shape, d, ol, hw):
+            pairs.append((A.view(*dst), A.view(*src)))
+    if len(_halo_views) >= 64:
+        _halo_views.pop(next(iter(_halo_views)))
+    _halo_views[id(A)] = (A, pairs)
+    return pairs
+
+
 def update_halo_(*arrays, stream=None, async_=False):
     """update_halo!(A...): the halo exchange of each array (periodic dimensions exchange with
     the process itself). Ordered on `stream` (the current stream by default) and, as in
@@ -148,16 +173,8 @@ def update_halo_(*arrays, stream=None, async_=False):
     for A in arrays:
         if not isinstance(A, core.DeviceArray) or isinstance(A, core.CellArray):
             raise ArgumentError("update_halo!: the arguments must be fields (device arrays)")
-        shape = tuple(A.shape) + (1,) * (3 - len(A.shape))
-        for d in range(3):
-            if not g.periods[d] or g.nxyz[d] <= 1 or shape[d] <= 1:
-                continue
-            ol = g.overlaps[d] + shape[d] - g.nxyz[d]
-            hw = g.halowidths[d]
-            if ol < 2 * hw:
-                raise ArgumentError(f"update_halo!: the overlap of the array in dimension {d + 1} ({ol}) is smaller than 2 x halowidth ({hw}).")
-            for dst, src in _halo_boxes(shape, d, ol, hw):
-                A.view(*dst).copy_(A.view(*src), stream=stream, async_=True)
+        for dst, src in _halo_pairs(A, g):  # dimension after dimension: edges and corners follow
+            dst.copy_(src, stream=stream, async_=True)
     if not async_:
         core.check(core.lib.ps_stream_sync(core._stream_handle(stream)))
 
