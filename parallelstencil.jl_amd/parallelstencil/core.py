@@ -22,6 +22,7 @@ import json
 import re
 import threading
 import warnings
+import weakref
 
 import numpy as np
 
@@ -850,8 +851,10 @@ class Kernel:
         """(kernel, arguments) with tuple and NamedTuple arguments (Data.ArrayTuple,
         Data.NamedArrayTuple) split into their components; each distinct structure of such
         arguments gets a kernel of its own, whose parameters are the components."""
-        spec, flat = {}, []
         dp = self.info.get("def_params", self.params)
+        if len(dp) == len(self.params) and not any(isinstance(v, (tuple, list, dict)) for v in args):
+            return self, args  # no tuple arguments
+        spec, flat = {}, []
         for i, v in enumerate(args):
             names = _tuple_names(v)
             if names is None or i >= len(dp):
@@ -901,6 +904,23 @@ class Kernel:
         raise ArgumentError(f"argument {self.params[i]}: unsupported type {type(v).__name__}")
 
     def _args(self, args):
+        """The argument block of a launch. Blocks are kept for the last few argument lists
+        (arrays by identity, through weak references, and scalars by type and value): a loop
+        that launches with the same arrays skips the rebuild."""
+        key = tuple(id(v) if isinstance(v, (DeviceArray, ArgSpec)) else (type(v), v) for v in args)
+        cache = self.__dict__.setdefault("_arg_blocks", {})
+        hit = cache.get(key)
+        if hit is not None:
+            arr, refs = hit
+            if all(r() is v for r, v in zip(refs, (v for v in args if isinstance(v, (DeviceArray, ArgSpec))))):
+                return arr
+        arr = self._build_args(args)
+        if len(cache) >= 16:
+            cache.pop(next(iter(cache)))
+        cache[key] = (arr, [weakref.ref(v) for v in args if isinstance(v, (DeviceArray, ArgSpec))])
+        return arr
+
+    def _build_args(self, args):
         if len(args) != len(self.params):
             raise ArgumentError(f"MethodError: kernel {self.name} takes {len(self.params)} arguments ({len(args)} given)")
         arr = (ps_arg * max(1, len(args)))()
