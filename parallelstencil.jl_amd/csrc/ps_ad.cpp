@@ -727,21 +727,88 @@ class Ad {
     }
     return false;
   }
-  void check_gather() const {
+  // The body as a list of guarded writes: every array write with the condition of its path
+  // and its right-hand side, locals inlined (a local set in branches becomes an ifelse of the
+  // branch values). Loop-free bodies only; a duplicated array may be written once per path.
+  struct FlatWrite {
+    NodeP cond, target, rhs;
+  };
+  std::vector<FlatWrite> flat;
+  std::string flat_why;
+  NodeP inline_locals(const NodeP& e, const std::map<std::string, NodeP>& env) const {
+    std::function<NodeP(const NodeP&)> f = [&](const NodeP& n) -> NodeP {
+      if (!n) return n;
+      if (n->k == NK::Ident) {
+        auto it = env.find(n->s);
+        if (it != env.end()) {
+          if (!it->second) throw std::string("local '" + n->s + "' is not defined on every path");
+          return clone(it->second);
+        }
+        return clone(n);
+      }
+      auto c = std::make_shared<Node>(*n);
+      for (auto& x : c->a) x = f(x);
+      return c;
+    };
+    return canonical_indices(f(e));
+  }
+  static NodeP conj(const NodeP& a, const NodeP& b) { return !a ? b : mk(NK::And, "&&", {a, b}); }
+  void flatten(const NodeP& blk, const NodeP& cond, std::map<std::string, NodeP>& env, std::set<std::string>& written) {
+    for (auto& st : blk->a) {
+      switch (st->k) {
+        case NK::Assign:
+          if (st->a[0]->k == NK::Ident) {
+            env[st->a[0]->s] = inline_locals(eff_rhs(st), env);
+          } else {
+            NodeP t = inline_locals(st->a[0], env);
+            const std::string& W = t->a[0]->s;
+            if (shadow.count(W) && written.count(W)) throw std::string("duplicated array '" + W + "' is written twice on one path");
+            written.insert(W);
+            flat.push_back({cond ? clone(cond) : nullptr, t, inline_locals(eff_rhs(st), env)});
+          }
+          break;
+        case NK::If: {
+          NodeP c = inline_locals(st->a[0], env);
+          auto e1 = env, e2 = env;
+          auto w1 = written, w2 = written;
+          flatten(st->a[1], conj(cond, c), e1, w1);
+          if (st->a.size() > 2 && st->a[2]) {
+            auto eb = st->a[2]->k == NK::If ? mk(NK::Block, "", {st->a[2]}) : st->a[2];
+            flatten(eb, conj(cond, mk(NK::Unary, "!", {clone(c)})), e2, w2);
+          }
+          std::set<std::string> names;
+          for (auto& kv : e1) names.insert(kv.first);
+          for (auto& kv : e2) names.insert(kv.first);
+          for (auto& x : names) {
+            NodeP v1 = e1.count(x) ? e1[x] : nullptr, v2 = e2.count(x) ? e2[x] : nullptr;
+            if (env.count(x) && v1 == env[x] && v2 == env[x]) continue;
+            env[x] = (v1 && v2) ? call("ifelse", {clone(c), v1, v2}) : nullptr;  // undefined on one path
+          }
+          written.insert(w1.begin(), w1.end());
+          written.insert(w2.begin(), w2.end());
+          break;
+        }
+        case NK::Block: flatten(st, cond, env, written); break;
+        case NK::For: throw std::string("the kernel has loops");
+        default: break;
+      }
+    }
+  }
+  void check_gather() {
     auto no = [&](const std::string& why) {
       throw PsError(PS_EARGUMENT, "∇: the gather form does not apply to kernel " + K.def.name + ": " + why);
     };
     if (!reverse) no("forward mode");
-    if (K.opt.kind != "parallel") no("not a @parallel kernel");
-    std::map<std::string, int> writes;
-    for (auto& st : K.body->a) {
-      if (!(st->k == NK::If && st->a[1]->k == NK::Block && st->a[1]->a.size() == 1 && st->a[1]->a[0]->k == NK::Assign &&
-            (st->a.size() < 3 || !st->a[2])))
-        no("unexpected statement form");
-      const NodeP& as = st->a[1]->a[0];
-      if (!as->s.empty() || !is_param_array(as->a[0])) no("unexpected assignment form");
-      writes[as->a[0]->a[0]->s]++;
-      walk(as->a[1], [&](const NodeP& x) {
+    flat.clear();
+    try {
+      std::map<std::string, NodeP> env;
+      std::set<std::string> written;
+      flatten(K.body, nullptr, env, written);
+    } catch (const std::string& why) {
+      no(why);
+    }
+    for (auto& w : flat) {
+      walk(w.rhs, [&](const NodeP& x) {
         if (!is_param_array(x) || !shadow.count(x->a[0]->s)) return;
         if (K.written.count(x->a[0]->s)) no("duplicated array '" + x->a[0]->s + "' is read and written");
         if ((int)x->a.size() - 1 != K.nidx) no("'" + x->a[0]->s + "' has other dimensions than the kernel");
@@ -750,8 +817,6 @@ class Ad {
           if (!offset_of(x->a[d + 1], d, c)) no("'" + x->a[0]->s + "' is read at a non-constant offset");
       });
     }
-    for (auto& w : writes)
-      if (shadow.count(w.first) && w.second > 1) no("duplicated array '" + w.first + "' is written by several statements");
   }
   // Copy of e at the point i = j - o: parallel index d becomes (index - o_d).
   NodeP shift(const NodeP& e, const long long o[3]) const {
@@ -773,28 +838,35 @@ class Ad {
       if (!shadow.count(R) || K.written.count(R) || !K.accessed.count(R)) continue;
       std::string acc = fresh("acc");
       body->a.push_back(assign(ident(acc), fnum(0.0)));
-      for (auto& st : K.body->a) {
-        const NodeP& as = st->a[1]->a[0];
-        const std::string& W = as->a[0]->a[0]->s;
+      for (auto& w : flat) {
+        const std::string& W = w.target->a[0]->s;
         if (!shadow.count(W)) continue;
         std::vector<NodeP> ia{ident(shadow.at(W))};
-        for (size_t i = 1; i < as->a[0]->a.size(); ++i) ia.push_back(clone(as->a[0]->a[i]));
+        for (size_t i = 1; i < w.target->a.size(); ++i) ia.push_back(clone(w.target->a[i]));
         auto sink = mk(NK::Block);
-        back(as->a[1], mk(NK::Index, "", ia), sink);
+        back(w.rhs, mk(NK::Index, "", ia), sink);
         for (auto& p : pending) {
           if (p.array != R) continue;
           long long o[3] = {0, 0, 0};
           for (int d = 0; d < K.nidx; ++d) offset_of(p.target->a[d + 1], d, o[d]);
+          // the source point i = j - o exists (it lies in the launch box) and takes this path
+          NodeP cond;
+          for (int d = 0; d < K.nidx; ++d) {
+            const std::string r0 = std::string("__r0") + "xyz"[d], rl = std::string("__rl") + "xyz"[d];
+            NodeP i = shift(ident(kPIdx[d]), o);
+            cond = conj(cond, cmp(ident(r0), "<=", clone(i)));
+            cond = conj(cond, cmp(i, "<", mk(NK::Binary, "+", {ident(r0), ident(rl)})));
+          }
+          if (w.cond) cond = conj(cond, shift(w.cond, o));
           auto then = mk(NK::Block, "", {assign(ident(acc), add(ident(acc), shift(p.sum, o)))});
-          body->a.push_back(mk(NK::If, "", {shift(st->a[0], o), then, nullptr}));
+          body->a.push_back(mk(NK::If, "", {cond, then, nullptr}));
         }
         pending.clear();
       }
       NodeP guard;
       std::vector<NodeP> ja{ident(shadow.at(R))};
       for (int d = 0; d < K.nidx; ++d) {
-        NodeP c = cmp(ident(kPIdx[d]), "<=", call("size", {ident(R), mknum_int(d + 1)}));
-        guard = guard ? mk(NK::And, "&&", {guard, c}) : c;
+        guard = conj(guard, cmp(ident(kPIdx[d]), "<=", call("size", {ident(R), mknum_int(d + 1)})));
         ja.push_back(ident(kPIdx[d]));
       }
       NodeP target = mk(NK::Index, "", ja);
@@ -805,17 +877,18 @@ class Ad {
   }
   NodeP clear_body() {
     auto body = mk(NK::Block);
-    for (auto& st : K.body->a) {
-      const NodeP& as = st->a[1]->a[0];
-      const std::string& W = as->a[0]->a[0]->s;
+    for (auto& w : flat) {
+      const std::string& W = w.target->a[0]->s;
       auto blk = mk(NK::Block);
       if (shadow.count(W)) {
         std::vector<NodeP> ia{ident(shadow.at(W))};
-        for (size_t i = 1; i < as->a[0]->a.size(); ++i) ia.push_back(clone(as->a[0]->a[i]));
+        for (size_t i = 1; i < w.target->a.size(); ++i) ia.push_back(clone(w.target->a[i]));
         blk->a.push_back(assign(mk(NK::Index, "", ia), fnum(0.0)));
       }
-      if (!noneed.count(W)) blk->a.push_back(clone(as));
-      if (!blk->a.empty()) body->a.push_back(mk(NK::If, "", {clone(st->a[0]), blk, nullptr}));
+      if (!noneed.count(W)) blk->a.push_back(assign(clone(w.target), clone(w.rhs)));
+      if (blk->a.empty()) continue;
+      if (w.cond) body->a.push_back(mk(NK::If, "", {clone(w.cond), blk, nullptr}));
+      else append(body, blk);
     }
     return body;
   }

@@ -762,6 +762,10 @@ EV Emitter::expr(const NodeP& n) {
     case NK::Ident: {
       for (int d = 0; d < 3; ++d)
         if (n->s == kIdx[d]) return EV{kIdxC[d], Ty::Int};
+      // the launch box (first index and length per dimension), used by generated kernels that
+      // ask whether another point of the box exists
+      if (n->s.size() == 5 && n->s.compare(0, 3, "__r") == 0 && (n->s[3] == '0' || n->s[3] == 'l') && n->s[4] >= 'x' && n->s[4] <= 'z')
+        return EV{std::string("ps_r") + n->s[3] + n->s[4], Ty::Int};
       if (n->s == "true" || n->s == "false") return EV{n->s, Ty::Bool};
       if (n->s == "pi" || n->s == "π") return lit(M_PI, NT);
       auto lt = locals.find(n->s);
@@ -2002,6 +2006,13 @@ struct Prover {
         v = e->ival;
         return true;
       case NK::Ident: {
+        if (e->s.size() == 5 && e->s.compare(0, 3, "__r") == 0 && e->s[4] >= 'x' && e->s[4] <= 'z') {  // the launch box
+          const int d = e->s[4] - 'x';
+          if (e->s[3] == '0') v = S.lo[d];
+          else if (e->s[3] == 'l') v = S.hi[d] - S.lo[d] + 1;
+          else return false;
+          return true;
+        }
         auto it = K.param_index.find(e->s);
         if (it == K.param_index.end()) return false;
         int i = it->second;

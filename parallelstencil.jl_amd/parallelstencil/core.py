@@ -1052,7 +1052,7 @@ class Kernel:
 
 class _DerivativeKernel(Kernel):
     """A derivative kernel made by ps_kernel_derivative from a defined kernel. Reverse mode of a
-    @parallel kernel also gets the two-kernel gather form (each point sums the adjoints that
+    loop-free kernel also gets the two-kernel gather form (each point sums the adjoints that
     reach it, then the seeds are cleared): it is used for launches with the default geometry,
     the atomic (scatter) form otherwise or where the gather form does not apply."""
 
@@ -1076,7 +1076,7 @@ class _DerivativeKernel(Kernel):
         self.numbertype = self.info["numbertype"]
         self._expanded = {}
         self._derivs = {}
-        if form == "scatter" and mode == "Reverse" and primal.info["kind"] == "parallel":
+        if form == "scatter" and mode == "Reverse":
             try:
                 self._gather = (_DerivativeKernel(primal, mode, annots, "gather"), _DerivativeKernel(primal, mode, annots, "clear"))
             except ArgumentError:

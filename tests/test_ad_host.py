@@ -226,3 +226,42 @@ def test_memopt_primal_gives_streaming_derivatives():
     g = W.define_diffusion3d(True, MOD)._derivative("Reverse", {"T2": "DuplicatedNoNeed", "T": "DuplicatedNoNeed"})._gather[0]
     assert set(g.info["optvars"]) == {"T2#shadow", "Cp"}
     assert g.info["stencilranges"]["T2#shadow"] == [[-1, 1], [-1, 1], [-1, 1]]
+
+
+def test_gather_form_of_parallel_indices_kernels():
+    """Loop-free @parallel_indices kernels get the gather form too: locals are inlined, a local
+    set in both branches becomes an ifelse, and every contribution is guarded by the source
+    point's existence in the launch box and by its path condition."""
+    k = _k("""
+    @parallel_indices (ix, iy) function g!(R, U, V)
+        if ix > 1 && ix < size(U,1)
+            a = U[ix+1,iy] - U[ix-1,iy]
+        else
+            a = 2.0*U[ix,iy]
+        end
+        R[ix,iy] = a*V[ix,iy]
+        return
+    end""", ndims=2)
+    d = k._derivative("Reverse", {"R": "DuplicatedNoNeed", "U": "DuplicatedNoNeed", "V": "DuplicatedNoNeed"})
+    assert d._gather is not None
+    S = ps.ArgSpec((9, 7))
+    g = _body(d._gather[0].source(S, S, S, S, S, S))
+    assert "unsafeAtomicAdd" not in g and "ps_r0x" in g and "ps_rlx" in g
+    loop = _k("""
+    @parallel_indices (ix) function l!(A, B)
+        for k = 1:2
+            A[ix] = B[ix]
+        end
+        return
+    end""")
+    assert loop._derivative("Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"})._gather is None
+    twice = _k("""
+    @parallel_indices (ix) function t!(A, B)
+        A[ix] = B[ix]
+        if ix > 1
+            A[ix] = 2.0*B[ix]
+        end
+        return
+    end""")
+    with pytest.raises(ps.ArgumentError, match="written twice on one path"):
+        ps.core._DerivativeKernel(twice, "Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"}, "gather")

@@ -238,7 +238,11 @@ def test_nonlinear_forward_against_complex_step():
     assert np.array_equal(R.to_host(), np.zeros(shape))
 
 
-def test_nonlinear_reverse_against_complex_step():
+@pytest.mark.parametrize("form", ["gather", "scatter"])
+def test_nonlinear_reverse_against_complex_step(form):
+    """@parallel_indices kernel with locals and branches: the gather form (locals inlined,
+    branch conditions shifted to the source point) and the atomic scatter form."""
+    print(f"ad: nonlinear reverse form={form}", flush=True)
     shape = (12, 10)
     U, V, K, s = _nl_inputs(shape, 23)
     rng = np.random.default_rng(24)
@@ -247,7 +251,11 @@ def test_nonlinear_reverse_against_complex_step():
     k = _nl_kernel()
     R, Ud, Vd, Kd = _dev(np.zeros(shape)), _dev(U), _dev(V), _dev(K)
     Rb, Ub, Vb, Kb = _dev(y), _dev(zU), _dev(zV), _dev(zK)
+    d = k._derivative("Reverse", {"R": "Duplicated", "U": "DuplicatedNoNeed", "V": "DuplicatedNoNeed", "K": "DuplicatedNoNeed"})
+    assert d._gather is not None
+    d.use_gather = form == "gather"
     k(R, Ud, Vd, Kd, s, grad={"R": Rb, "U": Ub, "V": Vb, "K": Kb}, ad_annotations={"Duplicated": "R", "Active": "s"})
+    d.use_gather = True
     gU, gV, gK = O.complex_step_vjp(lambda a, b, c: nl_ref(a, b, c, s), [U, V, K], [0, 1, 2], y)
     _close(Ub.to_host(), zU + gU)
     _close(Vb.to_host(), zV + gV)
@@ -437,3 +445,37 @@ def test_named_tuple_arguments_async_and_cellarray_error():
     end""", {}, MOD + ".nt")
     with pytest.raises(ps.ArgumentError, match="CellArray"):
         kc(R, C, grad=[(C, Cb)])
+
+
+def test_branch_local_gather_matches_complex_step():
+    """A local set in both branches (merged as an ifelse in the gather form)."""
+    _init(".br", ps.Float64, 2)
+    k = ps.core._define("parallel_indices", """
+    @parallel_indices (ix, iy) function g!(R, U, V)
+        if ix > 1 && ix < size(U,1)
+            a = U[ix+1,iy] - U[ix-1,iy]*U[ix,iy]
+        else
+            a = 2.0*U[ix,iy]
+        end
+        R[ix,iy] = a*V[ix,iy]
+        return
+    end""", {}, MOD + ".br")
+
+    def ref(U, V):
+        a = 2.0 * U
+        a[1:-1, :] = U[2:, :] - U[:-2, :] * U[1:-1, :]
+        return a * V
+
+    shape = (13, 9)
+    rng = np.random.default_rng(61)
+    U, V, y = (np.asfortranarray(rng.standard_normal(shape)) for _ in range(3))
+    for form in ("gather", "scatter"):
+        d = k._derivative("Reverse", {"R": "DuplicatedNoNeed", "U": "DuplicatedNoNeed", "V": "DuplicatedNoNeed"})
+        d.use_gather = form == "gather"
+        R, Ud, Vd = _dev(np.zeros(shape)), _dev(U), _dev(V)
+        Rb, Ub, Vb = _dev(y), _dev(np.zeros(shape)), _dev(np.zeros(shape))
+        k(R, Ud, Vd, grad=[(R, Rb), (Ud, Ub), (Vd, Vb)])
+        d.use_gather = True
+        gU, gV = O.complex_step_vjp(ref, [U, V], [0, 1], y)
+        _close(Ub.to_host(), gU)
+        _close(Vb.to_host(), gV)
