@@ -81,3 +81,36 @@ def test_fp32_gradient_against_fp64():
     rel = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
     print(f"FP32 gradient relative L2 error vs FP64: {rel:.3e}")
     assert rel < 1e-3
+
+
+def test_hand_written_adjoint_matches_derivative_kernels():
+    """The gradient of the hand-written adjoint (transposed stencils, checkpointing) against the
+    one obtained by differentiating the forward kernels themselves with @parallel ∇ (the
+    reference's route to this gradient, src/ParallelKernel/parallel.jl:265-301), step by step
+    backwards through stored states: record!, inject!, compute_PK! and compute_V! in reverse."""
+    s, inv, Dobs = _inv(MOD64, nx=48, ny=40, nt=60)
+    J, g = inv.gradient(s.K)
+    dt, rho, dx, dy = s.dt, s.rho, s.dx, s.dy
+    inv.K.copy_from(s.K)
+    inv._reset_state()
+    states = []
+    for n in range(s.nt):  # the state before each step
+        states.append(tuple(ps.zeros(*A.shape).copy_(A) for A in (inv.P, inv.Vx, inv.Vy)))
+        inv._step(n)
+    ps.synchronize()
+    D = inv.D.to_host()
+    Db = ps.zeros(s.nrec, s.nt).copy_from(np.asfortranarray(D - Dobs))
+    lP, lVx, lVy, gK = ps.zeros(s.nx, s.ny), ps.zeros(s.nx + 1, s.ny), ps.zeros(s.nx, s.ny + 1), ps.zeros(s.nx, s.ny)
+    P, Vx, Vy = ps.zeros(s.nx, s.ny), ps.zeros(s.nx + 1, s.ny), ps.zeros(s.nx, s.ny + 1)
+    for n in reversed(range(s.nt)):
+        for A, B in zip((P, Vx, Vy), states[n]):
+            A.copy_(B)
+        inv.k_v(Vx, Vy, P, dt, rho, dx, dy)  # V^{n+1} (compute_PK! reads it), P^n
+        inv.k_rec(inv.D, P, s.rx0, s.rdx, s.ry, n + 1, ranges=((1, s.nrec),), grad=[(inv.D, Db), (P, lP)])
+        inv.k_inj(P, s.src[n], s.isx, s.isy, ranges=((1, 1),), grad=[(P, lP)])
+        inv.k_p(P, Vx, Vy, inv.K, dt, dx, dy, grad=[(P, lP), (Vx, lVx), (Vy, lVy), (inv.K, gK)])
+        inv.k_v(Vx, Vy, P, dt, rho, dx, dy, grad=[(Vx, lVx), (Vy, lVy), (P, lP)])
+    ga = gK.to_host()
+    assert np.abs(ga).max() > 0
+    assert np.abs(ga - g).max() <= 1e-12 * np.abs(g).max()
+    assert np.array_equal(Db.to_host(), np.zeros((s.nrec, s.nt)))  # every residual consumed
