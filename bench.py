@@ -438,6 +438,42 @@ def ad_study(args):
         runs[name] = {"ms_per_launch": round(ms, 4), "bytes_per_point": bpp, "gbs": round(bpp * pts / ms / 1e6, 1)}
         log(f"diffusion3D {n}^3 {name}: {ms:.4f} ms/launch, {runs[name]['gbs']} GB/s algorithmic")
     head = runs["reverse mode (adjoint, gather + clear)"]
+    del T, T2, Cp, S1, S2
+    # acoustic2D (miniapps/acoustic2D.jl with a bulk-modulus field, as in the inversion): one
+    # adjoint step by the hand-written transposed-stencil kernels and by the derivative kernels
+    # of the forward kernels (compute_PK! then compute_V! in reverse)
+    from parallelstencil import acoustic_adjoint as AA
+    m2 = "bench.ad2"
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2, module=m2)
+    d2 = lambda text: ps.core._define("parallel", text, {}, m2)
+    kv, kp, kgk, kav, kap = d2(AA.FWD_V), d2(AA.FWD_P), d2(AA.GRAD_K), d2(AA.ADJ_V), d2(AA.ADJ_P)
+    na = 2048
+    P, Kf, lP, gK = (ps.fill(v, na, na) for v in (0.1, 1.0, 1e-3, 0.0))
+    Vx, lVx = ps.fill(0.2, na + 1, na), ps.fill(1e-3, na + 1, na)
+    Vy, lVy = ps.fill(0.3, na, na + 1), ps.fill(1e-3, na, na + 1)
+    sa = (1e-3, 0.5, 0.5)
+    dp = kp._derivative("Reverse", {"P": "DuplicatedNoNeed", "Vx": "DuplicatedNoNeed", "Vy": "DuplicatedNoNeed", "K": "DuplicatedNoNeed"})
+    dv = kv._derivative("Reverse", {"Vx": "DuplicatedNoNeed", "Vy": "DuplicatedNoNeed", "P": "DuplicatedNoNeed"})
+
+    def hand(i):
+        kgk(gK, lP, Vx, Vy, *sa, async_=True)
+        kav(lVx, lVy, lP, Kf, *sa, async_=True)
+        kap(lP, lVx, lVy, sa[0], 1.0, sa[1], sa[2], async_=True)
+
+    def generated(gather):
+        def run(i):
+            dp.use_gather = dv.use_gather = gather
+            dp(P, Vx, Vy, Kf, *sa, lP, lVx, lVy, gK, async_=True)
+            dv(Vx, Vy, P, sa[0], 1.0, sa[1], sa[2], lVx, lVy, lP, async_=True)
+            dp.use_gather = dv.use_gather = True
+        return run
+
+    for name, launch in (("acoustic2D 2048^2 adjoint step, hand-written (grad_K!, adjoint_V!, adjoint_P!)", hand),
+                         ("acoustic2D 2048^2 adjoint step, derivative kernels (gather form)", generated(True)),
+                         ("acoustic2D 2048^2 adjoint step, derivative kernels (atomic scatter form)", generated(False))):
+        wall, ms = time_kernel(ps, launch, args.steps, Wm)
+        runs[name] = {"ms_per_step": round(ms, 4)}
+        log(f"{name}: {ms:.4f} ms/step")
     out = {"metric": "GB/s algorithmic (reverse-mode derivative of diffusion3D)", "value": head["gbs"], "unit": "GB/s", "n_gpus": 1,
            "steps": K, "warmup": Wm, "ms_per_step": head["ms_per_launch"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",

@@ -810,7 +810,13 @@ class Ad {
     for (auto& w : flat) {
       walk(w.rhs, [&](const NodeP& x) {
         if (!is_param_array(x) || !shadow.count(x->a[0]->s)) return;
-        if (K.written.count(x->a[0]->s)) no("duplicated array '" + x->a[0]->s + "' is read and written");
+        if (K.written.count(x->a[0]->s)) {
+          // an in-place update reads the element it writes: its derivative stays with the
+          // element (applied by the clear kernel); any other read of a written array is refused
+          if (!(x->a[0]->s == w.target->a[0]->s && to_julia(x) == to_julia(w.target)))
+            no("duplicated array '" + x->a[0]->s + "' is read and written at different elements");
+          return;
+        }
         if ((int)x->a.size() - 1 != K.nidx) no("'" + x->a[0]->s + "' has other dimensions than the kernel");
         long long c;
         for (int d = 0; d < K.nidx; ++d)
@@ -876,20 +882,35 @@ class Ad {
     return body;
   }
   NodeP clear_body() {
+    no_temps = true;
     auto body = mk(NK::Block);
     for (auto& w : flat) {
       const std::string& W = w.target->a[0]->s;
       auto blk = mk(NK::Block);
       if (shadow.count(W)) {
+        // the seed is consumed: it becomes the adjoint of the element's old value, g * de/dW[i]
+        // for an in-place update (e reads W[i]) and zero otherwise
         std::vector<NodeP> ia{ident(shadow.at(W))};
         for (size_t i = 1; i < w.target->a.size(); ++i) ia.push_back(clone(w.target->a[i]));
-        blk->a.push_back(assign(mk(NK::Index, "", ia), fnum(0.0)));
+        NodeP seed = mk(NK::Index, "", ia);
+        std::string g = fresh("g");
+        auto sink = mk(NK::Block);
+        back(w.rhs, ident(g), sink);
+        NodeP own = fnum(0.0);
+        for (auto& p : pending)
+          if (p.array == W) own = add(own, p.sum);
+        pending.clear();
+        if (!(own->k == NK::Ident && own->s == g)) {  // a plain in-place update leaves the seed as it is
+          if (!is_num(own, 0)) blk->a.push_back(assign(ident(g), clone(seed)));
+          blk->a.push_back(assign(seed, own));
+        }
       }
       if (!noneed.count(W)) blk->a.push_back(assign(clone(w.target), clone(w.rhs)));
       if (blk->a.empty()) continue;
       if (w.cond) body->a.push_back(mk(NK::If, "", {clone(w.cond), blk, nullptr}));
       else append(body, blk);
     }
+    no_temps = false;
     return body;
   }
 

@@ -202,16 +202,34 @@ def test_gather_form_of_parallel_kernels():
     g.precompile(*args)
     c.precompile(*args)
     # no gather form: @parallel_indices kernels, forward mode, arrays read and written
-    assert _k(F_REF)._derivative("Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"})._gather is None
+    assert _k(F_REF)._derivative("Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"})._gather is not None  # in-place
     assert k._derivative("Forward", {"T2": "DuplicatedNoNeed", "T": "DuplicatedNoNeed"})._gather is None
+    # an in-place update (the element it writes is the only one of A it reads) keeps the gather
+    # form: the clear kernel applies A's own derivative
     inplace = _k("""
     @parallel function upd!(A, B)
-        @inn(A) = @inn(A) + @d2(B)
+        @inn(A) = 2.0*@inn(A) + @d2(B)
         return
     end""")
-    assert inplace._derivative("Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"})._gather is None
-    with pytest.raises(ps.ArgumentError, match="read and written"):
-        ps.core._DerivativeKernel(inplace, "Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"}, "gather")
+    d = inplace._derivative("Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"})
+    assert d._gather is not None
+    S1 = ps.ArgSpec((20,))
+    clear = _body(d._gather[1].source(S1, S1, S1, S1))
+    assert re.search(r"a2\[[^\]]*\] = \(v_ad_23g\d+ \* 0x1p\+1\)", clear) or re.search(r"a2\[[^\]]*\] = \(0x1p\+1 \* v_ad_23g\d+\)", clear)
+    neighbour = _k("""
+    @parallel_indices (ix) function nb!(A, B)
+        A[ix] = A[ix] + B[ix]*A[ix]
+        return
+    end""")
+    assert neighbour._derivative("Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed"})._gather is not None
+    other = _k("""
+    @parallel_indices (ix) function ot!(A, B, C)
+        A[ix] = B[ix]
+        C[ix] = A[ix] + 1.0
+        return
+    end""")
+    with pytest.raises(ps.ArgumentError, match="after writing it"):
+        other._derivative("Reverse", {"A": "DuplicatedNoNeed", "B": "DuplicatedNoNeed", "C": "DuplicatedNoNeed"})
 
 
 def test_memopt_primal_gives_streaming_derivatives():

@@ -370,8 +370,8 @@ def test_staggered_grids_gather_scatter_and_oracle():
     """Arrays of different sizes (the staggered velocities of miniapps/acoustic2D.jl). With the
     default ranges the gather form runs, with explicit ranges the scatter form; both against the
     dense Jacobian of the oracle's compute_V! (zero initial velocities). The kernel of the
-    miniapp itself reads Vx and Vy before updating them: it has the scatter form only, and its
-    seeds pass through unchanged (dVx_new/dVx_old = 1)."""
+    miniapp itself reads Vx and Vy before updating them (in-place updates): its seeds pass
+    through unchanged (dVx_new/dVx_old = 1), in the gather form and in the scatter form."""
     _init(".ac", ps.Float64, 2)
     grad_p = ps.core._define("parallel", """
     @parallel function grad_P!(Vx, Vy, P, dt, rho, dx, dy)
@@ -400,8 +400,8 @@ def test_staggered_grids_gather_scatter_and_oracle():
     want = J.T @ np.concatenate([np.where(inx, yx, 0).ravel(order="F"), np.where(iny, yy, 0).ravel(order="F")])
     annots = {"P": "DuplicatedNoNeed", "Vx": "DuplicatedNoNeed", "Vy": "DuplicatedNoNeed"}
     assert grad_p._derivative("Reverse", annots)._gather is not None
-    assert compute_v._derivative("Reverse", annots)._gather is None
-    for k, explicit in ((grad_p, False), (grad_p, True), (compute_v, False)):
+    assert compute_v._derivative("Reverse", annots)._gather is not None  # in-place update: the gather form applies
+    for k, explicit in ((grad_p, False), (grad_p, True), (compute_v, False), (compute_v, True)):
         print(f"ad: staggered {k.name} explicit_ranges={explicit}", flush=True)
         Vx, Vy, P = _dev(np.zeros((nx + 1, ny))), _dev(np.zeros((nx, ny + 1))), _dev(hP)
         Vxb, Vyb, Pb = _dev(yx), _dev(yy), _dev(np.zeros((nx, ny)))
