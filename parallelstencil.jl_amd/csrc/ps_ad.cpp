@@ -516,35 +516,54 @@ class Ad {
   }
 
   // ---- reverse mode --------------------------------------------------------------------------
+  // Conditions of the selections (ifelse, ?:, max, min) whose branch is being differentiated:
+  // a contribution made inside a branch is evaluated only where the branch is taken, as in the
+  // primal (its coefficients may read elements or take values that exist only there).
+  std::vector<NodeP> guards;
+  NodeP guard_cond() const {
+    NodeP c;
+    for (auto& g : guards) c = c ? mk(NK::And, "&&", {c, clone(g)}) : clone(g);
+    return c;
+  }
+  NodeP guarded(const NodeP& v) const {
+    if (guards.empty() || is_num(v, 0)) return v;
+    return call("ifelse", {guard_cond(), v, fnum(0.0)});
+  }
   NodeP share(const NodeP& v, const NodeP& out) {
     if (v->k == NK::Ident || v->k == NK::Num || no_temps) return v;
     std::string t = fresh("a");
-    out->a.push_back(assign(ident(t), v));
+    out->a.push_back(assign(ident(t), guarded(v)));
     return ident(t);
   }
   // Contributions of one statement to shadow elements, summed per element before they are
   // added (one atomic per element read, however often the statement reads it).
   struct Pending {
-    std::string key, array;  // shadow element as text, the duplicated array
-    NodeP target, sum;
+    std::string key, array;  // shadow element and branch conditions as text, the duplicated array
+    NodeP target, sum, cond;  // cond: the branch conditions of the read (null outside selections)
+    NodeP value() const { return cond ? call("ifelse", {clone(cond), sum, fnum(0.0)}) : sum; }
   };
   std::vector<Pending> pending;
   void accumulate(const NodeP& r, const NodeP& v, const NodeP&) {
     std::vector<NodeP> ia{ident(shadow.at(r->a[0]->s))};
     for (size_t i = 1; i < r->a.size(); ++i) ia.push_back(clone(r->a[i]));
     NodeP target = mk(NK::Index, "", ia);
-    const std::string key = to_julia(target);
+    NodeP c = guard_cond();
+    const std::string key = to_julia(target) + "|" + (c ? to_julia(c) : "");
     for (auto& p : pending)
       if (p.key == key) {
         p.sum = add(p.sum, v);
         return;
       }
-    pending.push_back({key, r->a[0]->s, target, v});
+    pending.push_back({key, r->a[0]->s, target, v, c});
   }
+  // Scatter form: one addition per element and branch condition; a read made inside a branch
+  // adds only where the branch is taken (its element may not exist elsewhere).
   void flush(const NodeP& out) {
     for (auto& p : pending) {
-      if (private_shadow.count(p.array)) out->a.push_back(assign(p.target, add(clone(p.target), p.sum)));
-      else out->a.push_back(mk(NK::Macro, "ps_atomic_add", {p.target, p.sum}));
+      NodeP st = private_shadow.count(p.array) ? assign(p.target, add(clone(p.target), p.sum))
+                                               : mk(NK::Macro, "ps_atomic_add", {p.target, p.sum});
+      if (p.cond) out->a.push_back(mk(NK::If, "", {p.cond, mk(NK::Block, "", {st}), nullptr}));
+      else out->a.push_back(st);
     }
     pending.clear();
   }
@@ -552,18 +571,21 @@ class Ad {
   void back(const NodeP& e, NodeP g, const NodeP& out) {
     if (!dep(e)) return;
     if (e->k == NK::Ident) {
-      out->a.push_back(assign(ident(adj(e->s)), add(ident(adj(e->s)), g)));
+      out->a.push_back(assign(ident(adj(e->s)), add(ident(adj(e->s)), guarded(g))));
       return;
     }
     if (e->k == NK::Index) {
-      accumulate(e, g, out);
+      accumulate(e, g, out);  // under the branch conditions (the element itself may exist only there)
       return;
     }
     NodeP c, a, b;
     if (selection(e, c, a, b)) {
       g = share(g, out);
-      back(a, sel(clone(c), g, fnum(0.0)), out);
-      back(b, sel(clone(c), fnum(0.0), g), out);
+      guards.push_back(clone(c));
+      back(a, g, out);
+      guards.back() = mk(NK::Unary, "!", {clone(c)});
+      back(b, g, out);
+      guards.pop_back();
       return;
     }
     auto ps = partials(e);
@@ -864,7 +886,7 @@ class Ad {
             cond = conj(cond, cmp(i, "<", mk(NK::Binary, "+", {ident(r0), ident(rl)})));
           }
           if (w.cond) cond = conj(cond, shift(w.cond, o));
-          auto then = mk(NK::Block, "", {assign(ident(acc), add(ident(acc), shift(p.sum, o)))});
+          auto then = mk(NK::Block, "", {assign(ident(acc), add(ident(acc), shift(p.value(), o)))});
           body->a.push_back(mk(NK::If, "", {cond, then, nullptr}));
         }
         pending.clear();
@@ -898,7 +920,7 @@ class Ad {
         back(w.rhs, ident(g), sink);
         NodeP own = fnum(0.0);
         for (auto& p : pending)
-          if (p.array == W) own = add(own, p.sum);
+          if (p.array == W) own = add(own, p.value());
         pending.clear();
         if (!(own->k == NK::Ident && own->s == g)) {  // a plain in-place update leaves the seed as it is
           if (!is_num(own, 0)) blk->a.push_back(assign(ident(g), clone(seed)));

@@ -24,3 +24,13 @@ def pytest_collection_modifyitems(config, items):
     for it in items:
         if "gpu" in it.keywords:
             it.add_marker(skip)
+
+
+@pytest.fixture(autouse=True)
+def _device_errors_stay_with_their_test(request):
+    """A GPU test ends with a device synchronisation, so that an error flagged by one of its
+    kernels (a suppressed out-of-bounds access) is reported by that test, not by a later one."""
+    yield
+    if "gpu" in request.keywords and _has_gpu():
+        import parallelstencil as ps
+        ps.synchronize()

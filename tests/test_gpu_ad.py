@@ -479,3 +479,38 @@ def test_branch_local_gather_matches_complex_step():
         gU, gV = O.complex_step_vjp(ref, [U, V], [0, 1], y)
         _close(Ub.to_host(), gU)
         _close(Vb.to_host(), gV)
+
+
+def test_untaken_branch_derivatives_stay_out():
+    """The derivative of the untaken branch of a selection is not evaluated: sqrt of a negative
+    value there would make 0 * NaN = NaN, and a neighbour read at the array's edge an
+    out-of-bounds access (the fixture's synchronisation after each test would report it)."""
+    _init(".nan", ps.Float64, 1)
+    k = ps.core._define("parallel_indices", """
+    @parallel_indices (ix) function h!(R, X)
+        R[ix] = X[ix] > 0.0 ? sqrt(X[ix]) : (ix > 1 ? X[ix-1]*X[ix] : 0.0)
+        return
+    end""", {}, MOD + ".nan")
+    hX = np.array([-1.0, 4.0, -2.0, 9.0, -0.5, 0.25])
+    y = np.arange(1.0, 7.0)
+    want = np.zeros(6)
+    for i in range(6):  # by hand: d sqrt(x) = 1/(2 sqrt x); d (x[i-1] x[i]) by the product rule
+        if hX[i] > 0:
+            want[i] += y[i] * 0.5 / np.sqrt(hX[i])
+        elif i > 0:
+            want[i - 1] += y[i] * hX[i]
+            want[i] += y[i] * hX[i - 1]
+    for form in ("gather", "scatter"):
+        d = k._derivative("Reverse", {"R": "DuplicatedNoNeed", "X": "DuplicatedNoNeed"})
+        d.use_gather = form == "gather"
+        R, X, Rb, Xb = _dev(np.zeros(6)), _dev(hX), _dev(y), _dev(np.zeros(6))
+        k(R, X, grad=[(R, Rb), (X, Xb)])
+        d.use_gather = True
+        ps.synchronize()
+        assert np.all(np.isfinite(Xb.to_host()))
+        _close(Xb.to_host(), want)
+    R, X, Rt, Xt = _dev(np.zeros(6)), _dev(hX), _dev(np.zeros(6)), _dev(np.ones(6))
+    k(R, X, grad=[(R, Rt), (X, Xt)], ad_mode=ps.Forward)
+    t = Rt.to_host()
+    assert np.all(np.isfinite(t))
+    _close(t, np.where(hX > 0, 0.5 / np.sqrt(np.abs(hX)), np.r_[0.0, hX[:-1] + hX[1:]]))
