@@ -514,3 +514,44 @@ def test_untaken_branch_derivatives_stay_out():
     t = Rt.to_host()
     assert np.all(np.isfinite(t))
     _close(t, np.where(hX > 0, 0.5 / np.sqrt(np.abs(hX)), np.r_[0.0, hX[:-1] + hX[1:]]))
+
+
+def test_memopt_parallel_indices_reverse_both_forms():
+    """A memopt @parallel_indices 3-D kernel (stencil offsets on both sides, a variable
+    coefficient): its gather kernel is a memopt kernel with launch-box guards on the streamed
+    dimension. Against the dense Jacobian of a numpy restatement (linear in A)."""
+    _init(".mo", ps.Float64, 3)
+    k = ps.core._define("parallel_indices", """
+    @parallel_indices (ix,iy,iz) memopt=true function st!(A2, A, B)
+        if ix > 1 && ix < size(A,1) && iy > 1 && iy < size(A,2) && iz > 1 && iz < size(A,3)
+            A2[ix,iy,iz] = A[ix,iy,iz+1] - 2.0*A[ix,iy,iz] + A[ix,iy,iz-1] + B[ix,iy,iz]*A[ix+1,iy,iz] - A[ix,iy-1,iz]
+        end
+        return
+    end""", {}, MOD + ".mo")
+    shape = (16, 8, 16)
+    rng = np.random.default_rng(71)
+    hB = np.asfortranarray(rng.standard_normal(shape))
+    y = np.asfortranarray(rng.standard_normal(shape))
+
+    def st(a):
+        A = a.reshape(shape, order="F")
+        out = np.zeros(shape)
+        s = (slice(1, -1),) * 3
+        out[s] = (A[1:-1, 1:-1, 2:] - 2.0 * A[1:-1, 1:-1, 1:-1] + A[1:-1, 1:-1, :-2] + hB[s] * A[2:, 1:-1, 1:-1]
+                  - A[1:-1, :-2, 1:-1])
+        return out
+
+    J = O.linear_jacobian(st, int(np.prod(shape)))
+    inner = np.zeros(shape, dtype=bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    want = (J.T @ np.where(inner, y, 0.0).ravel(order="F")).reshape(shape, order="F")
+    d = k._derivative("Reverse", {"A2": "DuplicatedNoNeed", "A": "DuplicatedNoNeed"})
+    assert d._gather is not None and d._gather[0].info["memopt"]
+    for form in ("gather", "scatter"):
+        d.use_gather = form == "gather"
+        A2, A, B = _dev(np.zeros(shape)), _dev(np.zeros(shape)), _dev(hB)
+        A2b, Ab = _dev(y), _dev(np.zeros(shape))
+        k(A2, A, B, grad=[(A2, A2b), (A, Ab)])
+        d.use_gather = True
+        _close(Ab.to_host(), want)
+        assert np.array_equal(A2b.to_host(), np.where(inner, 0.0, y))
