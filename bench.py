@@ -483,6 +483,51 @@ def ad_study(args):
     print(json.dumps(out), flush=True)
 
 
+def stokes3d_study(args):
+    """miniapps/Stokes3D.jl at its own configuration (127^3, inclusion viscosity 0.1, tolerance
+    1e-6 checked every 500 iterations): time per PT iteration, eager and as graph replays, T_eff
+    with the miniapp's A_eff = 8 n^3 x 8 B per iteration (:160), and the time to tolerance."""
+    import parallelstencil as ps
+    from parallelstencil import workloads as W
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3)
+    n = args.stokes3d_n
+    Mus, Rog = W.stokes3d_material(n, n, n)
+    a_eff = 8 * 8 * n ** 3
+    runs = {}
+    for graph in [m == "graph" for m in args.stokes3d_modes.split(",")]:
+        S = W.Stokes3D(__name__, n, n, n, graph=graph)
+        S.set_material(Mus, Rog)
+        S.reset()
+        S.iterate_block()
+        ps.synchronize()
+        t0 = time.perf_counter()
+        S.iterate_block()
+        ps.synchronize()
+        ms_it = (time.perf_counter() - t0) * 1e3 / S.nout
+        it, err, t_tol = None, None, None
+        if args.stokes3d_solve:
+            S.reset()
+            ps.synchronize()
+            t0 = time.perf_counter()
+            it, err = S.solve()
+            ps.synchronize()
+            t_tol = round(time.perf_counter() - t0, 4)
+        key = "graph" if graph else "eager"
+        runs[key] = {"ms_per_iteration": round(ms_it, 5), "T_eff_gbs": round(a_eff / (ms_it * 1e-3) / 1e9, 2),
+                     "iterations": it, "error": err, "s_to_tol": t_tol}
+        log(f"stokes3d {n}^3 {key}: {ms_it:.4f} ms/iteration, T_eff {a_eff / ms_it / 1e6:.1f} GB/s, "
+            f"{it} iterations to err {err} in {t_tol} s")
+        del S
+    head = runs.get("graph", runs.get("eager"))
+    out = {"metric": "Stokes3D PT iteration T_eff GB/s (miniapp A_eff = 8 n^3 x 8 B per iteration)", "value": head["T_eff_gbs"],
+           "unit": "GB/s", "n_gpus": 1, "steps": 500, "warmup": 500, "ms_per_step": head["ms_per_iteration"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"Stokes3D {n}^3 FP64, spherical inclusion (viscosity 0.1, density 1), eps 1e-6, nout 500"},
+           "roofline": {"bound": "hbm", "achieved": head["T_eff_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(head["T_eff_gbs"] / HBM_PEAK_GBS, 4), "traffic": None}, "runs": runs}
+    print(json.dumps(out), flush=True)
+
+
 def hidecomm_study(args):
     """examples/diffusion3D_multigpucpu_hidecomm_novis.jl on one GPU at n^3 FP64 (periodic in
     every dimension, so that update_halo! exchanges with the process itself): the example's loop
@@ -552,7 +597,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad | hidecomm")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes3d")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--adjoint-n", type=int, default=2048)
@@ -562,6 +607,9 @@ def main():
     ap.add_argument("--eval", type=int, default=24, help="stokes: held-out samples")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--shmem-n", type=int, default=16384)
+    ap.add_argument("--stokes3d-n", type=int, default=127)
+    ap.add_argument("--stokes3d-solve", type=int, default=1, help="stokes3d: 0 skips the solve to tolerance")
+    ap.add_argument("--stokes3d-modes", default="eager,graph", help="stokes3d: launch modes measured (eager, graph)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built", file=sys.stderr)
@@ -589,6 +637,9 @@ def main():
         return
     if args.workload == "hidecomm":
         hidecomm_study(args)
+        return
+    if args.workload == "stokes3d":
+        stokes3d_study(args)
         return
 
     import parallelstencil as ps

@@ -35,6 +35,14 @@ that parity is not anchored on a formula merely retyped:
                     kernel text; a uniform velocity field gives zero strain rates and
                     zero divergence; residual definition of miniapps/Stokes2D.jl:117-123.
   bc_x/bc_y         KAT on an index-valued array.
+  stokes3d_*        reduction to the 2-D C restatement: fields invariant along one axis,
+                    with no velocity along it, must give on each of the three planes
+                    (x-z, y-z, x-y) exactly what stokes2d_* gives on the plane's axes,
+                    through three iterations with the free-slip boundaries, while the
+                    off-plane stresses, residual and velocity stay zero (every term of
+                    every statement is active on one plane; a swapped spacing, dropped
+                    term, wrong damping or missing boundary launch fails, checked by
+                    planted mistakes); a uniform velocity gives no strain.
   diffusion2d_shmem_step  a linear field is stationary (flux differences are exactly 0);
                     single-cell brute force in Python floats (diffusion2d_shmem_cell);
                     a wrong sign or swapped neighbour changes a quadratic field's
@@ -201,6 +209,75 @@ def mean_abs(A):
     L.pso_mean_abs.restype = D
     a = np.asfortranarray(A, dtype=np.float64)
     return L.pso_mean_abs(ctypes.c_int64(a.size), _f(a))
+
+
+# ------------------------------------------------- Stokes3D (numpy, over the pinned fd3 helpers)
+def stokes3d_timesteps(dtVx, dtVy, dtVz, dtPt, Mus, Vsc, Ptsc, min_dxyz2, max_nxyz):
+    """compute_timesteps! of miniapps/Stokes3D.jl:11-17."""
+    dtVx[...] = Vsc * min_dxyz2 / fd3("av_xi", Mus) / 6.1
+    dtVy[...] = Vsc * min_dxyz2 / fd3("av_yi", Mus) / 6.1
+    dtVz[...] = Vsc * min_dxyz2 / fd3("av_zi", Mus) / 6.1
+    dtPt[...] = Ptsc * 6.1 * Mus / float(max_nxyz)
+
+
+def stokes3d_p(divV, Pt, Vx, Vy, Vz, dtPt, dx, dy, dz):
+    """compute_P! of miniapps/Stokes3D.jl:19-23."""
+    divV[...] = fd3("d_xa", Vx) / dx + fd3("d_ya", Vy) / dy + fd3("d_za", Vz) / dz
+    Pt[...] = Pt - dtPt * divV
+
+
+def stokes3d_tau(divV, txx, tyy, tzz, txy, txz, tyz, Vx, Vy, Vz, Mus, dx, dy, dz):
+    """compute_τ! of miniapps/Stokes3D.jl:25-33."""
+    third = 1.0 / 3.0
+    txx[...] = 2.0 * fd3("inn_yz", Mus) * (fd3("d_xi", Vx) / dx - third * fd3("inn_yz", divV))
+    tyy[...] = 2.0 * fd3("inn_xz", Mus) * (fd3("d_yi", Vy) / dy - third * fd3("inn_xz", divV))
+    tzz[...] = 2.0 * fd3("inn_xy", Mus) * (fd3("d_zi", Vz) / dz - third * fd3("inn_xy", divV))
+    txy[...] = 2.0 * fd3("av_xyi", Mus) * (0.5 * (fd3("d_yi", Vx) / dy + fd3("d_xi", Vy) / dx))
+    txz[...] = 2.0 * fd3("av_xzi", Mus) * (0.5 * (fd3("d_zi", Vx) / dz + fd3("d_xi", Vz) / dx))
+    tyz[...] = 2.0 * fd3("av_yzi", Mus) * (0.5 * (fd3("d_zi", Vy) / dz + fd3("d_yi", Vz) / dy))
+
+
+def stokes3d_dv(Rx, Ry, Rz, dVxdt, dVydt, dVzdt, Pt, Rog, txx, tyy, tzz, txy, txz, tyz, dampX, dampY, dampZ, dx, dy, dz):
+    """compute_dV! of miniapps/Stokes3D.jl:35-43."""
+    Rx[...] = fd3("d_xa", txx) / dx + fd3("d_ya", txy) / dy + fd3("d_za", txz) / dz - fd3("d_xi", Pt) / dx
+    Ry[...] = fd3("d_ya", tyy) / dy + fd3("d_xa", txy) / dx + fd3("d_za", tyz) / dz - fd3("d_yi", Pt) / dy
+    Rz[...] = fd3("d_za", tzz) / dz + fd3("d_xa", txz) / dx + fd3("d_ya", tyz) / dy - fd3("d_zi", Pt) / dz + fd3("av_zi", Rog)
+    dVxdt[...] = dampX * dVxdt + Rx
+    dVydt[...] = dampY * dVydt + Ry
+    dVzdt[...] = dampZ * dVzdt + Rz
+
+
+def stokes3d_v(Vx, Vy, Vz, dVxdt, dVydt, dVzdt, dtVx, dtVy, dtVz):
+    """compute_V! of miniapps/Stokes3D.jl:45-50."""
+    inn = (slice(1, -1),) * 3
+    Vx[inn] = Vx[inn] + dtVx * dVxdt
+    Vy[inn] = Vy[inn] + dtVy * dVydt
+    Vz[inn] = Vz[inn] + dtVz * dVzdt
+
+
+def stokes3d_bc(A, axis):
+    """bc_x! / bc_y! / bc_z! of miniapps/Stokes3D.jl:52-68 (axis 0, 1, 2): free slip, the first
+    and last plane copy their inner neighbours."""
+    lo, lo1, hi, hi1 = ([slice(None)] * 3 for _ in range(4))
+    lo[axis], lo1[axis], hi[axis], hi1[axis] = 0, 1, -1, -2
+    A[tuple(lo)] = A[tuple(lo1)]
+    A[tuple(hi)] = A[tuple(hi1)]
+
+
+def stokes3d_iterate(F, prm, n):
+    """n PT iterations (miniapps/Stokes3D.jl:133-142) on a dict of Fortran-ordered float64 arrays
+    named as the fields of the GPU solver (Pt, divV, Vx, ..., dtVz); prm holds dx, dy, dz and
+    dampX, dampY, dampZ."""
+    dx, dy, dz = prm["dx"], prm["dy"], prm["dz"]
+    for _ in range(n):
+        stokes3d_p(F["divV"], F["Pt"], F["Vx"], F["Vy"], F["Vz"], F["dtPt"], dx, dy, dz)
+        stokes3d_tau(F["divV"], F["txx"], F["tyy"], F["tzz"], F["txy"], F["txz"], F["tyz"], F["Vx"], F["Vy"], F["Vz"], F["Mus"],
+                     dx, dy, dz)
+        stokes3d_dv(F["Rx"], F["Ry"], F["Rz"], F["dVxdt"], F["dVydt"], F["dVzdt"], F["Pt"], F["Rog"], F["txx"], F["tyy"],
+                    F["tzz"], F["txy"], F["txz"], F["tyz"], prm["dampX"], prm["dampY"], prm["dampZ"], dx, dy, dz)
+        stokes3d_v(F["Vx"], F["Vy"], F["Vz"], F["dVxdt"], F["dVydt"], F["dVzdt"], F["dtVx"], F["dtVy"], F["dtVz"])
+        for axis, name in ((0, "Vy"), (0, "Vz"), (1, "Vx"), (1, "Vz"), (2, "Vx"), (2, "Vy")):
+            stokes3d_bc(F[name], axis)
 
 
 # ------------------------------------------------- finite-difference KATs

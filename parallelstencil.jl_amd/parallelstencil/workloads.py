@@ -344,6 +344,192 @@ class Stokes2D:
         return it, err
 
 
+# ------------------------------------------------------------------ Stokes3D (miniapps/Stokes3D.jl)
+STOKES3D_TIMESTEPS = """
+@parallel function compute_timesteps!(dτVx::Data.Array, dτVy::Data.Array, dτVz::Data.Array, dτPt::Data.Array, Mus::Data.Array, Vsc::Data.Number, Ptsc::Data.Number, min_dxyz2::Data.Number, max_nxyz::Int)
+    @all(dτVx) = Vsc*min_dxyz2/@av_xi(Mus)/6.1
+    @all(dτVy) = Vsc*min_dxyz2/@av_yi(Mus)/6.1
+    @all(dτVz) = Vsc*min_dxyz2/@av_zi(Mus)/6.1
+    @all(dτPt) = Ptsc*6.1*@all(Mus)/max_nxyz
+    return
+end
+"""
+STOKES3D_P = """
+@parallel function compute_P!(∇V::Data.Array, Pt::Data.Array, Vx::Data.Array, Vy::Data.Array, Vz::Data.Array, dτPt::Data.Array, dx::Data.Number, dy::Data.Number, dz::Data.Number)
+    @all(∇V) = @d_xa(Vx)/dx + @d_ya(Vy)/dy + @d_za(Vz)/dz
+    @all(Pt) = @all(Pt) - @all(dτPt)*@all(∇V)
+    return
+end
+"""
+STOKES3D_TAU = """
+@parallel function compute_τ!(∇V::Data.Array, τxx::Data.Array, τyy::Data.Array, τzz::Data.Array, τxy::Data.Array, τxz::Data.Array, τyz::Data.Array, Vx::Data.Array, Vy::Data.Array, Vz::Data.Array, Mus::Data.Array, dx::Data.Number, dy::Data.Number, dz::Data.Number)
+    @all(τxx) = 2.0*@inn_yz(Mus)*(@d_xi(Vx)/dx - 1.0/3.0*@inn_yz(∇V))
+    @all(τyy) = 2.0*@inn_xz(Mus)*(@d_yi(Vy)/dy - 1.0/3.0*@inn_xz(∇V))
+    @all(τzz) = 2.0*@inn_xy(Mus)*(@d_zi(Vz)/dz - 1.0/3.0*@inn_xy(∇V))
+    @all(τxy) = 2.0*@av_xyi(Mus)*(0.5*(@d_yi(Vx)/dy + @d_xi(Vy)/dx))
+    @all(τxz) = 2.0*@av_xzi(Mus)*(0.5*(@d_zi(Vx)/dz + @d_xi(Vz)/dx))
+    @all(τyz) = 2.0*@av_yzi(Mus)*(0.5*(@d_zi(Vy)/dz + @d_yi(Vz)/dy))
+    return
+end
+"""
+STOKES3D_DV = """
+@parallel function compute_dV!(Rx::Data.Array, Ry::Data.Array, Rz::Data.Array, dVxdτ::Data.Array, dVydτ::Data.Array, dVzdτ::Data.Array, Pt::Data.Array, Rog::Data.Array, τxx::Data.Array, τyy::Data.Array, τzz::Data.Array, τxy::Data.Array, τxz::Data.Array, τyz::Data.Array, dampX::Data.Number, dampY::Data.Number, dampZ::Data.Number, dx::Data.Number, dy::Data.Number, dz::Data.Number)
+    @all(Rx) = @d_xa(τxx)/dx + @d_ya(τxy)/dy + @d_za(τxz)/dz - @d_xi(Pt)/dx
+    @all(Ry) = @d_ya(τyy)/dy + @d_xa(τxy)/dx + @d_za(τyz)/dz - @d_yi(Pt)/dy
+    @all(Rz) = @d_za(τzz)/dz + @d_xa(τxz)/dx + @d_ya(τyz)/dy - @d_zi(Pt)/dz + @av_zi(Rog)
+    @all(dVxdτ) = dampX*@all(dVxdτ) + @all(Rx)
+    @all(dVydτ) = dampY*@all(dVydτ) + @all(Ry)
+    @all(dVzdτ) = dampZ*@all(dVzdτ) + @all(Rz)
+    return
+end
+"""
+STOKES3D_V = """
+@parallel function compute_V!(Vx::Data.Array, Vy::Data.Array, Vz::Data.Array, dVxdτ::Data.Array, dVydτ::Data.Array, dVzdτ::Data.Array, dτVx::Data.Array, dτVy::Data.Array, dτVz::Data.Array)
+    @inn(Vx) = @inn(Vx) + @all(dτVx)*@all(dVxdτ)
+    @inn(Vy) = @inn(Vy) + @all(dτVy)*@all(dVydτ)
+    @inn(Vz) = @inn(Vz) + @all(dτVz)*@all(dVzdτ)
+    return
+end
+"""
+STOKES3D_BC = {
+    "x": """
+@parallel_indices (iy,iz) function bc_x!(A::Data.Array)
+    A[1  , iy, iz] = A[2    , iy, iz]
+    A[end, iy, iz] = A[end-1, iy, iz]
+    return
+end
+""",
+    "y": """
+@parallel_indices (ix,iz) function bc_y!(A::Data.Array)
+    A[ix, 1  , iz] = A[ix, 2    , iz]
+    A[ix, end, iz] = A[ix, end-1, iz]
+    return
+end
+""",
+    "z": """
+@parallel_indices (ix,iy) function bc_z!(A::Data.Array)
+    A[ix, iy, 1  ] = A[ix, iy, 2    ]
+    A[ix, iy, end] = A[ix, iy, end-1]
+    return
+end
+""",
+}
+
+
+def stokes3d_material(nx, ny, nz, lx=10.0, ly=10.0, lz=10.0, mus0=1.0, musi=0.1, rogi=1.0):
+    """Viscosity and density*gravity of miniapps/Stokes3D.jl:110-115: a sphere of radius 1 at the
+    centre of the box (squared radius from ((ix-1)dx - lx/2)^2 + ..., summed left to right)."""
+    import numpy as np
+    dx, dy, dz = lx / (nx - 1), ly / (ny - 1), lz / (nz - 1)
+    x = (np.arange(nx) * dx - 0.5 * lx) ** 2
+    y = (np.arange(ny) * dy - 0.5 * ly) ** 2
+    z = (np.arange(nz) * dz - 0.5 * lz) ** 2
+    Radc = np.asfortranarray((x[:, None, None] + y[None, :, None]) + z[None, None, :])
+    Mus = np.asfortranarray(np.full((nx, ny, nz), mus0))
+    Rog = np.zeros((nx, ny, nz), order="F")
+    Mus[Radc <= 1.0] = musi
+    Rog[Radc <= 1.0] = rogi
+    return Mus, Rog
+
+
+class Stokes3D:
+    """The pseudo-transient Stokes solver of miniapps/Stokes3D.jl:71-143 on the GPU: compute_P!,
+    compute_τ!, compute_dV!, compute_V! and the six free-slip boundary launches per iteration
+    (:133-142), prepared once and issued asynchronously; the residual check every nout
+    iterations (:144-147) is max(mean |Rx|, mean |Ry|, mean |Rz|, mean |∇V|), a deterministic FP64
+    reduction on the GPU. `graph=True` replays each block of nout iterations as one HIP graph."""
+
+    def __init__(self, module, nx, ny, nz, lx=10.0, ly=10.0, lz=10.0, Vdmp=4.0, Vsc=1.0, Ptsc=0.25, eps=1e-6, nout=500,
+                 itermax=10000, graph=True):
+        self.n = (nx, ny, nz)
+        self.dx, self.dy, self.dz = lx / (nx - 1), ly / (ny - 1), lz / (nz - 1)
+        self.min_dxyz2 = min(self.dx, self.dy, self.dz) ** 2
+        self.max_nxyz = max(nx, ny, nz)
+        self.dampX, self.dampY, self.dampZ = 1.0 - Vdmp / nx, 1.0 - Vdmp / ny, 1.0 - Vdmp / nz
+        self.Vsc, self.Ptsc, self.eps, self.nout, self.itermax, self.use_graph = Vsc, Ptsc, eps, nout, itermax, graph
+        d = lambda text, kind="parallel": core._define(kind, text, {}, module)  # noqa: E731
+        self.k_ts, self.k_p, self.k_tau = d(STOKES3D_TIMESTEPS), d(STOKES3D_P), d(STOKES3D_TAU)
+        self.k_dv, self.k_v = d(STOKES3D_DV), d(STOKES3D_V)
+        self.k_bc = {a: d(t, "parallel_indices") for a, t in STOKES3D_BC.items()}
+        z = lambda *s: core.zeros(*s, eltype=core.Float64)  # noqa: E731
+        self.Pt, self.dtPt, self.divV = z(nx, ny, nz), z(nx, ny, nz), z(nx, ny, nz)
+        self.Vx, self.Vy, self.Vz = z(nx + 1, ny, nz), z(nx, ny + 1, nz), z(nx, ny, nz + 1)
+        self.txx, self.tyy, self.tzz = z(nx, ny - 2, nz - 2), z(nx - 2, ny, nz - 2), z(nx - 2, ny - 2, nz)
+        self.txy, self.txz, self.tyz = z(nx - 1, ny - 1, nz - 2), z(nx - 1, ny - 2, nz - 1), z(nx - 2, ny - 1, nz - 1)
+        sx, sy, sz = (nx - 1, ny - 2, nz - 2), (nx - 2, ny - 1, nz - 2), (nx - 2, ny - 2, nz - 1)
+        self.Rx, self.Ry, self.Rz = z(*sx), z(*sy), z(*sz)
+        self.dVxdt, self.dVydt, self.dVzdt = z(*sx), z(*sy), z(*sz)
+        self.dtVx, self.dtVy, self.dtVz = z(*sx), z(*sy), z(*sz)
+        self.Mus, self.Rog = z(nx, ny, nz), z(nx, ny, nz)
+        s = self
+        bc = lambda axis, A, r: self.k_bc[axis].launcher(A, ranges=r)  # noqa: E731
+        self._steps = [
+            s.k_p.launcher(s.divV, s.Pt, s.Vx, s.Vy, s.Vz, s.dtPt, s.dx, s.dy, s.dz),
+            s.k_tau.launcher(s.divV, s.txx, s.tyy, s.tzz, s.txy, s.txz, s.tyz, s.Vx, s.Vy, s.Vz, s.Mus, s.dx, s.dy, s.dz),
+            s.k_dv.launcher(s.Rx, s.Ry, s.Rz, s.dVxdt, s.dVydt, s.dVzdt, s.Pt, s.Rog, s.txx, s.tyy, s.tzz, s.txy, s.txz, s.tyz,
+                            s.dampX, s.dampY, s.dampZ, s.dx, s.dy, s.dz),
+            s.k_v.launcher(s.Vx, s.Vy, s.Vz, s.dVxdt, s.dVydt, s.dVzdt, s.dtVx, s.dtVy, s.dtVz),
+            bc("x", s.Vy, ((1, ny + 1), (1, nz))),
+            bc("x", s.Vz, ((1, ny), (1, nz + 1))),
+            bc("y", s.Vx, ((1, nx + 1), (1, nz))),
+            bc("y", s.Vz, ((1, nx), (1, nz + 1))),
+            bc("z", s.Vx, ((1, nx + 1), (1, ny))),
+            bc("z", s.Vy, ((1, nx), (1, ny + 1))),
+        ]
+        self._graph = None
+        self.iterate(1)  # compile and load every kernel (zeros map to zeros)
+        core.synchronize()
+
+    def fields(self):
+        """Name -> device array, for comparisons and checkpoints."""
+        return {k: v for k, v in vars(self).items() if isinstance(v, core.DeviceArray)}
+
+    def set_material(self, Mus, Rog):
+        self.Mus.copy_from(Mus)
+        self.Rog.copy_from(Rog)
+        self.k_ts(self.dtVx, self.dtVy, self.dtVz, self.dtPt, self.Mus, self.Vsc, self.Ptsc, self.min_dxyz2, self.max_nxyz)
+
+    def reset(self):
+        for A in (self.Pt, self.divV, self.Vx, self.Vy, self.Vz, self.txx, self.tyy, self.tzz, self.txy, self.txz, self.tyz,
+                  self.Rx, self.Ry, self.Rz, self.dVxdt, self.dVydt, self.dVzdt):
+            A.fill_(0.0)
+
+    def iterate(self, n):
+        """n PT iterations (miniapps/Stokes3D.jl:133-142), asynchronous."""
+        steps = self._steps
+        for _ in range(n):
+            for st in steps:
+                st()
+
+    def iterate_block(self):
+        if not self.use_graph:
+            self.iterate(self.nout)
+            return
+        if self._graph is None:
+            core.synchronize()
+            g = core.Graph()
+            with g:
+                self.iterate(self.nout)
+            self._graph = g
+        self._graph.launch()
+
+    def error(self):
+        """max(mean |Rx|, mean |Ry|, mean |Rz|, mean |∇V|) (miniapps/Stokes3D.jl:144-147)."""
+        return max(A.reduce("mean_abs") for A in (self.Rx, self.Ry, self.Rz, self.divV))
+
+    def solve(self):
+        """Iterates until the check every nout iterations is at or below eps (or itermax);
+        returns (iterations, error)."""
+        it, err = 0, float("inf")
+        while it < self.itermax:
+            self.iterate_block()
+            it += self.nout
+            err = self.error()
+            if err <= self.eps or not err < 1e30:
+                break
+        return it, err
+
+
 # ------------------------------------------------------------------ acoustic waves
 def acoustic3d_inputs(n, lx=40.0, k=1.0, rho=1.0):
     """Initial state of miniapps/acoustic3D.jl:24-40 on n^3 pressure nodes (dx = lx/(n-1)):

@@ -288,6 +288,103 @@ def test_mean_abs():
     assert O.mean_abs(a) == 2.5
 
 
+# ------------------------------------------------- Stokes3D: reduction to Stokes2D on a plane
+def _stokes3d_arrays(nx, ny, nz):
+    z = lambda *s: F(np.zeros(s))  # noqa: E731
+    sx, sy, sz = (nx - 1, ny - 2, nz - 2), (nx - 2, ny - 1, nz - 2), (nx - 2, ny - 2, nz - 1)
+    return dict(Pt=z(nx, ny, nz), dtPt=z(nx, ny, nz), divV=z(nx, ny, nz), Mus=z(nx, ny, nz), Rog=z(nx, ny, nz),
+                Vx=z(nx + 1, ny, nz), Vy=z(nx, ny + 1, nz), Vz=z(nx, ny, nz + 1),
+                txx=z(nx, ny - 2, nz - 2), tyy=z(nx - 2, ny, nz - 2), tzz=z(nx - 2, ny - 2, nz),
+                txy=z(nx - 1, ny - 1, nz - 2), txz=z(nx - 1, ny - 2, nz - 1), tyz=z(nx - 2, ny - 1, nz - 1),
+                Rx=z(*sx), Ry=z(*sy), Rz=z(*sz), dVxdt=z(*sx), dVydt=z(*sy), dVzdt=z(*sz), dtVx=z(*sx), dtVy=z(*sy), dtVz=z(*sz))
+
+
+def _crop(A, shape):
+    """The centre of A with the given shape (staggered and inner ranges differ by 0 or 2)."""
+    return A[tuple(slice((a - b) // 2, a - (a - b) // 2) for a, b in zip(A.shape, shape))]
+
+
+@pytest.mark.parametrize("plane", [(0, 2), (1, 2), (0, 1)])
+def test_stokes3d_reduces_to_stokes2d_on_a_plane(plane):
+    """Fields invariant along the third axis c, with no velocity along it: every kernel of the 3-D
+    restatement must give, on each plane, what the 2-D C oracle gives on the plane's axes (a, b):
+    a dropped or misplaced term, a wrong stagger or a swapped spacing fails. Iterated with the
+    free-slip boundaries; the off-plane stresses, residual and velocity must stay zero."""
+    a, b = plane
+    c = 3 - a - b
+    X = "xyz"
+    A, B, C = X[a], X[b], X[c]
+    n2 = (9, 7)
+    m = 5
+    n3 = [0, 0, 0]
+    n3[a], n3[b], n3[c] = n2[0], n2[1], m
+    rng = np.random.default_rng(17 + a * 3 + b)
+    s2 = _stokes_arrays(*n2)
+    for k in ("Pt", "Vx", "Vy", "dVxdt", "dVydt"):
+        s2[k][:] = rng.standard_normal(s2[k].shape)
+    Mus2 = F(1.0 + rng.random(n2))
+    Rog2 = F(rng.random(n2)) if b == 2 else F(np.zeros(n2))  # gravity acts along z in 3-D, along y in 2-D
+    s3 = _stokes3d_arrays(*n3)
+
+    def embed(A2, name3):
+        shp = s3[name3].shape
+        s3[name3][...] = np.broadcast_to(np.expand_dims(_crop(A2, (shp[a], shp[b])), c), shp)
+
+    embed(s2["Pt"], "Pt"), embed(Mus2, "Mus"), embed(Rog2, "Rog")
+    embed(s2["Vx"], "V" + A), embed(s2["Vy"], "V" + B)
+    embed(s2["dVxdt"], f"dV{A}dt"), embed(s2["dVydt"], f"dV{B}dt")
+    sp2 = (0.3, 0.4)
+    sp3 = [0.7, 0.7, 0.7]
+    sp3[a], sp3[b] = sp2
+    damp2 = (0.9, 0.8)
+    damp3 = [0.5, 0.5, 0.5]
+    damp3[a], damp3[b] = damp2
+    Vsc, Ptsc, mind2, maxn = 1.0, 0.25, min(sp2) ** 2, max(n3)
+    O.stokes2d_timesteps(s2["dtVx"], s2["dtVy"], s2["dtPt"], Mus2, Vsc, Ptsc, mind2, maxn)
+    O.stokes3d_timesteps(s3["dtVx"], s3["dtVy"], s3["dtVz"], s3["dtPt"], s3["Mus"], Vsc, Ptsc, mind2, maxn)
+    # the 3-D pseudo-time steps use 6.1 where the 2-D ones use 4.1
+    sl = [slice(None)] * 3
+    sl[c] = 0
+    sl = tuple(sl)
+    np.testing.assert_allclose(s3["dtV" + A][sl] * 6.1, _crop(s2["dtVx"], s3["dtV" + A][sl].shape) * 4.1, rtol=1e-14)
+    np.testing.assert_allclose(s3["dtV" + B][sl] * 6.1, _crop(s2["dtVy"], s3["dtV" + B][sl].shape) * 4.1, rtol=1e-14)
+    np.testing.assert_allclose(s3["dtPt"][sl] / 6.1, s2["dtPt"] / 4.1, rtol=1e-14)
+    embed(s2["dtVx"], "dtV" + A), embed(s2["dtVy"], "dtV" + B), embed(s2["dtPt"], "dtPt")
+    prm = dict(dx=sp3[0], dy=sp3[1], dz=sp3[2], dampX=damp3[0], dampY=damp3[1], dampZ=damp3[2])
+    pair = lambda u, v: "t" + "".join(sorted(u + v))  # noqa: E731
+    match = {"Pt": "Pt", "divV": "divV", "V" + A: "Vx", "V" + B: "Vy", "t" + A + A: "txx", "t" + B + B: "tyy",
+             pair(A, B): "txy", "R" + A: "Rx", "R" + B: "Ry", f"dV{A}dt": "dVxdt", f"dV{B}dt": "dVydt"}
+    zero = [pair(A, C), pair(B, C), "R" + C, f"dV{C}dt", "V" + C]
+    bc2 = {(0, 1): O.bc_x, (1, 0): O.bc_y}  # (boundary axis, velocity component) of the 2-D solver
+    for it in range(3):
+        O.stokes2d_p(s2["divV"], s2["Pt"], s2["Vx"], s2["Vy"], s2["dtPt"], *sp2)
+        O.stokes2d_tau(s2["divV"], s2["txx"], s2["tyy"], s2["txy"], s2["Vx"], s2["Vy"], Mus2, *sp2)
+        O.stokes2d_dv(s2["Rx"], s2["Ry"], s2["dVxdt"], s2["dVydt"], s2["Pt"], Rog2, s2["txx"], s2["tyy"], s2["txy"], *damp2, *sp2)
+        O.stokes2d_v(s2["Vx"], s2["Vy"], s2["dVxdt"], s2["dVydt"], s2["dtVx"], s2["dtVy"])
+        bc2[(1, 0)](s2["Vx"])
+        bc2[(0, 1)](s2["Vy"])
+        O.stokes3d_iterate(s3, prm, 1)
+        for n3name, n2name in match.items():
+            V3 = s3[n3name]
+            assert np.all(V3 == V3[sl][tuple(np.newaxis if d == c else slice(None) for d in range(3))]), (it, n3name)
+            np.testing.assert_allclose(V3[sl], _crop(s2[n2name], V3[sl].shape), rtol=1e-13, atol=1e-300,
+                                       err_msg=f"iteration {it}: {n3name} vs 2-D {n2name}")
+        for name in zero:
+            assert not np.any(s3[name]), (it, name)
+
+
+def test_stokes3d_rigid_motion_has_no_strain():
+    """A uniform velocity field: no divergence, no stress, and with a uniform pressure and no
+    buoyancy no residual."""
+    s3 = _stokes3d_arrays(8, 7, 6)
+    s3["Vx"][:], s3["Vy"][:], s3["Vz"][:], s3["Pt"][:] = 0.7, -0.3, 0.2, 1.5
+    s3["Mus"][:] = 1.0 + np.random.default_rng(3).random(s3["Mus"].shape)
+    O.stokes3d_iterate(s3, dict(dx=0.1, dy=0.2, dz=0.3, dampX=0.9, dampY=0.9, dampZ=0.9), 2)
+    for k in ("divV", "txx", "tyy", "tzz", "txy", "txz", "tyz", "Rx", "Ry", "Rz", "dVxdt"):
+        assert not np.any(s3[k]), k
+    assert np.all(s3["Pt"] == 1.5)
+
+
 # ------------------------------------------------- acoustic 2-D adjoint (no reference counterpart)
 def _ac2_case(nx=22, ny=19, nt=50):
     rng = np.random.default_rng(5)
