@@ -528,6 +528,39 @@ def stokes3d_study(args):
     print(json.dumps(out), flush=True)
 
 
+def thermal_study(args):
+    """miniapps/ThermalConvection2D.jl at its own configuration ((96 ar - 1) x 95, Ra 1e7, PT
+    tolerance 1e-4 checked every 100 iterations): PT iterations per second (graph replays of 100
+    iterations, 10 launches each) and the wall time of the first time steps."""
+    import parallelstencil as ps
+    from parallelstencil import workloads as W
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2)
+    S = W.ThermalConvection2D(__name__, graph=True)
+    S._block()
+    ps.synchronize()
+    nb = 20
+    t0 = time.perf_counter()
+    for _ in range(nb):
+        S._block()
+    ps.synchronize()
+    ms_it = (time.perf_counter() - t0) * 1e3 / (nb * S.nerr)
+    S.reset()
+    ps.synchronize()
+    steps = []
+    for _ in range(args.thermal_steps):
+        t0 = time.perf_counter()
+        niter, errV, errP, dt = S.time_step()
+        ps.synchronize()
+        steps.append({"pt_iterations": niter, "errV": errV, "errP": errP, "dt": dt, "s": round(time.perf_counter() - t0, 4)})
+        log(f"thermal step: {niter} PT iterations, errV {errV:.2e} errP {errP:.2e}, dt {dt:.3e}, {steps[-1]['s']} s")
+    out = {"metric": "ThermalConvection2D PT iterations per second", "value": round(1e3 / ms_it, 1), "unit": "iterations/s",
+           "n_gpus": 1, "steps": nb * S.nerr, "warmup": S.nerr, "ms_per_step": round(ms_it, 5), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"ThermalConvection2D {S.nx}x{S.ny} FP64, Ra 1e7, eps 1e-4, nerr 100, graph replays"},
+           "time_steps": steps}
+    print(json.dumps(out), flush=True)
+
+
 def hidecomm_study(args):
     """examples/diffusion3D_multigpucpu_hidecomm_novis.jl on one GPU at n^3 FP64 (periodic in
     every dimension, so that update_halo! exchanges with the process itself): the example's loop
@@ -597,7 +630,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes3d")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes3d | thermal")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--adjoint-n", type=int, default=2048)
@@ -610,6 +643,7 @@ def main():
     ap.add_argument("--stokes3d-n", type=int, default=127)
     ap.add_argument("--stokes3d-solve", type=int, default=1, help="stokes3d: 0 skips the solve to tolerance")
     ap.add_argument("--stokes3d-modes", default="eager,graph", help="stokes3d: launch modes measured (eager, graph)")
+    ap.add_argument("--thermal-steps", type=int, default=3, help="thermal: time steps run to PT tolerance")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built", file=sys.stderr)
@@ -640,6 +674,9 @@ def main():
         return
     if args.workload == "stokes3d":
         stokes3d_study(args)
+        return
+    if args.workload == "thermal":
+        thermal_study(args)
         return
 
     import parallelstencil as ps

@@ -35,6 +35,14 @@ that parity is not anchored on a formula merely retyped:
                     kernel text; a uniform velocity field gives zero strain rates and
                     zero divergence; residual definition of miniapps/Stokes2D.jl:117-123.
   bc_x/bc_y         KAT on an index-valued array.
+  thermal2d_0/1/2/v the Stokes part of ThermalConvection2D against stokes2d_* (exact:
+                    with rho = dtau_iter = 1 compute_1!/compute_2!/update_V! are compute_τ!,
+                    compute_P! with dτPt = dτ_iter/β, compute_dV!, compute_V!; rho = 2 halves
+                    the residuals exactly); Eta = eta0 exactly where T = -dT/2.
+  thermal2d_qt/advect  closed forms on integer grids: T = x^2 gives the backward and forward
+                    differences 2x-1 and 2x+1, so the upwind choice for either sign of the
+                    velocity is visible; the flux divergence of T = x^2 + y^2 is -4 DcT
+                    exactly; Julia's strong zero (false * Inf == 0).
   stokes3d_*        reduction to the 2-D C restatement: fields invariant along one axis,
                     with no velocity along it, must give on each of the three planes
                     (x-z, y-z, x-y) exactly what stokes2d_* gives on the plane's axes,
@@ -278,6 +286,114 @@ def stokes3d_iterate(F, prm, n):
         stokes3d_v(F["Vx"], F["Vy"], F["Vz"], F["dVxdt"], F["dVydt"], F["dVzdt"], F["dtVx"], F["dtVy"], F["dtVz"])
         for axis, name in ((0, "Vy"), (0, "Vz"), (1, "Vx"), (1, "Vz"), (2, "Vx"), (2, "Vy")):
             stokes3d_bc(F[name], axis)
+
+
+# ------------------------------------------------- ThermalConvection2D (numpy, over the pinned fd2 helpers)
+def julia_bool_mul(b, x):
+    """Julia's Bool * x for floats: x where b, else copysign(0, x) (a strong zero)."""
+    return np.where(b, x, np.copysign(0.0, x))
+
+
+def thermal2d_0(RogT, Eta, divV, T, Vx, Vy, rho0ga, eta0, deta_dT, dT, dx, dy):
+    """compute_0! of miniapps/ThermalConvection2D.jl:21-26."""
+    RogT[...] = rho0ga * T
+    Eta[...] = eta0 * (1.0 - deta_dT * (T + dT / 2.0))
+    divV[...] = fd2("d_xa", Vx) / dx + fd2("d_ya", Vy) / dy
+
+
+def thermal2d_1(Pt, txx, tyy, sxy, Eta, divV, Vx, Vy, dtau_iter, beta, dx, dy):
+    """compute_1! of miniapps/ThermalConvection2D.jl:28-34."""
+    Pt[...] = Pt - dtau_iter / beta * divV
+    txx[...] = 2.0 * Eta * (fd2("d_xa", Vx) / dx - 1.0 / 3.0 * divV)
+    tyy[...] = 2.0 * Eta * (fd2("d_ya", Vy) / dy - 1.0 / 3.0 * divV)
+    sxy[...] = 2.0 * fd2("av", Eta) * (0.5 * (fd2("d_yi", Vx) / dy + fd2("d_xi", Vy) / dx))
+
+
+def thermal2d_2(Rx, Ry, dVxdt, dVydt, Pt, RogT, txx, tyy, sxy, rho, dampX, dampY, dtau_iter, dx, dy):
+    """compute_2! of miniapps/ThermalConvection2D.jl:36-42."""
+    Rx[...] = 1.0 / rho * (fd2("d_xi", txx) / dx + fd2("d_ya", sxy) / dy - fd2("d_xi", Pt) / dx)
+    Ry[...] = 1.0 / rho * (fd2("d_yi", tyy) / dy + fd2("d_xa", sxy) / dx - fd2("d_yi", Pt) / dy + fd2("av_yi", RogT))
+    dVxdt[...] = dampX * dVxdt + Rx * dtau_iter
+    dVydt[...] = dampY * dVydt + Ry * dtau_iter
+
+
+def thermal2d_v(Vx, Vy, dVxdt, dVydt, dtau_iter):
+    """update_V! of miniapps/ThermalConvection2D.jl:44-48."""
+    Vx[1:-1, 1:-1] = Vx[1:-1, 1:-1] + dVxdt * dtau_iter
+    Vy[1:-1, 1:-1] = Vy[1:-1, 1:-1] + dVydt * dtau_iter
+
+
+def thermal2d_qt(qTx, qTy, T, DcT, dx, dy):
+    """compute_qT! of miniapps/ThermalConvection2D.jl:50-54."""
+    qTx[...] = -DcT * fd2("d_xi", T) / dx
+    qTy[...] = -DcT * fd2("d_yi", T) / dy
+
+
+def thermal2d_advect(dT_dt, qTx, qTy, T, Vx, Vy, dx, dy):
+    """advect_T! of miniapps/ThermalConvection2D.jl:56-63: flux divergence and first-order
+    upwind advection; dT_dt[i, j] (1-based) belongs to T[i+1, j+1]."""
+    n1, n2 = dT_dt.shape
+    c = (slice(1, n1 + 1), slice(1, n2 + 1))  # T[ix+1, iy+1]
+    xm, xp = (slice(0, n1), slice(1, n2 + 1)), (slice(2, n1 + 2), slice(1, n2 + 1))
+    ym, yp = (slice(1, n1 + 1), slice(0, n2)), (slice(1, n1 + 1), slice(2, n2 + 2))
+    Vxl, Vxr = Vx[1:n1 + 1, 1:n2 + 1], Vx[2:n1 + 2, 1:n2 + 1]
+    Vyl, Vyr = Vy[1:n1 + 1, 1:n2 + 1], Vy[1:n1 + 1, 2:n2 + 2]
+    div = (qTx[1:n1 + 1, :n2] - qTx[:n1, :n2]) / dx + (qTy[:n1, 1:n2 + 1] - qTy[:n1, :n2]) / dy
+    dT_dt[...] = ((((-div) - julia_bool_mul(Vxl > 0, Vxl) * (T[c] - T[xm]) / dx)
+                   - julia_bool_mul(Vxr < 0, Vxr) * (T[xp] - T[c]) / dx)
+                  - julia_bool_mul(Vyl > 0, Vyl) * (T[c] - T[ym]) / dy) - julia_bool_mul(Vyr < 0, Vyr) * (T[yp] - T[c]) / dy
+
+
+def thermal2d_t(T, T_old, dT_dt, dt):
+    """update_T! of miniapps/ThermalConvection2D.jl:65-68."""
+    T[1:-1, 1:-1] = T_old[1:-1, 1:-1] + dT_dt * dt
+
+
+def thermal2d_noflux(T):
+    """no_fluxY_T! of miniapps/ThermalConvection2D.jl:70-74 (the last row first, then the first)."""
+    T[-1, :] = T[-2, :]
+    T[0, :] = T[1, :]
+
+
+def thermal2d_pt_iterate(F, prm, n):
+    """n PT iterations of miniapps/ThermalConvection2D.jl:160-169 on a dict of float64 arrays
+    named as the GPU solver's fields; prm holds the solver's scalars."""
+    p = prm
+    for _ in range(n):
+        F["ErrV"][...] = F["Vy"]
+        F["ErrP"][...] = F["Pt"]
+        thermal2d_0(F["RogT"], F["Eta"], F["divV"], F["T"], F["Vx"], F["Vy"], p["rho0ga"], p["eta0"], p["deta_dT"], p["dT"], p["dx"], p["dy"])
+        thermal2d_1(F["Pt"], F["txx"], F["tyy"], F["sxy"], F["Eta"], F["divV"], F["Vx"], F["Vy"], p["dtau_iter"], p["beta"], p["dx"], p["dy"])
+        thermal2d_2(F["Rx"], F["Ry"], F["dVxdt"], F["dVydt"], F["Pt"], F["RogT"], F["txx"], F["tyy"], F["sxy"], p["rho"], p["dampX"],
+                    p["dampY"], p["dtau_iter"], p["dx"], p["dy"])
+        thermal2d_v(F["Vx"], F["Vy"], F["dVxdt"], F["dVydt"], p["dtau_iter"])
+        F["Vx"][:, 0], F["Vx"][:, -1] = F["Vx"][:, 1], F["Vx"][:, -2]
+        F["Vy"][0, :], F["Vy"][-1, :] = F["Vy"][1, :], F["Vy"][-2, :]
+        F["ErrV"][...] = F["ErrV"] - F["Vy"]
+        F["ErrP"][...] = F["ErrP"] - F["Pt"]
+
+
+def thermal2d_time_step(F, prm):
+    """One time step (miniapps/ThermalConvection2D.jl:155-183): PT iterations in blocks of nerr
+    until errV and errP are at or below eps (or itermax), then the thermal step. Returns
+    (iterations, errV, errP, dt)."""
+    p = prm
+    F["T_old"][...] = F["T"]
+    errV = errP = 2 * p["eps"]
+    niter = 0
+    while (errV > p["eps"] or errP > p["eps"]) and niter < p["itermax"]:
+        thermal2d_pt_iterate(F, p, p["nerr"])
+        niter += p["nerr"]
+        errV = np.abs(F["ErrV"]).max() / (1e-12 + np.abs(F["Vy"]).max())
+        errP = np.abs(F["ErrP"]).max() / (1e-12 + np.abs(F["Pt"]).max())
+    thermal2d_qt(F["qTx"], F["qTy"], F["T"], p["DcT"], p["dx"], p["dy"])
+    thermal2d_advect(F["dT_dt"], F["qTx"], F["qTy"], F["T"], F["Vx"], F["Vy"], p["dx"], p["dy"])
+    with np.errstate(divide="ignore"):
+        dt_adv = min(p["dx"] / np.abs(F["Vx"]).max(), p["dy"] / np.abs(F["Vy"]).max()) / 2.1
+    dt = min(p["dt_diff"], dt_adv)
+    thermal2d_t(F["T"], F["T_old"], F["dT_dt"], dt)
+    thermal2d_noflux(F["T"])
+    return niter, errV, errP, dt
 
 
 # ------------------------------------------------- finite-difference KATs

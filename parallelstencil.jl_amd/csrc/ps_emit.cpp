@@ -530,6 +530,13 @@ EV Emitter::binary(const std::string& op, EV a, EV b, const NodeP& rhsnode) {
     EV x = cast(a, t), y = cast(b, t);
     return EV{std::string(t == Ty::F32 ? "powf(" : "pow(") + x.c + ", " + y.c + ")", t};
   }
+  if (op == "*" && (a.t == Ty::Bool) != (b.t == Ty::Bool) && is_float(a.t == Ty::Bool ? b.t : a.t)) {
+    // Julia's Bool * x = ifelse(b, x, copysign(zero(x), x)): a strong zero (false * Inf == 0)
+    const EV& c = a.t == Ty::Bool ? a : b;
+    const EV& x = a.t == Ty::Bool ? b : a;
+    bool f32 = x.t == Ty::F32;
+    return EV{"(" + c.c + " ? " + x.c + " : " + (f32 ? "copysignf(0.0f, " : "copysign(0.0, ") + x.c + "))", x.t};
+  }
   if (a.t == Ty::Bool) a = cast(a, Ty::Int);
   if (b.t == Ty::Bool) b = cast(b, Ty::Int);
   if (op == "/") {

@@ -530,6 +530,215 @@ class Stokes3D:
         return it, err
 
 
+# ------------------------------------------------------------------ ThermalConvection2D (miniapps/ThermalConvection2D.jl)
+TC2D_ASSIGN = """
+@parallel function assign!(A::Data.Array, B::Data.Array)
+    @all(A) = @all(B)
+    return
+end
+"""
+TC2D_ERROR = """
+@parallel function compute_error!(Err_A::Data.Array, A::Data.Array)
+    @all(Err_A) = @all(Err_A) - @all(A)
+    return
+end
+"""
+TC2D_0 = """
+@parallel function compute_0!(RogT::Data.Array, Eta::Data.Array, ∇V::Data.Array, T::Data.Array, Vx::Data.Array, Vy::Data.Array, ρ0gα::Data.Number, η0::Data.Number, dη_dT::Data.Number, ΔT::Data.Number, dx::Data.Number, dy::Data.Number)
+    @all(RogT) = ρ0gα*@all(T)
+    @all(Eta)  = η0*(1.0 - dη_dT*(@all(T) + ΔT/2.0))
+    @all(∇V)   = @d_xa(Vx)/dx + @d_ya(Vy)/dy
+    return
+end
+"""
+TC2D_1 = """
+@parallel function compute_1!(Pt::Data.Array, τxx::Data.Array, τyy::Data.Array, σxy::Data.Array, Eta::Data.Array, ∇V::Data.Array, Vx::Data.Array, Vy::Data.Array, dτ_iter::Data.Number, β::Data.Number, dx::Data.Number, dy::Data.Number)
+    @all(Pt)  = @all(Pt) - dτ_iter/β*@all(∇V)
+    @all(τxx) = 2.0*@all(Eta)*(@d_xa(Vx)/dx - 1.0/3.0*@all(∇V))
+    @all(τyy) = 2.0*@all(Eta)*(@d_ya(Vy)/dy - 1.0/3.0*@all(∇V))
+    @all(σxy) = 2.0*@av(Eta)*(0.5*(@d_yi(Vx)/dy + @d_xi(Vy)/dx))
+    return
+end
+"""
+TC2D_2 = """
+@parallel function compute_2!(Rx::Data.Array, Ry::Data.Array, dVxdτ::Data.Array, dVydτ::Data.Array, Pt::Data.Array, RogT::Data.Array, τxx::Data.Array, τyy::Data.Array, σxy::Data.Array, ρ::Data.Number, dampX::Data.Number, dampY::Data.Number, dτ_iter::Data.Number, dx::Data.Number, dy::Data.Number)
+    @all(Rx)    = 1.0/ρ*(@d_xi(τxx)/dx + @d_ya(σxy)/dy - @d_xi(Pt)/dx)
+    @all(Ry)    = 1.0/ρ*(@d_yi(τyy)/dy + @d_xa(σxy)/dx - @d_yi(Pt)/dy + @av_yi(RogT))
+    @all(dVxdτ) = dampX*@all(dVxdτ) + @all(Rx)*dτ_iter
+    @all(dVydτ) = dampY*@all(dVydτ) + @all(Ry)*dτ_iter
+    return
+end
+"""
+TC2D_V = """
+@parallel function update_V!(Vx::Data.Array, Vy::Data.Array, dVxdτ::Data.Array, dVydτ::Data.Array, dτ_iter::Data.Number)
+    @inn(Vx) = @inn(Vx) + @all(dVxdτ)*dτ_iter
+    @inn(Vy) = @inn(Vy) + @all(dVydτ)*dτ_iter
+    return
+end
+"""
+TC2D_QT = """
+@parallel function compute_qT!(qTx::Data.Array, qTy::Data.Array, T::Data.Array, DcT::Data.Number, dx::Data.Number, dy::Data.Number)
+    @all(qTx) = -DcT*@d_xi(T)/dx
+    @all(qTy) = -DcT*@d_yi(T)/dy
+    return
+end
+"""
+TC2D_ADVECT = """
+@parallel_indices (ix,iy) function advect_T!(dT_dt::Data.Array, qTx::Data.Array, qTy::Data.Array, T::Data.Array, Vx::Data.Array, Vy::Data.Array, dx::Data.Number, dy::Data.Number)
+    if (ix<=size(dT_dt, 1) && iy<=size(dT_dt, 2)) dT_dt[ix,iy] = -((qTx[ix+1,iy]-qTx[ix,iy])/dx + (qTy[ix,iy+1]-qTy[ix,iy])/dy) -
+                                                                  (Vx[ix+1,iy+1]>0)*Vx[ix+1,iy+1]*(T[ix+1,iy+1]-T[ix  ,iy+1])/dx -
+                                                                  (Vx[ix+2,iy+1]<0)*Vx[ix+2,iy+1]*(T[ix+2,iy+1]-T[ix+1,iy+1])/dx -
+                                                                  (Vy[ix+1,iy+1]>0)*Vy[ix+1,iy+1]*(T[ix+1,iy+1]-T[ix+1,iy  ])/dy -
+                                                                  (Vy[ix+1,iy+2]<0)*Vy[ix+1,iy+2]*(T[ix+1,iy+2]-T[ix+1,iy+1])/dy   end
+    return
+end
+"""
+TC2D_T = """
+@parallel function update_T!(T::Data.Array, T_old::Data.Array, dT_dt::Data.Array, dt::Data.Number)
+    @inn(T) = @inn(T_old) + @all(dT_dt)*dt
+    return
+end
+"""
+TC2D_NOFLUX = """
+@parallel_indices (ix,iy) function no_fluxY_T!(T::Data.Array)
+    if (ix==size(T, 1) && iy<=size(T ,2)) T[ix,iy] = T[ix-1,iy] end
+    if (ix==1          && iy<=size(T ,2)) T[ix,iy] = T[ix+1,iy] end
+    return
+end
+"""
+
+
+def _div_inf(a, b):
+    """a / b with IEEE semantics for b == 0 (Julia's, where dx / 0.0 is Inf)."""
+    return a / b if b != 0.0 else (float("inf") if a > 0 else float("-inf") if a < 0 else float("nan"))
+
+
+def thermal_convection2d_setup(nx=None, ny=None, ar=3, ly=1.0, eta0=1.0, DcT=1.0, dT=1.0, Ra=1e7, Pra=1e3, itermax=5 * 10 ** 4,
+                               nerr=100, eps=1e-4, dmp=2.0):
+    """Scalars and initial temperature of miniapps/ThermalConvection2D.jl:90-124 (grid
+    (96 ar - 1) x 95 by default): returns (prm, T0) with prm holding nx, ny, dx, dy, rho0ga,
+    eta0, deta_dT, dT, DcT, rho, dt_diff, dtau_iter, beta, dampX, dampY, itermax, nerr, eps."""
+    import numpy as np
+    nx = 96 * ar - 1 if nx is None else nx
+    ny = 96 - 1 if ny is None else ny
+    lx = ar * ly
+    w = 1e-2 * ly
+    dx, dy = lx / (nx - 1), ly / (ny - 1)
+    mind = min(dx, dy)
+    rho = 1.0 / Pra * eta0 / DcT
+    dtau_iter = 1.0 / 6.1 * mind / float(np.sqrt(eta0 / rho))
+    prm = dict(nx=nx, ny=ny, dx=dx, dy=dy, rho0ga=Ra * eta0 * DcT / dT / ly ** 3, eta0=eta0, deta_dT=1e-10 / dT, dT=dT, DcT=DcT,
+               rho=rho, dt_diff=1.0 / 4.1 * mind ** 2 / DcT, dtau_iter=dtau_iter, beta=6.1 * dtau_iter ** 2 / mind ** 2 / rho,
+               dampX=1.0 - dmp / nx, dampY=1.0 - dmp / ny, itermax=itermax, nerr=nerr, eps=eps)
+    x = (np.arange(nx) * dx - 0.5 * lx) / w
+    y = (np.arange(ny) * dy - 0.5 * ly) / w
+    T0 = np.asfortranarray(dT * np.exp(-(x * x)[:, None] - (y * y)[None, :]))
+    T0[:, 0] = dT / 2.0
+    T0[:, -1] = -dT / 2.0
+    return prm, T0
+
+
+class ThermalConvection2D:
+    """Thermal convection of miniapps/ThermalConvection2D.jl:88-184 on the GPU: each time step
+    runs the pseudo-transient Stokes iteration (10 launches per iteration, :160-169) until the
+    relative changes errV and errP checked every nerr iterations (:170-174) are at or below eps,
+    then the thermal step (:177-183) with dt = min(dt_diff, dt_adv). The error check and the
+    advective time step use FP64 max-|.| reductions on the GPU. `graph=True` replays each block of
+    nerr iterations as one HIP graph."""
+
+    def __init__(self, module, nx=None, ny=None, graph=True, **physics):
+        prm, T0 = thermal_convection2d_setup(nx, ny, **physics)
+        self.prm, self.T0 = prm, T0
+        for k, v in prm.items():
+            setattr(self, k, v)
+        nx, ny = self.nx, self.ny
+        self.use_graph = graph
+        d = lambda text, kind="parallel": core._define(kind, text, {}, module)  # noqa: E731
+        self.k_assign, self.k_err, self.k_0, self.k_1, self.k_2, self.k_v = (d(t) for t in (TC2D_ASSIGN, TC2D_ERROR, TC2D_0, TC2D_1, TC2D_2, TC2D_V))
+        self.k_qt, self.k_t = d(TC2D_QT), d(TC2D_T)
+        self.k_adv, self.k_noflux = d(TC2D_ADVECT, "parallel_indices"), d(TC2D_NOFLUX, "parallel_indices")
+        self.k_bcx, self.k_bcy = d(STOKES2D_BC_X, "parallel_indices"), d(STOKES2D_BC_Y, "parallel_indices")
+        z = lambda *s: core.zeros(*s, eltype=core.Float64)  # noqa: E731
+        for name in ("T", "T_old", "Pt", "divV", "RogT", "Eta", "txx", "tyy", "ErrP"):
+            setattr(self, name, z(nx, ny))
+        self.Vx, self.Vy, self.ErrV, self.sxy = z(nx + 1, ny), z(nx, ny + 1), z(nx, ny + 1), z(nx - 1, ny - 1)
+        self.Rx, self.Ry, self.dVxdt, self.dVydt = z(nx - 1, ny - 2), z(nx - 2, ny - 1), z(nx - 1, ny - 2), z(nx - 2, ny - 1)
+        self.qTx, self.qTy, self.dT_dt = z(nx - 1, ny - 2), z(nx - 2, ny - 1), z(nx - 2, ny - 2)
+        s = self
+        self._pt = [
+            s.k_assign.launcher(s.ErrV, s.Vy), s.k_assign.launcher(s.ErrP, s.Pt),
+            s.k_0.launcher(s.RogT, s.Eta, s.divV, s.T, s.Vx, s.Vy, s.rho0ga, s.eta0, s.deta_dT, s.dT, s.dx, s.dy),
+            s.k_1.launcher(s.Pt, s.txx, s.tyy, s.sxy, s.Eta, s.divV, s.Vx, s.Vy, s.dtau_iter, s.beta, s.dx, s.dy),
+            s.k_2.launcher(s.Rx, s.Ry, s.dVxdt, s.dVydt, s.Pt, s.RogT, s.txx, s.tyy, s.sxy, s.rho, s.dampX, s.dampY, s.dtau_iter, s.dx, s.dy),
+            s.k_v.launcher(s.Vx, s.Vy, s.dVxdt, s.dVydt, s.dtau_iter),
+            s.k_bcy.launcher(s.Vx, ranges=((1, nx + 1),)),
+            s.k_bcx.launcher(s.Vy, ranges=((1, ny + 1),)),
+            s.k_err.launcher(s.ErrV, s.Vy), s.k_err.launcher(s.ErrP, s.Pt),
+        ]
+        self._graph = None
+        self.reset()
+        self.iterate(1)  # compile and load every kernel
+        self.k_qt(s.qTx, s.qTy, s.T, s.DcT, s.dx, s.dy)
+        self.k_adv(s.dT_dt, s.qTx, s.qTy, s.T, s.Vx, s.Vy, s.dx, s.dy)
+        self.k_t(s.T, s.T_old, s.dT_dt, 0.0)
+        self.k_noflux(s.T)
+        self.reset()
+        core.synchronize()
+
+    def fields(self):
+        return {k: v for k, v in vars(self).items() if isinstance(v, core.DeviceArray)}
+
+    def reset(self):
+        """The initial state: the Gaussian temperature anomaly with the hot bottom and cold top
+        rows (:121-124), every other field zero."""
+        for A in self.fields().values():
+            A.fill_(0.0)
+        self.T.copy_from(self.T0)
+
+    def iterate(self, n):
+        """n PT iterations, asynchronous."""
+        steps = self._pt
+        for _ in range(n):
+            for st in steps:
+                st()
+
+    def _block(self):
+        if not self.use_graph:
+            self.iterate(self.nerr)
+            return
+        if self._graph is None:
+            core.synchronize()
+            g = core.Graph()
+            with g:
+                self.iterate(self.nerr)
+            self._graph = g
+        self._graph.launch()
+
+    def errors(self):
+        """(errV, errP) of :171-172."""
+        errV = self.ErrV.reduce("max_abs") / (1e-12 + self.Vy.reduce("max_abs"))
+        errP = self.ErrP.reduce("max_abs") / (1e-12 + self.Pt.reduce("max_abs"))
+        return errV, errP
+
+    def time_step(self):
+        """One time step (:155-183); returns (PT iterations, errV, errP, dt)."""
+        s = self
+        s.k_assign(s.T_old, s.T, async_=True)
+        errV = errP = 2 * s.eps
+        niter = 0
+        while (errV > s.eps or errP > s.eps) and niter < s.itermax:
+            self._block()
+            niter += s.nerr
+            errV, errP = self.errors()
+        s.k_qt(s.qTx, s.qTy, s.T, s.DcT, s.dx, s.dy, async_=True)
+        s.k_adv(s.dT_dt, s.qTx, s.qTy, s.T, s.Vx, s.Vy, s.dx, s.dy, async_=True)
+        dt_adv = min(_div_inf(s.dx, s.Vx.reduce("max_abs")), _div_inf(s.dy, s.Vy.reduce("max_abs"))) / 2.1
+        dt = min(s.dt_diff, dt_adv)
+        s.k_t(s.T, s.T_old, s.dT_dt, dt, async_=True)
+        s.k_noflux(s.T, async_=True)
+        return niter, errV, errP, dt
+
+
 # ------------------------------------------------------------------ acoustic waves
 def acoustic3d_inputs(n, lx=40.0, k=1.0, rho=1.0):
     """Initial state of miniapps/acoustic3D.jl:24-40 on n^3 pressure nodes (dx = lx/(n-1)):

@@ -385,6 +385,82 @@ def test_stokes3d_rigid_motion_has_no_strain():
     assert np.all(s3["Pt"] == 1.5)
 
 
+# ------------------------------------------------- ThermalConvection2D
+def test_thermal2d_stokes_part_is_stokes2d():
+    """With rho = dtau_iter = 1 the PT kernels of ThermalConvection2D are Stokes2D's (compute_P!
+    with dτPt = dτ_iter/β, compute_τ! on Eta, compute_dV! on RogT, compute_V! with dτV = 1):
+    equal to the C restatement bit for bit; rho = 2 halves the residuals exactly."""
+    rng = np.random.default_rng(23)
+    nx, ny = 11, 9
+    s2 = _stokes_arrays(nx, ny)
+    for k in ("Pt", "Vx", "Vy", "dVxdt", "dVydt"):
+        s2[k][:] = rng.standard_normal(s2[k].shape)
+    t = {k: v.copy(order="F") for k, v in s2.items()}
+    T = F(rng.standard_normal((nx, ny)))
+    z = lambda *sh: F(np.zeros(sh))  # noqa: E731
+    RogT, Eta, sxy = z(nx, ny), z(nx, ny), z(nx - 1, ny - 1)
+    dx, dy, beta, rho0ga = 0.3, 0.4, 3.7, 2.5
+    O.thermal2d_0(RogT, Eta, t["divV"], T, t["Vx"], t["Vy"], rho0ga, 1.3, 0.2, 1.0, dx, dy)
+    O.thermal2d_1(t["Pt"], t["txx"], t["tyy"], sxy, Eta, t["divV"], t["Vx"], t["Vy"], 1.0, beta, dx, dy)
+    s2["dtPt"][:] = 1.0 / beta
+    O.stokes2d_p(s2["divV"], s2["Pt"], s2["Vx"], s2["Vy"], s2["dtPt"], dx, dy)
+    O.stokes2d_tau(s2["divV"], s2["txx"], s2["tyy"], s2["txy"], s2["Vx"], s2["Vy"], Eta, dx, dy)
+    for a, b in (("divV", "divV"), ("Pt", "Pt"), ("txx", "txx"), ("tyy", "tyy")):
+        assert np.array_equal(t[a], s2[b]), a
+    assert np.array_equal(sxy, s2["txy"])
+    r2 = {k: t[k].copy(order="F") for k in ("Rx", "Ry", "dVxdt", "dVydt")}
+    O.thermal2d_2(t["Rx"], t["Ry"], t["dVxdt"], t["dVydt"], t["Pt"], RogT, t["txx"], t["tyy"], sxy, 1.0, 0.9, 0.8, 1.0, dx, dy)
+    O.stokes2d_dv(s2["Rx"], s2["Ry"], s2["dVxdt"], s2["dVydt"], s2["Pt"], RogT, s2["txx"], s2["tyy"], s2["txy"], 0.9, 0.8, dx, dy)
+    for k in ("Rx", "Ry", "dVxdt", "dVydt"):
+        assert np.array_equal(t[k], s2[k]), k
+    O.thermal2d_2(r2["Rx"], r2["Ry"], r2["dVxdt"], r2["dVydt"], t["Pt"], RogT, t["txx"], t["tyy"], sxy, 2.0, 0.9, 0.8, 1.0, dx, dy)
+    assert np.array_equal(r2["Rx"] * 2.0, t["Rx"]) and np.array_equal(r2["Ry"] * 2.0, t["Ry"])
+    O.thermal2d_v(t["Vx"], t["Vy"], t["dVxdt"], t["dVydt"], 1.0)
+    s2["dtVx"][:], s2["dtVy"][:] = 1.0, 1.0
+    O.stokes2d_v(s2["Vx"], s2["Vy"], s2["dVxdt"], s2["dVydt"], s2["dtVx"], s2["dtVy"])
+    assert np.array_equal(t["Vx"], s2["Vx"]) and np.array_equal(t["Vy"], s2["Vy"])
+    Tm = F(np.full((nx, ny), -0.5))  # T = -dT/2: Eta = eta0 exactly, RogT = rho0ga * T
+    O.thermal2d_0(RogT, Eta, t["divV"], Tm, t["Vx"], t["Vy"], rho0ga, 1.3, 0.2, 1.0, dx, dy)
+    assert np.all(Eta == 1.3) and np.all(RogT == rho0ga * -0.5)
+
+
+@pytest.mark.parametrize("u", [0.5, -0.5])
+def test_thermal2d_upwind_advection_closed_forms(u):
+    """On an integer grid (dx = dy = 1, DcT = 1) T = x^2 has backward and forward differences
+    2x-1 and 2x+1 and a constant flux divergence -2: dT_dt = 2 - u (2x-1) for u > 0 (upwind from
+    the left) and 2 - u (2x+1) for u < 0."""
+    nx, ny = 9, 7
+    x = np.arange(nx, dtype=np.float64)
+    T = F(np.repeat((x * x)[:, None], ny, 1))
+    z = lambda *sh: F(np.zeros(sh))  # noqa: E731
+    qTx, qTy, dT_dt = z(nx - 1, ny - 2), z(nx - 2, ny - 1), z(nx - 2, ny - 2)
+    Vx, Vy = F(np.full((nx + 1, ny), u)), z(nx, ny + 1)
+    O.thermal2d_qt(qTx, qTy, T, 1.0, 1.0, 1.0)
+    O.thermal2d_advect(dT_dt, qTx, qTy, T, Vx, Vy, 1.0, 1.0)
+    r = x[1:-1][:, None]  # the T row of each dT_dt row
+    want = 2.0 - u * (2 * r - 1) if u > 0 else 2.0 - u * (2 * r + 1)
+    assert np.array_equal(dT_dt, np.broadcast_to(want, dT_dt.shape))
+    # the same in y, and the 2-D flux divergence of x^2 + y^2 with no velocity: -(-4) = 4
+    y = np.arange(ny, dtype=np.float64)
+    T2 = F((x * x)[:, None] + (y * y)[None, :])
+    O.thermal2d_qt(qTx, qTy, T2, 1.0, 1.0, 1.0)
+    O.thermal2d_advect(dT_dt, qTx, qTy, T2, z(nx + 1, ny), z(nx, ny + 1), 1.0, 1.0)
+    assert np.all(dT_dt == 4.0)
+
+
+def test_thermal2d_strong_zero():
+    """Julia's false * Inf is 0: an infinite velocity upwind-excluded at a node adds nothing there."""
+    nx, ny = 6, 5
+    T = F(np.repeat((np.arange(nx, dtype=np.float64) ** 2)[:, None], ny, 1))
+    z = lambda *sh: F(np.zeros(sh))  # noqa: E731
+    qTx, qTy, dT_dt = z(nx - 1, ny - 2), z(nx - 2, ny - 1), z(nx - 2, ny - 2)
+    Vx = z(nx + 1, ny)
+    Vx[3, 2] = np.inf  # the right face of dT_dt row 1 (Vx < 0 false there), the left face of row 2
+    O.thermal2d_advect(dT_dt, qTx, qTy, T, Vx, z(nx, ny + 1), 1.0, 1.0)
+    assert np.isfinite(dT_dt[1, 1]) and dT_dt[2, 1] == -np.inf
+    assert O.julia_bool_mul(np.array([False]), np.array([-np.inf]))[0] == 0.0
+
+
 # ------------------------------------------------- acoustic 2-D adjoint (no reference counterpart)
 def _ac2_case(nx=22, ny=19, nt=50):
     rng = np.random.default_rng(5)
