@@ -561,6 +561,27 @@ def thermal_study(args):
     print(json.dumps(out), flush=True)
 
 
+def porowaves_study(args):
+    """miniapps/scalar_porowaves2D.jl at its own configuration (256 x 512, blob setup): time per
+    time step (four launches; the time step is refreshed from a GPU maximum every 100 steps)."""
+    import parallelstencil as ps
+    from parallelstencil import workloads as W
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2)
+    S = W.Porowaves2D(__name__)
+    S.step(10)
+    ps.synchronize()
+    K = args.steps * 10
+    t0 = time.perf_counter()
+    S.step(K)
+    ps.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / K
+    log(f"porowaves {S.nx}x{S.ny}: {ms:.4f} ms per time step, t = {S.t:.4e}")
+    out = {"metric": "scalar porosity waves time steps per second", "value": round(1e3 / ms, 1), "unit": "steps/s", "n_gpus": 1,
+           "steps": K, "warmup": 10, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic", "config": {"workload": f"scalar_porowaves2D {S.nx}x{S.ny} FP64, blob setup"}}
+    print(json.dumps(out), flush=True)
+
+
 def hidecomm_study(args):
     """examples/diffusion3D_multigpucpu_hidecomm_novis.jl on one GPU at n^3 FP64 (periodic in
     every dimension, so that update_halo! exchanges with the process itself): the example's loop
@@ -630,7 +651,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes3d | thermal")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes3d | thermal | porowaves")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--adjoint-n", type=int, default=2048)
@@ -677,6 +698,9 @@ def main():
         return
     if args.workload == "thermal":
         thermal_study(args)
+        return
+    if args.workload == "porowaves":
+        porowaves_study(args)
         return
 
     import parallelstencil as ps

@@ -43,6 +43,11 @@ that parity is not anchored on a formula merely retyped:
                     differences 2x-1 and 2x+1, so the upwind choice for either sign of the
                     velocity is visible; the flux divergence of T = x^2 + y^2 is -4 DcT
                     exactly; Julia's strong zero (false * Inf == 0).
+  poro_*            conservation: with no compaction term (EtaR = Inf) the interior sum of
+                    Phi changes by dt times the boundary fluxes alone (telescoping sums of
+                    d_xa/d_ya), which a wrong sign or index breaks; uniform K and a linear
+                    Pe give the fluxes -K slope and -K (slope + Δρg); the clamp at 0; the
+                    strict guard of compute_EtaR! on a KAT (last row and column untouched).
   stokes3d_*        reduction to the 2-D C restatement: fields invariant along one axis,
                     with no velocity along it, must give on each of the three planes
                     (x-z, y-z, x-y) exactly what stokes2d_* gives on the plane's axes,
@@ -394,6 +399,52 @@ def thermal2d_time_step(F, prm):
     thermal2d_t(F["T"], F["T_old"], F["dT_dt"], dt)
     thermal2d_noflux(F["T"])
     return niter, errV, errP, dt
+
+
+# ------------------------------------------------- scalar porosity waves (numpy, over the pinned fd2 helpers)
+def poro_nonlin(K_mu, EtaR, Phi, Eta, k_mu0, n):
+    """compute_nonlin! of miniapps/scalar_porowaves2D.jl:11-15."""
+    K_mu[...] = k_mu0 * np.power(Phi, n)
+    EtaR[...] = Eta
+
+
+def poro_etar(EtaR, Pe, etaR):
+    """compute_EtaR! of miniapps/scalar_porowaves2D.jl:17-20: the guard is ix < size(EtaR, 1)
+    (strict), so the last row and column of EtaR are never set."""
+    n1, n2 = EtaR.shape
+    sub = EtaR[:n1 - 1, :n2 - 1]
+    sub[Pe[1:n1, 1:n2] < 0.0] = etaR
+
+
+def poro_flux(qx, qy, K_mu, Pe, drhog, dx, dy):
+    """compute_flux! of miniapps/scalar_porowaves2D.jl:22-26."""
+    qx[...] = -fd2("av_xi", K_mu) * fd2("d_xi", Pe) / dx
+    qy[...] = -fd2("av_yi", K_mu) * (fd2("d_yi", Pe) / dy + drhog)
+
+
+def poro_pe_phi(dPedt, Pe, Phi, dPhidt, qx, qy, EtaR, beta, dt, dx, dy):
+    """compute_Pe_Phi! of miniapps/scalar_porowaves2D.jl:28-34 (Pe updated before its second use)."""
+    inn = (slice(1, -1), slice(1, -1))
+    dPedt[...] = 1.0 / beta * ((-fd2("d_xa", qx) / dx - fd2("d_ya", qy) / dy) - 1.0 / EtaR * Pe[inn])
+    Pe[inn] = Pe[inn] + dt * dPedt
+    dPhidt[...] = -beta * dPedt - 1.0 / EtaR * Pe[inn]
+    Phi[inn] = np.maximum(0.0, Phi[inn] + dt * dPhidt)
+
+
+def poro_steps(F, prm, it0, dt, n):
+    """n time steps (miniapps/scalar_porowaves2D.jl:84-89) from step number it0 with time step dt;
+    returns (last step number, dt)."""
+    p = prm
+    it = it0
+    for _ in range(n):
+        it += 1
+        poro_nonlin(F["K_mu"], F["EtaR"], F["Phi"], F["Eta"], p["k_mu0"], p["n"])
+        poro_etar(F["EtaR"], F["Pe"], p["etaR"])
+        if it % 100 == 1:
+            dt = p["dx"] ** 2 / (F["K_mu"].max() / p["beta"]) / 5.0
+        poro_flux(F["qx"], F["qy"], F["K_mu"], F["Pe"], p["drhog"], p["dx"], p["dy"])
+        poro_pe_phi(F["dPedt"], F["Pe"], F["Phi"], F["dPhidt"], F["qx"], F["qy"], F["EtaR"], p["beta"], dt, p["dx"], p["dy"])
+    return it, dt
 
 
 # ------------------------------------------------- finite-difference KATs

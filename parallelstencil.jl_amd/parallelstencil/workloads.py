@@ -739,6 +739,102 @@ class ThermalConvection2D:
         return niter, errV, errP, dt
 
 
+# ------------------------------------------------------------------ scalar porosity waves (miniapps/scalar_porowaves2D.jl)
+PORO_NONLIN = """
+@parallel function compute_nonlin!(K_mu::Data.Array, EtaR::Data.Array, Phi::Data.Array, Eta::Data.Array, k_μ0::Data.Number, n::Data.Number)
+    @all(K_mu) = k_μ0 * @all(Phi)^n
+    @all(EtaR) = @all(Eta)
+    return
+end
+"""
+PORO_ETAR = """
+@parallel_indices (ix,iy) function compute_EtaR!(EtaR::Data.Array, Pe::Data.Array, ηR::Data.Number)
+    if (ix<size(EtaR,1) && iy<size(EtaR,2)) if (Pe[ix+1,iy+1]<0.0) EtaR[ix,iy] = ηR; end;  end
+    return
+end
+"""
+PORO_FLUX = """
+@parallel function compute_flux!(qx::Data.Array, qy::Data.Array, K_mu::Data.Array, Pe::Data.Array, Δρg::Data.Number, dx::Data.Number, dy::Data.Number)
+    @all(qx) = -@av_xi(K_mu)* @d_xi(Pe)/dx
+    @all(qy) = -@av_yi(K_mu)*(@d_yi(Pe)/dy + Δρg)
+    return
+end
+"""
+PORO_PE_PHI = """
+@parallel function compute_Pe_Phi!(dPedt::Data.Array, Pe::Data.Array, Phi::Data.Array, dPhidt::Data.Array, qx::Data.Array, qy::Data.Array, EtaR::Data.Array, β::Data.Number, dt::Data.Number, dx::Data.Number, dy::Data.Number)
+    @all(dPedt)  = 1.0/β*(-@d_xa(qx)/dx -@d_ya(qy)/dy -1.0/@all(EtaR)*@inn(Pe))
+    @inn(Pe)     = @inn(Pe) + dt*@all(dPedt)
+    @all(dPhidt) = -β*@all(dPedt) - 1.0/@all(EtaR)*@inn(Pe)
+    @inn(Phi)    = max(0.0, @inn(Phi) + dt*@all(dPhidt))
+    return
+end
+"""
+
+
+def porowaves2d_setup(nx=256, ny=512, chan=0):
+    """Scalars and initial porosity of miniapps/scalar_porowaves2D.jl:36-78 (blob setup by
+    default): returns (prm, Phi0). The initial porosity is the background plus the elliptic blob,
+    smoothed once by 1/4.1 of its discrete Laplacian in the interior."""
+    import numpy as np
+    sc = 1.0 + chan * 30
+    lx, ly = 250.0 / sc, 500.0 / sc
+    eta = 1.0
+    lam = 20.0 / sc
+    prm = dict(nx=nx, ny=ny, k_mu0=1.0, drhog=1.0, n=3.0, eta=eta, etaR=eta / (1.0 + chan * 1000), beta=0.2,
+               dx=lx / (nx - 1), dy=ly / (ny - 1))
+    x = (np.arange(nx) * prm["dx"] - 0.5 * lx) / lam / 4.0
+    y = (np.arange(ny) * prm["dy"] - 0.2 * ly) / lam
+    Radc = (x * x)[:, None] + (y * y)[None, :]
+    Phi = np.ones((nx, ny))
+    Phi[Radc < 1.0] = Phi[Radc < 1.0] + 2.0
+    lap = np.diff(np.diff(Phi[:, 1:-1], axis=0), axis=0) + np.diff(np.diff(Phi[1:-1, :], axis=1), axis=1)
+    Phi[1:-1, 1:-1] = Phi[1:-1, 1:-1] + 1.0 / 4.1 * lap
+    return prm, np.asfortranarray(Phi)
+
+
+class Porowaves2D:
+    """Scalar porosity waves of miniapps/scalar_porowaves2D.jl:79-97 on the GPU: four launches per
+    time step; every 100 steps (when mod(it, 100) == 1) dt = dx^2 / (maximum(K_mu) / β) / 5 from
+    an FP64 reduction on the GPU."""
+
+    def __init__(self, module, nx=256, ny=512, chan=0):
+        prm, Phi0 = porowaves2d_setup(nx, ny, chan)
+        self.prm, self.Phi0 = prm, Phi0
+        for k, v in prm.items():
+            setattr(self, k, v)
+        d = lambda text, kind="parallel": core._define(kind, text, {}, module)  # noqa: E731
+        self.k_nonlin, self.k_flux, self.k_pe = d(PORO_NONLIN), d(PORO_FLUX), d(PORO_PE_PHI)
+        self.k_etar = d(PORO_ETAR, "parallel_indices")
+        z = lambda *s: core.zeros(*s, eltype=core.Float64)  # noqa: E731
+        self.Eta, self.EtaR, self.dPedt, self.dPhidt = z(nx - 2, ny - 2), z(nx - 2, ny - 2), z(nx - 2, ny - 2), z(nx - 2, ny - 2)
+        self.Pe, self.K_mu, self.Phi = z(nx, ny), z(nx, ny), z(nx, ny)
+        self.qx, self.qy = z(nx - 1, ny - 2), z(nx - 2, ny - 1)
+        self.reset()
+
+    def fields(self):
+        return {k: v for k, v in vars(self).items() if isinstance(v, core.DeviceArray)}
+
+    def reset(self):
+        for A in self.fields().values():
+            A.fill_(0.0)
+        self.Eta.fill_(self.eta)
+        self.Phi.copy_from(self.Phi0)
+        self.it, self.dt, self.t = 0, 1.0, 0.0
+
+    def step(self, n=1):
+        """n time steps, asynchronous except for the time-step reduction every 100 steps."""
+        s = self
+        for _ in range(n):
+            s.it += 1
+            s.k_nonlin(s.K_mu, s.EtaR, s.Phi, s.Eta, s.k_mu0, s.n, async_=True)
+            s.k_etar(s.EtaR, s.Pe, s.etaR, async_=True)
+            if s.it % 100 == 1:
+                s.dt = s.dx ** 2 / (s.K_mu.reduce("max_abs") / s.beta) / 5.0
+            s.k_flux(s.qx, s.qy, s.K_mu, s.Pe, s.drhog, s.dx, s.dy, async_=True)
+            s.k_pe(s.dPedt, s.Pe, s.Phi, s.dPhidt, s.qx, s.qy, s.EtaR, s.beta, s.dt, s.dx, s.dy, async_=True)
+            s.t += s.dt
+
+
 # ------------------------------------------------------------------ acoustic waves
 def acoustic3d_inputs(n, lx=40.0, k=1.0, rho=1.0):
     """Initial state of miniapps/acoustic3D.jl:24-40 on n^3 pressure nodes (dx = lx/(n-1)):

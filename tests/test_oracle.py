@@ -461,6 +461,62 @@ def test_thermal2d_strong_zero():
     assert O.julia_bool_mul(np.array([False]), np.array([-np.inf]))[0] == 0.0
 
 
+# ------------------------------------------------- scalar porosity waves
+def _poro_fields(nx, ny):
+    z = lambda *sh: F(np.zeros(sh))  # noqa: E731
+    return dict(Eta=z(nx - 2, ny - 2), EtaR=z(nx - 2, ny - 2), dPedt=z(nx - 2, ny - 2), dPhidt=z(nx - 2, ny - 2), Pe=z(nx, ny),
+                K_mu=z(nx, ny), Phi=z(nx, ny), qx=z(nx - 1, ny - 2), qy=z(nx - 2, ny - 1))
+
+
+def test_poro_porosity_changes_by_the_boundary_fluxes_only():
+    """No compaction term (EtaR = Inf): dPhidt = -β dPedt = div q, so the interior sum of Phi
+    changes by dt times the net boundary flux (the telescoping sums of d_xa(qx), d_ya(qy))."""
+    rng = np.random.default_rng(29)
+    nx, ny = 12, 10
+    f = _poro_fields(nx, ny)
+    f["Phi"][:] = 1.0 + rng.random((nx, ny))
+    f["Pe"][1:-1, 1:-1] = rng.standard_normal((nx - 2, ny - 2))
+    f["K_mu"][:] = 0.5 + rng.random((nx, ny))
+    f["EtaR"][:] = np.inf
+    dx, dy, dt, beta = 0.5, 0.25, 1e-3, 0.2
+    O.poro_flux(f["qx"], f["qy"], f["K_mu"], f["Pe"], 1.0, dx, dy)
+    boundary = (f["qx"][-1, :].sum() - f["qx"][0, :].sum()) / dx + (f["qy"][:, -1].sum() - f["qy"][:, 0].sum()) / dy
+    s0 = f["Phi"][1:-1, 1:-1].sum()
+    O.poro_pe_phi(f["dPedt"], f["Pe"], f["Phi"], f["dPhidt"], f["qx"], f["qy"], f["EtaR"], beta, dt, dx, dy)
+    assert abs((f["Phi"][1:-1, 1:-1].sum() - s0) - dt * boundary) <= 1e-12 * (abs(s0) + 1)
+    assert abs(f["dPedt"].sum() * -beta - boundary) <= 1e-10 * (abs(boundary) + 1)
+
+
+def test_poro_fluxes_and_clamp():
+    nx, ny = 8, 7
+    f = _poro_fields(nx, ny)
+    f["K_mu"][:] = 2.0
+    x, y = np.arange(nx, dtype=np.float64), np.arange(ny, dtype=np.float64)
+    f["Pe"][:] = 3.0 * x[:, None] - 1.0 * y[None, :]
+    O.poro_flux(f["qx"], f["qy"], f["K_mu"], f["Pe"], 0.5, 1.0, 1.0)
+    assert np.all(f["qx"] == -2.0 * 3.0) and np.all(f["qy"] == -2.0 * (-1.0 + 0.5))
+    # a large outflow drives the porosity below zero: clamped at 0
+    f["Phi"][:] = 0.1
+    f["EtaR"][:] = 1.0
+    f["qx"][:] = 0.0
+    f["qy"][:] = 0.0
+    f["Pe"][:] = -100.0  # dPedt = 500, Pe -> 400, dPhidt = -100 - 400
+    O.poro_pe_phi(f["dPedt"], f["Pe"], f["Phi"], f["dPhidt"], f["qx"], f["qy"], f["EtaR"], 0.2, 1.0, 1.0, 1.0)
+    assert np.all(f["Phi"][1:-1, 1:-1] == 0.0) and np.all(f["Phi"][0] == 0.1)
+
+
+def test_poro_etar_strict_guard():
+    nx, ny = 7, 6
+    EtaR = F(np.full((nx - 2, ny - 2), 5.0))
+    Pe = F(-np.ones((nx, ny)))
+    O.poro_etar(EtaR, Pe, 0.25)
+    assert np.all(EtaR[:-1, :-1] == 0.25) and np.all(EtaR[-1, :] == 5.0) and np.all(EtaR[:, -1] == 5.0)
+    Pe[2, 2] = 1.0  # EtaR[1, 1] reads Pe[2, 2]
+    EtaR[:] = 5.0
+    O.poro_etar(EtaR, Pe, 0.25)
+    assert EtaR[1, 1] == 5.0 and EtaR[0, 0] == 0.25
+
+
 # ------------------------------------------------- acoustic 2-D adjoint (no reference counterpart)
 def _ac2_case(nx=22, ny=19, nt=50):
     rng = np.random.default_rng(5)
