@@ -582,6 +582,40 @@ def porowaves_study(args):
     print(json.dumps(out), flush=True)
 
 
+def hydromech_study(args):
+    """miniapps/HydroMech2D.jl at its own configuration (res 128: 127 x 255): time per PT
+    iteration (graph replays of 200 iterations, 11 launches each), T_eff against the miniapp's
+    A_eff = 16 n_x n_y x 8 B per iteration (:186), and the first time steps to tolerance."""
+    import parallelstencil as ps
+    from parallelstencil import workloads as W
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2)
+    S = W.HydroMech2D(__name__, 128, graph=True)
+    S._block()
+    ps.synchronize()
+    nb = 10
+    t0 = time.perf_counter()
+    for _ in range(nb):
+        S._block()
+    ps.synchronize()
+    ms_it = (time.perf_counter() - t0) * 1e3 / (nb * S.nout)
+    a_eff = 16 * S.nx * S.ny * 8
+    S.reset()
+    ps.synchronize()
+    steps = []
+    for _ in range(args.thermal_steps):
+        t0 = time.perf_counter()
+        niter, err, dt = S.time_step()
+        ps.synchronize()
+        steps.append({"pt_iterations": niter, "error": err, "dt": dt, "s": round(time.perf_counter() - t0, 4)})
+        log(f"hydromech step: {niter} PT iterations, err {err:.2e}, dt {dt:.3e}, {steps[-1]['s']} s")
+    out = {"metric": "HydroMech2D PT iteration T_eff GB/s (miniapp A_eff = 16 n_x n_y x 8 B per iteration)",
+           "value": round(a_eff / (ms_it * 1e-3) / 1e9, 2), "unit": "GB/s", "n_gpus": 1, "steps": nb * S.nout, "warmup": S.nout,
+           "ms_per_step": round(ms_it, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic", "config": {"workload": f"HydroMech2D {S.nx}x{S.ny} FP64, eps 1e-5, nout 200, graph replays"},
+           "time_steps": steps}
+    print(json.dumps(out), flush=True)
+
+
 def hidecomm_study(args):
     """examples/diffusion3D_multigpucpu_hidecomm_novis.jl on one GPU at n^3 FP64 (periodic in
     every dimension, so that update_halo! exchanges with the process itself): the example's loop
@@ -651,7 +685,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes3d | thermal | porowaves")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes3d | thermal | porowaves | hydromech")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--adjoint-n", type=int, default=2048)
@@ -664,7 +698,7 @@ def main():
     ap.add_argument("--stokes3d-n", type=int, default=127)
     ap.add_argument("--stokes3d-solve", type=int, default=1, help="stokes3d: 0 skips the solve to tolerance")
     ap.add_argument("--stokes3d-modes", default="eager,graph", help="stokes3d: launch modes measured (eager, graph)")
-    ap.add_argument("--thermal-steps", type=int, default=3, help="thermal: time steps run to PT tolerance")
+    ap.add_argument("--thermal-steps", type=int, default=3, help="thermal, hydromech: time steps run to PT tolerance")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built", file=sys.stderr)
@@ -701,6 +735,9 @@ def main():
         return
     if args.workload == "porowaves":
         porowaves_study(args)
+        return
+    if args.workload == "hydromech":
+        hydromech_study(args)
         return
 
     import parallelstencil as ps

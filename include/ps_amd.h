@@ -226,8 +226,9 @@ int ps_ranges_inner(const int64_t ranges[3][2], const int64_t boundary_width[3],
 
 /* ---- reductions (residual norms of the pseudo-transient solvers) ---- */
 /* FP64 reductions over a field on the GPU; replaces the host-side
- * mean(abs.(R)) / maximum(abs.(R)) of miniapps/Stokes2D.jl:117-123.
- * op: 0 sum, 1 sum of |x|, 2 max |x|, 3 mean |x|. Blocks on the stream. */
+ * mean(abs.(R)) / maximum(abs.(R)) of miniapps/Stokes2D.jl:117-123 and the norm(R) of
+ * miniapps/HydroMech2D.jl:171.
+ * op: 0 sum, 1 sum of |x|, 2 max |x|, 3 mean |x|, 4 L2 norm sqrt(sum x^2). Blocks on the stream. */
 int ps_field_reduce(const ps_field* f, int op, double* out, void* stream);
 
 /* ---- events (measurement; no reference counterpart: the reference times with

@@ -48,6 +48,10 @@ that parity is not anchored on a formula merely retyped:
                     d_xa/d_ya), which a wrong sign or index breaks; uniform K and a linear
                     Pe give the fluxes -K slope and -K (slope + Δρg); the clamp at 0; the
                     strict guard of compute_EtaR! on a KAT (last row and column untouched).
+  hm_*              the Stokes part against stokes2d_* (exact): compute_res! is compute_dV!
+                    with the buoyancy negated; compute_P_τ! with β_n = 0 and μs = 0.5 is
+                    compute_τ! on a uniform viscosity; the divergences are compute_P!'s;
+                    relaxation limits θ = 0 (unchanged) and θ = 1 with Pf = Pt (tanh 0 = 0).
   stokes3d_*        reduction to the 2-D C restatement: fields invariant along one axis,
                     with no velocity along it, must give on each of the three planes
                     (x-z, y-z, x-y) exactly what stokes2d_* gives on the plane's axes,
@@ -445,6 +449,84 @@ def poro_steps(F, prm, it0, dt, n):
         poro_flux(F["qx"], F["qy"], F["K_mu"], F["Pe"], p["drhog"], p["dx"], p["dy"])
         poro_pe_phi(F["dPedt"], F["Pe"], F["Phi"], F["dPhidt"], F["qx"], F["qy"], F["EtaR"], p["beta"], dt, p["dx"], p["dy"])
     return it, dt
+
+
+# ------------------------------------------------- HydroMech2D (numpy, over the pinned fd2 helpers)
+def hm_params(EtaC, K_muf, Rog, divV, divqD, Phi, Pf, Pt, Vx, Vy, qDx, qDy, p):
+    """compute_params_∇! of miniapps/HydroMech2D.jl:17-24."""
+    th_e, th_k = p["theta_e"], p["theta_k"]
+    EtaC[...] = (1.0 - th_e) * EtaC + th_e * (p["mus"] / Phi * p["eta2mus"] * (1.0 + 0.5 * (1.0 / p["R"] - 1.0) * (1.0 + np.tanh((Pf - Pt) / p["lamPe"]))))
+    K_muf[...] = (1.0 - th_k) * K_muf + th_k * (p["k_muf0"] * np.power(Phi / p["phi0"], p["nperm"]))
+    Rog[...] = p["rhofg"] * Phi + p["rhosg"] * (1.0 - Phi) - p["rhogBG"]
+    divV[...] = fd2("d_xa", Vx) / p["dx"] + fd2("d_ya", Vy) / p["dy"]
+    divqD[...] = fd2("d_xa", qDx) / p["dx"] + fd2("d_ya", qDy) / p["dy"]
+
+
+def hm_rp(dtPf, RPt, RPf, K_muf, divV, divqD, Pt, Pf, EtaC, Phi, p):
+    """compute_RP! of miniapps/HydroMech2D.jl:26-31."""
+    dtPf[1:-1, 1:-1] = p["min_dxy2"] / fd2("maxloc", K_muf) / 4.1 / p["Pfsc"]
+    c = (Pt - Pf) / (EtaC * (1.0 - Phi))
+    RPt[...] = -divV - c
+    RPf[...] = RPf * p["Pfdmp"] - divqD + c
+
+
+def hm_ptau(Pt, Pf, txx, tyy, sxy, RPt, RPf, dtPf, Vx, Vy, divV, p):
+    """compute_P_τ! of miniapps/HydroMech2D.jl:33-40."""
+    mus, bn = p["mus"], p["beta_n"]
+    Pt[...] = Pt + p["dtauPt"] * RPt
+    Pf[...] = Pf + dtPf * RPf
+    txx[...] = 2.0 * mus * (fd2("d_xa", Vx) / p["dx"] - 1.0 / 3.0 * divV - bn * RPt)
+    tyy[...] = 2.0 * mus * (fd2("d_ya", Vy) / p["dy"] - 1.0 / 3.0 * divV - bn * RPt)
+    sxy[...] = 2.0 * mus * (0.5 * (fd2("d_yi", Vx) / p["dy"] + fd2("d_xi", Vy) / p["dx"]))
+
+
+def hm_res(Rx, Ry, dVxdt, dVydt, txx, tyy, sxy, Pt, Rog, p):
+    """compute_res! of miniapps/HydroMech2D.jl:42-48."""
+    dx, dy = p["dx"], p["dy"]
+    Rx[...] = fd2("d_xi", txx) / dx + fd2("d_ya", sxy) / dy - fd2("d_xi", Pt) / dx
+    Ry[...] = fd2("d_yi", tyy) / dy + fd2("d_xa", sxy) / dx - fd2("d_yi", Pt) / dy - fd2("av_yi", Rog)
+    dVxdt[...] = p["dampX"] * dVxdt + Rx
+    dVydt[...] = p["dampY"] * dVydt + Ry
+
+
+def hm_update(Vx, Vy, qDx, qDy, Phi, dVxdt, dVydt, K_muf, Pf, Phi_o, divV, divV_o, dt, p):
+    """compute_update! of miniapps/HydroMech2D.jl:50-57."""
+    inn = (slice(1, -1), slice(1, -1))
+    Vx[inn] = Vx[inn] + p["dtauV"] * dVxdt
+    Vy[inn] = Vy[inn] + p["dtauV"] * dVydt
+    qDx[inn] = -fd2("av_xi", K_muf) * (fd2("d_xi", Pf) / p["dx"])
+    qDy[inn] = -fd2("av_yi", K_muf) * (fd2("d_yi", Pf) / p["dy"] + (p["rhofg"] - p["rhogBG"]))
+    CN = p["CN"]
+    Phi[...] = Phi_o + (1.0 - Phi) * (CN * divV_o + (1.0 - CN) * divV) * dt
+
+
+def hm_iterate(F, p, dt, n):
+    """n PT iterations of miniapps/HydroMech2D.jl:157-169 (the boundary launches with one index)."""
+    for _ in range(n):
+        hm_params(F["EtaC"], F["K_muf"], F["Rog"], F["divV"], F["divqD"], F["Phi"], F["Pf"], F["Pt"], F["Vx"], F["Vy"], F["qDx"], F["qDy"], p)
+        hm_rp(F["dtPf"], F["RPt"], F["RPf"], F["K_muf"], F["divV"], F["divqD"], F["Pt"], F["Pf"], F["EtaC"], F["Phi"], p)
+        F["dtPf"][0, :], F["dtPf"][-1, :] = F["dtPf"][1, :], F["dtPf"][-2, :]
+        F["dtPf"][:, 0], F["dtPf"][:, -1] = F["dtPf"][:, 1], F["dtPf"][:, -2]
+        hm_ptau(F["Pt"], F["Pf"], F["txx"], F["tyy"], F["sxy"], F["RPt"], F["RPf"], F["dtPf"], F["Vx"], F["Vy"], F["divV"], p)
+        hm_res(F["Rx"], F["Ry"], F["dVxdt"], F["dVydt"], F["txx"], F["tyy"], F["sxy"], F["Pt"], F["Rog"], p)
+        hm_update(F["Vx"], F["Vy"], F["qDx"], F["qDy"], F["Phi"], F["dVxdt"], F["dVydt"], F["K_muf"], F["Pf"], F["Phi_o"], F["divV"],
+                  F["divV_o"], dt, p)
+        for A in (F["Vx"], F["qDx"]):
+            A[:, 0], A[:, -1] = A[:, 1], A[:, -2]
+        for A in (F["Vy"], F["qDy"]):
+            A[0, :], A[-1, :] = A[1, :], A[-2, :]
+
+
+def hm_time_step(F, p, dt):
+    """One time step (miniapps/HydroMech2D.jl:151-197); returns (iterations, error, next dt)."""
+    F["Phi_o"][...] = F["Phi"]
+    F["divV_o"][...] = F["divV"]
+    err, niter = 2 * p["eps"], 0
+    while err > p["eps"] and niter < p["itermax"]:
+        hm_iterate(F, p, dt, p["nout"])
+        niter += p["nout"]
+        err = max(np.linalg.norm(F["Ry"]) / F["Ry"].size, np.linalg.norm(F["RPf"]) / F["RPf"].size)
+    return niter, err, p["dt_red"] / (1e-10 + np.abs(F["divV"]).max())
 
 
 # ------------------------------------------------- finite-difference KATs

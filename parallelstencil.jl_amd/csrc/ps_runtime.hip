@@ -96,57 +96,64 @@ __device__ __forceinline__ double ps_to_f64(T v) { return (double)v; }
 template <>
 __device__ __forceinline__ double ps_to_f64<unsigned short>(unsigned short v) { return (double)__uint_as_float(((unsigned)v) << 16); }
 
-// Stage 1: per-block partial (sum, sum|x|, max|x|) over the logical box in a
+// Stage 1: per-block partial (sum, sum|x|, max|x|, sum x^2) over the logical box in a
 // fixed order, so repeated reductions of the same data give the same bits.
 template <class T>
 __global__ void __launch_bounds__(256) ps_reduce1_k(const T* __restrict__ p, long long nx, long long ny, long long nz, long long sx,
                                                     long long sxy, double* __restrict__ part) {
-  __shared__ double s0[256], s1[256], s2[256];
+  __shared__ double s0[256], s1[256], s2[256], s3[256];
   long long n = nx * ny * nz;
-  double a = 0.0, b = 0.0, c = 0.0;
+  double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     long long x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
     double v = ps_to_f64<T>(p[x + y * sx + z * sxy]);
     a += v;
     b += fabs(v);
     c = fmax(c, fabs(v));
+    d += v * v;
   }
   s0[threadIdx.x] = a;
   s1[threadIdx.x] = b;
   s2[threadIdx.x] = c;
+  s3[threadIdx.x] = d;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       s0[threadIdx.x] += s0[threadIdx.x + w];
       s1[threadIdx.x] += s1[threadIdx.x + w];
       s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+      s3[threadIdx.x] += s3[threadIdx.x + w];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part[3 * blockIdx.x + 0] = s0[0];
-    part[3 * blockIdx.x + 1] = s1[0];
-    part[3 * blockIdx.x + 2] = s2[0];
+    part[4 * blockIdx.x + 0] = s0[0];
+    part[4 * blockIdx.x + 1] = s1[0];
+    part[4 * blockIdx.x + 2] = s2[0];
+    part[4 * blockIdx.x + 3] = s3[0];
   }
 }
 
 __global__ void __launch_bounds__(256) ps_reduce2_k(const double* __restrict__ part, int nparts, double* __restrict__ out) {
-  __shared__ double s0[256], s1[256], s2[256];
-  double a = 0.0, b = 0.0, c = 0.0;
+  __shared__ double s0[256], s1[256], s2[256], s3[256];
+  double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
   for (int i = threadIdx.x; i < nparts; i += 256) {
-    a += part[3 * i];
-    b += part[3 * i + 1];
-    c = fmax(c, part[3 * i + 2]);
+    a += part[4 * i];
+    b += part[4 * i + 1];
+    c = fmax(c, part[4 * i + 2]);
+    d += part[4 * i + 3];
   }
   s0[threadIdx.x] = a;
   s1[threadIdx.x] = b;
   s2[threadIdx.x] = c;
+  s3[threadIdx.x] = d;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       s0[threadIdx.x] += s0[threadIdx.x + w];
       s1[threadIdx.x] += s1[threadIdx.x + w];
       s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+      s3[threadIdx.x] += s3[threadIdx.x + w];
     }
     __syncthreads();
   }
@@ -154,6 +161,7 @@ __global__ void __launch_bounds__(256) ps_reduce2_k(const double* __restrict__ p
     out[0] = s0[0];
     out[1] = s1[0];
     out[2] = s2[0];
+    out[3] = s3[0];
   }
 }
 
@@ -684,7 +692,7 @@ extern "C" int ps_field_reduce(const ps_field* f_in, int op, double* out, void* 
   if (int r = require_init()) return r;
   const ps_field fd = data_of(*f_in);
   const ps_field* f = &fd;
-  if (op < 0 || op > 3) return fail(PS_EARGUMENT, "ps_field_reduce: op must be 0..3");
+  if (op < 0 || op > 4) return fail(PS_EARGUMENT, "ps_field_reduce: op must be 0..4");
   State& s = S();
   hipStream_t st = stream_of(stream);
   long long n = f->dims[0] * f->dims[1] * f->dims[2];
@@ -693,7 +701,7 @@ extern "C" int ps_field_reduce(const ps_field* f_in, int op, double* out, void* 
     std::lock_guard<std::mutex> lk(s.mu);
     if (s.nparts_cap < nb) {
       if (s.d_part) PS_HIP(hipFree(s.d_part));
-      PS_HIP(hipMalloc(&s.d_part, sizeof(double) * 3 * 2048));
+      PS_HIP(hipMalloc(&s.d_part, sizeof(double) * 4 * 2048));
       s.nparts_cap = 2048;
     }
   }
@@ -705,13 +713,14 @@ extern "C" int ps_field_reduce(const ps_field* f_in, int op, double* out, void* 
   }
   hipLaunchKernelGGL(ps_reduce2_k, dim3(1), dim3(256), 0, st, s.d_part, nb, s.d_out);
   PS_HIP(hipGetLastError());
-  double h[3];
-  PS_HIP(hipMemcpyAsync(h, s.d_out, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  double h[4];
+  PS_HIP(hipMemcpyAsync(h, s.d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
   PS_HIP(hipStreamSynchronize(st));
   if (op == 0) *out = h[0];
   else if (op == 1) *out = h[1];
   else if (op == 2) *out = h[2];
-  else *out = n ? h[1] / (double)n : 0.0;
+  else if (op == 3) *out = n ? h[1] / (double)n : 0.0;
+  else *out = std::sqrt(h[3]);
   return 0;
 }
 

@@ -301,7 +301,7 @@ class DeviceArray:
 
     def reduce(self, op, stream=None):
         out = ctypes.c_double()
-        code = {"sum": 0, "sum_abs": 1, "max_abs": 2, "mean_abs": 3}[op]
+        code = {"sum": 0, "sum_abs": 1, "max_abs": 2, "mean_abs": 3, "norm": 4}[op]
         check(lib.ps_field_reduce(ctypes.byref(self.f), code, ctypes.byref(out), _stream_handle(stream)))
         return out.value
 

@@ -835,6 +835,199 @@ class Porowaves2D:
             s.t += s.dt
 
 
+# ------------------------------------------------------------------ HydroMech2D (miniapps/HydroMech2D.jl)
+HM_OLD = """
+@parallel function update_old!(Phi_o::Data.Array, ∇V_o::Data.Array, Phi::Data.Array, ∇V::Data.Array)
+    @all(Phi_o) = @all(Phi)
+    @all(∇V_o)  = @all(∇V)
+    return
+end
+"""
+HM_PARAMS = """
+@parallel function compute_params_∇!(EtaC::Data.Array, K_muf::Data.Array, Rog::Data.Array, ∇V::Data.Array, ∇qD::Data.Array, Phi::Data.Array, Pf::Data.Array, Pt::Data.Array, Vx::Data.Array, Vy::Data.Array, qDx::Data.Array, qDy::Data.Array, μs::Data.Number, η2μs::Data.Number, R::Data.Number, λPe::Data.Number, k_μf0::Data.Number, ϕ0::Data.Number, nperm::Data.Number, θ_e::Data.Number, θ_k::Data.Number, ρfg::Data.Number, ρsg::Data.Number, ρgBG::Data.Number, dx::Data.Number, dy::Data.Number)
+    @all(EtaC)  = (1.0-θ_e)*@all(EtaC)  + θ_e*( μs/@all(Phi)*η2μs*(1.0+0.5*(1.0/R-1.0)*(1.0+tanh((@all(Pf)-@all(Pt))/λPe))) )
+    @all(K_muf) = (1.0-θ_k)*@all(K_muf) + θ_k*( k_μf0 * (@all(Phi)/ϕ0)^nperm )
+    @all(Rog)   = ρfg*@all(Phi) + ρsg*(1.0-@all(Phi)) - ρgBG
+    @all(∇V)    = @d_xa(Vx)/dx  + @d_ya(Vy)/dy
+    @all(∇qD)   = @d_xa(qDx)/dx + @d_ya(qDy)/dy
+    return
+end
+"""
+HM_RP = """
+@parallel function compute_RP!(dτPf::Data.Array, RPt::Data.Array, RPf::Data.Array, K_muf::Data.Array, ∇V::Data.Array, ∇qD::Data.Array, Pt::Data.Array, Pf::Data.Array, EtaC::Data.Array, Phi::Data.Array, Pfsc::Data.Number, Pfdmp::Data.Number, min_dxy2::Data.Number, dx::Data.Number, dy::Data.Number)
+    @inn(dτPf) = min_dxy2/@maxloc(K_muf)/4.1/Pfsc
+    @all(RPt)  =                 - @all(∇V)  - (@all(Pt) - @all(Pf))/(@all(EtaC)*(1.0-@all(Phi)))
+    @all(RPf)  = @all(RPf)*Pfdmp - @all(∇qD) + (@all(Pt) - @all(Pf))/(@all(EtaC)*(1.0-@all(Phi)))
+    return
+end
+"""
+HM_PTAU = """
+@parallel function compute_P_τ!(Pt::Data.Array, Pf::Data.Array, τxx::Data.Array, τyy::Data.Array, σxy::Data.Array, RPt::Data.Array, RPf::Data.Array, dτPf::Data.Array, Vx::Data.Array, Vy::Data.Array, ∇V::Data.Array, dτPt::Data.Number, μs::Data.Number, β_n::Data.Number, dx::Data.Number, dy::Data.Number)
+    @all(Pt)  = @all(Pt) +      dτPt *@all(RPt)
+    @all(Pf)  = @all(Pf) + @all(dτPf)*@all(RPf)
+    @all(τxx) = 2.0*μs*( @d_xa(Vx)/dx - 1.0/3.0*@all(∇V) - β_n*@all(RPt) )
+    @all(τyy) = 2.0*μs*( @d_ya(Vy)/dy - 1.0/3.0*@all(∇V) - β_n*@all(RPt) )
+    @all(σxy) = 2.0*μs*(0.5*( @d_yi(Vx)/dy + @d_xi(Vy)/dx ))
+    return
+end
+"""
+HM_RES = """
+@parallel function compute_res!(Rx::Data.Array, Ry::Data.Array, dVxdτ::Data.Array, dVydτ::Data.Array, τxx::Data.Array, τyy::Data.Array, σxy::Data.Array, Pt::Data.Array, Rog::Data.Array, dampX::Data.Number, dampY::Data.Number, dx::Data.Number, dy::Data.Number)
+    @all(Rx)    = @d_xi(τxx)/dx + @d_ya(σxy)/dy - @d_xi(Pt)/dx
+    @all(Ry)    = @d_yi(τyy)/dy + @d_xa(σxy)/dx - @d_yi(Pt)/dy - @av_yi(Rog)
+    @all(dVxdτ) = dampX*@all(dVxdτ) + @all(Rx)
+    @all(dVydτ) = dampY*@all(dVydτ) + @all(Ry)
+    return
+end
+"""
+HM_UPDATE = """
+@parallel function compute_update!(Vx::Data.Array, Vy::Data.Array, qDx::Data.Array, qDy::Data.Array, Phi::Data.Array, dVxdτ::Data.Array, dVydτ::Data.Array, K_muf::Data.Array, Pf::Data.Array, Phi_o::Data.Array, ∇V::Data.Array, ∇V_o::Data.Array, dτV::Data.Number, ρfg::Data.Number, ρgBG::Data.Number, CN::Data.Number, dt::Data.Number, dx::Data.Number, dy::Data.Number)
+    @inn(Vx)  =  @inn(Vx) + dτV*@all(dVxdτ)
+    @inn(Vy)  =  @inn(Vy) + dτV*@all(dVydτ)
+    @inn(qDx) = -@av_xi(K_muf)*(@d_xi(Pf)/dx)
+    @inn(qDy) = -@av_yi(K_muf)*(@d_yi(Pf)/dy + (ρfg - ρgBG))
+    @all(Phi) =  @all(Phi_o) + (1.0-@all(Phi))*(CN*@all(∇V_o) + (1.0-CN)*@all(∇V))*dt
+    return
+end
+"""
+
+
+def hydromech2d_setup(res=128):
+    """Scalars and initial fields of miniapps/HydroMech2D.jl:63-150 ((res-1) x (2 res - 1)):
+    returns (prm, init) with init holding Phi, EtaC, K_muf and qDy as float64 host arrays."""
+    import numpy as np
+    rhofg, k_muf0, etaC0, eta2mus, R, nperm, phi0, ra, lam0 = 1.0, 1.0, 1.0, 10.0, 500.0, 3.0, 0.01, 2, 1.0
+    rhosg = 2.0 * rhofg
+    lx = 20.0
+    ly = ra * lx
+    nx, ny = res - 1, ra * res - 1
+    dx, dy = lx / (nx - 1), ly / (ny - 1)
+    mus = etaC0 * phi0 / eta2mus
+    lam = lam0 * float(np.sqrt(k_muf0 * etaC0))
+    beta_n, Vdmp, Vsc, Ptsc = 1.0, 5.0, 2.0, 2.0
+    min_dxy2 = min(dx, dy) ** 2
+    prm = dict(nx=nx, ny=ny, dx=dx, dy=dy, rhofg=rhofg, rhosg=rhosg, k_muf0=k_muf0, eta2mus=eta2mus, R=R, nperm=nperm, phi0=phi0,
+               lamPe=0.01, mus=mus, rhogBG=rhofg * phi0 + rhosg * (1.0 - phi0), CN=0.5, eps=1e-5, itermax=5000, nout=200,
+               beta_n=beta_n, Pfdmp=0.8, Pfsc=4.0, theta_e=9e-1, theta_k=1e-1, dt_red=1e-3, min_dxy2=min_dxy2,
+               dtauV=min_dxy2 / mus / (1.0 + beta_n) / 4.1 / Vsc, dtauPt=4.1 * mus * (1.0 + beta_n) / max(nx, ny) / Ptsc,
+               dampX=1.0 - Vdmp / nx, dampY=1.0 - Vdmp / ny, dt0=1e-5, t_tot=0.02)
+    x = ((np.arange(nx) * dx - 0.5 * lx) / lam / 4.0) ** 2
+    y = ((np.arange(ny) * dy - 0.25 * ly) / lam) ** 2
+    Radc = x[:, None] + y[None, :]
+    Phi = np.full((nx, ny), phi0)
+    Phi[Radc < 1.0] = Phi[Radc < 1.0] + 2 * phi0
+    phibc = Phi[:, -1].copy()
+    qDy = np.zeros((nx, ny + 1))
+    qDy[:, 0] = qDy[:, -1] = (rhosg - rhofg) * (1.0 - phibc) * k_muf0 * (phibc / phi0) ** nperm
+    init = dict(Phi=Phi, EtaC=mus / Phi * eta2mus, K_muf=k_muf0 * (Phi / phi0), qDy=qDy)
+    return prm, {k: np.asfortranarray(v) for k, v in init.items()}
+
+
+class HydroMech2D:
+    """Two-phase hydro-mechanical flow of miniapps/HydroMech2D.jl:150-197 on the GPU: per time
+    step update_old! and PT iterations (five compute launches and six boundary launches each,
+    :157-169) until norm(Ry)/length(Ry) and norm(RPf)/length(RPf), checked every nout
+    iterations, are at or below eps; then dt = dt_red / (1e-10 + maximum(abs.(∇V))). The norms and
+    the maximum are FP64 reductions on the GPU; `graph=True` replays blocks of nout iterations."""
+
+    def __init__(self, module, res=128, graph=True):
+        prm, init = hydromech2d_setup(res)
+        self.prm, self.init = prm, init
+        for k, v in prm.items():
+            setattr(self, k, v)
+        nx, ny = self.nx, self.ny
+        self.use_graph = graph
+        d = lambda text, kind="parallel": core._define(kind, text, {}, module)  # noqa: E731
+        self.k_old, self.k_params, self.k_rp, self.k_ptau, self.k_res, self.k_upd = (
+            d(t) for t in (HM_OLD, HM_PARAMS, HM_RP, HM_PTAU, HM_RES, HM_UPDATE))
+        self.k_bcx, self.k_bcy = d(STOKES2D_BC_X, "parallel_indices"), d(STOKES2D_BC_Y, "parallel_indices")
+        z = lambda *s: core.zeros(*s, eltype=core.Float64)  # noqa: E731
+        for name in ("Phi_o", "Pt", "Pf", "Rog", "divV", "divV_o", "divqD", "dtPf", "RPt", "RPf", "txx", "tyy", "Phi", "EtaC", "K_muf"):
+            setattr(self, name, z(nx, ny))
+        self.sxy = z(nx - 1, ny - 1)
+        self.dVxdt, self.dVydt, self.Rx, self.Ry = z(nx - 1, ny - 2), z(nx - 2, ny - 1), z(nx - 1, ny - 2), z(nx - 2, ny - 1)
+        self.Vx, self.Vy, self.qDx, self.qDy = z(nx + 1, ny), z(nx, ny + 1), z(nx + 1, ny), z(nx, ny + 1)
+        s = self
+        self.dt_arg = None
+        self._pt = [
+            s.k_params.launcher(s.EtaC, s.K_muf, s.Rog, s.divV, s.divqD, s.Phi, s.Pf, s.Pt, s.Vx, s.Vy, s.qDx, s.qDy, s.mus, s.eta2mus, s.R,
+                                s.lamPe, s.k_muf0, s.phi0, s.nperm, s.theta_e, s.theta_k, s.rhofg, s.rhosg, s.rhogBG, s.dx, s.dy),
+            s.k_rp.launcher(s.dtPf, s.RPt, s.RPf, s.K_muf, s.divV, s.divqD, s.Pt, s.Pf, s.EtaC, s.Phi, s.Pfsc, s.Pfdmp, s.min_dxy2, s.dx, s.dy),
+            s.k_bcx.launcher(s.dtPf, ranges=((1, ny),)),
+            s.k_bcy.launcher(s.dtPf, ranges=((1, nx),)),
+            s.k_ptau.launcher(s.Pt, s.Pf, s.txx, s.tyy, s.sxy, s.RPt, s.RPf, s.dtPf, s.Vx, s.Vy, s.divV, s.dtauPt, s.mus, s.beta_n, s.dx, s.dy),
+            s.k_res.launcher(s.Rx, s.Ry, s.dVxdt, s.dVydt, s.txx, s.tyy, s.sxy, s.Pt, s.Rog, s.dampX, s.dampY, s.dx, s.dy),
+            s.k_upd.launcher(s.Vx, s.Vy, s.qDx, s.qDy, s.Phi, s.dVxdt, s.dVydt, s.K_muf, s.Pf, s.Phi_o, s.divV, s.divV_o, s.dtauV, s.rhofg,
+                             s.rhogBG, s.CN, s.dt0, s.dx, s.dy),
+            s.k_bcy.launcher(s.Vx, ranges=((1, nx + 1),)),
+            s.k_bcx.launcher(s.Vy, ranges=((1, ny + 1),)),
+            s.k_bcy.launcher(s.qDx, ranges=((1, nx + 1),)),
+            s.k_bcx.launcher(s.qDy, ranges=((1, ny + 1),)),
+        ]
+        self._graph = None
+        self.reset()
+        self.iterate(1)  # compile and load every kernel
+        self.k_old(s.Phi_o, s.divV_o, s.Phi, s.divV)
+        self.reset()
+        core.synchronize()
+
+    def fields(self):
+        return {k: v for k, v in vars(self).items() if isinstance(v, core.DeviceArray)}
+
+    def reset(self):
+        for A in self.fields().values():
+            A.fill_(0.0)
+        for k, v in self.init.items():
+            getattr(self, k).copy_from(v)
+        self.dt, self.t, self.it = self.dt0, 0.0, 1
+        self._set_dt(self.dt0)
+
+    def _set_dt(self, dt):
+        """The physical time step is an argument of compute_update!: replaced in the prepared
+        launch (the graph is captured again when it changes)."""
+        self._pt[6].set(16, dt)
+        self._graph = None
+
+    def iterate(self, n):
+        steps = self._pt
+        for _ in range(n):
+            for st in steps:
+                st()
+
+    def _block(self):
+        if not self.use_graph:
+            self.iterate(self.nout)
+            return
+        if self._graph is None:
+            core.synchronize()
+            g = core.Graph()
+            with g:
+                self.iterate(self.nout)
+            self._graph = g
+        self._graph.launch()
+
+    def error(self):
+        """max(norm(Ry)/length(Ry), norm(RPf)/length(RPf)) (:171)."""
+        n = lambda A: A.shape[0] * A.shape[1]  # noqa: E731
+        return max(self.Ry.reduce("norm") / n(self.Ry), self.RPf.reduce("norm") / n(self.RPf))
+
+    def time_step(self):
+        """One time step (:151-197); returns (PT iterations, error, dt used)."""
+        s = self
+        s.k_old(s.Phi_o, s.divV_o, s.Phi, s.divV, async_=True)
+        err, niter = 2 * s.eps, 0
+        while err > s.eps and niter < s.itermax:
+            s._block()
+            niter += s.nout
+            err = s.error()
+        dt_used = s.dt
+        s.dt = s.dt_red / (1e-10 + s.divV.reduce("max_abs"))
+        s.t += s.dt
+        s.it += 1
+        s._set_dt(s.dt)
+        return niter, err, dt_used
+
+
 # ------------------------------------------------------------------ acoustic waves
 def acoustic3d_inputs(n, lx=40.0, k=1.0, rho=1.0):
     """Initial state of miniapps/acoustic3D.jl:24-40 on n^3 pressure nodes (dx = lx/(n-1)):

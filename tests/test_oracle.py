@@ -517,6 +517,41 @@ def test_poro_etar_strict_guard():
     assert EtaR[1, 1] == 5.0 and EtaR[0, 0] == 0.25
 
 
+# ------------------------------------------------- HydroMech2D
+def test_hydromech_stokes_part_is_stokes2d():
+    rng = np.random.default_rng(31)
+    nx, ny = 11, 9
+    s2 = _stokes_arrays(nx, ny)
+    for k in ("Pt", "Vx", "Vy", "dVxdt", "dVydt"):
+        s2[k][:] = rng.standard_normal(s2[k].shape)
+    p = dict(dx=0.3, dy=0.4, mus=0.5, beta_n=0.0, dtauPt=0.0, dampX=0.9, dampY=0.8)
+    z = lambda *sh: F(np.zeros(sh))  # noqa: E731
+    h = {k: v.copy(order="F") for k, v in s2.items()}
+    RPt, RPf, dtPf, Rog = F(rng.standard_normal((nx, ny))), z(nx, ny), z(nx, ny), F(rng.random((nx, ny)))
+    O.stokes2d_p(s2["divV"], s2["Pt"], s2["Vx"], s2["Vy"], s2["dtPt"], p["dx"], p["dy"])  # dtPt = 0: Pt unchanged
+    h["divV"][...] = s2["divV"]
+    O.hm_ptau(h["Pt"], z(nx, ny), h["txx"], h["tyy"], h["txy"], RPt, RPf, dtPf, h["Vx"], h["Vy"], h["divV"], p)
+    O.stokes2d_tau(s2["divV"], s2["txx"], s2["tyy"], s2["txy"], s2["Vx"], s2["Vy"], F(np.full((nx, ny), 0.5)), p["dx"], p["dy"])
+    for k in ("Pt", "txx", "tyy", "txy"):
+        assert np.array_equal(h[k], s2[k]), k
+    O.hm_res(h["Rx"], h["Ry"], h["dVxdt"], h["dVydt"], h["txx"], h["tyy"], h["txy"], h["Pt"], Rog, p)
+    O.stokes2d_dv(s2["Rx"], s2["Ry"], s2["dVxdt"], s2["dVydt"], s2["Pt"], F(-Rog), s2["txx"], s2["tyy"], s2["txy"], 0.9, 0.8, p["dx"], p["dy"])
+    for k in ("Rx", "Ry", "dVxdt", "dVydt"):
+        assert np.array_equal(h[k], s2[k]), k
+    # relaxation limits of compute_params_∇!
+    q = dict(p, theta_e=0.0, theta_k=1.0, eta2mus=10.0, R=500.0, lamPe=0.01, k_muf0=1.0, phi0=0.01, nperm=3.0, rhofg=1.0, rhosg=2.0,
+             rhogBG=1.99)
+    Phi = F(0.01 + 0.01 * rng.random((nx, ny)))
+    EtaC, K, divqD = F(rng.random((nx, ny))), z(nx, ny), z(nx, ny)
+    E0 = EtaC.copy()
+    Pt = F(rng.random((nx, ny)))
+    O.hm_params(EtaC, K, Rog, h["divV"], divqD, Phi, Pt.copy(), Pt, h["Vx"], h["Vy"], z(nx + 1, ny), z(nx, ny + 1), q)
+    assert np.array_equal(EtaC, E0) and np.allclose(K, (Phi / 0.01) ** 3, rtol=1e-15) and np.array_equal(h["divV"], s2["divV"])
+    q["theta_e"] = 1.0
+    O.hm_params(EtaC, K, Rog, h["divV"], divqD, Phi, Pt.copy(), Pt, h["Vx"], h["Vy"], z(nx + 1, ny), z(nx, ny + 1), q)
+    np.testing.assert_allclose(EtaC, 0.5 / Phi * 10.0 * (1.0 + 0.5 * (1.0 / 500.0 - 1.0)), rtol=1e-15)
+
+
 # ------------------------------------------------- acoustic 2-D adjoint (no reference counterpart)
 def _ac2_case(nx=22, ny=19, nt=50):
     rng = np.random.default_rng(5)
