@@ -535,7 +535,7 @@ def thermal_study(args):
     import parallelstencil as ps
     from parallelstencil import workloads as W
     ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2)
-    S = W.ThermalConvection2D(__name__, graph=True)
+    S = W.ThermalConvection2D(__name__, graph=bool(args.graph))
     S._block()
     ps.synchronize()
     nb = 20
@@ -556,7 +556,7 @@ def thermal_study(args):
     out = {"metric": "ThermalConvection2D PT iterations per second", "value": round(1e3 / ms_it, 1), "unit": "iterations/s",
            "n_gpus": 1, "steps": nb * S.nerr, "warmup": S.nerr, "ms_per_step": round(ms_it, 5), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"ThermalConvection2D {S.nx}x{S.ny} FP64, Ra 1e7, eps 1e-4, nerr 100, graph replays"},
+           "config": {"workload": f"ThermalConvection2D {S.nx}x{S.ny} FP64, Ra 1e7, eps 1e-4, nerr 100, " + ("graph replays" if args.graph else "eager launches")},
            "time_steps": steps}
     print(json.dumps(out), flush=True)
 
@@ -589,7 +589,7 @@ def hydromech_study(args):
     import parallelstencil as ps
     from parallelstencil import workloads as W
     ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2)
-    S = W.HydroMech2D(__name__, 128, graph=True)
+    S = W.HydroMech2D(__name__, 128, graph=bool(args.graph))
     S._block()
     ps.synchronize()
     nb = 10
@@ -611,7 +611,7 @@ def hydromech_study(args):
     out = {"metric": "HydroMech2D PT iteration T_eff GB/s (miniapp A_eff = 16 n_x n_y x 8 B per iteration)",
            "value": round(a_eff / (ms_it * 1e-3) / 1e9, 2), "unit": "GB/s", "n_gpus": 1, "steps": nb * S.nout, "warmup": S.nout,
            "ms_per_step": round(ms_it, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-           "data": "synthetic", "config": {"workload": f"HydroMech2D {S.nx}x{S.ny} FP64, eps 1e-5, nout 200, graph replays"},
+           "data": "synthetic", "config": {"workload": f"HydroMech2D {S.nx}x{S.ny} FP64, eps 1e-5, nout 200, " + ("graph replays" if args.graph else "eager launches")},
            "time_steps": steps}
     print(json.dumps(out), flush=True)
 
@@ -698,6 +698,7 @@ def main():
     ap.add_argument("--stokes3d-n", type=int, default=127)
     ap.add_argument("--stokes3d-solve", type=int, default=1, help="stokes3d: 0 skips the solve to tolerance")
     ap.add_argument("--stokes3d-modes", default="eager,graph", help="stokes3d: launch modes measured (eager, graph)")
+    ap.add_argument("--graph", type=int, default=1, help="thermal, hydromech: 0 launches each PT iteration eagerly")
     ap.add_argument("--thermal-steps", type=int, default=3, help="thermal, hydromech: time steps run to PT tolerance")
     args = ap.parse_args()
     if args.gpus != 1:
