@@ -948,7 +948,6 @@ class HydroMech2D:
         self.dVxdt, self.dVydt, self.Rx, self.Ry = z(nx - 1, ny - 2), z(nx - 2, ny - 1), z(nx - 1, ny - 2), z(nx - 2, ny - 1)
         self.Vx, self.Vy, self.qDx, self.qDy = z(nx + 1, ny), z(nx, ny + 1), z(nx + 1, ny), z(nx, ny + 1)
         s = self
-        self.dt_arg = None
         self._pt = [
             s.k_params.launcher(s.EtaC, s.K_muf, s.Rog, s.divV, s.divqD, s.Phi, s.Pf, s.Pt, s.Vx, s.Vy, s.qDx, s.qDy, s.mus, s.eta2mus, s.R,
                                 s.lamPe, s.k_muf0, s.phi0, s.nperm, s.theta_e, s.theta_k, s.rhofg, s.rhosg, s.rhogBG, s.dx, s.dy),
