@@ -107,7 +107,7 @@ typedef struct ps_devinfo {
   int64_t lds_bytes_per_block;
   int64_t total_mem_bytes;
   int64_t l2_bytes;
-  double  hbm_peak_gbs;    /* memory clock x bus width (x2 for DDR)          */
+  double  hbm_peak_gbs;    /* 4 x reported memory clock x bus width (HBM3E)  */
   int32_t clock_khz;
   int32_t mem_clock_khz;
   int32_t mem_bus_width;

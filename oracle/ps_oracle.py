@@ -28,12 +28,23 @@ that parity is not anchored on a formula merely retyped:
                     except boundary fluxes; single-cell brute-force evaluation.
   diffusion3d_d2    mathematically equal to the flux form (tolerance), closed form on
                     a quadratic field (d2 = 2 exactly on integer grids).
-  acoustic2d/3d     discrete energy/mass: sum(P) is conserved by compute_P! when the
-                    velocity boundaries are zero (telescoping sums); brute-force
-                    single-point evaluation; FP32/BF16 variants against FP64.
-  stokes2d_*        brute-force single-point evaluation of each statement from the
-                    kernel text; a uniform velocity field gives zero strain rates and
-                    zero divergence; residual definition of miniapps/Stokes2D.jl:117-123.
+  acoustic2d/3d     closed forms (tests/test_oracle_closed_forms.py): a quadratic P
+                    gives every inner face exactly V - dt/rho * dP/dx of the continuous
+                    field at the face position, and leaves the boundary faces alone; a
+                    quadratic V gives P - dt*K*div V at the centres (a flipped sign, a
+                    difference shifted by one cell or a wrong mask fails, shown by planted
+                    mistakes); sum(P) is conserved with zero boundary velocities;
+                    FP32/BF16 variants against FP64.
+  stokes2d_*        closed forms on polynomial fields at the staggered positions
+                    (tests/test_oracle_closed_forms.py): pseudo-time steps of a linear
+                    viscosity are the face viscosity's; divergence, pressure update and
+                    stresses of quadratic/shear velocities; residuals of polynomial
+                    stresses and pressure are the momentum balance at the faces with the
+                    buoyancy average; hydrostatic pressure balances a uniform buoyancy
+                    exactly; the velocity update lands on the inner faces; the whole PT
+                    chain with free-slip walls converges to the hydrostatic state
+                    (V = 0, dPt/dy = rho g). Fifteen planted mistakes (signs, offsets,
+                    masks, spacings, damping, the 1/3) each fail at least one of them.
   bc_x/bc_y         KAT on an index-valued array.
   thermal2d_0/1/2/v the Stokes part of ThermalConvection2D against stokes2d_* (exact:
                     with rho = dtau_iter = 1 compute_1!/compute_2!/update_V! are compute_τ!,
@@ -68,8 +79,9 @@ that parity is not anchored on a formula merely retyped:
                     (I*nc + j), B >= N stores each value contiguously (j*B + I); a
                     blocked case against numpy's own reshape/transpose of the
                     (B, nblocks, nc) view; every offset hit exactly once.
-  copy3d_cells      addition of the cells (a retyped formula: pinned only through
-                    the layout checks and integer-valued inputs, exact).
+  copy3d_cells      closed form: cells holding their position plus values holding their
+                    index inside the cell add to a known integer per value (a transposed
+                    cell or a dropped operand fails).
   wave_*            closed forms in tests/test_oracle.py: the xor butterfly over
                     masks 32..1 leaves every lane with the wave sum; shfl_up then
                     shfl_down by the same delta restores the lanes away from the

@@ -91,6 +91,9 @@ __global__ void __launch_bounds__(256) ps_copy_k(T* __restrict__ d, const T* __r
 
 __global__ void ps_set_i32_k(int* p, int v) { *p = v; }
 
+// max that keeps a NaN once one has been seen, as Julia's maximum(abs.(R)) does (fmax drops it)
+__device__ __forceinline__ double ps_nanmax(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
+
 template <class T>
 __device__ __forceinline__ double ps_to_f64(T v) { return (double)v; }
 template <>
@@ -109,7 +112,7 @@ __global__ void __launch_bounds__(256) ps_reduce1_k(const T* __restrict__ p, lon
     double v = ps_to_f64<T>(p[x + y * sx + z * sxy]);
     a += v;
     b += fabs(v);
-    c = fmax(c, fabs(v));
+    c = ps_nanmax(c, fabs(v));
     d += v * v;
   }
   s0[threadIdx.x] = a;
@@ -121,7 +124,7 @@ __global__ void __launch_bounds__(256) ps_reduce1_k(const T* __restrict__ p, lon
     if ((int)threadIdx.x < w) {
       s0[threadIdx.x] += s0[threadIdx.x + w];
       s1[threadIdx.x] += s1[threadIdx.x + w];
-      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+      s2[threadIdx.x] = ps_nanmax(s2[threadIdx.x], s2[threadIdx.x + w]);
       s3[threadIdx.x] += s3[threadIdx.x + w];
     }
     __syncthreads();
@@ -140,7 +143,7 @@ __global__ void __launch_bounds__(256) ps_reduce2_k(const double* __restrict__ p
   for (int i = threadIdx.x; i < nparts; i += 256) {
     a += part[4 * i];
     b += part[4 * i + 1];
-    c = fmax(c, part[4 * i + 2]);
+    c = ps_nanmax(c, part[4 * i + 2]);
     d += part[4 * i + 3];
   }
   s0[threadIdx.x] = a;
@@ -152,7 +155,7 @@ __global__ void __launch_bounds__(256) ps_reduce2_k(const double* __restrict__ p
     if ((int)threadIdx.x < w) {
       s0[threadIdx.x] += s0[threadIdx.x + w];
       s1[threadIdx.x] += s1[threadIdx.x + w];
-      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+      s2[threadIdx.x] = ps_nanmax(s2[threadIdx.x], s2[threadIdx.x + w]);
       s3[threadIdx.x] += s3[threadIdx.x + w];
     }
     __syncthreads();
@@ -164,6 +167,9 @@ __global__ void __launch_bounds__(256) ps_reduce2_k(const double* __restrict__ p
     out[3] = s3[0];
   }
 }
+
+// blocks of the first reduction stage (each strides over the field): partial sums per block
+constexpr int kReduceParts = 2048;
 
 // ---------------------------------------------------------------------------
 // global state
@@ -439,6 +445,8 @@ extern "C" int ps_init(int device) {
   PS_HIP(hipMemset(s.d_err, 0, 256));
   PS_HIP(hipHostMalloc(&s.h_out, 4 * sizeof(double), hipHostMallocDefault));
   PS_HIP(hipMalloc(&s.d_out, 4 * sizeof(double)));
+  PS_HIP(hipMalloc(&s.d_part, sizeof(double) * 4 * kReduceParts));
+  s.nparts_cap = kReduceParts;
   s.device = device;
   s.inited = true;
   return 0;
@@ -459,7 +467,9 @@ extern "C" int ps_device_info(ps_devinfo* out) {
   out->clock_khz = p.clockRate;
   out->mem_clock_khz = p.memoryClockRate;
   out->mem_bus_width = p.memoryBusWidth;
-  out->hbm_peak_gbs = 2.0 * (double)p.memoryClockRate * 1e3 * (double)p.memoryBusWidth / 8.0 / 1e9;
+  // HBM3/HBM3E parts report a memory clock of a quarter of the per-pin data rate: MI300X reports
+  // 1.3 GHz on 8192 bits (5.3 TB/s published), MI355X 2.0 GHz on 8192 bits (8 TB/s published)
+  out->hbm_peak_gbs = 4.0 * (double)p.memoryClockRate * 1e3 * (double)p.memoryBusWidth / 8.0 / 1e9;
   return 0;
 }
 
@@ -490,6 +500,17 @@ extern "C" int ps_device_sync(void) {
 // ---------------------------------------------------------------------------
 // fields
 // ---------------------------------------------------------------------------
+// Allocation clears the new memory on the library stream and waits for it; while that stream is
+// being captured into a graph the clear would become a graph node and the wait would break the
+// capture, so allocating then is refused.
+static int refuse_during_capture(const char* what) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  PS_HIP(hipStreamIsCapturing(S().stream0, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    return fail(PS_EARGUMENT, std::string("cannot ") + what + " while a graph is being captured on the library stream");
+  return 0;
+}
+
 extern "C" int ps_field_alloc(int ndims, const int64_t dims[3], int dtype, ps_field* out) {
   if (int r = require_init()) return r;
   if (ndims < 1 || ndims > 3) return fail(PS_EARGUMENT, "ps_field_alloc: ndims must be 1, 2 or 3");
@@ -508,6 +529,7 @@ extern "C" int ps_field_alloc(int ndims, const int64_t dims[3], int dtype, ps_fi
   if (d[1] > 1 && g_pitch_pad > 0 && ((pitch * (int64_t)es) % 4096) == 0) pitch += g_pitch_pad / (int64_t)es;
   int64_t plane = pitch * d[1];
   int64_t bytes = std::max<int64_t>(plane * d[2] * (int64_t)es, 256);
+  if (int r = refuse_during_capture("allocate a field")) return r;
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, (size_t)bytes);
   if (e != hipSuccess) return fail(PS_EOOM, "OutOfMemoryError: cannot allocate " + std::to_string(bytes) + " bytes of device memory");
@@ -550,6 +572,7 @@ extern "C" int ps_cellarray_alloc(int ndims, const int64_t dims[3], int cellndim
   const int64_t B = blocklength == 0 ? std::max<int64_t>(N, 1) : blocklength;
   const int64_t nblk = (N + B - 1) / B;
   const int64_t bytes = std::max<int64_t>(nblk * B * nc * (int64_t)es, 256);
+  if (int r = refuse_during_capture("allocate a field")) return r;
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, (size_t)bytes);
   if (e != hipSuccess) return fail(PS_EOOM, "OutOfMemoryError: cannot allocate " + std::to_string(bytes) + " bytes of device memory");
@@ -696,15 +719,7 @@ extern "C" int ps_field_reduce(const ps_field* f_in, int op, double* out, void* 
   State& s = S();
   hipStream_t st = stream_of(stream);
   long long n = f->dims[0] * f->dims[1] * f->dims[2];
-  int nb = (int)std::min<long long>(std::max<long long>((n + 255) / 256, 1), 2048);
-  {
-    std::lock_guard<std::mutex> lk(s.mu);
-    if (s.nparts_cap < nb) {
-      if (s.d_part) PS_HIP(hipFree(s.d_part));
-      PS_HIP(hipMalloc(&s.d_part, sizeof(double) * 4 * 2048));
-      s.nparts_cap = 2048;
-    }
-  }
+  int nb = (int)std::min<long long>(std::max<long long>((n + 255) / 256, 1), kReduceParts);
   switch (f->dtype) {
     case PS_F64: hipLaunchKernelGGL(ps_reduce1_k<double>, dim3(nb), dim3(256), 0, st, (const double*)f->ptr, f->dims[0], f->dims[1], f->dims[2], f->pitch, f->plane, s.d_part); break;
     case PS_F32: hipLaunchKernelGGL(ps_reduce1_k<float>, dim3(nb), dim3(256), 0, st, (const float*)f->ptr, f->dims[0], f->dims[1], f->dims[2], f->pitch, f->plane, s.d_part); break;
@@ -1288,6 +1303,12 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
   if (int r = require_init()) return r;
   PS_GUARD_BEGIN
   KernelObj* ko = (KernelObj*)k;
+  for (int i = 0; i < nargs; ++i) {
+    const ps_field& f = args[i].field;
+    if (args[i].kind == PS_ARG_FIELD && !f.ptr && f.dims[0] * f.dims[1] * f.dims[2] > 0)
+      return fail(PS_EARGUMENT, "argument " + std::to_string(i + 1) + " of kernel " + ko->K.def.name +
+                                    ": the field has no device memory (it was freed or never allocated)");
+  }
   std::string key = launch_key(args, nargs, launch);
   std::shared_ptr<LaunchEntry> le;
   {

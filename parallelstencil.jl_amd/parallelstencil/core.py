@@ -15,11 +15,13 @@ and passed as text; all parsing, lowering, compilation and launching happens
 in the shared library. There is no CPU execution path: without the library
 or without a GPU the calls raise.
 """
+import atexit
 import contextlib
 import ctypes
 import inspect
 import json
 import re
+import sys
 import threading
 import warnings
 import weakref
@@ -209,6 +211,22 @@ class Data:
 # ---------------------------------------------------------------------------
 # device arrays
 # ---------------------------------------------------------------------------
+# number of explicit DeviceArray.free() calls so far: cached argument blocks built before the
+# latest free are checked again before they are launched
+_FREED = [0]
+
+
+def _check_launch_args(name, args):
+    """A launch needs device memory behind every array argument: ArgSpecs (shapes only) and freed
+    arrays are refused before they reach a kernel."""
+    for i, v in enumerate(args):
+        if isinstance(v, ArgSpec):
+            raise ArgumentError(f"argument {i + 1} of kernel {name} is an ArgSpec (a shape without memory); "
+                                "ArgSpecs only serve compilation and geometry queries")
+        if isinstance(v, DeviceArray) and not v._alive():
+            raise ArgumentError(f"argument {i + 1} of kernel {name}: the array has been freed")
+
+
 class DeviceArray:
     """A pitch-aligned device field (x fastest, 1-based in kernels)."""
 
@@ -308,10 +326,24 @@ class DeviceArray:
     def free(self):
         if self._owner and self.f.base:
             check(lib.ps_field_free(ctypes.byref(self.f)))
+            _FREED[0] += 1
+
+    def _alive(self):
+        """False once the memory of this array (or of the array it views) has been freed."""
+        a = self
+        while a is not None:
+            if not a.f.ptr and a.f.dims[0] * a.f.dims[1] * a.f.dims[2] > 0:
+                return False
+            if a._owner and not a.f.base and a.f.dims[0] * a.f.dims[1] * a.f.dims[2] > 0:
+                return False
+            a = a._parent
+        return True
 
     def __del__(self):
+        # at interpreter teardown the process is about to release its device memory anyway, and
+        # the runtime may already be shutting down: no frees from finalisers then
         try:
-            if self._owner and self.f.base and lib is not None:
+            if self._owner and self.f.base and lib is not None and not sys.is_finalizing():
                 lib.ps_field_free(ctypes.byref(self.f))
         except Exception:
             pass
@@ -680,19 +712,39 @@ class Event:
 
     def __del__(self):
         try:
-            lib.ps_event_destroy(self.h)
+            if not sys.is_finalizing():
+                lib.ps_event_destroy(self.h)
         except Exception:
             pass
 
 
+_LIVE_GRAPHS = weakref.WeakSet()
+_GRAPH_ATEXIT = [False]
+
+
+def _destroy_graphs():
+    """Destroys every graph still alive. Registered with atexit once the device is initialised,
+    so it runs while the process is whole: before interpreter teardown (where objects are freed in
+    no fixed order) and before the C-level exit handlers of the HIP runtime and of tools loaded
+    into the process."""
+    for g in list(_LIVE_GRAPHS):
+        g.destroy()
+
+
 class Graph:
     """HIP graph of the asynchronous launches issued on a stream inside `with graph:`; `launch()`
-    replays them with one launch. Kernels must have run once (compiled) before the capture."""
+    replays them with one launch. Kernels must have run once (compiled) before the capture.
+    `destroy()` releases the executable graph; graphs still alive at exit are destroyed by an
+    atexit hook, never from a finaliser during interpreter teardown."""
 
     def __init__(self, stream=None):
         ensure_device()
         self.stream = stream
         self.h = None
+        if not _GRAPH_ATEXIT[0]:
+            atexit.register(_destroy_graphs)
+            _GRAPH_ATEXIT[0] = True
+        _LIVE_GRAPHS.add(self)
 
     def __enter__(self):
         check(lib.ps_graph_begin(_stream_handle(self.stream)))
@@ -710,13 +762,19 @@ class Graph:
 
     def launch(self, stream=None):
         if self.h is None:
-            raise ArgumentError("the graph has not been captured")
+            raise ArgumentError("the graph has not been captured (or has been destroyed)")
         check(lib.ps_graph_launch(self.h, _stream_handle(self.stream if stream is None else stream)))
+
+    def destroy(self):
+        h, self.h = self.h, None
+        if h is not None:
+            check(lib.ps_graph_destroy(h))
 
     def __del__(self):
         try:
-            if self.h is not None:
+            if self.h is not None and not sys.is_finalizing():
                 lib.ps_graph_destroy(self.h)
+                self.h = None
         except Exception:
             pass
 
@@ -725,27 +783,51 @@ class Graph:
 # kernels
 # ---------------------------------------------------------------------------
 class BoundLaunch:
-    """A kernel launch with a fixed argument block (see Kernel.launcher)."""
+    """A kernel launch with a fixed argument block (see Kernel.launcher). With `weak`, the
+    arrays are held through weak references only (`alive()` tells whether they all still exist)."""
 
-    def __init__(self, kernel, args, launch):
+    def __init__(self, kernel, args, launch, weak=False):
         self.kernel = kernel
         self.args = list(args)
         self._launch = launch
         self._build()
+        self._refs = None
+        if weak:
+            self._refs = [(i, weakref.ref(a)) for i, a in enumerate(self.args) if isinstance(a, DeviceArray)]
+            self.args = [None if isinstance(a, DeviceArray) else a for a in self.args]
+
+    def _current_args(self):
+        if self._refs is None:
+            return self.args
+        a = list(self.args)
+        for i, r in self._refs:
+            a[i] = r()
+            if a[i] is None:
+                raise ArgumentError(f"kernel {self.kernel.name}: an array of this prepared launch no longer exists")
+        return a
+
+    def alive(self):
+        return self._refs is None or all(r() is not None for _, r in self._refs)
 
     def _build(self):
         k = self.kernel
-        self._arr = k._args(self.args)
+        self._arr = k._args(self.args, launch=True)
+        self._gen = _FREED[0]
         self._L = k._launch_struct(**self._launch)
         self._Lp = ctypes.byref(self._L)
         self._n = len(self.args)
 
     def set(self, i, value):
         """Replaces argument i (a field or a scalar)."""
+        if self._refs is not None:
+            raise ArgumentError("arguments of a prepared launch with weak references cannot be replaced")
         self.args[i] = value
         self._build()
 
     def __call__(self):
+        if self._gen != _FREED[0]:
+            _check_launch_args(self.kernel.name, self._current_args())
+            self._gen = _FREED[0]
         rc = lib.ps_kernel_launch(self.kernel._h, self._arr, self._n, self._Lp)
         if rc < 0:
             check(rc)
@@ -903,21 +985,27 @@ class Kernel:
             return PS_F64
         raise ArgumentError(f"argument {self.params[i]}: unsupported type {type(v).__name__}")
 
-    def _args(self, args):
+    def _args(self, args, launch=False):
         """The argument block of a launch. Blocks are kept for the last few argument lists
         (arrays by identity, through weak references, and scalars by type and value): a loop
-        that launches with the same arrays skips the rebuild."""
+        that launches with the same arrays skips the rebuild. For a launch (`launch`), the arrays
+        are checked to have memory when the block is built and again after any free()."""
         key = tuple(id(v) if isinstance(v, (DeviceArray, ArgSpec)) else (type(v), v) for v in args)
         cache = self.__dict__.setdefault("_arg_blocks", {})
         hit = cache.get(key)
         if hit is not None:
-            arr, refs = hit
+            arr, refs, gen = hit
             if all(r() is v for r, v in zip(refs, (v for v in args if isinstance(v, (DeviceArray, ArgSpec))))):
+                if launch and gen != _FREED[0]:
+                    _check_launch_args(self.name, args)
+                    cache[key] = (arr, refs, _FREED[0])
                 return arr
+        if launch:
+            _check_launch_args(self.name, args)
         arr = self._build_args(args)
         if len(cache) >= 16:
             cache.pop(next(iter(cache)))
-        cache[key] = (arr, [weakref.ref(v) for v in args if isinstance(v, (DeviceArray, ArgSpec))])
+        cache[key] = (arr, [weakref.ref(v) for v in args if isinstance(v, (DeviceArray, ArgSpec))], _FREED[0] if launch else -1)
         return arr
 
     def _build_args(self, args):
@@ -995,18 +1083,27 @@ class Kernel:
                 raise ArgumentError(ERRMSG_OVERLAP)
             stream, async_ = get_stream(len(ov) + 1), True
             ov.append(stream)
-        ensure_device()
         k, args = self._resolve(args)
-        arr = k._args(args)
+        arr = k._args(args, launch=True)
+        ensure_device()
         L = k._launch_struct(ranges, nblocks, nthreads, stream, async_, memopt, shmem)
         check(lib.ps_kernel_launch(k._h, arr, len(args), ctypes.byref(L)))
+
+    def _launcher_weak(self, args, kw):
+        k, args = self._resolve(tuple(args))
+        _check_launch_args(k.name, args)
+        ensure_device()
+        full = dict(ranges=None, nblocks=None, nthreads=None, stream=None, async_=True, memopt=None, shmem=None)
+        full.update(kw)
+        return BoundLaunch(k, args, full, weak=True)
 
     def launcher(self, *args, ranges=None, nblocks=None, nthreads=None, stream=None, async_=True, memopt=None, shmem=None):
         """A prepared launch of this kernel with these arguments (asynchronous by default):
         calling it launches again without rebuilding the argument block, for launch-bound loops
         and graph capture. It keeps its arguments alive; `set(i, value)` replaces one."""
-        ensure_device()
         k, args = self._resolve(args)
+        _check_launch_args(k.name, args)
+        ensure_device()
         return BoundLaunch(k, args, dict(ranges=ranges, nblocks=nblocks, nthreads=nthreads, stream=stream,
                                             async_=async_, memopt=memopt, shmem=shmem))
 
@@ -1424,6 +1521,8 @@ def hide_communication(boundary_width, computations, communication, ranges_inner
         raise KeywordArgumentError("Invalid keyword argument in @hide_communication: computation_calls must be >= 1.")
     key = _hc_key(calls, boundary_width, ranges_inner)
     plan = _HC_PLANS.get(key)
+    if plan is not None and not all(L.alive() for L in plan[0] + plan[1]):
+        plan = None  # an array of the cached plan was released (its id may now name another array)
     if plan is None:
         if ranges_inner is None:
             rngs = [_maxsize_ranges(a) for _, a, _ in calls]
@@ -1434,8 +1533,10 @@ def hide_communication(boundary_width, computations, communication, ranges_inner
         else:
             outer, inner = boundary_width, ranges_inner
         bs, ins = get_priority_stream(1), get_stream(1)
-        plan = ([k.launcher(*a, ranges=r, stream=bs, async_=True, **kw) for r in outer for k, a, kw in calls],
-                [k.launcher(*a, ranges=r, stream=ins, async_=True, **kw) for r in inner for k, a, kw in calls], bs, ins)
+        # prepared launches hold their arrays through weak references, so a cached plan never keeps
+        # fields of an earlier call signature alive in HBM
+        plan = ([k._launcher_weak(a, dict(ranges=r, stream=bs, async_=True, **kw)) for r in outer for k, a, kw in calls],
+                [k._launcher_weak(a, dict(ranges=r, stream=ins, async_=True, **kw)) for r in inner for k, a, kw in calls], bs, ins)
         if len(_HC_PLANS) >= 64:
             _HC_PLANS.pop(next(iter(_HC_PLANS)))
         _HC_PLANS[key] = plan

@@ -1,0 +1,58 @@
+"""The exact kernels that bench.py times, at the benched size: triad and 3-D heat diffusion
+(plain and memopt) on 512^3 FP64 fields, bit for bit against the oracle after a few steps.
+
+The memopt launch at this size is the one the benchmark runs: 128x4 blocks, LDS plane tiles,
+XCD-aware block order, loopsize 32 (examples/memcopy3D.jl:10-13,
+examples/diffusion3D_novis_noperf.jl:10-13 in the flux form of test/test_parallel.jl:419-426)."""
+import numpy as np
+import pytest
+
+import parallelstencil as ps
+from parallelstencil import workloads as W
+from oracle import ps_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MOD = __name__
+ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3, module=MOD)
+N = 512
+
+
+def test_triad_512():
+    rng = np.random.default_rng(11)
+    B = np.asfortranarray(rng.random((N, N, N)))
+    C = np.asfortranarray(rng.random((N, N, N)))
+    k = W.define_triad3d(MOD)
+    A, Bd, Cd = ps.zeros(N, N, N), ps.zeros(N, N, N), ps.zeros(N, N, N)
+    Bd.copy_from(B)
+    Cd.copy_from(C)
+    k(A, Bd, Cd)
+    got = A.to_host()
+    assert np.array_equal(got, B + C)
+    del B, C, got
+
+
+@pytest.mark.parametrize("memopt", [False, True])
+def test_diffusion_512_bitexact(memopt):
+    nt = 3
+    T0, Cp, p = O.diffusion_inputs(N, seed=1)
+    args = (p["lam"], p["dt"], p["dx"], p["dy"], p["dz"])
+    k = W.define_diffusion3d(memopt, MOD)
+    geo = k.geometry(*(ps.ArgSpec((N, N, N)) for _ in range(3)), *args)
+    if memopt:
+        assert geo.memopt and geo.nthreads == (128, 4, 1) and geo.nblocks == (4, 128, 16), geo
+    T, T2, Cd = ps.zeros(N, N, N), ps.zeros(N, N, N), ps.zeros(N, N, N)
+    T.copy_from(T0)
+    T2.copy_from(T0)
+    Cd.copy_from(Cp)
+    for _ in range(nt):
+        k(T2, T, Cd, *args, async_=True)
+        T, T2 = T2, T
+    ps.synchronize()
+    a, b = np.asfortranarray(T0.copy()), np.asfortranarray(T0.copy())
+    for _ in range(nt):
+        O.diffusion3d_flux(b, a, Cp, *args)
+        a, b = b, a
+    got = T.to_host()
+    bad = np.argwhere(got != a)
+    assert bad.shape[0] == 0, f"{bad.shape[0]} cells differ, first {bad[:4].tolist()}, max |d| = {np.abs(got - a).max()}"

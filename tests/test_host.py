@@ -407,3 +407,27 @@ end""")
 def test_forall_errors(text, match):
     with pytest.raises(ps.PSException, match=match):
         ps.parallel_indices(f"@parallel_indices (ix) function f!(A, n)\n    {text}\n    return\nend")
+
+
+def test_launch_refuses_argspecs_and_freed_arrays():
+    # an ArgSpec is a shape without memory, and a freed array has none left: both are refused on
+    # the host, before anything reaches a kernel (no GPU is touched)
+    k = _k("@parallel function launch_probe!(A, B)\n @all(A) = @all(B)\n return\nend")
+    spec = ps.ArgSpec((8, 8, 8))
+    with pytest.raises(ps.ArgumentError, match="ArgSpec"):
+        k(spec, ps.ArgSpec((8, 8, 8)))
+    with pytest.raises(ps.ArgumentError, match="ArgSpec"):
+        k.launcher(spec, ps.ArgSpec((8, 8, 8)))
+    f = ps.core.ps_field()
+    for d in range(3):
+        f.dims[d] = 8
+    f.ndims, f.dtype, f.pitch, f.plane = 3, ps.core.PS_F64, 8, 64
+    freed = ps.core.DeviceArray(f)  # ptr and base NULL: what free() leaves behind
+    with pytest.raises(ps.ArgumentError, match="freed"):
+        k(freed, freed)
+    view = ps.core.DeviceArray(ps.core.ps_field(), owner=False, parent=freed)
+    view.f.ptr, view.f.ndims, view.f.dtype = 1 << 44, 3, ps.core.PS_F64
+    for d in range(3):
+        view.f.dims[d] = 4
+    with pytest.raises(ps.ArgumentError, match="freed"):
+        k(view, view)
