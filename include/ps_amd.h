@@ -230,6 +230,14 @@ int ps_ranges_inner(const int64_t ranges[3][2], const int64_t boundary_width[3],
  * miniapps/HydroMech2D.jl:171.
  * op: 0 sum, 1 sum of |x|, 2 max |x|, 3 mean |x|, 4 L2 norm sqrt(sum x^2). Blocks on the stream. */
 int ps_field_reduce(const ps_field* f, int op, double* out, void* stream);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight solver loops, harnesses that forbid hipMalloc): ps_field_reduce of nfields
+ * fields (1..4) in one call, asynchronous on `stream`, out[i] (DEVICE memory, nfields doubles)
+ * receiving the statistic of fields[i]. The partial sums live in a static device array of the
+ * library (256 KiB per GPU, held until the process ends); the call allocates nothing, copies
+ * nothing to the host and does not wait, so it can be captured into a graph. More than 4 fields
+ * fail with PS_EARGUMENT. Calls that overlap on one GPU share that array and race. */
+int ps_field_reduce_noalloc(const ps_field* fields, int nfields, int op, double* out, void* stream);
 
 /* ---- events (measurement; no reference counterpart: the reference times with
  * wall-clock T_eff prints, README.md:457-464) ---- */

@@ -168,8 +168,57 @@ __global__ void __launch_bounds__(256) ps_reduce2_k(const double* __restrict__ p
   }
 }
 
-// blocks of the first reduction stage (each strides over the field): partial sums per block
+// Fields reduced by one call of ps_field_reduce_noalloc, and its scratch: the stage-1 partials of
+// each field (kReduceParts blocks x 4 statistics), 4 x 2048 x 4 x 8 B = 256 KiB per GPU. Each GPU
+// the library runs on holds its own copy until the process ends.
+constexpr int kNoallocFields = 4;
 constexpr int kReduceParts = 2048;
+static __device__ __align__(256) double scratch[kNoallocFields * kReduceParts * 4];
+
+struct ps_red_meta {
+  int nparts[kNoallocFields];
+  long long n[kNoallocFields];
+};
+
+// Block b combines the partials of field b exactly as ps_reduce2_k does, then writes the requested
+// statistic (op as in ps_field_reduce) to out[b].
+__global__ void __launch_bounds__(256) ps_reduce2_op_k(const double* __restrict__ part, ps_red_meta meta, int op,
+                                                       double* __restrict__ out) {
+  __shared__ double s0[256], s1[256], s2[256], s3[256];
+  const double* pb = part + (size_t)blockIdx.x * kReduceParts * 4;
+  const int nparts = meta.nparts[blockIdx.x];
+  double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) {
+    a += pb[4 * i];
+    b += pb[4 * i + 1];
+    c = ps_nanmax(c, pb[4 * i + 2]);
+    d += pb[4 * i + 3];
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  s2[threadIdx.x] = c;
+  s3[threadIdx.x] = d;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] = ps_nanmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+      s3[threadIdx.x] += s3[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const long long n = meta.n[blockIdx.x];
+    double r;
+    if (op == 0) r = s0[0];
+    else if (op == 1) r = s1[0];
+    else if (op == 2) r = s2[0];
+    else if (op == 3) r = n ? s1[0] / (double)n : 0.0;
+    else r = sqrt(s3[0]);
+    out[blockIdx.x] = r;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // global state
@@ -207,6 +256,8 @@ struct State {
   double* h_out = nullptr;
   int nparts_cap = 0;
   std::string cache_dir;
+  // device address of `scratch` per device (hipGetSymbolAddress at ps_init, that device current)
+  std::map<int, double*> scratch_ptr;
 };
 
 State& S() {
@@ -447,6 +498,13 @@ extern "C" int ps_init(int device) {
   PS_HIP(hipMalloc(&s.d_out, 4 * sizeof(double)));
   PS_HIP(hipMalloc(&s.d_part, sizeof(double) * 4 * kReduceParts));
   s.nparts_cap = kReduceParts;
+  {
+    // the array's name is a host-side stand-in: kernels get this device address instead
+    double* sp = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**)&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp) return fail(PS_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    s.scratch_ptr[device] = sp;
+  }
   s.device = device;
   s.inited = true;
   return 0;
@@ -736,6 +794,50 @@ extern "C" int ps_field_reduce(const ps_field* f_in, int op, double* out, void* 
   else if (op == 2) *out = h[2];
   else if (op == 3) *out = n ? h[1] / (double)n : 0.0;
   else *out = std::sqrt(h[3]);
+  return 0;
+}
+
+// The variant of ps_field_reduce for callers that cannot allocate: up to kNoallocFields fields per
+// call, asynchronous on `stream`, results into the caller's device memory out[0..nfields-1]. The
+// partials go to the library's static scratch array of the current device; nothing is allocated,
+// copied to the host or waited for, so the call can be captured into a graph. Every partial that
+// stage 2 reads is written by stage 1 of the same call, so nothing needs clearing between calls.
+// Calls that overlap on one GPU share the scratch array and race.
+extern "C" int ps_field_reduce_noalloc(const ps_field* fields, int nfields, int op, double* out, void* stream) {
+  if (int r = require_init()) return r;
+  if (nfields < 1 || nfields > kNoallocFields)
+    return fail(PS_EARGUMENT, "ps_field_reduce_noalloc: nfields must be 1.." + std::to_string(kNoallocFields) + " (obtained " +
+                                  std::to_string(nfields) + "); the scratch array holds the partials of at most " +
+                                  std::to_string(kNoallocFields) + " fields");
+  if (op < 0 || op > 4) return fail(PS_EARGUMENT, "ps_field_reduce_noalloc: op must be 0..4");
+  if (!fields || !out) return fail(PS_EARGUMENT, "ps_field_reduce_noalloc: fields and out must not be NULL");
+  State& s = S();
+  auto it = s.scratch_ptr.find(s.device);
+  if (it == s.scratch_ptr.end()) return fail(PS_EINTERNAL, "ps_field_reduce_noalloc: no scratch address for this device");
+  double* part = it->second;
+  hipStream_t st = stream_of(stream);
+  ps_red_meta meta{};
+  for (int i = 0; i < nfields; ++i) {
+    const ps_field fd = data_of(fields[i]);
+    const long long n = fd.dims[0] * fd.dims[1] * fd.dims[2];
+    if (n > 0 && !fd.ptr) return fail(PS_EARGUMENT, "ps_field_reduce_noalloc: field " + std::to_string(i + 1) + " has no device memory");
+    if (fd.dtype != PS_F64 && fd.dtype != PS_F32 && fd.dtype != PS_BF16)
+      return fail(PS_EARGUMENT, "ps_field_reduce_noalloc: floating-point fields only");
+    meta.nparts[i] = (int)std::min<long long>(std::max<long long>((n + 255) / 256, 1), kReduceParts);
+    meta.n[i] = n;
+  }
+  for (int i = 0; i < nfields; ++i) {
+    const ps_field fd = data_of(fields[i]);
+    double* pf = part + (size_t)i * kReduceParts * 4;
+    const dim3 g(meta.nparts[i]), b(256);
+    switch (fd.dtype) {
+      case PS_F64: hipLaunchKernelGGL(ps_reduce1_k<double>, g, b, 0, st, (const double*)fd.ptr, fd.dims[0], fd.dims[1], fd.dims[2], fd.pitch, fd.plane, pf); break;
+      case PS_F32: hipLaunchKernelGGL(ps_reduce1_k<float>, g, b, 0, st, (const float*)fd.ptr, fd.dims[0], fd.dims[1], fd.dims[2], fd.pitch, fd.plane, pf); break;
+      default: hipLaunchKernelGGL(ps_reduce1_k<unsigned short>, g, b, 0, st, (const unsigned short*)fd.ptr, fd.dims[0], fd.dims[1], fd.dims[2], fd.pitch, fd.plane, pf); break;
+    }
+  }
+  hipLaunchKernelGGL(ps_reduce2_op_k, dim3(nfields), dim3(256), 0, st, (const double*)part, meta, op, out);
+  PS_HIP(hipGetLastError());
   return 0;
 }
 

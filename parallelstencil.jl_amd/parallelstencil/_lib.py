@@ -158,6 +158,7 @@ _SIGS = {
     "ps_field_fill": (ctypes.c_int, [_P(ps_field), ctypes.c_double, ctypes.c_void_p]),
     "ps_field_copy": (ctypes.c_int, [_P(ps_field), _P(ps_field), ctypes.c_void_p]),
     "ps_field_reduce": (ctypes.c_int, [_P(ps_field), ctypes.c_int, _P(ctypes.c_double), ctypes.c_void_p]),
+    "ps_field_reduce_noalloc": (ctypes.c_int, [_P(ps_field), ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "ps_stream_get": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _P(ctypes.c_void_p)]),
     "ps_stream_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_kernel_define": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P(ctypes.c_void_p)]),
