@@ -40,18 +40,44 @@ def time_kernel(ps, launch, steps, warmup):
 
 
 def _traffic(n, head):
-    """HBM bytes per launch of the head kernel, from the committed rocprofv3 counter summary
-    (FETCH_SIZE scaled by the triad calibration + WRITE_SIZE); None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"diffusion3d_{n}_bench.md")
+    """(HBM bytes per launch of the head kernel, where the figure comes from): read from the
+    committed rocprofv3 counter summary of the same kernel (FETCH_SIZE scaled by the triad
+    calibration + WRITE_SIZE), not measured in this run; (None, None) when there is none."""
+    rel = os.path.join("profiles", f"diffusion3d_{n}_bench.md")
+    path = os.path.join(ROOT, rel)
     if head == "triad" or not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         for line in f:
             p = line.split()
             if len(p) == 3 and p[0] == "hbm_bytes_per_launch" and p[1].startswith("psk_diffusion3D") and \
                     (("_memopt" in p[1]) == (head == "memopt")) and "_fastmath" not in p[1]:
-                return int(p[2])
-    return None
+                return int(p[2]), f"{rel} (rocprofv3 FETCH_SIZE/WRITE_SIZE of {p[1]}, committed profile, not this run)"
+    return None, None
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def per_launch_ms(ps, launch, n=20):
+    """Event-timed durations of n single launches (after the timed loop): min and median, to
+    show the spread behind the mean."""
+    ev = [(ps.Event(), ps.Event()) for _ in range(n)]
+    for i, (a, b) in enumerate(ev):
+        a.record()
+        launch(i)
+        b.record()
+    ps.synchronize()
+    t = sorted(a.elapsed_ms(b) for a, b in ev)
+    return t[0], t[len(t) // 2]
 
 
 def cpu_baseline(n, budget_s=12.0):
@@ -72,59 +98,82 @@ def cpu_baseline(n, budget_s=12.0):
         steps += 1
     dt = (time.perf_counter() - t0) / steps
     return {"value": round(3 * 8 * n ** 3 / dt / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": "port",
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
             "sample": f"{steps} steps of the diffusion3D {n}^3 FP64 step (oracle, contraction allowed, OpenMP)"}
 
 
 def stokes_warm_start(args):
     """Stokes2D time to tolerance (residual 1e-8, FP64 on the GPU), warm start from the U-Net
-    (inference included) against the plain PT iteration from rest, on held-out samples. The
-    network is trained here, on the GPU with the library's own kernels, on samples the solver
-    generates."""
+    (inference included) against the plain PT iteration from rest, on held-out samples at the
+    training resolution and at --stokes-n2. The network is trained here, on the GPU with the
+    library's own kernels, on samples the solver generates. The solver is the two-launch fused
+    iteration (bit-identical to the miniapp's six launches); with --stokes-ts maxV the velocity
+    pseudo-time steps use the 3x3 maximum viscosity, which keeps the iteration stable at the
+    contrasts of --stokes-contrast."""
     import numpy as np
     import parallelstencil as ps
     from parallelstencil import unet as U
-    from parallelstencil import workloads as W
+    from parallelstencil.stokes_fused import Stokes2DFused
 
     mod = "bench.stokes"
     ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2, module=mod)
-    n, contrast = args.stokes_n, 1.0
-    solver = W.Stokes2D(mod, n, n, eps=1e-8, nout=200)
+    n, contrast = args.stokes_n, args.stokes_contrast
+    solver = Stokes2DFused(mod, n, n, eps=1e-8, nout=200, eta_ts=args.stokes_ts, history=True)
     t0 = time.perf_counter()
-    data, stats = U.generate_samples(solver, range(1, args.train + 1), contrast)
+    data, stats = [], []
+    for i0 in range(1, args.train + 1, 32):
+        d, st = U.generate_samples(solver, range(i0, min(args.train, i0 + 31) + 1), contrast)
+        data += d
+        stats += st
+        log(f"stokes {n}^2: {len(stats)} training solves, {time.perf_counter() - t0:.1f} s, last {st[-1][0]} iterations")
     t1 = time.perf_counter()
-    ok = [s[2] <= solver.eps for s in stats]
-    data = [d for d, k in zip(data, ok) if k]  # samples on which the plain iteration diverged are dropped
-    log(f"stokes {n}^2: {len(stats)} training solves in {t1 - t0:.1f} s, {sum(ok)} converged, "
-        f"mean {np.mean([s[0] for s, k in zip(stats, ok) if k]):.0f} PT iterations")
+    ok = [s_[2] <= solver.eps for s_ in stats]
+    data = [d for d, k in zip(data, ok) if k]  # samples on which the plain iteration did not converge are dropped
+    its = [s_[0] for s_, k in zip(stats, ok) if k]
+    log(f"stokes {n}^2: {len(stats)} training solves in {t1 - t0:.1f} s, {sum(ok)} converged, mean {np.mean(its):.0f} PT iterations, "
+        f"{1e6 * (t1 - t0) / max(1, sum(s_[0] for s_ in stats)):.2f} us per iteration")
     ws = U.WarmStart(n, n, base=32, seed=0)
     X, T, Mk = U.dataset(ws, data)
     ws.set_normalisation(T, Mk)
+    del data
     hist = ws.train(X, T, Mk, epochs=args.epochs, batch=8, lr=3e-3, log=log, cosine=True)
     ps.synchronize()
     t2 = time.perf_counter()
     log(f"training: {args.epochs} epochs in {t2 - t1:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
-    ev_all = U.evaluate(ws, solver, range(100001, 100001 + args.eval), contrast)
-    ev = [e for e in ev_all if e["err_cold"] <= solver.eps]  # held-out samples the plain iteration solves
-    cold_s, warm_s = sum(e["cold_s"] for e in ev), sum(e["warm_s"] for e in ev)
-    cold_it, warm_it = np.mean([e["cold_iters"] for e in ev]), np.mean([e["warm_iters"] for e in ev])
-    log(f"eval: cold {cold_it:.0f} iterations / {cold_s:.3f} s, warm {warm_it:.0f} iterations / {warm_s:.3f} s")
+    evals = {}
+    sizes = [(n, args.eval, solver, None)]
+    if args.stokes_n2 and args.eval2:
+        s2 = Stokes2DFused(mod, args.stokes_n2, args.stokes_n2, eps=1e-8, nout=200, eta_ts=args.stokes_ts, history=True)
+        sizes.append((args.stokes_n2, args.eval2, s2, U.resampled_predictor(ws, n, args.stokes_n2)))
+    for m, ne, sv, pred in sizes:
+        ev_all = U.evaluate(ws, sv, range(100001, 100001 + ne), contrast, predict=pred)
+        ev = [e for e in ev_all if e["err_cold"] <= sv.eps]  # held-out samples the plain iteration solves
+        cold_s, warm_s = sum(e["cold_s"] for e in ev), sum(e["warm_s"] for e in ev)
+        cold_it, warm_it = np.mean([e["cold_iters"] for e in ev]), np.mean([e["warm_iters"] for e in ev])
+        log(f"eval {m}^2: cold {cold_it:.0f} iterations / {cold_s:.3f} s, warm {warm_it:.0f} iterations / {warm_s:.3f} s, "
+            f"gain {cold_s / warm_s:.3f}x")
+        evals[m] = dict(n=m, samples=len(ev), dropped_not_converged=len(ev_all) - len(ev), cold_iters_mean=float(cold_it),
+                        warm_iters_mean=float(warm_it), cold_s=cold_s, warm_s=warm_s, gain=cold_s / warm_s,
+                        warm_fallbacks=sum(e["fallback"] for e in ev), warm_converged=all(e["err_warm"] <= sv.eps for e in ev),
+                        per_sample=[{k: (round(v, 6) if isinstance(v, float) else v) for k, v in e.items()} for e in ev])
+    head = evals[n]
     out = {
         "metric": "T_eff GB/s, %HBM peak (diffusion3D 512³); Stokes2D time-to-tol: warm vs plain",
-        "value": round(cold_s / warm_s, 3),
+        "value": round(head["gain"], 3),
         "unit": "x (plain PT time / warm-started time, inference included)",
-        "n_gpus": 1, "steps": len(ev), "warmup": 0, "ms_per_step": round(1e3 * warm_s / len(ev), 3),
+        "n_gpus": 1, "steps": head["samples"], "warmup": 0, "ms_per_step": round(1e3 * head["warm_s"] / max(1, head["samples"]), 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 (solver), bf16/fp32 (network)",
         "data": "synthetic",
         "config": {"workload": f"Stokes2D {n}x{n} pseudo-transient to residual 1e-8, 1-8 inclusions, log10 contrast in "
-                               f"[-{contrast:g}, {contrast:g}]; U-Net 4 levels, 32 base channels, BF16",
-                   "train_samples": len(data), "eval_samples": len(ev), "epochs": args.epochs},
-        "stokes": {"cold_iters_mean": float(cold_it), "warm_iters_mean": float(warm_it), "cold_s": cold_s, "warm_s": warm_s,
-                   "datagen_s": t1 - t0, "train_s": t2 - t1, "loss_first": hist[0], "loss_last": hist[-1],
-                   "train_dropped_diverged": len(stats) - len(data), "eval_dropped_diverged": len(ev_all) - len(ev),
-                   "warm_fallbacks": sum(e["fallback"] for e in ev),
-                   "warm_converged": all(e["err_warm"] <= solver.eps for e in ev),
-                   "per_sample": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in e.items()} for e in ev]},
+                               f"[-{contrast:g}, {contrast:g}], pseudo-time steps {args.stokes_ts}; U-Net 4 levels, 32 base channels, BF16",
+                   "train_samples": len(X), "eval_samples": {str(k): v["samples"] for k, v in evals.items()}, "epochs": args.epochs},
+        "stokes": {"datagen_s": t1 - t0, "train_s": t2 - t1, "loss_first": hist[0], "loss_last": hist[-1],
+                   "train_dropped_not_converged": len(stats) - len(X), "train_iters_mean": float(np.mean(its)),
+                   "eval": {str(k): {kk: vv for kk, vv in v.items() if kk != "per_sample"} for k, v in evals.items()}},
     }
+    if args.stokes_out:
+        with open(args.stokes_out, "w") as f:
+            json.dump(dict(out, per_sample={str(k): v["per_sample"] for k, v in evals.items()}, loss_history=hist), f)
     print(json.dumps(out), flush=True)
 
 
@@ -690,9 +739,14 @@ def main():
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--adjoint-n", type=int, default=2048)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
-    ap.add_argument("--stokes-n", type=int, default=127)
+    ap.add_argument("--stokes-n", type=int, default=511)
+    ap.add_argument("--stokes-n2", type=int, default=1023, help="stokes: second evaluation resolution (0: none)")
+    ap.add_argument("--stokes-contrast", type=float, default=2.0, help="stokes: log10 viscosity contrast range [-c, c]")
+    ap.add_argument("--stokes-ts", default="maxV", help="stokes: pseudo-time steps 'av' (miniapp) or 'maxV'")
+    ap.add_argument("--stokes-out", default="", help="stokes: JSON file for per-sample results and residual histories")
     ap.add_argument("--train", type=int, default=256, help="stokes: training samples")
-    ap.add_argument("--eval", type=int, default=24, help="stokes: held-out samples")
+    ap.add_argument("--eval", type=int, default=24, help="stokes: held-out samples at --stokes-n")
+    ap.add_argument("--eval2", type=int, default=4, help="stokes: held-out samples at --stokes-n2")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--shmem-n", type=int, default=16384)
     ap.add_argument("--stokes3d-n", type=int, default=127)
@@ -780,15 +834,17 @@ def main():
             k(st[1], st[0], Cpd, *pa, async_=True)
             st[0], st[1] = st[1], st[0]
         ms, kms = time_kernel(ps, step, K, Wm)
-        res[name] = (ms, kms)
-        log(f"diffusion {name} {n}^3: {ms:.4f} ms/step wall, {kms:.4f} ms kernel, T_eff {bytes_step / kms / 1e6:.1f} GB/s")
+        lo, med = per_launch_ms(ps, step)
+        res[name] = (ms, kms, lo, med)
+        log(f"diffusion {name} {n}^3: {ms:.4f} ms/step wall, {kms:.4f} ms kernel (single launches: min {lo:.4f}, median {med:.4f}), "
+            f"T_eff {bytes_step / kms / 1e6:.1f} GB/s")
     del T0, Cp
 
     head = "memopt" if "memopt" in res else ("plain" if "plain" in res else "triad")
-    ms, kms = res[head]
+    ms, kms = res[head][:2]
     teff_wall = bytes_step / (ms * 1e-3) / 1e9
     achieved = bytes_step / (kms * 1e-3) / 1e9
-    traffic = _traffic(n, head)
+    traffic, traffic_src = _traffic(n, head)
     out = {
         "metric": "T_eff GB/s, %HBM peak (diffusion3D 512³); Stokes2D time-to-tol: warm vs plain",
         "value": round(teff_wall, 2),
@@ -806,9 +862,11 @@ def main():
                    "n": n, "fields": "T, T2, Cp (3 x 8 B per cell per step)", "hbm_peak_gbs": HBM_PEAK_GBS},
         "hbm_frac": round(teff_wall / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
-        "kernels": {k_: {"ms_wall": round(v[0], 5), "ms_kernel": round(v[1], 5),
-                         "T_eff_gbs": round(bytes_step / (v[1] * 1e-3) / 1e9, 2)} for k_, v in res.items()},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src},
+        "kernels": {k_: dict({"ms_wall": round(v[0], 5), "ms_kernel": round(v[1], 5),
+                              "T_eff_gbs": round(bytes_step / (v[1] * 1e-3) / 1e9, 2)},
+                             **({"ms_single_min": round(v[2], 5), "ms_single_median": round(v[3], 5)} if len(v) > 2 else {}))
+                    for k_, v in res.items()},
     }
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -75,6 +75,14 @@ typedef struct ps_field {
   int32_t celldims[4];  /* cell dimensions, 1 for absent ones                  */
   int64_t blocklen;     /* values of one cell field stored contiguously        */
   const char* cellnames;/* comma-separated field names of a @CellType, or NULL */
+  /* Views (ps_field_view): the view's first element sits at index voff+1 of its parent
+   * (per dimension, 0-based offset) and the parent holds pdims elements. pdims all 0: the field
+   * is not a view (its own parent). Kernels defined with padding=true address the parents of
+   * their field arguments, as the padded Fields of src/ParallelKernel/FieldAllocators.jl:622-634
+   * (SubArray views of a larger parent) are addressed through A.parent
+   * (src/ParallelKernel/parallel.jl:572-581). */
+  int64_t voff[3];
+  int64_t pdims[3];
 } ps_field;
 
 typedef struct ps_arg {

@@ -666,12 +666,26 @@ EV Emitter::call(const NodeP& n) {
     if (d < 1) throw PsError(PS_EARGUMENT, "size(A, d): d must be >= 1");
     if (d > 3) return EV{"((ps_int)1)", Ty::Int};
     static const char* s[3] = {"_nx", "_ny", "_nz"};
+    // padding=true: the kernel holds the parent; size() is the view's (its box in the parent)
+    if (K.opt.padding) return EV{"(" + an(ai) + "_hi" + "xyz"[d - 1] + " - " + an(ai) + "_lo" + "xyz"[d - 1] + " + 1)", Ty::Int};
     return EV{an(ai) + s[d - 1], Ty::Int};
   }
+  if (f == "__psize" || f == "__first" || f == "__last") {  // padding=true: parent extent, view box
+    int ai = n->a.size() == 2 && n->a[0]->k == NK::Ident ? field_index(n->a[0]->s) : -1;
+    if (ai < 0 || !(n->a[1]->k == NK::Num && n->a[1]->is_int) || n->a[1]->ival < 1 || n->a[1]->ival > 3)
+      throw PsError(PS_EINTERNAL, f + ": expects an array argument and a dimension 1..3");
+    const int d = (int)n->a[1]->ival - 1;
+    static const char* s[3] = {"_nx", "_ny", "_nz"};
+    if (f == "__psize") return EV{an(ai) + s[d], Ty::Int};
+    return EV{an(ai) + (f == "__first" ? "_lo" : "_hi") + "xyz"[d], Ty::Int};
+  }
+  if (f == "__pgt") throw PsError(PS_EINTERNAL, "padding: a parent-size comparison was not specialised before code generation");
   if (f == "length") {
     int ai = field_index(n->a.at(0)->s);
     if (ai < 0) throw PsError(PS_EARGUMENT, "length(): not an array argument");
     std::string A = an(ai);
+    if (K.opt.padding)
+      return EV{"((" + A + "_hix - " + A + "_lox + 1)*(" + A + "_hiy - " + A + "_loy + 1)*(" + A + "_hiz - " + A + "_loz + 1))", Ty::Int};
     return EV{"(" + A + "_nx*" + A + "_ny*" + A + "_nz)", Ty::Int};
   }
   if (f == "cellsize") {
@@ -682,6 +696,10 @@ EV Emitter::call(const NodeP& n) {
     long long d = n->a[1]->ival;
     if (d < 1) throw PsError(PS_EARGUMENT, "cellsize(A, d): d must be >= 1");
     return EV{"((ps_int)" + std::to_string(d <= 3 ? kinds[ai].celldims[d - 1] : 1) + ")", Ty::Int};
+  }
+  if ((f == "firstindex" || f == "lastindex") && K.opt.padding && n->a.size() == 2) {
+    // padding=true: A.indices[d][1] / A.indices[d][end] (src/ParallelKernel/kernel_language.jl:605-617)
+    return call(mk(NK::Call, f == "firstindex" ? "__first" : "__last", n->a));
   }
   if (f == "firstindex") return EV{"((ps_int)1)", Ty::Int};
   if (f == "lastindex") {
@@ -951,7 +969,7 @@ void Emitter::assign(const NodeP& s) {
     o << pad() << "if (ps_chk(" << ic[0] << ", " << ic[1] << ", " << ic[2] << ", " << A << "_nx, " << A << "_ny, " << A
       << "_nz, ps_err)) " << A << "[" << off << "] = " << v << ";\n";
   } else {
-    if (K.opt.nt_store) o << pad() << "__builtin_nontemporal_store(" << v << ", &" << A << "[" << off << "]);\n";
+    if (V.nt) o << pad() << "__builtin_nontemporal_store(" << v << ", &" << A << "[" << off << "]);\n";
     else o << pad() << A << "[" << off << "] = " << v << ";\n";
   }
   if (blk) {
@@ -1761,70 +1779,118 @@ void Emitter::emit_memopt_body(int L) {
     for (int j = 0; j < t.nslots; ++j)
       for (int d = 0; d < PD; ++d)
         o << pad() << t.name << "_s" << j << "_h" << d << " = " << ldh(t, j, "ps_zb + (" + std::to_string(t.oz + d) + ")", within(d)) << ";\n";
-  o << pad() << "for (ps_int " << lc << " = ps_zb; " << lc << " < ps_ze; ++" << lc << ") {\n";
-  ++ind;
-  // the loads issued in this step serve step lc + PD
-  o << pad() << "const bool ps_more = " << lc << " + " << PD << " < ps_ze;\n";
-  for (auto& c : cols) {
-    int w = c.zhi - c.zlo + 1;
-    o << pad() << c.name << "_" << (w - 1) << " = " << c.name << "_n0;\n";
-    for (int d = 0; d + 1 < PD; ++d) o << pad() << c.name << "_n" << d << " = " << c.name << "_n" << d + 1 << ";\n";
-    o << pad() << c.name << "_n" << PD - 1 << " = " << ldq(c, lc + " + (" + std::to_string(c.zhi + PD) + ")", "ps_more && ") << ";\n";
-  }
-  if (!tiles.empty()) {
-    o << pad() << "const int ps_buf = (int)((" << lc << " - ps_zb) & 1);\n";
+  // Unchecked variants split the stream in two loops: in the first, every load issued for a later
+  // step (plane lc + offset + PD) lies inside its array, so those loads need neither the end-of-
+  // chunk nor the z-range guard, and their x-y clamp is hoisted out of the loop; the second loop
+  // (the last PD steps of the chunk, and planes near the array's end) keeps the guarded loads.
+  const bool split = !V.checked && (!cols.empty() || !tiles.empty());
+  if (split) {
+    // the first loop ends where the furthest load of any array would leave it, and is empty when
+    // the chunk starts so low that a load would fall below the array
+    int minoff = 1 << 30;
+    std::string zs = "ps_ze - " + std::to_string(PD);
+    for (auto& c : cols) {
+      zs = "ps_min(" + zs + ", " + an(c.ai) + szn[L] + " - (" + std::to_string(c.zhi + PD) + ") + 1)";
+      minoff = std::min(minoff, c.zhi + PD);
+    }
     for (auto& t : tiles) {
-      const Col& c = cols[t.own_col];
-      o << pad() << t.name << "[ps_buf][(ps_ly + " << (c.o[1] - t.ylo) << ") * " << t.W << " + ps_lx + " << (c.o[0] - t.xlo)
-        << "] = " << c.name << "_" << (t.oz - c.zlo) << ";\n";
+      zs = "ps_min(" + zs + ", " + an(t.ai) + "_nz - (" + std::to_string(t.oz + PD) + ") + 1)";
+      minoff = std::min(minoff, t.oz + PD);
+    }
+    o << pad() << "const ps_int ps_zs = (ps_zb >= (ps_int)(" << (1 - minoff) << ")) ? ps_max(ps_zb, " << zs << ") : ps_zb;\n";
+    for (auto& c : cols) o << pad() << "const ps_int " << c.name << "_bs = " << c.name << "_ok ? " << c.name << "_b : (ps_int)0;\n";
+    for (auto& t : tiles)
       for (int j = 0; j < t.nslots; ++j) {
         std::string S = t.name + "_s" + std::to_string(j);
-        o << pad() << "if (" << S << "_v) " << t.name << "[ps_buf][" << S << "_l] = " << S << "_h0;\n";
-        for (int d = 0; d + 1 < PD; ++d) o << pad() << S << "_h" << d << " = " << S << "_h" << d + 1 << ";\n";
-        o << pad() << S << "_h" << PD - 1 << " = " << ldh(t, j, lc + " + (" + std::to_string(t.oz + PD) + ")", "ps_more && ") << ";\n";
+        o << pad() << "const ps_int " << S << "_bs = " << S << "_ok ? " << S << "_b : (ps_int)0;\n";
+      }
+  }
+  auto ldq_fast = [&](const Col& c, const std::string& z) {
+    std::string A = an(c.ai);
+    std::string v = A + "[" + c.name + "_bs + (" + z + " - 1)*" + A + str[L] + "]";
+    if (kinds[c.ai].dtype == PS_BF16) v = "ps_bf16_ld(" + v + ")";
+    return v;
+  };
+  auto ldh_fast = [&](const Tile& t, int j, const std::string& z) {
+    std::string A = an(t.ai);
+    std::string S = t.name + "_s" + std::to_string(j);
+    std::string v = A + "[" + S + "_bs + (" + z + " - 1)*" + A + "_sxy]";
+    if (kinds[t.ai].dtype == PS_BF16) v = "ps_bf16_ld(" + v + ")";
+    return v;
+  };
+  auto emit_loop = [&](const std::string& to, bool fast) {
+    o << pad() << "for (; " << lc << " < " << to << "; ++" << lc << ") {\n";
+    ++ind;
+    // the loads issued in this step serve step lc + PD
+    if (!fast) o << pad() << "const bool ps_more = " << lc << " + " << PD << " < ps_ze;\n";
+    for (auto& c : cols) {
+      int w = c.zhi - c.zlo + 1;
+      o << pad() << c.name << "_" << (w - 1) << " = " << c.name << "_n0;\n";
+      for (int d = 0; d + 1 < PD; ++d) o << pad() << c.name << "_n" << d << " = " << c.name << "_n" << d + 1 << ";\n";
+      const std::string z = lc + " + (" + std::to_string(c.zhi + PD) + ")";
+      o << pad() << c.name << "_n" << PD - 1 << " = " << (fast ? ldq_fast(c, z) : ldq(c, z, "ps_more && ")) << ";\n";
+    }
+    if (!tiles.empty()) {
+      o << pad() << "const int ps_buf = (int)((" << lc << " - ps_zb) & 1);\n";
+      for (auto& t : tiles) {
+        const Col& c = cols[t.own_col];
+        o << pad() << t.name << "[ps_buf][(ps_ly + " << (c.o[1] - t.ylo) << ") * " << t.W << " + ps_lx + " << (c.o[0] - t.xlo)
+          << "] = " << c.name << "_" << (t.oz - c.zlo) << ";\n";
+        for (int j = 0; j < t.nslots; ++j) {
+          std::string S = t.name + "_s" + std::to_string(j);
+          o << pad() << "if (" << S << "_v) " << t.name << "[ps_buf][" << S << "_l] = " << S << "_h0;\n";
+          for (int d = 0; d + 1 < PD; ++d) o << pad() << S << "_h" << d << " = " << S << "_h" << d + 1 << ";\n";
+          const std::string z = lc + " + (" + std::to_string(t.oz + PD) + ")";
+          // lanes without a halo slot skip the load (whole waves of them branch around it)
+          o << pad() << (fast ? "if (" + S + "_v) " : std::string()) << S << "_h" << PD - 1 << " = "
+            << (fast ? ldh_fast(t, j, z) : ldh(t, j, z, "ps_more && ")) << ";\n";
+        }
+      }
+      o << pad() << "__syncthreads();\n";
+    }
+    // center offsets of every field at this step
+    for (size_t i = 0; i < kinds.size(); ++i) {
+      if (!kinds[i].is_field) continue;
+      std::string A = an((int)i);
+      o << pad() << "const ps_int " << A << "_o = (ix - 1) + (iy - 1)*" << A << "_sx + (iz - 1)*" << A << "_sxy;\n";
+    }
+    cache.clear();
+    for (auto& c : cols) {
+      int w = c.zhi - c.zlo + 1;
+      for (int k = 0; k < w; ++k) {
+        int off[3] = {c.o[0], c.o[1], c.o[2]};
+        off[L] = c.zlo + k;
+        cache[std::make_tuple(c.ai, off[0], off[1], off[2])] = c.name + "_" + std::to_string(k);
       }
     }
-    o << pad() << "__syncthreads();\n";
-  }
-  // center offsets of every field at this step
-  for (size_t i = 0; i < kinds.size(); ++i) {
-    if (!kinds[i].is_field) continue;
-    std::string A = an((int)i);
-    o << pad() << "const ps_int " << A << "_o = (ix - 1) + (iy - 1)*" << A << "_sx + (iz - 1)*" << A << "_sxy;\n";
-  }
-  cache.clear();
-  for (auto& c : cols) {
-    int w = c.zhi - c.zlo + 1;
-    for (int k = 0; k < w; ++k) {
-      int off[3] = {c.o[0], c.o[1], c.o[2]};
-      off[L] = c.zlo + k;
-      cache[std::make_tuple(c.ai, off[0], off[1], off[2])] = c.name + "_" + std::to_string(k);
+    for (auto& t : tiles)
+      for (auto& p : t.cols_at) {
+        auto key = std::make_tuple(t.ai, p.first, p.second, t.oz);
+        if (cache.count(key)) continue;  // a register of the thread's own column
+        cache[key] = t.name + "[ps_buf][(ps_ly + " + std::to_string(p.second - t.ylo) + ") * " + std::to_string(t.W) + " + ps_lx + " +
+                     std::to_string(p.first - t.xlo) + "]";
+      }
+    in_loop = true;
+    if (V.lds) {
+      o << pad() << "if (ps_act) {\n";
+      ++ind;
     }
-  }
-  for (auto& t : tiles)
-    for (auto& p : t.cols_at) {
-      auto key = std::make_tuple(t.ai, p.first, p.second, t.oz);
-      if (cache.count(key)) continue;  // a register of the thread's own column
-      cache[key] = t.name + "[ps_buf][(ps_ly + " + std::to_string(p.second - t.ylo) + ") * " + std::to_string(t.W) + " + ps_lx + " +
-                   std::to_string(p.first - t.xlo) + "]";
+    block(K.body);
+    if (V.lds) {
+      --ind;
+      o << pad() << "}\n";
     }
-  in_loop = true;
-  if (V.lds) {
-    o << pad() << "if (ps_act) {\n";
-    ++ind;
-  }
-  block(K.body);
-  if (V.lds) {
+    in_loop = false;
+    for (auto& c : cols) {
+      int w = c.zhi - c.zlo + 1;
+      for (int k = 0; k + 1 < w; ++k) o << pad() << c.name << "_" << k << " = " << c.name << "_" << (k + 1) << ";\n";
+    }
     --ind;
     o << pad() << "}\n";
-  }
-  in_loop = false;
-  for (auto& c : cols) {
-    int w = c.zhi - c.zlo + 1;
-    for (int k = 0; k + 1 < w; ++k) o << pad() << c.name << "_" << k << " = " << c.name << "_" << (k + 1) << ";\n";
-  }
-  --ind;
-  o << pad() << "}\n";
+  };
+  o << pad() << "ps_int " << lc << " = ps_zb;\n";
+  if (split) emit_loop("ps_zs", true);
+  emit_loop("ps_ze", false);
 }
 
 Emitted Emitter::run() {
@@ -1857,6 +1923,11 @@ Emitted Emitter::run() {
       sig << "    ps_int " << A << "_nx, ps_int " << A << "_ny, ps_int " << A << "_nz, ps_int " << A << "_sx, ps_int " << A << "_sxy,\n";
       E.params.push_back({0, (int)i, 0, 8, 0});
       for (int w = 0; w < 5; ++w) E.params.push_back({1, (int)i, w, ib, 0});
+      if (K.opt.padding) {  // the argument's box in its parent: firstindex / lastindex per dimension
+        sig << "    ps_int " << A << "_lox, ps_int " << A << "_loy, ps_int " << A << "_loz, ps_int " << A << "_hix, ps_int " << A
+            << "_hiy, ps_int " << A << "_hiz,\n";
+        for (int w = 0; w < 6; ++w) E.params.push_back({10, (int)i, w, ib, 0});
+      }
       if (k.ncomp > 0) {  // CellArray: values of one cell field stored contiguously
         sig << "    ps_int " << A << "_B,\n";
         E.params.push_back({9, (int)i, 0, ib, 0});
@@ -2029,11 +2100,25 @@ struct Prover {
         return true;
       }
       case NK::Call: {
-        if (e->s == "size" && e->a.size() == 2 && e->a[0]->k == NK::Ident && e->a[1]->k == NK::Num && e->a[1]->is_int) {
+        if ((e->s == "size" || e->s == "__psize" || e->s == "__first" || e->s == "__last") && e->a.size() == 2 &&
+            e->a[0]->k == NK::Ident && e->a[1]->k == NK::Num && e->a[1]->is_int) {
           auto it = K.param_index.find(e->a[0]->s);
           if (it == K.param_index.end() || !S.kinds[it->second].is_field) return false;
           long long d = e->a[1]->ival;
-          v = (d >= 1 && d <= 3) ? S.dims[it->second][d - 1] : 1;
+          const int i = it->second;
+          const bool box = K.opt.padding && i < (int)S.vlo.size();
+          if (d < 1 || d > 3) {
+            if (e->s == "__first" || e->s == "__last") return false;
+            v = 1;
+          } else if (e->s == "__psize" || (e->s == "size" && !K.opt.padding)) {
+            v = S.dims[i][d - 1];
+          } else if (!box) {
+            return false;
+          } else if (e->s == "size") {
+            v = S.vhi[i][d - 1] - S.vlo[i][d - 1] + 1;
+          } else {
+            v = e->s == "__first" ? S.vlo[i][d - 1] : S.vhi[i][d - 1];
+          }
           return true;
         }
         if (e->s == "cellsize" && e->a.size() == 2 && e->a[0]->k == NK::Ident && e->a[1]->k == NK::Num && e->a[1]->is_int) {

@@ -20,6 +20,9 @@
 namespace ps {
 
 static const char* kIdx[3] = {"__ix", "__iy", "__iz"};
+// padding=true: the staggered index roles (ixd) stay symbolic until the statement's written array
+// is known (src/ParallelKernel/parallel.jl:508-549)
+static const char* kIdxD[3] = {"__ixd", "__iyd", "__izd"};
 
 // ---------------------------------------------------------------------------
 // Finite-difference macro tables. Each entry restates the index algebra of
@@ -310,7 +313,7 @@ static void read_options(Options& o, const JVal& j) {
       if (o.prefetch < 1 || o.prefetch > 4) throw PsError(PS_EKEYWORD, "prefetch must be 1, 2, 3 or 4");
     } else if (k == "lds") o.lds = v.b;
     else if (k == "xcd_remap") o.xcd_remap = v.b;
-    else if (k == "nt_store") o.nt_store = v.b;
+    else if (k == "nt_store") o.nt_store = v.b ? 1 : 0;
     else if (k == "tuple_params") {
       o.tuple_params.clear();
       for (auto& e : v.obj) {
@@ -382,7 +385,7 @@ static std::map<std::string, NodeP>& template_cache() {
   return c;
 }
 
-static NodeP expand_macro(int nd, const std::string& name, const std::string& arr) {
+static NodeP expand_macro(int nd, const std::string& name, const std::string& arr, bool padding = false) {
   const MacroDef* m = find_macro(nd, name);
   if (!m) {
     throw PsError(PS_EPLUGIN, "macro @" + name + " is not defined for " + std::to_string(nd) +
@@ -396,25 +399,30 @@ static NodeP expand_macro(int nd, const std::string& name, const std::string& ar
   for (int d = 0; d < 3; ++d) {
     NodeP v = mk(NK::Ident, kIdx[d]);
     sub[std::string("i") + dn[d] + "a"] = v;
-    sub[std::string("i") + dn[d] + "d"] = v;  // padding=false: ixd -> ix
-    sub[std::string("i") + dn[d] + "i"] = mk(NK::Binary, "+", {mk(NK::Ident, kIdx[d]), mknum_int(1)});  // ixi -> ix+1
+    if (padding) {  // substitute_indices_inn / _dir with padding: ixi -> ix, ixd resolved per statement
+      sub[std::string("i") + dn[d] + "d"] = mk(NK::Ident, kIdxD[d]);
+      sub[std::string("i") + dn[d] + "i"] = mk(NK::Ident, kIdx[d]);
+    } else {
+      sub[std::string("i") + dn[d] + "d"] = v;  // padding=false: ixd -> ix
+      sub[std::string("i") + dn[d] + "i"] = mk(NK::Binary, "+", {mk(NK::Ident, kIdx[d]), mknum_int(1)});  // ixi -> ix+1
+    }
   }
   sub["A"] = mk(NK::Ident, arr);
   return subst_idents(cache[key], sub);
 }
 
-static NodeP expand_all_macros(int nd, const NodeP& n) {
+static NodeP expand_all_macros(int nd, const NodeP& n, bool padding = false) {
   return rewrite(n, [&](const NodeP& x) -> NodeP {
     if (x->k == NK::Macro) {
       if (x->a.size() != 1 || x->a[0]->k != NK::Ident)
         throw PsError(PS_EARGUMENT, "finite-difference macro @" + x->s + " expects a single array argument");
-      return expand_macro(nd, x->s, x->a[0]->s);
+      return expand_macro(nd, x->s, x->a[0]->s, padding);
     }
     return x;
   });
 }
 
-static NodeP within_mask(int nd, const std::string& mname, const std::string& arr) {
+static NodeP within_mask(int nd, const std::string& mname, const std::string& arr, bool padding = false) {
   const WithinDef* tab = nd == 1 ? kWithin1 : nd == 2 ? kWithin2 : kWithin3;
   size_t n = nd == 1 ? 2 : nd == 2 ? 4 : 8;
   const WithinDef* w = nullptr;
@@ -428,7 +436,17 @@ static NodeP within_mask(int nd, const std::string& mname, const std::string& ar
     NodeP sz = mk(NK::Call, "size", {mk(NK::Ident, arr), mknum_int(d + 1)});
     NodeP v = mk(NK::Ident, kIdx[d]);
     NodeP lo, hi;
-    if (w->roles[d] == 'a') {  // 1 <= ix <= size(A,d)
+    if (padding) {
+      // firstindex(A,d) = A.indices[d][1], lastindex(A,d) = A.indices[d][end]
+      // (src/ParallelKernel/kernel_language.jl:605-617): the view's box in its parent
+      NodeP f = mk(NK::Call, "__first", {mk(NK::Ident, arr), mknum_int(d + 1)});
+      NodeP l = mk(NK::Call, "__last", {mk(NK::Ident, arr), mknum_int(d + 1)});
+      const char* op = w->roles[d] == 'a' ? "<=" : "<";
+      lo = mk(NK::Cmp, "", {f, clone(v)});
+      lo->ops = {op};
+      hi = mk(NK::Cmp, "", {clone(v), l});
+      hi->ops = {op};
+    } else if (w->roles[d] == 'a') {  // 1 <= ix <= size(A,d)
       lo = mk(NK::Cmp, "", {mknum_int(1), clone(v)});
       lo->ops = {"<="};
       hi = mk(NK::Cmp, "", {clone(v), sz});
@@ -463,6 +481,140 @@ static NodeP insert_onthefly(const NodeP& n, const std::map<std::string, NodeP>&
     }
     return x;
   });
+}
+
+// padding=true: the local index roles of an on-the-fly array read at index expression e in
+// dimension d (determine_local_index_dir, src/parallel.jl:829-835): its ix is e, and its ixd is e
+// with every __dirf(d, n) raised to __dirf(d, n+1), every ixd turned into __dirf(d, 2) and every
+// ix into ixd. __dirf(d, n) stands for ix - n÷2 until the statement's written array is known.
+static NodeP local_index_dir(const NodeP& e, int d) {
+  NodeP r = rewrite(clone(e), [&](const NodeP& x) -> NodeP {
+    if (x->k == NK::Call && x->s == "__dirf" && x->a.size() == 2 && x->a[0]->ival == d)
+      return mk(NK::Call, "__dirf", {mknum_int(d), mknum_int(x->a[1]->ival + 1)});
+    return x;
+  });
+  r = rewrite(r, [&](const NodeP& x) -> NodeP {
+    if (x->k == NK::Ident && x->s == kIdxD[d]) return mk(NK::Call, "__dirf", {mknum_int(d), mknum_int(2)});
+    return x;
+  });
+  return rewrite(r, [&](const NodeP& x) -> NodeP {
+    if (x->k == NK::Ident && x->s == kIdx[d]) return mk(NK::Ident, kIdxD[d]);
+    return x;
+  });
+}
+
+static NodeP insert_onthefly_padded(const NodeP& n, const std::map<std::string, NodeP>& otf, int nd, int depth = 0) {
+  if (depth > 32) throw PsError(PS_EARGUMENT, "on-the-fly arrays are nested too deeply (cyclic definition?)");
+  return rewrite(n, [&](const NodeP& x) -> NodeP {
+    if (x->k == NK::Index && x->a[0]->k == NK::Ident) {
+      auto it = otf.find(x->a[0]->s);
+      if (it == otf.end()) return x;
+      if ((int)x->a.size() - 1 != nd)
+        throw PsError(PS_EARGUMENT, "unsupported kernel statements in @parallel kernel definition: wrong number of indices in " +
+                                        x->a[0]->s + " (expected " + std::to_string(nd) + " indices).");
+      std::map<std::string, NodeP> sub;
+      for (int d = 0; d < nd; ++d) {
+        NodeP e = insert_onthefly_padded(x->a[1 + d], otf, nd, depth + 1);
+        sub[kIdxD[d]] = local_index_dir(e, d);
+        sub[kIdx[d]] = e;
+      }
+      NodeP e = subst_idents(it->second, sub);
+      return insert_onthefly_padded(e, otf, nd, depth + 1);
+    }
+    return x;
+  });
+}
+
+// padding=true: resolves the staggered index roles of one statement whose written array is B
+// (substitute_indices_dir, src/ParallelKernel/parallel.jl:514-548): in a read of array A, ixd is
+// ix and __dirf(d, n) is ix - n÷2, each minus (size(B.parent,d) > size(A.parent,d)) when A is not
+// B. The comparison stays a __pgt(B, A, d) call, recorded in K.pgt_terms, until a launch knows the
+// parents' sizes.
+static NodeP resolve_dir_indices(Kernel& K, const NodeP& rhs, const std::string& B) {
+  auto pgt = [&](const std::string& A, int d) -> NodeP {
+    std::array<int, 3> t{K.param_index.at(B), K.param_index.at(A), d + 1};
+    if (std::find(K.pgt_terms.begin(), K.pgt_terms.end(), t) == K.pgt_terms.end()) K.pgt_terms.push_back(t);
+    return mk(NK::Call, "__pgt", {mk(NK::Ident, B), mk(NK::Ident, A), mknum_int(d + 1)});
+  };
+  NodeP r = rewrite(rhs, [&](const NodeP& x) -> NodeP {
+    if (x->k != NK::Index || x->a[0]->k != NK::Ident || !K.param_index.count(x->a[0]->s)) return x;
+    const std::string A = x->a[0]->s;
+    for (size_t i = 1; i < x->a.size(); ++i) {
+      x->a[i] = rewrite(x->a[i], [&](const NodeP& e) -> NodeP {
+        for (int d = 0; d < 3; ++d) {
+          NodeP base;
+          if (e->k == NK::Ident && e->s == kIdxD[d]) base = mk(NK::Ident, kIdx[d]);
+          else if (e->k == NK::Call && e->s == "__dirf" && e->a[0]->ival == d)
+            base = mk(NK::Binary, "-", {mk(NK::Ident, kIdx[d]), mknum_int(e->a[1]->ival / 2)});
+          if (!base) continue;
+          return A == B ? base : mk(NK::Binary, "-", {base, pgt(A, d)});
+        }
+        return e;
+      });
+    }
+    return x;
+  });
+  walk(r, [&](const NodeP& x) {
+    if ((x->k == NK::Ident && (x->s == kIdxD[0] || x->s == kIdxD[1] || x->s == kIdxD[2])) || (x->k == NK::Call && x->s == "__dirf"))
+      throw PsError(PS_EINTERNAL, "padding: a staggered index remains outside an array access");
+  });
+  return r;
+}
+
+// padding=true in @parallel_indices kernels (substitute_view_accesses,
+// src/ParallelKernel/parallel.jl:572-581, is_access in src/ParallelKernel/shared.jl:328-330): a read
+// or write of an array whose every index involves a parallel index addresses the array's parent
+// (`end` there is the parent's extent); any other access addresses the view itself, which here
+// means its indices are shifted by the view's first parent index.
+static NodeP padded_view_accesses(const Kernel& K, const NodeP& body) {
+  auto involves_index = [&](const NodeP& e) {
+    bool hit = false;
+    walk(e, [&](const NodeP& y) {
+      for (int d = 0; d < K.nidx; ++d)
+        if (y->k == NK::Ident && y->s == kIdx[d]) hit = true;
+    });
+    return hit;
+  };
+  return rewrite(body, [&](const NodeP& x) -> NodeP {
+    if (x->k != NK::Index || x->a[0]->k != NK::Ident || !K.param_index.count(x->a[0]->s)) return x;
+    const std::string A = x->a[0]->s;
+    const size_t ni = x->a.size() - 1;
+    bool promoted = (int)ni == K.nidx;
+    for (size_t i = 1; i < x->a.size() && promoted; ++i) promoted = involves_index(x->a[i]);
+    for (size_t i = 1; i < x->a.size(); ++i) {
+      const int d = (int)i;
+      NodeP e = padded_view_accesses(K, x->a[i]);
+      e = rewrite(e, [&](const NodeP& y) -> NodeP {
+        if (y->k == NK::EndKw) return mk(NK::Call, promoted ? "__psize" : "size", {mk(NK::Ident, A), mknum_int(d)});
+        return y;
+      });
+      if (!promoted && ni > 1)
+        e = mk(NK::Binary, "-", {mk(NK::Binary, "+", {e, mk(NK::Call, "__first", {mk(NK::Ident, A), mknum_int(d)})}), mknum_int(1)});
+      x->a[i] = e;
+    }
+    return x;
+  });
+}
+
+static void memopt_analysis(Kernel& K);
+
+Kernel specialize_padding(const Kernel& K0, const std::string& bits) {
+  Kernel K = K0;
+  if (bits.size() != K.pgt_terms.size()) throw PsError(PS_EINTERNAL, "specialize_padding: wrong number of bits");
+  K.body = rewrite(clone(K0.body), [&](const NodeP& x) -> NodeP {
+    if (x->k != NK::Call || x->s != "__pgt") return x;
+    std::array<int, 3> t{K.param_index.at(x->a[0]->s), K.param_index.at(x->a[1]->s), (int)x->a[2]->ival};
+    size_t i = std::find(K.pgt_terms.begin(), K.pgt_terms.end(), t) - K.pgt_terms.begin();
+    if (i >= bits.size()) throw PsError(PS_EINTERNAL, "specialize_padding: unknown parent-size comparison");
+    return mknum_int(bits[i] == '1' ? 1 : 0);
+  });
+  K.body = canon_indices(K.body);
+  K.pgt_terms.clear();
+  if (K.opt.memopt) {
+    K.mm = MemoptMeta();
+    memopt_analysis(K);
+  }
+  return K;
 }
 
 // ---------------------------------------------------------------------------
@@ -1022,11 +1174,17 @@ Kernel define_kernel(const std::string& text, const JVal& opts) {
       if (s->a[0]->a.size() != 1 || s->a[0]->a[0]->k != NK::Ident)
         throw PsError(PS_EARGUMENT, "unsupported kernel statements in @parallel kernel definition: the left-hand side must be @macro(A) with A an array.");
     }
-    // interpolated indices ($ix, $iy, $iz) name the parallel indices
+    // interpolated indices: $ix, $iy, $iz name the parallel indices (ParallelStencil.INDICES) and
+    // $ixi, $iyi, $izi the inner ones (INDICES_INN: ix+1 without padding, ix with padding,
+    // src/ParallelKernel/parallel.jl:498-504)
     body = rewrite(body, [&](const NodeP& x) -> NodeP {
       if (x->k == NK::Interp) {
-        for (int d = 0; d < 3; ++d)
-          if (x->s == std::string(1, "xyz"[d]).insert(0, "i")) return mk(NK::Ident, kIdx[d]);
+        for (int d = 0; d < 3; ++d) {
+          const std::string nm = std::string("i") + "xyz"[d];
+          if (x->s == nm) return mk(NK::Ident, kIdx[d]);
+          if (x->s == nm + "i")
+            return K.opt.padding ? mk(NK::Ident, kIdx[d]) : mk(NK::Binary, "+", {mk(NK::Ident, kIdx[d]), mknum_int(1)});
+        }
       }
       return x;
     });
@@ -1053,7 +1211,7 @@ Kernel define_kernel(const std::string& text, const JVal& opts) {
         if (refers_to(s->a[1], w))
           throw PsError(PS_EARGUMENT, "unsupported kernel statements in @parallel kernel definition: the assignment of " + A +
                                           " should be done on the fly as it is not among the arguments of the kernel; however, this is not possible because it depends on at least one variable that is not read-only within the scope of the kernel (any of: " + w + ").");
-      otf[A] = expand_all_macros(nd, s->a[1]);
+      otf[A] = expand_all_macros(nd, s->a[1], K.opt.padding);
       K.otf_vars.push_back(A);
     }
     // masks, macro expansion, on-the-fly insertion
@@ -1061,11 +1219,15 @@ Kernel define_kernel(const std::string& text, const JVal& opts) {
     for (auto& s : remaining->a) {
       const std::string& m = s->a[0]->s;
       const std::string& A = s->a[0]->a[0]->s;
-      NodeP mask = within_mask(nd, m, A);
-      NodeP lhs = expand_macro(nd, m, A);
+      NodeP mask = within_mask(nd, m, A, K.opt.padding);
+      NodeP lhs = expand_macro(nd, m, A, K.opt.padding);
       if (lhs->k != NK::Index) throw PsError(PS_EARGUMENT, "@" + m + " cannot be assigned to");
-      NodeP rhs = expand_all_macros(nd, s->a[1]);
-      rhs = insert_onthefly(rhs, otf, nd);
+      NodeP rhs = expand_all_macros(nd, s->a[1], K.opt.padding);
+      if (K.opt.padding) {
+        rhs = resolve_dir_indices(K, insert_onthefly_padded(rhs, otf, nd), A);
+      } else {
+        rhs = insert_onthefly(rhs, otf, nd);
+      }
       auto as = mk(NK::Assign, "", {canon_indices(lhs), canon_indices(rhs)});
       auto blk = mk(NK::Block, "", {as});
       out->a.push_back(mk(NK::If, "", {mask, blk, nullptr}));
@@ -1128,12 +1290,21 @@ Kernel define_kernel(const std::string& text, const JVal& opts) {
       });
       if (!used) throw PsError(PS_EARGUMENT, "@parallel_indices: all parallel indices must be used in the kernel body (unused indices: " + K.opt.indices[d] + ").");
     }
+    if (K.opt.padding) body = padded_view_accesses(K, body);
     K.body = canon_indices(body);
   } else {
     throw PsError(PS_EARGUMENT, "unknown kernel kind '" + K.opt.kind + "'");
   }
   collect_access(K.body, K.accessed, K.written, K.param_index);
-  if (K.opt.memopt) memopt_analysis(K);
+  // a kernel whose indices still compare parent sizes is analysed for memopt per launch, once the
+  // comparisons have values (specialize_padding); its definition-time metadata take them as 0
+  if (K.opt.memopt) {
+    if (K.pgt_terms.empty()) memopt_analysis(K);
+    else {
+      Kernel Z = specialize_padding(K, std::string(K.pgt_terms.size(), '0'));
+      K.mm = Z.mm;
+    }
+  }
   return K;
 }
 
@@ -1223,8 +1394,8 @@ std::string kernel_info_json(const Kernel& K) {
 
 std::string Variant::key() const {
   char b[96];
-  std::snprintf(b, sizeof b, "m%dc%di%da%dt%dl%d_%dx%d", memopt, checked, idx64, noalias, maxthreads, lds, bx, by);
-  return std::string(b) + "d" + divmode;
+  std::snprintf(b, sizeof b, "m%dc%di%da%dt%dl%d_%dx%dn%d", memopt, checked, idx64, noalias, maxthreads, lds, bx, by, nt);
+  return std::string(b) + "d" + divmode + (padbits.empty() ? "" : "p" + padbits);
 }
 
 }  // namespace ps

@@ -56,8 +56,10 @@ struct Options {
   std::map<std::string, std::vector<std::pair<int, int>>> optranges;
   // memopt block shape limits: 128 x 4 threads streaming along z measured fastest on gfx950
   // (wider rows of HBM per wave, smaller LDS halo share than 64 x 4)
-  int nthreads_max_memopt = 512;
-  int nthreads_x_max_memopt = 128;
+  // 0: automatic (128 x 4, or 256 x 4 for 3-D streams whose x range is a multiple of 256 of at
+  // least 512 cells: wider rows of HBM per wave, measured fastest at 512^3)
+  int nthreads_max_memopt = 0;
+  int nthreads_x_max_memopt = 0;
   int nthreads_x_max = 64;
   int nthreads_max = 256;
   bool fp_contract = false;               // Julia semantics: no FMA contraction
@@ -66,8 +68,10 @@ struct Options {
   bool fastmath = false;
   int prefetch = 1;                       // memopt: loads are issued this many steps ahead
   bool lds = true;                        // memopt: stage x-y neighbours of optvars through LDS
-  bool xcd_remap = true;                  // XCD-aware block ordering for memopt kernels
-  bool nt_store = false;                  // nontemporal stores for array writes
+  bool xcd_remap = false;                 // XCD-aware block ordering for memopt kernels
+  // nontemporal stores for array writes: 1 on, 0 off, -1 automatic (on for the 256-wide
+  // LDS-tiled memopt blocks, where they measured faster; off elsewhere)
+  int nt_store = -1;
   // arguments that are tuples / NamedTuples of arrays (Data.ArrayTuple, Data.NamedArrayTuple):
   // name -> component names ("1", "2", ... for plain tuples); each component becomes a
   // parameter of its own, named name.component
@@ -90,7 +94,16 @@ struct Kernel {
   bool kernel_language = false;
   bool needs_idx64 = false;
   std::vector<std::string> def_params;  // parameters as written (tuple arguments not yet split)
+  // padding=true: the (written array B, read array A, dimension 1..3) triples of the index shifts
+  // `size(B.parent, d) > size(A.parent, d)` left in the body as __pgt(B, A, d) calls; a launch
+  // replaces them by their values for its arguments (specialize_padding)
+  std::vector<std::array<int, 3>> pgt_terms;
 };
+
+// padding=true kernels whose body compares parent sizes (pgt_terms): the kernel with every
+// __pgt(B, A, d) replaced by bits[i] of its term i, indices canonicalised and the memopt analysis
+// redone on the result.
+Kernel specialize_padding(const Kernel& K, const std::string& bits);
 
 struct Variant {
   bool memopt = false;
@@ -99,10 +112,13 @@ struct Variant {
   bool noalias = true;    // __restrict__ on array pointers
   int maxthreads = 256;   // __launch_bounds__
   bool lds = false;       // memopt: stage x-y neighbours of optvars through LDS tiles
+  bool nt = false;        // nontemporal stores for array writes
   int bx = 0, by = 0;     // block shape baked into LDS variants
   // per argument: 'p' / 'n' a positive / negative float scalar accepted as reciprocal divisor,
   // 'x' a float scalar that must be divided by exactly, '-' anything else
   std::string divmode;
+  // padding=true: results of the kernel's parent-size comparisons ('0'/'1' per term)
+  std::string padbits;
   std::string key() const;
 };
 
@@ -113,7 +129,8 @@ struct Emitted {
   struct P {
     int kind;      // 0 field ptr, 1 int size, 2 scalar, 3 range start, 4 range len, 5 err ptr,
                    // 6 RN(1/|b|) of a uniform divisor or 0 (exact division), 7 RN(1/b) (fastmath),
-                   // 8 bytes of dynamic LDS of the launch (@sharedMem), 9 blocklen of a CellArray
+                   // 8 bytes of dynamic LDS of the launch (@sharedMem), 9 blocklen of a CellArray,
+                   // 10 view box of a field in its parent (padding=true): which 0..2 first, 3..5 last index
     int arg;       // arg index (fields/scalars)
     int which;     // for sizes: 0 nx 1 ny 2 nz 3 sx 4 sxy; for ranges: dim
     int bytes;     // 4 or 8
@@ -149,7 +166,8 @@ std::string kernel_info_json(const Kernel& k);
 // array for the given launch box and argument shapes.
 struct LaunchShape {
   std::vector<ArgKind> kinds;
-  std::vector<std::array<long long, 3>> dims;  // per arg (fields)
+  std::vector<std::array<long long, 3>> dims;  // per arg (fields); the parents' for padding=true
+  std::vector<std::array<long long, 3>> vlo, vhi;  // padding=true: each field's box in its parent
   std::vector<long long> ivals;                 // integer scalar values
   long long lo[3], hi[3];                       // index box (1-based inclusive)
 };

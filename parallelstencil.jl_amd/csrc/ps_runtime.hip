@@ -241,6 +241,9 @@ struct KernelObj {
   // same shapes skip the planning, the bounds proof and the variant lookup
   std::mutex lmu;
   std::unordered_map<std::string, std::shared_ptr<LaunchEntry>> launches;
+  // padding=true kernels comparing parent sizes: the kernel per pattern of comparison results
+  std::mutex pmu;
+  std::map<std::string, std::shared_ptr<Kernel>> padspec;
 };
 
 struct State {
@@ -387,6 +390,25 @@ ps_field data_of(const ps_field& f) {
   return d;
 }
 
+// The parent of a view (ps_field_view) and the field itself otherwise: extent per dimension,
+// offset of the field's first element in it, and the address of the parent's first element.
+int64_t pdim(const ps_field& f, int d) { return f.pdims[0] ? f.pdims[d] : f.dims[d]; }
+int64_t voffset(const ps_field& f, int d) { return f.pdims[0] ? f.voff[d] : 0; }
+void* parent_ptr(const ps_field& f) {
+  if (!f.pdims[0] || !f.ptr) return f.ptr;
+  return (char*)f.ptr - (f.voff[0] + f.voff[1] * f.pitch + f.voff[2] * f.plane) * (int64_t)elsize(f.dtype);
+}
+ps_field parent_of(const ps_field& f) {
+  ps_field p = f;
+  p.ptr = parent_ptr(f);
+  for (int d = 0; d < 3; ++d) {
+    p.dims[d] = pdim(f, d);
+    p.voff[d] = 0;
+    p.pdims[d] = 0;
+  }
+  return p;
+}
+
 // Bytes a field argument spans from its first element (aliasing analysis).
 int64_t extent_bytes(const ps_field& f) {
   const ps_field d = data_of(f);
@@ -441,14 +463,32 @@ std::string kinds_key(const std::vector<ArgKind>& k) {
   return s;
 }
 
-std::shared_ptr<Module> get_module(KernelObj* ko, const std::vector<ArgKind>& kinds, const Variant& v, bool load) {
+// The kernel a launch with these arguments runs: the definition itself, or for padding=true
+// kernels whose indices compare parent sizes, its specialisation to the arguments' parents
+// (*bits receives the comparison results, "" when there are none).
+const Kernel& kernel_for(KernelObj* ko, const ps_arg* args, int nargs, std::string* bits) {
+  const Kernel& K = ko->K;
+  bits->clear();
+  if (K.pgt_terms.empty()) return K;
+  for (auto& t : K.pgt_terms) {
+    if (t[0] >= nargs || t[1] >= nargs || args[t[0]].kind != PS_ARG_FIELD || args[t[1]].kind != PS_ARG_FIELD)
+      throw PsError(PS_EARGUMENT, "kernel " + K.def.name + ": padding=true needs array arguments where the kernel reads staggered arrays");
+    bits->push_back(pdim(args[t[0]].field, t[2] - 1) > pdim(args[t[1]].field, t[2] - 1) ? '1' : '0');
+  }
+  std::lock_guard<std::mutex> lk(ko->pmu);
+  auto& slot = ko->padspec[*bits];
+  if (!slot) slot = std::make_shared<Kernel>(specialize_padding(K, *bits));
+  return *slot;
+}
+
+std::shared_ptr<Module> get_module(KernelObj* ko, const Kernel& K, const std::vector<ArgKind>& kinds, const Variant& v, bool load) {
   std::string key = kinds_key(kinds) + v.key();
   std::lock_guard<std::mutex> lk(ko->mu);
   auto it = ko->variants.find(key);
   if (it != ko->variants.end() && (!load || it->second->fn)) return it->second;
   auto m = std::make_shared<Module>();
-  m->em = lower_kernel(ko->K, kinds, v);
-  std::vector<char> code = code_object(ko->K, m->em);
+  m->em = lower_kernel(K, kinds, v);
+  std::vector<char> code = code_object(K, m->em);
   if (load) {
     hipError_t e = hipModuleLoadData(&m->mod, code.data());
     if (e != hipSuccess) throw PsError(PS_EHIP, std::string("hipModuleLoadData: ") + hipGetErrorString(e));
@@ -667,7 +707,11 @@ extern "C" int ps_field_view(const ps_field* parent, const int64_t lo[3], const 
   *out = *parent;
   size_t es = elsize(parent->dtype);
   out->ptr = (char*)parent->ptr + ((lo[0] - 1) + (lo[1] - 1) * parent->pitch + (lo[2] - 1) * parent->plane) * (int64_t)es;
-  for (int i = 0; i < 3; ++i) out->dims[i] = hi[i] - lo[i] + 1;
+  for (int i = 0; i < 3; ++i) {
+    out->dims[i] = hi[i] - lo[i] + 1;
+    out->voff[i] = voffset(*parent, i) + lo[i] - 1;  // a view of a view lies in the same parent
+  }
+  for (int i = 0; i < 3; ++i) out->pdims[i] = pdim(*parent, i);
   out->base = nullptr;
   out->alloc_bytes = 0;
   return 0;
@@ -1097,6 +1141,7 @@ static Variant variant_from_flags(const Kernel& K, int flags) {
     v.by = (flags >> 20) & 0xfff;
     if (!v.memopt || v.bx < 1 || v.by < 1) throw PsError(PS_EARGUMENT, "LDS variants need memopt and a block shape");
   }
+  v.nt = K.opt.nt_store == 1 || (K.opt.nt_store == -1 && v.lds && v.bx >= 256);
   return v;
 }
 
@@ -1105,7 +1150,9 @@ extern "C" int ps_kernel_source(ps_kernel k, const ps_arg* args, int nargs, int 
   KernelObj* ko = (KernelObj*)k;
   Variant v = variant_from_flags(ko->K, flags);
   v.divmode = divmode_of(args, nargs);
-  Emitted em = lower_kernel(ko->K, arg_kinds(args, nargs), v);
+  const Kernel K0 = ko->K.pgt_terms.empty() ? ko->K : specialize_padding(ko->K, std::string(ko->K.pgt_terms.size(), '0'));
+  if (!ko->K.pgt_terms.empty()) v.padbits = std::string(ko->K.pgt_terms.size(), '0');
+  Emitted em = lower_kernel(K0, arg_kinds(args, nargs), v);
   if (em.source.size() + 1 > cap) return fail(PS_EARGUMENT, "ps_kernel_source: buffer too small (need " + std::to_string(em.source.size() + 1) + ")");
   std::memcpy(out, em.source.c_str(), em.source.size() + 1);
   return 0;
@@ -1117,7 +1164,12 @@ extern "C" int ps_kernel_precompile(ps_kernel k, const ps_arg* args, int nargs, 
   KernelObj* ko = (KernelObj*)k;
   Variant v = variant_from_flags(ko->K, flags);
   v.divmode = divmode_of(args, nargs);
-  get_module(ko, arg_kinds(args, nargs), v, false);
+  if (!ko->K.pgt_terms.empty()) {
+    v.padbits = std::string(ko->K.pgt_terms.size(), '0');
+    get_module(ko, specialize_padding(ko->K, v.padbits), arg_kinds(args, nargs), v, false);
+  } else {
+    get_module(ko, ko->K, arg_kinds(args, nargs), v, false);
+  }
   return 0;
   PS_GUARD_END
 }
@@ -1137,10 +1189,16 @@ static const char* kErrAutoRanges =
 static const char* kErrNbIdx =
     "the number of parallel indices of the kernel does not match the number of dimensions of the kernel arguments (maximal number of dimensions of array arguments); specify the ranges explicitly or use @parallel_indices with the matching number of indices.";
 
-Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch* L, std::vector<ArgKind>& kinds) {
+Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch* L, std::vector<ArgKind>& kinds,
+                 const std::string& padbits = "") {
   (void)kErrAutoRanges;
   Plan P;
   kinds = arg_kinds(args, nargs);
+  const bool pad = K.opt.padding;
+  if (pad)
+    for (int i = 0; i < nargs; ++i)
+      if (args[i].kind == PS_ARG_FIELD && args[i].field.ncomp > 0)
+        throw PsError(PS_EARGUMENT, "padding=true is not supported with CellArray arguments (argument " + std::to_string(i + 1) + ")");
   if (nargs != (int)K.def.params.size())
     throw PsError(PS_EARGUMENT, "MethodError: kernel " + K.def.name + " takes " + std::to_string(K.def.params.size()) + " arguments (" +
                                     std::to_string(nargs) + " given)");
@@ -1162,9 +1220,9 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   P.memopt = memopt;
   int64_t maxsize[3] = {1, 1, 1};
   int maxnd = 1;
-  for (int i = 0; i < nargs; ++i) {
+  for (int i = 0; i < nargs; ++i) {  // maxsize: of the parents for padding=true (parallel.jl:786)
     if (args[i].kind != PS_ARG_FIELD) continue;
-    for (int d = 0; d < 3; ++d) maxsize[d] = std::max<int64_t>(maxsize[d], args[i].field.dims[d]);
+    for (int d = 0; d < 3; ++d) maxsize[d] = std::max<int64_t>(maxsize[d], pad ? pdim(args[i].field, d) : args[i].field.dims[d]);
     maxnd = std::max(maxnd, (int)args[i].field.ndims);
   }
   bool user_ranges = L && L->has_ranges;
@@ -1191,13 +1249,23 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
     int Ld = mm.loopdim - 1;
     int64_t loopsizes[3] = {1, 1, 1};
     loopsizes[Ld] = mm.loopsize;
-    // x block width: the tuned width unless half of it pads the x range to fewer threads
-    int xmax = K.opt.nthreads_x_max_memopt;
+    // block shape: 128 x 4 threads, or 256 x 4 for 3-D streams whose x range is a multiple of
+    // 256 of at least 512 cells (wider HBM rows per wave; measured fastest at 512^3), unless the
+    // kernel sets nthreads_x_max_memopt / nthreads_max_memopt; then the x width is halved where
+    // that pads the x range to fewer threads
+    int xmax = K.opt.nthreads_x_max_memopt, tmax = K.opt.nthreads_max_memopt;
+    const bool autoshape = xmax == 0 && tmax == 0;
+    if (xmax == 0) xmax = 128;
+    if (tmax == 0) tmax = 512;
+    if (autoshape && mm.loopdim == 3 && len[0] >= 512 && len[0] % 256 == 0) {
+      xmax = 256;
+      tmax = 1024;
+    }
     if (xmax >= 128 && (len[0] + 63) / 64 * 64 < (len[0] + xmax - 1) / xmax * xmax) xmax = 64;
     if (K.opt.kind == "parallel" && !user_ranges) {
       // get_ranges_memopt: pad to multiples of nthreads (src/parallel.jl:560-571)
       int32_t nt[3];
-      ps_compute_nthreads(len, xmax, K.opt.nthreads_max_memopt, mm.loopdim, nt);
+      ps_compute_nthreads(len, xmax, tmax, mm.loopdim, nt);
       for (int d = 0; d < 3; ++d) {
         int64_t rest = len[d] % nt[d];
         if (rest) len[d] += nt[d] - rest;
@@ -1212,7 +1280,7 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
         P.nblocks[d] = L->nblocks[d] ? L->nblocks[d] : (int32_t)((ms[d] + P.nthreads[d] - 1) / P.nthreads[d]);
       }
     } else {
-      ps_compute_nthreads(ms, xmax, K.opt.nthreads_max_memopt, mm.loopdim, P.nthreads);
+      ps_compute_nthreads(ms, xmax, tmax, mm.loopdim, P.nthreads);
       // sizes that the tuned block does not divide get the reference's AMDGPU block limits
       // (64 threads along x, 256 per block: src/shared.jl:29, src/ParallelKernel/shared.jl:33)
       if (ms[0] % P.nthreads[0] || ms[1] % P.nthreads[1] || ms[2] % P.nthreads[2])
@@ -1256,6 +1324,7 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
     v.by = P.nthreads[1];
   }
   v.maxthreads = ((int64_t)P.nthreads[0] * P.nthreads[1] * P.nthreads[2] > 256) ? 1024 : 256;
+  v.nt = K.opt.nt_store == 1 || (K.opt.nt_store == -1 && v.lds && v.bx >= 256);
   bool big = false;
   for (int i = 0; i < nargs; ++i) {
     if (args[i].kind == PS_ARG_FIELD) {
@@ -1269,12 +1338,14 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
     if (P.ranges[d][1] > INT32_MAX / 2 || P.ranges[d][0] < INT32_MIN / 2) big = true;
   v.idx64 = big || K.needs_idx64;
   v.divmode = divmode_of(args, nargs);
-  // aliasing: a written field that overlaps another field argument
+  v.padbits = padbits;
+  // aliasing: a written field that overlaps another field argument (padding=true kernels reach the
+  // whole parent of every argument)
   for (int i = 0; i < nargs && v.noalias; ++i) {
     if (args[i].kind != PS_ARG_FIELD) continue;
     for (int j = i + 1; j < nargs; ++j) {
       if (args[j].kind != PS_ARG_FIELD) continue;
-      const ps_field &a = args[i].field, &b = args[j].field;
+      const ps_field a = pad ? parent_of(args[i].field) : args[i].field, b = pad ? parent_of(args[j].field) : args[j].field;
       const char* a0 = (const char*)a.ptr;
       const char* a1 = a0 + extent_bytes(a);
       const char* b0 = (const char*)b.ptr;
@@ -1287,11 +1358,22 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   sh.kinds = kinds;
   sh.dims.resize(nargs);
   sh.ivals.resize(nargs);
+  if (pad) {
+    sh.vlo.resize(nargs);
+    sh.vhi.resize(nargs);
+  }
   for (int i = 0; i < nargs; ++i) {
-    if (args[i].kind == PS_ARG_FIELD)
-      for (int d = 0; d < 3; ++d) sh.dims[i][d] = args[i].field.dims[d];
-    else
+    if (args[i].kind == PS_ARG_FIELD) {
+      for (int d = 0; d < 3; ++d) {
+        sh.dims[i][d] = pad ? pdim(args[i].field, d) : args[i].field.dims[d];
+        if (pad) {
+          sh.vlo[i][d] = voffset(args[i].field, d) + 1;
+          sh.vhi[i][d] = voffset(args[i].field, d) + args[i].field.dims[d];
+        }
+      }
+    } else {
       sh.ivals[i] = args[i].i;
+    }
   }
   for (int d = 0; d < 3; ++d) {
     sh.lo[d] = P.ranges[d][0];
@@ -1310,6 +1392,7 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
 struct LaunchEntry {
   Plan P;
   std::shared_ptr<Module> m;  // null for an empty launch box
+  bool pad = false;           // padding=true: fields are passed as their parents and view boxes
 };
 
 // Everything plan_launch and the variant depend on: argument kinds and types, field shapes and
@@ -1328,6 +1411,10 @@ std::string launch_key(const ps_arg* args, int nargs, const ps_launch* L) {
       put(&a.field.pitch, 8);
       put(&a.field.plane, 8);
       put(&a.field.ndims, 4);
+      if (a.field.pdims[0]) {  // a view: its place in the parent (padding=true kernels address the parent)
+        put(a.field.voff, 24);
+        put(a.field.pdims, 24);
+      }
       if (a.field.ncomp > 0) {
         put(&a.field.ncomp, 4);
         put(a.field.celldims, 16);
@@ -1379,7 +1466,9 @@ extern "C" int ps_kernel_geometry(ps_kernel k, const ps_arg* args, int nargs, co
   PS_GUARD_BEGIN
   KernelObj* ko = (KernelObj*)k;
   std::vector<ArgKind> kinds;
-  Plan P = plan_launch(ko->K, args, nargs, launch, kinds);
+  std::string bits;
+  const Kernel& K = kernel_for(ko, args, nargs, &bits);
+  Plan P = plan_launch(K, args, nargs, launch, kinds, bits);
   for (int d = 0; d < 3; ++d) {
     nthreads[d] = P.nthreads[d];
     nblocks[d] = P.nblocks[d];
@@ -1395,8 +1484,10 @@ extern "C" int ps_kernel_precompile_launch(ps_kernel k, const ps_arg* args, int 
   PS_GUARD_BEGIN
   KernelObj* ko = (KernelObj*)k;
   std::vector<ArgKind> kinds;
-  Plan P = plan_launch(ko->K, args, nargs, launch, kinds);
-  get_module(ko, kinds, P.v, false);
+  std::string bits;
+  const Kernel& K = kernel_for(ko, args, nargs, &bits);
+  Plan P = plan_launch(K, args, nargs, launch, kinds, bits);
+  get_module(ko, K, kinds, P.v, false);
   return 0;
   PS_GUARD_END
 }
@@ -1421,11 +1512,14 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
   if (!le) {
     le = std::make_shared<LaunchEntry>();
     std::vector<ArgKind> kinds;
-    le->P = plan_launch(ko->K, args, nargs, launch, kinds);
+    std::string bits;
+    const Kernel& K = kernel_for(ko, args, nargs, &bits);
+    le->P = plan_launch(K, args, nargs, launch, kinds, bits);
+    le->pad = K.opt.padding;
     bool empty = false;
     for (int d = 0; d < 3; ++d)
       if (le->P.nblocks[d] == 0) empty = true;
-    if (!empty) le->m = get_module(ko, kinds, le->P.v, true);
+    if (!empty) le->m = get_module(ko, K, kinds, le->P.v, true);
     std::lock_guard<std::mutex> g(ko->lmu);
     if (ko->launches.size() >= 256) ko->launches.clear();
     ko->launches[key] = le;
@@ -1442,13 +1536,22 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
     long long iv = 0;
     bool isint = true;
     switch (p.kind) {
-      case 0:
-        std::memcpy(&v, &args[p.arg].field.ptr, 8);
+      case 0: {
+        void* fp = le->pad ? parent_ptr(args[p.arg].field) : args[p.arg].field.ptr;
+        std::memcpy(&v, &fp, 8);
         isint = false;
         break;
+      }
       case 1: {
         const ps_field& f = args[p.arg].field;
-        iv = p.which == 0 ? f.dims[0] : p.which == 1 ? f.dims[1] : p.which == 2 ? f.dims[2] : p.which == 3 ? f.pitch : f.plane;
+        if (p.which < 3) iv = le->pad ? pdim(f, p.which) : f.dims[p.which];
+        else iv = p.which == 3 ? f.pitch : f.plane;
+        break;
+      }
+      case 10: {  // view box in the parent: first (which 0..2) and last (3..5) index
+        const ps_field& f = args[p.arg].field;
+        const int d = p.which % 3;
+        iv = p.which < 3 ? voffset(f, d) + 1 : voffset(f, d) + f.dims[d];
         break;
       }
       case 2: {

@@ -97,6 +97,8 @@ class ps_field(ctypes.Structure):
         ("celldims", ctypes.c_int32 * 4),
         ("blocklen", ctypes.c_int64),
         ("cellnames", ctypes.c_char_p),
+        ("voff", ctypes.c_int64 * 3),
+        ("pdims", ctypes.c_int64 * 3),
     ]
 
 

@@ -106,8 +106,6 @@ def init_parallel_stencil(package=AMDGPU, numbertype=None, ndims=None, *, inboun
     for name, val in (("memopt", memopt), ("nonconst_metadata", nonconst_metadata), ("inbounds", inbounds), ("padding", padding)):
         if not isinstance(val, bool):
             raise ArgumentError(f"{name} must be evaluatable at parse time (e.g. literal or constant) and has to be of type Bool.")
-    if padding:
-        raise ArgumentError("padding=true is not supported by the gfx950 backend: fields are pitch-aligned instead")
     cfg = _Config(package="AMDGPU", numbertype=numbertype, ndims=ndims, inbounds=inbounds, padding=padding, memopt=memopt,
                   nonconst_metadata=nonconst_metadata)
     if mod in _configs:
@@ -316,6 +314,20 @@ class DeviceArray:
         check(lib.ps_field_view(ctypes.byref(self.f), lo3, hi3, ctypes.byref(v)))
         v.ndims = self.f.ndims
         return DeviceArray(v, owner=False, parent=self)
+
+    @property
+    def parent(self):
+        """A.parent of a SubArray: the array this one is a view of (itself if it is none)."""
+        a = self
+        while a._parent is not None:
+            a = a._parent
+        return a
+
+    @property
+    def indices(self):
+        """A.indices of a SubArray: the index ranges of this array in its parent (1-based)."""
+        off = [int(self.f.voff[d]) if self.f.pdims[0] else 0 for d in range(self.f.ndims)]
+        return tuple(UnitRange(off[d] + 1, off[d] + self.shape[d]) for d in range(self.f.ndims))
 
     def reduce(self, op, stream=None):
         out = ctypes.c_double()

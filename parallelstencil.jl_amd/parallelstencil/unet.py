@@ -347,9 +347,45 @@ def dataset(ws, data):
     return X, T, Mk
 
 
-def evaluate(ws, solver, seeds, log10_contrast=1.0):
+def resampled_predictor(ws, n, n2, lx=10.0):
+    """Warm start on an n2 x n2 grid from the network trained on n x n: log10 of the viscosity is
+    interpolated (bilinear) onto the n-grid's nodes, the network predicts there, and the predicted
+    Vx, Vy, Pt are interpolated back onto the n2-grid's staggered positions (miniapps/Stokes2D.jl:
+    nodes at (i-1)dx, Vx between them at (i-3/2)dx, Vy at (j-3/2)dy, dx = lx/(n-1))."""
+    from scipy.ndimage import map_coordinates
+
+    def resample(A, src_pos, dst_pos):
+        # A sampled at positions src_pos[d][i] (uniform per axis) -> values at dst_pos grid
+        coords = []
+        for d in range(2):
+            s0, ds = src_pos[d][0], src_pos[d][1] - src_pos[d][0]
+            coords.append((dst_pos[d] - s0) / ds)
+        X, Y = np.meshgrid(coords[0], coords[1], indexing="ij")
+        return np.asfortranarray(map_coordinates(np.asarray(A, dtype=np.float64), [X, Y], order=1, mode="nearest"))
+
+    def axes(m):
+        dx = lx / (m - 1)
+        nodes = np.arange(m) * dx
+        faces = (np.arange(m + 1) - 0.5) * dx
+        return nodes, faces
+
+    n_nodes, n_faces = axes(n)
+    m_nodes, m_faces = axes(n2)
+
+    def predict(Mus2):
+        logm = resample(np.log10(Mus2), (m_nodes, m_nodes), (n_nodes, n_nodes))
+        Vx, Vy, Pt = ws.predict(10.0 ** logm)
+        return (resample(Vx, (n_faces, n_nodes), (m_faces, m_nodes)), resample(Vy, (n_nodes, n_faces), (m_nodes, m_faces)),
+                resample(Pt, (n_nodes, n_nodes), (m_nodes, m_nodes)))
+    return predict
+
+
+def evaluate(ws, solver, seeds, log10_contrast=1.0, predict=None):
     """Per held-out seed: (cold iterations, cold seconds, warm iterations, warm seconds including
-    inference and the host/device copies of the prediction, final residuals)."""
+    inference and the host/device copies of the prediction, final residuals); with a solver that
+    records its residual history, the cold and warm histories (one residual per check). `predict`
+    maps the viscosity to (Vx, Vy, Pt) (default: the network at the solver's resolution)."""
+    predict = predict or ws.predict
     import time
     from .workloads import stokes2d_inclusions
     out = []
@@ -361,12 +397,13 @@ def evaluate(ws, solver, seeds, log10_contrast=1.0):
         t0 = time.perf_counter()
         itc, errc = solver.solve()
         t1 = time.perf_counter()
+        hist_cold = list(getattr(solver, "history", None) or [])
         solver.reset()
         core.synchronize()
         t2 = time.perf_counter()
-        Vx, Vy, Pt = ws.predict(Mus)
+        Vx, Vy, Pt = predict(Mus)
         fallback = not (np.all(np.isfinite(Vx)) and np.all(np.isfinite(Vy)) and np.all(np.isfinite(Pt)))
-        itw, errw = 0, float("inf")
+        itw, errw, hist_warm = 0, float("inf"), []
         if not fallback:  # the warm-started attempt gets at most twice the plain iteration count
             solver.reset(Vx, Vy, Pt)
             itmax = solver.itermax
@@ -375,12 +412,17 @@ def evaluate(ws, solver, seeds, log10_contrast=1.0):
                 itw, errw = solver.solve()
             finally:
                 solver.itermax = itmax
+            hist_warm = list(getattr(solver, "history", None) or [])
             fallback = not errw <= solver.eps
         if fallback:  # the plain iteration from rest decides; its cost is charged to the warm start
             solver.reset()
             it2, errw = solver.solve()
             itw += it2
         t3 = time.perf_counter()
-        out.append(dict(seed=s, cold_iters=itc, cold_s=t1 - t0, warm_iters=itw, warm_s=t3 - t2, err_cold=errc, err_warm=errw,
-                        fallback=fallback))
+        rec = dict(seed=s, cold_iters=itc, cold_s=t1 - t0, warm_iters=itw, warm_s=t3 - t2, err_cold=errc, err_warm=errw,
+                   fallback=fallback)
+        if hist_cold:
+            rec["hist_cold"] = [float(f"{e:.4e}") for e in hist_cold]
+            rec["hist_warm"] = [float(f"{e:.4e}") for e in hist_warm]
+        out.append(rec)
     return out

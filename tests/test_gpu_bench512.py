@@ -1,8 +1,8 @@
 """The exact kernels that bench.py times, at the benched size: triad and 3-D heat diffusion
 (plain and memopt) on 512^3 FP64 fields, bit for bit against the oracle after a few steps.
 
-The memopt launch at this size is the one the benchmark runs: 128x4 blocks, LDS plane tiles,
-XCD-aware block order, loopsize 32 (examples/memcopy3D.jl:10-13,
+The memopt launch at this size is the one the benchmark runs: 256x4 blocks, LDS plane tiles,
+nontemporal stores, loopsize 32 (examples/memcopy3D.jl:10-13,
 examples/diffusion3D_novis_noperf.jl:10-13 in the flux form of test/test_parallel.jl:419-426)."""
 import numpy as np
 import pytest
@@ -40,7 +40,7 @@ def test_diffusion_512_bitexact(memopt):
     k = W.define_diffusion3d(memopt, MOD)
     geo = k.geometry(*(ps.ArgSpec((N, N, N)) for _ in range(3)), *args)
     if memopt:
-        assert geo.memopt and geo.nthreads == (128, 4, 1) and geo.nblocks == (4, 128, 16), geo
+        assert geo.memopt and geo.nthreads == (256, 4, 1) and geo.nblocks == (2, 128, 16), geo
     T, T2, Cd = ps.zeros(N, N, N), ps.zeros(N, N, N), ps.zeros(N, N, N)
     T.copy_from(T0)
     T2.copy_from(T0)
