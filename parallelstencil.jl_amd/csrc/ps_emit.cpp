@@ -105,11 +105,11 @@ static std::string sanitize(const std::string& s) {
 static const char* kPrelude = R"PSD(
 #define PS_DI __device__ __forceinline__
 PS_DI float ps_bf16_ld(unsigned short v) { return __uint_as_float(((unsigned)v) << 16); }
+// FP32 to BFloat16 bits: nearest even, every NaN to 0x7fc0 (the BFloat16(::Float32) of
+// BFloat16s.jl); gfx950 rounds in one instruction
 PS_DI unsigned short ps_bf16_st(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)0x7fc0u;
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
+  const unsigned short u = __builtin_bit_cast(unsigned short, (__bf16)f);
+  return f != f ? (unsigned short)0x7fc0u : u;
 }
 PS_DI double ps_max(double x, double y) { bool py = (y > x) || (__builtin_signbit(y) < __builtin_signbit(x)); return py ? (x != x ? x : y) : (y != y ? y : x); }
 PS_DI float  ps_max(float x, float y)   { bool py = (y > x) || (__builtin_signbit(y) < __builtin_signbit(x)); return py ? (x != x ? x : y) : (y != y ? y : x); }
@@ -161,6 +161,12 @@ PS_DI float ps_frcp(float b) {
   const float r = __builtin_fmaf(r0, __builtin_fmaf(-b, r0, 1.0f), r0);
   return r == r ? r : r0;
 }
+// Element through a uniform element offset u and a per-thread 32-bit byte offset vo: the address
+// form of global loads and stores with an SGPR base and a VGPR offset
+template <class T>
+PS_DI T& ps_at(T* p, ps_int u, unsigned vo) { return *(T*)((char*)(p + u) + vo); }
+template <class T>
+PS_DI const T& ps_at(const T* p, ps_int u, unsigned vo) { return *(const T*)((const char*)(p + u) + vo); }
 // Guarded load without a branch: element 0 of the array stands in for an index outside it
 // (the runtime only launches these kernels on non-empty arrays), and the value is then zeroed.
 template <class T>
@@ -243,6 +249,10 @@ class Emitter {
   std::map<int, Ty> rcp_params;
   bool fast_div = false;   // emitting the reciprocal form of a statement
   bool used_fast = false;  // the current statement used it
+  // scalar subexpressions the host evaluates once per launch: canonical text -> program index
+  std::map<std::string, int> uni_index;
+  std::vector<Emitted::UProg> uprogs;
+  bool uniform_prog(const NodeP& n, std::vector<Emitted::UOp>& ops, Ty& t, int& nops) const;
   int tmp_counter = 0;
   int E_lds_bytes = 0;  // static LDS of the emitted kernel
   EV divide(const NodeP& n);
@@ -285,7 +295,7 @@ class Emitter {
 
   EV expr(const NodeP& n);
   EV load(const NodeP& idxnode);
-  std::string offset(int ai, const std::vector<NodeP>& idx, std::string idxc[3], bool* affine_ok);
+  std::string offset(int ai, const std::vector<NodeP>& idx, std::string idxc[3], bool* affine_ok, std::string* ref32 = nullptr);
   std::vector<NodeP> norm_indices(const NodeP& idxnode, int ai);
   EV call(const NodeP& n);
   EV binary(const std::string& op, EV a, EV b, const NodeP& rhsnode);
@@ -316,7 +326,9 @@ std::vector<NodeP> Emitter::norm_indices(const NodeP& n, int ai) {
 }
 
 // Element offset of A[i,j,k] and the C expressions of the three indices.
-std::string Emitter::offset(int ai, const std::vector<NodeP>& idx, std::string idxc[3], bool* affine_ok) {
+// With ref32, an affine access of a plain off32 variant also gets its element reference in the
+// SGPR-base form (ps_at).
+std::string Emitter::offset(int ai, const std::vector<NodeP>& idx, std::string idxc[3], bool* affine_ok, std::string* ref32) {
   static const char* szn[3] = {"_nx", "_ny", "_nz"};
   std::string A = an(ai);
   long long c[3] = {0, 0, 0};
@@ -395,6 +407,7 @@ std::string Emitter::offset(int ai, const std::vector<NodeP>& idx, std::string i
     if (c[0]) k += " + (" + std::to_string(c[0]) + ")";
     if (c[1]) k += " + (" + std::to_string(c[1]) + ")*" + A + "_sx";
     if (c[2]) k += " + (" + std::to_string(c[2]) + ")*" + A + "_sxy";
+    if (ref32 && V.off32 && !V.memopt && kinds[ai].ncomp == 0) *ref32 = "ps_at(" + A + ", (ps_int)0" + k + ", " + A + "_vo)";
     return "(" + off + k + ")";
   }
   return "((" + idxc[0] + " - 1) + (" + idxc[1] + " - 1)*" + A + "_sx + (" + idxc[2] + " - 1)*" + A + "_sxy)";
@@ -476,9 +489,10 @@ EV Emitter::load(const NodeP& n) {
   }
   std::string ic[3];
   bool aff = false;
-  std::string off = offset(ai, idx, ic, &aff);
+  std::string ref32;
+  std::string off = offset(ai, idx, ic, &aff, &ref32);
   std::string A = an(ai);
-  std::string raw = A + "[" + off + "]";
+  std::string raw = ref32.empty() ? A + "[" + off + "]" : ref32;
   int dt = kinds[ai].dtype;
   std::string val;
   if (V.checked) {
@@ -779,7 +793,120 @@ EV Emitter::call(const NodeP& n) {
   throw PsError(PS_EARGUMENT, "unsupported function '" + f + "' in kernel " + K.def.name);
 }
 
+static int uprog_ty(Ty t) { return t == Ty::Int ? 0 : t == Ty::F32 ? 1 : 2; }
+
+// Postfix program of an expression built only from scalar arguments, literals and + - * /
+// (and literal powers), typed by the rules expr() applies; false for anything else, and for
+// integer arithmetic, which stays on the device. Without contraction and with every quotient
+// correctly rounded, the host's IEEE evaluation of the program is the value the device would
+// compute for the expression in every cell.
+bool Emitter::uniform_prog(const NodeP& n, std::vector<Emitted::UOp>& ops, Ty& t, int& nops) const {
+  using U = Emitted::UOp;
+  switch (n->k) {
+    case NK::Num: {
+      U u;
+      u.op = U::Lit;
+      if (n->is_int) {
+        if (!V.idx64 && (n->ival > INT32_MAX || n->ival < INT32_MIN)) return false;
+        t = Ty::Int;
+        u.ilit = n->ival;
+      } else {
+        t = n->is_f32 ? Ty::F32 : NT;
+        u.lit = n->num;
+      }
+      u.ty = uprog_ty(t);
+      ops.push_back(u);
+      return true;
+    }
+    case NK::Ident: {
+      for (int d = 0; d < 3; ++d)
+        if (n->s == kIdx[d]) return false;
+      if (locals.count(n->s)) return false;
+      U u;
+      if (n->s == "pi" || n->s == "π") {
+        t = NT;
+        u.op = U::Lit;
+        u.lit = M_PI;
+      } else {
+        auto it = K.param_index.find(n->s);
+        if (it == K.param_index.end()) return false;
+        const ArgKind& k = kinds[it->second];
+        if (k.is_field || !(k.dtype == PS_F64 || k.dtype == PS_F32 || k.dtype == PS_I32 || k.dtype == PS_I64)) return false;
+        t = value_ty(k.dtype);
+        u.op = U::Arg;
+        u.arg = it->second;
+      }
+      u.ty = uprog_ty(t);
+      ops.push_back(u);
+      return true;
+    }
+    case NK::Unary: {
+      if (n->s != "-" && n->s != "+") return false;
+      if (!uniform_prog(n->a[0], ops, t, nops)) return false;
+      if (n->s == "-") {
+        if (t == Ty::Int) return false;
+        U u;
+        u.op = U::Neg;
+        u.ty = uprog_ty(t);
+        ops.push_back(u);
+      }
+      return true;
+    }
+    case NK::Binary: {
+      const std::string& op = n->s;
+      U u;
+      if (op == "^") {  // Base.literal_pow, for the exponents binary() inlines
+        const NodeP& e = n->a[1];
+        long long k = 99;
+        if (e->k == NK::Num && e->is_int) k = e->ival;
+        else if (e->k == NK::Unary && e->s == "-" && e->a[0]->k == NK::Num && e->a[0]->is_int) k = -e->a[0]->ival;
+        if (k < -2 || k > 3) return false;
+        if (!uniform_prog(n->a[0], ops, t, nops) || t == Ty::Int) return false;
+        if (k == 1) return true;
+        u.op = k == 0 ? U::Pow0 : k == 2 ? U::Pow2 : k == 3 ? U::Pow3 : k == -1 ? U::PowM1 : U::PowM2;
+      } else {
+        if (op != "+" && op != "-" && op != "*" && op != "/") return false;
+        Ty ta, tb;
+        if (!uniform_prog(n->a[0], ops, ta, nops) || !uniform_prog(n->a[1], ops, tb, nops)) return false;
+        t = promote(ta, tb);
+        if (op == "/" && t == Ty::Int) t = Ty::F64;
+        if (t == Ty::Int) return false;
+        u.op = op == "+" ? U::Add : op == "-" ? U::Sub : op == "*" ? U::Mul : U::Div;
+      }
+      u.ty = uprog_ty(t);
+      ops.push_back(u);
+      ++nops;
+      return true;
+    }
+    default: return false;
+  }
+}
+
 EV Emitter::expr(const NodeP& n) {
+  // a float subexpression of scalar arguments and literals becomes one hidden argument that the
+  // host evaluates (not under fastmath or contraction, whose device forms differ from IEEE)
+  if ((n->k == NK::Binary || n->k == NK::Unary) && !K.opt.fastmath && !K.opt.fp_contract) {
+    std::vector<Emitted::UOp> ops;
+    Ty t = Ty::Int;
+    int nops = 0;
+    if (uniform_prog(n, ops, t, nops) && nops > 0 && is_float(t)) {
+      const std::string key = to_julia(n);
+      auto it = uni_index.find(key);
+      int u;
+      if (it == uni_index.end()) {
+        u = (int)uprogs.size();
+        uni_index[key] = u;
+        Emitted::UProg p;
+        p.ops = std::move(ops);
+        p.ty = uprog_ty(t);
+        p.idx64 = V.idx64;
+        uprogs.push_back(std::move(p));
+      } else {
+        u = it->second;
+      }
+      return EV{"ps_u" + std::to_string(u), t};
+    }
+  }
   switch (n->k) {
     case NK::Num:
       if (n->is_int) return EV{"((ps_int)" + std::to_string(n->ival) + ")", Ty::Int};
@@ -955,8 +1082,8 @@ void Emitter::assign(const NodeP& s) {
   bool blk = false;
   EV r = rhs_value(rhs, &blk);
   auto idx = norm_indices(lhs, ai);
-  std::string ic[3];
-  std::string off = offset(ai, idx, ic, nullptr);
+  std::string ic[3], ref32;
+  std::string off = offset(ai, idx, ic, nullptr, &ref32);
   int dt = kinds[ai].dtype;
   std::string v;
   if (dt == PS_BF16) v = "ps_bf16_st(" + cast(r, Ty::F32).c + ")";
@@ -965,12 +1092,13 @@ void Emitter::assign(const NodeP& s) {
   else if (dt == PS_BOOL) v = "((unsigned char)(" + r.c + "))";
   else v = std::string("((") + elem_cty(dt) + ")(" + r.c + "))";
   std::string A = an(ai);
+  const std::string ref = ref32.empty() ? A + "[" + off + "]" : ref32;
   if (V.checked) {
     o << pad() << "if (ps_chk(" << ic[0] << ", " << ic[1] << ", " << ic[2] << ", " << A << "_nx, " << A << "_ny, " << A
-      << "_nz, ps_err)) " << A << "[" << off << "] = " << v << ";\n";
+      << "_nz, ps_err)) " << ref << " = " << v << ";\n";
   } else {
-    if (V.nt) o << pad() << "__builtin_nontemporal_store(" << v << ", &" << A << "[" << off << "]);\n";
-    else o << pad() << A << "[" << off << "] = " << v << ";\n";
+    if (V.nt) o << pad() << "__builtin_nontemporal_store(" << v << ", &" << ref << ");\n";
+    else o << pad() << ref << " = " << v << ";\n";
   }
   if (blk) {
     --ind;
@@ -1022,7 +1150,9 @@ EV Emitter::rhs_value(const NodeP& rhs, bool* opened) {
   ++ind;
   o << pad() << "bool ps_bad = false;\n";
   o << pad() << cty(f.t) << " " << var << " = " << f.c << ";\n";
-  o << pad() << "if (ps_bad) " << var << " = " << cast(e, f.t).c << ";\n";
+  // the empty volatile asm keeps the rarely taken IEEE form behind its branch; without it the
+  // compiler may evaluate that form in every cell and select
+  o << pad() << "if (ps_bad) { __asm__ volatile(\"\"); " << var << " = " << cast(e, f.t).c << "; }\n";
   *opened = true;
   return EV{var, f.t};
 }
@@ -1895,9 +2025,17 @@ void Emitter::emit_memopt_body(int L) {
 
 Emitted Emitter::run() {
   Emitted E;
-  // variants carry their execution path in the symbol so that profiles tell them apart
+  // variants carry their execution path in the symbol so that profiles tell them apart; kernels
+  // whose array arguments are all FP32 or all BFloat16 also carry their storage type
   std::string entry = "psk_" + sanitize(K.def.name) + (V.memopt ? (V.lds ? "_memopt_lds" : "_memopt") : "") + (V.checked ? "_checked" : "") +
                       (K.opt.fastmath ? "_fastmath" : "");
+  {
+    int fdt = 0;
+    for (auto& k : kinds)
+      if (k.is_field) fdt = (fdt == 0 || fdt == k.dtype) ? k.dtype : -1;
+    if (fdt == PS_F32) entry += "_f32";
+    if (fdt == PS_BF16) entry += "_bf16";
+  }
   E.entry = entry;
   const char* I = V.idx64 ? "long long" : "int";
   int ib = V.idx64 ? 8 : 4;
@@ -1953,6 +2091,12 @@ Emitted Emitter::run() {
     sig << ",\n    " << cty(rp.second) << " " << an(rp.first) << "_r";
     E.params.push_back({K.opt.fastmath ? 7 : 6, rp.first, 0, rp.second == Ty::F64 ? 8 : 4, rp.second == Ty::F64 ? PS_F64 : PS_F32});
   }
+  for (size_t u = 0; u < uprogs.size(); ++u) {  // host-evaluated scalar subexpressions
+    const bool f64 = uprogs[u].ty == 2;
+    sig << ",\n    " << (f64 ? "double" : "float") << " ps_u" << u;
+    E.params.push_back({11, (int)u, 0, f64 ? 8 : 4, f64 ? PS_F64 : PS_F32});
+  }
+  E.uprogs = uprogs;
 
   std::ostringstream src;
   src << "// generated by libpsamd for kernel " << K.def.name << " (variant " << V.key() << ")\n";
@@ -1969,7 +2113,25 @@ Emitted Emitter::run() {
     for (size_t i = 0; i < kinds.size(); ++i) {
       if (!kinds[i].is_field) continue;
       std::string A = an((int)i);
-      o << "  const ps_int " << A << "_o = (ix - 1) + (iy - 1)*" << A << "_sx + (iz - 1)*" << A << "_sxy; (void)" << A << "_o;\n";
+      // fields with the strides of an earlier one share its cell offset
+      int same = -1;
+      if (V.sclass.size() == kinds.size() && V.sclass[i] != '-')
+        for (size_t j = 0; j < i && same < 0; ++j)
+          if (V.sclass[j] == V.sclass[i]) same = (int)j;
+      if (same >= 0) {
+        o << "  const ps_int " << A << "_o = " << an(same) << "_o; (void)" << A << "_o;\n";
+      } else if (V.u24) {
+        o << "  const ps_int " << A << "_o = (ix - 1) + (ps_int)__umul24((unsigned)(iy - 1), (unsigned)" << A << "_sx) + (ps_int)__umul24((unsigned)(iz - 1), (unsigned)"
+          << A << "_sxy); (void)" << A << "_o;\n";
+      } else {
+        o << "  const ps_int " << A << "_o = (ix - 1) + (iy - 1)*" << A << "_sx + (iz - 1)*" << A << "_sxy; (void)" << A << "_o;\n";
+      }
+      if (V.off32 && kinds[i].ncomp == 0) {
+        if (same >= 0 && kinds[same].ncomp == 0 && dtype_bytes(kinds[same].dtype) == dtype_bytes(kinds[i].dtype))
+          o << "  const unsigned " << A << "_vo = " << an(same) << "_vo; (void)" << A << "_vo;\n";
+        else
+          o << "  const unsigned " << A << "_vo = (unsigned)" << A << "_o * " << dtype_bytes(kinds[i].dtype) << "u; (void)" << A << "_vo;\n";
+      }
     }
     for (auto& v : local_order) o << "  " << cty(locals[v]) << " " << local_c[v] << " = 0;\n";
     for (auto& sm : sm_dtype) {

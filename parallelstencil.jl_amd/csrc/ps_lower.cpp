@@ -1394,8 +1394,8 @@ std::string kernel_info_json(const Kernel& K) {
 
 std::string Variant::key() const {
   char b[96];
-  std::snprintf(b, sizeof b, "m%dc%di%da%dt%dl%d_%dx%dn%d", memopt, checked, idx64, noalias, maxthreads, lds, bx, by, nt);
-  return std::string(b) + "d" + divmode + (padbits.empty() ? "" : "p" + padbits);
+  std::snprintf(b, sizeof b, "m%dc%di%da%dt%dl%d_%dx%dn%do%du%d", memopt, checked, idx64, noalias, maxthreads, lds, bx, by, nt, off32, u24);
+  return std::string(b) + "d" + divmode + (padbits.empty() ? "" : "p" + padbits) + (sclass.empty() ? "" : "s" + sclass);
 }
 
 }  // namespace ps

@@ -54,10 +54,9 @@ struct Options {
   int loopdim = 0;
   std::vector<std::string> optvars;
   std::map<std::string, std::vector<std::pair<int, int>>> optranges;
-  // memopt block shape limits: 128 x 4 threads streaming along z measured fastest on gfx950
-  // (wider rows of HBM per wave, smaller LDS halo share than 64 x 4)
-  // 0: automatic (128 x 4, or 256 x 4 for 3-D streams whose x range is a multiple of 256 of at
-  // least 512 cells: wider rows of HBM per wave, measured fastest at 512^3)
+  // memopt block shape limits; 0: automatic (128 x 4, or 256 x 4 for 3-D streams whose x range
+  // is a multiple of 256 of at least 512 cells: wider rows of HBM per wave and a smaller LDS halo
+  // share than 64 x 4, measured fastest at 512^3)
   int nthreads_max_memopt = 0;
   int nthreads_x_max_memopt = 0;
   int nthreads_x_max = 64;
@@ -113,6 +112,15 @@ struct Variant {
   int maxthreads = 256;   // __launch_bounds__
   bool lds = false;       // memopt: stage x-y neighbours of optvars through LDS tiles
   bool nt = false;        // nontemporal stores for array writes
+  // plain kernels: affine accesses address an SGPR base (array + uniform offset) plus a 32-bit
+  // per-thread byte offset; the host sets it when every thread's cell offset is non-negative and
+  // every byte offset of the launch box fits in 31 bits
+  bool off32 = false;
+  // plain kernels: cell offsets from 24-bit multiplies (every stride and index below 2^24)
+  bool u24 = false;
+  // plain kernels: per argument, a letter shared by fields with equal strides (one cell offset
+  // serves them all), '-' for scalars; empty: every field computes its own
+  std::string sclass;
   int bx = 0, by = 0;     // block shape baked into LDS variants
   // per argument: 'p' / 'n' a positive / negative float scalar accepted as reciprocal divisor,
   // 'x' a float scalar that must be divided by exactly, '-' anything else
@@ -131,6 +139,7 @@ struct Emitted {
                    // 6 RN(1/|b|) of a uniform divisor or 0 (exact division), 7 RN(1/b) (fastmath),
                    // 8 bytes of dynamic LDS of the launch (@sharedMem), 9 blocklen of a CellArray,
                    // 10 view box of a field in its parent (padding=true): which 0..2 first, 3..5 last index
+                   // 11 value of the host-evaluated scalar subexpression uprogs[arg]
     int arg;       // arg index (fields/scalars)
     int which;     // for sizes: 0 nx 1 ny 2 nz 3 sx 4 sxy; for ranges: dim
     int bytes;     // 4 or 8
@@ -138,6 +147,21 @@ struct Emitted {
   };
   std::vector<P> params;
   int lds_bytes = 0;
+  // Subexpressions of scalar arguments and literals only (dt/ρ, dt*k): the host evaluates them
+  // once per launch, in the kernel's arithmetic, as postfix programs over a typed stack.
+  struct UOp {
+    enum Code { Arg, Lit, Neg, Add, Sub, Mul, Div, Pow0, Pow2, Pow3, PowM1, PowM2 } op;
+    int arg = -1;         // Arg: argument index
+    int ty = 0;           // type of the result: 0 int (ps_int), 1 float, 2 double
+    double lit = 0.0;     // Lit of a float type
+    long long ilit = 0;   // Lit of the integer type
+  };
+  struct UProg {
+    std::vector<UOp> ops;
+    int ty = 2;           // 1 float, 2 double
+    bool idx64 = false;   // ps_int is 64-bit (integer arguments and literals are truncated to it)
+  };
+  std::vector<UProg> uprogs;
 };
 
 // Divisors for which the reciprocal-based division of the generated code is exact.

@@ -1112,6 +1112,93 @@ extern "C" int ps_kernel_info(ps_kernel k, char* out, size_t cap) {
   PS_GUARD_END
 }
 
+// Host evaluation of a hoisted scalar subexpression (Emitted::UProg) in the arithmetic the device
+// would use for it: every operation in its result type (float or double) after converting both
+// operands to that type, integers truncated to ps_int, no contraction. IEEE +, -, *, / are
+// correctly rounded on both sides, so the value is the one each cell would have computed.
+namespace {
+struct UVal {
+  int ty;  // 0 ps_int, 1 float, 2 double
+  long long i;
+  float f;
+  double d;
+};
+UVal uval_to(const UVal& v, int ty) {
+  if (v.ty == ty) return v;
+  UVal r{ty, 0, 0.0f, 0.0};
+  if (ty == 1 && v.ty == 0) r.f = (float)v.i;
+  else if (ty == 2 && v.ty == 0) r.d = (double)v.i;
+  else if (ty == 2 && v.ty == 1) r.d = (double)v.f;
+  else throw PsError(PS_EINTERNAL, "hoisted subexpression narrows a value");
+  return r;
+}
+template <class T>
+T uarith(int op, T a, T b) {
+#pragma clang fp contract(off)
+  using U = Emitted::UOp;
+  switch (op) {
+    case U::Add: return a + b;
+    case U::Sub: return a - b;
+    case U::Mul: return a * b;
+    case U::Div: return a / b;
+  }
+  throw PsError(PS_EINTERNAL, "hoisted subexpression: bad operation");
+}
+template <class T>
+T uunary(int op, T a) {
+#pragma clang fp contract(off)
+  using U = Emitted::UOp;
+  switch (op) {
+    case U::Neg: return -a;
+    case U::Pow0: return (T)1;
+    case U::Pow2: return a * a;
+    case U::Pow3: return a * a * a;
+    case U::PowM1: return (T)1 / a;
+    case U::PowM2: {
+      const T i = (T)1 / a;
+      return i * i;
+    }
+  }
+  throw PsError(PS_EINTERNAL, "hoisted subexpression: bad operation");
+}
+UVal eval_uprog(const Emitted::UProg& p, const ps_arg* args) {
+  using U = Emitted::UOp;
+  std::vector<UVal> st;
+  for (const U& u : p.ops) {
+    if (u.op == U::Arg || u.op == U::Lit) {
+      UVal v{u.ty, 0, 0.0f, 0.0};
+      const double fv = u.op == U::Arg ? args[u.arg].f : u.lit;
+      const long long iv = u.op == U::Arg ? args[u.arg].i : u.ilit;
+      if (u.ty == 2) v.d = fv;
+      else if (u.ty == 1) v.f = (float)fv;
+      else v.i = p.idx64 ? iv : (long long)(int32_t)iv;
+      st.push_back(v);
+    } else if (u.op >= U::Add && u.op <= U::Div) {
+      if (st.size() < 2) throw PsError(PS_EINTERNAL, "malformed hoisted subexpression");
+      const UVal b = uval_to(st.back(), u.ty);
+      st.pop_back();
+      const UVal a = uval_to(st.back(), u.ty);
+      st.pop_back();
+      UVal r{u.ty, 0, 0.0f, 0.0};
+      if (u.ty == 1) r.f = uarith<float>(u.op, a.f, b.f);
+      else if (u.ty == 2) r.d = uarith<double>(u.op, a.d, b.d);
+      else throw PsError(PS_EINTERNAL, "hoisted subexpression: integer arithmetic");
+      st.push_back(r);
+    } else {
+      if (st.empty()) throw PsError(PS_EINTERNAL, "malformed hoisted subexpression");
+      UVal& x = st.back();
+      if (x.ty == 1) x.f = uunary<float>(u.op, x.f);
+      else if (x.ty == 2) x.d = uunary<double>(u.op, x.d);
+      else throw PsError(PS_EINTERNAL, "hoisted subexpression: integer arithmetic");
+    }
+  }
+  if (st.size() != 1 || st.back().ty != p.ty) throw PsError(PS_EINTERNAL, "malformed hoisted subexpression");
+  return st.back();
+}
+double eval_uprog_f64(const Emitted::UProg& p, const ps_arg* args) { return eval_uprog(p, args).d; }
+float eval_uprog_f32(const Emitted::UProg& p, const ps_arg* args) { return eval_uprog(p, args).f; }
+}  // namespace
+
 // Division mode of every argument (see Variant::divmode): float scalars whose magnitude the
 // reciprocal division handles exactly are 'p' / 'n' by sign, other float scalars 'x'.
 static std::string divmode_of(const ps_arg* args, int nargs) {
@@ -1142,6 +1229,8 @@ static Variant variant_from_flags(const Kernel& K, int flags) {
     if (!v.memopt || v.bx < 1 || v.by < 1) throw PsError(PS_EARGUMENT, "LDS variants need memopt and a block shape");
   }
   v.nt = K.opt.nt_store == 1 || (K.opt.nt_store == -1 && v.lds && v.bx >= 256);
+  v.off32 = (flags & 64) != 0 && !v.memopt && !v.checked;  // 32-bit per-thread byte offsets
+  v.u24 = v.off32 && (flags & 128) != 0;                      // 24-bit cell-offset multiplies
   return v;
 }
 
@@ -1385,6 +1474,36 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   // asserts that every access is inside its array and no check is emitted
   std::string why;
   v.checked = !K.opt.inbounds && !prove_in_bounds(K, sh, memopt, &why);
+  // 32-bit per-thread byte offsets (Variant::off32) for plain kernels whose launch box starts at
+  // index 1 or above in every dimension and whose fields' byte offsets over the box stay < 2^31
+  if (!memopt && !v.checked && !pad) {
+    bool ok = sh.lo[0] >= 1 && sh.lo[1] >= 1 && sh.lo[2] >= 1;
+    for (int i = 0; i < nargs && ok; ++i) {
+      if (args[i].kind != PS_ARG_FIELD) continue;
+      const ps_field& f = args[i].field;
+      const int64_t top = (sh.hi[0] - 1) + (sh.hi[1] - 1) * f.pitch + (sh.hi[2] - 1) * f.plane;
+      if (extent_bytes(f) >= ((int64_t)1 << 31) || top < 0 || top * (int64_t)elsize(f.dtype) >= ((int64_t)1 << 31)) ok = false;
+    }
+    v.off32 = ok;
+    // 24-bit multiplies for the cell offsets: every stride and every index of the box below 2^24
+    // (the products then stay below 2^31 by the test above)
+    bool u24 = ok && sh.hi[0] <= (1 << 24) && sh.hi[1] <= (1 << 24) && sh.hi[2] <= (1 << 24);
+    for (int i = 0; i < nargs && u24; ++i)
+      if (args[i].kind == PS_ARG_FIELD && (args[i].field.pitch >= (1 << 24) || args[i].field.plane >= (1 << 24))) u24 = false;
+    v.u24 = u24;
+    // stride classes: fields (not CellArrays) with equal strides share one cell offset
+    std::string sc(nargs, '-');
+    std::vector<std::pair<int64_t, int64_t>> seen;
+    for (int i = 0; i < nargs; ++i) {
+      if (args[i].kind != PS_ARG_FIELD || args[i].field.ncomp > 0) continue;
+      const std::pair<int64_t, int64_t> st{args[i].field.pitch, args[i].field.plane};
+      size_t c = 0;
+      while (c < seen.size() && seen[c] != st) ++c;
+      if (c == seen.size()) seen.push_back(st);
+      sc[i] = (char)('a' + (c % 26));
+    }
+    if (seen.size() <= 26) v.sclass = sc;
+  }
   P.v = v;
   return P;
 }
@@ -1586,6 +1705,18 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
         } else {
           float b = std::fabs((float)a.f), r = rcp_safe32(b) ? 1.0f / b : 0.0f;
           std::memcpy(&v, &r, 4);
+        }
+        isint = false;
+        break;
+      }
+      case 11: {  // host-evaluated scalar subexpression
+        const auto& up = m->em.uprogs[p.arg];
+        if (up.ty == 2) {
+          double d = eval_uprog_f64(up, args);
+          std::memcpy(&v, &d, 8);
+        } else {
+          float f = eval_uprog_f32(up, args);
+          std::memcpy(&v, &f, 4);
         }
         isint = false;
         break;

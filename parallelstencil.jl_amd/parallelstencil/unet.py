@@ -1,7 +1,7 @@
 """U-Net warm start for the pseudo-transient Stokes solver, on the library's own GPU kernels.
 
 No counterpart in the reference (ParallelStencil.jl has no learned component). The network maps
-the viscosity field (log10 eta) to a starting point (Vx, Vy, P) for the PT iteration of
+the viscosity field (log10 eta) and the body force to a starting point (Vx, Vy, P) for the PT iteration of
 miniapps/Stokes2D.jl; the plain iteration then runs to the same residual tolerance from there,
 so the solver stays the judge of the answer.
 
@@ -138,7 +138,7 @@ class Conv:
 
 
 class UNet:
-    """4-level U-Net (see the module docstring). Input (B, H, W, 1), output (B, H, W, 3); H and W
+    """4-level U-Net (see the module docstring). Input (B, H, W, cin), output (B, H, W, 3); H and W
     divisible by 8."""
 
     def __init__(self, base=32, cin=1, cout=3, seed=0):
@@ -231,13 +231,14 @@ class UNet:
 
 class WarmStart:
     """Training data handling and inference for the Stokes warm start on an nx x ny grid:
-    input log10(Mus) (nx x ny, padded to a multiple of 8 by edge replication), targets Vx, Vy, Pt
-    on the padded grid (padding masked out of the loss), normalised per channel."""
+    inputs log10(Mus) and the body force Rog (nx x ny, padded to a multiple of 8 by edge
+    replication; the body force marks inclusions whose viscosity equals the matrix's), targets
+    Vx, Vy, Pt on the padded grid (padding masked out of the loss), normalised per channel."""
 
     def __init__(self, nx, ny, base=32, seed=0):
         self.nx, self.ny = nx, ny
         self.H, self.W = _up8(ny + 1), _up8(nx + 1)
-        self.net = UNet(base=base, seed=seed)
+        self.net = UNet(base=base, cin=2, seed=seed)
         self.mu = np.zeros(3, np.float32)
         self.sd = np.ones(3, np.float32)
 
@@ -246,8 +247,8 @@ class WarmStart:
         P = np.asarray(A, dtype=np.float32).T
         return np.pad(P, ((0, self.H - P.shape[0]), (0, self.W - P.shape[1])), mode="edge")
 
-    def inputs(self, Mus):
-        return self._plane(np.log10(Mus))[..., None]
+    def inputs(self, Mus, Rog):
+        return np.stack([self._plane(np.log10(Mus)), self._plane(Rog)], -1)
 
     def targets(self, Vx, Vy, Pt):
         """Pressure is defined up to a constant (the PT iteration drifts it per sample): the
@@ -267,12 +268,14 @@ class WarmStart:
 
     def train(self, X, T, Mk, epochs, batch=8, lr=1e-3, seed=0, log=None, cosine=False):
         """Adam on the masked MSE of the normalised targets (learning rate constant, or decaying
-        as a half cosine over the epochs); returns the loss history (per epoch)."""
+        as a half cosine over the epochs), in batches of `batch` samples (at most the sample
+        count); returns the loss history (per epoch)."""
         lr0 = lr
         n = X.shape[0]
+        batch = max(1, min(batch, n))
         H, W = self.H, self.W
         Tn = ((T - self.mu) / self.sd).astype(np.float32)
-        xb = Buf((batch, H, W, 1))
+        xb = Buf((batch, H, W, X.shape[-1]))
         tb, mb = Buf((batch, H, W, 3), "f32"), Buf((batch, H, W, 3), "f32")
         dout, loss = Buf((batch, H, W, 3)), Buf((1,), "f32")
         rng = np.random.default_rng(seed)
@@ -301,12 +304,12 @@ class WarmStart:
                 log(f"epoch {ep}: loss {hist[-1]:.4e}")
         return hist
 
-    def predict(self, Mus):
+    def predict(self, Mus, Rog):
         """(Vx, Vy, Pt) host arrays on the solver's staggered shapes."""
         H, W = self.H, self.W
-        xb = self._in if getattr(self, "_in", None) is not None else Buf((1, H, W, 1))
+        xb = self._in if getattr(self, "_in", None) is not None else Buf((1, H, W, 2))
         self._in = xb
-        xb.upload(self.inputs(Mus)[None])
+        xb.upload(self.inputs(Mus, Rog)[None])
         y = self.net.forward(xb, 1, H, W).download()[0]
         y = y * self.sd + self.mu
         nx, ny = self.nx, self.ny
@@ -322,7 +325,7 @@ def _up8(n):
 
 def generate_samples(solver, seeds, log10_contrast=1.0):
     """Cold PT solves (to the solver's tolerance) of the inclusion samples `seeds`; returns
-    (list of (Mus, Vx, Vy, Pt), list of (iterations, seconds, residual))."""
+    (list of (Mus, Rog, Vx, Vy, Pt), list of (iterations, seconds, residual))."""
     import time
     from .workloads import stokes2d_inclusions
     data, stats = [], []
@@ -335,21 +338,21 @@ def generate_samples(solver, seeds, log10_contrast=1.0):
         it, err = solver.solve()
         t1 = time.perf_counter()
         stats.append((it, t1 - t0, err))
-        data.append((Mus, solver.Vx.to_host(), solver.Vy.to_host(), solver.Pt.to_host()))
+        data.append((Mus, Rog, solver.Vx.to_host(), solver.Vy.to_host(), solver.Pt.to_host()))
     return data, stats
 
 
 def dataset(ws, data):
-    X = np.stack([ws.inputs(d[0]) for d in data]).astype(np.float32)
-    TM = [ws.targets(d[1], d[2], d[3]) for d in data]
+    X = np.stack([ws.inputs(d[0], d[1]) for d in data]).astype(np.float32)
+    TM = [ws.targets(d[2], d[3], d[4]) for d in data]
     T = np.stack([t for t, _ in TM]).astype(np.float32)
     Mk = np.stack([m for _, m in TM]).astype(np.float32)
     return X, T, Mk
 
 
 def resampled_predictor(ws, n, n2, lx=10.0):
-    """Warm start on an n2 x n2 grid from the network trained on n x n: log10 of the viscosity is
-    interpolated (bilinear) onto the n-grid's nodes, the network predicts there, and the predicted
+    """Warm start on an n2 x n2 grid from the network trained on n x n: log10 of the viscosity and
+    the body force are interpolated (bilinear) onto the n-grid's nodes, the network predicts there, and the predicted
     Vx, Vy, Pt are interpolated back onto the n2-grid's staggered positions (miniapps/Stokes2D.jl:
     nodes at (i-1)dx, Vx between them at (i-3/2)dx, Vy at (j-3/2)dy, dx = lx/(n-1))."""
     from scipy.ndimage import map_coordinates
@@ -372,19 +375,34 @@ def resampled_predictor(ws, n, n2, lx=10.0):
     n_nodes, n_faces = axes(n)
     m_nodes, m_faces = axes(n2)
 
-    def predict(Mus2):
+    def predict(Mus2, Rog2):
         logm = resample(np.log10(Mus2), (m_nodes, m_nodes), (n_nodes, n_nodes))
-        Vx, Vy, Pt = ws.predict(10.0 ** logm)
+        rog = resample(Rog2, (m_nodes, m_nodes), (n_nodes, n_nodes))
+        Vx, Vy, Pt = ws.predict(10.0 ** logm, rog)
         return (resample(Vx, (n_faces, n_nodes), (m_faces, m_nodes)), resample(Vy, (n_nodes, n_faces), (m_nodes, m_faces)),
                 resample(Pt, (n_nodes, n_nodes), (m_nodes, m_nodes)))
     return predict
 
 
+def _pred_err(pred, sol):
+    """Relative L2 error of predicted (Vx, Vy, Pt) against a solution (pressures mean-free)."""
+    out = {}
+    for name, p, r in zip(("Vx", "Vy", "P"), pred, sol):
+        p, r = np.asarray(p, np.float64), np.asarray(r, np.float64)
+        if name == "P":
+            p, r = p - p.mean(), r - r.mean()
+        den = float(np.sqrt(np.sum(r * r)))
+        out[name] = float(np.sqrt(np.sum((p - r) ** 2)) / den) if den > 0 and np.all(np.isfinite(p)) else float("inf")
+    return out
+
+
 def evaluate(ws, solver, seeds, log10_contrast=1.0, predict=None):
     """Per held-out seed: (cold iterations, cold seconds, warm iterations, warm seconds including
-    inference and the host/device copies of the prediction, final residuals); with a solver that
-    records its residual history, the cold and warm histories (one residual per check). `predict`
-    maps the viscosity to (Vx, Vy, Pt) (default: the network at the solver's resolution)."""
+    inference and the host/device copies of the prediction, final residuals, relative L2 error of
+    the prediction against the converged cold solution per field, pressure mean-free); with a
+    solver that records its residual history, the cold and warm histories (one residual per
+    check). `predict` maps (viscosity, body force) to (Vx, Vy, Pt) (default: the network at the
+    solver's resolution)."""
     predict = predict or ws.predict
     import time
     from .workloads import stokes2d_inclusions
@@ -398,10 +416,11 @@ def evaluate(ws, solver, seeds, log10_contrast=1.0, predict=None):
         itc, errc = solver.solve()
         t1 = time.perf_counter()
         hist_cold = list(getattr(solver, "history", None) or [])
+        sol = (solver.Vx.to_host(), solver.Vy.to_host(), solver.Pt.to_host())
         solver.reset()
         core.synchronize()
         t2 = time.perf_counter()
-        Vx, Vy, Pt = predict(Mus)
+        Vx, Vy, Pt = predict(Mus, Rog)
         fallback = not (np.all(np.isfinite(Vx)) and np.all(np.isfinite(Vy)) and np.all(np.isfinite(Pt)))
         itw, errw, hist_warm = 0, float("inf"), []
         if not fallback:  # the warm-started attempt gets at most twice the plain iteration count
@@ -420,7 +439,7 @@ def evaluate(ws, solver, seeds, log10_contrast=1.0, predict=None):
             itw += it2
         t3 = time.perf_counter()
         rec = dict(seed=s, cold_iters=itc, cold_s=t1 - t0, warm_iters=itw, warm_s=t3 - t2, err_cold=errc, err_warm=errw,
-                   fallback=fallback)
+                   fallback=fallback, pred_err=_pred_err((Vx, Vy, Pt), sol))
         if hist_cold:
             rec["hist_cold"] = [float(f"{e:.4e}") for e in hist_cold]
             rec["hist_warm"] = [float(f"{e:.4e}") for e in hist_warm]

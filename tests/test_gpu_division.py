@@ -72,3 +72,56 @@ def test_scalar_divisor_f32(b):
     with np.errstate(all="ignore"):
         ref = (a / np.float32(b)) + (a / np.float32(0.7))
     assert _same(R.to_host(), ref.astype(np.float32), np.uint32)
+
+
+HOIST = """@parallel function hoist!(R, A, a::Data.Number, b::Data.Number, n::Int)
+    @all(R) = @all(A)*(a/b) + (n*a - b^2)/(3*pi) + (-a)^-2 + a^3/n - @all(A)/(2*b) + @all(A)*(a*b^-1)
+    return
+end"""
+
+
+def _hoist_ref(x, a, b, n, t):
+    """The statement of HOIST evaluated left to right in type t with numpy's IEEE scalars."""
+    a, b, n3, pi = t(a), t(b), t(n), t(np.pi)
+    ab = a / b
+    c = (n3 * a - b * b) / (t(3) * pi)
+    d = (t(1) / -a) * (t(1) / -a)
+    e = (a * a * a) / n3
+    f = x / (t(2) * b)
+    g = a * (t(1) / b)
+    with np.errstate(all="ignore"):
+        return ((((x * ab + c) + d) + e) - f) + x * g
+
+
+@pytest.mark.parametrize("mod,t,ut", [(__name__, np.float64, np.uint64), (MOD32, np.float32, np.uint32)])
+@pytest.mark.parametrize("a,b,n", [(0.3, 7.1, 5), (-2.5, 1e-3, -3), (1e-20, 3e15, 1)])
+def test_hoisted_scalar_subexpressions(mod, t, ut, a, b, n):
+    # a/b, (n*a - b^2)/(3*pi), (-a)^-2, a^3/n, 2*b and a*b^-1 are evaluated once by the host; every
+    # cell must still see the value its own IEEE evaluation would give
+    k = ps.core._define("parallel", HOIST, {}, mod)
+    x = _wide((29, 6, 5), np.random.default_rng(5), t)
+    A = ps.zeros(*x.shape, eltype=ps.Float64 if t is np.float64 else ps.Float32)
+    A.copy_from(x)
+    R = ps.zeros(*x.shape, eltype=ps.Float64 if t is np.float64 else ps.Float32)
+    k(R, A, a, b, n)
+    ref = _hoist_ref(x, a, b, n, t).astype(t)
+    assert _same(R.to_host(), ref, ut)
+
+
+def test_bf16_stores_round_to_nearest_even():
+    # FP32 -> BFloat16 stores: nearest even at every magnitude (ties, subnormals, the overflow
+    # threshold) and every NaN to 0x7fc0, as BFloat16(::Float32) of BFloat16s.jl
+    special = [0x3F808000, 0x3F818000, 0x3F808001, 0x3F807FFF, 0x00000001, 0x00008000, 0x00018000, 0x007FFFFF, 0x00007FFF,
+               0x7F7FFFFF, 0x7F7F7FFF, 0x7F7F8000, 0x7F800000, 0xFF800000, 0x7FC00000, 0x7F800001, 0xFFC00001, 0x7FFFFFFF,
+               0xBF808000, 0xBF818000, 0x80008000, 0x80018000, 0x00000000, 0x80000000]
+    rng = np.random.default_rng(17)
+    u = rng.integers(0, 2 ** 32, size=(97, 31, 13), dtype=np.uint64).astype(np.uint32)
+    u.reshape(-1)[:len(special)] = special
+    a = np.asfortranarray(u.view(np.float32))
+    k = ps.core._define("parallel", "@parallel function cvt!(B, A)\n @all(B) = @all(A)\n return\nend", {}, MOD32)
+    A = ps.zeros(*a.shape, eltype=ps.Float32)
+    A.copy_from(a)
+    B = ps.zeros(*a.shape, eltype=ps.BFloat16)
+    k(B, A)
+    got = np.asarray(B.to_host()).view(np.uint16)
+    assert np.array_equal(got, ps.f32_to_bf16(a))

@@ -38,7 +38,7 @@ def test_warm_start_gains_and_converges():
         solver.reset()
         solver.solve()
         ref = [solver.Vx.to_host(), solver.Vy.to_host()]
-        pred = ws.predict(Mus)[:2]
+        pred = ws.predict(Mus, Rog)[:2]
         err = np.sqrt(sum(np.sum((p - r) ** 2) for p, r in zip(pred, ref)) / sum(np.sum(r ** 2) for r in ref))
         gains.append(err)
         print(f"seed {s}: velocity error of the prediction {err:.3f} (start from rest: 1.0); "

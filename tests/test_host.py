@@ -431,3 +431,17 @@ def test_launch_refuses_argspecs_and_freed_arrays():
         view.f.dims[d] = 4
     with pytest.raises(ps.ArgumentError, match="freed"):
         k(view, view)
+
+
+def test_uniform_subexpressions_become_host_arguments():
+    """dt/ρ of the acoustic velocity update depends on scalar arguments only: the generated kernel
+    receives it as one hidden argument, and ρ is no longer a device divisor. Under fastmath the
+    device keeps its own form."""
+    from parallelstencil import workloads as W
+    kV, _ = W.define_acoustic3d(__name__, False)
+    n = 12
+    F = [ps.ArgSpec((n + 1, n, n)), ps.ArgSpec((n, n + 1, n)), ps.ArgSpec((n, n, n + 1)), ps.ArgSpec((n, n, n))]
+    src = kV.source(*F, 0.1, 2.0, 0.3, 0.3, 0.3)
+    assert "double ps_u0" in src and "ps_u1" not in src and "ps_u0 * " in src and "a5_r" not in src
+    kf = ps.parallel(W.ACOUSTIC3D_V, fastmath=True)
+    assert "ps_u0" not in kf.source(*F, 0.1, 2.0, 0.3, 0.3, 0.3)
