@@ -170,7 +170,10 @@ def stokes_warm_start(args):
     log(f"training: {args.epochs} epochs in {t2 - t1:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
     evals = {}
     sizes = []
-    for m, ne in ((n, args.eval), (args.stokes_n2, args.eval2)):
+    todo = [(n, args.eval), (args.stokes_n2, args.eval2)]
+    if args.eval_train and nt not in (n, args.stokes_n2):  # the training resolution itself
+        todo.insert(0, (nt, args.eval_train))
+    for m, ne in todo:
         if not m or not ne:
             continue
         sv = tsolver if m == nt else Stokes2DFused(mod, m, m, eps=1e-8, nout=200, eta_ts=args.stokes_ts, history=True)
@@ -786,6 +789,7 @@ def main():
     ap.add_argument("--train", type=int, default=256, help="stokes: training samples")
     ap.add_argument("--eval", type=int, default=24, help="stokes: held-out samples at --stokes-n")
     ap.add_argument("--eval2", type=int, default=4, help="stokes: held-out samples at --stokes-n2")
+    ap.add_argument("--eval-train", type=int, default=0, help="stokes: held-out samples at --stokes-train-n as well")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--shmem-n", type=int, default=16384)
     ap.add_argument("--stokes3d-n", type=int, default=127)

@@ -311,6 +311,10 @@ static void read_options(Options& o, const JVal& j) {
     else if (k == "prefetch") {
       o.prefetch = (int)v.n;
       if (o.prefetch < 1 || o.prefetch > 4) throw PsError(PS_EKEYWORD, "prefetch must be 1, 2, 3 or 4");
+    } else if (k == "cells_x") {
+      o.cells_x = (int)v.n;
+      if (!(o.cells_x == 0 || o.cells_x == 1 || o.cells_x == 2 || o.cells_x == 4 || o.cells_x == 8))
+        throw PsError(PS_EKEYWORD, "cells_x must be 0 (automatic), 1, 2, 4 or 8");
     } else if (k == "lds") o.lds = v.b;
     else if (k == "xcd_remap") o.xcd_remap = v.b;
     else if (k == "nt_store") o.nt_store = v.b ? 1 : 0;
@@ -1394,7 +1398,7 @@ std::string kernel_info_json(const Kernel& K) {
 
 std::string Variant::key() const {
   char b[96];
-  std::snprintf(b, sizeof b, "m%dc%di%da%dt%dl%d_%dx%dn%do%du%d", memopt, checked, idx64, noalias, maxthreads, lds, bx, by, nt, off32, u24);
+  std::snprintf(b, sizeof b, "m%dc%di%da%dt%dl%d_%dx%dn%do%du%dg%d", memopt, checked, idx64, noalias, maxthreads, lds, bx, by, nt, off32, u24, cx);
   return std::string(b) + "d" + divmode + (padbits.empty() ? "" : "p" + padbits) + (sclass.empty() ? "" : "s" + sclass);
 }
 

@@ -66,6 +66,8 @@ struct Options {
   // by other values through a refined hardware reciprocal (within a few ulp per operation)
   bool fastmath = false;
   int prefetch = 1;                       // memopt: loads are issued this many steps ahead
+  // plain kernels: adjacent x cells per thread (x groups); 0: automatic (by element size), 1: off
+  int cells_x = 0;
   bool lds = true;                        // memopt: stage x-y neighbours of optvars through LDS
   bool xcd_remap = false;                 // XCD-aware block ordering for memopt kernels
   // nontemporal stores for array writes: 1 on, 0 off, -1 automatic (on for the 256-wide
@@ -121,6 +123,8 @@ struct Variant {
   // plain kernels: per argument, a letter shared by fields with equal strides (one cell offset
   // serves them all), '-' for scalars; empty: every field computes its own
   std::string sclass;
+  // plain kernels: adjacent x cells per thread (1: one cell per thread)
+  int cx = 1;
   int bx = 0, by = 0;     // block shape baked into LDS variants
   // per argument: 'p' / 'n' a positive / negative float scalar accepted as reciprocal divisor,
   // 'x' a float scalar that must be divided by exactly, '-' anything else
@@ -184,6 +188,10 @@ NodeP canonical_indices(const NodeP& n);
 // updates); it raises an ArgumentError where it does not apply.
 Kernel derivative_kernel(const Kernel& k, const JVal& spec);
 Emitted lower_kernel(const Kernel& k, const std::vector<ArgKind>& kinds, const Variant& v);
+// Whether a plain kernel's body can run as x groups (Variant::cx > 1): a sequence of array
+// assignments, each optionally under an if without else whose condition bounds the x index by
+// an interval; no local variables, loops, kernel language or atomic updates.
+bool groupable_body(const Kernel& k);
 std::string kernel_info_json(const Kernel& k);
 
 // Bounds proof: true if every array access of the kernel stays inside its

@@ -1231,6 +1231,8 @@ static Variant variant_from_flags(const Kernel& K, int flags) {
   v.nt = K.opt.nt_store == 1 || (K.opt.nt_store == -1 && v.lds && v.bx >= 256);
   v.off32 = (flags & 64) != 0 && !v.memopt && !v.checked;  // 32-bit per-thread byte offsets
   v.u24 = v.off32 && (flags & 128) != 0;                      // 24-bit cell-offset multiplies
+  if (v.off32 && !v.lds && ((flags >> 8) & 0xf) > 1 && groupable_body(K))  // x groups: cells per thread in bits 8..11
+    v.cx = (flags >> 8) & 0xf;
   return v;
 }
 
@@ -1269,7 +1271,8 @@ struct Plan {
   bool memopt = false;
   int64_t ranges[3][2];
   int32_t nthreads[3];
-  int32_t nblocks[3];
+  int32_t nblocks[3];  // the reference's geometry (what geometry() reports)
+  int32_t grid_x = 0;  // x groups (v.cx > 1): the blocks launched along x
   Variant v;
 };
 
@@ -1503,6 +1506,24 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
       sc[i] = (char)('a' + (c % 26));
     }
     if (seen.size() <= 26) v.sclass = sc;
+    // x groups (Variant::cx): with automatic geometry and a body of array assignments under
+    // x-interval masks, each thread takes cx adjacent x cells (by default 8 bytes of the narrowest
+    // array per thread: FP32 2 cells, BFloat16 4), so that their accesses merge into wide loads
+    bool cell = false;
+    int64_t es = 8;
+    for (int i = 0; i < nargs; ++i)
+      if (args[i].kind == PS_ARG_FIELD) {
+        cell = cell || args[i].field.ncomp > 0;
+        es = std::min<int64_t>(es, elsize(args[i].field.dtype));
+      }
+    if (ok && !user_geom && !cell && groupable_body(K)) {
+      int cx = K.opt.cells_x;
+      if (cx == 0) cx = es >= 8 ? 1 : (int)(8 / es);
+      if (cx > 1 && len[0] >= 2 * (int64_t)cx) {
+        v.cx = cx;
+        P.grid_x = (int32_t)(((len[0] + cx - 1) / cx + P.nthreads[0] - 1) / P.nthreads[0]);
+      }
+    }
   }
   P.v = v;
   return P;
@@ -1757,7 +1778,8 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
                                     " bytes of LDS a workgroup can use");
   }
   if (P.v.checked) g_err_armed.store(true, std::memory_order_relaxed);
-  hipError_t e = hipModuleLaunchKernel(m->fn, (unsigned)P.nblocks[0], (unsigned)P.nblocks[1], (unsigned)P.nblocks[2], (unsigned)P.nthreads[0],
+  const unsigned gx = (unsigned)(P.v.cx > 1 ? P.grid_x : P.nblocks[0]);
+  hipError_t e = hipModuleLaunchKernel(m->fn, gx, (unsigned)P.nblocks[1], (unsigned)P.nblocks[2], (unsigned)P.nthreads[0],
                                        (unsigned)P.nthreads[1], (unsigned)P.nthreads[2], shmem, st, ptrs.data(), nullptr);
   if (e != hipSuccess) return fail(PS_EHIP, std::string("hipModuleLaunchKernel: ") + hipGetErrorString(e));
   if (!(launch && launch->async)) {

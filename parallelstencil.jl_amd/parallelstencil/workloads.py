@@ -1065,13 +1065,12 @@ def define_acoustic2d(module, memopt=False, **kw):
     return core._define("parallel", ACOUSTIC2D_V, kw, module), core._define("parallel", ACOUSTIC2D_P, kw, module)
 
 
-def acoustic3d_run(module, P0, prm, steps, eltype, memopt=False):
+def acoustic3d_run(module, P0, prm, steps, eltype, memopt=False, **kw):
     """Runs `steps` acoustic time steps (V then P, miniapps/acoustic3D.jl:48-52) on the GPU from
     P0 (float64 host array) in storage type `eltype` and returns host copies (P, Vx, Vy, Vz) as
-    float64 (BFloat16 widened)."""
-    import numpy as np
+    float64 (BFloat16 widened); `kw` are kernel keywords (fastmath, cells_x, ...)."""
     n1, n2, n3 = P0.shape
-    kV, kP = define_acoustic3d(module, memopt)
+    kV, kP = define_acoustic3d(module, memopt, **kw)
     P = core.zeros(n1, n2, n3, eltype=eltype)
     Vx, Vy, Vz = (core.zeros(n1 + 1, n2, n3, eltype=eltype), core.zeros(n1, n2 + 1, n3, eltype=eltype),
                   core.zeros(n1, n2, n3 + 1, eltype=eltype))

@@ -1,0 +1,74 @@
+"""Plain kernels whose threads take several adjacent x cells (x groups): bit for bit the same
+fields as one cell per thread, for FP64, FP32 and BFloat16 arrays, x extents that the group size
+does not divide, staggered arrays, masks of @all and @inn, neighbours in x, y and z and uniform
+divisions with their IEEE fallback."""
+import numpy as np
+import pytest
+
+import parallelstencil as ps
+from parallelstencil import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+M64, M32 = __name__ + ".f64", __name__ + ".f32"
+ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3, module=M64)
+ps.init_parallel_stencil(ps.AMDGPU, ps.Float32, 3, module=M32)
+
+# race-free: no statement reads, at another cell, an array that the kernel writes
+STENCIL = """
+@parallel function mix!(A, B, C, D, s::Data.Number, h::Data.Number)
+    @inn(A) = @inn(A) + s/h*(@d2_xi(B) + @d2_yi(B) + @d2_zi(B)) - @inn(C)/h
+    @all(D) = @all(C)*s + @all(B)/h
+    return
+end
+"""
+
+
+def _fields(shape, elt, rng):
+    t = np.float64 if elt is ps.Float64 else np.float32
+    out = []
+    for k in range(4):
+        a = np.asfortranarray(rng.standard_normal(shape).astype(t))
+        if k == 0:
+            a.reshape(-1)[::97] = 0.0  # zero quotients take the fast path, as they must
+        if k == 2:  # quotients below the exact range: the group's IEEE recomputation runs
+            a.reshape(-1)[5::211] = 1e-200 if t is np.float64 else 1e-20
+        d = ps.zeros(*shape, eltype=elt)
+        d.copy_from(ps.f32_to_bf16(a) if elt is ps.BFloat16 else a)
+        out.append((a, d))
+    return out
+
+
+@pytest.mark.parametrize("elt,mod", [(ps.Float64, M64), (ps.Float32, M32), (ps.BFloat16, M32)])
+@pytest.mark.parametrize("shape", [(130, 7, 6), (67, 5, 9), (256, 3, 4)])
+@pytest.mark.parametrize("cx", [2, 4, 8])
+def test_x_groups_match_one_cell_per_thread(elt, mod, shape, cx):
+    rng = np.random.default_rng(sum(shape) + cx)
+    ref = ps.core._define("parallel", STENCIL, {"cells_x": 1}, mod)
+    grp = ps.core._define("parallel", STENCIL, {"cells_x": cx}, mod)
+    f1 = _fields(shape, elt, rng)
+    f2 = [(a, ps.zeros(*shape, eltype=elt)) for a, _ in f1]
+    for (_, d1), (_, d2) in zip(f1, f2):
+        d2.copy_from(d1.to_host())
+    for it in range(3):
+        ref(*[d for _, d in f1], 0.37, 1e-3 if it == 1 else 0.9)
+        grp(*[d for _, d in f2], 0.37, 1e-3 if it == 1 else 0.9)
+    for (_, d1), (_, d2) in zip(f1, f2):
+        a, b = np.ascontiguousarray(d1.to_host()), np.ascontiguousarray(d2.to_host())
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (elt, shape, cx)
+
+
+def test_x_groups_acoustic3d_bf16_against_oracle_run():
+    # the acoustic kernels pick x groups by themselves for FP32 (2 cells) and BFloat16 (4 cells);
+    # the reported geometry stays the reference's
+    n, steps = 45, 20
+    P0, prm = W.acoustic3d_inputs(n)
+    r16 = W.acoustic3d_run(M32, P0, prm, steps, ps.BFloat16)
+    r1 = W.acoustic3d_run(M32, P0, prm, steps, ps.BFloat16, cells_x=1)
+    for a, b in zip(r16, r1):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    F = [ps.ArgSpec((n + 1, n, n), ps.BFloat16), ps.ArgSpec((n, n + 1, n), ps.BFloat16), ps.ArgSpec((n, n, n + 1), ps.BFloat16),
+         ps.ArgSpec((n, n, n), ps.BFloat16)]
+    g = W.define_acoustic3d(M32, False)[0].geometry(*F, 0.1, 1.0, 1.0, 1.0, 1.0)
+    g1 = W.define_acoustic3d(M32, False, cells_x=1)[0].geometry(*F, 0.1, 1.0, 1.0, 1.0, 1.0)
+    assert g == g1, (g, g1)

@@ -445,3 +445,22 @@ def test_uniform_subexpressions_become_host_arguments():
     assert "double ps_u0" in src and "ps_u1" not in src and "ps_u0 * " in src and "a5_r" not in src
     kf = ps.parallel(W.ACOUSTIC3D_V, fastmath=True)
     assert "ps_u0" not in kf.source(*F, 0.1, 2.0, 0.3, 0.3, 0.3)
+
+
+def test_x_group_lowering():
+    """Plain kernels made of masked array assignments run as x groups: a fast path for groups whose
+    masks all hold, with the cells' accesses one row base apart by constant offsets, and a cell by
+    cell path for the rest. Kernels with local variables keep one cell per thread."""
+    from parallelstencil import workloads as W
+    kV, _ = W.define_acoustic3d(__name__, False)
+    n = 20
+    F = [ps.ArgSpec((n + 1, n, n)), ps.ArgSpec((n, n + 1, n)), ps.ArgSpec((n, n, n + 1)), ps.ArgSpec((n, n, n))]
+    src = kV.source(*F, 0.1, 2.0, 0.3, 0.3, 0.3, off32=True, cells_x=4)
+    assert "if (ps_grp) {" in src and "const ps_int ps_cx = 3;" in src and "(int)ps_cx + 1)" in src
+    # one cell per thread: x offsets in the uniform part (no merged loads that overlap between lanes)
+    assert "ps_at(a3, (ps_int)1 + (1)*a3_sx + (1)*a3_sxy, a3_vo, 0)" in kV.source(*F, 0.1, 2.0, 0.3, 0.3, 0.3, off32=True)
+    assert "ps_grp" not in kV.source(*F, 0.1, 2.0, 0.3, 0.3, 0.3, off32=True)
+    k = ps.parallel_indices("@parallel_indices (ix,iy,iz) function loc!(A, B)\n t = B[ix,iy,iz]*2\n A[ix,iy,iz] = t\n return\nend")
+    assert "ps_grp" not in k.source(ps.ArgSpec((n, n, n)), ps.ArgSpec((n, n, n)), off32=True, cells_x=4)
+    k = ps.parallel_indices("@parallel_indices (ix,iy,iz) function cp!(A, B)\n A[ix,iy,iz] = B[ix,iy,iz]*2\n return\nend")
+    assert "if (ps_grp) {" in k.source(ps.ArgSpec((n, n, n)), ps.ArgSpec((n, n, n)), off32=True, cells_x=2)
