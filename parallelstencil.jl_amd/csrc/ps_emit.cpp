@@ -147,6 +147,29 @@ PS_DI float ps_divp(float a, float b, float r, bool& bad) {
   bad |= !(aq <= 0x1p50f) | ((aq < 0x1p-50f) & (aq != 0.0f));
   return q;
 }
+// The same quotient for statements whose value is NaN whenever one of their quotients is NaN
+// (NaN-transparent statements): instead of testing every quotient, the frexp exponent of the
+// dividend joins the statement's running minimum and maximum. A dividend with |a| in
+// [2^-600, 2^600) (double; [2^-80, 2^80) float) or zero keeps every intermediate of the
+// division normal for the accepted divisors, so the quotient is exact; an infinite or NaN
+// dividend gives a NaN quotient, which reaches the statement's value. The statement falls back
+// to IEEE division when an exponent left [-599, 600] ([-79, 80]) or its value is NaN.
+PS_DI double ps_divx(double a, double b, double r, int& lo, int& hi) {
+  const double q0 = a * r;
+  const double e = __builtin_fma(q0, b, -a);
+  const int x = __builtin_amdgcn_frexp_exp(a);
+  lo = x < lo ? x : lo;
+  hi = x > hi ? x : hi;
+  return __builtin_fma(-e, r, q0);
+}
+PS_DI float ps_divx(float a, float b, float r, int& lo, int& hi) {
+  const float q0 = a * r;
+  const float e = __builtin_fmaf(q0, b, -a);
+  const int x = __builtin_amdgcn_frexp_expf(a);
+  lo = x < lo ? x : lo;
+  hi = x > hi ? x : hi;
+  return __builtin_fmaf(-e, r, q0);
+}
 // fastmath reciprocal: hardware estimate refined by Newton steps r += r (1 - b r) (two for
 // double, one for float); the raw estimate is kept where the refinement is not finite
 // (b = 0, b = inf, NaN)
@@ -250,6 +273,13 @@ class Emitter {
   std::map<int, Ty> rcp_params;
   bool fast_div = false;   // emitting the reciprocal form of a statement
   bool used_fast = false;  // the current statement used it
+  // the current statement is NaN-transparent: its quotients use ps_divx, whose checks (per type:
+  // [0] double, [1] float) are made once on the statement's value
+  bool div_x = false;
+  bool used_x[2] = {false, false};
+  bool nan_transparent(const NodeP& n) const;
+  std::string fast_decls() const;
+  std::string fast_bad(const std::string& v) const;
   // scalar subexpressions the host evaluates once per launch: canonical text -> program index
   std::map<std::string, int> uni_index;
   std::vector<Emitted::UProg> uprogs;
@@ -649,8 +679,14 @@ EV Emitter::divide(const NodeP& n) {
         rcp_params[k] = t;
         used_fast = true;
         std::string A = an(k);
-        if (mode == 'p') return EV{"ps_divp(" + cast(a, t).c + ", " + A + ", " + A + "_r, ps_bad)", t};
-        return EV{"ps_divp(-(" + cast(a, t).c + "), -" + A + ", " + A + "_r, ps_bad)", t};
+        std::string tail = ", ps_bad)";
+        if (div_x) {
+          used_x[t == Ty::F64 ? 0 : 1] = true;
+          tail = t == Ty::F64 ? ", ps_lo, ps_hi)" : ", ps_lo32, ps_hi32)";
+        }
+        const std::string fn = div_x ? "ps_divx(" : "ps_divp(";
+        if (mode == 'p') return EV{fn + cast(a, t).c + ", " + A + ", " + A + "_r" + tail, t};
+        return EV{fn + "-(" + cast(a, t).c + "), -" + A + ", " + A + "_r" + tail, t};
       }
     }
   }
@@ -660,6 +696,10 @@ EV Emitter::divide(const NodeP& n) {
     if (ok && v > 0) {
       used_fast = true;
       double r = t == Ty::F32 ? (double)(1.0f / (float)v) : 1.0 / v;
+      if (div_x) {
+        used_x[t == Ty::F64 ? 0 : 1] = true;
+        return EV{"ps_divx(" + cast(a, t).c + ", " + hexlit(v, t) + ", " + hexlit(r, t) + (t == Ty::F64 ? ", ps_lo, ps_hi)" : ", ps_lo32, ps_hi32)"), t};
+      }
       return EV{"ps_divp(" + cast(a, t).c + ", " + hexlit(v, t) + ", " + hexlit(r, t) + ", ps_bad)", t};
     }
   }
@@ -1177,7 +1217,7 @@ static void read_arrays(const NodeP& n, std::set<std::string>& out) {
 // precedes its stores, and a statement that reads what an earlier one writes sees it.
 void Emitter::group_statements(const std::vector<NodeP>& sts) {
   struct G {
-    std::string g, ref;
+    std::string g, ref, decl, badv, bad;
     EV f, e;
     bool uf;
     int dt;
@@ -1209,10 +1249,16 @@ void Emitter::group_statements(const std::vector<NodeP>& sts) {
       G x;
       fast_div = true;
       used_fast = false;
+      div_x = nan_transparent(rhs);
+      used_x[0] = used_x[1] = false;
       x.f = expr(rhs);
       fast_div = false;
       x.uf = used_fast;
       x.e = x.uf ? expr(rhs) : x.f;
+      x.decl = fast_decls();
+      x.badv = "ps_v";  // the cell's value, named in its scope
+      x.bad = fast_bad(x.badv);
+      div_x = false;
       auto idx = norm_indices(lhs, ai);
       std::string ic[3], ref32;
       const std::string off = offset(ai, idx, ic, nullptr, &ref32);
@@ -1228,8 +1274,11 @@ void Emitter::group_statements(const std::vector<NodeP>& sts) {
     for (auto& x : batch)
       for (int c = 0; c < CX; ++c) {
         const std::string vc = "ps_g" + x.g + "_" + std::to_string(c);
-        o << pad() << "{ " << group_scope(c) << (x.uf ? "bool ps_bad = false; " : "") << vc << " = " << x.f.c << ";"
-          << (x.uf ? " ps_g" + x.g + "_bad |= ps_bad;" : "") << " }\n";
+        if (x.uf)
+          o << pad() << "{ " << group_scope(c) << x.decl << cty(x.f.t) << " ps_v = " << x.f.c << "; " << vc << " = ps_v; ps_g" << x.g
+            << "_bad |= " << x.bad << "; }\n";
+        else
+          o << pad() << "{ " << group_scope(c) << vc << " = " << x.f.c << "; }\n";
       }
     for (auto& x : batch) {
       if (!x.uf) continue;
@@ -1251,25 +1300,92 @@ void Emitter::group_statements(const std::vector<NodeP>& sts) {
   }
 }
 
+// Whether a NaN anywhere in the expression reaches its value: arithmetic, literal powers other
+// than 0, NaN-propagating max/min and elementary functions of non-Bool operands; not comparisons,
+// conditionals, Bool operands (false * NaN is 0), conversions or other calls.
+bool Emitter::nan_transparent(const NodeP& n) const {
+  static const std::set<std::string> fns = {"max", "min", "abs", "sqrt", "exp", "log", "log10", "log2", "exp2", "expm1", "log1p",
+                                            "sin", "cos", "tan", "sinh", "cosh", "tanh", "asin", "acos", "atan"};
+  switch (n->k) {
+    case NK::Num: return true;
+    case NK::Ident: {
+      if (n->s == "true" || n->s == "false") return false;
+      auto lt = locals.find(n->s);
+      if (lt != locals.end()) return lt->second != Ty::Bool;
+      auto it = K.param_index.find(n->s);
+      if (it != K.param_index.end()) return kinds[it->second].is_field || kinds[it->second].dtype != PS_BOOL;
+      return true;
+    }
+    case NK::Index: {  // a load of a non-Bool array
+      if (n->a[0]->k != NK::Ident) return false;
+      const int ai = field_index(n->a[0]->s);
+      return ai >= 0 && kinds[ai].dtype != PS_BOOL;
+    }
+    case NK::Unary: return (n->s == "-" || n->s == "+") && nan_transparent(n->a[0]);
+    case NK::Binary:
+      if (n->s == "^") {
+        const NodeP& e = n->a[1];
+        const bool lit = (e->k == NK::Num && e->is_int && e->ival != 0) ||
+                         (e->k == NK::Unary && e->s == "-" && e->a[0]->k == NK::Num && e->a[0]->is_int && e->a[0]->ival != 0);
+        return lit && nan_transparent(n->a[0]);
+      }
+      if (n->s != "+" && n->s != "-" && n->s != "*" && n->s != "/") return false;
+      return nan_transparent(n->a[0]) && nan_transparent(n->a[1]);
+    case NK::Call:
+      if (!fns.count(n->s)) return false;
+      for (auto& c : n->a)
+        if (!nan_transparent(c)) return false;
+      return true;
+    default: return false;
+  }
+}
+
+// Declarations and the fallback test of the fast form of a statement (ps_bad, or the exponent
+// ranges and the NaN test of ps_divx).
+std::string Emitter::fast_decls() const {
+  std::string d;
+  if (div_x) {
+    if (used_x[0]) d += "int ps_lo = 0, ps_hi = 0; ";
+    if (used_x[1]) d += "int ps_lo32 = 0, ps_hi32 = 0; ";
+  } else {
+    d = "bool ps_bad = false; ";
+  }
+  return d;
+}
+std::string Emitter::fast_bad(const std::string& v) const {
+  if (!div_x) return "ps_bad";
+  std::string c = "(" + v + " != " + v + ")";
+  if (used_x[0]) c += " | (ps_lo < -599) | (ps_hi > 600)";
+  if (used_x[1]) c += " | (ps_lo32 < -79) | (ps_hi32 > 80)";
+  return "(" + c + ")";
+}
+
 // Value of a right-hand side. When it divides by uniform divisors, the fast (reciprocal) form
-// is evaluated first and the IEEE form only if a quotient left the exact range; the emitted
-// code then opens a block that the caller closes after using the value.
+// is evaluated first and the IEEE form only if a quotient may have left the exact range; the
+// emitted code then opens a block that the caller closes after using the value.
 EV Emitter::rhs_value(const NodeP& rhs, bool* opened) {
   fast_div = true;
   used_fast = false;
+  div_x = nan_transparent(rhs);
+  used_x[0] = used_x[1] = false;
   EV f = expr(rhs);
   fast_div = false;
   *opened = false;
-  if (!used_fast) return f;
+  if (!used_fast) {
+    div_x = false;
+    return f;
+  }
   EV e = expr(rhs);
   std::string var = "ps_v" + std::to_string(++tmp_counter);
   o << pad() << "{\n";
   ++ind;
-  o << pad() << "bool ps_bad = false;\n";
+  const std::string decl = fast_decls();
+  o << pad() << decl << "\n";
   o << pad() << cty(f.t) << " " << var << " = " << f.c << ";\n";
   // the empty volatile asm keeps the rarely taken IEEE form behind its branch; without it the
   // compiler may evaluate that form in every cell and select
-  o << pad() << "if (ps_bad) { __asm__ volatile(\"\"); " << var << " = " << cast(e, f.t).c << "; }\n";
+  o << pad() << "if (" << fast_bad(var) << ") { __asm__ volatile(\"\"); " << var << " = " << cast(e, f.t).c << "; }\n";
+  div_x = false;
   *opened = true;
   return EV{var, f.t};
 }

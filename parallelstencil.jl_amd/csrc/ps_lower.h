@@ -68,6 +68,8 @@ struct Options {
   int prefetch = 1;                       // memopt: loads are issued this many steps ahead
   // plain kernels: adjacent x cells per thread (x groups); 0: automatic (by element size), 1: off
   int cells_x = 0;
+  // plain kernels: 32-bit per-thread byte offsets on scalar bases where they fit (Variant::off32)
+  bool addr32 = true;
   bool lds = true;                        // memopt: stage x-y neighbours of optvars through LDS
   bool xcd_remap = false;                 // XCD-aware block ordering for memopt kernels
   // nontemporal stores for array writes: 1 on, 0 off, -1 automatic (on for the 256-wide

@@ -1478,8 +1478,11 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   std::string why;
   v.checked = !K.opt.inbounds && !prove_in_bounds(K, sh, memopt, &why);
   // 32-bit per-thread byte offsets (Variant::off32) for plain kernels whose launch box starts at
-  // index 1 or above in every dimension and whose fields' byte offsets over the box stay < 2^31
-  if (!memopt && !v.checked && !pad) {
+  // index 1 or above in every dimension and whose fields' byte offsets over the box stay < 2^31.
+  // They are used by x groups only: with one cell per thread the scalar bases add scalar
+  // instructions to every wavefront, and in kernels that also check exact quotients (scalar
+  // masks) that made the 512^3 diffusion step 25% slower (1.078 against 0.862 ms).
+  if (!memopt && !v.checked && !pad && K.opt.addr32) {
     bool ok = sh.lo[0] >= 1 && sh.lo[1] >= 1 && sh.lo[2] >= 1;
     for (int i = 0; i < nargs && ok; ++i) {
       if (args[i].kind != PS_ARG_FIELD) continue;
@@ -1487,7 +1490,6 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
       const int64_t top = (sh.hi[0] - 1) + (sh.hi[1] - 1) * f.pitch + (sh.hi[2] - 1) * f.plane;
       if (extent_bytes(f) >= ((int64_t)1 << 31) || top < 0 || top * (int64_t)elsize(f.dtype) >= ((int64_t)1 << 31)) ok = false;
     }
-    v.off32 = ok;
     // 24-bit multiplies for the cell offsets: every stride and every index of the box below 2^24
     // (the products then stay below 2^31 by the test above)
     bool u24 = ok && sh.hi[0] <= (1 << 24) && sh.hi[1] <= (1 << 24) && sh.hi[2] <= (1 << 24);
@@ -1521,6 +1523,7 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
       if (cx == 0) cx = es >= 8 ? 1 : (int)(8 / es);
       if (cx > 1 && len[0] >= 2 * (int64_t)cx) {
         v.cx = cx;
+        v.off32 = true;
         P.grid_x = (int32_t)(((len[0] + cx - 1) / cx + P.nthreads[0] - 1) / P.nthreads[0]);
       }
     }

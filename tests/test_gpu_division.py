@@ -28,6 +28,11 @@ def _wide(shape, rng, dtype=np.float64):
     flat = a.reshape(-1)
     flat[:4] = [0.0, -0.0, np.inf, -np.inf]
     flat[4] = np.nan
+    # subnormal dividends and dividends next to the overflow threshold
+    if dtype == np.float64:
+        flat[5:10] = [5e-324, -1e-310, 2.2250738585072014e-308, 1.7e308, -1.0e308]
+    else:
+        flat[5:10] = [1e-45, -1e-40, 1.1754944e-38, 3.3e38, -1.0e38]
     return np.asfortranarray(a)
 
 
@@ -125,3 +130,21 @@ def test_bf16_stores_round_to_nearest_even():
     k(B, A)
     got = np.asarray(B.to_host()).view(np.uint16)
     assert np.array_equal(got, ps.f32_to_bf16(a))
+
+
+@pytest.mark.parametrize("mod,t,ut", [(__name__, np.float64, np.uint64), (MOD32, np.float32, np.uint32)])
+def test_quotients_under_comparisons_keep_per_quotient_checks(mod, t, ut):
+    # a comparison can hide a NaN quotient, so this statement tests each quotient itself;
+    # infinite dividends must still give IEEE results
+    k = ps.core._define("parallel", "@parallel divc!(R, A, b::Data.Number) = (@all(R) = (@all(A)/b > 1.0) ? @all(A)/b : -@all(A)/b; return)",
+                        {}, mod)
+    a = _wide((31, 9, 4), np.random.default_rng(9), t)
+    A = ps.zeros(*a.shape, eltype=ps.Float64 if t is np.float64 else ps.Float32)
+    A.copy_from(a)
+    R = ps.zeros(*a.shape, eltype=ps.Float64 if t is np.float64 else ps.Float32)
+    b = t(0.3)
+    k(R, A, float(b))
+    with np.errstate(all="ignore"):
+        q = a / b
+        ref = np.where(q > 1.0, q, -q).astype(t)
+    assert _same(R.to_host(), ref, ut)

@@ -464,3 +464,14 @@ def test_x_group_lowering():
     assert "ps_grp" not in k.source(ps.ArgSpec((n, n, n)), ps.ArgSpec((n, n, n)), off32=True, cells_x=4)
     k = ps.parallel_indices("@parallel_indices (ix,iy,iz) function cp!(A, B)\n A[ix,iy,iz] = B[ix,iy,iz]*2\n return\nend")
     assert "if (ps_grp) {" in k.source(ps.ArgSpec((n, n, n)), ps.ArgSpec((n, n, n)), off32=True, cells_x=2)
+
+
+def test_quotient_checks_per_statement():
+    """Statements whose value is NaN whenever a quotient is NaN check their quotients once (the
+    dividends' exponent range and the value); a comparison keeps the check of each quotient."""
+    k = ps.parallel("@parallel function q1!(R, A, b)\n @all(R) = @all(A)/b + 2.0*@all(A)/b\n return\nend")
+    body = k.source(ps.ArgSpec((8, 8, 8)), ps.ArgSpec((8, 8, 8)), 0.3).split("__global__")[1]
+    assert "ps_divx(" in body and "ps_bad" not in body and "(ps_lo < -599) | (ps_hi > 600)" in body
+    k = ps.parallel("@parallel function q2!(R, A, b)\n @all(R) = (@all(A)/b > 1.0) ? 1.0 : 0.0\n return\nend")
+    body = k.source(ps.ArgSpec((8, 8, 8)), ps.ArgSpec((8, 8, 8)), 0.3).split("__global__")[1]
+    assert "ps_divp(" in body and "ps_divx(" not in body
