@@ -338,6 +338,7 @@ class Emitter {
   // x groups (V.cx > 1): array assignments for all cells of the group, in batches, loads first
   void group_statements(const std::vector<NodeP>& sts);
   std::string group_scope(int c) const;
+  bool x_bounds(const NodeP& c, std::vector<std::string>& lo, std::vector<std::string>& hi);
   std::string store_value(int dt, const EV& r) const;
   void emit_memopt_body(int L);
 };
@@ -1198,6 +1199,39 @@ std::string Emitter::store_value(int dt, const EV& r) const {
 // Cell c of an x group: its x index and its x offset within the group, shadowing the group's.
 std::string Emitter::group_scope(int c) const {
   return "const ps_int ix = ps_ix0 + " + std::to_string(c) + "; const ps_int ps_cx = " + std::to_string(c) + "; (void)ix; (void)ps_cx; ";
+}
+
+// Bounds on the x index that a mask (a conjunction of comparisons, x_interval) puts: for each
+// comparison of ix with an integer expression, the expression (adjusted for strictness) as a lower
+// or upper bound. False if a bound is not an integer.
+bool Emitter::x_bounds(const NodeP& c, std::vector<std::string>& lo, std::vector<std::string>& hi) {
+  bool has_ix = false;
+  std::function<bool(const NodeP&)> uses = [&](const NodeP& n) {
+    if (!n) return false;
+    if (n->k == NK::Ident && n->s == kIdx[0]) return true;
+    for (auto& a : n->a)
+      if (uses(a)) return true;
+    return false;
+  };
+  has_ix = uses(c);
+  if (!has_ix) return true;
+  if (c->k == NK::And) return x_bounds(c->a[0], lo, hi) && x_bounds(c->a[1], lo, hi);
+  if (c->k != NK::Cmp) return false;
+  for (size_t i = 0; i + 1 < c->a.size(); ++i) {
+    const NodeP &l = c->a[i], &r = c->a[i + 1];
+    const bool lx = l->k == NK::Ident && l->s == kIdx[0], rx = r->k == NK::Ident && r->s == kIdx[0];
+    if (!lx && !rx) continue;
+    const std::string& op = c->ops[i];
+    if (op != "<" && op != "<=" && op != ">" && op != ">=") return false;
+    EV b = expr(lx ? r : l);
+    if (b.t != Ty::Int) return false;
+    const std::string v = "(ps_int)(" + b.c + ")";
+    // ix op b, or b op ix
+    const bool less = op == "<" || op == "<=", strict = op == "<" || op == ">";
+    if (lx == less) hi.push_back(strict ? v + " - 1" : v);
+    else lo.push_back(strict ? v + " + 1" : v);
+  }
+  return true;
 }
 
 // Array names that an expression reads (indexed argument names).
@@ -2342,10 +2376,28 @@ Emitted Emitter::run() {
     // x groups: thread tx owns the CX adjacent cells from ps_ix0; the cell offsets below are the
     // group's first cell's (each cell adds its x offset ps_cx to the uniform part of an access)
     const char* X = CX > 1 ? "ps_ix0" : "ix";
-    o << "  const ps_int tx = (ps_int)blockIdx.x * (ps_int)blockDim.x + (ps_int)threadIdx.x;\n";
-    o << "  const ps_int ty = (ps_int)blockIdx.y * (ps_int)blockDim.y + (ps_int)threadIdx.y;\n";
-    o << "  const ps_int tz = (ps_int)blockIdx.z * (ps_int)blockDim.z + (ps_int)threadIdx.z;\n";
-    if (CX > 1) {
+    o << "  const ps_int ps_bx = blockIdx.x, ps_by = blockIdx.y, ps_bz = blockIdx.z;\n";
+    o << "  const ps_int tx = ps_bx * (ps_int)blockDim.x + (ps_int)threadIdx.x;\n";
+    o << "  const ps_int ty = ps_by * (ps_int)blockDim.y + (ps_int)threadIdx.y;\n";
+    o << "  const ps_int tz = ps_bz * (ps_int)blockDim.z + (ps_int)threadIdx.z;\n";
+    // x groups with the row split: the x interval in which every statement's mask holds (from
+    // the masks' bounds on ix) holds the row's interior groups, which the first threads of the
+    // row take in order; the cells outside it (the row ends) go one at a time to the threads of
+    // the last block column, so that interior wavefronts never run the per-cell path
+    std::vector<std::string> xlo, xhi;
+    bool split = CX > 1;
+    if (CX > 1)
+      for (auto& st : K.body->a)
+        if (st->k == NK::If && !x_bounds(st->a[0], xlo, xhi)) split = false;
+    if (split) {
+      o << "  if (ty >= ps_rly || tz >= ps_rlz) return;\n";
+      o << "  const ps_int iy = ps_r0y + ty;\n  const ps_int iz = ps_r0z + tz;\n  (void)iy; (void)iz;\n";
+      o << "  ps_int ps_xl = ps_r0x, ps_xh = ps_r0x + ps_rlx - 1;\n";
+      for (auto& b : xlo) o << "  ps_xl = max(ps_xl, " << b << ");\n";
+      for (auto& b : xhi) o << "  ps_xh = min(ps_xh, " << b << ");\n";
+      o << "  const ps_int ps_ng = ps_xh >= ps_xl ? (ps_xh - ps_xl + 1) / " << CX << " : 0;  // interior groups\n";
+      o << "  const ps_int ps_ngp = (ps_ng + (ps_int)blockDim.x - 1) / (ps_int)blockDim.x * (ps_int)blockDim.x;\n";
+    } else if (CX > 1) {
       o << "  if (tx * " << CX << " >= ps_rlx || ty >= ps_rly || tz >= ps_rlz) return;\n";
       o << "  const ps_int ps_ix0 = ps_r0x + tx * " << CX << ";\n  const ps_int iy = ps_r0y + ty;\n  const ps_int iz = ps_r0z + tz;\n";
       o << "  (void)iy; (void)iz;\n";
@@ -2354,29 +2406,33 @@ Emitted Emitter::run() {
       o << "  const ps_int ix = ps_r0x + tx;\n  const ps_int iy = ps_r0y + ty;\n  const ps_int iz = ps_r0z + tz;\n";
       o << "  (void)ix; (void)iy; (void)iz;\n";
     }
-    for (size_t i = 0; i < kinds.size(); ++i) {
-      if (!kinds[i].is_field) continue;
-      std::string A = an((int)i);
-      // fields with the strides of an earlier one share its cell offset
-      int same = -1;
-      if (V.sclass.size() == kinds.size() && V.sclass[i] != '-')
-        for (size_t j = 0; j < i && same < 0; ++j)
-          if (V.sclass[j] == V.sclass[i]) same = (int)j;
-      if (same >= 0) {
-        o << "  const ps_int " << A << "_o = " << an(same) << "_o; (void)" << A << "_o;\n";
-      } else if (V.u24) {
-        o << "  const ps_int " << A << "_o = (" << X << " - 1) + (ps_int)__umul24((unsigned)(iy - 1), (unsigned)" << A
-          << "_sx) + (ps_int)__umul24((unsigned)(iz - 1), (unsigned)" << A << "_sxy); (void)" << A << "_o;\n";
-      } else {
-        o << "  const ps_int " << A << "_o = (" << X << " - 1) + (iy - 1)*" << A << "_sx + (iz - 1)*" << A << "_sxy; (void)" << A << "_o;\n";
+    // per-field cell offsets of the thread's (first) cell X
+    auto offsets = [&](const std::string& in) {
+      for (size_t i = 0; i < kinds.size(); ++i) {
+        if (!kinds[i].is_field) continue;
+        std::string A = an((int)i);
+        // fields with the strides of an earlier one share its cell offset
+        int same = -1;
+        if (V.sclass.size() == kinds.size() && V.sclass[i] != '-')
+          for (size_t j = 0; j < i && same < 0; ++j)
+            if (V.sclass[j] == V.sclass[i]) same = (int)j;
+        if (same >= 0) {
+          o << in << "const ps_int " << A << "_o = " << an(same) << "_o; (void)" << A << "_o;\n";
+        } else if (V.u24) {
+          o << in << "const ps_int " << A << "_o = (" << X << " - 1) + (ps_int)__umul24((unsigned)(iy - 1), (unsigned)" << A
+            << "_sx) + (ps_int)__umul24((unsigned)(iz - 1), (unsigned)" << A << "_sxy); (void)" << A << "_o;\n";
+        } else {
+          o << in << "const ps_int " << A << "_o = (" << X << " - 1) + (iy - 1)*" << A << "_sx + (iz - 1)*" << A << "_sxy; (void)" << A << "_o;\n";
+        }
+        if (V.off32 && kinds[i].ncomp == 0) {
+          if (same >= 0 && kinds[same].ncomp == 0 && dtype_bytes(kinds[same].dtype) == dtype_bytes(kinds[i].dtype))
+            o << in << "const unsigned " << A << "_vo = " << an(same) << "_vo; (void)" << A << "_vo;\n";
+          else
+            o << in << "const unsigned " << A << "_vo = (unsigned)" << A << "_o * " << dtype_bytes(kinds[i].dtype) << "u; (void)" << A << "_vo;\n";
+        }
       }
-      if (V.off32 && kinds[i].ncomp == 0) {
-        if (same >= 0 && kinds[same].ncomp == 0 && dtype_bytes(kinds[same].dtype) == dtype_bytes(kinds[i].dtype))
-          o << "  const unsigned " << A << "_vo = " << an(same) << "_vo; (void)" << A << "_vo;\n";
-        else
-          o << "  const unsigned " << A << "_vo = (unsigned)" << A << "_o * " << dtype_bytes(kinds[i].dtype) << "u; (void)" << A << "_vo;\n";
-      }
-    }
+    };
+    if (!split) offsets("  ");
     for (auto& v : local_order) o << "  " << cty(locals[v]) << " " << local_c[v] << " = 0;\n";
     for (auto& sm : sm_dtype) {
       const std::string P = "sm_" + sanitize(sm.first);
@@ -2384,18 +2440,22 @@ Emitted Emitter::run() {
       o << "  (void)" << P << "; (void)" << P << "_n1; (void)" << P << "_n2; (void)" << P << "_n3;\n";
     }
     ind = 1;
-    if (CX > 1) {
-      // every mask is an x interval (groupable_body), so the group's first and last cells decide
-      // whether all its cells pass all of them
-      o << "  bool ps_grp = tx * " << CX << " + " << (CX - 1) << " < ps_rlx;\n";
+    // the group from ps_ix0: all statements for all cells in batches when the masks hold at the
+    // group's first and last cell (every mask is an x interval, groupable_body), else statement by
+    // statement, each for the group's cells in the launch range, in the order in which the lanes of
+    // a wavefront run one cell each (a trailing return would end the group after its first cell
+    // and is left out)
+    auto group = [&](int b, const std::string& first) {
+      const std::string in(2 * b, ' ');
+      o << in << "bool ps_grp = " << first << ";\n";
       for (int c : {0, CX - 1}) {
         std::string conds;
         for (auto& st : K.body->a)
           if (st->k == NK::If) conds += " && (" + expr(st->a[0]).c + ")";
-        if (!conds.empty()) o << "  { " << group_scope(c) << "ps_grp = ps_grp" << conds << "; }\n";
+        if (!conds.empty()) o << in << "{ " << group_scope(c) << "ps_grp = ps_grp" << conds << "; }\n";
       }
-      o << "  if (ps_grp) {\n";
-      ind = 2;
+      o << in << "if (ps_grp) {\n";
+      ind = b + 1;
       std::vector<NodeP> sts;
       for (auto& st : K.body->a) {
         if (st->k == NK::If)
@@ -2404,20 +2464,42 @@ Emitted Emitter::run() {
           sts.push_back(st);
       }
       group_statements(sts);
-      // other groups: statement by statement, each for the group's cells in the launch range, in
-      // the order in which the lanes of a wavefront run one cell each (a trailing return would end
-      // the group after its first cell and is left out)
-      o << "  } else {\n";
+      o << in << "} else {\n";
       for (auto& st : K.body->a) {
         if (st->k == NK::Return) continue;
         for (int c = 0; c < CX; ++c) {
-          o << "    { " << group_scope(c) << "if (ix < ps_r0x + ps_rlx) {\n";
-          ind = 3;
+          o << in << "  { " << group_scope(c) << "if (ix < ps_r0x + ps_rlx) {\n";
+          ind = b + 2;
           stmt(st);
-          o << "    } }\n";
+          o << in << "  } }\n";
         }
       }
-      o << "  }\n";
+      o << in << "}\n";
+    };
+    if (split) {
+      o << "  if (tx < ps_ng) {\n";
+      o << "    const ps_int ps_ix0 = ps_xl + tx * " << CX << ";\n";
+      offsets("    ");
+      group(2, "true");
+      // row ends: edge cell e of the row is the e-th cell left of ps_xl or right of the interior
+      // groups; the edge threads of the row take them in turn
+      o << "  } else if (tx >= ps_ngp) {\n";
+      o << "    const ps_int ps_nl = ps_ng > 0 ? ps_xl - ps_r0x : ps_rlx;\n";
+      o << "    const ps_int ps_ne = ps_rlx - ps_ng * " << CX << ";\n";
+      o << "    const ps_int ps_es = (ps_int)gridDim.x * (ps_int)blockDim.x - ps_ngp;\n";
+      o << "    for (ps_int ps_e = tx - ps_ngp; ps_e < ps_ne; ps_e += ps_es) {\n";
+      o << "      const ps_int ps_ix0 = ps_e < ps_nl ? ps_r0x + ps_e : ps_xl + ps_ng * " << CX << " + (ps_e - ps_nl);\n";
+      offsets("      ");
+      for (auto& st : K.body->a) {
+        if (st->k == NK::Return) continue;
+        o << "      { " << group_scope(0) << "\n";
+        ind = 4;
+        stmt(st);
+        o << "      }\n";
+      }
+      o << "    }\n  }\n";
+    } else if (CX > 1) {
+      group(1, "tx * " + std::to_string(CX) + " + " + std::to_string(CX - 1) + " < ps_rlx");
     } else {
       block(K.body);
     }

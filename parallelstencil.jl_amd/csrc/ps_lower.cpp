@@ -315,9 +315,15 @@ static void read_options(Options& o, const JVal& j) {
       o.cells_x = (int)v.n;
       if (!(o.cells_x == 0 || o.cells_x == 1 || o.cells_x == 2 || o.cells_x == 4 || o.cells_x == 8))
         throw PsError(PS_EKEYWORD, "cells_x must be 0 (automatic), 1, 2, 4 or 8");
+    } else if (k == "nthreads_x_groups") {
+      o.nthreads_x_groups = (int)v.n;
+      if (!(o.nthreads_x_groups == 0 || (o.nthreads_x_groups >= 1 && o.nthreads_x_groups <= 1024 &&
+                                         (o.nthreads_x_groups & (o.nthreads_x_groups - 1)) == 0)))
+        throw PsError(PS_EKEYWORD, "nthreads_x_groups must be 0 (automatic) or a power of 2 up to 1024");
     } else if (k == "addr32") o.addr32 = v.b;
     else if (k == "lds") o.lds = v.b;
     else if (k == "xcd_remap") o.xcd_remap = v.b;
+    else if (k == "xcd_spread") o.xcd_spread = v.b;
     else if (k == "nt_store") o.nt_store = v.b ? 1 : 0;
     else if (k == "tuple_params") {
       o.tuple_params.clear();

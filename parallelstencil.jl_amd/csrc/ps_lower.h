@@ -68,10 +68,16 @@ struct Options {
   int prefetch = 1;                       // memopt: loads are issued this many steps ahead
   // plain kernels: adjacent x cells per thread (x groups); 0: automatic (by element size), 1: off
   int cells_x = 0;
+  // plain kernels run as x groups: threads along x of the launched block (the block keeps its
+  // thread count); 0: automatic (128 cells along x per block row)
+  int nthreads_x_groups = 0;
   // plain kernels: 32-bit per-thread byte offsets on scalar bases where they fit (Variant::off32)
   bool addr32 = true;
   bool lds = true;                        // memopt: stage x-y neighbours of optvars through LDS
   bool xcd_remap = false;                 // XCD-aware block ordering for memopt kernels
+  // 3-D plain kernels run as x groups: an odd number of block columns along x, so that the
+  // round-robin dispatch of blocks to the 8 XCDs rotates the x columns over them
+  bool xcd_spread = true;
   // nontemporal stores for array writes: 1 on, 0 off, -1 automatic (on for the 256-wide
   // LDS-tiled memopt blocks, where they measured faster; off elsewhere)
   int nt_store = -1;

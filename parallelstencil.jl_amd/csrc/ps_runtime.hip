@@ -1272,7 +1272,10 @@ struct Plan {
   int64_t ranges[3][2];
   int32_t nthreads[3];
   int32_t nblocks[3];  // the reference's geometry (what geometry() reports)
-  int32_t grid_x = 0;  // x groups (v.cx > 1): the blocks launched along x
+  // the launched block and grid: the reference's geometry, or for automatic geometry of plain
+  // kernels the x-group block shape and the grid padded for the XCDs
+  int32_t lthreads[3];
+  int32_t lblocks[3];
   Variant v;
 };
 
@@ -1403,8 +1406,11 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
       ps_compute_nblocks(len, P.nthreads, P.nblocks);
     }
   }
-  for (int d = 0; d < 3; ++d)
+  for (int d = 0; d < 3; ++d) {
     if (P.nthreads[d] < 1 || P.nblocks[d] < 0) throw PsError(PS_EARGUMENT, "invalid nthreads / nblocks");
+    P.lthreads[d] = P.nthreads[d];
+    P.lblocks[d] = P.nblocks[d];
+  }
   if ((int64_t)P.nthreads[0] * P.nthreads[1] * P.nthreads[2] > 1024) throw PsError(PS_EARGUMENT, "at most 1024 threads per block are supported");
   if (P.nblocks[1] > 65535 || P.nblocks[2] > 65535) throw PsError(PS_EARGUMENT, "nblocks in y and z must not exceed 65535");
   // variant
@@ -1524,10 +1530,44 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
       if (cx > 1 && len[0] >= 2 * (int64_t)cx) {
         v.cx = cx;
         v.off32 = true;
-        P.grid_x = (int32_t)(((len[0] + cx - 1) / cx + P.nthreads[0] - 1) / P.nthreads[0]);
+        // Block shape of the launch: the reference's thread count per block, bx threads along x,
+        // by default 128 cells along x per block row (BFloat16 4 cells per thread: 32 threads and
+        // two y rows per wavefront; FP32 2 cells: 64 threads). Measured on acoustic3D 384^3
+        // against 16 to 64 threads: the fastest, or within the run-to-run spread of it.
+        const int64_t ng = (len[0] + cx - 1) / cx;
+        const int32_t txy = P.nthreads[0] * P.nthreads[1];
+        auto pow2_ceil = [](int64_t n) { int64_t p = 1; while (p < n) p <<= 1; return p; };
+        int64_t bx = K.opt.nthreads_x_groups ? K.opt.nthreads_x_groups : std::max<int64_t>(1, 128 / cx);
+        bx = std::min<int64_t>({bx, (int64_t)txy, pow2_ceil(ng)});
+        int64_t by = std::max<int64_t>(1, txy / bx);
+        if (by > pow2_ceil(len[1])) {
+          by = pow2_ceil(len[1]);
+          bx = std::max<int64_t>(1, txy / by);
+        }
+        if ((len[1] + by - 1) / by > 65535) {
+          bx = P.nthreads[0];
+          by = P.nthreads[1];
+        }
+        P.lthreads[0] = (int32_t)bx;
+        P.lthreads[1] = (int32_t)by;
+        // one block column more than the groups need: its threads take the cells at the row ends,
+        // outside the x interval in which every statement's mask holds (the kernel computes it)
+        P.lblocks[0] = (int32_t)((ng + bx - 1) / bx + 1);
+        P.lblocks[1] = (int32_t)((len[1] + by - 1) / by);
       }
     }
   }
+  // XCD spread of x-group launches: workgroups go to the 8 XCDs round-robin in linear block
+  // order (x fastest), each XCD with its own L2. With an even number of block columns along x an
+  // XCD only ever gets some of the columns (all of one column when there are 8), so that its L2
+  // and its share of the memory requests see a few narrow x ranges of every row; an odd number
+  // rotates the columns over the XCDs from one block row to the next. An even x grid therefore
+  // gets one more block column, whose threads find no cells and return. Measured on acoustic3D
+  // 384^3 with x groups: 10-35% faster where the grid was even. One-cell-per-thread launches keep
+  // the reference's grid (diffusion3D 512^3 FP64, 8 columns, ran 3% slower with 9), and so do
+  // memopt launches (diffusion3D 512^3: 2 columns, 3% slower with 3, and 2-5% slower with the
+  // blocks of each XCD taken from one contiguous eighth of the grid, xcd_remap).
+  if (v.cx > 1 && K.opt.xcd_spread && P.lblocks[2] > 1 && P.lblocks[0] % 2 == 0) P.lblocks[0] += 1;
   P.v = v;
   return P;
 }
@@ -1781,9 +1821,8 @@ extern "C" int ps_kernel_launch(ps_kernel k, const ps_arg* args, int nargs, cons
                                     " bytes of LDS a workgroup can use");
   }
   if (P.v.checked) g_err_armed.store(true, std::memory_order_relaxed);
-  const unsigned gx = (unsigned)(P.v.cx > 1 ? P.grid_x : P.nblocks[0]);
-  hipError_t e = hipModuleLaunchKernel(m->fn, gx, (unsigned)P.nblocks[1], (unsigned)P.nblocks[2], (unsigned)P.nthreads[0],
-                                       (unsigned)P.nthreads[1], (unsigned)P.nthreads[2], shmem, st, ptrs.data(), nullptr);
+  hipError_t e = hipModuleLaunchKernel(m->fn, (unsigned)P.lblocks[0], (unsigned)P.lblocks[1], (unsigned)P.lblocks[2], (unsigned)P.lthreads[0],
+                                       (unsigned)P.lthreads[1], (unsigned)P.lthreads[2], shmem, st, ptrs.data(), nullptr);
   if (e != hipSuccess) return fail(PS_EHIP, std::string("hipModuleLaunchKernel: ") + hipGetErrorString(e));
   if (!(launch && launch->async)) {
     PS_HIP(hipStreamSynchronize(st));

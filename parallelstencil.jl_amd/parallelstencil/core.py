@@ -1208,7 +1208,7 @@ class _DerivativeKernel(Kernel):
 
 _KW_DEF = ("ndims", "N", "inbounds", "padding", "memopt", "optvars", "loopdim", "loopsize", "optranges", "useshmemhalos",
            "optimize_halo_read", "fp_contract", "fastmath", "prefetch", "lds", "xcd_remap", "nt_store", "nthreads_max_memopt",
-           "nthreads_x_max_memopt", "cells_x", "addr32")
+           "nthreads_x_max_memopt", "cells_x", "nthreads_x_groups", "xcd_spread", "addr32")
 
 
 _PREFIX_RE = re.compile(r"^(\s*@parallel(?:_indices)?\b\s*(?:\([^()]*\)|\w+\.\.\.)?)((?:\s*\w+\s*=\s*(?:\([^()]*\)|[^\s(]+))*)", re.S)
@@ -1333,7 +1333,7 @@ def _define(kind, text, kw, mod, indices=None):
     if indices is not None:
         opts["indices"] = list(indices)
     for k in ("optvars", "loopdim", "loopsize", "fp_contract", "fastmath", "prefetch", "lds", "xcd_remap", "nt_store", "nthreads_max_memopt",
-              "nthreads_x_max_memopt", "cells_x", "addr32"):
+              "nthreads_x_max_memopt", "cells_x", "nthreads_x_groups", "xcd_spread", "addr32"):
         if k in kw:
             v = kw[k]
             opts[k] = list(v) if k == "optvars" and isinstance(v, (tuple, list)) else v

@@ -58,6 +58,81 @@ def test_x_groups_match_one_cell_per_thread(elt, mod, shape, cx):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (elt, shape, cx)
 
 
+@pytest.mark.parametrize("elt,mod", [(ps.Float64, M64), (ps.Float32, M32), (ps.BFloat16, M32)])
+@pytest.mark.parametrize("shape", [(130, 7, 6), (300, 2, 3), (9, 40, 5)])
+@pytest.mark.parametrize("width", [1, 8, 64, 256])
+def test_x_group_block_shapes_match_one_cell_per_thread(elt, mod, shape, width):
+    # the launched block of x groups: `width` threads along x, the rest along y (clamped to the
+    # y extent, which then widens x again)
+    rng = np.random.default_rng(sum(shape) + width)
+    ref = ps.core._define("parallel", STENCIL, {"cells_x": 1}, mod)
+    grp = ps.core._define("parallel", STENCIL, {"cells_x": 2, "nthreads_x_groups": width}, mod)
+    f1 = _fields(shape, elt, rng)
+    f2 = [(a, ps.zeros(*shape, eltype=elt)) for a, _ in f1]
+    for (_, d1), (_, d2) in zip(f1, f2):
+        d2.copy_from(d1.to_host())
+    for it in range(2):
+        ref(*[d for _, d in f1], 0.37, 1e-3 if it == 1 else 0.9)
+        grp(*[d for _, d in f2], 0.37, 1e-3 if it == 1 else 0.9)
+    for (_, d1), (_, d2) in zip(f1, f2):
+        a, b = np.ascontiguousarray(d1.to_host()), np.ascontiguousarray(d2.to_host())
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (elt, shape, width)
+
+
+# row split: the interior (every x mask holds) is [5, nx - 60], 64 edge cells per row; with a
+# second statement under ix <= 3 no cell of the row passes both masks (no interior); a float
+# bound leaves the kernel without the split
+EDGES = """
+@parallel_indices (ix, iy, iz) function edges!(A, B, C, s::Data.Number)
+    if ix >= 5 && ix <= size(A, 1) - 60
+        A[ix, iy, iz] = B[ix, iy, iz] * s + C[ix + 1, iy, iz]
+    end
+    return
+end
+"""
+NO_INTERIOR = """
+@parallel_indices (ix, iy, iz) function nointerior!(A, B, C, D, s::Data.Number)
+    if ix >= 5 && ix <= size(A, 1) - 60
+        A[ix, iy, iz] = B[ix, iy, iz] * s + C[ix + 1, iy, iz]
+    end
+    if ix <= 3
+        D[ix, iy, iz] = B[ix, iy, iz] - s
+    end
+    return
+end
+"""
+FLOAT_BOUND = """
+@parallel_indices (ix, iy, iz) function halfrow!(A, B, C, s::Data.Number)
+    if ix <= size(A, 1) / 2
+        A[ix, iy, iz] = B[ix, iy, iz] * s + C[ix + 1, iy, iz]
+    end
+    return
+end
+"""
+
+
+@pytest.mark.parametrize("elt,mod", [(ps.Float64, M64), (ps.Float32, M32), (ps.BFloat16, M32)])
+@pytest.mark.parametrize("src,nf", [(EDGES, 3), (NO_INTERIOR, 4), (FLOAT_BOUND, 3)])
+@pytest.mark.parametrize("cx,width", [(2, 8), (4, 64), (8, 4)])
+def test_x_group_row_split_edges(elt, mod, src, nf, cx, width):
+    # more edge cells than edge threads (the edge threads loop), rows without interior, and the
+    # unsplit layout for a float bound: the same fields as one cell per thread
+    shape = (300, 40, 3)
+    rng = np.random.default_rng(cx * 7 + width + nf)
+    ref = ps.core._define("parallel_indices", src, {"cells_x": 1}, mod)
+    grp = ps.core._define("parallel_indices", src, {"cells_x": cx, "nthreads_x_groups": width}, mod)
+    f1 = _fields(shape, elt, rng)[:nf]
+    f2 = [(a, ps.zeros(*shape, eltype=elt)) for a, _ in f1]
+    for (_, d1), (_, d2) in zip(f1, f2):
+        d2.copy_from(d1.to_host())
+    box = ((2, 299), (1, 40), (1, 3))  # the launch box starts at x index 2
+    ref(*[d for _, d in f1], 0.37, ranges=box)
+    grp(*[d for _, d in f2], 0.37, ranges=box)
+    for (_, d1), (_, d2) in zip(f1, f2):
+        a, b = np.ascontiguousarray(d1.to_host()), np.ascontiguousarray(d2.to_host())
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (elt, cx, width)
+
+
 def test_x_groups_acoustic3d_bf16_against_oracle_run():
     # the acoustic kernels pick x groups by themselves for FP32 (2 cells) and BFloat16 (4 cells);
     # the reported geometry stays the reference's

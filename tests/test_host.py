@@ -466,6 +466,28 @@ def test_x_group_lowering():
     assert "if (ps_grp) {" in k.source(ps.ArgSpec((n, n, n)), ps.ArgSpec((n, n, n)), off32=True, cells_x=2)
 
 
+def test_x_group_row_split():
+    """x groups split each row: the interior, where every statement's x mask holds (the maximum of
+    the masks' lower bounds and the minimum of their upper bounds), in groups from its first cell;
+    the cells outside it one at a time in the threads past the interior. A mask bound that is not
+    an integer keeps groups from the start of the launch range."""
+    from parallelstencil import workloads as W
+    kV, _ = W.define_acoustic3d(__name__, False)
+    n = 20
+    F = [ps.ArgSpec((n + 1, n, n)), ps.ArgSpec((n, n + 1, n)), ps.ArgSpec((n, n, n + 1)), ps.ArgSpec((n, n, n))]
+    src = kV.source(*F, 0.1, 2.0, 0.3, 0.3, 0.3, off32=True, cells_x=4)
+    for a in ("a0", "a1", "a2"):  # @inn of each velocity: 1 <= ix <= n_x - 2 in the kernel's indices
+        assert f"ps_xh = min(ps_xh, (ps_int)(({a}_nx - ((ps_int)2))));" in src
+    assert src.count("ps_xl = max(ps_xl, (ps_int)(((ps_int)1)));") == 3
+    assert "const ps_int ps_ix0 = ps_xl + tx * 4;" in src and "for (ps_int ps_e = tx - ps_ngp; ps_e < ps_ne; ps_e += ps_es)" in src
+    strict = ps.parallel_indices("@parallel_indices (ix,iy,iz) function st!(A, B)\n if ix > 2 && 7 > ix\n A[ix,iy,iz] = B[ix,iy,iz]\n end\n return\nend")
+    s2 = strict.source(ps.ArgSpec((n, n, n)), ps.ArgSpec((n, n, n)), off32=True, cells_x=2)
+    assert "ps_xl = max(ps_xl, (ps_int)(((ps_int)2)) + 1);" in s2 and "ps_xh = min(ps_xh, (ps_int)(((ps_int)7)) - 1);" in s2
+    half = ps.parallel_indices("@parallel_indices (ix,iy,iz) function hf!(A, B)\n if ix <= size(A, 1) / 2\n A[ix,iy,iz] = B[ix,iy,iz]\n end\n return\nend")
+    s3 = half.source(ps.ArgSpec((n, n, n)), ps.ArgSpec((n, n, n)), off32=True, cells_x=2)
+    assert "ps_ng" not in s3 and "const ps_int ps_ix0 = ps_r0x + tx * 2;" in s3
+
+
 def test_quotient_checks_per_statement():
     """Statements whose value is NaN whenever a quotient is NaN check their quotients once (the
     dividends' exponent range and the value); a comparison keeps the check of each quotient."""
