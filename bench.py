@@ -274,6 +274,74 @@ def acoustic_study(args):
     print(json.dumps(out), flush=True)
 
 
+def xgroups_study(args):
+    """acoustic3D n^3 plain kernels (miniapps/acoustic3D.jl), exact, in BFloat16, FP32 and FP64
+    storage: kernel time per step (V + P, HIP events) over x-group launch shapes: cells per
+    thread (cells_x), block width along x (nthreads_x_groups) and the odd x grid (xcd_spread),
+    with the library's default shape beside them. Under sustained load the GPU's clock drops over
+    tens of seconds (by up to 30% for instruction-bound kernels), so every shape of every
+    precision is measured once per round, rounds alternate the order, and each shape reports the
+    median of its rounds."""
+    import numpy as np
+    import parallelstencil as ps
+    from parallelstencil import workloads as W
+
+    n, burst, rounds = args.acoustic_n, 100, 5
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float32, 3, module="bench.x32")
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3, module="bench.x64")
+    P0, prm = W.acoustic3d_inputs(n)
+    a = (prm["dt"], prm["rho"], prm["dx"], prm["dy"], prm["dz"])
+    b = (prm["dt"], prm["k"], prm["dx"], prm["dy"], prm["dz"])
+    fields = {}
+    for prec, elt in (("bf16", ps.BFloat16), ("f32", ps.Float32), ("f64", ps.Float64)):
+        F = (ps.zeros(n, n, n, eltype=elt), ps.zeros(n + 1, n, n, eltype=elt), ps.zeros(n, n + 1, n, eltype=elt), ps.zeros(n, n, n + 1, eltype=elt))
+        F[0].copy_from(W._to_storage(P0, elt))
+        fields[prec] = F
+    cfgs = [("f64", 0, 0, True)]
+    cfgs += [("bf16", 0, 0, True), ("bf16", 1, 0, True)] + [("bf16", cx, w, sp) for cx in (2, 4, 8) for w in (16, 32, 64) for sp in (False, True)]
+    cfgs += [("f32", 0, 0, True), ("f32", 1, 0, True)] + [("f32", cx, w, sp) for cx in (2, 4) for w in (16, 32, 64) for sp in (False, True)]
+    kern = {c: W.define_acoustic3d("bench.x64" if c[0] == "f64" else "bench.x32", False, cells_x=c[1], nthreads_x_groups=c[2], xcd_spread=c[3])
+            for c in cfgs}
+    res = {c: [] for c in cfgs}
+    for r in range(rounds):
+        for c in (cfgs if r % 2 == 0 else cfgs[::-1]):
+            kV, kP = kern[c]
+            P, Vx, Vy, Vz = fields[c[0]]
+            for _ in range(20):
+                kV(Vx, Vy, Vz, P, *a, async_=True)
+                kP(P, Vx, Vy, Vz, *b, async_=True)
+            e = [ps.Event() for _ in range(3)]
+            e[0].record()
+            for _ in range(burst):
+                kV(Vx, Vy, Vz, P, *a, async_=True)
+            e[1].record()
+            for _ in range(burst):
+                kP(P, Vx, Vy, Vz, *b, async_=True)
+            e[2].record()
+            ps.synchronize()
+            res[c].append((e[0].elapsed_ms(e[1]) / burst, e[1].elapsed_ms(e[2]) / burst))
+        log(f"round {r + 1} of {rounds} done")
+    out_shapes = {}
+    for c in cfgs:
+        prec, cx, w, sp = c
+        key = f"{prec} " + ("default" if cx == 0 else f"cells_x={cx}" + (f" width={w} odd_grid={int(sp)}" if cx > 1 else ""))
+        v = float(np.median([x[0] for x in res[c]]))
+        p = float(np.median([x[1] for x in res[c]]))
+        tot = [x[0] + x[1] for x in res[c]]
+        out_shapes[key] = {"V_ms": round(v, 4), "P_ms": round(p, 4), "V+P_ms": round(float(np.median(tot)), 4),
+                           "V+P_min_max": [round(min(tot), 4), round(max(tot), 4)]}
+        log(f"acoustic3D {n}^3 {key}: V {v:.4f} ms, P {p:.4f} ms, V+P median {np.median(tot):.4f} ms (rounds {min(tot):.4f} - {max(tot):.4f})")
+    ms16, ms32, ms64 = (out_shapes[f"{p} default"]["V+P_ms"] for p in ("bf16", "f32", "f64"))
+    log(f"default shapes: BF16 {ms16:.4f} ms, FP32 {ms32:.4f} ms, FP64 {ms64:.4f} ms per step; BF16 / FP32 speed-up {ms32 / ms16:.2f}x")
+    out = {"metric": "acoustic3D plain kernel time per step (V + P) by x-group launch shape", "value": ms16,
+           "unit": "ms per step (BFloat16, default shape)", "n_gpus": 1, "steps": burst * rounds, "warmup": 20, "ms_per_step": ms16,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16-storage / f32 / f64", "data": "synthetic",
+           "config": {"workload": f"acoustic3D {n}^3 plain kernels, exact, miniapp initial state; {rounds} interleaved rounds of {burst} steps"},
+           "f32_default_ms_per_step": ms32, "f64_default_ms_per_step": ms64, "bf16_speedup_over_f32": round(ms32 / ms16, 3),
+           "shapes": out_shapes}
+    print(json.dumps(out), flush=True)
+
+
 def adjoint_study(args):
     """Acoustic 2-D inversion gradient at n^2 over `steps` steps (sqrt(N) checkpointing): time of
     the forward run and of the checkpointed gradient in FP32 and FP64, the FP32 gradient against
@@ -293,7 +361,8 @@ def adjoint_study(args):
         inv = AA.Acoustic2DInversion(mod, s)
         Dobs = inv.forward(s.K0)  # observed data from the background model
         inv.set_observed(Dobs)
-        inv.forward(s.K)  # warm-up
+        inv.forward(s.K)  # warm-up: compiles the forward and adjoint kernels outside the timing
+        inv.gradient(s.K)
         t0 = time.perf_counter()
         inv.forward(s.K)
         t1 = time.perf_counter()
@@ -774,7 +843,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes3d | thermal | porowaves | hydromech")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | xgroups | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes3d | thermal | porowaves | hydromech")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--acoustic-exact-only", action="store_true", help="acoustic: skip the fastmath variants")
@@ -809,6 +878,9 @@ def main():
         return
     if args.workload == "adjoint":
         adjoint_study(args)
+        return
+    if args.workload == "xgroups":
+        xgroups_study(args)
         return
     if args.workload == "gemm":
         gemm_study(args)

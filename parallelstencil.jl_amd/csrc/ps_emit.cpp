@@ -111,6 +111,10 @@ PS_DI unsigned short ps_bf16_st(float f) {
   const unsigned short u = __builtin_bit_cast(unsigned short, (__bf16)f);
   return f != f ? (unsigned short)0x7fc0u : u;
 }
+// the rounding alone, for values that are not NaN or are the canonical NaN 0x7fc00000 (which
+// rounds to 0x7fc0); ps_nan_canon maps every NaN to it
+PS_DI unsigned short ps_bf16_cvt(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
+PS_DI float ps_nan_canon(float f) { return f != f ? __uint_as_float(0x7fc00000u) : f; }
 PS_DI double ps_max(double x, double y) { bool py = (y > x) || (__builtin_signbit(y) < __builtin_signbit(x)); return py ? (x != x ? x : y) : (y != y ? y : x); }
 PS_DI float  ps_max(float x, float y)   { bool py = (y > x) || (__builtin_signbit(y) < __builtin_signbit(x)); return py ? (x != x ? x : y) : (y != y ? y : x); }
 PS_DI double ps_min(double x, double y) { bool py = (y < x) || (__builtin_signbit(y) > __builtin_signbit(x)); return py ? (x != x ? x : y) : (y != y ? y : x); }
@@ -1254,6 +1258,7 @@ void Emitter::group_statements(const std::vector<NodeP>& sts) {
     std::string g, ref, decl, badv, bad;
     EV f, e;
     bool uf;
+    bool nan_in_bad;  // the flag of the IEEE recomputation is set by any NaN value of the group
     int dt;
   };
   const int CX = V.cx;
@@ -1292,6 +1297,7 @@ void Emitter::group_statements(const std::vector<NodeP>& sts) {
       x.decl = fast_decls();
       x.badv = "ps_v";  // the cell's value, named in its scope
       x.bad = fast_bad(x.badv);
+      x.nan_in_bad = x.uf && div_x;
       div_x = false;
       auto idx = norm_indices(lhs, ai);
       std::string ic[3], ref32;
@@ -1314,17 +1320,41 @@ void Emitter::group_statements(const std::vector<NodeP>& sts) {
         else
           o << pad() << "{ " << group_scope(c) << vc << " = " << x.f.c << "; }\n";
       }
+    // BFloat16 stores round in one instruction; a group's NaNs become the canonical NaN first, in
+    // a branch that only groups holding a NaN take (for statements whose recomputation flag is
+    // set by NaN values, inside the recomputation)
+    auto canon = [&](const G& x, const std::string& in) {
+      for (int c = 0; c < CX; ++c) {
+        const std::string vc = "ps_g" + x.g + "_" + std::to_string(c);
+        o << in << vc << " = ps_nan_canon(" << vc << ");\n";
+      }
+    };
     for (auto& x : batch) {
-      if (!x.uf) continue;
-      o << pad() << "if (ps_g" << x.g << "_bad) {\n";
-      o << pad() << "  __asm__ volatile(\"\");\n";
-      for (int c = 0; c < CX; ++c)
-        o << pad() << "  { " << group_scope(c) << "ps_g" << x.g << "_" << c << " = " << cast(x.e, x.f.t).c << "; }\n";
-      o << pad() << "}\n";
+      const bool bf = x.dt == PS_BF16 && x.f.t == Ty::F32;
+      if (x.uf) {
+        o << pad() << "if (ps_g" << x.g << "_bad) {\n";
+        o << pad() << "  __asm__ volatile(\"\");\n";
+        for (int c = 0; c < CX; ++c)
+          o << pad() << "  { " << group_scope(c) << "ps_g" << x.g << "_" << c << " = " << cast(x.e, x.f.t).c << "; }\n";
+        if (bf && x.nan_in_bad) canon(x, pad() + "  ");
+        o << pad() << "}\n";
+      }
+      if (bf && !x.nan_in_bad) {
+        std::string any;
+        for (int c = 0; c < CX; ++c) {
+          const std::string vc = "ps_g" + x.g + "_" + std::to_string(c);
+          any += std::string(c ? " | " : "") + "(" + vc + " != " + vc + ")";
+        }
+        o << pad() << "if (" << any << ") {\n";
+        canon(x, pad() + "  ");
+        o << pad() << "}\n";
+      }
     }
     for (auto& x : batch)
       for (int c = 0; c < CX; ++c) {
-        const std::string v = store_value(x.dt, EV{"ps_g" + x.g + "_" + std::to_string(c), x.f.t});
+        const bool bf = x.dt == PS_BF16 && x.f.t == Ty::F32;
+        const std::string vn = "ps_g" + x.g + "_" + std::to_string(c);
+        const std::string v = bf ? "ps_bf16_cvt(" + vn + ")" : store_value(x.dt, EV{vn, x.f.t});
         o << pad() << "{ " << group_scope(c);
         if (V.nt) o << "__builtin_nontemporal_store(" << v << ", &" << x.ref << ");";
         else o << x.ref << " = " << v << ";";
