@@ -331,6 +331,44 @@ def xgroups_study(args):
         out_shapes[key] = {"V_ms": round(v, 4), "P_ms": round(p, 4), "V+P_ms": round(float(np.median(tot)), 4),
                            "V+P_min_max": [round(min(tot), 4), round(max(tot), 4)]}
         log(f"acoustic3D {n}^3 {key}: V {v:.4f} ms, P {p:.4f} ms, V+P median {np.median(tot):.4f} ms (rounds {min(tot):.4f} - {max(tot):.4f})")
+    # 2-D: acoustic2D 4096^2 (miniapps/acoustic2D.jl), same rounds
+    n2 = 4096
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float32, 2, module="bench.x2d")
+    P2, prm2 = W.acoustic2d_inputs(n2)
+    a2 = (prm2["dt"], prm2["rho"], prm2["dx"], prm2["dy"])
+    b2 = (prm2["dt"], prm2["k"], prm2["dx"], prm2["dy"])
+    f2 = {}
+    for prec, elt in (("bf16", ps.BFloat16), ("f32", ps.Float32)):
+        F = (ps.zeros(n2, n2, eltype=elt), ps.zeros(n2 + 1, n2, eltype=elt), ps.zeros(n2, n2 + 1, eltype=elt))
+        F[0].copy_from(W._to_storage(P2, elt))
+        f2[prec] = F
+    c2 = [(prec, cx, w) for prec in ("bf16", "f32") for cx, w in ((0, 0), (1, 0), (2, 32), (2, 64), (4, 16), (4, 32), (4, 64))]
+    k2 = {c: W.define_acoustic2d("bench.x2d", False, cells_x=c[1], nthreads_x_groups=c[2]) for c in c2}
+    r2 = {c: [] for c in c2}
+    for r in range(rounds):
+        for c in (c2 if r % 2 == 0 else c2[::-1]):
+            kV, kP = k2[c]
+            P, Vx, Vy = f2[c[0]]
+            for _ in range(20):
+                kV(Vx, Vy, P, *a2, async_=True)
+                kP(P, Vx, Vy, *b2, async_=True)
+            e = [ps.Event() for _ in range(3)]
+            e[0].record()
+            for _ in range(burst):
+                kV(Vx, Vy, P, *a2, async_=True)
+            e[1].record()
+            for _ in range(burst):
+                kP(P, Vx, Vy, *b2, async_=True)
+            e[2].record()
+            ps.synchronize()
+            r2[c].append((e[0].elapsed_ms(e[1]) / burst, e[1].elapsed_ms(e[2]) / burst))
+    for c in c2:
+        prec, cx, w = c
+        key = f"2d {prec} " + ("default" if cx == 0 else f"cells_x={cx}" + (f" width={w}" if cx > 1 else ""))
+        tot = [x[0] + x[1] for x in r2[c]]
+        out_shapes[key] = {"V_ms": round(float(np.median([x[0] for x in r2[c]])), 4), "P_ms": round(float(np.median([x[1] for x in r2[c]])), 4),
+                           "V+P_ms": round(float(np.median(tot)), 4), "V+P_min_max": [round(min(tot), 4), round(max(tot), 4)]}
+        log(f"acoustic2D {n2}^2 {key}: V+P median {np.median(tot):.4f} ms (rounds {min(tot):.4f} - {max(tot):.4f})")
     ms16, ms32, ms64 = (out_shapes[f"{p} default"]["V+P_ms"] for p in ("bf16", "f32", "f64"))
     log(f"default shapes: BF16 {ms16:.4f} ms, FP32 {ms32:.4f} ms, FP64 {ms64:.4f} ms per step; BF16 / FP32 speed-up {ms32 / ms16:.2f}x")
     out = {"metric": "acoustic3D plain kernel time per step (V + P) by x-group launch shape", "value": ms16,

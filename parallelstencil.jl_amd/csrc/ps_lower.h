@@ -66,10 +66,11 @@ struct Options {
   // by other values through a refined hardware reciprocal (within a few ulp per operation)
   bool fastmath = false;
   int prefetch = 1;                       // memopt: loads are issued this many steps ahead
-  // plain kernels: adjacent x cells per thread (x groups); 0: automatic (by element size), 1: off
+  // plain kernels: adjacent x cells per thread (x groups); 0: automatic (4 for arrays of 2 and 4
+  // bytes, 1 for 8 bytes), 1: off
   int cells_x = 0;
   // plain kernels run as x groups: threads along x of the launched block (the block keeps its
-  // thread count); 0: automatic (128 cells along x per block row)
+  // thread count); 0: automatic (128 cells along x per block row in 3-D, 256 in 2-D)
   int nthreads_x_groups = 0;
   // plain kernels: 32-bit per-thread byte offsets on scalar bases where they fit (Variant::off32)
   bool addr32 = true;

@@ -1515,8 +1515,9 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
     }
     if (seen.size() <= 26) v.sclass = sc;
     // x groups (Variant::cx): with automatic geometry and a body of array assignments under
-    // x-interval masks, each thread takes cx adjacent x cells (by default 8 bytes of the narrowest
-    // array per thread: FP32 2 cells, BFloat16 4), so that their accesses merge into wide loads
+    // x-interval masks, each thread takes cx adjacent x cells (by default 4 when the narrowest
+    // array has 2- or 4-byte elements: 8 or 16 bytes per thread, whose accesses merge into wide
+    // loads; FP64 arrays keep one cell, where groups measured slower)
     bool cell = false;
     int64_t es = 8;
     for (int i = 0; i < nargs; ++i)
@@ -1526,18 +1527,19 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
       }
     if (ok && !user_geom && !cell && groupable_body(K)) {
       int cx = K.opt.cells_x;
-      if (cx == 0) cx = es >= 8 ? 1 : (int)(8 / es);
+      if (cx == 0) cx = es >= 8 ? 1 : 4;
       if (cx > 1 && len[0] >= 2 * (int64_t)cx) {
         v.cx = cx;
         v.off32 = true;
         // Block shape of the launch: the reference's thread count per block, bx threads along x,
-        // by default 128 cells along x per block row (BFloat16 4 cells per thread: 32 threads and
-        // two y rows per wavefront; FP32 2 cells: 64 threads). Measured on acoustic3D 384^3
-        // against 16 to 64 threads: the fastest, or within the run-to-run spread of it.
+        // by default 128 cells along x per block row for 3-D launches (4 cells per thread: 32
+        // threads, two y rows per wavefront) and 256 for 2-D ones (64 threads). Measured on
+        // acoustic3D 384^3 and acoustic2D 4096^2 over 1-8 cells per thread and 16-64 threads at
+        // the sustained clock: the fastest in FP32 and BFloat16.
         const int64_t ng = (len[0] + cx - 1) / cx;
         const int32_t txy = P.nthreads[0] * P.nthreads[1];
         auto pow2_ceil = [](int64_t n) { int64_t p = 1; while (p < n) p <<= 1; return p; };
-        int64_t bx = K.opt.nthreads_x_groups ? K.opt.nthreads_x_groups : std::max<int64_t>(1, 128 / cx);
+        int64_t bx = K.opt.nthreads_x_groups ? K.opt.nthreads_x_groups : std::max<int64_t>(1, (len[2] > 1 ? 128 : 256) / cx);
         bx = std::min<int64_t>({bx, (int64_t)txy, pow2_ceil(ng)});
         int64_t by = std::max<int64_t>(1, txy / bx);
         if (by > pow2_ceil(len[1])) {

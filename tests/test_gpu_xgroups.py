@@ -134,7 +134,7 @@ def test_x_group_row_split_edges(elt, mod, src, nf, cx, width):
 
 
 def test_x_groups_acoustic3d_bf16_against_oracle_run():
-    # the acoustic kernels pick x groups by themselves for FP32 (2 cells) and BFloat16 (4 cells);
+    # the acoustic kernels pick x groups of 4 cells by themselves for FP32 and BFloat16;
     # the reported geometry stays the reference's
     n, steps = 45, 20
     P0, prm = W.acoustic3d_inputs(n)
