@@ -1205,6 +1205,19 @@ std::string Emitter::group_scope(int c) const {
   return "const ps_int ix = ps_ix0 + " + std::to_string(c) + "; const ps_int ps_cx = " + std::to_string(c) + "; (void)ix; (void)ps_cx; ";
 }
 
+// The constant c of an x index of the form ix + c or ix - c (k unchanged otherwise).
+static void x_index_offset(const NodeP& e, long long& k) {
+  if (e->k == NK::Ident && e->s == kIdx[0]) {
+    k = 0;
+    return;
+  }
+  if (e->k == NK::Binary && (e->s == "+" || e->s == "-") && e->a.size() == 2) {
+    const NodeP &l = e->a[0], &r = e->a[1];
+    if (l->k == NK::Ident && l->s == kIdx[0] && r->k == NK::Num && r->is_int) k = e->s == "+" ? r->ival : -r->ival;
+    else if (e->s == "+" && r->k == NK::Ident && r->s == kIdx[0] && l->k == NK::Num && l->is_int) k = l->ival;
+  }
+}
+
 // Bounds on the x index that a mask (a conjunction of comparisons, x_interval) puts: for each
 // comparison of ix with an integer expression, the expression (adjusted for strictness) as a lower
 // or upper bound. False if a bound is not an integer.
@@ -2420,13 +2433,35 @@ Emitted Emitter::run() {
       for (auto& st : K.body->a)
         if (st->k == NK::If && !x_bounds(st->a[0], xlo, xhi)) split = false;
     if (split) {
+      // line phase: groups start where the first statement's left-hand side access (x offset
+      // k) falls on a 128-byte line of the narrowest arrays (A cells), so that a block row of
+      // groups covers whole lines of the rows (256-byte aligned); threads before the first
+      // interior group take none, and the interior cells before it are edge cells
+      long long k = 0;
+      for (auto& st : K.body->a) {
+        const NodeP* a = nullptr;
+        if (st->k == NK::Assign) a = &st;
+        else if (st->k == NK::If && !st->a[1]->a.empty()) a = &st->a[1]->a[0];
+        if (!a) continue;
+        const NodeP& lhs = (*a)->a[0];
+        if (lhs->a.size() > 1) x_index_offset(lhs->a[1], k);
+        break;
+      }
+      int es = 8;
+      for (auto& kd : kinds)
+        if (kd.is_field) es = std::min(es, dtype_bytes(kd.dtype));
+      const int A = 128 / es;
       o << "  if (ty >= ps_rly || tz >= ps_rlz) return;\n";
       o << "  const ps_int iy = ps_r0y + ty;\n  const ps_int iz = ps_r0z + tz;\n  (void)iy; (void)iz;\n";
       o << "  ps_int ps_xl = ps_r0x, ps_xh = ps_r0x + ps_rlx - 1;\n";
       for (auto& b : xlo) o << "  ps_xl = max(ps_xl, " << b << ");\n";
       for (auto& b : xhi) o << "  ps_xh = min(ps_xh, " << b << ");\n";
-      o << "  const ps_int ps_ng = ps_xh >= ps_xl ? (ps_xh - ps_xl + 1) / " << CX << " : 0;  // interior groups\n";
-      o << "  const ps_int ps_ngp = (ps_ng + (ps_int)blockDim.x - 1) / (ps_int)blockDim.x * (ps_int)blockDim.x;\n";
+      o << "  const ps_int ps_e0 = ps_xl - 1 + (" << k << ");  // element of the first interior cell's access\n";
+      o << "  const ps_int ps_g0 = ps_e0 - ((ps_e0 % " << A << ") + " << A << ") % " << A << " + 1 - (" << k << ");  // thread 0's first cell\n";
+      o << "  const ps_int ps_s = ps_xl > ps_g0 ? (ps_xl - ps_g0 + " << (CX - 1) << ") / " << CX << " : 0;  // threads without a group\n";
+      o << "  const ps_int ps_gs = ps_g0 + ps_s * " << CX << ";  // first cell of the interior groups\n";
+      o << "  const ps_int ps_ng = ps_xh >= ps_gs ? (ps_xh - ps_gs + 1) / " << CX << " : 0;  // interior groups\n";
+      o << "  const ps_int ps_ngp = (ps_s + ps_ng + (ps_int)blockDim.x - 1) / (ps_int)blockDim.x * (ps_int)blockDim.x;\n";
     } else if (CX > 1) {
       o << "  if (tx * " << CX << " >= ps_rlx || ty >= ps_rly || tz >= ps_rlz) return;\n";
       o << "  const ps_int ps_ix0 = ps_r0x + tx * " << CX << ";\n  const ps_int iy = ps_r0y + ty;\n  const ps_int iz = ps_r0z + tz;\n";
@@ -2507,18 +2542,18 @@ Emitted Emitter::run() {
       o << in << "}\n";
     };
     if (split) {
-      o << "  if (tx < ps_ng) {\n";
-      o << "    const ps_int ps_ix0 = ps_xl + tx * " << CX << ";\n";
+      o << "  if (tx >= ps_s && tx < ps_s + ps_ng) {\n";
+      o << "    const ps_int ps_ix0 = ps_g0 + tx * " << CX << ";\n";
       offsets("    ");
       group(2, "true");
-      // row ends: edge cell e of the row is the e-th cell left of ps_xl or right of the interior
-      // groups; the edge threads of the row take them in turn
+      // row ends: edge cell e of the row is the e-th cell left of the interior groups or right of
+      // them; the edge threads of the row take them in turn
       o << "  } else if (tx >= ps_ngp) {\n";
-      o << "    const ps_int ps_nl = ps_ng > 0 ? ps_xl - ps_r0x : ps_rlx;\n";
+      o << "    const ps_int ps_nl = ps_ng > 0 ? ps_gs - ps_r0x : ps_rlx;\n";
       o << "    const ps_int ps_ne = ps_rlx - ps_ng * " << CX << ";\n";
       o << "    const ps_int ps_es = (ps_int)gridDim.x * (ps_int)blockDim.x - ps_ngp;\n";
       o << "    for (ps_int ps_e = tx - ps_ngp; ps_e < ps_ne; ps_e += ps_es) {\n";
-      o << "      const ps_int ps_ix0 = ps_e < ps_nl ? ps_r0x + ps_e : ps_xl + ps_ng * " << CX << " + (ps_e - ps_nl);\n";
+      o << "      const ps_int ps_ix0 = ps_e < ps_nl ? ps_r0x + ps_e : ps_gs + ps_ng * " << CX << " + (ps_e - ps_nl);\n";
       offsets("      ");
       for (auto& st : K.body->a) {
         if (st->k == NK::Return) continue;

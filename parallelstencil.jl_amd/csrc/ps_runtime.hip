@@ -1552,9 +1552,11 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
         }
         P.lthreads[0] = (int32_t)bx;
         P.lthreads[1] = (int32_t)by;
-        // one block column more than the groups need: its threads take the cells at the row ends,
-        // outside the x interval in which every statement's mask holds (the kernel computes it)
-        P.lblocks[0] = (int32_t)((ng + bx - 1) / bx + 1);
+        // room for the threads before the first interior group (the groups start on a 128-byte
+        // line: up to 128 / es / cx of them) and one block column more: its threads take the cells
+        // at the row ends, outside the interior groups (the kernel computes both)
+        const int64_t lead = std::max<int64_t>(1, 128 / (es * cx));
+        P.lblocks[0] = (int32_t)((ng + lead + bx - 1) / bx + 1);
         P.lblocks[1] = (int32_t)((len[1] + by - 1) / by);
       }
     }

@@ -479,13 +479,64 @@ def test_x_group_row_split():
     for a in ("a0", "a1", "a2"):  # @inn of each velocity: 1 <= ix <= n_x - 2 in the kernel's indices
         assert f"ps_xh = min(ps_xh, (ps_int)(({a}_nx - ((ps_int)2))));" in src
     assert src.count("ps_xl = max(ps_xl, (ps_int)(((ps_int)1)));") == 3
-    assert "const ps_int ps_ix0 = ps_xl + tx * 4;" in src and "for (ps_int ps_e = tx - ps_ngp; ps_e < ps_ne; ps_e += ps_es)" in src
+    # groups start where Vx[ix+1, ...] (x offset 1) falls on a 128-byte line: 16 FP64 cells
+    assert "const ps_int ps_e0 = ps_xl - 1 + (1);" in src and "ps_g0 = ps_e0 - ((ps_e0 % 16) + 16) % 16 + 1 - (1);" in src
+    assert "const ps_int ps_ix0 = ps_g0 + tx * 4;" in src and "for (ps_int ps_e = tx - ps_ngp; ps_e < ps_ne; ps_e += ps_es)" in src
     strict = ps.parallel_indices("@parallel_indices (ix,iy,iz) function st!(A, B)\n if ix > 2 && 7 > ix\n A[ix,iy,iz] = B[ix,iy,iz]\n end\n return\nend")
     s2 = strict.source(ps.ArgSpec((n, n, n)), ps.ArgSpec((n, n, n)), off32=True, cells_x=2)
     assert "ps_xl = max(ps_xl, (ps_int)(((ps_int)2)) + 1);" in s2 and "ps_xh = min(ps_xh, (ps_int)(((ps_int)7)) - 1);" in s2
     half = ps.parallel_indices("@parallel_indices (ix,iy,iz) function hf!(A, B)\n if ix <= size(A, 1) / 2\n A[ix,iy,iz] = B[ix,iy,iz]\n end\n return\nend")
     s3 = half.source(ps.ArgSpec((n, n, n)), ps.ArgSpec((n, n, n)), off32=True, cells_x=2)
     assert "ps_ng" not in s3 and "const ps_int ps_ix0 = ps_r0x + tx * 2;" in s3
+
+
+def _row_split_cells(r0, rl, xl, xh, k, A, CX, bx, nthreads):
+    """The cells that the threads of one row take in the row split of x groups, as the emitted
+    prologue computes them: (interior groups, edge cells)."""
+    xl, xh = max(r0, xl), min(r0 + rl - 1, xh)  # the mask bounds within the launch range
+    e0 = xl - 1 + k
+    g0 = e0 - ((e0 % A) + A) % A + 1 - k
+    s = (xl - g0 + CX - 1) // CX if xl > g0 else 0
+    gs = g0 + s * CX
+    ng = (xh - gs + 1) // CX if xh >= gs else 0
+    ngp = (s + ng + bx - 1) // bx * bx
+    groups, edges = [], []
+    for tx in range(nthreads):
+        if s <= tx < s + ng:
+            groups.append(g0 + tx * CX)
+        elif tx >= ngp:
+            nl = gs - r0 if ng > 0 else rl
+            ne = rl - ng * CX
+            e = tx - ngp
+            while e < ne:
+                edges.append(r0 + e if e < nl else gs + ng * CX + (e - nl))
+                e += nthreads - ngp
+    return groups, edges, g0
+
+
+def test_x_group_row_split_covers_every_cell_once():
+    """Every cell of the launch range goes to exactly one thread of the row split, interior groups
+    lie inside the interval where all masks hold, and the groups' leading accesses start on
+    128-byte lines; the host's x grid leaves room for the leading threads and the edge column."""
+    import itertools
+    rng = np.random.default_rng(5)
+    for es, CX, bx in itertools.product((2, 4), (2, 4, 8), (8, 32, 64)):
+        A = 128 // es
+        for _ in range(40):
+            r0 = int(rng.integers(1, 6))
+            rl = int(rng.integers(2 * CX, 700))
+            xl = int(rng.integers(r0 - 3, r0 + 20))
+            xh = int(rng.integers(r0 + rl - 30, r0 + rl + 3))
+            k = int(rng.integers(-1, 3))
+            ng_host = (rl + CX - 1) // CX
+            lead = max(1, 128 // (es * CX))
+            nthreads = ((ng_host + lead + bx - 1) // bx + 1) * bx
+            groups, edges, g0 = _row_split_cells(r0, rl, xl, xh, k, A, CX, bx, nthreads)
+            cells = [g + c for g in groups for c in range(CX)] + edges
+            assert sorted(cells) == list(range(r0, r0 + rl)), (es, CX, bx, r0, rl, xl, xh, k)
+            for g in groups:
+                assert max(xl, r0) <= g and g + CX - 1 <= min(xh, r0 + rl - 1)
+            assert ((g0 - 1 + k) * es) % 128 == 0
 
 
 def test_quotient_checks_per_statement():
