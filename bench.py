@@ -297,10 +297,12 @@ def xgroups_study(args):
         F = (ps.zeros(n, n, n, eltype=elt), ps.zeros(n + 1, n, n, eltype=elt), ps.zeros(n, n + 1, n, eltype=elt), ps.zeros(n, n, n + 1, eltype=elt))
         F[0].copy_from(W._to_storage(P0, elt))
         fields[prec] = F
-    cfgs = [("f64", 0, 0, True)]
+    cfgs = [("f64", 0, 0, True)] + [("f64", cx, w, True) for cx, w in ((2, 64), (2, 32), (4, 32), (4, 16))]
+    cfgs += [(p, cx, w, True, cz) for p, cx, w in (("f64", 1, 0), ("f32", 4, 32), ("bf16", 4, 32)) for cz in (2, 4)]
     cfgs += [("bf16", 0, 0, True), ("bf16", 1, 0, True)] + [("bf16", cx, w, sp) for cx in (2, 4, 8) for w in (16, 32, 64) for sp in (False, True)]
     cfgs += [("f32", 0, 0, True), ("f32", 1, 0, True)] + [("f32", cx, w, sp) for cx in (2, 4) for w in (16, 32, 64) for sp in (False, True)]
-    kern = {c: W.define_acoustic3d("bench.x64" if c[0] == "f64" else "bench.x32", False, cells_x=c[1], nthreads_x_groups=c[2], xcd_spread=c[3])
+    kern = {c: W.define_acoustic3d("bench.x64" if c[0] == "f64" else "bench.x32", False, cells_x=c[1], nthreads_x_groups=c[2], xcd_spread=c[3],
+                                   cells_z=c[4] if len(c) > 4 else 1)
             for c in cfgs}
     res = {c: [] for c in cfgs}
     for r in range(rounds):
@@ -323,8 +325,11 @@ def xgroups_study(args):
         log(f"round {r + 1} of {rounds} done")
     out_shapes = {}
     for c in cfgs:
-        prec, cx, w, sp = c
+        prec, cx, w, sp = c[:4]
+        cz = c[4] if len(c) > 4 else 0
         key = f"{prec} " + ("default" if cx == 0 else f"cells_x={cx}" + (f" width={w} odd_grid={int(sp)}" if cx > 1 else ""))
+        if cz:
+            key = f"{prec} cells_x={cx} cells_z={cz}" + (f" width={w}" if cx > 1 else "")
         v = float(np.median([x[0] for x in res[c]]))
         p = float(np.median([x[1] for x in res[c]]))
         tot = [x[0] + x[1] for x in res[c]]

@@ -206,7 +206,8 @@ int ps_kernel_info(ps_kernel k, char* out, size_t cap);
  * (src/ParallelKernel/shared.jl:553-570). flags: bit0 memopt, bit1 checked, bit2 64-bit
  * indices, bit3 no __restrict__, bit4 1024-thread bound, bit5 LDS tiles (block shape in bits
  * 8..19 / 20..31), bit6 32-bit per-thread byte offsets and bit7 24-bit cell-offset multiplies
- * (plain kernels), bits 8..11 with bit6 and without bit5: x cells per thread (x groups). */
+ * (plain kernels), bits 8..11 and 12..15 with bit6 and without bit5: x and z cells per thread
+ * (groups of cells). */
 int ps_kernel_source(ps_kernel k, const ps_arg* args, int nargs, int flags, char* out, size_t cap);
 /* Compiles (or loads from the cache) the code object for the given argument kinds
  * without touching a GPU. Used to fill the kernel cache ahead of time. */

@@ -457,7 +457,12 @@ std::string Emitter::offset(int ai, const std::vector<NodeP>& idx, std::string i
       std::string kyz;
       if (c[1]) kyz += " + (" + std::to_string(c[1]) + ")*" + A + "_sx";
       if (c[2]) kyz += " + (" + std::to_string(c[2]) + ")*" + A + "_sxy";
-      if (V.cx > 1) *ref32 = "ps_at(" + A + ", (ps_int)0" + kyz + ", " + A + "_vo, (int)ps_cx + " + std::to_string(c[0]) + ")";
+      if (V.cz > 1) {  // the cell's z offset within its group joins the uniform part
+        kyz.clear();
+        if (c[1]) kyz += " + (" + std::to_string(c[1]) + ")*" + A + "_sx";
+        kyz += " + (ps_cz + (" + std::to_string(c[2]) + "))*" + A + "_sxy";
+      }
+      if (V.cx > 1 || V.cz > 1) *ref32 = "ps_at(" + A + ", (ps_int)0" + kyz + ", " + A + "_vo, (int)ps_cx + " + std::to_string(c[0]) + ")";
       else *ref32 = "ps_at(" + A + ", (ps_int)" + std::to_string(c[0]) + kyz + ", " + A + "_vo, 0)";
     }
     return "(" + off + k + ")";
@@ -1200,9 +1205,13 @@ std::string Emitter::store_value(int dt, const EV& r) const {
   return std::string("((") + elem_cty(dt) + ")(" + r.c + "))";
 }
 
-// Cell c of an x group: its x index and its x offset within the group, shadowing the group's.
-std::string Emitter::group_scope(int c) const {
-  return "const ps_int ix = ps_ix0 + " + std::to_string(c) + "; const ps_int ps_cx = " + std::to_string(c) + "; (void)ix; (void)ps_cx; ";
+// Cell q of a group (x offset q % cx, z offset q / cx): its x and z indices and its offsets
+// within the group, shadowing the group's.
+std::string Emitter::group_scope(int q) const {
+  const int c = q % V.cx, d = q / V.cx;
+  std::string r = "const ps_int ix = ps_ix0 + " + std::to_string(c) + "; const ps_int ps_cx = " + std::to_string(c) + "; (void)ix; (void)ps_cx; ";
+  if (V.cz > 1) r += "const ps_int iz = ps_iz0 + " + std::to_string(d) + "; const ps_int ps_cz = " + std::to_string(d) + "; (void)iz; (void)ps_cz; ";
+  return r;
 }
 
 // The constant c of an x index of the form ix + c or ix - c (k unchanged otherwise).
@@ -1218,7 +1227,7 @@ static void x_index_offset(const NodeP& e, long long& k) {
   }
 }
 
-// Bounds on the x index that a mask (a conjunction of comparisons, x_interval) puts: for each
+// Bounds on the x index that a mask (a conjunction of comparisons, idx_interval) puts: for each
 // comparison of ix with an integer expression, the expression (adjusted for strictness) as a lower
 // or upper bound. False if a bound is not an integer.
 bool Emitter::x_bounds(const NodeP& c, std::vector<std::string>& lo, std::vector<std::string>& hi) {
@@ -1274,7 +1283,7 @@ void Emitter::group_statements(const std::vector<NodeP>& sts) {
     bool nan_in_bad;  // the flag of the IEEE recomputation is set by any NaN value of the group
     int dt;
   };
-  const int CX = V.cx;
+  const int CX = V.cx * V.cz;  // cells of the group
   size_t i = 0;
   while (i < sts.size()) {
     std::set<std::string> written;
@@ -2414,11 +2423,13 @@ Emitted Emitter::run() {
   if (!K.shared_arrays.empty()) src << "extern __shared__ __attribute__((aligned(16))) unsigned char ps_dyn_lds[];\n";
   src << "extern \"C\" __global__ void __launch_bounds__(" << V.maxthreads << ") " << entry << "(\n" << sig.str() << ") {\n";
   if (!V.memopt) {
-    const int CX = V.cx;
-    if (CX > 1 && !groupable_body(K)) throw PsError(PS_EINTERNAL, "x groups requested for a kernel body that cannot run as groups");
-    // x groups: thread tx owns the CX adjacent cells from ps_ix0; the cell offsets below are the
-    // group's first cell's (each cell adds its x offset ps_cx to the uniform part of an access)
-    const char* X = CX > 1 ? "ps_ix0" : "ix";
+    const int CX = V.cx, CZ = V.cz;
+    const bool GR = CX > 1 || CZ > 1;  // groups of CX x CZ cells per thread
+    if (GR && !groupable_body(K, CZ > 1)) throw PsError(PS_EINTERNAL, "groups of cells requested for a kernel body that cannot run as groups");
+    // groups: thread (tx, tz) owns the CX x CZ cells from (ps_ix0, ps_iz0); the cell offsets below
+    // are the group's first cell's (each cell adds its x offset ps_cx as a constant element offset
+    // and its z offset ps_cz to the uniform part of an access)
+    const char* X = GR ? "ps_ix0" : "ix";
     o << "  const ps_int ps_bx = blockIdx.x, ps_by = blockIdx.y, ps_bz = blockIdx.z;\n";
     o << "  const ps_int tx = ps_bx * (ps_int)blockDim.x + (ps_int)threadIdx.x;\n";
     o << "  const ps_int ty = ps_by * (ps_int)blockDim.y + (ps_int)threadIdx.y;\n";
@@ -2428,8 +2439,8 @@ Emitted Emitter::run() {
     // row take in order; the cells outside it (the row ends) go one at a time to the threads of
     // the last block column, so that interior wavefronts never run the per-cell path
     std::vector<std::string> xlo, xhi;
-    bool split = CX > 1;
-    if (CX > 1)
+    bool split = GR;
+    if (GR)
       for (auto& st : K.body->a)
         if (st->k == NK::If && !x_bounds(st->a[0], xlo, xhi)) split = false;
     if (split) {
@@ -2451,8 +2462,8 @@ Emitted Emitter::run() {
       for (auto& kd : kinds)
         if (kd.is_field) es = std::min(es, dtype_bytes(kd.dtype));
       const int A = 128 / es;
-      o << "  if (ty >= ps_rly || tz >= ps_rlz) return;\n";
-      o << "  const ps_int iy = ps_r0y + ty;\n  const ps_int iz = ps_r0z + tz;\n  (void)iy; (void)iz;\n";
+      o << "  if (ty >= ps_rly || tz * " << CZ << " >= ps_rlz) return;\n";
+      o << "  const ps_int iy = ps_r0y + ty;\n  const ps_int ps_iz0 = ps_r0z + tz * " << CZ << ";\n  const ps_int iz = ps_iz0;\n  (void)iy; (void)iz;\n";
       o << "  ps_int ps_xl = ps_r0x, ps_xh = ps_r0x + ps_rlx - 1;\n";
       for (auto& b : xlo) o << "  ps_xl = max(ps_xl, " << b << ");\n";
       for (auto& b : xhi) o << "  ps_xh = min(ps_xh, " << b << ");\n";
@@ -2462,9 +2473,10 @@ Emitted Emitter::run() {
       o << "  const ps_int ps_gs = ps_g0 + ps_s * " << CX << ";  // first cell of the interior groups\n";
       o << "  const ps_int ps_ng = ps_xh >= ps_gs ? (ps_xh - ps_gs + 1) / " << CX << " : 0;  // interior groups\n";
       o << "  const ps_int ps_ngp = (ps_s + ps_ng + (ps_int)blockDim.x - 1) / (ps_int)blockDim.x * (ps_int)blockDim.x;\n";
-    } else if (CX > 1) {
-      o << "  if (tx * " << CX << " >= ps_rlx || ty >= ps_rly || tz >= ps_rlz) return;\n";
-      o << "  const ps_int ps_ix0 = ps_r0x + tx * " << CX << ";\n  const ps_int iy = ps_r0y + ty;\n  const ps_int iz = ps_r0z + tz;\n";
+    } else if (GR) {
+      o << "  if (tx * " << CX << " >= ps_rlx || ty >= ps_rly || tz * " << CZ << " >= ps_rlz) return;\n";
+      o << "  const ps_int ps_ix0 = ps_r0x + tx * " << CX << ";\n  const ps_int iy = ps_r0y + ty;\n";
+      o << "  const ps_int ps_iz0 = ps_r0z + tz * " << CZ << ";\n  const ps_int iz = ps_iz0;\n";
       o << "  (void)iy; (void)iz;\n";
     } else {
       o << "  if (tx >= ps_rlx || ty >= ps_rly || tz >= ps_rlz) return;\n";
@@ -2505,15 +2517,19 @@ Emitted Emitter::run() {
       o << "  (void)" << P << "; (void)" << P << "_n1; (void)" << P << "_n2; (void)" << P << "_n3;\n";
     }
     ind = 1;
-    // the group from ps_ix0: all statements for all cells in batches when the masks hold at the
-    // group's first and last cell (every mask is an x interval, groupable_body), else statement by
-    // statement, each for the group's cells in the launch range, in the order in which the lanes of
-    // a wavefront run one cell each (a trailing return would end the group after its first cell
-    // and is left out)
+    // the group from (ps_ix0, ps_iz0): all statements for all cells in batches when the masks hold
+    // at the group's corner cells (every mask is an interval in x and z, groupable_body), else
+    // statement by statement, each for the group's cells in the launch range, in the order in
+    // which the lanes of a wavefront run one cell each (a trailing return would end the group
+    // after its first cell and is left out)
+    const int NC = CX * CZ;
+    std::vector<int> corners{0};
+    for (int q : {CX - 1, (CZ - 1) * CX, NC - 1})
+      if (std::find(corners.begin(), corners.end(), q) == corners.end()) corners.push_back(q);
     auto group = [&](int b, const std::string& first) {
       const std::string in(2 * b, ' ');
       o << in << "bool ps_grp = " << first << ";\n";
-      for (int c : {0, CX - 1}) {
+      for (int c : corners) {
         std::string conds;
         for (auto& st : K.body->a)
           if (st->k == NK::If) conds += " && (" + expr(st->a[0]).c + ")";
@@ -2532,8 +2548,8 @@ Emitted Emitter::run() {
       o << in << "} else {\n";
       for (auto& st : K.body->a) {
         if (st->k == NK::Return) continue;
-        for (int c = 0; c < CX; ++c) {
-          o << in << "  { " << group_scope(c) << "if (ix < ps_r0x + ps_rlx) {\n";
+        for (int c = 0; c < NC; ++c) {
+          o << in << "  { " << group_scope(c) << "if (ix < ps_r0x + ps_rlx && iz < ps_r0z + ps_rlz) {\n";
           ind = b + 2;
           stmt(st);
           o << in << "  } }\n";
@@ -2545,7 +2561,7 @@ Emitted Emitter::run() {
       o << "  if (tx >= ps_s && tx < ps_s + ps_ng) {\n";
       o << "    const ps_int ps_ix0 = ps_g0 + tx * " << CX << ";\n";
       offsets("    ");
-      group(2, "true");
+      group(2, CZ > 1 ? "tz * " + std::to_string(CZ) + " + " + std::to_string(CZ - 1) + " < ps_rlz" : std::string("true"));
       // row ends: edge cell e of the row is the e-th cell left of the interior groups or right of
       // them; the edge threads of the row take them in turn
       o << "  } else if (tx >= ps_ngp) {\n";
@@ -2555,16 +2571,18 @@ Emitted Emitter::run() {
       o << "    for (ps_int ps_e = tx - ps_ngp; ps_e < ps_ne; ps_e += ps_es) {\n";
       o << "      const ps_int ps_ix0 = ps_e < ps_nl ? ps_r0x + ps_e : ps_gs + ps_ng * " << CX << " + (ps_e - ps_nl);\n";
       offsets("      ");
-      for (auto& st : K.body->a) {
-        if (st->k == NK::Return) continue;
-        o << "      { " << group_scope(0) << "\n";
-        ind = 4;
-        stmt(st);
-        o << "      }\n";
-      }
+      for (int d = 0; d < CZ; ++d)  // the edge cell's z cells, each with all statements
+        for (auto& st : K.body->a) {
+          if (st->k == NK::Return) continue;
+          o << "      { " << group_scope(d * CX) << (CZ > 1 ? "if (iz < ps_r0z + ps_rlz) " : "") << "{\n";
+          ind = 4;
+          stmt(st);
+          o << "      } }\n";
+        }
       o << "    }\n  }\n";
-    } else if (CX > 1) {
-      group(1, "tx * " + std::to_string(CX) + " + " + std::to_string(CX - 1) + " < ps_rlx");
+    } else if (GR) {
+      group(1, "tx * " + std::to_string(CX) + " + " + std::to_string(CX - 1) + " < ps_rlx && tz * " + std::to_string(CZ) + " + " +
+                   std::to_string(CZ - 1) + " < ps_rlz");
     } else {
       block(K.body);
     }
@@ -2614,26 +2632,27 @@ Emitted Emitter::run() {
   return E;
 }
 
-static bool uses_ix(const NodeP& n) {
+static bool uses_idx(const NodeP& n, int q) {
   if (!n) return false;
-  if (n->k == NK::Ident && n->s == kIdx[0]) return true;
+  if (n->k == NK::Ident && n->s == kIdx[q]) return true;
   for (auto& c : n->a)
-    if (uses_ix(c)) return true;
+    if (uses_idx(c, q)) return true;
   return false;
 }
 
-// A conjunction of comparisons in which the x index stands alone on one side of <, <=, > or >=.
-static bool x_interval(const NodeP& c) {
-  if (!uses_ix(c)) return true;
-  if (c->k == NK::And) return x_interval(c->a[0]) && x_interval(c->a[1]);
+// A conjunction of comparisons in which index q (0 x, 2 z) stands alone on one side of <, <=, >
+// or >= wherever it appears.
+static bool idx_interval(const NodeP& c, int q) {
+  if (!uses_idx(c, q)) return true;
+  if (c->k == NK::And) return idx_interval(c->a[0], q) && idx_interval(c->a[1], q);
   if (c->k != NK::Cmp) return false;
   for (size_t i = 0; i + 1 < c->a.size(); ++i) {
     const NodeP &l = c->a[i], &r = c->a[i + 1];
-    if (!uses_ix(l) && !uses_ix(r)) continue;
+    if (!uses_idx(l, q) && !uses_idx(r, q)) continue;
     const std::string& op = c->ops[i];
     if (op != "<" && op != "<=" && op != ">" && op != ">=") return false;
-    const bool lx = l->k == NK::Ident && l->s == kIdx[0], rx = r->k == NK::Ident && r->s == kIdx[0];
-    if (!((lx && !uses_ix(r)) || (rx && !uses_ix(l)))) return false;
+    const bool lx = l->k == NK::Ident && l->s == kIdx[q], rx = r->k == NK::Ident && r->s == kIdx[q];
+    if (!((lx && !uses_idx(r, q)) || (rx && !uses_idx(l, q)))) return false;
   }
   return true;
 }
@@ -2642,14 +2661,15 @@ static bool array_assign(const NodeP& s) {
   return s->k == NK::Assign && s->a[0]->k == NK::Index && s->a[0]->a[0]->k == NK::Ident;
 }
 
-bool groupable_body(const Kernel& K) {
+bool groupable_body(const Kernel& K, bool z) {
   if (K.kernel_language || !K.shared_arrays.empty() || !K.body || K.nidx < 1) return false;
+  if (z && K.nidx < 3) return false;
   const auto& st = K.body->a;
   for (size_t i = 0; i < st.size(); ++i) {
     const NodeP& s = st[i];
     if (array_assign(s)) continue;
     if (s->k == NK::Return && s->a.empty() && i + 1 == st.size()) continue;
-    if (s->k == NK::If && (s->a.size() < 3 || !s->a[2]) && x_interval(s->a[0])) {
+    if (s->k == NK::If && (s->a.size() < 3 || !s->a[2]) && idx_interval(s->a[0], 0) && (!z || idx_interval(s->a[0], 2))) {
       for (auto& t : s->a[1]->a)
         if (!array_assign(t)) return false;
       continue;

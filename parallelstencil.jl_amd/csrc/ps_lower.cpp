@@ -315,6 +315,10 @@ static void read_options(Options& o, const JVal& j) {
       o.cells_x = (int)v.n;
       if (!(o.cells_x == 0 || o.cells_x == 1 || o.cells_x == 2 || o.cells_x == 4 || o.cells_x == 8))
         throw PsError(PS_EKEYWORD, "cells_x must be 0 (automatic), 1, 2, 4 or 8");
+    } else if (k == "cells_z") {
+      o.cells_z = (int)v.n;
+      if (!(o.cells_z == 0 || o.cells_z == 1 || o.cells_z == 2 || o.cells_z == 4 || o.cells_z == 8))
+        throw PsError(PS_EKEYWORD, "cells_z must be 0 (automatic), 1, 2, 4 or 8");
     } else if (k == "nthreads_x_groups") {
       o.nthreads_x_groups = (int)v.n;
       if (!(o.nthreads_x_groups == 0 || (o.nthreads_x_groups >= 1 && o.nthreads_x_groups <= 1024 &&
@@ -1405,7 +1409,8 @@ std::string kernel_info_json(const Kernel& K) {
 
 std::string Variant::key() const {
   char b[96];
-  std::snprintf(b, sizeof b, "m%dc%di%da%dt%dl%d_%dx%dn%do%du%dg%d", memopt, checked, idx64, noalias, maxthreads, lds, bx, by, nt, off32, u24, cx);
+  std::snprintf(b, sizeof b, "m%dc%di%da%dt%dl%d_%dx%dn%do%du%dg%dz%d", memopt, checked, idx64, noalias, maxthreads, lds, bx, by, nt, off32, u24, cx,
+                cz);
   return std::string(b) + "d" + divmode + (padbits.empty() ? "" : "p" + padbits) + (sclass.empty() ? "" : "s" + sclass);
 }
 

@@ -72,6 +72,9 @@ struct Options {
   // plain kernels run as x groups: threads along x of the launched block (the block keeps its
   // thread count); 0: automatic (128 cells along x per block row in 3-D, 256 in 2-D)
   int nthreads_x_groups = 0;
+  // plain kernels run as groups: adjacent z cells per thread (a z plane that several of a thread's
+  // cells read is loaded once); 0: automatic (2 for arrays of 8 bytes, 1 otherwise), 1: off
+  int cells_z = 0;
   // plain kernels: 32-bit per-thread byte offsets on scalar bases where they fit (Variant::off32)
   bool addr32 = true;
   bool lds = true;                        // memopt: stage x-y neighbours of optvars through LDS
@@ -134,6 +137,8 @@ struct Variant {
   std::string sclass;
   // plain kernels: adjacent x cells per thread (1: one cell per thread)
   int cx = 1;
+  // plain kernels: adjacent z cells per thread (groups of cx x cz cells; 1: one z cell)
+  int cz = 1;
   int bx = 0, by = 0;     // block shape baked into LDS variants
   // per argument: 'p' / 'n' a positive / negative float scalar accepted as reciprocal divisor,
   // 'x' a float scalar that must be divided by exactly, '-' anything else
@@ -197,10 +202,11 @@ NodeP canonical_indices(const NodeP& n);
 // updates); it raises an ArgumentError where it does not apply.
 Kernel derivative_kernel(const Kernel& k, const JVal& spec);
 Emitted lower_kernel(const Kernel& k, const std::vector<ArgKind>& kinds, const Variant& v);
-// Whether a plain kernel's body can run as x groups (Variant::cx > 1): a sequence of array
-// assignments, each optionally under an if without else whose condition bounds the x index by
-// an interval; no local variables, loops, kernel language or atomic updates.
-bool groupable_body(const Kernel& k);
+// Whether a plain kernel's body can run as groups of cells (Variant::cx, Variant::cz): a sequence
+// of array assignments, each optionally under an if without else whose condition bounds the x
+// index (and, for z groups, the z index) by an interval; no local variables, loops, kernel
+// language or atomic updates.
+bool groupable_body(const Kernel& k, bool z = false);
 std::string kernel_info_json(const Kernel& k);
 
 // Bounds proof: true if every array access of the kernel stays inside its

@@ -1231,8 +1231,9 @@ static Variant variant_from_flags(const Kernel& K, int flags) {
   v.nt = K.opt.nt_store == 1 || (K.opt.nt_store == -1 && v.lds && v.bx >= 256);
   v.off32 = (flags & 64) != 0 && !v.memopt && !v.checked;  // 32-bit per-thread byte offsets
   v.u24 = v.off32 && (flags & 128) != 0;                      // 24-bit cell-offset multiplies
-  if (v.off32 && !v.lds && ((flags >> 8) & 0xf) > 1 && groupable_body(K))  // x groups: cells per thread in bits 8..11
-    v.cx = (flags >> 8) & 0xf;
+  // groups: x cells per thread in bits 8..11, z cells in bits 12..15
+  if (v.off32 && !v.lds && ((flags >> 8) & 0xf) > 1 && groupable_body(K)) v.cx = (flags >> 8) & 0xf;
+  if (v.off32 && !v.lds && ((flags >> 12) & 0xf) > 1 && groupable_body(K, true)) v.cz = (flags >> 12) & 0xf;
   return v;
 }
 
@@ -1528,8 +1529,17 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
     if (ok && !user_geom && !cell && groupable_body(K)) {
       int cx = K.opt.cells_x;
       if (cx == 0) cx = es >= 8 ? 1 : 4;
-      if (cx > 1 && len[0] >= 2 * (int64_t)cx) {
+      if (len[0] < 2 * (int64_t)cx) cx = 1;
+      // z cells per thread: a z plane that several of the thread's cells read comes from one load.
+      // By default 2 for FP64 arrays (one x cell) in rows of at least 256 cells, none otherwise:
+      // measured at the sustained clock, acoustic3D 384^3 FP64 6% faster per step (FP32 and
+      // BFloat16 10-15% slower), Stokes3D 127^3 FP64 4% slower (its rows are 127 cells: the
+      // edge block column and the threads before the first interior group double the waves)
+      int cz = K.opt.cells_z == 0 ? (es >= 8 && cx == 1 && len[0] >= 256 ? 2 : 1) : K.opt.cells_z;
+      if (cz > 1 && (P.nthreads[2] != 1 || len[2] < 2 * (int64_t)cz || !groupable_body(K, true))) cz = 1;
+      if (cx > 1 || cz > 1) {
         v.cx = cx;
+        v.cz = cz;
         v.off32 = true;
         // Block shape of the launch: the reference's thread count per block, bx threads along x,
         // by default 128 cells along x per block row for 3-D launches (4 cells per thread: 32
@@ -1539,7 +1549,8 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
         const int64_t ng = (len[0] + cx - 1) / cx;
         const int32_t txy = P.nthreads[0] * P.nthreads[1];
         auto pow2_ceil = [](int64_t n) { int64_t p = 1; while (p < n) p <<= 1; return p; };
-        int64_t bx = K.opt.nthreads_x_groups ? K.opt.nthreads_x_groups : std::max<int64_t>(1, (len[2] > 1 ? 128 : 256) / cx);
+        int64_t bx = K.opt.nthreads_x_groups ? K.opt.nthreads_x_groups
+                                             : cx > 1 ? std::max<int64_t>(1, (len[2] > 1 ? 128 : 256) / cx) : (int64_t)P.nthreads[0];
         bx = std::min<int64_t>({bx, (int64_t)txy, pow2_ceil(ng)});
         int64_t by = std::max<int64_t>(1, txy / bx);
         if (by > pow2_ceil(len[1])) {
@@ -1558,6 +1569,7 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
         const int64_t lead = std::max<int64_t>(1, 128 / (es * cx));
         P.lblocks[0] = (int32_t)((ng + lead + bx - 1) / bx + 1);
         P.lblocks[1] = (int32_t)((len[1] + by - 1) / by);
+        if (cz > 1) P.lblocks[2] = (int32_t)((len[2] + cz - 1) / cz);  // blocks of one z slab (nthreads[2] == 1)
       }
     }
   }
@@ -1571,7 +1583,7 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
   // the reference's grid (diffusion3D 512^3 FP64, 8 columns, ran 3% slower with 9), and so do
   // memopt launches (diffusion3D 512^3: 2 columns, 3% slower with 3, and 2-5% slower with the
   // blocks of each XCD taken from one contiguous eighth of the grid, xcd_remap).
-  if (v.cx > 1 && K.opt.xcd_spread && P.lblocks[2] > 1 && P.lblocks[0] % 2 == 0) P.lblocks[0] += 1;
+  if ((v.cx > 1 || v.cz > 1) && K.opt.xcd_spread && P.lblocks[2] > 1 && P.lblocks[0] % 2 == 0) P.lblocks[0] += 1;
   P.v = v;
   return P;
 }

@@ -1132,15 +1132,16 @@ class Kernel:
                                          ctypes.cast(rg, ctypes.POINTER(ctypes.c_int64)), ctypes.byref(lds)))
         return Geometry(bool(m & 1), bool(m & 2), tuple(nt), tuple(nb), tuple((rg[d][0], rg[d][1]) for d in range(3)))
 
-    def source(self, *args, memopt=None, checked=False, idx64=False, off32=False, cells_x=1):
+    def source(self, *args, memopt=None, checked=False, idx64=False, off32=False, cells_x=1, cells_z=1):
         """The generated HIP source for these argument kinds (plain kernels: with `off32`, 32-bit
-        per-thread byte offsets and 24-bit offset multiplies, and `cells_x` x cells per thread)."""
+        per-thread byte offsets and 24-bit offset multiplies, and groups of `cells_x` x `cells_z`
+        cells per thread)."""
         k, args = self._resolve(args)
         arr = k._args(args)
         mo = k.info["memopt"] if memopt is None else memopt
         flags = (1 if mo else 0) | (2 if checked else 0) | (4 if idx64 else 0)
         if off32 and not mo:
-            flags |= 64 | 128 | ((cells_x & 0xF) << 8 if cells_x > 1 else 0)
+            flags |= 64 | 128 | ((cells_x & 0xF) << 8 if cells_x > 1 else 0) | ((cells_z & 0xF) << 12 if cells_z > 1 else 0)
         buf = ctypes.create_string_buffer(1 << 22)
         check(lib.ps_kernel_source(k._h, arr, len(args), flags, buf, len(buf)))
         return buf.value.decode()
@@ -1208,7 +1209,7 @@ class _DerivativeKernel(Kernel):
 
 _KW_DEF = ("ndims", "N", "inbounds", "padding", "memopt", "optvars", "loopdim", "loopsize", "optranges", "useshmemhalos",
            "optimize_halo_read", "fp_contract", "fastmath", "prefetch", "lds", "xcd_remap", "nt_store", "nthreads_max_memopt",
-           "nthreads_x_max_memopt", "cells_x", "nthreads_x_groups", "xcd_spread", "addr32")
+           "nthreads_x_max_memopt", "cells_x", "cells_z", "nthreads_x_groups", "xcd_spread", "addr32")
 
 
 _PREFIX_RE = re.compile(r"^(\s*@parallel(?:_indices)?\b\s*(?:\([^()]*\)|\w+\.\.\.)?)((?:\s*\w+\s*=\s*(?:\([^()]*\)|[^\s(]+))*)", re.S)
@@ -1333,7 +1334,7 @@ def _define(kind, text, kw, mod, indices=None):
     if indices is not None:
         opts["indices"] = list(indices)
     for k in ("optvars", "loopdim", "loopsize", "fp_contract", "fastmath", "prefetch", "lds", "xcd_remap", "nt_store", "nthreads_max_memopt",
-              "nthreads_x_max_memopt", "cells_x", "nthreads_x_groups", "xcd_spread", "addr32"):
+              "nthreads_x_max_memopt", "cells_x", "cells_z", "nthreads_x_groups", "xcd_spread", "addr32"):
         if k in kw:
             v = kw[k]
             opts[k] = list(v) if k == "optvars" and isinstance(v, (tuple, list)) else v

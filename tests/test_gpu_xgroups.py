@@ -133,6 +133,48 @@ def test_x_group_row_split_edges(elt, mod, src, nf, cx, width):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (elt, cx, width)
 
 
+@pytest.mark.parametrize("elt,mod", [(ps.Float64, M64), (ps.Float32, M32), (ps.BFloat16, M32)])
+@pytest.mark.parametrize("shape", [(130, 7, 6), (67, 5, 9), (33, 8, 11)])
+@pytest.mark.parametrize("cx,cz", [(1, 2), (4, 2), (2, 4), (8, 2), (1, 8)])
+def test_z_groups_match_one_cell_per_thread(elt, mod, shape, cx, cz):
+    # groups of cx x cz cells: z extents that cz does not divide, z neighbours (@d2_zi) shared
+    # between a thread's cells
+    rng = np.random.default_rng(sum(shape) + 10 * cx + cz)
+    ref = ps.core._define("parallel", STENCIL, {"cells_x": 1, "cells_z": 1}, mod)
+    grp = ps.core._define("parallel", STENCIL, {"cells_x": cx, "cells_z": cz}, mod)
+    f1 = _fields(shape, elt, rng)
+    f2 = [(a, ps.zeros(*shape, eltype=elt)) for a, _ in f1]
+    for (_, d1), (_, d2) in zip(f1, f2):
+        d2.copy_from(d1.to_host())
+    for it in range(3):
+        ref(*[d for _, d in f1], 0.37, 1e-3 if it == 1 else 0.9)
+        grp(*[d for _, d in f2], 0.37, 1e-3 if it == 1 else 0.9)
+    for (_, d1), (_, d2) in zip(f1, f2):
+        a, b = np.ascontiguousarray(d1.to_host()), np.ascontiguousarray(d2.to_host())
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (elt, shape, cx, cz)
+
+
+@pytest.mark.parametrize("elt,mod", [(ps.Float32, M32), (ps.BFloat16, M32)])
+@pytest.mark.parametrize("src,nf", [(EDGES, 3), (NO_INTERIOR, 4)])
+@pytest.mark.parametrize("cz", [2, 4])
+def test_z_group_row_split_edges(elt, mod, src, nf, cz):
+    # row ends of groups with several z cells: each edge thread takes its cell's z cells
+    shape = (300, 40, 7)
+    rng = np.random.default_rng(cz + nf)
+    ref = ps.core._define("parallel_indices", src, {"cells_x": 1, "cells_z": 1}, mod)
+    grp = ps.core._define("parallel_indices", src, {"cells_x": 4, "cells_z": cz, "nthreads_x_groups": 8}, mod)
+    f1 = _fields(shape, elt, rng)[:nf]
+    f2 = [(a, ps.zeros(*shape, eltype=elt)) for a, _ in f1]
+    for (_, d1), (_, d2) in zip(f1, f2):
+        d2.copy_from(d1.to_host())
+    box = ((2, 299), (1, 40), (2, 7))
+    ref(*[d for _, d in f1], 0.37, ranges=box)
+    grp(*[d for _, d in f2], 0.37, ranges=box)
+    for (_, d1), (_, d2) in zip(f1, f2):
+        a, b = np.ascontiguousarray(d1.to_host()), np.ascontiguousarray(d2.to_host())
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (elt, cz)
+
+
 def test_x_groups_acoustic3d_bf16_against_oracle_run():
     # the acoustic kernels pick x groups of 4 cells by themselves for FP32 and BFloat16;
     # the reported geometry stays the reference's
