@@ -302,7 +302,7 @@ def xgroups_study(args):
     cfgs += [("bf16", 0, 0, True), ("bf16", 1, 0, True)] + [("bf16", cx, w, sp) for cx in (2, 4, 8) for w in (16, 32, 64) for sp in (False, True)]
     cfgs += [("f32", 0, 0, True), ("f32", 1, 0, True)] + [("f32", cx, w, sp) for cx in (2, 4) for w in (16, 32, 64) for sp in (False, True)]
     kern = {c: W.define_acoustic3d("bench.x64" if c[0] == "f64" else "bench.x32", False, cells_x=c[1], nthreads_x_groups=c[2], xcd_spread=c[3],
-                                   cells_z=c[4] if len(c) > 4 else 1)
+                                   cells_z=c[4] if len(c) > 4 else 0 if c[1] == 0 else 1)
             for c in cfgs}
     res = {c: [] for c in cfgs}
     for r in range(rounds):
