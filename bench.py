@@ -149,25 +149,43 @@ def stokes_warm_start(args):
     tsolver = Stokes2DFused(mod, nt, nt, eps=1e-8, nout=200, eta_ts=args.stokes_ts, history=True)
     t0 = time.perf_counter()
     data, stats = [], []
-    for i0 in range(1, args.train + 1, 64):
+    for i0 in range(1, (0 if args.stokes_load else args.train) + 1, 64):
         d, st = U.generate_samples(tsolver, range(i0, min(args.train, i0 + 63) + 1), contrast)
         data += d
         stats += st
         log(f"stokes {nt}^2: {len(stats)} training solves, {time.perf_counter() - t0:.1f} s, last {st[-1][0]} iterations")
     t1 = time.perf_counter()
-    ok = [s_[2] <= tsolver.eps for s_ in stats]
-    data = [d for d, k in zip(data, ok) if k]  # samples on which the plain iteration did not converge are dropped
-    its = [s_[0] for s_, k in zip(stats, ok) if k]
-    log(f"stokes {nt}^2: {len(stats)} training solves in {t1 - t0:.1f} s, {sum(ok)} converged, mean {np.mean(its):.0f} PT iterations, "
-        f"{1e6 * (t1 - t0) / max(1, sum(s_[0] for s_ in stats)):.2f} us per iteration")
-    ws = U.WarmStart(nt, nt, base=32, seed=0)
-    X, T, Mk = U.dataset(ws, data)
-    ws.set_normalisation(T, Mk)
-    del data
-    hist = ws.train(X, T, Mk, epochs=args.epochs, batch=8, lr=3e-3, log=log, cosine=True)
-    ps.synchronize()
-    t2 = time.perf_counter()
-    log(f"training: {args.epochs} epochs in {t2 - t1:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    if args.stokes_load:
+        ws = U.WarmStart.load(args.stokes_load)
+        if (ws.nx, ws.ny) != (nt, nt):
+            raise SystemExit(f"{args.stokes_load} holds a warm start for {ws.nx}x{ws.ny}, not the training resolution {nt}")
+        m = ws.meta
+        ok, its, hist, t2, n_train = [True] * m.get("train_solves_converged", 0), [0], [float("nan")], t1, m.get("train_samples", 0)
+        n_solves = m.get("train_solves", 0)
+        args.epochs, args.stokes_mirror = m.get("epochs"), m.get("mirrored", args.stokes_mirror)
+        log(f"warm start loaded from {args.stokes_load}: {m}")
+    else:
+        n_solves = len(stats)
+        ok = [s_[2] <= tsolver.eps for s_ in stats]
+        data = [d for d, k in zip(data, ok) if k]  # samples on which the plain iteration did not converge are dropped
+        its = [s_[0] for s_, k in zip(stats, ok) if k]
+        log(f"stokes {nt}^2: {len(stats)} training solves in {t1 - t0:.1f} s, {sum(ok)} converged, mean {np.mean(its):.0f} PT iterations, "
+            f"{1e6 * (t1 - t0) / max(1, sum(s_[0] for s_ in stats)):.2f} us per iteration")
+        if args.stokes_mirror:  # the problem's mirror symmetry in x doubles the training set
+            data = data + [U.mirror_x(d) for d in data]
+        n_train = len(data)
+        ws = U.WarmStart(nt, nt, base=32, seed=0)
+        X, T, Mk = U.dataset(ws, data)
+        ws.set_normalisation(T, Mk)
+        del data
+        hist = ws.train(X, T, Mk, epochs=args.epochs, batch=8, lr=3e-3, log=log, cosine=True)
+        ps.synchronize()
+        t2 = time.perf_counter()
+        log(f"training: {n_train} samples, {args.epochs} epochs in {t2 - t1:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+        if args.stokes_save:
+            ws.save(args.stokes_save, dict(train_samples=n_train, train_solves_converged=sum(ok), train_solves=len(stats), epochs=args.epochs,
+                                           mirrored=args.stokes_mirror, contrast_x10=round(10 * contrast)))
+            log(f"warm start saved to {args.stokes_save}")
     evals = {}
     sizes = []
     todo = [(n, args.eval), (args.stokes_n2, args.eval2)]
@@ -179,7 +197,7 @@ def stokes_warm_start(args):
         sv = tsolver if m == nt else Stokes2DFused(mod, m, m, eps=1e-8, nout=200, eta_ts=args.stokes_ts, history=True)
         sizes.append((m, ne, sv, None if m == nt else U.resampled_predictor(ws, nt, m)))
     for m, ne, sv, pred in sizes:
-        ev_all = U.evaluate(ws, sv, range(100001, 100001 + ne), contrast, predict=pred)
+        ev_all = U.evaluate(ws, sv, range(100001, 100001 + ne), contrast, predict=pred, log=log)
         ev = [e for e in ev_all if e["err_cold"] <= sv.eps]  # held-out samples the plain iteration solves
         cold_s, warm_s = sum(e["cold_s"] for e in ev), sum(e["warm_s"] for e in ev)
         cold_it, warm_it = np.mean([e["cold_iters"] for e in ev]), np.mean([e["warm_iters"] for e in ev])
@@ -201,10 +219,11 @@ def stokes_warm_start(args):
         "config": {"workload": f"Stokes2D {n}x{n} pseudo-transient to residual 1e-8, 1-8 inclusions, log10 contrast in "
                                f"[-{contrast:g}, {contrast:g}], pseudo-time steps {args.stokes_ts}; U-Net 4 levels, 32 base channels, BF16, "
                                f"inputs log10 viscosity and body force, trained at {nt}x{nt}",
-                   "train_samples": len(X), "train_n": nt, "eval_samples": {str(k): v["samples"] for k, v in evals.items()},
-                   "epochs": args.epochs},
+                   "train_samples": n_train, "train_solves_converged": sum(ok), "train_mirrored": bool(args.stokes_mirror),
+                   "train_n": nt, "eval_samples": {str(k): v["samples"] for k, v in evals.items()}, "epochs": args.epochs,
+                   "warm_start_loaded_from": args.stokes_load or None},
         "stokes": {"datagen_s": t1 - t0, "train_s": t2 - t1, "loss_first": hist[0], "loss_last": hist[-1],
-                   "train_dropped_not_converged": len(stats) - len(X), "train_iters_mean": float(np.mean(its)),
+                   "train_dropped_not_converged": n_solves - sum(ok), "train_iters_mean": float(np.mean(its)) if stats else None,
                    "eval": {str(k): {kk: vv for kk, vv in v.items() if kk != "per_sample"} for k, v in evals.items()}},
     }
     if args.stokes_out:
@@ -898,6 +917,9 @@ def main():
     ap.add_argument("--stokes-contrast", type=float, default=2.0, help="stokes: log10 viscosity contrast range [-c, c]")
     ap.add_argument("--stokes-ts", default="maxV", help="stokes: pseudo-time steps 'av' (miniapp) or 'maxV'")
     ap.add_argument("--stokes-out", default="", help="stokes: JSON file for per-sample results and residual histories")
+    ap.add_argument("--stokes-save", default="", help="stokes: write the trained warm start (.npz)")
+    ap.add_argument("--stokes-load", default="", help="stokes: evaluate the warm start in this .npz instead of training one")
+    ap.add_argument("--stokes-mirror", type=int, default=1, help="stokes: 1 adds the x-mirrored training samples")
     ap.add_argument("--train", type=int, default=256, help="stokes: training samples")
     ap.add_argument("--eval", type=int, default=24, help="stokes: held-out samples at --stokes-n")
     ap.add_argument("--eval2", type=int, default=4, help="stokes: held-out samples at --stokes-n2")

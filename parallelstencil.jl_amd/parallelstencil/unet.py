@@ -241,6 +241,7 @@ class WarmStart:
         self.net = UNet(base=base, cin=2, seed=seed)
         self.mu = np.zeros(3, np.float32)
         self.sd = np.ones(3, np.float32)
+        self.meta = {}
 
     def _plane(self, A):
         """Fortran field A[ix, iy] -> padded [H, W] plane (edge replication)."""
@@ -260,6 +261,34 @@ class WarmStart:
         m[:Vy.shape[1], :Vy.shape[0], 1] = 1
         m[:Pt.shape[1], :Pt.shape[0], 2] = 1
         return t, m
+
+    def save(self, path, meta=None):
+        """Writes the grid, the FP32 master weights, the target normalisation and the integers of
+        `meta` (a dict, e.g. how it was trained) to `path` (.npz)."""
+        arrs = {"grid": np.array([self.nx, self.ny, self.net.c[0]], np.int64), "mu": self.mu, "sd": self.sd}
+        for k, v in (meta or {}).items():
+            arrs["meta_" + k] = np.array(int(v), np.int64)
+        for i, (w, b) in enumerate(self.net.weights()):
+            arrs[f"w{i}"], arrs[f"b{i}"] = w, b
+        np.savez(path, **arrs)
+
+    @classmethod
+    def load(cls, path):
+        """A warm start with the weights and normalisation that `save` wrote."""
+        with np.load(path) as z:  # plain arrays only (allow_pickle stays off)
+            nx, ny, base = (int(v) for v in z["grid"])
+            ws = cls(nx, ny, base=base)
+            for i, layer in enumerate(ws.net.layers()):
+                w, b = z[f"w{i}"], z[f"b{i}"]
+                if w.shape != layer.w.shape or b.shape != layer.b.shape:
+                    raise ValueError(f"{path}: layer {i} has shape {w.shape}, the network expects {layer.w.shape}")
+                layer.w.upload(w)
+                layer.w16.upload(w)
+                layer.b.upload(b)
+            ws.mu[:] = z["mu"]
+            ws.sd[:] = z["sd"]
+            ws.meta = {k[5:]: int(z[k]) for k in z.files if k.startswith("meta_")}
+        return ws
 
     def set_normalisation(self, T, Mk):
         for c in range(3):
@@ -342,6 +371,15 @@ def generate_samples(solver, seeds, log10_contrast=1.0):
     return data, stats
 
 
+def mirror_x(sample):
+    """A sample (Mus, Rog, Vx, Vy, Pt) mirrored in x: the miniapp's problem is symmetric about
+    x = lx/2 (free-slip walls, the body force along y), so the mirrored fields solve the mirrored
+    inclusion setup, with Vx (on the x faces) changing sign."""
+    Mus, Rog, Vx, Vy, Pt = sample
+    f = lambda A: np.asfortranarray(np.asarray(A)[::-1, :])  # noqa: E731
+    return f(Mus), f(Rog), np.asfortranarray(-np.asarray(Vx)[::-1, :]), f(Vy), f(Pt)
+
+
 def dataset(ws, data):
     X = np.stack([ws.inputs(d[0], d[1]) for d in data]).astype(np.float32)
     TM = [ws.targets(d[2], d[3], d[4]) for d in data]
@@ -396,18 +434,21 @@ def _pred_err(pred, sol):
     return out
 
 
-def evaluate(ws, solver, seeds, log10_contrast=1.0, predict=None):
+def evaluate(ws, solver, seeds, log10_contrast=1.0, predict=None, log=None):
     """Per held-out seed: (cold iterations, cold seconds, warm iterations, warm seconds including
     inference and the host/device copies of the prediction, final residuals, relative L2 error of
     the prediction against the converged cold solution per field, pressure mean-free); with a
     solver that records its residual history, the cold and warm histories (one residual per
     check). `predict` maps (viscosity, body force) to (Vx, Vy, Pt) (default: the network at the
-    solver's resolution)."""
+    solver's resolution). A seed whose plain iteration does not converge has no reference to
+    compare with: it is recorded with its cold run only (err_cold above the tolerance). `log`
+    receives one line per seed."""
     predict = predict or ws.predict
     import time
     from .workloads import stokes2d_inclusions
     out = []
-    for s in seeds:
+    seeds = list(seeds)
+    for k, s in enumerate(seeds):
         Mus, Rog = stokes2d_inclusions(solver.nx, solver.ny, s, log10_contrast=log10_contrast)
         solver.set_material(Mus, Rog)
         solver.reset()
@@ -416,6 +457,11 @@ def evaluate(ws, solver, seeds, log10_contrast=1.0, predict=None):
         itc, errc = solver.solve()
         t1 = time.perf_counter()
         hist_cold = list(getattr(solver, "history", None) or [])
+        if not errc <= solver.eps:
+            out.append(dict(seed=s, cold_iters=itc, cold_s=t1 - t0, err_cold=errc))
+            if log:
+                log(f"held-out {solver.nx}^2 {k + 1}/{len(seeds)} (seed {s}): plain iteration not converged after {itc} iterations")
+            continue
         sol = (solver.Vx.to_host(), solver.Vy.to_host(), solver.Pt.to_host())
         solver.reset()
         core.synchronize()
@@ -444,4 +490,7 @@ def evaluate(ws, solver, seeds, log10_contrast=1.0, predict=None):
             rec["hist_cold"] = [float(f"{e:.4e}") for e in hist_cold]
             rec["hist_warm"] = [float(f"{e:.4e}") for e in hist_warm]
         out.append(rec)
+        if log:
+            log(f"held-out {solver.nx}^2 {k + 1}/{len(seeds)} (seed {s}): cold {itc} iterations / {t1 - t0:.3f} s, "
+                f"warm {itw} iterations / {t3 - t2:.3f} s")
     return out
