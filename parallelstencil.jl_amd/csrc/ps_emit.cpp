@@ -2679,6 +2679,36 @@ bool groupable_body(const Kernel& K, bool z) {
   return true;
 }
 
+// Largest spread of z offsets (max - min of c in an index iz + c) with which the kernel body
+// accesses one array: the number of z planes beyond its own that a cell of that array reads.
+int z_span(const Kernel& K) {
+  std::map<std::string, std::pair<long long, long long>> r;
+  std::function<void(const NodeP&)> walk = [&](const NodeP& n) {
+    if (!n) return;
+    if (n->k == NK::Index && n->a.size() >= 4 && n->a[0]->k == NK::Ident) {
+      const NodeP& e = n->a[3];
+      long long c = 0;
+      bool ok = false;
+      if (e->k == NK::Ident && e->s == kIdx[2]) ok = true;
+      else if (e->k == NK::Binary && (e->s == "+" || e->s == "-") && e->a.size() == 2 && e->a[0]->k == NK::Ident && e->a[0]->s == kIdx[2] &&
+               e->a[1]->k == NK::Num && e->a[1]->is_int) {
+        ok = true;
+        c = e->s == "+" ? e->a[1]->ival : -e->a[1]->ival;
+      }
+      if (ok) {
+        auto it = r.find(n->a[0]->s);
+        if (it == r.end()) r[n->a[0]->s] = {c, c};
+        else it->second = {std::min(it->second.first, c), std::max(it->second.second, c)};
+      }
+    }
+    for (auto& a : n->a) walk(a);
+  };
+  walk(K.body);
+  long long span = 0;
+  for (auto& kv : r) span = std::max(span, kv.second.second - kv.second.first);
+  return (int)span;
+}
+
 Emitted lower_kernel(const Kernel& k, const std::vector<ArgKind>& kinds, const Variant& v) {
   Emitter e(k, kinds, v);
   return e.run();

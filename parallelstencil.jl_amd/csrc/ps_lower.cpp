@@ -1336,7 +1336,8 @@ std::string kernel_info_json(const Kernel& K) {
          for (size_t i = 0; i < K.def_params.size(); ++i) r += (i ? "," : "") + json_str(K.def_params[i]);
          return r;
        }() << "]"
-    << ",\"numbertype\":" << json_str(K.opt.numbertype) << ",\"memopt\":" << (K.opt.memopt ? "true" : "false") << ",\"params\":[";
+    << ",\"numbertype\":" << json_str(K.opt.numbertype) << ",\"memopt\":" << (K.opt.memopt ? "true" : "false")
+    << ",\"groupable\":" << (groupable_body(K) ? "true" : "false") << ",\"z_reach\":" << (K.body ? z_span(K) : 0) << ",\"params\":[";
   for (size_t i = 0; i < K.def.params.size(); ++i)
     o << (i ? "," : "") << "{\"name\":" << json_str(K.def.params[i].name) << ",\"annot\":" << json_str(K.def.params[i].annot) << "}";
   o << "],\"written\":[";

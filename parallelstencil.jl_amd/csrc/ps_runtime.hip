@@ -1531,11 +1531,19 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
       if (cx == 0) cx = es >= 8 ? 1 : 4;
       if (len[0] < 2 * (int64_t)cx) cx = 1;
       // z cells per thread: a z plane that several of the thread's cells read comes from one load.
-      // By default 2 for FP64 arrays (one x cell) in rows of at least 256 cells, none otherwise:
-      // measured at the sustained clock, acoustic3D 384^3 FP64 6% faster per step (FP32 and
-      // BFloat16 10-15% slower), Stokes3D 127^3 FP64 4% slower (its rows are 127 cells: the
-      // edge block column and the threads before the first interior group double the waves)
-      int cz = K.opt.cells_z == 0 ? (es >= 8 && cx == 1 && len[0] >= 256 ? 2 : 1) : K.opt.cells_z;
+      // By default, for FP64 arrays (one x cell) in rows of at least 256 cells, twice the z reach
+      // of the stencil (so a thread reads 1.5 planes per cell), at most 8; none otherwise.
+      // Measured at the sustained clock: diffusion3D 512^3 (reach 2) 0.860 ms per step with one
+      // cell, 0.739 with 2, 0.688 with 4, 0.704 with 8; acoustic3D 384^3 (reach 1) 1.026, 0.961
+      // with 2, 0.998 with 4. FP32 and BFloat16 x groups were 10-15% slower with z cells, and
+      // Stokes3D at 127^3 (rows of 127 cells: the edge block column and the threads before the
+      // first interior group double the waves) 4% slower.
+      int cz = 1;
+      if (K.opt.cells_z != 0) cz = K.opt.cells_z;
+      else if (es >= 8 && cx == 1 && len[0] >= 256) {
+        const int reach = z_span(K);
+        cz = reach == 0 ? 1 : reach == 1 ? 2 : reach == 2 ? 4 : 8;
+      }
       if (cz > 1 && (P.nthreads[2] != 1 || len[2] < 2 * (int64_t)cz || !groupable_body(K, true))) cz = 1;
       if (cx > 1 || cz > 1) {
         v.cx = cx;

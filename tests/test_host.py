@@ -490,6 +490,21 @@ def test_x_group_row_split():
     assert "ps_ng" not in s3 and "const ps_int ps_ix0 = ps_r0x + tx * 2;" in s3
 
 
+def test_z_reach_of_kernels():
+    """The z reach (largest spread of z offsets into one array) that sets the default z cells of
+    FP64 groups: 2 for the 7-point diffusion stencil, 1 for acoustic3D's staggered updates, 0
+    without z neighbours; kernels with local variables are not groupable."""
+    from parallelstencil import workloads as W
+    kd = W.define_diffusion3d(False, __name__)
+    kV, kP = W.define_acoustic3d(__name__, False)
+    assert kd.info["groupable"] and kd.info["z_reach"] == 2
+    assert kV.info["z_reach"] == 1 and kP.info["z_reach"] == 1
+    k0 = ps.parallel_indices("@parallel_indices (ix,iy,iz) function cp!(A, B)\n A[ix,iy,iz] = B[ix+1,iy,iz]*2\n return\nend")
+    assert k0.info["groupable"] and k0.info["z_reach"] == 0
+    kl = ps.parallel_indices("@parallel_indices (ix,iy,iz) function loc!(A, B)\n t = B[ix,iy,iz]*2\n A[ix,iy,iz] = t\n return\nend")
+    assert not kl.info["groupable"]
+
+
 def _row_split_cells(r0, rl, xl, xh, k, A, CX, bx, nthreads):
     """The cells that the threads of one row take in the row split of x groups, as the emitted
     prologue computes them: (interior groups, edge cells)."""
