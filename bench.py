@@ -318,7 +318,7 @@ def xgroups_study(args):
         fields[prec] = F
     cfgs = [("f64", 0, 0, True)] + [("f64", cx, w, True) for cx, w in ((2, 64), (2, 32), (4, 32), (4, 16))]
     cfgs += [(p, cx, w, True, cz) for p, cx, w in (("f64", 1, 0), ("f32", 4, 32), ("bf16", 4, 32)) for cz in (2, 4)]
-    cfgs += [(p, 2, w, True, 2) for p in ("f32", "bf16") for w in (32, 64)]
+    cfgs += [(p, 2, w, True, 2) for p in ("f32", "bf16", "f64") for w in (32, 64)]
     cfgs += [("bf16", 0, 0, True), ("bf16", 1, 0, True)] + [("bf16", cx, w, sp) for cx in (2, 4, 8) for w in (16, 32, 64) for sp in (False, True)]
     cfgs += [("f32", 0, 0, True), ("f32", 1, 0, True)] + [("f32", cx, w, sp) for cx in (2, 4) for w in (16, 32, 64) for sp in (False, True)]
     kern = {c: W.define_acoustic3d("bench.x64" if c[0] == "f64" else "bench.x32", False, cells_x=c[1], nthreads_x_groups=c[2], xcd_spread=c[3],

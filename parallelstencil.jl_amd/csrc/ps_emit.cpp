@@ -2679,18 +2679,19 @@ bool groupable_body(const Kernel& K, bool z) {
   return true;
 }
 
-// Largest spread of z offsets (max - min of c in an index iz + c) with which the kernel body
-// accesses one array: the number of z planes beyond its own that a cell of that array reads.
-int z_span(const Kernel& K) {
+// Largest spread of offsets along dimension q (0 x, 2 z; max - min of c in an index i + c)
+// with which the kernel body accesses one array: the number of neighbours beyond its own cell
+// that a cell of that array reads along q.
+int idx_span(const Kernel& K, int q) {
   std::map<std::string, std::pair<long long, long long>> r;
   std::function<void(const NodeP&)> walk = [&](const NodeP& n) {
     if (!n) return;
-    if (n->k == NK::Index && n->a.size() >= 4 && n->a[0]->k == NK::Ident) {
-      const NodeP& e = n->a[3];
+    if (n->k == NK::Index && (int)n->a.size() >= q + 2 && n->a[0]->k == NK::Ident) {
+      const NodeP& e = n->a[q + 1];
       long long c = 0;
       bool ok = false;
-      if (e->k == NK::Ident && e->s == kIdx[2]) ok = true;
-      else if (e->k == NK::Binary && (e->s == "+" || e->s == "-") && e->a.size() == 2 && e->a[0]->k == NK::Ident && e->a[0]->s == kIdx[2] &&
+      if (e->k == NK::Ident && e->s == kIdx[q]) ok = true;
+      else if (e->k == NK::Binary && (e->s == "+" || e->s == "-") && e->a.size() == 2 && e->a[0]->k == NK::Ident && e->a[0]->s == kIdx[q] &&
                e->a[1]->k == NK::Num && e->a[1]->is_int) {
         ok = true;
         c = e->s == "+" ? e->a[1]->ival : -e->a[1]->ival;

@@ -1337,7 +1337,7 @@ std::string kernel_info_json(const Kernel& K) {
          return r;
        }() << "]"
     << ",\"numbertype\":" << json_str(K.opt.numbertype) << ",\"memopt\":" << (K.opt.memopt ? "true" : "false")
-    << ",\"groupable\":" << (groupable_body(K) ? "true" : "false") << ",\"z_reach\":" << (K.body ? z_span(K) : 0) << ",\"params\":[";
+    << ",\"groupable\":" << (groupable_body(K) ? "true" : "false") << ",\"x_reach\":" << (K.body ? idx_span(K, 0) : 0) << ",\"z_reach\":" << (K.body ? idx_span(K, 2) : 0) << ",\"params\":[";
   for (size_t i = 0; i < K.def.params.size(); ++i)
     o << (i ? "," : "") << "{\"name\":" << json_str(K.def.params[i].name) << ",\"annot\":" << json_str(K.def.params[i].annot) << "}";
   o << "],\"written\":[";

@@ -207,8 +207,9 @@ Emitted lower_kernel(const Kernel& k, const std::vector<ArgKind>& kinds, const V
 // index (and, for z groups, the z index) by an interval; no local variables, loops, kernel
 // language or atomic updates.
 bool groupable_body(const Kernel& k, bool z = false);
-// Largest spread of z offsets with which the kernel body accesses one array (0: no z neighbours).
-int z_span(const Kernel& k);
+// Largest spread of offsets along dimension q (0 x, 2 z) with which the kernel body accesses one
+// array (0: no neighbours along q).
+int idx_span(const Kernel& k, int q);
 std::string kernel_info_json(const Kernel& k);
 
 // Bounds proof: true if every array access of the kernel stays inside its

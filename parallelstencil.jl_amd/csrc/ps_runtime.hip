@@ -1527,23 +1527,25 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
         es = std::min<int64_t>(es, elsize(args[i].field.dtype));
       }
     if (ok && !user_geom && !cell && groupable_body(K)) {
-      int cx = K.opt.cells_x;
-      if (cx == 0) cx = es >= 8 ? 1 : 4;
-      if (len[0] < 2 * (int64_t)cx) cx = 1;
-      // z cells per thread: a z plane that several of the thread's cells read comes from one load.
-      // By default, for FP64 arrays (one x cell) in rows of at least 256 cells, twice the z reach
-      // of the stencil (so a thread reads 1.5 planes per cell), at most 8; none otherwise.
-      // Measured at the sustained clock: diffusion3D 512^3 (reach 2) 0.860 ms per step with one
-      // cell, 0.739 with 2, 0.688 with 4, 0.704 with 8; acoustic3D 384^3 (reach 1) 1.026, 0.961
-      // with 2, 0.998 with 4. FP32 and BFloat16 x groups were 10-15% slower with z cells, and
-      // Stokes3D at 127^3 (rows of 127 cells: the edge block column and the threads before the
-      // first interior group double the waves) 4% slower.
+      // Cells per thread. FP32 and BFloat16: 4 x cells. FP64 (3-D groups, rows of at least 256
+      // cells): z cells, twice the stencil's z reach (a thread then reads 1.5 planes per cell) and
+      // at most 8, so that a z plane that several of the thread's cells read is one load; and 2 x
+      // cells with them when the stencil reaches 2 cells along x. Measured at the sustained clock:
+      // diffusion3D 512^3 (reaches 2 and 2) 0.860 ms per step with one cell, 0.739 / 0.684 /
+      // 0.704 with 2 / 4 / 8 z cells, 0.652 with 2 x 4; acoustic3D 384^3 (reaches 1 and 1)
+      // 1.026, 0.961 with 2 z cells, 0.998 with 4, 1.001 with 2 x 2. FP32 and BFloat16 groups
+      // were 10-15% slower with z cells, and Stokes3D at 127^3 (rows of 127 cells: the edge block
+      // column and the threads before the first interior group double the waves) 4% slower.
+      const bool fp64 = es >= 8 && len[0] >= 256 && groupable_body(K, true);
       int cz = 1;
       if (K.opt.cells_z != 0) cz = K.opt.cells_z;
-      else if (es >= 8 && cx == 1 && len[0] >= 256) {
-        const int reach = z_span(K);
+      else if (fp64) {
+        const int reach = idx_span(K, 2);
         cz = reach == 0 ? 1 : reach == 1 ? 2 : reach == 2 ? 4 : 8;
       }
+      int cx = K.opt.cells_x;
+      if (cx == 0) cx = es < 8 ? 4 : fp64 && cz > 1 && idx_span(K, 0) >= 2 ? 2 : 1;
+      if (len[0] < 2 * (int64_t)cx) cx = 1;
       if (cz > 1 && (P.nthreads[2] != 1 || len[2] < 2 * (int64_t)cz || !groupable_body(K, true))) cz = 1;
       if (cx > 1 || cz > 1) {
         v.cx = cx;
@@ -1557,8 +1559,11 @@ Plan plan_launch(const Kernel& K, const ps_arg* args, int nargs, const ps_launch
         const int64_t ng = (len[0] + cx - 1) / cx;
         const int32_t txy = P.nthreads[0] * P.nthreads[1];
         auto pow2_ceil = [](int64_t n) { int64_t p = 1; while (p < n) p <<= 1; return p; };
+        // (FP64 groups with z cells: 32 threads of 2 x cells, measured 0.652 against 0.675 ms
+        // with 64 on the diffusion3D step at 512^3)
         int64_t bx = K.opt.nthreads_x_groups ? K.opt.nthreads_x_groups
-                                             : cx > 1 ? std::max<int64_t>(1, (len[2] > 1 ? 128 : 256) / cx) : (int64_t)P.nthreads[0];
+                     : cx > 1 ? (cz > 1 ? 32 : std::max<int64_t>(1, (len[2] > 1 ? 128 : 256) / cx))
+                              : (int64_t)P.nthreads[0];
         bx = std::min<int64_t>({bx, (int64_t)txy, pow2_ceil(ng)});
         int64_t by = std::max<int64_t>(1, txy / bx);
         if (by > pow2_ceil(len[1])) {

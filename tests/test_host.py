@@ -497,10 +497,10 @@ def test_z_reach_of_kernels():
     from parallelstencil import workloads as W
     kd = W.define_diffusion3d(False, __name__)
     kV, kP = W.define_acoustic3d(__name__, False)
-    assert kd.info["groupable"] and kd.info["z_reach"] == 2
-    assert kV.info["z_reach"] == 1 and kP.info["z_reach"] == 1
+    assert kd.info["groupable"] and kd.info["z_reach"] == 2 and kd.info["x_reach"] == 2
+    assert kV.info["z_reach"] == 1 and kP.info["z_reach"] == 1 and kV.info["x_reach"] == 1
     k0 = ps.parallel_indices("@parallel_indices (ix,iy,iz) function cp!(A, B)\n A[ix,iy,iz] = B[ix+1,iy,iz]*2\n return\nend")
-    assert k0.info["groupable"] and k0.info["z_reach"] == 0
+    assert k0.info["groupable"] and k0.info["z_reach"] == 0 and k0.info["x_reach"] == 0
     kl = ps.parallel_indices("@parallel_indices (ix,iy,iz) function loc!(A, B)\n t = B[ix,iy,iz]*2\n A[ix,iy,iz] = t\n return\nend")
     assert not kl.info["groupable"]
 
