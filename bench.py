@@ -457,9 +457,11 @@ def adjoint_study(args):
 
 
 def gemm_study(args):
-    """BF16 MFMA GEMM of the warm-start network (ps_nn_gemm), HIP-event timed: the three products
-    of a 3x3 convolution layer of the U-Net at batch 8 x 128^2 (forward, data gradient, weight
-    gradient) and a square 4096^3 product, as TFLOP/s against the BF16 dense MFMA peak."""
+    """BF16 MFMA GEMM of the warm-start network, HIP-event timed: the three products of a 3x3
+    convolution layer of the U-Net at batch 8 x 128^2 (forward, weight gradient, data gradient),
+    with the im2col operand gathered inside the GEMM (ps_nn_gemm_im2col, what the network runs)
+    and through a stored column matrix (ps_nn_im2col / ps_nn_col2im + ps_nn_gemm), and a square
+    4096^3 product, as TFLOP/s against the BF16 dense MFMA peak."""
     import numpy as np
     import parallelstencil as ps
     from parallelstencil import unet as U
@@ -467,37 +469,69 @@ def gemm_study(args):
     ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 3, module="bench.gemm")
     peak = 2500.0  # TFLOP/s, MI355X BF16 dense MFMA
     rng = np.random.default_rng(0)
-    cases = []
-    P, cin, cout = 8 * 128 * 128, 32, 32
-    K = 9 * cin
-    cases.append(("conv 32->32 forward (col . W)", P, cout, K, (P, K), K, 1, (K, cout), cout, 1, U.EPI_BIAS_RELU, 1))
-    cases.append(("conv 32->32 data gradient (dY . W^T)", P, K, cout, (P, cout), cout, 1, (K, cout), 1, cout, U.EPI_BF16, 1))
-    cases.append(("conv 32->32 weight gradient (col^T . dY)", K, cout, P, (P, K), 1, K, (P, cout), cout, 1, U.EPI_F32_ADD, U._splitk(K, cout, P)))
-    cases.append(("square 4096^3", 4096, 4096, 4096, (4096, 4096), 4096, 1, (4096, 4096), 4096, 1, U.EPI_F32, 1))
+    Bn, H, W, cin, cout = 8, 128, 128, 32, 32
+    P, K = Bn * H * W, 9 * cin
+
+    def buf(shape, dt="bf16"):
+        return U.Buf(shape, dt).upload(rng.standard_normal(shape).astype(np.float32))
+
+    x, col, w, dy = buf((P, cin)), buf((P, K)), buf((K, cout)), buf((P, cout))
+    y, dcol, gw, bias = U.Buf((P, cout)), U.Buf((P, K)), U.Buf((K, cout), "f32"), U.Buf((cout,), "f32")
+    wf, dx = U.Buf((9 * cout, cin)), U.Buf((P, cin))
+
+    def dgrad_implicit():
+        ps.core.check(U.lib.ps_nn_conv_flip(w.p, cin, cout, wf.p, None))
+        U.gemm_im2col(0, P, cin, 9 * cout, dy, H, W, cout, wf, cin, 1, dx, cin)
+
+    a4, b4, c4 = buf((4096, 4096)), buf((4096, 4096)), U.Buf((4096, 4096), "f32")
+    sk = U._splitk(K, cout, P)
+    cases = [
+        ("conv 32->32 forward, im2col inside the GEMM (x . W)", P, cout, K,
+         lambda: U.gemm_im2col(0, P, cout, K, x, H, W, cin, w, cout, 1, y, cout, bias, U.EPI_BIAS_RELU)),
+        ("conv 32->32 weight gradient, im2col inside the GEMM (col(x)^T . dY)", K, cout, P,
+         lambda: U.gemm_im2col(1, K, cout, P, x, H, W, cin, dy, cout, 1, gw, cout, epi=U.EPI_F32_ADD, splitk=sk)),
+        ("conv 32->32 data gradient, im2col of dY inside the GEMM (ps_nn_conv_flip + col(dY) . Wf)", P, cin, 9 * cout, dgrad_implicit),
+        ("im2col of x (the column matrix, 9x the activations)", 0, 0, 0,
+         lambda: ps.core.check(U.lib.ps_nn_im2col(x.p, Bn, H, W, cin, col.p, None))),
+        ("conv 32->32 forward on the column matrix (col . W)", P, cout, K,
+         lambda: U.gemm(P, cout, K, col, K, 1, w, cout, 1, y, cout, bias=bias, epi=U.EPI_BIAS_RELU)),
+        ("conv 32->32 weight gradient on the column matrix (col^T . dY)", K, cout, P,
+         lambda: U.gemm(K, cout, P, col, 1, K, dy, cout, 1, gw, cout, epi=U.EPI_F32_ADD, splitk=sk)),
+        ("conv 32->32 data gradient into the column matrix (dY . W^T)", P, K, cout,
+         lambda: U.gemm(P, K, cout, dy, cout, 1, w, 1, cout, dcol, K, epi=U.EPI_BF16)),
+        ("col2im of the column gradient", 0, 0, 0, lambda: ps.core.check(U.lib.ps_nn_col2im(dcol.p, Bn, H, W, cin, dx.p, None))),
+        ("square 4096^3", 4096, 4096, 4096, lambda: U.gemm(4096, 4096, 4096, a4, 4096, 1, b4, 4096, 1, c4, 4096, epi=U.EPI_F32)),
+    ]
     res = []
-    for name, M, N, Kd, ash, sam, sak, bsh, sbk, sbn, epi, split in cases:
-        A = U.Buf(ash).upload(rng.standard_normal(ash).astype(np.float32))
-        B = U.Buf(bsh).upload(rng.standard_normal(bsh).astype(np.float32))
-        C = U.Buf((M, N), "f32" if epi in (U.EPI_F32, U.EPI_F32_ADD) else "bf16")
-        bias = U.Buf((N,), "f32")
+    for name, M, N, Kd, fn in cases:
         for _ in range(3):
-            U.gemm(M, N, Kd, A, sam, sak, B, sbk, sbn, C, N, bias=bias, epi=epi, splitk=split)
+            fn()
         ps.synchronize()
         reps = 20
         e0, e1 = ps.Event(), ps.Event()
         e0.record()
         for _ in range(reps):
-            U.gemm(M, N, Kd, A, sam, sak, B, sbk, sbn, C, N, bias=bias, epi=epi, splitk=split)
+            fn()
         e1.record()
         ps.synchronize()
         ms = e0.elapsed_ms(e1) / reps
+        if M == 0:  # a data-movement kernel: HBM bytes, the activations once and the column matrix
+            gbs = (1 + 9) * P * cin * 2 / (ms * 1e-3) / 1e9
+            res.append({"case": name, "ms": round(ms, 4), "GB/s": round(gbs, 1)})
+            log(f"{name}: {ms:.4f} ms, {gbs:.0f} GB/s")
+            continue
         tf = 2.0 * M * N * Kd / (ms * 1e-3) / 1e12
         res.append({"case": name, "M": M, "N": N, "K": Kd, "ms": round(ms, 4), "tflops": round(tf, 2), "frac_of_peak": round(tf / peak, 4)})
         log(f"{name}: M={M} N={N} K={Kd}: {ms:.4f} ms, {tf:.1f} TFLOP/s ({100 * tf / peak:.2f}% of BF16 MFMA peak)")
+    ms = [r["ms"] for r in res]
+    implicit, explicit = ms[0] + ms[1] + ms[2], sum(ms[3:8])
+    log(f"forward + weight gradient + data gradient of the layer: {implicit:.4f} ms with im2col inside the GEMM, "
+        f"{explicit:.4f} ms through the column matrix (im2col, col2im and the products)")
     out = {"metric": "BF16 MFMA GEMM TFLOP/s (U-Net convolution products)", "value": res[0]["tflops"], "unit": "TFLOP/s",
            "n_gpus": 1, "steps": 20, "warmup": 3, "ms_per_step": res[0]["ms"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
-           "config": {"workload": "ps_nn_gemm on the U-Net's 3x3 convolution products and a square 4096^3 product"},
+           "config": {"workload": "ps_nn_gemm_im2col / ps_nn_gemm on the U-Net's 3x3 convolution products (batch 8 x 128^2, 32->32 channels) "
+                                  "and a square 4096^3 product"},
            "roofline": {"bound": "mfma", "achieved": res[0]["tflops"], "peak": peak, "unit": "TFLOP/s",
                         "frac": res[0]["frac_of_peak"], "traffic": None},
            "cases": res}

@@ -31,8 +31,20 @@ extern "C" {
 int ps_nn_gemm(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B, int64_t sbk, int64_t sbn, void* C,
                int64_t ldc, const float* bias, int epi, int splitk, void* stream);
 
+/* ps_nn_gemm with A the 3x3 im2col of the NHWC input x (ps_nn_im2col's col, never stored):
+ * transpose 0: A = col (M = B*H*W pixels, K = 9*C); transpose 1: A = col^T (M = 9*C,
+ * K = B*H*W), the weight gradient's operand. Same products in the same order as
+ * ps_nn_im2col followed by ps_nn_gemm, so the same results bit for bit. No counterpart in the
+ * reference (the learned warm start has none); replaces the im2col buffer of the convolutions. */
+int ps_nn_gemm_im2col(int transpose, int M, int N, int K, const void* x, int H, int W, int C, const void* B, int64_t sbk, int64_t sbn,
+                      void* Cm, int64_t ldc, const float* bias, int epi, int splitk, void* stream);
+
 /* 3x3, stride 1, zero padding 1, NHWC: col[(b*H*W + y*W + x)*9*C + (ky*3+kx)*C + c] */
 int ps_nn_im2col(const void* x, int B, int H, int W, int C, void* col, void* stream);
+/* weights of the data gradient of a 3x3 convolution with weights w (9*Cin x Cout, row
+ * (ky*3+kx)*Cin + ci): taps reversed, wf[(t*Cout + co)*Cin + ci] =
+ * w[((8-t)*Cin + ci)*Cout + co], so dx = ps_nn_gemm_im2col(0, ..., dy, ..., wf, ...) */
+int ps_nn_conv_flip(const void* w, int Cin, int Cout, void* wf, void* stream);
 /* adjoint of ps_nn_im2col: dx = sum of the column entries that read each input element */
 int ps_nn_col2im(const void* dcol, int B, int H, int W, int C, void* dx, void* stream);
 /* 2x2 average pooling of an HxW input (H, W even) and its adjoint */

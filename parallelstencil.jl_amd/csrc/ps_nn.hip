@@ -33,12 +33,14 @@ __device__ __forceinline__ bf16_t f2bf(float f) {  // round to nearest even, NaN
 
 constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;
 
-// C[m, n] = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn].
+// C[m, n] = sum_k A(m, k) B(k, n), B(k, n) = B[k*sbk + n*sbn]; A(m, k) = A[m*sam + k*sak] (AM 0),
+// or A is the 3x3 im2col of the NHWC input A (cH x cW x cC per image) without the column matrix:
+// AM 1 A(m, k) = col[m][k] (m a pixel, k = tap*cC + c), AM 2 A(m, k) = col[k][m].
 // blockIdx.z splits K into chunks of kchunk (a multiple of TK).
-template <int EPI>
+template <int EPI, int AM>
 __global__ void __launch_bounds__(256) nn_gemm_k(int M, int N, int K, const bf16_t* __restrict__ A, long long sam, long long sak,
                                                  const bf16_t* __restrict__ B, long long sbk, long long sbn, void* __restrict__ C,
-                                                 long long ldc, const float* __restrict__ bias, int kchunk) {
+                                                 long long ldc, const float* __restrict__ bias, int kchunk, int cH, int cW, int cC) {
   __shared__ __attribute__((aligned(16))) bf16_t As[TM * LDK];
   __shared__ __attribute__((aligned(16))) bf16_t Bs[TN * LDK];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -52,12 +54,25 @@ __global__ void __launch_bounds__(256) nn_gemm_k(int M, int N, int K, const bf16
   // row when k has unit stride, else 8 consecutive rows of one k (unit stride along m / n).
   // Whole, aligned pieces travel as one 16-byte load; edge or strided pieces element by element.
   // The next step's pieces are loaded while the current tile is multiplied.
-  const bool a_kfast = sak == 1, b_kfast = sbk == 1;
+  const bool a_kfast = AM == 0 ? sak == 1 : AM == 1, b_kfast = sbk == 1;
   const int ar = a_kfast ? tid / 4 : (tid % 8) * 8, ac = a_kfast ? (tid % 4) * 8 : tid / 8;
   const int br = b_kfast ? tid / 4 : (tid % 8) * 8, bc = b_kfast ? (tid % 4) * 8 : tid / 8;
   // A piece starts at an element index that is a multiple of 8 when the stride across pieces is
   // (k0, ac, ar, m0 are all multiples of 8), so the 16-byte load is aligned
-  const bool a_vec = (a_kfast ? sam : sak) % 8 == 0 && ((uintptr_t)A & 15) == 0 && (a_kfast || sam == 1);
+  const bool a_vec = AM == 0 ? (a_kfast ? sam : sak) % 8 == 0 && ((uintptr_t)A & 15) == 0 && (a_kfast || sam == 1)
+                             : cC % 8 == 0 && ((uintptr_t)A & 15) == 0;
+  // implicit im2col: the input element that column j of pixel p reads (0 outside the image)
+  auto col_at = [&](long long p, int j, int& tap, int& c, long long& src) -> bool {
+    tap = j / cC;
+    c = j - tap * cC;
+    const int xx = (int)(p % cW);
+    const long long t2 = p / cW;
+    const int yy = (int)(t2 % cH);
+    const long long b = t2 / cH;
+    const int sy = yy + tap / 3 - 1, sx = xx + tap % 3 - 1;
+    src = ((b * cH + sy) * cW + sx) * cC + c;
+    return sy >= 0 && sy < cH && sx >= 0 && sx < cW;
+  };
   const bool b_vec = (b_kfast ? sbn : sbk) % 8 == 0 && ((uintptr_t)B & 15) == 0 && (b_kfast || sbn == 1);
   s16x8 pa, pb;
   auto fetch = [&](int k0) {
@@ -65,14 +80,32 @@ __global__ void __launch_bounds__(256) nn_gemm_k(int M, int N, int K, const bf16
       const long long m = m0 + ar;
       const int k = k0 + ac;
       const bool full = a_kfast ? (m < M && k + 7 < ke) : (m + 7 < M && k < ke);
-      if (a_vec && full)
-        pa = *(const s16x8*)(A + m * sam + (long long)k * sak);
-      else
-        for (int t = 0; t < 8; ++t) {
-          const long long mm = a_kfast ? m : m + t;
-          const int kk = a_kfast ? k + t : k;
-          pa[t] = (mm < M && kk < ke) ? (short)A[mm * sam + (long long)kk * sak] : (short)0;
+      if (AM == 0) {
+        if (a_vec && full)
+          pa = *(const s16x8*)(A + m * sam + (long long)k * sak);
+        else
+          for (int t = 0; t < 8; ++t) {
+            const long long mm = a_kfast ? m : m + t;
+            const int kk = a_kfast ? k + t : k;
+            pa[t] = (mm < M && kk < ke) ? (short)A[mm * sam + (long long)kk * sak] : (short)0;
+          }
+      } else {
+        // the piece is columns j..j+7 of pixel p (AM 1: p = m, j = k; AM 2: p = k, j = m); with cC a
+        // multiple of 8 they are 8 channels of one tap, one 16-byte load (j is a multiple of 8)
+        const long long p = AM == 1 ? m : (long long)k;
+        const int j = AM == 1 ? k : (int)m;
+        int tap, c;
+        long long src;
+        if (a_vec && full) {
+          if (col_at(p, j, tap, c, src)) pa = *(const s16x8*)(A + src);
+          else pa = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        } else {
+          for (int t = 0; t < 8; ++t) {
+            const bool in = AM == 1 ? (m < M && k + t < ke) : (m + t < M && k < ke);
+            pa[t] = in && col_at(p, j + t, tap, c, src) ? (short)A[src] : (short)0;
+          }
         }
+      }
     }
     {  // B piece
       const long long n = n0 + br;
@@ -311,6 +344,16 @@ __global__ void __launch_bounds__(256) nn_bf16_to_f32_k(const bf16_t* __restrict
 __global__ void __launch_bounds__(256) nn_fill_f32_k(float* __restrict__ x, long long n, float v) {
   GS_LOOP(e, n) x[e] = v;
 }
+// the weights of a 3x3 convolution's data gradient: wf[(t*Cout + co)*Cin + ci] = w[((8-t)*Cin + ci)*Cout + co]
+// (taps reversed, each tap's Cin x Cout block transposed), so that dx = im2col(dy) . wf
+__global__ void __launch_bounds__(256) nn_conv_flip_k(const bf16_t* __restrict__ w, int Cin, int Cout, bf16_t* __restrict__ wf) {
+  GS_LOOP(e, 9LL * Cin * Cout) {
+    const int ci = (int)(e % Cin);
+    const long long r = e / Cin;
+    const int co = (int)(r % Cout), t = (int)(r / Cout);
+    wf[e] = w[((long long)(8 - t) * Cin + ci) * Cout + co];
+  }
+}
 __global__ void __launch_bounds__(256) nn_add_bf16_k(bf16_t* __restrict__ y, const bf16_t* __restrict__ x, long long n) {
   GS_LOOP(e, n) y[e] = f2bf(bf2f(y[e]) + bf2f(x[e]));
 }
@@ -327,9 +370,11 @@ int launched(const char* what) {
   if (int r_ = ps_internal_require_init()) return r_; \
   hipStream_t st = ps_internal_stream(stream);
 
-extern "C" int ps_nn_gemm(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B, int64_t sbk, int64_t sbn, void* C,
-                          int64_t ldc, const float* bias, int epi, int splitk, void* stream) {
-  NN_BEGIN
+namespace {
+
+template <int AM>
+int gemm_launch(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B, int64_t sbk, int64_t sbn, void* C, int64_t ldc,
+                const float* bias, int epi, int splitk, hipStream_t st, int cH, int cW, int cC) {
   if (M < 0 || N < 0 || K < 0 || splitk < 1) return ps_internal_fail(PS_EARGUMENT, "ps_nn_gemm: negative size or splitk < 1");
   if (M == 0 || N == 0) return 0;
   if ((epi == PS_NN_EPI_BF16_BIAS_RELU || epi == PS_NN_EPI_BF16_BIAS) && !bias) return ps_internal_fail(PS_EARGUMENT, "ps_nn_gemm: bias missing");
@@ -342,21 +387,39 @@ extern "C" int ps_nn_gemm(int M, int N, int K, const void* A, int64_t sam, int64
   if (grid.y > 65535 || grid.z > 65535) return ps_internal_fail(PS_EARGUMENT, "ps_nn_gemm: N or split too large");
   const bf16_t* a = (const bf16_t*)A;
   const bf16_t* b = (const bf16_t*)B;
+#define NN_GEMM_CASE(E)                                                                                                        \
+  case E:                                                                                                                      \
+    hipLaunchKernelGGL((nn_gemm_k<E, AM>), grid, dim3(256), 0, st, M, N, K, a, sam, sak, b, sbk, sbn, C, ldc, bias, kchunk, cH, cW, cC); \
+    break;
   switch (epi) {
-    case PS_NN_EPI_BF16: hipLaunchKernelGGL(nn_gemm_k<PS_NN_EPI_BF16>, grid, dim3(256), 0, st, M, N, K, a, sam, sak, b, sbk, sbn, C, ldc, bias, kchunk); break;
-    case PS_NN_EPI_BF16_BIAS_RELU:
-      hipLaunchKernelGGL(nn_gemm_k<PS_NN_EPI_BF16_BIAS_RELU>, grid, dim3(256), 0, st, M, N, K, a, sam, sak, b, sbk, sbn, C, ldc, bias, kchunk);
-      break;
-    case PS_NN_EPI_BF16_BIAS:
-      hipLaunchKernelGGL(nn_gemm_k<PS_NN_EPI_BF16_BIAS>, grid, dim3(256), 0, st, M, N, K, a, sam, sak, b, sbk, sbn, C, ldc, bias, kchunk);
-      break;
-    case PS_NN_EPI_F32_ADD:
-      hipLaunchKernelGGL(nn_gemm_k<PS_NN_EPI_F32_ADD>, grid, dim3(256), 0, st, M, N, K, a, sam, sak, b, sbk, sbn, C, ldc, bias, kchunk);
-      break;
-    case PS_NN_EPI_F32: hipLaunchKernelGGL(nn_gemm_k<PS_NN_EPI_F32>, grid, dim3(256), 0, st, M, N, K, a, sam, sak, b, sbk, sbn, C, ldc, bias, kchunk); break;
+    NN_GEMM_CASE(PS_NN_EPI_BF16)
+    NN_GEMM_CASE(PS_NN_EPI_BF16_BIAS_RELU)
+    NN_GEMM_CASE(PS_NN_EPI_BF16_BIAS)
+    NN_GEMM_CASE(PS_NN_EPI_F32_ADD)
+    NN_GEMM_CASE(PS_NN_EPI_F32)
     default: return ps_internal_fail(PS_EARGUMENT, "ps_nn_gemm: unknown epilogue");
   }
+#undef NN_GEMM_CASE
   return launched("ps_nn_gemm");
+}
+
+}  // namespace
+
+extern "C" int ps_nn_gemm(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B, int64_t sbk, int64_t sbn, void* C,
+                          int64_t ldc, const float* bias, int epi, int splitk, void* stream) {
+  NN_BEGIN
+  return gemm_launch<0>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, epi, splitk, st, 1, 1, 1);
+}
+
+extern "C" int ps_nn_gemm_im2col(int transpose, int M, int N, int K, const void* x, int H, int W, int Cin, const void* B, int64_t sbk, int64_t sbn,
+                                 void* C, int64_t ldc, const float* bias, int epi, int splitk, void* stream) {
+  NN_BEGIN
+  if (H < 1 || W < 1 || Cin < 1) return ps_internal_fail(PS_EARGUMENT, "ps_nn_gemm_im2col: H, W and C must be positive");
+  const long long npix = transpose ? (long long)K : (long long)M, ncol = transpose ? (long long)M : (long long)K;
+  if (ncol != 9LL * Cin || npix % ((long long)H * W) != 0)
+    return ps_internal_fail(PS_EARGUMENT, "ps_nn_gemm_im2col: the im2col operand is (B*H*W) x (9*C): sizes do not match");
+  return transpose ? gemm_launch<2>(M, N, K, x, 0, 0, B, sbk, sbn, C, ldc, bias, epi, splitk, st, H, W, Cin)
+                   : gemm_launch<1>(M, N, K, x, 0, 0, B, sbk, sbn, C, ldc, bias, epi, splitk, st, H, W, Cin);
 }
 
 #define NN_ELEMENTWISE(name, n, ...)                                            \
@@ -367,6 +430,9 @@ extern "C" int ps_nn_gemm(int M, int N, int K, const void* A, int64_t sam, int64
 
 extern "C" int ps_nn_im2col(const void* x, int B, int H, int W, int C, void* col, void* stream) {
   NN_ELEMENTWISE(nn_im2col_k, (long long)B * H * W * 9 * C, (const bf16_t*)x, B, H, W, C, (bf16_t*)col)
+}
+extern "C" int ps_nn_conv_flip(const void* w, int Cin, int Cout, void* wf, void* stream) {
+  NN_ELEMENTWISE(nn_conv_flip_k, 9LL * Cin * Cout, (const bf16_t*)w, Cin, Cout, (bf16_t*)wf)
 }
 extern "C" int ps_nn_col2im(const void* dcol, int B, int H, int W, int C, void* dx, void* stream) {
   NN_ELEMENTWISE(nn_col2im_k, (long long)B * H * W * C, (const bf16_t*)dcol, B, H, W, C, (bf16_t*)dx)

@@ -193,6 +193,8 @@ _V, _I64, _F = ctypes.c_void_p, ctypes.c_int64, ctypes.c_float
 _I = ctypes.c_int
 _SIGS.update({
     "ps_nn_gemm": (_I, [_I, _I, _I, _V, _I64, _I64, _V, _I64, _I64, _V, _I64, _V, _I, _I, _V]),
+    "ps_nn_conv_flip": (_I, [_V, _I, _I, _V, _V]),
+    "ps_nn_gemm_im2col": (_I, [_I, _I, _I, _I, _V, _I, _I, _I, _V, _I64, _I64, _V, _I64, _V, _I, _I, _V]),
     "ps_nn_im2col": (_I, [_V, _I, _I, _I, _I, _V, _V]),
     "ps_nn_col2im": (_I, [_V, _I, _I, _I, _I, _V, _V]),
     "ps_nn_pool2": (_I, [_V, _I, _I, _I, _I, _V, _V]),

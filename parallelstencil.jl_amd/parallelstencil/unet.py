@@ -68,6 +68,13 @@ def gemm(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias=None, epi=EPI_BF16, spl
     check(lib.ps_nn_gemm(M, N, K, A.p, sam, sak, B.p, sbk, sbn, C.p, ldc, bias.p if bias is not None else None, epi, splitk, None))
 
 
+def gemm_im2col(transpose, M, N, K, x, H, W, C, B, sbk, sbn, Cm, ldc, bias=None, epi=EPI_BF16, splitk=1):
+    """gemm with A the 3x3 im2col of the NHWC input x (never stored): transpose 0 A = col
+    (M pixels x K = 9C), transpose 1 A = col^T (the weight gradient's operand)."""
+    check(lib.ps_nn_gemm_im2col(transpose, M, N, K, x.p, H, W, C, B.p, sbk, sbn, Cm.p, ldc, bias.p if bias is not None else None, epi,
+                                splitk, None))
+
+
 def _splitk(M, N, K):
     """K chunks for a product whose M x N tiles (64 x 64) alone do not fill the GPU: enough
     chunks for about 1024 workgroups, each chunk at least 256 deep."""
@@ -102,16 +109,15 @@ class Conv:
         return b
 
     def forward(self, x, B, H, W):
+        """y = act(conv(x) + b); 3x3 convolutions read their im2col columns straight from x."""
         M = B * H * W
-        if self.k == 3:
-            col = self._buf("col", (M, self.K))
-            check(lib.ps_nn_im2col(x.p, B, H, W, self.cin, col.p, None))
-        else:
-            col = x
-        self.x_col, self.geom = col, (B, H, W)
+        self.x, self.geom = x, (B, H, W)
         y = self._buf("y", (B, H, W, self.cout))
-        gemm(M, self.cout, self.K, col, self.K, 1, self.w16, self.cout, 1, y, self.cout, self.b,
-             EPI_BIAS_RELU if self.relu else EPI_BIAS)
+        epi = EPI_BIAS_RELU if self.relu else EPI_BIAS
+        if self.k == 3:
+            gemm_im2col(0, M, self.cout, self.K, x, H, W, self.cin, self.w16, self.cout, 1, y, self.cout, self.b, epi)
+        else:
+            gemm(M, self.cout, self.K, x, self.K, 1, self.w16, self.cout, 1, y, self.cout, self.b, epi)
         self.y = y
         return y
 
@@ -124,16 +130,22 @@ class Conv:
             check(lib.ps_nn_relu_bwd(dy.p, self.y.p, dy.n, None))
         check(lib.ps_nn_bias_grad(dy.p, M, self.cout, self.gb.p, None))
         # gw[kk, co] += sum_pixels col[p, kk] dy[p, co]
-        gemm(self.K, self.cout, M, self.x_col, 1, self.K, dy, self.cout, 1, self.gw, self.cout, epi=EPI_F32_ADD, splitk=_splitk(self.K, self.cout, M))
+        sk = _splitk(self.K, self.cout, M)
+        if self.k == 3:
+            gemm_im2col(1, self.K, self.cout, M, self.x, H, W, self.cin, dy, self.cout, 1, self.gw, self.cout, epi=EPI_F32_ADD, splitk=sk)
+        else:
+            gemm(self.K, self.cout, M, self.x, 1, self.K, dy, self.cout, 1, self.gw, self.cout, epi=EPI_F32_ADD, splitk=sk)
         if not need_dx:
             return None
-        dcol = self._buf("dcol", (M, self.K))
-        # dcol[p, kk] = sum_co dy[p, co] w[kk, co]
-        gemm(M, self.K, self.cout, dy, self.cout, 1, self.w16, 1, self.cout, dcol, self.K)
-        if self.k == 1:
-            return dcol
         dx = self._buf("dx", (B, H, W, self.cin))
-        check(lib.ps_nn_col2im(dcol.p, B, H, W, self.cin, dx.p, None))
+        if self.k == 1:  # dx[p, ci] = sum_co dy[p, co] w[ci, co]
+            gemm(M, self.cin, self.cout, dy, self.cout, 1, self.w16, 1, self.cout, dx, self.cin)
+            return dx
+        # dx[q, ci] = sum over taps t, co of dy[q - off(t), co] w[t*cin + ci, co]: the 3x3 im2col of
+        # dy (off(8-t) = -off(t)) times the weights with reversed taps, each tap transposed
+        wf = self._buf("wf", (9 * self.cout, self.cin))
+        check(lib.ps_nn_conv_flip(self.w16.p, self.cin, self.cout, wf.p, None))
+        gemm_im2col(0, M, self.cin, 9 * self.cout, dy, H, W, self.cout, wf, self.cin, 1, dx, self.cin)
         return dx
 
 

@@ -49,6 +49,24 @@ class FakeLib:
             c[idx] = r
         return 0
 
+    def ps_nn_gemm_im2col(self, transpose, M, N, K, x, H, W, C, B, sbk, sbn, Cm, ldc, bias, epi, splitk, stream):
+        # the definition in include/ps_nn.h: ps_nn_im2col into a column matrix, then ps_nn_gemm on it
+        npix = K if transpose else M
+        k = next(_ids)
+        _mem[k] = np.zeros(npix * 9 * C)
+        self.ps_nn_im2col(x, npix // (H * W), H, W, C, k, stream)
+        if transpose:
+            r = self.ps_nn_gemm(M, N, K, k, 1, 9 * C, B, sbk, sbn, Cm, ldc, bias, epi, splitk, stream)
+        else:
+            r = self.ps_nn_gemm(M, N, K, k, 9 * C, 1, B, sbk, sbn, Cm, ldc, bias, epi, splitk, stream)
+        del _mem[k]
+        return r
+
+    def ps_nn_conv_flip(self, w, Cin, Cout, wf, stream):
+        n = 9 * Cin * Cout
+        _a(wf)[:n] = _a(w)[:n].reshape(9, Cin, Cout)[::-1].transpose(0, 2, 1).reshape(-1)
+        return 0
+
     def ps_nn_im2col(self, x, B, H, W, C, col, stream):
         X = _a(x)[:B * H * W * C].reshape(B, H, W, C)
         Xp = np.pad(X, ((0, 0), (1, 1), (1, 1), (0, 0)))

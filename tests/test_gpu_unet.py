@@ -174,3 +174,69 @@ def test_unet_training_reduces_loss():
     hist = ws.train(X, T, np.ones_like(T), epochs=40, batch=B, lr=1e-2)
     print("loss", hist[0], "->", hist[-1])
     assert hist[-1] < 0.2 * hist[0]
+
+
+@pytest.mark.parametrize("C,misalign", [(2, 0), (5, 0), (32, 0), (64, 0), (32, 1)])
+def test_implicit_im2col_gemm_matches_explicit(C, misalign):
+    """ps_nn_gemm_im2col reads the 3x3 im2col columns straight from the NHWC input: bit for bit
+    ps_nn_im2col + ps_nn_gemm, for the convolution (transpose 0, both epilogues) and its weight
+    gradient (transpose 1, split-K); a misaligned input takes the element-by-element loads."""
+    import ctypes
+    rng = np.random.default_rng(C + 100 * misalign)
+    B, H, W, N = 2, 6, 10, 48
+    P, K9 = B * H * W, 9 * C
+    x = _bf(rng.standard_normal((B, H, W, C)))
+    xs = U.Buf((P * C + misalign,))
+    xs.upload(np.concatenate([np.zeros(misalign), x.reshape(-1)]))
+    xp = ctypes.c_void_p(xs.p.value + 2 * misalign)
+    col = U.Buf((P, K9))
+    ps.core.check(U.lib.ps_nn_im2col(xp, B, H, W, C, col.p, None))
+    w = U.Buf((K9, N)).upload(_bf(rng.standard_normal((K9, N))))
+    bias = U.Buf((N,), "f32").upload(rng.standard_normal(N).astype(np.float32))
+    for epi, dt in ((U.EPI_F32, "f32"), (U.EPI_BIAS_RELU, "bf16")):
+        ye, yi = U.Buf((P, N), dt), U.Buf((P, N), dt)
+        b = bias if epi == U.EPI_BIAS_RELU else None
+        U.gemm(P, N, K9, col, K9, 1, w, N, 1, ye, N, bias=b, epi=epi)
+        ps.core.check(U.lib.ps_nn_gemm_im2col(0, P, N, K9, xp, H, W, C, w.p, N, 1, yi.p, N, b.p if b else None, epi, 1, None))
+        assert np.array_equal(ye.download(), yi.download()), (C, epi)
+        if epi == U.EPI_F32 and not misalign:  # the convolution itself, in float64
+            xpad = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
+            cref = np.concatenate([xpad[:, dy:dy + H, dx:dx + W, :] for dy in range(3) for dx in range(3)], axis=3)
+            ref = cref.reshape(P, K9) @ w.download().astype(np.float64)
+            assert np.abs(yi.download() - ref).max() <= 1e-5 * np.abs(ref).max()
+    dy = U.Buf((P, N)).upload(_bf(rng.standard_normal((P, N))))
+    for sk in (1, 3):
+        ge, gi = U.Buf((K9, N), "f32").zero(), U.Buf((K9, N), "f32").zero()
+        U.gemm(K9, N, P, col, 1, K9, dy, N, 1, ge, N, epi=U.EPI_F32_ADD, splitk=sk)
+        ps.core.check(U.lib.ps_nn_gemm_im2col(1, K9, N, P, xp, H, W, C, dy.p, N, 1, gi.p, N, None, U.EPI_F32_ADD, sk, None))
+        if sk == 1:
+            assert np.array_equal(ge.download(), gi.download()), (C, sk)
+        else:  # split-K partial sums meet in atomic adds of either order
+            g = ge.download()
+            assert np.abs(gi.download() - g).max() <= 1e-6 * np.abs(g).max(), (C, sk)
+    with pytest.raises(Exception, match="sizes do not match"):
+        ps.core.check(U.lib.ps_nn_gemm_im2col(0, P, N, K9 + 1, xp, H, W, C, w.p, N, 1, col.p, N, None, U.EPI_F32, 1, None))
+
+
+@pytest.mark.parametrize("cin,cout", [(3, 5), (32, 32), (16, 64)])
+def test_implicit_data_gradient(cin, cout):
+    """dx of a 3x3 convolution as im2col(dy) . wf (ps_nn_conv_flip: taps reversed, each tap
+    transposed; an exact permutation) against the adjoint in float64: dcol = dy . W^T summed back
+    by col2im; dx is BF16 from FP32 sums."""
+    rng = np.random.default_rng(cin * cout)
+    B, H, W = 2, 6, 10
+    P = B * H * W
+    w = _bf(rng.standard_normal((9 * cin, cout)))
+    dyh = _bf(rng.standard_normal((P, cout)))
+    wb, dyb = U.Buf(w.shape).upload(w), U.Buf(dyh.shape).upload(dyh)
+    wf = U.Buf((9 * cout, cin))
+    ps.core.check(U.lib.ps_nn_conv_flip(wb.p, cin, cout, wf.p, None))
+    assert np.array_equal(wf.download(), w.reshape(9, cin, cout)[::-1].transpose(0, 2, 1).reshape(9 * cout, cin))
+    dx = U.Buf((P, cin))
+    U.gemm_im2col(0, P, cin, 9 * cout, dyb, H, W, cout, wf, cin, 1, dx, cin)
+    dcol = (dyh @ w.T).reshape(B, H, W, 9, cin)
+    ref = np.zeros((B, H + 2, W + 2, cin))
+    for t in range(9):
+        ref[:, t // 3:t // 3 + H, t % 3:t % 3 + W, :] += dcol[:, :, :, t, :]
+    ref = ref[:, 1:H + 1, 1:W + 1, :].reshape(P, cin)
+    assert np.abs(dx.download() - ref).max() <= 8e-3 * np.abs(ref).max()
