@@ -140,7 +140,7 @@ def stokes_warm_start(args):
     import numpy as np
     import parallelstencil as ps
     from parallelstencil import unet as U
-    from parallelstencil.stokes_fused import Stokes2DFused
+    from parallelstencil.stokes_fused import Stokes2DBatch, Stokes2DFused
 
     mod = "bench.stokes"
     ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2, module=mod)
@@ -149,11 +149,19 @@ def stokes_warm_start(args):
     tsolver = Stokes2DFused(mod, nt, nt, eps=1e-8, nout=200, eta_ts=args.stokes_ts, history=True)
     t0 = time.perf_counter()
     data, stats = [], []
-    for i0 in range(1, (0 if args.stokes_load else args.train) + 1, 64):
-        d, st = U.generate_samples(tsolver, range(i0, min(args.train, i0 + 63) + 1), contrast)
-        data += d
-        stats += st
-        log(f"stokes {nt}^2: {len(stats)} training solves, {time.perf_counter() - t0:.1f} s, last {st[-1][0]} iterations")
+    if args.stokes_batch > 1 and not args.stokes_load and args.train:
+        # the training solves S at a time (Stokes2DBatch's slot queue): the same iterations, residuals
+        # and fields per seed as the single-sample solver
+        bsolver = Stokes2DBatch(mod, nt, nt, args.stokes_batch, eps=1e-8, nout=200, eta_ts=args.stokes_ts)
+        data, stats = U.generate_samples_batch(bsolver, range(1, args.train + 1), contrast, log=log)
+        bsolver.close()
+        del bsolver
+    else:
+        for i0 in range(1, (0 if args.stokes_load else args.train) + 1, 64):
+            d, st = U.generate_samples(tsolver, range(i0, min(args.train, i0 + 63) + 1), contrast)
+            data += d
+            stats += st
+            log(f"stokes {nt}^2: {len(stats)} training solves, {time.perf_counter() - t0:.1f} s, last {st[-1][0]} iterations")
     t1 = time.perf_counter()
     if args.stokes_load:
         ws = U.WarmStart.load(args.stokes_load)
@@ -170,7 +178,8 @@ def stokes_warm_start(args):
         data = [d for d, k in zip(data, ok) if k]  # samples on which the plain iteration did not converge are dropped
         its = [s_[0] for s_, k in zip(stats, ok) if k]
         log(f"stokes {nt}^2: {len(stats)} training solves in {t1 - t0:.1f} s, {sum(ok)} converged, mean {np.mean(its):.0f} PT iterations, "
-            f"{1e6 * (t1 - t0) / max(1, sum(s_[0] for s_ in stats)):.2f} us per iteration")
+            f"{1e6 * (t1 - t0) / max(1, sum(s_[0] for s_ in stats)):.2f} us per sample-iteration"
+            f"{f' ({args.stokes_batch} samples per launch pair)' if args.stokes_batch > 1 else ''}")
         if args.stokes_mirror:  # the problem's mirror symmetry in x doubles the training set
             data = data + [U.mirror_x(d) for d in data]
         n_train = len(data)
@@ -222,7 +231,8 @@ def stokes_warm_start(args):
                    "train_samples": n_train, "train_solves_converged": sum(ok), "train_mirrored": bool(args.stokes_mirror),
                    "train_n": nt, "eval_samples": {str(k): v["samples"] for k, v in evals.items()}, "epochs": args.epochs,
                    "warm_start_loaded_from": args.stokes_load or None},
-        "stokes": {"datagen_s": t1 - t0, "train_s": t2 - t1, "loss_first": hist[0], "loss_last": hist[-1],
+        "stokes": {"datagen_s": t1 - t0, "datagen_samples_per_launch": args.stokes_batch if stats and args.stokes_batch > 1 else 1,
+                   "train_s": t2 - t1, "loss_first": hist[0], "loss_last": hist[-1],
                    "train_dropped_not_converged": n_solves - sum(ok), "train_iters_mean": float(np.mean(its)) if stats else None,
                    "eval": {str(k): {kk: vv for kk, vv in v.items() if kk != "per_sample"} for k, v in evals.items()}},
     }
@@ -535,6 +545,77 @@ def gemm_study(args):
            "roofline": {"bound": "mfma", "achieved": res[0]["tflops"], "peak": peak, "unit": "TFLOP/s",
                         "frac": res[0]["frac_of_peak"], "traffic": None},
            "cases": res}
+    print(json.dumps(out), flush=True)
+
+
+def stokes_batch_study(args):
+    """Stokes2D PT iterations of S samples at once (Stokes2DBatch: the two fused launches over
+    (n, n, S) arrays) against one sample at a time (Stokes2DFused), at 127^2 (the training
+    resolution of the warm start) and 511^2: graph replays of 200-iteration blocks with their
+    per-sample residual reductions, HIP-event timed over 1000 iterations. Reported per batch
+    iteration and per sample-iteration; the effective rate counts the 27 field sweeps of n^2
+    doubles that one fused iteration makes (10 by compute_Pτ!, 17 by compute_dVV!). Before the
+    timing, each batch's first plane is checked bit for bit against the single-sample solver
+    after the same iterations."""
+    import numpy as np
+    import parallelstencil as ps
+    from parallelstencil import stokes_fused as F
+    from parallelstencil import workloads as W
+
+    mod = "bench.stokesbatch"
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2, module=mod)
+    nblk = 5
+    runs = []
+    for n, sizes in ((127, (1, 4, 16, 64)), (511, (1, 2, 4, 8, 16))):
+        mats = [W.stokes2d_inclusions(n, n, 7000 + s, log10_contrast=1.0) for s in range(max(sizes))]
+        one = F.Stokes2DFused(mod, n, n)
+        one.set_material(*mats[0])
+        one.reset()
+        one.iterate_block()
+        ps.synchronize()
+        ref_vx = one.Vx.to_host()
+        e0, e1 = ps.Event(), ps.Event()
+        e0.record()
+        for _ in range(nblk):
+            one.iterate_block()
+        e1.record()
+        ps.synchronize()
+        t1 = e0.elapsed_ms(e1) / (nblk * one.nout) * 1e3
+        one.close()
+        runs.append({"n": n, "samples": 0, "solver": "Stokes2DFused", "us_per_iteration": round(t1, 3), "us_per_sample_iteration": round(t1, 3),
+                     "eff_TBs": round(27 * 8 * n * n / (t1 * 1e-6) / 1e12, 3)})
+        log(f"{n}^2 one sample (Stokes2DFused): {t1:.2f} us per iteration, {27 * 8 * n * n / (t1 * 1e-6) / 1e12:.2f} TB/s effective")
+        for S in sizes:
+            B = F.Stokes2DBatch(mod, n, n, S)
+            B.set_material(np.asfortranarray(np.stack([m for m, _ in mats[:S]], axis=2)),
+                           np.asfortranarray(np.stack([r for _, r in mats[:S]], axis=2)))
+            B.reset()
+            B.iterate_block()
+            ps.synchronize()
+            same = bool(np.array_equal(np.asarray(B.Vx.to_host())[:, :, 0].view(np.uint64), np.asarray(ref_vx).view(np.uint64)))
+            e0.record()
+            for _ in range(nblk):
+                B.iterate_block()
+            e1.record()
+            ps.synchronize()
+            t = e0.elapsed_ms(e1) / (nblk * B.nout) * 1e3
+            B.close()
+            del B
+            eff = 27 * 8 * n * n * S / (t * 1e-6) / 1e12
+            runs.append({"n": n, "samples": S, "solver": "Stokes2DBatch", "us_per_iteration": round(t, 3), "us_per_sample_iteration": round(t / S, 3),
+                         "speedup_per_sample": round(t1 / (t / S), 2), "eff_TBs": round(eff, 3), "plane0_bit_identical": same})
+            log(f"{n}^2 batch of {S}: {t:.2f} us per iteration, {t / S:.3f} us per sample-iteration ({t1 / (t / S):.2f}x one sample at a time), "
+                f"{eff:.2f} TB/s effective; first plane bit-identical to the single solver: {same}")
+            if not same:
+                raise SystemExit("batched Stokes2D differs from the single-sample solver")
+    best = max((r for r in runs if r["samples"] and r["n"] == 127), key=lambda r: r["speedup_per_sample"])
+    out = {"metric": "Stokes2D PT sample-iterations per second (batched samples)", "value": round(1e6 / best["us_per_sample_iteration"], 1),
+           "unit": "sample-iterations/s", "n_gpus": 1, "steps": nblk * 200, "warmup": 200, "ms_per_step": round(best["us_per_iteration"] / 1e3, 5),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (inclusion samples, contrast 10^[-1,1])",
+           "config": {"workload": f"Stokes2D PT iteration, {best['samples']} samples of 127^2 per launch pair, graph replays"},
+           "roofline": {"bound": "hbm", "achieved": round(best["eff_TBs"] * 1e3, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(best["eff_TBs"] * 1e3 / HBM_PEAK_GBS, 4), "traffic": None},
+           "runs": runs}
     print(json.dumps(out), flush=True)
 
 
@@ -940,7 +1021,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | xgroups | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes3d | thermal | porowaves | hydromech")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | xgroups | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes-batch | stokes3d | thermal | porowaves | hydromech")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--acoustic-exact-only", action="store_true", help="acoustic: skip the fastmath variants")
@@ -955,6 +1036,8 @@ def main():
     ap.add_argument("--stokes-save", default="", help="stokes: write the trained warm start (.npz)")
     ap.add_argument("--stokes-load", default="", help="stokes: evaluate the warm start in this .npz instead of training one")
     ap.add_argument("--stokes-mirror", type=int, default=1, help="stokes: 1 adds the x-mirrored training samples")
+    ap.add_argument("--stokes-batch", type=int, default=64,
+                    help="stokes: training solves this many samples per launch pair (Stokes2DBatch); 1 solves them one at a time")
     ap.add_argument("--train", type=int, default=256, help="stokes: training samples")
     ap.add_argument("--eval", type=int, default=24, help="stokes: held-out samples at --stokes-n")
     ap.add_argument("--eval2", type=int, default=4, help="stokes: held-out samples at --stokes-n2")
@@ -981,6 +1064,9 @@ def main():
         return
     if args.workload == "xgroups":
         xgroups_study(args)
+        return
+    if args.workload == "stokes-batch":
+        stokes_batch_study(args)
         return
     if args.workload == "gemm":
         gemm_study(args)

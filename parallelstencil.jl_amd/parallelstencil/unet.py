@@ -383,6 +383,35 @@ def generate_samples(solver, seeds, log10_contrast=1.0):
     return data, stats
 
 
+def generate_samples_batch(bsolver, seeds, log10_contrast=1.0, log=None, every=64):
+    """generate_samples on a Stokes2DBatch: the solves run S at a time through its slot queue,
+    with the same results per seed (iterations, residual and fields bit for bit those of the
+    single-sample solver). Seconds per seed are the batch's wall time shared in proportion to
+    the seed's iterations. `log` receives a line every `every` finished solves."""
+    import time
+    from .workloads import stokes2d_inclusions
+    seeds = list(seeds)
+    res = [None] * len(seeds)
+    core.synchronize()
+    t0 = time.perf_counter()
+    mats = (stokes2d_inclusions(bsolver.nx, bsolver.ny, s, log10_contrast=log10_contrast) for s in seeds)
+    kept = {}
+
+    def samples():
+        for k, m in enumerate(mats):
+            kept[k] = m
+            yield m
+
+    for k, (i, it, err, Vx, Vy, Pt) in enumerate(bsolver.solve_queue(samples())):
+        Mus, Rog = kept.pop(i)
+        res[i] = (it, err, (Mus, Rog, Vx, Vy, Pt))
+        if log and (k + 1) % every == 0:
+            log(f"stokes {bsolver.nx}^2 batch of {bsolver.S}: {k + 1}/{len(seeds)} solves, {time.perf_counter() - t0:.1f} s")
+    wall = time.perf_counter() - t0
+    total = max(sum(r[0] for r in res), 1)
+    return [r[2] for r in res], [(r[0], wall * r[0] / total, r[1]) for r in res]
+
+
 def mirror_x(sample):
     """A sample (Mus, Rog, Vx, Vy, Pt) mirrored in x: the miniapp's problem is symmetric about
     x = lx/2 (free-slip walls, the body force along y), so the mirrored fields solve the mirrored

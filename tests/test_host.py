@@ -563,3 +563,28 @@ def test_quotient_checks_per_statement():
     k = ps.parallel("@parallel function q2!(R, A, b)\n @all(R) = (@all(A)/b > 1.0) ? 1.0 : 0.0\n return\nend")
     body = k.source(ps.ArgSpec((8, 8, 8)), ps.ArgSpec((8, 8, 8)), 0.3).split("__global__")[1]
     assert "ps_divp(" in body and "ps_divx(" not in body
+
+
+def test_batched_stokes_kernel_texts():
+    # every array access of the 2-D fused kernels gains the sample index, nothing else changes
+    import re
+    from parallelstencil import stokes_fused as F
+    from parallelstencil import workloads as W
+    for text in (F.STOKES2D_PTAU, F.STOKES2D_DVV, W.STOKES2D_MAXLOC_X, W.STOKES2D_MAXLOC_Y):
+        b = F.batched_text(text)
+        assert re.search(r"@parallel_indices \(ix, iy, iz\) function \S+_batch!\(", b)
+        n = len(re.findall(r"\w+\[[^\[\]]+\]", text))
+        assert n > 0 and len(re.findall(r"\w+\[[^\[\]]+, iz\]", b)) == n
+        assert re.sub(r", iz\]", "]", b.replace("(ix, iy, iz)", "(ix, iy)").replace("_batch!(", "!(", 1)).replace("(ix,iy)", "(ix, iy)") \
+            == text.replace("(ix,iy)", "(ix, iy)")
+
+
+def test_batched_stokes_kernel_guard():
+    # the guarded form: Act first, every statement inside one `if (Act[1, 1, iz] != 0.0)`
+    from parallelstencil import stokes_fused as F
+    for text in (F.STOKES2D_PTAU, F.STOKES2D_DVV):
+        plain, g = F.batched_text(text), F.batched_text(text, guard=True)
+        assert "_batch!(Act::Data.Array, " in g
+        body = g.split("\n", 2)[2]
+        assert body.startswith("    if (Act[1, 1, iz] != 0.0)\n") and body.endswith("    end\n    return\nend\n")
+        assert body[len("    if (Act[1, 1, iz] != 0.0)\n"):-len("    end\n    return\nend\n")] == plain.split("\n", 2)[2][:-len("    return\nend\n")]
