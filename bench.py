@@ -548,6 +548,78 @@ def gemm_study(args):
     print(json.dumps(out), flush=True)
 
 
+def unet_study(args):
+    """The warm-start U-Net on the GPU (4 levels, 32 base channels, inputs 2 channels, BF16
+    storage with FP32 accumulation): the forward pass at 512^2 (batch 1 and 8) and a training
+    step at batch 8 x 128^2 (forward, backward, Adam; the backward counted as twice the forward
+    products, less the first layer's data gradient), HIP-event timed. FLOPs are those of the
+    convolution products, 2 B H W K Cout per layer at its level's resolution; reported against
+    the BF16 dense MFMA peak."""
+    import numpy as np
+    import parallelstencil as ps
+    from parallelstencil import unet as U
+
+    ps.init_parallel_stencil(ps.AMDGPU, ps.Float64, 2, module="bench.unet")
+    peak = 2500.0
+    net = U.UNet(base=32, cin=2, seed=0)
+    rng = np.random.default_rng(0)
+
+    def conv_flops(B, H, W):
+        f = 0
+        for l in range(4):
+            for cv in net.enc[l]:
+                f += 2 * B * (H >> l) * (W >> l) * cv.K * cv.cout
+        for l in range(3):
+            for cv in net.dec[l]:
+                f += 2 * B * (H >> l) * (W >> l) * cv.K * cv.cout
+        return f + 2 * B * H * W * net.head.K * net.head.cout
+
+    def timed(fn, reps):
+        for _ in range(3):
+            fn()
+        ps.synchronize()
+        e0, e1 = ps.Event(), ps.Event()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        ps.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    runs = []
+    for B, H in ((1, 512), (8, 512)):
+        x = U.Buf((B, H, H, 2)).upload(rng.standard_normal((B, H, H, 2)).astype(np.float32))
+        ms = timed(lambda: net.forward(x, B, H, H), 10)
+        fl = conv_flops(B, H, H)
+        tf = fl / (ms * 1e-3) / 1e12
+        runs.append({"case": f"forward {B} x {H}^2", "ms": round(ms, 3), "GFLOP": round(fl / 1e9, 2), "tflops": round(tf, 1), "frac_of_peak": round(tf / peak, 4)})
+        log(f"U-Net forward, batch {B} x {H}^2: {ms:.3f} ms, {fl / 1e9:.1f} GFLOP, {tf:.1f} TFLOP/s ({100 * tf / peak:.2f}% of BF16 MFMA peak)")
+    B, H = 8, 128
+    x = U.Buf((B, H, H, 2)).upload(rng.standard_normal((B, H, H, 2)).astype(np.float32))
+    dout = U.Buf((B, H, H, 3)).upload(0.01 * rng.standard_normal((B, H, H, 3)).astype(np.float32))
+
+    def step():
+        net.zero_grad()
+        net.forward(x, B, H, H)
+        net.backward(dout)
+        net.adam(1e-4)
+
+    ms = timed(step, 10)
+    f1 = conv_flops(B, H, H)
+    fl = 3 * f1 - 2 * B * H * H * net.enc[0][0].K * net.enc[0][0].cout
+    tf = fl / (ms * 1e-3) / 1e12
+    runs.append({"case": f"training step {B} x {H}^2", "ms": round(ms, 3), "GFLOP": round(fl / 1e9, 2), "tflops": round(tf, 1), "frac_of_peak": round(tf / peak, 4)})
+    log(f"U-Net training step, batch {B} x {H}^2: {ms:.3f} ms, {fl / 1e9:.1f} GFLOP, {tf:.1f} TFLOP/s ({100 * tf / peak:.2f}% of BF16 MFMA peak)")
+    head = runs[0]
+    out = {"metric": "U-Net forward TFLOP/s (warm-start network, 512^2 sample)", "value": head["tflops"], "unit": "TFLOP/s", "n_gpus": 1,
+           "steps": 10, "warmup": 3, "ms_per_step": head["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
+           "config": {"workload": "U-Net 4 levels, 32 base channels, 3x3 convolutions as implicit-GEMM MFMA products, inputs 2 x 512^2"},
+           "roofline": {"bound": "mfma", "achieved": head["tflops"], "peak": peak, "unit": "TFLOP/s", "frac": head["frac_of_peak"], "traffic": None},
+           "runs": runs}
+    print(json.dumps(out), flush=True)
+
+
 def stokes_batch_study(args):
     """Stokes2D PT iterations of S samples at once (Stokes2DBatch: the two fused launches over
     (n, n, S) arrays) against one sample at a time (Stokes2DFused), at 127^2 (the training
@@ -1021,7 +1093,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="memopt | plain | triad (profiling)")
-    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | xgroups | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes-batch | stokes3d | thermal | porowaves | hydromech")
+    ap.add_argument("--workload", default="diffusion", help="diffusion (default) | stokes (learned warm start) | acoustic | xgroups | adjoint | gemm | shmem | cellarray | ad | hidecomm | stokes-batch | unet | stokes3d | thermal | porowaves | hydromech")
     ap.add_argument("--acoustic-n", type=int, default=384)
     ap.add_argument("--acoustic-steps", type=int, default=1000)
     ap.add_argument("--acoustic-exact-only", action="store_true", help="acoustic: skip the fastmath variants")
@@ -1064,6 +1136,9 @@ def main():
         return
     if args.workload == "xgroups":
         xgroups_study(args)
+        return
+    if args.workload == "unet":
+        unet_study(args)
         return
     if args.workload == "stokes-batch":
         stokes_batch_study(args)
