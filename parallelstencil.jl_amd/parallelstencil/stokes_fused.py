@@ -261,7 +261,8 @@ class Stokes2DBatch:
             raise core.ArgumentError("eta_ts must be 'av' or 'maxV'")
         if nsamples < 1:
             raise core.ArgumentError("nsamples must be at least 1")
-        self.nx, self.ny, self.S = nx, ny, S = nx, ny, int(nsamples)
+        S = int(nsamples)
+        self.nx, self.ny, self.S = nx, ny, S
         self.dx, self.dy = lx / (nx - 1), ly / (ny - 1)
         self.min_dxy2 = min(self.dx, self.dy) ** 2
         self.max_nxy = max(nx, ny)
