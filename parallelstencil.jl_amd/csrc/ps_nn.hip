@@ -1,8 +1,9 @@
 // Neural-network kernels of the Stokes warm-start U-Net (include/ps_nn.h).
 //
 // BF16 storage, FP32 accumulation. Activations are NHWC (channels fastest), so a 3x3
-// convolution is im2col (rows = pixels, columns = (ky, kx, c)) followed by a GEMM with the
-// weights stored [9*Cin][Cout]. The GEMM runs on the gfx950 matrix cores:
+// convolution is the product of im2col(x) (rows = pixels, columns = (ky, kx, c)) with the
+// weights stored [9*Cin][Cout]; the GEMM gathers the im2col operand from x as it loads its
+// tiles (ps_nn_gemm_im2col), or reads a stored matrix (ps_nn_gemm). It runs on the gfx950 matrix cores:
 // v_mfma_f32_16x16x32_bf16, lane l holding A[l%16][(l/16)*8 .. +7], B[(l/16)*8 .. +7][l%16] and
 // D[(l/16)*4 + r][l%16], r = 0..3. A workgroup of four waves computes a 64x64 tile, each wave a
 // 32x32 quarter as 2x2 MFMA blocks, over K steps of 32 staged through LDS.

@@ -6,7 +6,8 @@ miniapps/Stokes2D.jl; the plain iteration then runs to the same residual toleran
 so the solver stays the judge of the answer.
 
 Everything that computes runs in libpsamd.so (include/ps_nn.h): BF16 storage, FP32
-accumulation, 3x3 convolutions as im2col + MFMA GEMM, average pooling, nearest upsampling,
+accumulation, 3x3 convolutions (and their gradients) as MFMA GEMMs that gather the im2col
+operand from the activations (no column buffer), average pooling, nearest upsampling,
 skip concatenation, ReLU, masked MSE, Adam on FP32 master weights. This module only allocates
 buffers, moves host data and sequences the calls.
 
