@@ -123,3 +123,30 @@ def test_solve_queue_matches_single_sample_solves():
     assert got[3][0] == 300 and got[7][0] < got[0][0]
     B.close()
     one.close()
+
+
+def test_batch_fields_at_511():
+    # the warm start's resolution: two samples (contrast 10^[-2,2], maxV steps) after 40 fused
+    # iterations, every field bit for bit the single-sample solver's
+    n = 511
+    seeds = [100003, 100004]
+    Mus, Rog = _materials(n, n, seeds, [2.0, 2.0])
+    B = F.Stokes2DBatch(MOD, n, n, 2, eta_ts="maxV", graph=False)
+    B.set_material(Mus, Rog)
+    B.reset()
+    B.iterate(40)
+    B._check()
+    ps.synchronize()
+    eb = B.errors()
+    one = F.Stokes2DFused(MOD, n, n, eta_ts="maxV", graph=False)
+    for s in range(2):
+        one.set_material(np.asfortranarray(Mus[:, :, s]), np.asfortranarray(Rog[:, :, s]))
+        one.reset()
+        one.iterate(40)
+        one._check()
+        ps.synchronize()
+        for name in ("Pt", "Vx", "Vy", "txy", "Rx", "Ry"):
+            assert np.array_equal(_bits(getattr(B, name).to_host()[:, :, s]), _bits(getattr(one, name).to_host())), (name, s)
+        assert _bits([eb[s]])[0] == _bits([one.error()])[0]
+    B.close()
+    one.close()

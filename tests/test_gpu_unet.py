@@ -240,3 +240,25 @@ def test_implicit_data_gradient(cin, cout):
         ref[:, t // 3:t // 3 + H, t % 3:t % 3 + W, :] += dcol[:, :, :, t, :]
     ref = ref[:, 1:H + 1, 1:W + 1, :].reshape(P, cin)
     assert np.abs(dx.download() - ref).max() <= 8e-3 * np.abs(ref).max()
+
+
+def test_implicit_im2col_gemm_at_network_size():
+    # a full-resolution layer of the network (1 x 512^2, 32 -> 32 channels): the forward and the
+    # weight gradient bit for bit those of the stored column matrix
+    rng = np.random.default_rng(7)
+    B, H, W, C, N = 1, 512, 512, 32, 32
+    P, K9 = B * H * W, 9 * C
+    x = U.Buf((P, C)).upload(rng.standard_normal((P, C)).astype(np.float32))
+    col = U.Buf((P, K9))
+    ps.core.check(U.lib.ps_nn_im2col(x.p, B, H, W, C, col.p, None))
+    w = U.Buf((K9, N)).upload(rng.standard_normal((K9, N)).astype(np.float32))
+    bias = U.Buf((N,), "f32").upload(rng.standard_normal(N).astype(np.float32))
+    ye, yi = U.Buf((P, N)), U.Buf((P, N))
+    U.gemm(P, N, K9, col, K9, 1, w, N, 1, ye, N, bias=bias, epi=U.EPI_BIAS_RELU)
+    U.gemm_im2col(0, P, N, K9, x, H, W, C, w, N, 1, yi, N, bias, U.EPI_BIAS_RELU)
+    assert np.array_equal(ye.download(), yi.download())
+    dy = U.Buf((P, N)).upload(rng.standard_normal((P, N)).astype(np.float32))
+    ge, gi = U.Buf((K9, N), "f32").zero(), U.Buf((K9, N), "f32").zero()
+    U.gemm(K9, N, P, col, 1, K9, dy, N, 1, ge, N, epi=U.EPI_F32_ADD, splitk=1)
+    U.gemm_im2col(1, K9, N, P, x, H, W, C, dy, N, 1, gi, N, epi=U.EPI_F32_ADD, splitk=1)
+    assert np.array_equal(ge.download(), gi.download())
