@@ -63,15 +63,19 @@ __global__ void __launch_bounds__(256) nn_gemm_k(int M, int N, int K, const bf16
   const bool a_vec = AM == 0 ? (a_kfast ? sam : sak) % 8 == 0 && ((uintptr_t)A & 15) == 0 && (a_kfast || sam == 1)
                              : cC % 8 == 0 && ((uintptr_t)A & 15) == 0;
   // implicit im2col: the input element that column j of pixel p reads (0 outside the image)
+  // Every call has p < M or p < K and j < K or j < M (all int), so the 32-bit unsigned quotients
+  // are exact; only the address needs 64 bits.
   auto col_at = [&](long long p, int j, int& tap, int& c, long long& src) -> bool {
-    tap = j / cC;
-    c = j - tap * cC;
-    const int xx = (int)(p % cW);
-    const long long t2 = p / cW;
-    const int yy = (int)(t2 % cH);
-    const long long b = t2 / cH;
+    const unsigned pu = (unsigned)p, ju = (unsigned)j;
+    const unsigned tu = ju / (unsigned)cC;
+    tap = (int)tu;
+    c = (int)(ju - tu * (unsigned)cC);
+    const unsigned t2 = pu / (unsigned)cW;
+    const int xx = (int)(pu - t2 * (unsigned)cW);
+    const unsigned b = t2 / (unsigned)cH;
+    const int yy = (int)(t2 - b * (unsigned)cH);
     const int sy = yy + tap / 3 - 1, sx = xx + tap % 3 - 1;
-    src = ((b * cH + sy) * cW + sx) * cC + c;
+    src = (((long long)b * cH + sy) * cW + sx) * cC + c;
     return sy >= 0 && sy < cH && sx >= 0 && sx < cW;
   };
   const bool b_vec = (b_kfast ? sbn : sbk) % 8 == 0 && ((uintptr_t)B & 15) == 0 && (b_kfast || sbn == 1);
