@@ -119,3 +119,22 @@ def test_solve_freezes_each_sample_where_it_ends():
         assert iters[s] == it and errs[s] == err, (s, iters[s], it)
     # a finished sample stops advancing: its block count is where its solve ended
     assert fb.blocks[:2] == [3, 2] and fb.blocks[3] == 1 and fb.blocks[2] == 9
+
+
+def test_generate_samples_batch_orders_results_by_seed(monkeypatch):
+    # generate_samples_batch over the stand-in: results in seed order whatever order the solves
+    # finish in, iterations and residuals of the single-sample loop, time shared by iterations
+    from parallelstencil import unet as U
+    from parallelstencil import workloads as W
+    traj = {s: [10.0 ** (-(s % 7 + 1) * k / 3.0) for k in range(1, 40)] for s in range(10, 30)}
+    monkeypatch.setattr(W, "stokes2d_inclusions", lambda nx, ny, seed, log10_contrast=1.0: (np.float64(seed), np.float64(-seed)))
+    monkeypatch.setattr(U.core, "synchronize", lambda *a, **k: None)  # no device here
+    fb = FakeBatch(4, traj, nout=100, itermax=3000)
+    fb.nx = fb.ny = 8
+    data, stats = U.generate_samples_batch(fb, range(10, 30))
+    assert [int(d[0]) for d in data] == list(range(10, 30)) and [int(-d[1]) for d in data] == list(range(10, 30))
+    for s, (it, sec, err) in zip(range(10, 30), stats):
+        assert (it, err) == _single(traj[s], 100, 3000, 1e-8)
+    its = np.array([st[0] for st in stats], float)
+    secs = np.array([st[1] for st in stats])
+    assert np.allclose(secs / secs.sum(), its / its.sum())
