@@ -3,7 +3,7 @@
 No counterpart in the reference (ParallelStencil.jl has no learned component). The network maps
 the viscosity field (log10 eta) and the body force to a starting point (Vx, Vy, P) for the PT iteration of
 miniapps/Stokes2D.jl; the plain iteration then runs to the same residual tolerance from there,
-so the solver stays the judge of the answer.
+so the solver alone decides the answer.
 
 Everything that computes runs in libpsamd.so (include/ps_nn.h): BF16 storage, FP32
 accumulation, 3x3 convolutions (and their gradients) as MFMA GEMMs that gather the im2col
