@@ -427,6 +427,16 @@ extern "C" int ps_nn_gemm_im2col(int transpose, int M, int N, int K, const void*
                    : gemm_launch<1>(M, N, K, x, 0, 0, B, sbk, sbn, C, ldc, bias, epi, splitk, st, H, W, Cin);
 }
 
+// Element counts are products of the sizes: a pair of negative sizes would make a positive count
+// for indices the kernels never expect, so any negative size is refused before the launch.
+inline bool nn_negative(std::initializer_list<long long> v) {
+  for (long long x : v)
+    if (x < 0) return true;
+  return false;
+}
+#define NN_SIZES(what, ...) \
+  if (nn_negative({__VA_ARGS__})) return ps_internal_fail(PS_EARGUMENT, what ": negative size");
+
 #define NN_ELEMENTWISE(name, n, ...)                                            \
   NN_BEGIN                                                                      \
   if ((n) <= 0) return 0;                                                       \
@@ -434,32 +444,41 @@ extern "C" int ps_nn_gemm_im2col(int transpose, int M, int N, int K, const void*
   return launched(#name);
 
 extern "C" int ps_nn_im2col(const void* x, int B, int H, int W, int C, void* col, void* stream) {
+  NN_SIZES("ps_nn_im2col", B, H, W, C)
   NN_ELEMENTWISE(nn_im2col_k, (long long)B * H * W * 9 * C, (const bf16_t*)x, B, H, W, C, (bf16_t*)col)
 }
 extern "C" int ps_nn_conv_flip(const void* w, int Cin, int Cout, void* wf, void* stream) {
+  NN_SIZES("ps_nn_conv_flip", Cin, Cout)
   NN_ELEMENTWISE(nn_conv_flip_k, 9LL * Cin * Cout, (const bf16_t*)w, Cin, Cout, (bf16_t*)wf)
 }
 extern "C" int ps_nn_col2im(const void* dcol, int B, int H, int W, int C, void* dx, void* stream) {
+  NN_SIZES("ps_nn_col2im", B, H, W, C)
   NN_ELEMENTWISE(nn_col2im_k, (long long)B * H * W * C, (const bf16_t*)dcol, B, H, W, C, (bf16_t*)dx)
 }
 extern "C" int ps_nn_pool2(const void* x, int B, int H, int W, int C, void* y, void* stream) {
+  NN_SIZES("ps_nn_pool2", B, H, W, C)
   if ((H | W) & 1) return ps_internal_fail(PS_EARGUMENT, "ps_nn_pool2: H and W must be even");
   NN_ELEMENTWISE(nn_pool2_k, (long long)B * (H / 2) * (W / 2) * C, (const bf16_t*)x, B, H, W, C, (bf16_t*)y)
 }
 extern "C" int ps_nn_pool2_bwd(const void* dy, int B, int H, int W, int C, void* dx, void* stream) {
+  NN_SIZES("ps_nn_pool2_bwd", B, H, W, C)
   if ((H | W) & 1) return ps_internal_fail(PS_EARGUMENT, "ps_nn_pool2_bwd: H and W must be even");
   NN_ELEMENTWISE(nn_pool2_bwd_k, (long long)B * H * W * C, (const bf16_t*)dy, B, H, W, C, (bf16_t*)dx)
 }
 extern "C" int ps_nn_up2(const void* x, int B, int H, int W, int C, void* y, void* stream) {
+  NN_SIZES("ps_nn_up2", B, H, W, C)
   NN_ELEMENTWISE(nn_up2_k, (long long)B * 4 * H * W * C, (const bf16_t*)x, B, H, W, C, (bf16_t*)y)
 }
 extern "C" int ps_nn_up2_bwd(const void* dy, int B, int H, int W, int C, void* dx, void* stream) {
+  NN_SIZES("ps_nn_up2_bwd", B, H, W, C)
   NN_ELEMENTWISE(nn_up2_bwd_k, (long long)B * H * W * C, (const bf16_t*)dy, B, H, W, C, (bf16_t*)dx)
 }
 extern "C" int ps_nn_concat(const void* a, int Ca, const void* b, int Cb, int64_t P, void* y, void* stream) {
+  NN_SIZES("ps_nn_concat", Ca, Cb, P)
   NN_ELEMENTWISE(nn_concat_k, (long long)P * (Ca + Cb), (const bf16_t*)a, Ca, (const bf16_t*)b, Cb, (long long)P, (bf16_t*)y)
 }
 extern "C" int ps_nn_split(const void* dy, int Ca, int Cb, int64_t P, void* da, void* db, void* stream) {
+  NN_SIZES("ps_nn_split", Ca, Cb, P)
   NN_ELEMENTWISE(nn_split_k, (long long)P * (Ca + Cb), (const bf16_t*)dy, Ca, Cb, (long long)P, (bf16_t*)da, (bf16_t*)db)
 }
 extern "C" int ps_nn_relu_bwd(void* dy, const void* y, int64_t n, void* stream) {

@@ -119,3 +119,24 @@ def test_mirrored_stokes_samples_solve_the_mirrored_problem():
     for got, want in ((Vx2, mVx), (Vy2, mVy), (Pt2, mPt)):
         scale = np.max(np.abs(want))
         assert scale > 0 and np.max(np.abs(got - want)) <= 1e-12 * scale
+
+
+def test_network_entry_points_refuse_negative_sizes():
+    # a pair of negative sizes multiplies to a positive element count: refused before any launch
+    import pytest
+    from parallelstencil import _lib
+    L = _lib.lib
+    calls = [
+        ("ps_nn_conv_flip", (None, -2, -3, None, None)),
+        ("ps_nn_im2col", (None, 1, -4, -4, 2, None, None)),
+        ("ps_nn_col2im", (None, -1, 4, 4, -2, None, None)),
+        ("ps_nn_pool2", (None, 1, -4, 4, 2, None, None)),
+        ("ps_nn_pool2_bwd", (None, -1, 4, 4, -2, None, None)),
+        ("ps_nn_up2", (None, 1, -4, -4, 2, None, None)),
+        ("ps_nn_up2_bwd", (None, 1, 4, -4, -2, None, None)),
+        ("ps_nn_concat", (None, -3, None, 5, 10, None, None)),
+        ("ps_nn_split", (None, 5, -3, 10, None, None, None)),
+    ]
+    for name, args in calls:
+        with pytest.raises(_lib.ArgumentError, match=f"{name}: negative size"):
+            _lib.check(getattr(L, name)(*args))
